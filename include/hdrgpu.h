@@ -1,0 +1,318 @@
+/*
+ * hdrgpu.h -- C-ABI of the MI355X (gfx950) per-pixel hot path of the hdRestir
+ * spectral renderer: camera rays -> BVH closest-hit / shadow traversal ->
+ * UsdPreviewSurface/GGX over 4 hero wavelengths -> MIS / RIS / ReSTIR direct
+ * lighting -> spectral->RGB -> accumulation.
+ *
+ * Pure C, POD structs with fixed-width fields, no torch / HIP / USD types.
+ * Every call returns an hdr_status and never throws; hdr_last_error() returns
+ * a thread-local message for the last failing call on the calling thread.
+ *
+ * A context (and every scene / view created from it) is used from one host
+ * thread at a time.  Device state (scene BVHs, light tables, per-view
+ * reservoirs and accumulation) stays resident in HBM between calls.
+ *
+ * Reference interfaces replaced (paths relative to the hdRestir checkout):
+ *   - RenderPass{Inputs,Outputs,_execute}            source/passes/render_pass.h:13-42
+ *   - RaycastPass::_execute  (G-buffer, AOVs)        source/passes/raycast_pass.cpp:57-162
+ *   - IntegrationPass::_execute (per-pixel Li)       source/passes/integration_pass.cpp:157-252
+ *   - AccumulationPass / GpuAccumulationPass         source/passes/accumulation_pass.cpp:64-139,
+ *                                                    source/gpu_passes/gpu_accumulation_pass.cpp:24-85
+ *   - IDirectLightIntegrator::Li (MIS / RIS)         source/integrators/direct_light_integrator_interface.h:44-55
+ *   - RisDirectLightIntegrator::MakeFactory(M, useReservoir, skipVisibility)
+ *                                                    source/restir/ris_direct_light_integrator.h:73-74
+ *   - PathTracePassSettings / RISPipelineSettings    source/integrators/path_integrator.h:12-19,
+ *                                                    source/restir_pipeline/ris_pipeline.h:15-21
+ *   - IScene (IntersectScene, GetMaterial, GetLights, GetLightAtHit, GetEnvironment)
+ *                                                    source/sceneInterface/scene_interface.h:36-62
+ *   - RenderPipeline::ClearPersistentBuffers         source/pipeline/render_pipeline.cpp:42-45
+ */
+#ifndef HDRGPU_H
+#define HDRGPU_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HDRGPU_ABI_VERSION 1u
+
+typedef enum hdr_status {
+    HDR_OK = 0,
+    HDR_ERR_INVALID_ARGUMENT = 1,
+    HDR_ERR_OUT_OF_MEMORY = 2,
+    HDR_ERR_DEVICE = 3,
+    HDR_ERR_UNSUPPORTED = 4,
+    HDR_ERR_CANCELLED = 5,
+    HDR_ERR_CAPACITY = 6
+} hdr_status;
+
+/* ---- scene description (host memory; copied to the device by hdr_scene_upload) ---- */
+
+/* One triangle mesh subset = one BLAS (reference: MeshSubset, source/sceneInterface/mesh.h:19-25,
+ * built by BVH::Build, source/renderContext/bvh.cpp:66-182). Attribute arrays are optional
+ * (NULL / 0) and are either per-vertex (count == num_points) or face-varying
+ * (count == 3 * num_triangles), as in BVH::Build. */
+typedef struct hdr_mesh_desc {
+    const float   *points;     /* xyz, num_points */
+    int32_t        num_points;
+    const int32_t *indices;    /* 3 per triangle */
+    int32_t        num_triangles;
+    const float   *normals;    /* xyz */
+    int32_t        num_normals;
+    const float   *uvs;        /* uv */
+    int32_t        num_uvs;
+    const float   *colors;     /* rgb (displayColor) */
+    int32_t        num_colors;
+} hdr_mesh_desc;
+
+/* UsdPreviewSurface parameters (reference: PreviewSurfaceParams,
+ * source/materials/preview_surface_params.h:12-52), textures excluded. */
+typedef struct hdr_material_desc {
+    float diffuse_color[3];
+    float metallic;
+    float roughness;
+    float specular_color[3];
+    float specular;
+    float opacity;
+    float ior;
+    float transmission;
+    float transmission_color[3];
+    float transmission_depth;
+    float emission_color[3];
+    float emission;
+    float coat;
+    float coat_color[3];
+    float coat_roughness;
+    float coat_ior;
+    float sheen;
+    float sheen_color[3];
+    float sheen_roughness;
+    float subsurface;
+    float subsurface_color[3];
+    int32_t thin_walled;
+} hdr_material_desc;
+
+/* One mesh instance (reference: Scene::MeshInstance, source/scene/scene.h:112-122).
+ * transform is a GfMatrix4f in row-vector convention (p' = p * M), row-major. */
+typedef struct hdr_instance_desc {
+    float   transform[16];
+    int32_t mesh;       /* index into hdr_scene_desc::meshes */
+    int32_t material;   /* index into materials, or -1 for DefaultMaterial */
+    int32_t light;      /* light this mesh prim is (Scene::GetLightAtHit), or -1 */
+    int32_t _pad;
+} hdr_instance_desc;
+
+typedef enum hdr_light_type {
+    HDR_LIGHT_DISTANT = 0,
+    HDR_LIGHT_DOME = 1,
+    HDR_LIGHT_RECT = 2,
+    HDR_LIGHT_POINT = 3
+} hdr_light_type;
+
+/* One light (reference: LightParams source/sceneInterface/light_params.h:15-31 and the
+ * LightType switch in source/scene/light_factory.cpp:11-25). transform is a GfMatrix4d,
+ * row-vector convention, row-major. For HDR_LIGHT_DOME, dome_pixels holds
+ * dome_width*dome_height linear RGB float texels, row 0 first. */
+typedef struct hdr_light_desc {
+    int32_t      type;
+    float        color[3];
+    float        intensity;
+    float        exposure;
+    double       transform[16];
+    float        width;
+    float        height;
+    float        shaping_cone_angle;     /* degrees, 180 = no shaping */
+    float        shaping_cone_softness;
+    const float *dome_pixels;
+    int32_t      dome_width;
+    int32_t      dome_height;
+} hdr_light_desc;
+
+typedef struct hdr_scene_desc {
+    uint32_t                 struct_size;   /* sizeof(hdr_scene_desc) */
+    const hdr_mesh_desc     *meshes;
+    int32_t                  num_meshes;
+    const hdr_material_desc *materials;
+    int32_t                  num_materials;
+    const hdr_instance_desc *instances;
+    int32_t                  num_instances;
+    const hdr_light_desc    *lights;        /* order == Scene::GetLights() order */
+    int32_t                  num_lights;
+    int32_t                  environment;   /* index of the dome light that is the environment, or -1 */
+} hdr_scene_desc;
+
+/* ---- per-frame inputs ---- */
+
+/* reference: CameraParams, source/scene/camera_params.h:8-30 */
+typedef struct hdr_camera_params {
+    int32_t enable_dof;
+    int32_t enable_exposure;
+    float   focal_length;
+    float   f_stop;
+    float   focus_distance;
+    int32_t bokeh_blades;
+    float   lens_distortion;
+    float   iso;
+    float   shutter_speed;
+} hdr_camera_params;
+
+/* reference: RenderContext, source/renderContext/render_context.h:21-37 */
+typedef struct hdr_frame_desc {
+    uint32_t          struct_size;        /* sizeof(hdr_frame_desc) */
+    double            view[16];           /* GfMatrix4d world->camera, row-vector convention */
+    double            proj[16];           /* GfMatrix4d projection */
+    int32_t           frame_index;        /* drives the per-pixel seeds */
+    int32_t           has_camera_params;
+    hdr_camera_params camera;
+} hdr_frame_desc;
+
+typedef enum hdr_direct_light_kind {
+    HDR_DIRECT_MIS = 0,   /* PathTracer pipeline: MisDirectLightIntegrator */
+    HDR_DIRECT_RIS = 1    /* RIS pipeline: RisDirectLightIntegrator */
+} hdr_direct_light_kind;
+
+/* reference: PathTracePassSettings + maxDepth (source/integrators/path_integrator.h:12-19),
+ * RISPipelineSettings (source/restir_pipeline/ris_pipeline.h:15-21),
+ * Denoiser::Config::EnableFireflyFilter (accumulation pass). */
+typedef struct hdr_pipeline_desc {
+    uint32_t struct_size;               /* sizeof(hdr_pipeline_desc) */
+    int32_t  direct_light;              /* hdr_direct_light_kind */
+    int32_t  candidate_count;           /* RIS M */
+    int32_t  use_reservoir;
+    int32_t  skip_visibility;
+    int32_t  max_depth;
+    int32_t  max_reflection_bounces;
+    int32_t  max_refraction_bounces;
+    int32_t  enable_subsurface;
+    int32_t  render_ibl_background;
+    float    roulette_aggressiveness;
+    int32_t  enable_firefly_filter;
+    /* Extension beyond the reference (0 = reference behaviour): after the temporal merge,
+     * merge spatial_count neighbour reservoirs of the previous frame from a disc of
+     * spatial_radius pixels. */
+    int32_t  spatial_count;
+    float    spatial_radius;
+} hdr_pipeline_desc;
+
+/* Per-pixel persistent reservoir (reference: WeightedReservoir<RISLightCandidate>,
+ * source/restir/reservoir.h:69-161 + RISLightCandidate ris_direct_light_integrator.h:26-43),
+ * stored as an 80-byte POD. */
+#define HDR_RES_HAS_SAMPLE      0x01u   /* ChosenSample engaged */
+#define HDR_RES_HAS_CANDIDATE   0x02u   /* ChosenSample->Candidate engaged */
+#define HDR_RES_VIS_TESTED      0x04u   /* VisibilityTested */
+#define HDR_RES_PDF_SOLID_ANGLE 0x08u   /* Ls.Pdf.space == SolidAngle */
+#define HDR_RES_LIGHT_DELTA     0x10u   /* Light->IsDeltaLight() */
+typedef struct hdr_reservoir {
+    float    dir[3];
+    float    dist;
+    float    color[3];
+    float    pdf;
+    float    light_normal[3];
+    float    target;            /* targetFunction p-hat (always float-representable) */
+    float    throughput[4];
+    float    wsum;
+    int32_t  count;             /* CandidateCount (M) */
+    float    w;
+    uint32_t flags_light;       /* HDR_RES_* flags in bits 24..31, light index + 1 in bits 0..23 */
+} hdr_reservoir;
+
+/* Per-pixel parity record (optional output of hdr_render_frame): the same record is
+ * written by the CPU oracle so integer streams can be compared bit-exactly. */
+typedef struct hdr_pixel_debug {
+    uint32_t draws;          /* RNG draws consumed by the pixel this frame */
+    uint32_t light_hash;     /* FNV-1a over every uniform light-selection index, in order */
+    uint32_t select_hash;    /* FNV-1a over every RIS / reservoir selection result, in order */
+    int32_t  n_light_picks;  /* number of light-selection draws */
+    int32_t  prim_instance;  /* G-buffer instance id (PrimId), -1 on miss */
+    int32_t  prim_triangle;  /* G-buffer triangle index within its mesh, -1 on miss */
+    float    prim_t;         /* G-buffer hit distance */
+    float    sample_rgb[3];  /* this frame's Color before accumulation */
+} hdr_pixel_debug;
+
+/* ---- opaque handles ---- */
+typedef struct hdr_context hdr_context;
+typedef struct hdr_scene   hdr_scene;
+typedef struct hdr_view    hdr_view;
+
+const char *hdr_last_error(void);
+uint32_t    hdr_abi_version(void);
+
+hdr_status hdr_context_create(int32_t device, hdr_context **out);
+hdr_status hdr_context_destroy(hdr_context *ctx);
+/* cooperative cancellation (reference: IRenderJob::IsCancelled, source/rendererInterface/render_job.h:9-35):
+ * a set flag makes the next hdr_render_frame return HDR_ERR_CANCELLED before any launch. */
+hdr_status hdr_context_set_cancel(hdr_context *ctx, int32_t cancelled);
+
+/* Builds BLAS/TLAS (midpoint-split, as Scene::BuildRenderState / BVH::Build), light and
+ * material tables on the host and uploads them (reference: Scene::BuildRenderState,
+ * source/scene/scene.cpp:118-271). The scene stays resident until hdr_scene_destroy. */
+hdr_status hdr_scene_upload(hdr_context *ctx, const hdr_scene_desc *desc, hdr_scene **out);
+hdr_status hdr_scene_destroy(hdr_scene *scene);
+/* device bytes held by the scene */
+hdr_status hdr_scene_device_bytes(const hdr_scene *scene, uint64_t *out);
+
+/* A view owns the device-resident persistent buffers of one pipeline instance:
+ * reservoirs (__ris_reservoir) and accumulation (__accum_color), plus the G-buffer. */
+hdr_status hdr_view_create(hdr_context *ctx, int32_t width, int32_t height, hdr_view **out);
+hdr_status hdr_view_destroy(hdr_view *view);
+/* == RenderPipeline::ClearPersistentBuffers: zero reservoirs and accumulation */
+hdr_status hdr_view_reset(hdr_view *view);
+hdr_status hdr_view_device_bytes(const hdr_view *view, uint64_t *out);
+
+/* One frame of RaycastPass -> {RIS,}PathTracePass -> AccumulationPass on the device.
+ * Asynchronous with respect to the host unless an output below is requested.
+ * color_out (optional, host, width*height*4 floats): accumulated "Color" (RGBA32F).
+ * debug_out (optional, host, width*height records): parity records. */
+hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
+                            const hdr_pipeline_desc *pipeline, float *color_out,
+                            hdr_pixel_debug *debug_out);
+
+/* Copies of the view's persistent state (host buffers of width*height elements). */
+hdr_status hdr_view_read_color(hdr_view *view, float *rgba_out);
+hdr_status hdr_view_read_reservoirs(hdr_view *view, hdr_reservoir *out);
+/* RaycastPass AOVs: Depth (1 float), Albedo (3), Normal (3) per pixel; misses write 1 / 0 / 0. */
+hdr_status hdr_view_read_aovs(hdr_view *view, float *depth, float *albedo, float *normal);
+/* Device pointer of the accumulated colour (RGBA32F) for zero-copy consumers. */
+hdr_status hdr_view_color_device_ptr(hdr_view *view, void **out);
+/* Blocks until all work queued on the view's stream has finished. */
+hdr_status hdr_view_synchronize(hdr_view *view);
+/* Device time (ms) of the last hdr_render_frame's integration kernel and of the whole frame. */
+hdr_status hdr_view_last_timing(hdr_view *view, float *integrate_ms, float *frame_ms);
+
+/* ---- parity / measurement entry points ---- */
+
+/* Closest hit for n world-space rays (reference: Scene::IntersectScene, scene.cpp:515-535).
+ * Outputs per ray: instance id (-1 miss), triangle index within its mesh, t. */
+hdr_status hdr_scene_intersect(hdr_scene *scene, const float *origins, const float *dirs, int32_t n,
+                               int32_t *instance_out, int32_t *triangle_out, float *t_out);
+
+/* First n_draws raw 64-bit outputs of mt19937_64(seed) for each of n seeds, computed on the
+ * device by the same cursor engine the integrator uses (reference: Rng, source/core/rng.h:10-56). */
+hdr_status hdr_rng_raw(hdr_context *ctx, const uint32_t *seeds, int32_t n, int32_t n_draws, uint64_t *out);
+/* Same, mapped through uniform_real_distribution<float> (Rng::NextFloat). */
+hdr_status hdr_rng_float(hdr_context *ctx, const uint32_t *seeds, int32_t n, int32_t n_draws, float *out);
+
+/* Work counters of the last frame (rays by class, BVH node visits, triangle tests), filled
+ * only when counting is enabled on the view (costs atomics; off for timing runs). */
+typedef struct hdr_frame_counters {
+    uint64_t primary_rays;
+    uint64_t bounce_rays;
+    uint64_t shadow_rays;
+    uint64_t candidate_rays;        /* BSDF-candidate rays actually traced */
+    uint64_t candidate_rays_skipped;/* BSDF-candidate rays proven unable to contribute */
+    uint64_t node_visits;
+    uint64_t triangle_tests;
+    uint64_t instance_visits;
+    uint64_t rng_draws;
+} hdr_frame_counters;
+hdr_status hdr_view_set_counting(hdr_view *view, int32_t enable);
+hdr_status hdr_view_read_counters(hdr_view *view, hdr_frame_counters *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HDRGPU_H */
