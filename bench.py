@@ -1,0 +1,175 @@
+"""Benchmark: 1920x1080 many-lights ReSTIR on one MI355X.
+
+Workload (one step = one frame of the per-pixel hot path over the whole image):
+  scene   512x512-vertex value-noise height field (522,242 triangles) + 500 instances of a
+          320-triangle UV sphere (682,242 instanced triangles), 4096 lights (2048 point, 2048 rect),
+          intensities 10^U[0,4], blackbody colours U[2000,10000] K;
+  path    RaycastPass G-buffer ray -> jittered camera ray -> 4 hero wavelengths -> RIS with
+          M = 32 candidates (16 light + 16 BSDF), skipVisibility (one shadow ray for the winner),
+          temporal reservoir reuse, max depth 1 -> spectral->RGB -> accumulation;
+  camera  static (progressive accumulation with temporal reuse); the linear-dolly variant is
+          reported beside it (a camera move clears reservoirs and accumulation, as the renderer does).
+
+Prints one JSON line. `value` = ReSTIR frames/s at 1080p with scene, reservoirs and accumulation
+resident in HBM. The CPU baseline is the oracle (a faithful C++ restatement of the reference's CPU
+path) timed on a bounded sample of the same frame on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 32   # one BVH node record (6 fp32 bounds + 2 int)
+TRI_BYTES = 36    # three fp32x3 vertices read by the ray-triangle test
+STATE_BYTES = 208  # per pixel: accumulation rd+wr 32, colour wr 16, reservoir rd+wr 160
+
+
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def read_traffic():
+    """HBM bytes per integrate-kernel launch from the committed rocprofv3 --pmc summary, or None."""
+    path = os.path.join(ROOT, "profiles", "integrate_many_lights_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.startswith("#"):
+            k, v = line.strip().split("=", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(scene, pipe, rows=8, threads=0):
+    """Oracle (faithful restatement, per-path light sampler as in the reference) on `rows` full
+    rows of frame 0; extrapolated to frames/s of the whole 1920x1080 frame."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    o = oracle_py.Oracle(scene, hoist_light_set=False, threads=threads)
+    y0 = scene.height // 2
+    rect = (0, y0, scene.width, y0 + rows)
+    o.render(scene.frame(0), pipe, rect=rect, debug=False)  # warm-up
+    o.render(scene.frame(1), pipe, rect=rect, debug=False)
+    secs = o.last_seconds
+    o.close()
+    frame_secs = secs * (scene.height / rows)
+    return {"value": 1.0 / frame_secs, "unit": "frames/s", "cores": threads or os.cpu_count(), "kind": "port",
+            "sample": f"{scene.width}x{rows} pixel rows of frame 1 (after a warm-up frame), faithful per-path "
+                      f"light-sampler construction, extrapolated x{scene.height // rows} to the full frame"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by pixels and requests, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from hdrestir_amd import abi, gpu, scenes
+    _log("building scene")
+    sc = scenes.many_lights(frames=max(args.steps, 2), dolly=True)
+    pipe = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=1)
+    ctx = gpu.Context(0)
+    gs = gpu.GpuScene(ctx, sc)
+    r = gpu.Renderer(ctx, gs, sc.width, sc.height)
+    P = sc.width * sc.height
+    static = [sc.frame(0, camera_index=0)]
+
+    def frame_desc(i, moving):
+        return sc.frame(i, camera_index=(i % len(sc.cameras)) if moving else 0)
+
+    # counting run (untimed): rays by class, node visits, triangle tests of this frame
+    r.set_counting(True)
+    r.reset()
+    r.render(frame_desc(0, False), pipe, read_color=False)
+    r.render(frame_desc(1, False), pipe, read_color=False)
+    r.synchronize()
+    counts = r.counters()
+    r.set_counting(False)
+    _log(f"counters {counts}")
+
+    # static camera: progressive frames with temporal reuse
+    r.reset()
+    for i in range(args.warmup):
+        r.render(frame_desc(i, False), pipe, read_color=False)
+    r.synchronize()
+    integ = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r.render(frame_desc(args.warmup + i, False), pipe, read_color=False)
+        integ.append(r.last_timing()[0])
+    r.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    fps = 1000.0 / ms_per_step
+    kern_ms = sum(integ) / len(integ)
+
+    # dolly: the camera moves every frame, which clears reservoirs and accumulation
+    t2 = time.perf_counter()
+    nd = min(args.steps, len(sc.cameras))
+    for i in range(nd):
+        r.reset()
+        r.render(frame_desc(i, True), pipe, read_color=False)
+    r.synchronize()
+    dolly_fps = nd / (time.perf_counter() - t2)
+    img = r.read_color()
+
+    rays = counts["primary_rays"] + counts["bounce_rays"] + counts["shadow_rays"] + counts["candidate_rays"]
+    ref_rays = rays + counts["candidate_rays_skipped"]
+    alg_bytes = NODE_BYTES * counts["node_visits"] + TRI_BYTES * counts["triangle_tests"] + STATE_BYTES * P
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    line = {
+        "metric": "ReSTIR frames/s at 1080p (many-lights, M=32, 1 temporal); Mrays/s alongside",
+        "value": round(fps, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (deterministic generator, seed 0)",
+        "config": {"workload": "many_lights_1080p_restir", "width": sc.width, "height": sc.height,
+                   "triangles_unique": sc.triangle_counts()[0], "triangles_instanced": sc.triangle_counts()[1],
+                   "lights": len(sc.lights), "candidate_count": 32, "skip_visibility": True, "max_depth": 1,
+                   "temporal_reuse": True, "spatial_reuse": 0, "camera": "static", "parallelism": "single-gpu"},
+        "mrays_per_s": round(rays * fps / 1e6, 2),
+        "mrays_per_s_reference_equivalent": round(ref_rays * fps / 1e6, 2),
+        "rays_per_frame": {"primary": counts["primary_rays"], "bounce": counts["bounce_rays"],
+                           "shadow": counts["shadow_rays"], "bsdf_candidate_traced": counts["candidate_rays"],
+                           "skipped_cannot_contribute": counts["candidate_rays_skipped"]},
+        "dolly_frames_per_s": round(dolly_fps, 3),
+        "integrate_kernel_ms": round(kern_ms, 4),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
+                     "algorithmic_bytes_per_launch": alg_bytes},
+        "image_mean_rgb": [round(float(v), 3) for v in img[..., :3].reshape(-1, 3).mean(0)],
+    }
+    r.close()
+    gs.close()
+    ctx.close()
+    if not args.no_cpu_baseline:
+        _log("cpu baseline")
+        line["cpu_baseline"] = cpu_baseline(sc, pipe, rows=args.cpu_rows)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,316 @@
+// dscene.h -- device-resident scene (HBM layout) and BVH traversal.
+//
+// Layout (all arrays in one device allocation, 256-byte aligned sub-ranges):
+//   nodes      GNode[ ]   32 B: BLAS nodes of every mesh, then TLAS nodes. Internal node:
+//                         left = index of the left child (right = left + 1); leaf:
+//                         left = -(first triangle) - 1, count = triangles (<= 4) or instances (<= 2).
+//   tris       GTri[ ]    48 B: v0, v1, v2 (+ original triangle index, mesh-local) -- the
+//                         36 bytes the Moller-Trumbore test reads, one 16-B aligned record.
+//   attrs      GTriAttr[] 72 B: shading normals and vertex colours, read once per ray for the
+//                         winning triangle only.
+//   tlas_idx   int[ ]         TLAS leaf slots -> instance index (TlasInstanceIndices).
+//   instances  GInstance[] 144 B: inverse / forward GfMatrix4f, BLAS root, material, light.
+// The trees are the reference's midpoint-split BVH2 (leaf <= 4 triangles) and TLAS (leaf <= 2
+// instances), built on the host in the reference's order (source/renderContext/bvh.cpp:184-241,
+// source/scene/scene.cpp:352-407), so traversal order -- and with it the winner among equal-t
+// hits -- is the reference's.
+#pragma once
+
+#include "gfm.h"
+
+namespace hdr {
+
+struct alignas(16) GNode {
+    float mnx, mny, mnz;
+    int left;
+    float mxx, mxy, mxz;
+    int count;
+};
+struct alignas(16) GTri {
+    float v0x, v0y, v0z;
+    int orig;
+    float v1x, v1y, v1z;
+    int _p1;
+    float v2x, v2y, v2z;
+    int _p2;
+};
+struct GTriAttr {
+    float n0[3], n1[3], n2[3];
+    float c0[3], c1[3], c2[3];
+};
+struct GInstance {
+    float inv[16];
+    float xf[16];
+    int root;      // BLAS root node (global index), -1 for an empty mesh
+    int material;  // material record (default material = num_materials)
+    int light;     // light index this mesh prim is (GetLightAtHit), or -1
+    int has_params;
+};
+
+// Closure parameters of one material (PreviewSurfaceParams -> BSDFClosure, preview_surface.cpp:61-124).
+struct GMaterial {
+    float base[3];
+    float emission[3];    // EmissionColor * Emission
+    float spec_color[3];
+    float coat_color[3];
+    float sheen_color[3];
+    float ss_color[3];
+    float trans_color[3];
+    float metallic, roughness, ior, specular, transmission, trans_depth;
+    float coat, coat_roughness, coat_ior, sheen, sheen_roughness, subsurface, opacity;
+    int thin_walled;
+    int is_default;       // DefaultMaterial: base = hit albedo, normal = geometric, Lambert sampling
+    int _pad[3];
+};
+
+// One light with the per-sample constants of SampleLight precomputed on the host with the same
+// float operations (rect_light.cpp:14-37, point_light.cpp:16-55, distant_light.cpp:10-17).
+struct GLight {
+    int type;
+    int is_delta;
+    float width, height;
+    float color[3];       // Color * EffectiveIntensity()
+    float area;           // Width * Height
+    float m[16];          // GfMatrix4f(Transform)
+    float normal[3];      // rect: Normalize(TransformDir(0,0,-1)); distant: its direction
+    float inv_area;       // 1 / area
+    float pos[3];         // point: ExtractTranslation
+    float cone_angle;
+    float cos_cone, cos_inner, cone_rad, softness;
+    int dome;             // index into the dome table, -1
+    float eff_int;        // EffectiveIntensity() (dome: texel * Color * this, dome_light.cpp:98,147)
+    int _pad[2];
+    float raw_color[3];   // Color
+    float _pad2;
+};
+
+struct GDome {
+    const float *pixels;  // w*h*3
+    const float *row_cdf; // h+1
+    const float *col_cdf; // h*(w+1)
+    int w, h;
+    float total_lum;
+    float pdf_scale;      // totalLum * (pi/h) * (2pi/w)
+};
+
+struct DeviceScene {
+    const GNode *nodes;
+    const GTri *tris;
+    const GTriAttr *attrs;
+    const int *tlas_idx;
+    const GInstance *instances;
+    const GMaterial *materials;
+    const GLight *lights;
+    const GDome *domes;
+    int tlas_root;        // -1 when the scene has no instances
+    int num_instances;
+    int num_materials;
+    int num_lights;
+    int environment;      // light index of the environment dome, -1
+    int env_dome;         // dome index of the environment, -1
+    int has_mesh_lights;  // some instance maps to a light (GetLightAtHit can return non-null)
+    int has_emission;     // some material has non-zero emission
+};
+
+// Traversal statistics (only accumulated when counting is enabled).
+struct TraceStats {
+    uint32_t nodes, tris, instances;
+};
+
+// IntersectAABB (bvh.cpp:32-64 with double thresholds; scene.cpp:13-46 with float thresholds).
+template <bool kFloatThresholds>
+__device__ __forceinline__ bool aabb_hit(const GNode &n, V3 o, V3 d, float &tmin_out) {
+    if (n.mnx > n.mxx || n.mny > n.mxy || n.mnz > n.mxz) return false;  // GfRange3f::IsEmpty
+    float tmin = -1e30f, tmax = 1e30f;
+    const float mn[3] = {n.mnx, n.mny, n.mnz};
+    const float mx[3] = {n.mxx, n.mxy, n.mxz};
+    const float oo[3] = {o.x, o.y, o.z};
+    const float dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const bool parallel = kFloatThresholds ? (fabsf(dd[i]) < 1e-8f) : (double(fabsf(dd[i])) < 1e-8);
+        if (parallel) {
+            if (oo[i] < mn[i] || oo[i] > mx[i]) return false;
+        } else {
+            const float inv = 1.0f / dd[i];
+            float t1 = (mn[i] - oo[i]) * inv;
+            float t2 = (mx[i] - oo[i]) * inv;
+            if (t1 > t2) {
+                const float s = t1;
+                t1 = t2;
+                t2 = s;
+            }
+            tmin = smax(tmin, t1);
+            tmax = smin(tmax, t2);
+            if (tmin > tmax) return false;
+        }
+    }
+    tmin_out = tmin;
+    if (kFloatThresholds) return tmax > 0.0f && tmax > 1e-4f;
+    return tmax > 0.0f && double(tmax) > 1e-4;
+}
+
+// IntersectTriangle, bvh.cpp:8-30 (Moller-Trumbore; reject |det| < 1e-8, accept t > 1e-4).
+__device__ __forceinline__ bool tri_hit(const GTri &t, V3 o, V3 d, float &tt, float &uu, float &vv) {
+    const V3 v0 = v3(t.v0x, t.v0y, t.v0z);
+    const V3 e1 = v3(t.v1x, t.v1y, t.v1z) - v0;
+    const V3 e2 = v3(t.v2x, t.v2y, t.v2z) - v0;
+    const V3 pvec = cross(d, e2);
+    const float det = dot(e1, pvec);
+    if (double(fabsf(det)) < 1e-8) return false;
+    const float invDet = 1.0f / det;
+    const V3 tvec = o - v0;
+    const float u = dot(tvec, pvec) * invDet;
+    if (u < 0.0f || u > 1.0f) return false;
+    const V3 qvec = cross(tvec, e1);
+    const float v = dot(d, qvec) * invDet;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    tt = dot(e2, qvec) * invDet;
+    uu = u;
+    vv = v;
+    return double(tt) > 1e-4;
+}
+
+__device__ __forceinline__ GNode load_node(const GNode *nodes, int i) {
+    const float4 *p = reinterpret_cast<const float4 *>(nodes + i);
+    const float4 a = p[0], b = p[1];
+    GNode n;
+    n.mnx = a.x;
+    n.mny = a.y;
+    n.mnz = a.z;
+    n.left = __float_as_int(a.w);
+    n.mxx = b.x;
+    n.mxy = b.y;
+    n.mxz = b.z;
+    n.count = __float_as_int(b.w);
+    return n;
+}
+
+struct RawHit {
+    float t;
+    int inst;
+    int tri;   // global triangle record index
+    float u, v;
+};
+
+constexpr int kStackDepth = 64;
+
+// Scene::Intersect (scene.cpp:409-513) + BVH::_IntersectNode (bvh.cpp:251-305) as two explicit
+// stacks: TLAS children pushed right-then-left, BLAS left subtree fully before the right one,
+// every node's box tested against the best t at the time it is reached. Finds the reference's
+// winner: strict t < best, first found among equal t.
+template <bool kCount>
+__device__ bool trace_closest(const DeviceScene &s, V3 o, V3 d, RawHit &hit, TraceStats &st) {
+    hit.t = 1e30f;
+    hit.inst = -1;
+    if (s.tlas_root < 0) return false;
+    int tstack[kStackDepth];
+    int bstack[kStackDepth];
+    int tsp = 0;
+    tstack[tsp++] = s.tlas_root;
+    bool was = false;
+    while (tsp > 0) {
+        const int ni = tstack[--tsp];
+        const GNode n = load_node(s.nodes, ni);
+        if (kCount) ++st.nodes;
+        float tAabb = 0.0f;
+        if (!aabb_hit<true>(n, o, d, tAabb) || tAabb > hit.t) continue;
+        if (n.left >= 0) {
+            tstack[tsp++] = n.left + 1;
+            tstack[tsp++] = n.left;
+            continue;
+        }
+        const int start = -n.left - 1;
+        for (int k = 0; k < n.count; ++k) {
+            const int ii = s.tlas_idx[start + k];
+            const GInstance &inst = s.instances[ii];
+            if (kCount) ++st.instances;
+            if (inst.root < 0) continue;
+            const V3 oo = xform_point(inst.inv, o);
+            const V3 od = xform_dir(inst.inv, d);
+            float instT = hit.t;
+            int bestTri = -1;
+            float bu = 0.f, bv = 0.f;
+            int bsp = 0;
+            bstack[bsp++] = inst.root;
+            while (bsp > 0) {
+                const int bi = bstack[--bsp];
+                const GNode b = load_node(s.nodes, bi);
+                if (kCount) ++st.nodes;
+                float ta;
+                if (!aabb_hit<false>(b, oo, od, ta)) continue;
+                if (ta > instT) continue;
+                if (b.left >= 0) {
+                    bstack[bsp++] = b.left + 1;
+                    bstack[bsp++] = b.left;
+                    continue;
+                }
+                const int ts = -b.left - 1;
+                for (int q = 0; q < b.count; ++q) {
+                    const GTri tr = s.tris[ts + q];
+                    if (kCount) ++st.tris;
+                    float tt, uu, vv;
+                    if (tri_hit(tr, oo, od, tt, uu, vv) && tt < instT) {
+                        instT = tt;
+                        bestTri = ts + q;
+                        bu = uu;
+                        bv = vv;
+                    }
+                }
+            }
+            if (bestTri < 0 || instT >= hit.t) continue;
+            hit.t = instT;
+            hit.inst = ii;
+            hit.tri = bestTri;
+            hit.u = bu;
+            hit.v = bv;
+            was = true;
+        }
+    }
+    return was;
+}
+
+// HitRecord of the winner (Scene::Intersect :456-497 and IntersectScene :515-535): world
+// normals through TransformDir + GetNormalized, albedo = material diffuse colour or the
+// interpolated vertex colour.
+struct HitRec {
+    V3 pos, ng, ns, albedo;
+    float depth;
+    int inst;
+    int tri;   // mesh-local original triangle index
+};
+
+__device__ __forceinline__ HitRec make_hit_record(const DeviceScene &s, V3 o, V3 d, const RawHit &h) {
+    const GInstance &inst = s.instances[h.inst];
+    const GTri tr = s.tris[h.tri];
+    const GTriAttr &at = s.attrs[h.tri];
+    const V3 v0 = v3(tr.v0x, tr.v0y, tr.v0z), v1 = v3(tr.v1x, tr.v1y, tr.v1z), v2 = v3(tr.v2x, tr.v2y, tr.v2z);
+    const V3 nObj = normalized(cross(v1 - v0, v2 - v0));
+    const float w = 1.0f - h.u - h.v;
+    const V3 n0 = v3(at.n0[0], at.n0[1], at.n0[2]);
+    V3 sObj;
+    if (dot(n0, n0) > 1e-6f) {
+        const V3 n1 = v3(at.n1[0], at.n1[1], at.n1[2]), n2 = v3(at.n2[0], at.n2[1], at.n2[2]);
+        sObj = normalized(n0 * w + n1 * h.u + n2 * h.v);
+    } else {
+        sObj = nObj;
+    }
+    HitRec r;
+    r.pos = o + h.t * d;
+    r.ng = normalized(xform_dir(inst.xf, nObj));
+    r.ns = normalized(xform_dir(inst.xf, sObj));
+    if (inst.has_params) {
+        const GMaterial &m = s.materials[inst.material];
+        r.albedo = v3(m.base[0], m.base[1], m.base[2]);
+    } else {
+        const V3 c0 = v3(at.c0[0], at.c0[1], at.c0[2]), c1 = v3(at.c1[0], at.c1[1], at.c1[2]),
+                 c2 = v3(at.c2[0], at.c2[1], at.c2[2]);
+        r.albedo = c0 * w + c1 * h.u + c2 * h.v;
+    }
+    r.depth = h.t;
+    r.inst = h.inst;
+    r.tri = tr.orig;
+    return r;
+}
+
+}  // namespace hdr

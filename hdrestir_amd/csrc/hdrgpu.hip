@@ -1,0 +1,687 @@
+// hdrgpu.hip -- kernels and C-ABI of libhdrgpu.so (declared in include/hdrgpu.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/hdrgpu.h"
+#include "host_scene.h"
+#include "integrator.h"
+
+using namespace hdr;
+
+// ============================================================================ kernels
+
+// RaycastPass + IntegrationPass + (fused) AccumulationPass for one 8x8 pixel tile per wave.
+template <bool kCount, bool kDebug>
+__global__ void __launch_bounds__(kLanes) integrate_kernel(const FrameArgs a) {
+    extern __shared__ double wl[];
+    const int lane = threadIdx.x;
+    const int x = blockIdx.x * 8 + (lane & 7);
+    const int y = blockIdx.y * 8 + (lane >> 3);
+    if (x >= a.W || y >= a.H) return;
+    const int pixel = y * a.W + x;
+    const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
+    PixelStats st = {};
+    Tracer<kCount> tracer{a.s, st};
+
+    // RaycastPass (raycast_pass.cpp:105-157): closest hit of the pixel-centre ray
+    V3 co, cd;
+    if (a.has_cam) {
+        Mt64 crng;
+        crng.init(seed, nullptr);
+        camera_ray(a, float(x) + 0.5f, float(y) + 0.5f, &crng, co, cd);
+    } else {
+        camera_ray(a, float(x) + 0.5f, float(y) + 0.5f, nullptr, co, cd);
+    }
+    HitRec g;
+    if (kCount) ++st.primary;
+    const bool gHit = tracer.trace(co, cd, g);
+    if (a.aov_depth) {
+        a.aov_depth[pixel] = gHit ? g.depth : 1.0f;
+        const V3 al = gHit ? g.albedo : v3s(0.0f);
+        const V3 nn = gHit ? g.ng : v3s(0.0f);
+        a.aov_albedo[3 * pixel + 0] = al.x;
+        a.aov_albedo[3 * pixel + 1] = al.y;
+        a.aov_albedo[3 * pixel + 2] = al.z;
+        a.aov_normal[3 * pixel + 0] = nn.x;
+        a.aov_normal[3 * pixel + 1] = nn.y;
+        a.aov_normal[3 * pixel + 2] = nn.z;
+    }
+
+    // IntegrationPass (integration_pass.cpp:208-243)
+    Mt64 rng;
+    uint64_t *slot = a.spill ? a.spill + size_t(pixel) * (2 * kMtN) : nullptr;
+    rng.init(seed, slot);
+    const float px = float(x) + rng.next_float();
+    const float py = float(y) + rng.next_float();
+    V3 ro, rd;
+    camera_ray(a, px, py, &rng, ro, rd);
+    const SpecTab tab = make_tab(rng.next_float());
+    Trace dbg = {2166136261u, 2166136261u, 0};
+    PixelCtx<kCount, kDebug> p{a, tab, rng, tracer, st, dbg, wl + lane, slot ? slot + kMtN : nullptr};
+    const Spec L = path_li(p, rd, gHit, g, pixel);
+    V3 rgb = spec2rgb(tab, L);
+    if (a.has_cam && a.cam.enable_exposure) rgb = rgb * a.exposure_mul;
+    const float4 c4 = make_float4(rgb.x, rgb.y, rgb.z, 1.0f);
+    if (a.firefly) {
+        a.frame_rgb[pixel] = c4;
+    } else {
+        // AccumulationPass (accumulation_pass.cpp:98-106): accum += c; out = accum * invN
+        float4 acc = a.accum[pixel];
+        acc.x += c4.x;
+        acc.y += c4.y;
+        acc.z += c4.z;
+        acc.w += c4.w;
+        a.accum[pixel] = acc;
+        a.color[pixel] = make_float4(acc.x * a.inv_n, acc.y * a.inv_n, acc.z * a.inv_n, acc.w * a.inv_n);
+    }
+    if (kDebug) {
+        hdr_pixel_debug d;
+        d.draws = rng.k;
+        d.light_hash = dbg.lhash;
+        d.select_hash = dbg.shash;
+        d.n_light_picks = dbg.picks;
+        d.prim_instance = gHit ? g.inst : -1;
+        d.prim_triangle = gHit ? g.tri : -1;
+        d.prim_t = gHit ? g.depth : 0.0f;
+        d.sample_rgb[0] = rgb.x;
+        d.sample_rgb[1] = rgb.y;
+        d.sample_rgb[2] = rgb.z;
+        a.dbg[pixel] = d;
+    }
+    if (kCount) {
+        unsigned long long *c = a.counters;
+        atomicAdd(c + 0, (unsigned long long)st.primary);
+        atomicAdd(c + 1, (unsigned long long)st.bounce);
+        atomicAdd(c + 2, (unsigned long long)st.shadow);
+        atomicAdd(c + 3, (unsigned long long)st.cand);
+        atomicAdd(c + 4, (unsigned long long)st.cand_skipped);
+        atomicAdd(c + 5, (unsigned long long)st.ts.nodes);
+        atomicAdd(c + 6, (unsigned long long)st.ts.tris);
+        atomicAdd(c + 7, (unsigned long long)st.ts.instances);
+        atomicAdd(c + 8, (unsigned long long)rng.k);
+    }
+}
+
+// Out-of-place 3x3 firefly clamp (the reference's GPU accumulation shader semantics,
+// accumulation.slang:55-81; threshold 4 * max neighbour luminance + 0.5) + accumulation.
+__global__ void firefly_accumulate_kernel(const float4 *frame, float4 *accum, float4 *color, int W, int H,
+                                          float invN) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    auto lum = [](float4 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; };
+    const int i = y * W + x;
+    float4 c = frame[i];
+    const float l = lum(c);
+    float maxN = 0.0f;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+            if (dx == 0 && dy == 0) continue;
+            const int nx = iclamp(x + dx, 0, W - 1), ny = iclamp(y + dy, 0, H - 1);
+            maxN = smax(maxN, lum(frame[ny * W + nx]));
+        }
+    const float thr = maxN * 4.0f + 0.5f;
+    if (l > thr && l > 0.0f) {
+        const float s = thr / l;
+        c.x *= s;
+        c.y *= s;
+        c.z *= s;
+    }
+    float4 acc = accum[i];
+    acc.x += c.x;
+    acc.y += c.y;
+    acc.z += c.z;
+    acc.w += c.w;
+    accum[i] = acc;
+    color[i] = make_float4(acc.x * invN, acc.y * invN, acc.z * invN, acc.w * invN);
+}
+
+__global__ void rng_kernel(const uint32_t *seeds, int n, int nd, uint64_t *raw, float *flt, uint64_t *spill) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Mt64 r;
+    r.init(seeds[i], spill ? spill + size_t(i) * kMtN : nullptr);
+    for (int k = 0; k < nd; ++k) {
+        const uint64_t v = r.next_raw();
+        if (raw) raw[size_t(i) * nd + k] = v;
+        if (flt) flt[size_t(i) * nd + k] = mt_canonical(v);
+    }
+}
+
+__global__ void intersect_kernel(DeviceScene s, const float *o, const float *d, int n, int *inst, int *tri,
+                                 float *t) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    TraceStats ts = {};
+    RawHit h;
+    const V3 oo = v3(o[3 * i], o[3 * i + 1], o[3 * i + 2]);
+    const V3 dd = v3(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+    if (trace_closest<false>(s, oo, dd, h, ts)) {
+        const HitRec r = make_hit_record(s, oo, dd, h);
+        inst[i] = r.inst;
+        tri[i] = r.tri;
+        t[i] = r.depth;
+    } else {
+        inst[i] = -1;
+        tri[i] = -1;
+        t[i] = 0.0f;
+    }
+}
+
+// ============================================================================ host side
+
+namespace {
+
+thread_local std::string g_err;
+
+hdr_status fail(hdr_status s, const std::string &msg) {
+    g_err = msg;
+    return s;
+}
+
+#define HIP_TRY(expr)                                                                                    \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            return fail(e_ == hipErrorOutOfMemory ? HDR_ERR_OUT_OF_MEMORY : HDR_ERR_DEVICE,              \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                              \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t bytes) {
+        n = bytes;
+        return hipMalloc(&p, bytes ? bytes : 1);
+    }
+};
+
+}  // namespace
+
+struct hdr_context {
+    int device = 0;
+    int cancelled = 0;
+};
+
+struct hdr_scene {
+    hdr_context *ctx = nullptr;
+    DevBuf mem;
+    DeviceScene ds{};
+    HostScene hs;
+    uint64_t bytes = 0;
+    int max_sample_draws_per_vertex = 3;  // SampleBounce draws: 3, +3 per coat / sheen layer present
+    bool any_translucent = false;         // some material has opacity < 0.999 (unbounded draws)
+};
+
+struct hdr_view {
+    hdr_context *ctx = nullptr;
+    int W = 0, H = 0;
+    hipStream_t stream = nullptr;
+    DevBuf res, accum, color, frame, dbg, depth, albedo, normal, counters, spill;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    int counting = 0;
+    int frames = 0;
+    float last_integrate_ms = 0.0f, last_frame_ms = 0.0f;
+    bool timing_pending = false;
+};
+
+extern "C" {
+
+const char *hdr_last_error(void) { return g_err.c_str(); }
+uint32_t hdr_abi_version(void) { return HDRGPU_ABI_VERSION; }
+
+hdr_status hdr_context_create(int32_t device, hdr_context **out) {
+    if (!out) return fail(HDR_ERR_INVALID_ARGUMENT, "out is null");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(HDR_ERR_INVALID_ARGUMENT, "invalid device index");
+    HIP_TRY(hipSetDevice(device));
+    auto *c = new (std::nothrow) hdr_context();
+    if (!c) return fail(HDR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    c->device = device;
+    *out = c;
+    return HDR_OK;
+}
+
+hdr_status hdr_context_destroy(hdr_context *ctx) {
+    delete ctx;
+    return HDR_OK;
+}
+
+hdr_status hdr_context_set_cancel(hdr_context *ctx, int32_t cancelled) {
+    if (!ctx) return fail(HDR_ERR_INVALID_ARGUMENT, "ctx is null");
+    ctx->cancelled = cancelled;
+    return HDR_OK;
+}
+
+hdr_status hdr_scene_upload(hdr_context *ctx, const hdr_scene_desc *desc, hdr_scene **out) {
+    if (!ctx || !desc || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    std::unique_ptr<hdr_scene> sc(new (std::nothrow) hdr_scene());
+    if (!sc) return fail(HDR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    sc->ctx = ctx;
+    const std::string err = build_host_scene(*desc, sc->hs);
+    if (!err.empty()) return fail(HDR_ERR_INVALID_ARGUMENT, err);
+    HostScene &h = sc->hs;
+    for (int i = 0; i < desc->num_materials; ++i) {
+        if (desc->materials[i].coat > 0.0f) sc->max_sample_draws_per_vertex += 3;
+        if (desc->materials[i].sheen > 0.0f) sc->max_sample_draws_per_vertex += 3;
+        if (desc->materials[i].opacity < 0.999f) sc->any_translucent = true;
+    }
+    sc->max_sample_draws_per_vertex = std::min(sc->max_sample_draws_per_vertex, 9);
+    // one allocation, 256-byte aligned sub-ranges
+    size_t off = 0;
+    auto place = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~size_t(255);
+        return o;
+    };
+    const size_t oNodes = place(h.nodes.size() * sizeof(GNode));
+    const size_t oTris = place(h.tris.size() * sizeof(GTri));
+    const size_t oAttrs = place(h.attrs.size() * sizeof(GTriAttr));
+    const size_t oTidx = place(h.tlas_idx.size() * sizeof(int));
+    const size_t oInst = place(h.instances.size() * sizeof(GInstance));
+    const size_t oMat = place(h.materials.size() * sizeof(GMaterial));
+    const size_t oLights = place(h.lights.size() * sizeof(GLight));
+    const size_t oDomes = place(h.domes.size() * sizeof(GDome));
+    std::vector<size_t> oDomePix(h.domes.size()), oDomeRow(h.domes.size()), oDomeCol(h.domes.size());
+    for (size_t k = 0; k < h.domes.size(); ++k) {
+        oDomePix[k] = place(h.domes[k].pixels.size() * sizeof(float));
+        oDomeRow[k] = place(h.domes[k].row_cdf.size() * sizeof(float));
+        oDomeCol[k] = place(h.domes[k].col_cdf.size() * sizeof(float));
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(sc->mem.alloc(off));
+    char *base = static_cast<char *>(sc->mem.p);
+    auto up = [&](size_t o, const void *src, size_t bytes) -> hipError_t {
+        return bytes ? hipMemcpy(base + o, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    HIP_TRY(up(oNodes, h.nodes.data(), h.nodes.size() * sizeof(GNode)));
+    HIP_TRY(up(oTris, h.tris.data(), h.tris.size() * sizeof(GTri)));
+    HIP_TRY(up(oAttrs, h.attrs.data(), h.attrs.size() * sizeof(GTriAttr)));
+    HIP_TRY(up(oTidx, h.tlas_idx.data(), h.tlas_idx.size() * sizeof(int)));
+    HIP_TRY(up(oInst, h.instances.data(), h.instances.size() * sizeof(GInstance)));
+    HIP_TRY(up(oMat, h.materials.data(), h.materials.size() * sizeof(GMaterial)));
+    HIP_TRY(up(oLights, h.lights.data(), h.lights.size() * sizeof(GLight)));
+    std::vector<GDome> gd(h.domes.size());
+    for (size_t k = 0; k < h.domes.size(); ++k) {
+        HIP_TRY(up(oDomePix[k], h.domes[k].pixels.data(), h.domes[k].pixels.size() * sizeof(float)));
+        HIP_TRY(up(oDomeRow[k], h.domes[k].row_cdf.data(), h.domes[k].row_cdf.size() * sizeof(float)));
+        HIP_TRY(up(oDomeCol[k], h.domes[k].col_cdf.data(), h.domes[k].col_cdf.size() * sizeof(float)));
+        gd[k].pixels = reinterpret_cast<const float *>(base + oDomePix[k]);
+        gd[k].row_cdf = reinterpret_cast<const float *>(base + oDomeRow[k]);
+        gd[k].col_cdf = reinterpret_cast<const float *>(base + oDomeCol[k]);
+        gd[k].w = h.domes[k].w;
+        gd[k].h = h.domes[k].h;
+        gd[k].total_lum = h.domes[k].total_lum;
+        gd[k].pdf_scale = h.domes[k].pdf_scale;
+    }
+    HIP_TRY(up(oDomes, gd.data(), gd.size() * sizeof(GDome)));
+    DeviceScene &ds = sc->ds;
+    ds.nodes = reinterpret_cast<const GNode *>(base + oNodes);
+    ds.tris = reinterpret_cast<const GTri *>(base + oTris);
+    ds.attrs = reinterpret_cast<const GTriAttr *>(base + oAttrs);
+    ds.tlas_idx = reinterpret_cast<const int *>(base + oTidx);
+    ds.instances = reinterpret_cast<const GInstance *>(base + oInst);
+    ds.materials = reinterpret_cast<const GMaterial *>(base + oMat);
+    ds.lights = reinterpret_cast<const GLight *>(base + oLights);
+    ds.domes = reinterpret_cast<const GDome *>(base + oDomes);
+    ds.tlas_root = h.tlas_root;
+    ds.num_instances = int(h.instances.size());
+    ds.num_materials = desc->num_materials;
+    ds.num_lights = int(h.lights.size());
+    ds.environment = h.environment;
+    ds.env_dome = h.env_dome;
+    ds.has_mesh_lights = h.has_mesh_lights;
+    ds.has_emission = h.has_emission;
+    sc->bytes = off;
+    *out = sc.release();
+    return HDR_OK;
+}
+
+hdr_status hdr_scene_destroy(hdr_scene *scene) {
+    if (scene) (void)hipSetDevice(scene->ctx->device);
+    delete scene;
+    return HDR_OK;
+}
+
+hdr_status hdr_scene_device_bytes(const hdr_scene *scene, uint64_t *out) {
+    if (!scene || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    *out = scene->bytes;
+    return HDR_OK;
+}
+
+hdr_status hdr_view_create(hdr_context *ctx, int32_t width, int32_t height, hdr_view **out) {
+    if (!ctx || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    if (width <= 0 || height <= 0 || int64_t(width) * height > (int64_t(1) << 28))
+        return fail(HDR_ERR_INVALID_ARGUMENT, "invalid view size");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<hdr_view> v(new (std::nothrow) hdr_view());
+    if (!v) return fail(HDR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    v->ctx = ctx;
+    v->W = width;
+    v->H = height;
+    const size_t P = size_t(width) * height;
+    HIP_TRY(v->res.alloc(P * sizeof(hdr_reservoir)));
+    HIP_TRY(v->accum.alloc(P * sizeof(float4)));
+    HIP_TRY(v->color.alloc(P * sizeof(float4)));
+    HIP_TRY(v->counters.alloc(sizeof(hdr_frame_counters)));
+    HIP_TRY(hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&v->ev0));
+    HIP_TRY(hipEventCreate(&v->ev1));
+    HIP_TRY(hipEventCreate(&v->ev2));
+    HIP_TRY(hipMemsetAsync(v->res.p, 0, v->res.n, v->stream));
+    HIP_TRY(hipMemsetAsync(v->accum.p, 0, v->accum.n, v->stream));
+    HIP_TRY(hipMemsetAsync(v->color.p, 0, v->color.n, v->stream));
+    HIP_TRY(hipMemsetAsync(v->counters.p, 0, v->counters.n, v->stream));
+    HIP_TRY(hipStreamSynchronize(v->stream));
+    *out = v.release();
+    return HDR_OK;
+}
+
+hdr_status hdr_view_destroy(hdr_view *view) {
+    if (!view) return HDR_OK;
+    (void)hipSetDevice(view->ctx->device);
+    if (view->stream) (void)hipStreamSynchronize(view->stream);
+    if (view->ev0) (void)hipEventDestroy(view->ev0);
+    if (view->ev1) (void)hipEventDestroy(view->ev1);
+    if (view->ev2) (void)hipEventDestroy(view->ev2);
+    if (view->stream) (void)hipStreamDestroy(view->stream);
+    delete view;
+    return HDR_OK;
+}
+
+hdr_status hdr_view_reset(hdr_view *view) {
+    if (!view) return fail(HDR_ERR_INVALID_ARGUMENT, "view is null");
+    HIP_TRY(hipSetDevice(view->ctx->device));
+    HIP_TRY(hipMemsetAsync(view->res.p, 0, view->res.n, view->stream));
+    HIP_TRY(hipMemsetAsync(view->accum.p, 0, view->accum.n, view->stream));
+    view->frames = 0;
+    return HDR_OK;
+}
+
+hdr_status hdr_view_device_bytes(const hdr_view *v, uint64_t *out) {
+    if (!v || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    *out = v->res.n + v->accum.n + v->color.n + v->frame.n + v->dbg.n + v->depth.n + v->albedo.n + v->normal.n +
+           v->counters.n + v->spill.n;
+    return HDR_OK;
+}
+
+hdr_status hdr_view_set_counting(hdr_view *view, int32_t enable) {
+    if (!view) return fail(HDR_ERR_INVALID_ARGUMENT, "view is null");
+    view->counting = enable != 0;
+    return HDR_OK;
+}
+
+static hdr_status ensure(DevBuf &b, size_t bytes) {
+    if (b.n >= bytes && b.p) return HDR_OK;
+    if (b.p) {
+        (void)hipFree(b.p);
+        b.p = nullptr;
+    }
+    HIP_TRY(b.alloc(bytes));
+    return HDR_OK;
+}
+
+hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
+                            const hdr_pipeline_desc *pipe, float *color_out, hdr_pixel_debug *debug_out) {
+    if (!view || !scene || !frame || !pipe) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    if (frame->struct_size != sizeof(hdr_frame_desc)) return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_frame_desc size");
+    if (pipe->struct_size != sizeof(hdr_pipeline_desc))
+        return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_pipeline_desc size");
+    if (view->ctx->cancelled) return fail(HDR_ERR_CANCELLED, "render cancelled");
+    if (pipe->direct_light != HDR_DIRECT_MIS && pipe->direct_light != HDR_DIRECT_RIS)
+        return fail(HDR_ERR_INVALID_ARGUMENT, "unknown direct_light kind");
+    if (pipe->max_depth < 1 || pipe->max_depth > 1024) return fail(HDR_ERR_INVALID_ARGUMENT, "max_depth");
+    if (pipe->candidate_count < 1 || pipe->candidate_count > 64)
+        return fail(HDR_ERR_INVALID_ARGUMENT, "candidate_count must be in [1, 64]");
+    if (pipe->spatial_count != 0) return fail(HDR_ERR_UNSUPPORTED, "spatial reuse is not available in this build");
+    HIP_TRY(hipSetDevice(view->ctx->device));
+    const int W = view->W, H = view->H;
+    const size_t P = size_t(W) * H;
+
+    FrameArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.s = scene->ds;
+    double m[4][4], inv[4][4];
+    for (int i = 0; i < 16; ++i) m[i / 4][i % 4] = frame->view[i];
+    invert4(m, inv);
+    for (int i = 0; i < 16; ++i) a.inv_view[i] = inv[i / 4][i % 4];
+    for (int i = 0; i < 16; ++i) m[i / 4][i % 4] = frame->proj[i];
+    invert4(m, inv);
+    for (int i = 0; i < 16; ++i) a.inv_proj[i] = inv[i / 4][i % 4];
+    a.W = W;
+    a.H = H;
+    a.frame_index = frame->frame_index;
+    a.has_cam = frame->has_camera_params != 0;
+    a.cam = frame->camera;
+    if (a.has_cam && a.cam.enable_exposure) {  // GetExposureMultiplier, camera_params.h:23-30
+        const float safeF = std::max(a.cam.f_stop, 0.1f);
+        a.exposure_mul = (a.cam.iso / 100.0f) * a.cam.shutter_speed / (safeF * safeF) * 100.0f;
+    } else {
+        a.exposure_mul = 1.0f;
+    }
+    a.direct = pipe->direct_light;
+    a.use_res = pipe->use_reservoir != 0;
+    a.skip_vis = pipe->skip_visibility != 0;
+    a.max_depth = pipe->max_depth;
+    a.max_refl = pipe->max_reflection_bounces;
+    a.max_refr = pipe->max_refraction_bounces;
+    a.enable_ss = pipe->enable_subsurface != 0;
+    a.ibl_bg = pipe->render_ibl_background != 0;
+    a.roulette = pipe->roulette_aggressiveness;
+    // technique split (ris_direct_light_integrator.cpp:93-100); the environment is always one of the
+    // lights, so the light sampler covers it and no separate sky technique exists.
+    const int M = pipe->candidate_count;
+    a.n_nee = std::max(1, M / 2);
+    a.n_sky = 0;
+    a.n_bsdf = std::max(0, M - a.n_nee - a.n_sky);
+    a.n_cand = a.n_nee + a.n_sky + a.n_bsdf;
+    a.trace_generated = (scene->ds.has_mesh_lights || scene->ds.environment >= 0) ? 1 : 0;
+    a.trace_last_connection = (scene->ds.has_emission || scene->ds.environment >= 0) ? 1 : 0;
+    a.light_select_pdf = scene->ds.num_lights > 0 ? 1.0f / float(scene->ds.num_lights) : 0.0f;
+    a.firefly = pipe->enable_firefly_filter != 0;
+    a.inv_n = 1.0f / float(frame->frame_index + 1);
+    a.res = static_cast<hdr_reservoir *>(view->res.p);
+    a.accum = static_cast<float4 *>(view->accum.p);
+    a.color = static_cast<float4 *>(view->color.p);
+
+    // upper bound of RNG draws per pixel; past 311 the engine needs its spill slots
+    int bound = 5;
+    {
+        const int sb = scene->max_sample_draws_per_vertex;
+        int perVertex = 1 + sb + 1;  // opacity, bounce sample, roulette
+        if (a.direct == HDR_DIRECT_MIS) perVertex += 3;
+        else perVertex += a.n_nee * 3 + std::max(0, a.n_bsdf - 1) * sb + a.n_cand + 1;
+        bound += pipe->max_depth * perVertex;
+    }
+    if (scene->any_translucent || bound >= kMtN) {
+        if (hdr_status s = ensure(view->spill, P * 2 * kMtN * sizeof(uint64_t))) return s;
+        a.spill = static_cast<uint64_t *>(view->spill.p);
+    }
+    if (a.firefly) {
+        if (hdr_status s = ensure(view->frame, P * sizeof(float4))) return s;
+        a.frame_rgb = static_cast<float4 *>(view->frame.p);
+    }
+    if (debug_out) {
+        if (hdr_status s = ensure(view->dbg, P * sizeof(hdr_pixel_debug))) return s;
+        a.dbg = static_cast<hdr_pixel_debug *>(view->dbg.p);
+    }
+    // AOVs are written every frame once allocated (first read allocates them)
+    if (view->depth.p) {
+        a.aov_depth = static_cast<float *>(view->depth.p);
+        a.aov_albedo = static_cast<float *>(view->albedo.p);
+        a.aov_normal = static_cast<float *>(view->normal.p);
+    }
+    if (view->counting) {
+        HIP_TRY(hipMemsetAsync(view->counters.p, 0, view->counters.n, view->stream));
+        a.counters = static_cast<unsigned long long *>(view->counters.p);
+    }
+    const dim3 grid((W + 7) / 8, (H + 7) / 8);
+    const size_t lds = size_t(a.direct == HDR_DIRECT_RIS ? a.n_cand : 1) * kLanes * sizeof(double);
+    HIP_TRY(hipEventRecord(view->ev0, view->stream));
+    if (view->counting) {
+        if (debug_out) hipLaunchKernelGGL((integrate_kernel<true, true>), grid, dim3(kLanes), lds, view->stream, a);
+        else hipLaunchKernelGGL((integrate_kernel<true, false>), grid, dim3(kLanes), lds, view->stream, a);
+    } else {
+        if (debug_out) hipLaunchKernelGGL((integrate_kernel<false, true>), grid, dim3(kLanes), lds, view->stream, a);
+        else hipLaunchKernelGGL((integrate_kernel<false, false>), grid, dim3(kLanes), lds, view->stream, a);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(view->ev1, view->stream));
+    if (a.firefly) {
+        const dim3 b2(16, 16), g2((W + 15) / 16, (H + 15) / 16);
+        hipLaunchKernelGGL(firefly_accumulate_kernel, g2, b2, 0, view->stream, a.frame_rgb, a.accum, a.color, W, H,
+                           a.inv_n);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(view->ev2, view->stream));
+    view->timing_pending = true;
+    view->frames++;
+    if (color_out) {
+        HIP_TRY(hipMemcpyAsync(color_out, view->color.p, P * sizeof(float4), hipMemcpyDeviceToHost, view->stream));
+    }
+    if (debug_out) {
+        HIP_TRY(hipMemcpyAsync(debug_out, view->dbg.p, P * sizeof(hdr_pixel_debug), hipMemcpyDeviceToHost,
+                               view->stream));
+    }
+    if (color_out || debug_out) HIP_TRY(hipStreamSynchronize(view->stream));
+    return HDR_OK;
+}
+
+hdr_status hdr_view_synchronize(hdr_view *view) {
+    if (!view) return fail(HDR_ERR_INVALID_ARGUMENT, "view is null");
+    HIP_TRY(hipSetDevice(view->ctx->device));
+    HIP_TRY(hipStreamSynchronize(view->stream));
+    return HDR_OK;
+}
+
+hdr_status hdr_view_last_timing(hdr_view *view, float *integrate_ms, float *frame_ms) {
+    if (!view) return fail(HDR_ERR_INVALID_ARGUMENT, "view is null");
+    if (view->timing_pending) {
+        HIP_TRY(hipEventSynchronize(view->ev2));
+        HIP_TRY(hipEventElapsedTime(&view->last_integrate_ms, view->ev0, view->ev1));
+        HIP_TRY(hipEventElapsedTime(&view->last_frame_ms, view->ev0, view->ev2));
+        view->timing_pending = false;
+    }
+    if (integrate_ms) *integrate_ms = view->last_integrate_ms;
+    if (frame_ms) *frame_ms = view->last_frame_ms;
+    return HDR_OK;
+}
+
+hdr_status hdr_view_read_color(hdr_view *view, float *rgba_out) {
+    if (!view || !rgba_out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipSetDevice(view->ctx->device));
+    HIP_TRY(hipMemcpyAsync(rgba_out, view->color.p, view->color.n, hipMemcpyDeviceToHost, view->stream));
+    HIP_TRY(hipStreamSynchronize(view->stream));
+    return HDR_OK;
+}
+
+hdr_status hdr_view_read_reservoirs(hdr_view *view, hdr_reservoir *out) {
+    if (!view || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipSetDevice(view->ctx->device));
+    HIP_TRY(hipMemcpyAsync(out, view->res.p, view->res.n, hipMemcpyDeviceToHost, view->stream));
+    HIP_TRY(hipStreamSynchronize(view->stream));
+    return HDR_OK;
+}
+
+hdr_status hdr_view_read_aovs(hdr_view *view, float *depth, float *albedo, float *normal) {
+    if (!view) return fail(HDR_ERR_INVALID_ARGUMENT, "view is null");
+    HIP_TRY(hipSetDevice(view->ctx->device));
+    const size_t P = size_t(view->W) * view->H;
+    if (!view->depth.p) {
+        // first request: allocate; AOVs are produced from the next frame on
+        if (hdr_status s = ensure(view->depth, P * sizeof(float))) return s;
+        if (hdr_status s = ensure(view->albedo, P * 3 * sizeof(float))) return s;
+        if (hdr_status s = ensure(view->normal, P * 3 * sizeof(float))) return s;
+        HIP_TRY(hipMemsetAsync(view->depth.p, 0, view->depth.n, view->stream));
+        HIP_TRY(hipMemsetAsync(view->albedo.p, 0, view->albedo.n, view->stream));
+        HIP_TRY(hipMemsetAsync(view->normal.p, 0, view->normal.n, view->stream));
+    }
+    if (depth) HIP_TRY(hipMemcpyAsync(depth, view->depth.p, P * sizeof(float), hipMemcpyDeviceToHost, view->stream));
+    if (albedo)
+        HIP_TRY(hipMemcpyAsync(albedo, view->albedo.p, P * 3 * sizeof(float), hipMemcpyDeviceToHost, view->stream));
+    if (normal)
+        HIP_TRY(hipMemcpyAsync(normal, view->normal.p, P * 3 * sizeof(float), hipMemcpyDeviceToHost, view->stream));
+    HIP_TRY(hipStreamSynchronize(view->stream));
+    return HDR_OK;
+}
+
+hdr_status hdr_view_color_device_ptr(hdr_view *view, void **out) {
+    if (!view || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    *out = view->color.p;
+    return HDR_OK;
+}
+
+hdr_status hdr_view_read_counters(hdr_view *view, hdr_frame_counters *out) {
+    if (!view || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipSetDevice(view->ctx->device));
+    HIP_TRY(hipMemcpyAsync(out, view->counters.p, sizeof(hdr_frame_counters), hipMemcpyDeviceToHost, view->stream));
+    HIP_TRY(hipStreamSynchronize(view->stream));
+    return HDR_OK;
+}
+
+hdr_status hdr_scene_intersect(hdr_scene *scene, const float *origins, const float *dirs, int32_t n,
+                               int32_t *instance_out, int32_t *triangle_out, float *t_out) {
+    if (!scene || n < 0 || (n && (!origins || !dirs || !instance_out || !triangle_out || !t_out)))
+        return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    if (n == 0) return HDR_OK;
+    HIP_TRY(hipSetDevice(scene->ctx->device));
+    DevBuf dO, dD, dI, dT, dt;
+    HIP_TRY(dO.alloc(size_t(n) * 12));
+    HIP_TRY(dD.alloc(size_t(n) * 12));
+    HIP_TRY(dI.alloc(size_t(n) * 4));
+    HIP_TRY(dT.alloc(size_t(n) * 4));
+    HIP_TRY(dt.alloc(size_t(n) * 4));
+    HIP_TRY(hipMemcpy(dO.p, origins, size_t(n) * 12, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dD.p, dirs, size_t(n) * 12, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(intersect_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, scene->ds,
+                       static_cast<const float *>(dO.p), static_cast<const float *>(dD.p), n,
+                       static_cast<int *>(dI.p), static_cast<int *>(dT.p), static_cast<float *>(dt.p));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(instance_out, dI.p, size_t(n) * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(triangle_out, dT.p, size_t(n) * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(t_out, dt.p, size_t(n) * 4, hipMemcpyDeviceToHost));
+    return HDR_OK;
+}
+
+static hdr_status rng_common(hdr_context *ctx, const uint32_t *seeds, int32_t n, int32_t nd, uint64_t *raw,
+                             float *flt) {
+    if (!ctx || !seeds || n < 0 || nd < 0) return fail(HDR_ERR_INVALID_ARGUMENT, "invalid argument");
+    if (n == 0 || nd == 0) return HDR_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    DevBuf dS, dOut, dSpill;
+    HIP_TRY(dS.alloc(size_t(n) * 4));
+    HIP_TRY(dOut.alloc(size_t(n) * nd * (raw ? 8 : 4)));
+    if (nd > 311) HIP_TRY(dSpill.alloc(size_t(n) * kMtN * 8));
+    HIP_TRY(hipMemcpy(dS.p, seeds, size_t(n) * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(rng_kernel, dim3((n + 63) / 64), dim3(64), 0, 0, static_cast<const uint32_t *>(dS.p), n, nd,
+                       raw ? static_cast<uint64_t *>(dOut.p) : nullptr, raw ? nullptr : static_cast<float *>(dOut.p),
+                       static_cast<uint64_t *>(dSpill.p));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(raw ? static_cast<void *>(raw) : static_cast<void *>(flt), dOut.p, dOut.n,
+                      hipMemcpyDeviceToHost));
+    return HDR_OK;
+}
+
+hdr_status hdr_rng_raw(hdr_context *ctx, const uint32_t *seeds, int32_t n, int32_t n_draws, uint64_t *out) {
+    if (!out) return fail(HDR_ERR_INVALID_ARGUMENT, "out is null");
+    return rng_common(ctx, seeds, n, n_draws, out, nullptr);
+}
+
+hdr_status hdr_rng_float(hdr_context *ctx, const uint32_t *seeds, int32_t n, int32_t n_draws, float *out) {
+    if (!out) return fail(HDR_ERR_INVALID_ARGUMENT, "out is null");
+    return rng_common(ctx, seeds, n, n_draws, nullptr, out);
+}
+
+}  // extern "C"

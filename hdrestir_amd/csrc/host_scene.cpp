@@ -1,0 +1,490 @@
+// host_scene.cpp -- builds the device scene layout on the host.
+//
+// BLAS: BVH::Build / BVH::_Subdivide (source/renderContext/bvh.cpp:66-241): midpoint split on
+// the longest axis of the node bounds, leaf <= 4 triangles, in-place partition by centroid with
+// the same swap sequence, fallback to a median split. TLAS: Scene::_BuildTLAS /
+// _SubdivideTLAS (source/scene/scene.cpp:333-407), leaf <= 2 instances. Reproducing the trees
+// node for node keeps the traversal order -- and the choice among equal-t hits -- identical.
+// Light, material and dome constants are evaluated here once with the float operations the
+// reference performs per sample (rect_light.cpp, point_light.cpp, distant_light.cpp,
+// dome_light.cpp:21-77, preview_surface.cpp:61-124).
+#include "host_scene.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+namespace hdr {
+namespace {
+
+struct F3 {
+    float x, y, z;
+};
+inline F3 add(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline F3 sub(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline F3 mulf(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline F3 muld(F3 a, double s) { return {float(double(a.x) * s), float(double(a.y) * s), float(double(a.z) * s)}; }
+inline float dotf(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline F3 normalizedf(F3 a) {
+    const float len = std::sqrt(dotf(a, a));
+    const float eps = float(1e-10);
+    return muld(a, 1.0 / double(len > eps ? len : eps));
+}
+inline float comp(F3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+struct Box {  // GfRange3f
+    F3 mn{FLT_MAX, FLT_MAX, FLT_MAX}, mx{-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    void extend(F3 p) {
+        if (p.x < mn.x) mn.x = p.x;
+        if (p.y < mn.y) mn.y = p.y;
+        if (p.z < mn.z) mn.z = p.z;
+        if (p.x > mx.x) mx.x = p.x;
+        if (p.y > mx.y) mx.y = p.y;
+        if (p.z > mx.z) mx.z = p.z;
+    }
+    bool empty() const { return mn.x > mx.x || mn.y > mx.y || mn.z > mx.z; }
+    F3 size() const { return sub(mx, mn); }
+};
+
+GNode make_node(const Box &b, int left, int count) {
+    GNode n;
+    n.mnx = b.mn.x;
+    n.mny = b.mn.y;
+    n.mnz = b.mn.z;
+    n.mxx = b.mx.x;
+    n.mxy = b.mx.y;
+    n.mxz = b.mx.z;
+    n.left = left;
+    n.count = count;
+    return n;
+}
+
+struct BuildTri {
+    F3 v0, v1, v2, centroid;
+    int orig;
+    GTriAttr attr;
+};
+
+struct BlasBuilder {
+    std::vector<BuildTri> tris;
+    std::vector<GNode> nodes;  // local indices; leaf left = -(local start) - 1
+    std::vector<Box> bounds;
+    int depth = 0;
+
+    void subdivide(int nodeIdx, int start, int end, int d) {
+        depth = std::max(depth, d);
+        Box b;
+        for (int i = start; i < end; ++i) {
+            b.extend(tris[i].v0);
+            b.extend(tris[i].v1);
+            b.extend(tris[i].v2);
+        }
+        const int count = end - start;
+        if (count <= 4) {
+            nodes[nodeIdx] = make_node(b, -start - 1, count);
+            return;
+        }
+        const F3 size = b.size();
+        int axis = 0;
+        if (size.y > size.x) axis = 1;
+        if (size.z > comp(size, axis)) axis = 2;
+        const float splitPos = comp(b.mn, axis) + comp(size, axis) * 0.5f;
+        int i = start, j = end - 1;
+        while (i <= j) {
+            if (comp(tris[i].centroid, axis) < splitPos) {
+                i++;
+            } else {
+                std::swap(tris[i], tris[j]);
+                j--;
+            }
+        }
+        const int leftCount = i - start;
+        if (leftCount == 0 || leftCount == count) i = start + count / 2;
+        const int leftIdx = int(nodes.size());
+        nodes.push_back(GNode{});
+        nodes.push_back(GNode{});
+        nodes[nodeIdx] = make_node(b, leftIdx, 0);
+        subdivide(leftIdx, start, i, d + 1);
+        subdivide(leftIdx + 1, i, end, d + 1);
+    }
+};
+
+}  // namespace
+
+void invert4(const double a[4][4], double out[4][4]) {
+    const double s0 = a[0][0] * a[1][1] - a[1][0] * a[0][1];
+    const double s1 = a[0][0] * a[1][2] - a[1][0] * a[0][2];
+    const double s2 = a[0][0] * a[1][3] - a[1][0] * a[0][3];
+    const double s3 = a[0][1] * a[1][2] - a[1][1] * a[0][2];
+    const double s4 = a[0][1] * a[1][3] - a[1][1] * a[0][3];
+    const double s5 = a[0][2] * a[1][3] - a[1][2] * a[0][3];
+    const double c5 = a[2][2] * a[3][3] - a[3][2] * a[2][3];
+    const double c4 = a[2][1] * a[3][3] - a[3][1] * a[2][3];
+    const double c3 = a[2][1] * a[3][2] - a[3][1] * a[2][2];
+    const double c2 = a[2][0] * a[3][3] - a[3][0] * a[2][3];
+    const double c1 = a[2][0] * a[3][2] - a[3][0] * a[2][2];
+    const double c0 = a[2][0] * a[3][1] - a[3][0] * a[2][1];
+    const double det = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+    if (det == 0.0) {
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) out[i][j] = (i == j) ? double(FLT_MAX) : 0.0;
+        return;
+    }
+    const double r = 1.0 / det;
+    out[0][0] = (a[1][1] * c5 - a[1][2] * c4 + a[1][3] * c3) * r;
+    out[0][1] = (-a[0][1] * c5 + a[0][2] * c4 - a[0][3] * c3) * r;
+    out[0][2] = (a[3][1] * s5 - a[3][2] * s4 + a[3][3] * s3) * r;
+    out[0][3] = (-a[2][1] * s5 + a[2][2] * s4 - a[2][3] * s3) * r;
+    out[1][0] = (-a[1][0] * c5 + a[1][2] * c2 - a[1][3] * c1) * r;
+    out[1][1] = (a[0][0] * c5 - a[0][2] * c2 + a[0][3] * c1) * r;
+    out[1][2] = (-a[3][0] * s5 + a[3][2] * s2 - a[3][3] * s1) * r;
+    out[1][3] = (a[2][0] * s5 - a[2][2] * s2 + a[2][3] * s1) * r;
+    out[2][0] = (a[1][0] * c4 - a[1][1] * c2 + a[1][3] * c0) * r;
+    out[2][1] = (-a[0][0] * c4 + a[0][1] * c2 - a[0][3] * c0) * r;
+    out[2][2] = (a[3][0] * s4 - a[3][1] * s2 + a[3][3] * s0) * r;
+    out[2][3] = (-a[2][0] * s4 + a[2][1] * s2 - a[2][3] * s0) * r;
+    out[3][0] = (-a[1][0] * c3 + a[1][1] * c1 - a[1][2] * c0) * r;
+    out[3][1] = (a[0][0] * c3 - a[0][1] * c1 + a[0][2] * c0) * r;
+    out[3][2] = (-a[3][0] * s3 + a[3][1] * s1 - a[3][2] * s0) * r;
+    out[3][3] = (a[2][0] * s3 - a[2][1] * s1 + a[2][2] * s0) * r;
+}
+
+namespace {
+
+// GfMatrix4f::Transform (float, with GfProject)
+F3 xform_point_h(const float *m, F3 p) {
+    const float x = p.x * m[0] + p.y * m[4] + p.z * m[8] + m[12];
+    const float y = p.x * m[1] + p.y * m[5] + p.z * m[9] + m[13];
+    const float z = p.x * m[2] + p.y * m[6] + p.z * m[10] + m[14];
+    const float w = p.x * m[3] + p.y * m[7] + p.z * m[11] + m[15];
+    const float inv = (w != 0.0f) ? 1.0f / w : 1.0f;
+    return {inv * x, inv * y, inv * z};
+}
+F3 xform_dir_h(const float *m, F3 p) {
+    return {p.x * m[0] + p.y * m[4] + p.z * m[8], p.x * m[1] + p.y * m[5] + p.z * m[9],
+            p.x * m[2] + p.y * m[6] + p.z * m[10]};
+}
+
+bool finite_all(const float *p, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+}  // namespace
+
+std::string build_host_scene(const hdr_scene_desc &d, HostScene &out) {
+    if (d.struct_size != sizeof(hdr_scene_desc)) return "hdr_scene_desc.struct_size mismatch";
+    if (d.num_meshes < 0 || d.num_materials < 0 || d.num_instances < 0 || d.num_lights < 0)
+        return "negative element count in hdr_scene_desc";
+    if ((d.num_meshes && !d.meshes) || (d.num_materials && !d.materials) || (d.num_instances && !d.instances) ||
+        (d.num_lights && !d.lights))
+        return "null array with a non-zero count in hdr_scene_desc";
+    if (d.environment >= d.num_lights || (d.environment >= 0 && d.lights[d.environment].type != HDR_LIGHT_DOME))
+        return "environment must index a dome light";
+
+    // ---- BLAS per mesh (BVH::Build, bvh.cpp:66-182)
+    std::vector<int> meshRoot(d.num_meshes, -1);
+    std::vector<Box> meshRange(d.num_meshes);
+    for (int mi = 0; mi < d.num_meshes; ++mi) {
+        const hdr_mesh_desc &m = d.meshes[mi];
+        if (m.num_points < 0 || m.num_triangles < 0) return "negative mesh counts";
+        if ((m.num_points && !m.points) || (m.num_triangles && !m.indices)) return "mesh arrays missing";
+        if (m.num_points && !finite_all(m.points, size_t(m.num_points) * 3)) return "non-finite mesh point";
+        const int nIdx = m.num_triangles;
+        BlasBuilder bb;
+        bb.tris.reserve(nIdx);
+        auto P = [&](int k) { return F3{m.points[3 * k], m.points[3 * k + 1], m.points[3 * k + 2]}; };
+        for (int i = 0; i < nIdx; ++i) {
+            const int ids[3] = {m.indices[3 * i], m.indices[3 * i + 1], m.indices[3 * i + 2]};
+            bool ok = true;
+            for (int k = 0; k < 3; ++k) ok = ok && ids[k] >= 0 && ids[k] < m.num_points;
+            if (!ok) continue;
+            BuildTri t;
+            t.v0 = P(ids[0]);
+            t.v1 = P(ids[1]);
+            t.v2 = P(ids[2]);
+            for (int k = 0; k < 3; ++k) {
+                F3 n{0.0f, 0.0f, 0.0f}, c{1.0f, 1.0f, 1.0f};
+                if (m.num_normals > 0 && m.normals) {
+                    const int src = (m.num_normals == nIdx * 3) ? i * 3 + k : ids[k];
+                    if (m.num_normals == nIdx * 3 || ids[k] < m.num_normals)
+                        n = F3{m.normals[3 * src], m.normals[3 * src + 1], m.normals[3 * src + 2]};
+                    else
+                        n = F3{0.0f, 1.0f, 0.0f};
+                }
+                if (m.num_colors > 0 && m.colors) {
+                    const int src = (m.num_colors == nIdx * 3) ? i * 3 + k : ids[k];
+                    if (m.num_colors == nIdx * 3 || ids[k] < m.num_colors)
+                        c = F3{m.colors[3 * src], m.colors[3 * src + 1], m.colors[3 * src + 2]};
+                }
+                float *nn = k == 0 ? t.attr.n0 : (k == 1 ? t.attr.n1 : t.attr.n2);
+                float *cc = k == 0 ? t.attr.c0 : (k == 1 ? t.attr.c1 : t.attr.c2);
+                nn[0] = n.x;
+                nn[1] = n.y;
+                nn[2] = n.z;
+                cc[0] = c.x;
+                cc[1] = c.y;
+                cc[2] = c.z;
+            }
+            t.centroid = muld(add(add(t.v0, t.v1), t.v2), 1.0 / double(3.0f));
+            t.orig = i;
+            bb.tris.push_back(t);
+        }
+        Box range;
+        for (int i = 0; i < nIdx; ++i)
+            for (int k = 0; k < 3; ++k) {
+                const int id = m.indices[3 * i + k];
+                if (id >= 0 && id < m.num_points) range.extend(P(id));
+            }
+        meshRange[mi] = range;
+        if (bb.tris.empty()) continue;
+        bb.nodes.reserve(size_t(nIdx) * 2);
+        bb.nodes.push_back(GNode{});
+        bb.subdivide(0, 0, int(bb.tris.size()), 1);
+        out.max_blas_depth = std::max(out.max_blas_depth, bb.depth);
+        const int nodeBase = int(out.nodes.size());
+        const int triBase = int(out.tris.size());
+        for (GNode n : bb.nodes) {
+            if (n.left < 0) n.left = -(triBase + (-n.left - 1)) - 1;
+            else n.left += nodeBase;
+            out.nodes.push_back(n);
+        }
+        for (const BuildTri &t : bb.tris) {
+            GTri g{};
+            g.v0x = t.v0.x;
+            g.v0y = t.v0.y;
+            g.v0z = t.v0.z;
+            g.v1x = t.v1.x;
+            g.v1y = t.v1.y;
+            g.v1z = t.v1.z;
+            g.v2x = t.v2.x;
+            g.v2y = t.v2.y;
+            g.v2z = t.v2.z;
+            g.orig = t.orig;
+            out.tris.push_back(g);
+            out.attrs.push_back(t.attr);
+        }
+        meshRoot[mi] = nodeBase;
+    }
+
+    // ---- materials (+ default material record at index num_materials)
+    for (int i = 0; i < d.num_materials; ++i) {
+        const hdr_material_desc &s = d.materials[i];
+        GMaterial g{};
+        auto cp = [](float *dst, const float *src) { dst[0] = src[0], dst[1] = src[1], dst[2] = src[2]; };
+        cp(g.base, s.diffuse_color);
+        g.emission[0] = s.emission_color[0] * s.emission;
+        g.emission[1] = s.emission_color[1] * s.emission;
+        g.emission[2] = s.emission_color[2] * s.emission;
+        cp(g.spec_color, s.specular_color);
+        cp(g.coat_color, s.coat_color);
+        cp(g.sheen_color, s.sheen_color);
+        cp(g.ss_color, s.subsurface_color);
+        cp(g.trans_color, s.transmission_color);
+        g.metallic = s.metallic;
+        g.roughness = std::max(0.001f, s.roughness);
+        g.ior = s.ior;
+        g.specular = s.specular;
+        g.transmission = s.transmission;
+        g.trans_depth = s.transmission_depth;
+        g.coat = s.coat;
+        g.coat_roughness = s.coat_roughness;
+        g.coat_ior = s.coat_ior;
+        g.sheen = s.sheen;
+        g.sheen_roughness = s.sheen_roughness;
+        g.subsurface = s.subsurface;
+        g.opacity = s.opacity;
+        g.thin_walled = s.thin_walled;
+        g.is_default = 0;
+        if (g.emission[0] != 0.0f || g.emission[1] != 0.0f || g.emission[2] != 0.0f) out.has_emission = 1;
+        out.materials.push_back(g);
+    }
+    {
+        GMaterial g{};  // BSDFClosure defaults (shading_types.h:18-42) with DefaultMaterial's overrides
+        const float one[3] = {1.0f, 1.0f, 1.0f};
+        std::memcpy(g.spec_color, one, sizeof one);
+        std::memcpy(g.coat_color, one, sizeof one);
+        std::memcpy(g.sheen_color, one, sizeof one);
+        std::memcpy(g.trans_color, one, sizeof one);
+        g.ss_color[0] = 0.7f;
+        g.ss_color[1] = 0.2f;
+        g.ss_color[2] = 0.2f;
+        g.metallic = 0.0f;
+        g.roughness = 1.0f;
+        g.ior = 1.5f;
+        g.specular = 0.5f;
+        g.coat_ior = 1.5f;
+        g.sheen_roughness = 0.3f;
+        g.opacity = 1.0f;
+        g.is_default = 1;
+        out.materials.push_back(g);
+    }
+
+    // ---- instances (appendMeshInstance, scene.cpp:178-207) + TLAS
+    std::vector<Box> instBounds(d.num_instances);
+    std::vector<F3> instCentroid(d.num_instances);
+    for (int i = 0; i < d.num_instances; ++i) {
+        const hdr_instance_desc &s = d.instances[i];
+        if (s.mesh < 0 || s.mesh >= d.num_meshes) return "instance references an invalid mesh";
+        if (s.material >= d.num_materials) return "instance references an invalid material";
+        if (s.light >= d.num_lights) return "instance references an invalid light";
+        if (!finite_all(s.transform, 16)) return "non-finite instance transform";
+        GInstance g{};
+        std::memcpy(g.xf, s.transform, sizeof g.xf);
+        double a[4][4], o[4][4];
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c) a[r][c] = s.transform[r * 4 + c];
+        invert4(a, o);
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c) g.inv[r * 4 + c] = float(o[r][c]);
+        g.root = meshRoot[s.mesh];
+        g.has_params = s.material >= 0 ? 1 : 0;
+        g.material = s.material >= 0 ? s.material : d.num_materials;
+        g.light = s.light;
+        if (s.light >= 0) out.has_mesh_lights = 1;
+        out.instances.push_back(g);
+        // _TransformBounds, scene.cpp:100-116
+        const Box &r = meshRange[s.mesh];
+        Box b;
+        if (r.empty()) {
+            b = r;
+        } else {
+            const F3 mn = r.mn, mx = r.mx;
+            const F3 corners[8] = {{mn.x, mn.y, mn.z}, {mx.x, mn.y, mn.z}, {mn.x, mx.y, mn.z}, {mx.x, mx.y, mn.z},
+                                   {mn.x, mn.y, mx.z}, {mx.x, mn.y, mx.z}, {mn.x, mx.y, mx.z}, {mx.x, mx.y, mx.z}};
+            for (const F3 &c : corners) b.extend(xform_point_h(s.transform, c));
+        }
+        instBounds[i] = b;
+        instCentroid[i] = mulf(add(b.mn, b.mx), 0.5f);
+    }
+    if (d.num_instances > 0) {
+        std::vector<int> idx(d.num_instances);
+        for (int i = 0; i < d.num_instances; ++i) idx[i] = i;
+        std::vector<GNode> tl;
+        tl.reserve(size_t(d.num_instances) * 2);
+        tl.push_back(GNode{});
+        int tdepth = 0;
+        // Scene::_SubdivideTLAS, scene.cpp:352-407 (children allocated before the leaf test)
+        struct Rec {
+            static void go(std::vector<GNode> &tl, std::vector<int> &idx, const std::vector<Box> &bounds,
+                           const std::vector<F3> &cen, int nodeIdx, int start, int end, int dpt, int &maxd) {
+                maxd = std::max(maxd, dpt);
+                const int leftIdx = int(tl.size());
+                tl.push_back(GNode{});
+                tl.push_back(GNode{});
+                Box b;
+                for (int i = start; i < end; ++i) {
+                    b.extend(bounds[idx[i]].mn);
+                    b.extend(bounds[idx[i]].mx);
+                }
+                const int count = end - start;
+                if (count <= 2) {
+                    tl[nodeIdx] = make_node(b, -start - 1, count);
+                    return;
+                }
+                tl[nodeIdx] = make_node(b, leftIdx, 0);
+                const F3 size = b.size();
+                int axis = 0;
+                if (size.y > size.x) axis = 1;
+                if (size.z > comp(size, axis)) axis = 2;
+                const float splitPos = comp(b.mn, axis) + comp(size, axis) * 0.5f;
+                int i = start, j = end - 1;
+                while (i <= j) {
+                    if (comp(cen[idx[i]], axis) < splitPos) {
+                        ++i;
+                    } else {
+                        std::swap(idx[i], idx[j]);
+                        --j;
+                    }
+                }
+                if (i == start || i == end) i = start + count / 2;
+                go(tl, idx, bounds, cen, leftIdx, start, i, dpt + 1, maxd);
+                go(tl, idx, bounds, cen, leftIdx + 1, i, end, dpt + 1, maxd);
+            }
+        };
+        Rec::go(tl, idx, instBounds, instCentroid, 0, 0, d.num_instances, 1, tdepth);
+        out.max_tlas_depth = tdepth;
+        const int base = int(out.nodes.size());
+        for (GNode n : tl) {
+            if (n.left >= 0 && n.count == 0) n.left += base;
+            out.nodes.push_back(n);
+        }
+        out.tlas_idx = idx;
+        out.tlas_root = base;
+    }
+    if (out.max_blas_depth > kStackDepth - 2 || out.max_tlas_depth > kStackDepth - 2)
+        return "BVH deeper than the traversal stack";
+
+    // ---- lights (per-sample constants of SampleLight)
+    for (int i = 0; i < d.num_lights; ++i) {
+        const hdr_light_desc &s = d.lights[i];
+        if (s.type < HDR_LIGHT_DISTANT || s.type > HDR_LIGHT_POINT) return "unknown light type";
+        GLight g{};
+        g.type = s.type;
+        g.is_delta = s.type == HDR_LIGHT_DISTANT ? 1 : 0;
+        g.width = s.width;
+        g.height = s.height;
+        for (int k = 0; k < 16; ++k) g.m[k] = float(s.transform[k]);
+        const float eff = s.intensity * std::exp2(s.exposure);  // LightParams::EffectiveIntensity
+        g.eff_int = eff;
+        for (int k = 0; k < 3; ++k) {
+            g.raw_color[k] = s.color[k];
+            g.color[k] = s.color[k] * eff;
+        }
+        g.area = s.width * s.height;
+        g.inv_area = 1.f / g.area;
+        const F3 down = normalizedf(xform_dir_h(g.m, F3{0.0f, 0.0f, -1.0f}));
+        g.normal[0] = down.x;
+        g.normal[1] = down.y;
+        g.normal[2] = down.z;
+        g.pos[0] = g.m[12];
+        g.pos[1] = g.m[13];
+        g.pos[2] = g.m[14];
+        g.cone_angle = s.shaping_cone_angle;
+        g.softness = s.shaping_cone_softness;
+        g.cone_rad = s.shaping_cone_angle * float(M_PI / 180.0);
+        g.cos_cone = std::cos(g.cone_rad);
+        g.cos_inner = std::cos(g.cone_rad * (1.0f - s.shaping_cone_softness));
+        g.dome = -1;
+        if (s.type == HDR_LIGHT_DOME && s.dome_pixels && s.dome_width > 0 && s.dome_height > 0) {
+            HostScene::Dome dm;
+            dm.w = s.dome_width;
+            dm.h = s.dome_height;
+            const int w = dm.w, h = dm.h;
+            dm.pixels.assign(s.dome_pixels, s.dome_pixels + size_t(w) * h * 3);
+            dm.row_cdf.assign(h + 1, 0.0f);
+            dm.col_cdf.assign(size_t(h) * (w + 1), 0.0f);
+            // DomeLight::Prepare, dome_light.cpp:49-76
+            float total = 0.0f;
+            for (int y = 0; y < h; ++y) {
+                float rowLum = 0.0f;
+                const float sinTheta = float(std::sin(M_PI * double(float(y + 0.5f)) / double(float(h))));
+                for (int x = 0; x < w; ++x) {
+                    const size_t p = size_t(y * w + x) * 3;
+                    float lum = 0.2126f * dm.pixels[p] + 0.7152f * dm.pixels[p + 1] + 0.0722f * dm.pixels[p + 2];
+                    lum *= sinTheta;
+                    rowLum += lum;
+                    dm.col_cdf[size_t(y) * (w + 1) + x + 1] = dm.col_cdf[size_t(y) * (w + 1) + x] + lum;
+                }
+                if (rowLum > 0)
+                    for (int x = 0; x <= w; ++x) dm.col_cdf[size_t(y) * (w + 1) + x] /= rowLum;
+                total += rowLum;
+                dm.row_cdf[y + 1] = dm.row_cdf[y] + rowLum;
+            }
+            if (total > 0)
+                for (int y = 0; y <= h; ++y) dm.row_cdf[y] /= total;
+            dm.total_lum = total;
+            dm.pdf_scale = total * (float(M_PI) / float(h)) * (2.0f * float(M_PI) / float(w));
+            g.dome = int(out.domes.size());
+            out.domes.push_back(std::move(dm));
+        }
+        out.lights.push_back(g);
+    }
+    out.environment = d.environment;
+    out.env_dome = d.environment >= 0 ? out.lights[d.environment].dome : -1;
+    return std::string();
+}
+
+}  // namespace hdr

@@ -1,0 +1,42 @@
+// host_scene.h -- host-side scene build: BLAS/TLAS construction and packing into the device layout.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/hdrgpu.h"
+#include "dscene.h"
+
+namespace hdr {
+
+struct HostScene {
+    std::vector<GNode> nodes;        // BLAS nodes of all meshes, then TLAS nodes
+    std::vector<GTri> tris;
+    std::vector<GTriAttr> attrs;
+    std::vector<int> tlas_idx;
+    std::vector<GInstance> instances;
+    std::vector<GMaterial> materials; // num_materials records + the default material
+    std::vector<GLight> lights;
+    struct Dome {
+        std::vector<float> pixels, row_cdf, col_cdf;
+        int w = 0, h = 0;
+        float total_lum = 0.0f, pdf_scale = 0.0f;
+    };
+    std::vector<Dome> domes;
+    int tlas_root = -1;
+    int environment = -1;
+    int env_dome = -1;
+    int has_mesh_lights = 0;
+    int has_emission = 0;
+    int max_blas_depth = 0;
+    int max_tlas_depth = 0;
+};
+
+// Validates the descriptor and builds everything; returns an empty string on success or an
+// error message.
+std::string build_host_scene(const hdr_scene_desc &desc, HostScene &out);
+
+// GfMatrix4d / GfMatrix4f inverse by 2x2 sub-determinants in double (singular -> FLT_MAX scale).
+void invert4(const double a[4][4], double out[4][4]);
+
+}  // namespace hdr

@@ -1,0 +1,213 @@
+"""Python front end of libhdrgpu.so (the MI355X hot path) over its C-ABI.
+
+Mirrors the reference's pass interface for the per-pixel path: a `Renderer` owns one view
+(RaycastPass -> {RIS,}PathTracePass -> AccumulationPass with device-resident reservoirs and
+accumulation) over a resident scene. There is no CPU fallback: if the shared library is
+missing or no HIP device is present, construction raises.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhdrgpu.so")
+CSRC = os.path.join(_HERE, "csrc")
+_lib = None
+
+HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+SOURCES = ["hdrgpu.hip", "host_scene.cpp"]
+
+
+def build(verbose=False):
+    """Compiles libhdrgpu.so for gfx950 in-tree (hipcc cross-compiles without a GPU)."""
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    tmp = LIB_PATH + ".tmp"
+    cmd = ["/opt/rocm/bin/hipcc"] + HIPCC_FLAGS + ["-o", tmp] + srcs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stderr[-4000:])
+    os.replace(tmp, LIB_PATH)
+    if verbose:
+        print("built", LIB_PATH)
+    return LIB_PATH
+
+
+class HdrError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads libhdrgpu.so and declares the C-ABI signatures (no GPU is touched here)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HdrError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        I = C.c_int32
+        L.hdr_last_error.restype = C.c_char_p
+        L.hdr_abi_version.restype = C.c_uint32
+        sigs = {
+            "hdr_context_create": [I, C.POINTER(P)],
+            "hdr_context_destroy": [P],
+            "hdr_context_set_cancel": [P, I],
+            "hdr_scene_upload": [P, C.POINTER(abi.SceneDesc), C.POINTER(P)],
+            "hdr_scene_destroy": [P],
+            "hdr_scene_device_bytes": [P, C.POINTER(C.c_uint64)],
+            "hdr_view_create": [P, I, I, C.POINTER(P)],
+            "hdr_view_destroy": [P],
+            "hdr_view_reset": [P],
+            "hdr_view_device_bytes": [P, C.POINTER(C.c_uint64)],
+            "hdr_render_frame": [P, P, C.POINTER(abi.FrameDesc), C.POINTER(abi.PipelineDesc), P, P],
+            "hdr_view_read_color": [P, P],
+            "hdr_view_read_reservoirs": [P, P],
+            "hdr_view_read_aovs": [P, P, P, P],
+            "hdr_view_color_device_ptr": [P, C.POINTER(P)],
+            "hdr_view_synchronize": [P],
+            "hdr_view_last_timing": [P, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+            "hdr_scene_intersect": [P, P, P, I, P, P, P],
+            "hdr_rng_raw": [P, P, I, I, P],
+            "hdr_rng_float": [P, P, I, I, P],
+            "hdr_view_set_counting": [P, I],
+            "hdr_view_read_counters": [P, C.POINTER(abi.FrameCounters)],
+        }
+        for name, args in sigs.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != abi.HDR_OK:
+        msg = lib().hdr_last_error().decode(errors="replace")
+        raise HdrError(f"{abi.STATUS_NAMES.get(status, status)}: {msg}")
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().hdr_context_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().hdr_context_destroy(self.h)
+            self.h = None
+
+    def set_cancel(self, flag):
+        check(lib().hdr_context_set_cancel(self.h, int(flag)))
+
+    def rng_raw(self, seeds, n_draws):
+        seeds = np.ascontiguousarray(seeds, np.uint32)
+        out = np.zeros((len(seeds), n_draws), np.uint64)
+        check(lib().hdr_rng_raw(self.h, _p(seeds), len(seeds), n_draws, _p(out)))
+        return out
+
+    def rng_float(self, seeds, n_draws):
+        seeds = np.ascontiguousarray(seeds, np.uint32)
+        out = np.zeros((len(seeds), n_draws), np.float32)
+        check(lib().hdr_rng_float(self.h, _p(seeds), len(seeds), n_draws, _p(out)))
+        return out
+
+
+class GpuScene:
+    """A scene resident in HBM (BVHs, light and material tables)."""
+
+    def __init__(self, ctx, scene):
+        h = C.c_void_p()
+        check(lib().hdr_scene_upload(ctx.h, C.byref(scene.desc), C.byref(h)))
+        self.h, self.ctx, self.scene = h, ctx, scene
+
+    def device_bytes(self):
+        v = C.c_uint64()
+        check(lib().hdr_scene_device_bytes(self.h, C.byref(v)))
+        return int(v.value)
+
+    def intersect(self, origins, dirs):
+        origins = np.ascontiguousarray(origins, np.float32)
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        n = origins.shape[0]
+        inst = np.zeros(n, np.int32)
+        tri = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        check(lib().hdr_scene_intersect(self.h, _p(origins), _p(dirs), n, _p(inst), _p(tri), _p(t)))
+        return inst, tri, t
+
+    def close(self):
+        if self.h:
+            lib().hdr_scene_destroy(self.h)
+            self.h = None
+
+
+class Renderer:
+    """One render view (== one RenderPipeline instance with its persistent store) on the GPU."""
+
+    def __init__(self, ctx, gscene, width, height):
+        h = C.c_void_p()
+        check(lib().hdr_view_create(ctx.h, width, height, C.byref(h)))
+        self.h, self.ctx, self.gscene = h, ctx, gscene
+        self.W, self.H = width, height
+        self.color = np.zeros((height, width, 4), np.float32)
+        self.debug = np.zeros(height * width, abi.debug_dtype())
+
+    def close(self):
+        if self.h:
+            lib().hdr_view_destroy(self.h)
+            self.h = None
+
+    def reset(self):
+        check(lib().hdr_view_reset(self.h))
+
+    def render(self, frame, pipe, read_color=True, debug=False):
+        check(lib().hdr_render_frame(self.h, self.gscene.h, C.byref(frame), C.byref(pipe),
+                                     _p(self.color) if read_color else None,
+                                     _p(self.debug) if debug else None))
+        return self.color
+
+    def synchronize(self):
+        check(lib().hdr_view_synchronize(self.h))
+
+    def last_timing(self):
+        a, b = C.c_float(), C.c_float()
+        check(lib().hdr_view_last_timing(self.h, C.byref(a), C.byref(b)))
+        return float(a.value), float(b.value)
+
+    def read_color(self):
+        check(lib().hdr_view_read_color(self.h, _p(self.color)))
+        return self.color
+
+    def reservoirs(self):
+        out = np.zeros(self.H * self.W, abi.reservoir_dtype())
+        check(lib().hdr_view_read_reservoirs(self.h, _p(out)))
+        return out
+
+    def aovs(self):
+        d = np.zeros(self.H * self.W, np.float32)
+        a = np.zeros((self.H * self.W, 3), np.float32)
+        n = np.zeros((self.H * self.W, 3), np.float32)
+        check(lib().hdr_view_read_aovs(self.h, _p(d), _p(a), _p(n)))
+        return d, a, n
+
+    def set_counting(self, on):
+        check(lib().hdr_view_set_counting(self.h, int(on)))
+
+    def counters(self):
+        c = abi.FrameCounters()
+        check(lib().hdr_view_read_counters(self.h, C.byref(c)))
+        return c.as_dict()
+
+    def device_bytes(self):
+        v = C.c_uint64()
+        check(lib().hdr_view_device_bytes(self.h, C.byref(v)))
+        return int(v.value)

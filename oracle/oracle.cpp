@@ -2426,3 +2426,132 @@ void oracle_camera_rays(const hdr_frame_desc *frame, int32_t W, int32_t H, int32
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- unit-level exports (tests only)
+extern "C" {
+
+// WeightedReservoir over `n` candidate weights drawn from mt19937_64(seed): Update each weight
+// (CountMiss for NaN entries), then Finalize(perFrame) with p-hat = phat[selected].
+// Outputs: selected index (-1 none), WSum, CandidateCount, W, RNG draws used.
+void oracle_reservoir_run(const float *weights, const float *phat, int32_t n, int32_t perFrame, uint32_t seed,
+                          int32_t *sel, float *wsum, int32_t *count, float *W, int32_t *draws) {
+    Rng rng(seed);
+    Reservoir r;
+    int s = -1;
+    for (int k = 0; k < n; ++k) {
+        if (std::isnan(weights[k])) {
+            r.CountMiss();
+            continue;
+        }
+        RISCandidate c;
+        c.targetFunction = phat[k];
+        r.Update(c, weights[k], rng, k, s);
+    }
+    r.Finalize(perFrame);
+    *sel = r.ChosenSample ? s : -1;
+    *wsum = r.WSum;
+    *count = r.CandidateCount;
+    *W = r.W;
+    *draws = int32_t(rng.draws);
+}
+
+// RIS::RandomIndexFromWeights with a given uniform u (random_index.h:27-63): chosen index or -1.
+int32_t oracle_random_index(const double *weights, int32_t n, float u, double *weightSum) {
+    double sum = 0.0;
+    for (int k = 0; k < n; ++k)
+        if (weights[k] > 0.0) sum += weights[k];
+    *weightSum = sum;
+    if (sum <= 0.0) return -1;
+    double rem = double(u) * sum;
+    for (int k = 0; k < n; ++k) {
+        if (weights[k] <= 0.0) continue;
+        rem -= weights[k];
+        if (rem <= 0.0) return k;
+    }
+    return -1;
+}
+
+// SpectrumToRGB(RGBToSpectrum(rgb)) at the hero wavelengths of uniform u.
+void oracle_spectrum_roundtrip(const float *rgb, float u, float *out) {
+    const Lambda lam = Lambda::SampleUniform(u);
+    const V3 r = SpectrumToRGB(RGBToSpectrum(V3(rgb[0], rgb[1], rgb[2]), lam), lam);
+    out[0] = r[0];
+    out[1] = r[1];
+    out[2] = r[2];
+}
+
+// SpectrumToRGB of an explicit 4-sample spectrum at the wavelengths of u.
+void oracle_spectrum_to_rgb(const float *s, float u, float *out) {
+    const Lambda lam = Lambda::SampleUniform(u);
+    Spec sp;
+    for (int k = 0; k < kSpec; ++k) sp.v[k] = s[k];
+    const V3 r = SpectrumToRGB(sp, lam);
+    out[0] = r[0];
+    out[1] = r[1];
+    out[2] = r[2];
+}
+
+// GGXBsdf::Eval / Pdf for a PreviewSurface closure with the given parameters, shading normal n.
+void oracle_ggx(const float *base, float metallic, float roughness, float specular, float ior, const float *n,
+                const float *wo, const float *wi, float *f_out, float *pdf_out) {
+    Closure c;
+    c.BaseColor = V3(base[0], base[1], base[2]);
+    c.Metallic = metallic;
+    c.Roughness = std::max(0.001f, roughness);
+    c.Specular = specular;
+    c.Ior = ior;
+    c.SpecularColor = V3(1.0f);
+    c.Subsurface = 0.0f;
+    const V3 nn(n[0], n[1], n[2]), o(wo[0], wo[1], wo[2]), i(wi[0], wi[1], wi[2]);
+    const V3 f = GGXEval(c, nn, o, i);
+    f_out[0] = f[0];
+    f_out[1] = f[1];
+    f_out[2] = f[2];
+    *pdf_out = GGXPdfFull(c, nn, o, i);
+}
+
+// PowerHeuristic(f, nf, g, ng, h, nh)
+float oracle_power_heuristic(float f, int32_t nf, float g, int32_t ng, float h, int32_t nh) {
+    return PowerHeuristic3(f, nf, g, ng, h, nh);
+}
+
+// PreviewSurfaceMaterial::SampleBounce on a flat surface (normal +Y, ray straight down) with the
+// closure above; returns the direction, BsdfPdf and the draws used, for n seeds.
+void oracle_sample_bounce(const float *base, float metallic, float roughness, float specular, float ior,
+                          const float *rayDir, const uint32_t *seeds, int32_t n, float *dirs, float *pdfs,
+                          int32_t *draws) {
+    MaterialParams p;
+    p.DiffuseColor = V3(base[0], base[1], base[2]);
+    p.Metallic = metallic;
+    p.Roughness = roughness;
+    p.Specular = specular;
+    p.Ior = ior;
+    HitRecord h;
+    h.SmoothNormal = V3(0, 1, 0);
+    h.Normal = V3(0, 1, 0);
+    const Closure c = GetClosure(&p, h);
+    const Lambda lam = Lambda::SampleUniform(0.5f);
+    const ShadingPoint sp{&c, c.Normal, &lam, false};
+    for (int k = 0; k < n; ++k) {
+        Rng rng(seeds[k]);
+        BounceState st{};
+        BounceSample b;
+        const bool ok = PreviewSampleBounce(sp, V3(0, 0, 0), V3(rayDir[0], rayDir[1], rayDir[2]), BounceConfig{8, 8},
+                                            st, rng, b);
+        dirs[3 * k] = ok ? b.NextRay.d[0] : 0.0f;
+        dirs[3 * k + 1] = ok ? b.NextRay.d[1] : 0.0f;
+        dirs[3 * k + 2] = ok ? b.NextRay.d[2] : 0.0f;
+        pdfs[k] = ok ? b.BsdfPdf.value : -1.0f;
+        draws[k] = int32_t(rng.draws);
+    }
+}
+
+// Inverse of a 4x4 double matrix (the GfMatrix4d::GetInverse stand-in).
+void oracle_inverse4(const double *m, double *out) {
+    double a[4][4], o[4][4];
+    for (int i = 0; i < 16; ++i) a[i / 4][i % 4] = m[i];
+    Inverse4(a, o);
+    for (int i = 0; i < 16; ++i) out[i] = o[i / 4][i % 4];
+}
+
+}  // extern "C"
