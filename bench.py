@@ -27,6 +27,7 @@ HBM_PEAK_GBS = 8000.0
 NODE_BYTES = 32   # one BVH node record (6 fp32 bounds + 2 int)
 TRI_BYTES = 36    # three fp32x3 vertices read by the ray-triangle test
 STATE_BYTES = 208  # per pixel: accumulation rd+wr 32, colour wr 16, reservoir rd+wr 160
+GBUF_BYTES = 64    # per pixel G-buffer record, written by the RaycastPass kernel, read by integration
 
 
 def _log(msg):
@@ -74,6 +75,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=8)
+    ap.add_argument("--dolly", action="store_true",
+                    help="also time the moving-camera variant (each move clears reservoirs and accumulation)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -88,50 +91,66 @@ def main():
     gs = gpu.GpuScene(ctx, sc)
     r = gpu.Renderer(ctx, gs, sc.width, sc.height)
     P = sc.width * sc.height
-    static = [sc.frame(0, camera_index=0)]
 
     def frame_desc(i, moving):
         return sc.frame(i, camera_index=(i % len(sc.cameras)) if moving else 0)
 
-    # counting run (untimed): rays by class, node visits, triangle tests of this frame
+    # counting pass (untimed): the same frame sequence as the timed pass below, with per-frame
+    # counters of rays by class, BVH node visits and triangle tests (deterministic, so the timed
+    # pass repeats exactly this work); averaged over the timed frames
     r.set_counting(True)
     r.reset()
-    r.render(frame_desc(0, False), pipe, read_color=False)
-    r.render(frame_desc(1, False), pipe, read_color=False)
-    r.synchronize()
-    counts = r.counters()
+    for i in range(args.warmup):
+        r.render(frame_desc(i, False), pipe, read_color=False)
+    counts = None
+    for i in range(args.steps):
+        r.render(frame_desc(args.warmup + i, False), pipe, read_color=False)
+        r.synchronize()
+        c = r.counters()
+        counts = c if counts is None else {k: counts[k] + c[k] for k in c}
+    counts = {k: v / args.steps for k, v in counts.items()}
     r.set_counting(False)
-    _log(f"counters {counts}")
+    _log(f"counters per frame {counts}")
 
     # static camera: progressive frames with temporal reuse
     r.reset()
     for i in range(args.warmup):
         r.render(frame_desc(i, False), pipe, read_color=False)
     r.synchronize()
-    integ = []
+    gb_ms, integ = [], []
     t0 = time.perf_counter()
     for i in range(args.steps):
         r.render(frame_desc(args.warmup + i, False), pipe, read_color=False)
-        integ.append(r.last_timing()[0])
+        g_ms, i_ms, _ = r.last_timing()  # HIP events on the view's stream around each kernel
+        gb_ms.append(g_ms)
+        integ.append(i_ms)
     r.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     fps = 1000.0 / ms_per_step
     kern_ms = sum(integ) / len(integ)
-
-    # dolly: the camera moves every frame, which clears reservoirs and accumulation
-    t2 = time.perf_counter()
-    nd = min(args.steps, len(sc.cameras))
-    for i in range(nd):
-        r.reset()
-        r.render(frame_desc(i, True), pipe, read_color=False)
-    r.synchronize()
-    dolly_fps = nd / (time.perf_counter() - t2)
+    gbuf_ms = sum(gb_ms) / len(gb_ms)
     img = r.read_color()
+
+    dolly_fps = None
+    if args.dolly:  # the camera moves every frame, which clears reservoirs and accumulation
+        t2 = time.perf_counter()
+        nd = min(args.steps, len(sc.cameras))
+        for i in range(nd):
+            r.reset()
+            r.render(frame_desc(i, True), pipe, read_color=False)
+        r.synchronize()
+        dolly_fps = nd / (time.perf_counter() - t2)
 
     rays = counts["primary_rays"] + counts["bounce_rays"] + counts["shadow_rays"] + counts["candidate_rays"]
     ref_rays = rays + counts["candidate_rays_skipped"]
-    alg_bytes = NODE_BYTES * counts["node_visits"] + TRI_BYTES * counts["triangle_tests"] + STATE_BYTES * P
+    # algorithmic bytes per launch: BVH node records and triangle vertices of the rays each kernel
+    # traces, plus the per-pixel state it streams
+    sec_nodes = counts["node_visits"] - counts["primary_node_visits"]
+    sec_tris = counts["triangle_tests"] - counts["primary_triangle_tests"]
+    alg_bytes = NODE_BYTES * sec_nodes + TRI_BYTES * sec_tris + (STATE_BYTES + GBUF_BYTES) * P
+    gbuf_bytes = NODE_BYTES * counts["primary_node_visits"] + TRI_BYTES * counts["primary_triangle_tests"] + \
+        GBUF_BYTES * P
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
     line = {
         "metric": "ReSTIR frames/s at 1080p (many-lights, M=32, 1 temporal); Mrays/s alongside",
@@ -155,11 +174,15 @@ def main():
         "rays_per_frame": {"primary": counts["primary_rays"], "bounce": counts["bounce_rays"],
                            "shadow": counts["shadow_rays"], "bsdf_candidate_traced": counts["candidate_rays"],
                            "skipped_cannot_contribute": counts["candidate_rays_skipped"]},
-        "dolly_frames_per_s": round(dolly_fps, 3),
+        "dolly_frames_per_s": None if dolly_fps is None else round(dolly_fps, 3),
         "integrate_kernel_ms": round(kern_ms, 4),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "gbuffer_kernel_ms": round(gbuf_ms, 4),
+        "gbuffer_roofline": {"achieved": round(gbuf_bytes / (gbuf_ms * 1e-3) / 1e9, 2), "unit": "GB/s",
+                             "algorithmic_bytes_per_launch": gbuf_bytes},
+        "roofline": {"kernel": "integrate_kernel", "bound": "hbm", "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
-                     "algorithmic_bytes_per_launch": alg_bytes},
+                     "algorithmic_bytes_per_launch": round(alg_bytes)},
         "image_mean_rgb": [round(float(v), 3) for v in img[..., :3].reshape(-1, 3).mean(0)],
     }
     r.close()

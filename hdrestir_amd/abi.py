@@ -98,7 +98,8 @@ class FrameCounters(C.Structure):
     _fields_ = [("primary_rays", C.c_uint64), ("bounce_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("candidate_rays", C.c_uint64), ("candidate_rays_skipped", C.c_uint64),
                 ("node_visits", C.c_uint64), ("triangle_tests", C.c_uint64),
-                ("instance_visits", C.c_uint64), ("rng_draws", C.c_uint64)]
+                ("instance_visits", C.c_uint64), ("rng_draws", C.c_uint64),
+                ("primary_node_visits", C.c_uint64), ("primary_triangle_tests", C.c_uint64)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -135,18 +136,6 @@ def frame(view, proj, frame_index):
     f.frame_index = frame_index
     f.has_camera_params = 0
     return f
-
-
-def reservoirs_to_numpy(buf):
-    """hdr_reservoir[] -> structured numpy array view."""
-    import numpy as np
-    dt = np.dtype([("dir", "<f4", 3), ("dist", "<f4"), ("color", "<f4", 3), ("pdf", "<f4"),
-                   ("light_normal", "<f4", 3), ("target", "<f4"), ("throughput", "<f4", 4),
-                   ("wsum", "<f4"), ("count", "<i4"), ("w", "<f4"), ("flags_light", "<u4")])
-    return np.frombuffer(buf, dtype=dt)
-
-
-DEBUG_DTYPE = None
 
 
 def debug_dtype():

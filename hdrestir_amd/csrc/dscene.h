@@ -200,7 +200,7 @@ constexpr int kStackDepth = 64;
 // every node's box tested against the best t at the time it is reached. Finds the reference's
 // winner: strict t < best, first found among equal t.
 template <bool kCount>
-__device__ bool trace_closest(const DeviceScene &s, V3 o, V3 d, RawHit &hit, TraceStats &st) {
+__device__ __forceinline__ bool trace_closest(const DeviceScene &s, V3 o, V3 d, RawHit &hit, TraceStats &st) {
     hit.t = 1e30f;
     hit.inst = -1;
     if (s.tlas_root < 0) return false;
@@ -270,6 +270,61 @@ __device__ bool trace_closest(const DeviceScene &s, V3 o, V3 d, RawHit &hit, Tra
     return was;
 }
 
+// Shadow query: is there a hit with 1e-4 < t < tmax? The reference fires a closest-hit ray and
+// tests Depth < Dist - 1e-3 (ris_direct_light_integrator.cpp:513-519, mis_direct_light_integrator.cpp:100-105);
+// the same traversal started at t = tmax and stopped at the first accepted triangle answers the
+// same question (a box entered before the closest hit cannot be pruned by a larger bound).
+template <bool kCount>
+__device__ __forceinline__ bool trace_any(const DeviceScene &s, V3 o, V3 d, float tmax, TraceStats &st) {
+    if (s.tlas_root < 0) return false;
+    int tstack[kStackDepth];
+    int bstack[kStackDepth];
+    int tsp = 0;
+    tstack[tsp++] = s.tlas_root;
+    while (tsp > 0) {
+        const int ni = tstack[--tsp];
+        const GNode n = load_node(s.nodes, ni);
+        if (kCount) ++st.nodes;
+        float tAabb = 0.0f;
+        if (!aabb_hit<true>(n, o, d, tAabb) || tAabb > tmax) continue;
+        if (n.left >= 0) {
+            tstack[tsp++] = n.left + 1;
+            tstack[tsp++] = n.left;
+            continue;
+        }
+        const int start = -n.left - 1;
+        for (int k = 0; k < n.count; ++k) {
+            const GInstance &inst = s.instances[s.tlas_idx[start + k]];
+            if (kCount) ++st.instances;
+            if (inst.root < 0) continue;
+            const V3 oo = xform_point(inst.inv, o);
+            const V3 od = xform_dir(inst.inv, d);
+            int bsp = 0;
+            bstack[bsp++] = inst.root;
+            while (bsp > 0) {
+                const int bi = bstack[--bsp];
+                const GNode b = load_node(s.nodes, bi);
+                if (kCount) ++st.nodes;
+                float ta;
+                if (!aabb_hit<false>(b, oo, od, ta) || ta > tmax) continue;
+                if (b.left >= 0) {
+                    bstack[bsp++] = b.left + 1;
+                    bstack[bsp++] = b.left;
+                    continue;
+                }
+                const int ts = -b.left - 1;
+                for (int q = 0; q < b.count; ++q) {
+                    const GTri tr = s.tris[ts + q];
+                    if (kCount) ++st.tris;
+                    float tt, uu, vv;
+                    if (tri_hit(tr, oo, od, tt, uu, vv) && tt < tmax) return true;
+                }
+            }
+        }
+    }
+    return false;
+}
+
 // HitRecord of the winner (Scene::Intersect :456-497 and IntersectScene :515-535): world
 // normals through TransformDir + GetNormalized, albedo = material diffuse colour or the
 // interpolated vertex colour.
@@ -311,6 +366,20 @@ __device__ __forceinline__ HitRec make_hit_record(const DeviceScene &s, V3 o, V3
     r.inst = h.inst;
     r.tri = tr.orig;
     return r;
+}
+
+// Out-of-line copies for the shading megakernel: one traversal body (and one stack frame)
+// instead of one inlined copy per call site.
+template <bool kCount>
+__device__ __noinline__ bool trace_closest_call(const DeviceScene &s, V3 o, V3 d, RawHit &hit, TraceStats &st) {
+    return trace_closest<kCount>(s, o, d, hit, st);
+}
+template <bool kCount>
+__device__ __noinline__ bool trace_any_call(const DeviceScene &s, V3 o, V3 d, float tmax, TraceStats &st) {
+    return trace_any<kCount>(s, o, d, tmax, st);
+}
+__device__ __noinline__ HitRec make_hit_record_call(const DeviceScene &s, V3 o, V3 d, const RawHit &h) {
+    return make_hit_record(s, o, d, h);
 }
 
 }  // namespace hdr

@@ -18,35 +18,63 @@ using namespace hdr;
 
 // ============================================================================ kernels
 
-// RaycastPass + IntegrationPass + (fused) AccumulationPass for one 8x8 pixel tile per wave.
-template <bool kCount, bool kDebug>
-__global__ void __launch_bounds__(kLanes) integrate_kernel(const FrameArgs a) {
-    extern __shared__ double wl[];
-    const int lane = threadIdx.x;
-    const int x = blockIdx.x * 8 + (lane & 7);
-    const int y = blockIdx.y * 8 + (lane >> 3);
-    if (x >= a.W || y >= a.H) return;
-    const int pixel = y * a.W + x;
-    const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
-    PixelStats st = {};
-    Tracer<kCount> tracer{a.s, st};
+// G-buffer record of one pixel: 4 x float4 = 64 B, written and read coalesced.
+struct GRec {
+    float4 a;  // pos.xyz, depth
+    float4 b;  // ng.xyz, instance (bits; -1 = miss)
+    float4 c;  // ns.xyz, triangle (bits)
+    float4 d;  // albedo.xyz, unused
+};
 
-    // RaycastPass (raycast_pass.cpp:105-157): closest hit of the pixel-centre ray
+__device__ __forceinline__ void store_grec(GRec *dst, bool hit, const HitRec &g) {
+    GRec r;
+    r.a = make_float4(g.pos.x, g.pos.y, g.pos.z, g.depth);
+    r.b = make_float4(g.ng.x, g.ng.y, g.ng.z, __int_as_float(hit ? g.inst : -1));
+    r.c = make_float4(g.ns.x, g.ns.y, g.ns.z, __int_as_float(g.tri));
+    r.d = make_float4(g.albedo.x, g.albedo.y, g.albedo.z, 0.0f);
+    *dst = r;
+}
+__device__ __forceinline__ bool load_grec(const GRec *src, HitRec &g) {
+    const GRec r = *src;
+    g.pos = v3(r.a.x, r.a.y, r.a.z);
+    g.depth = r.a.w;
+    g.ng = v3(r.b.x, r.b.y, r.b.z);
+    g.inst = __float_as_int(r.b.w);
+    g.ns = v3(r.c.x, r.c.y, r.c.z);
+    g.tri = __float_as_int(r.c.w);
+    g.albedo = v3(r.d.x, r.d.y, r.d.z);
+    return g.inst >= 0;
+}
+
+// RaycastPass (raycast_pass.cpp:57-162): closest hit of the pixel-centre ray, one 8x8 tile per wave.
+// Sum over the 64 lanes of the wave (every lane active), added once by lane 0.
+__device__ __forceinline__ void wave_add(unsigned long long *dst, unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(dst, v);
+}
+
+template <bool kCount>
+__device__ __forceinline__ void gbuffer_pixel(const FrameArgs &a, GRec *gbuf, int x, int y, TraceStats &ts) {
+    const int pixel = y * a.W + x;
     V3 co, cd;
     if (a.has_cam) {
+        const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
         Mt64 crng;
         crng.init(seed, nullptr);
         camera_ray(a, float(x) + 0.5f, float(y) + 0.5f, &crng, co, cd);
     } else {
         camera_ray(a, float(x) + 0.5f, float(y) + 0.5f, nullptr, co, cd);
     }
-    HitRec g;
-    if (kCount) ++st.primary;
-    const bool gHit = tracer.trace(co, cd, g);
+    RawHit rh;
+    HitRec g = {};
+    const bool hit = trace_closest<kCount>(a.s, co, cd, rh, ts);
+    if (hit) g = make_hit_record(a.s, co, cd, rh);
+    store_grec(gbuf + pixel, hit, g);
     if (a.aov_depth) {
-        a.aov_depth[pixel] = gHit ? g.depth : 1.0f;
-        const V3 al = gHit ? g.albedo : v3s(0.0f);
-        const V3 nn = gHit ? g.ng : v3s(0.0f);
+        a.aov_depth[pixel] = hit ? g.depth : 1.0f;
+        const V3 al = hit ? g.albedo : v3s(0.0f);
+        const V3 nn = hit ? g.ng : v3s(0.0f);
         a.aov_albedo[3 * pixel + 0] = al.x;
         a.aov_albedo[3 * pixel + 1] = al.y;
         a.aov_albedo[3 * pixel + 2] = al.z;
@@ -54,6 +82,36 @@ __global__ void __launch_bounds__(kLanes) integrate_kernel(const FrameArgs a) {
         a.aov_normal[3 * pixel + 1] = nn.y;
         a.aov_normal[3 * pixel + 2] = nn.z;
     }
+}
+
+template <bool kCount>
+__global__ void __launch_bounds__(kLanes) gbuffer_kernel(const FrameArgs a, GRec *gbuf) {
+    const int lane = threadIdx.x;
+    const int x = blockIdx.x * 8 + (lane & 7);
+    const int y = blockIdx.y * 8 + (lane >> 3);
+    const bool valid = x < a.W && y < a.H;
+    TraceStats ts = {};
+    if (valid) gbuffer_pixel<kCount>(a, gbuf, x, y, ts);
+    if (kCount) {
+        unsigned long long *c = a.counters;
+        wave_add(c + 0, valid ? 1ULL : 0ULL);
+        wave_add(c + 5, ts.nodes);
+        wave_add(c + 6, ts.tris);
+        wave_add(c + 7, ts.instances);
+        wave_add(c + 9, ts.nodes);
+        wave_add(c + 10, ts.tris);
+    }
+}
+
+// IntegrationPass (integration_pass.cpp:208-243) + fused accumulation for one pixel.
+template <bool kCount, bool kDebug>
+__device__ __forceinline__ void integrate_pixel(const FrameArgs &a, const GRec *gbuf, int x, int y, double *wlane,
+                                                PixelStats &st, uint32_t &draws) {
+    const int pixel = y * a.W + x;
+    const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
+    Tracer<kCount, false> tracer{a.s, st};
+    HitRec g;
+    const bool gHit = load_grec(gbuf + pixel, g);
 
     // IntegrationPass (integration_pass.cpp:208-243)
     Mt64 rng;
@@ -65,7 +123,7 @@ __global__ void __launch_bounds__(kLanes) integrate_kernel(const FrameArgs a) {
     camera_ray(a, px, py, &rng, ro, rd);
     const SpecTab tab = make_tab(rng.next_float());
     Trace dbg = {2166136261u, 2166136261u, 0};
-    PixelCtx<kCount, kDebug> p{a, tab, rng, tracer, st, dbg, wl + lane, slot ? slot + kMtN : nullptr};
+    PixelCtx<kCount, kDebug, false> p{a, tab, rng, tracer, st, dbg, wlane, slot ? slot + kMtN : nullptr};
     const Spec L = path_li(p, rd, gHit, g, pixel);
     V3 rgb = spec2rgb(tab, L);
     if (a.has_cam && a.cam.enable_exposure) rgb = rgb * a.exposure_mul;
@@ -96,18 +154,188 @@ __global__ void __launch_bounds__(kLanes) integrate_kernel(const FrameArgs a) {
         d.sample_rgb[2] = rgb.z;
         a.dbg[pixel] = d;
     }
+    draws = rng.k;
+}
+
+// IntegrationPass + (fused) AccumulationPass for one 8x8 pixel tile per wave.
+template <bool kCount, bool kDebug>
+__global__ void __launch_bounds__(kLanes) integrate_kernel(const FrameArgs a, const GRec *gbuf) {
+    extern __shared__ double wl[];
+    const int lane = threadIdx.x;
+    const int x = blockIdx.x * 8 + (lane & 7);
+    const int y = blockIdx.y * 8 + (lane >> 3);
+    const bool valid = x < a.W && y < a.H;
+    PixelStats st = {};
+    uint32_t draws = 0;
+    if (valid) integrate_pixel<kCount, kDebug>(a, gbuf, x, y, wl + lane, st, draws);
     if (kCount) {
         unsigned long long *c = a.counters;
-        atomicAdd(c + 0, (unsigned long long)st.primary);
-        atomicAdd(c + 1, (unsigned long long)st.bounce);
-        atomicAdd(c + 2, (unsigned long long)st.shadow);
-        atomicAdd(c + 3, (unsigned long long)st.cand);
-        atomicAdd(c + 4, (unsigned long long)st.cand_skipped);
-        atomicAdd(c + 5, (unsigned long long)st.ts.nodes);
-        atomicAdd(c + 6, (unsigned long long)st.ts.tris);
-        atomicAdd(c + 7, (unsigned long long)st.ts.instances);
-        atomicAdd(c + 8, (unsigned long long)rng.k);
+        wave_add(c + 1, st.bounce);
+        wave_add(c + 2, st.shadow);
+        wave_add(c + 3, st.cand);
+        wave_add(c + 4, st.cand_skipped);
+        wave_add(c + 5, st.ts.nodes);
+        wave_add(c + 6, st.ts.tris);
+        wave_add(c + 7, st.ts.instances);
+        wave_add(c + 8, draws);
     }
+}
+
+// ---------------------------------------------------------------------------- deferred-shadow path
+// RIS at depth 1 with skipVisibility in a scene where nothing BSDF-sampled can reach an emitter:
+// the only ray after the G-buffer is the RIS winner's shadow ray. Shading runs without traversal
+// code (more waves in flight), the shadow rays are compacted into a queue for an any-hit kernel,
+// and a last pass picks the visible / occluded colour and accumulates.
+
+struct ShadowQuery {
+    float4 o;  // origin.xyz, tmax
+    float4 d;  // dir.xyz, pixel (bits)
+};
+
+template <bool kCount, bool kDebug>
+__device__ __forceinline__ void shade_direct_pixel(const FrameArgs &a, const GRec *gbuf, int x, int y, double *wlane,
+                                                   PixelStats &st, uint32_t &draws, float4 *rgb2, uint32_t *flags,
+                                                   bool &query, ShadowQuery &q) {
+    const int pixel = y * a.W + x;
+    const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
+    Tracer<kCount, true> tracer{a.s, st};
+    HitRec g;
+    const bool gHit = load_grec(gbuf + pixel, g);
+    Mt64 rng;
+    uint64_t *slot = a.spill ? a.spill + size_t(pixel) * (2 * kMtN) : nullptr;
+    rng.init(seed, slot);
+    const float px = float(x) + rng.next_float();
+    const float py = float(y) + rng.next_float();
+    V3 ro, rd;
+    camera_ray(a, px, py, &rng, ro, rd);
+    const SpecTab tab = make_tab(rng.next_float());
+    Trace dbg = {2166136261u, 2166136261u, 0};
+    PixelCtx<kCount, kDebug, true> p{a, tab, rng, tracer, st, dbg, wlane, slot ? slot + kMtN : nullptr};
+    Spec lVis = spec(0.0f), lOcc = spec(0.0f);
+    if (gHit) {  // PathIntegrator::Li / _li at depth 1 (path_integrator.cpp:112-177); opacity >= 0.999
+        const Clo c = get_closure(a.s, g, a.enable_ss != 0);
+        Spec throughput = spec(1.0f);
+        const Vertex v0 = make_vertex(a, tab, g, c, rd, throughput);
+        const Spec total = rgb2spec(tab, mv3(c.m->emission));
+        int refl = 0, refr = 0;
+        Conn first;
+        const bool ok = bounce_with_connection(p, v0, refl, refr, true, first);
+        const Spec direct = direct_light(p, v0, ok ? &first : nullptr, true, pixel);
+        lVis = throughput * (total + direct);
+        lOcc = throughput * (total + spec(0.0f));
+    } else if (a.s.environment >= 0 && a.ibl_bg) {
+        lVis = lOcc = rgb2spec(tab, env_radiance(a.s, rd));
+    }
+    V3 cv = spec2rgb(tab, lVis), co = spec2rgb(tab, lOcc);
+    if (a.has_cam && a.cam.enable_exposure) {
+        cv = cv * a.exposure_mul;
+        co = co * a.exposure_mul;
+    }
+    rgb2[2 * pixel] = make_float4(cv.x, cv.y, cv.z, 1.0f);
+    rgb2[2 * pixel + 1] = make_float4(co.x, co.y, co.z, 1.0f);
+    query = tracer.has_query;
+    flags[pixel] = query ? 1u : 0u;
+    if (query) {
+        q.o = make_float4(tracer.qo.x, tracer.qo.y, tracer.qo.z, tracer.qtmax);
+        q.d = make_float4(tracer.qd.x, tracer.qd.y, tracer.qd.z, __int_as_float(pixel));
+    }
+    if (kDebug) {
+        hdr_pixel_debug d;
+        d.draws = rng.k;
+        d.light_hash = dbg.lhash;
+        d.select_hash = dbg.shash;
+        d.n_light_picks = dbg.picks;
+        d.prim_instance = gHit ? g.inst : -1;
+        d.prim_triangle = gHit ? g.tri : -1;
+        d.prim_t = gHit ? g.depth : 0.0f;
+        d.sample_rgb[0] = cv.x;
+        d.sample_rgb[1] = cv.y;
+        d.sample_rgb[2] = cv.z;
+        a.dbg[pixel] = d;
+    }
+    draws = rng.k;
+}
+
+template <bool kCount, bool kDebug>
+__global__ void __launch_bounds__(kLanes) shade_direct_kernel(const FrameArgs a, const GRec *gbuf, float4 *rgb2,
+                                                              uint32_t *flags, ShadowQuery *queue,
+                                                              unsigned int *queue_count) {
+    extern __shared__ double wl[];
+    const int lane = threadIdx.x;
+    const int x = blockIdx.x * 8 + (lane & 7);
+    const int y = blockIdx.y * 8 + (lane >> 3);
+    const bool valid = x < a.W && y < a.H;
+    PixelStats st = {};
+    uint32_t draws = 0;
+    bool query = false;
+    ShadowQuery q;
+    if (valid) shade_direct_pixel<kCount, kDebug>(a, gbuf, x, y, wl + lane, st, draws, rgb2, flags, query, q);
+    // wave-aggregated append to the shadow queue
+    const unsigned long long mask = __ballot(query);
+    const int n = __popcll(mask);
+    if (n > 0) {
+        unsigned int base = 0;
+        if (lane == 0) base = atomicAdd(queue_count, unsigned(n));
+        base = __shfl(base, 0);
+        if (query) {
+            const int rank = __popcll(mask & ((1ULL << lane) - 1ULL));
+            queue[base + rank] = q;
+        }
+    }
+    if (kCount) {
+        unsigned long long *c = a.counters;
+        wave_add(c + 2, query ? 1ULL : 0ULL);
+        wave_add(c + 4, st.cand_skipped);
+        wave_add(c + 8, draws);
+    }
+}
+
+// Any-hit shadow rays of the queue; an occluded winner keeps zero throughput in its reservoir
+// (the visibility fix-up of ris_direct_light_integrator.cpp:163-175).
+template <bool kCount>
+__global__ void __launch_bounds__(kLanes) shadow_kernel(const FrameArgs a, const ShadowQuery *queue,
+                                                        const unsigned int *queue_count, uint32_t *flags) {
+    const unsigned int n = *queue_count;
+    TraceStats ts = {};
+    for (unsigned int i = blockIdx.x * kLanes + threadIdx.x; i < n; i += gridDim.x * kLanes) {
+        const ShadowQuery q = queue[i];
+        const int pixel = __float_as_int(q.d.w);
+        if (trace_any<kCount>(a.s, v3(q.o.x, q.o.y, q.o.z), v3(q.d.x, q.d.y, q.d.z), q.o.w, ts)) {
+            flags[pixel] = 3u;
+            hdr_reservoir &r = a.res[pixel];
+            r.throughput[0] = r.throughput[1] = r.throughput[2] = r.throughput[3] = 0.0f;
+        }
+    }
+    if (kCount) {
+        unsigned long long *c = a.counters;
+        wave_add(c + 5, ts.nodes);
+        wave_add(c + 6, ts.tris);
+        wave_add(c + 7, ts.instances);
+    }
+}
+
+// Picks the visible or occluded colour and accumulates (AccumulationPass, accumulation_pass.cpp:98-106).
+template <bool kDebug>
+__global__ void accumulate_deferred_kernel(const FrameArgs a, const float4 *rgb2, const uint32_t *flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.W * a.H) return;
+    const float4 c4 = rgb2[2 * i + (flags[i] == 3u ? 1 : 0)];
+    if (kDebug) {
+        a.dbg[i].sample_rgb[0] = c4.x;
+        a.dbg[i].sample_rgb[1] = c4.y;
+        a.dbg[i].sample_rgb[2] = c4.z;
+    }
+    if (a.firefly) {
+        a.frame_rgb[i] = c4;
+        return;
+    }
+    float4 acc = a.accum[i];
+    acc.x += c4.x;
+    acc.y += c4.y;
+    acc.z += c4.z;
+    acc.w += c4.w;
+    a.accum[i] = acc;
+    a.color[i] = make_float4(acc.x * a.inv_n, acc.y * a.inv_n, acc.z * a.inv_n, acc.w * a.inv_n);
 }
 
 // Out-of-place 3x3 firefly clamp (the reference's GPU accumulation shader semantics,
@@ -228,11 +456,11 @@ struct hdr_view {
     hdr_context *ctx = nullptr;
     int W = 0, H = 0;
     hipStream_t stream = nullptr;
-    DevBuf res, accum, color, frame, dbg, depth, albedo, normal, counters, spill;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    DevBuf res, accum, color, frame, dbg, depth, albedo, normal, counters, spill, gbuf, rgb2, flags, queue, qcount;
+    hipEvent_t evg = nullptr, ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     int counting = 0;
     int frames = 0;
-    float last_integrate_ms = 0.0f, last_frame_ms = 0.0f;
+    float last_gbuffer_ms = 0.0f, last_integrate_ms = 0.0f, last_frame_ms = 0.0f;
     bool timing_pending = false;
 };
 
@@ -376,7 +604,10 @@ hdr_status hdr_view_create(hdr_context *ctx, int32_t width, int32_t height, hdr_
     HIP_TRY(v->accum.alloc(P * sizeof(float4)));
     HIP_TRY(v->color.alloc(P * sizeof(float4)));
     HIP_TRY(v->counters.alloc(sizeof(hdr_frame_counters)));
+    HIP_TRY(v->gbuf.alloc(P * sizeof(GRec)));
+    HIP_TRY(v->qcount.alloc(sizeof(unsigned int)));
     HIP_TRY(hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&v->evg));
     HIP_TRY(hipEventCreate(&v->ev0));
     HIP_TRY(hipEventCreate(&v->ev1));
     HIP_TRY(hipEventCreate(&v->ev2));
@@ -393,6 +624,7 @@ hdr_status hdr_view_destroy(hdr_view *view) {
     if (!view) return HDR_OK;
     (void)hipSetDevice(view->ctx->device);
     if (view->stream) (void)hipStreamSynchronize(view->stream);
+    if (view->evg) (void)hipEventDestroy(view->evg);
     if (view->ev0) (void)hipEventDestroy(view->ev0);
     if (view->ev1) (void)hipEventDestroy(view->ev1);
     if (view->ev2) (void)hipEventDestroy(view->ev2);
@@ -413,7 +645,7 @@ hdr_status hdr_view_reset(hdr_view *view) {
 hdr_status hdr_view_device_bytes(const hdr_view *v, uint64_t *out) {
     if (!v || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
     *out = v->res.n + v->accum.n + v->color.n + v->frame.n + v->dbg.n + v->depth.n + v->albedo.n + v->normal.n +
-           v->counters.n + v->spill.n;
+           v->counters.n + v->spill.n + v->gbuf.n + v->rgb2.n + v->flags.n + v->queue.n + v->qcount.n;
     return HDR_OK;
 }
 
@@ -529,13 +761,56 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     }
     const dim3 grid((W + 7) / 8, (H + 7) / 8);
     const size_t lds = size_t(a.direct == HDR_DIRECT_RIS ? a.n_cand : 1) * kLanes * sizeof(double);
+    GRec *gb = static_cast<GRec *>(view->gbuf.p);
+    // The deferred-shadow path applies when the winner's shadow ray is the only ray after the
+    // G-buffer: RIS at depth 1, skipVisibility, no translucent material (no pass-through rays),
+    // nothing BSDF-sampled can reach an emitter (no emissive material, mesh light or environment).
+    const bool deferred = a.direct == HDR_DIRECT_RIS && a.max_depth == 1 && a.skip_vis && !scene->any_translucent &&
+                          !a.trace_generated && !a.trace_last_connection;
+    if (deferred) {
+        if (hdr_status st = ensure(view->rgb2, P * 2 * sizeof(float4))) return st;
+        if (hdr_status st = ensure(view->flags, P * sizeof(uint32_t))) return st;
+        if (hdr_status st = ensure(view->queue, P * sizeof(ShadowQuery))) return st;
+    }
+    HIP_TRY(hipEventRecord(view->evg, view->stream));
+    if (view->counting) hipLaunchKernelGGL((gbuffer_kernel<true>), grid, dim3(kLanes), 0, view->stream, a, gb);
+    else hipLaunchKernelGGL((gbuffer_kernel<false>), grid, dim3(kLanes), 0, view->stream, a, gb);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(view->ev0, view->stream));
-    if (view->counting) {
-        if (debug_out) hipLaunchKernelGGL((integrate_kernel<true, true>), grid, dim3(kLanes), lds, view->stream, a);
-        else hipLaunchKernelGGL((integrate_kernel<true, false>), grid, dim3(kLanes), lds, view->stream, a);
+    if (deferred) {
+        float4 *rgb2 = static_cast<float4 *>(view->rgb2.p);
+        uint32_t *flags = static_cast<uint32_t *>(view->flags.p);
+        ShadowQuery *queue = static_cast<ShadowQuery *>(view->queue.p);
+        unsigned int *qc = static_cast<unsigned int *>(view->qcount.p);
+        HIP_TRY(hipMemsetAsync(qc, 0, sizeof(unsigned int), view->stream));
+#define HDR_SHADE(C, D) \
+    hipLaunchKernelGGL((shade_direct_kernel<C, D>), grid, dim3(kLanes), lds, view->stream, a, gb, rgb2, flags, queue, qc)
+        if (view->counting) {
+            if (debug_out) HDR_SHADE(true, true);
+            else HDR_SHADE(true, false);
+        } else {
+            if (debug_out) HDR_SHADE(false, true);
+            else HDR_SHADE(false, false);
+        }
+#undef HDR_SHADE
+        HIP_TRY(hipGetLastError());
+        const int sblocks = int(std::min<size_t>((P + kLanes - 1) / kLanes, 8192));
+        if (view->counting)
+            hipLaunchKernelGGL((shadow_kernel<true>), dim3(sblocks), dim3(kLanes), 0, view->stream, a, queue, qc, flags);
+        else
+            hipLaunchKernelGGL((shadow_kernel<false>), dim3(sblocks), dim3(kLanes), 0, view->stream, a, queue, qc, flags);
+        HIP_TRY(hipGetLastError());
+        const int ablocks = int((P + 255) / 256);
+        if (debug_out)
+            hipLaunchKernelGGL((accumulate_deferred_kernel<true>), dim3(ablocks), dim3(256), 0, view->stream, a, rgb2, flags);
+        else
+            hipLaunchKernelGGL((accumulate_deferred_kernel<false>), dim3(ablocks), dim3(256), 0, view->stream, a, rgb2, flags);
+    } else if (view->counting) {
+        if (debug_out) hipLaunchKernelGGL((integrate_kernel<true, true>), grid, dim3(kLanes), lds, view->stream, a, gb);
+        else hipLaunchKernelGGL((integrate_kernel<true, false>), grid, dim3(kLanes), lds, view->stream, a, gb);
     } else {
-        if (debug_out) hipLaunchKernelGGL((integrate_kernel<false, true>), grid, dim3(kLanes), lds, view->stream, a);
-        else hipLaunchKernelGGL((integrate_kernel<false, false>), grid, dim3(kLanes), lds, view->stream, a);
+        if (debug_out) hipLaunchKernelGGL((integrate_kernel<false, true>), grid, dim3(kLanes), lds, view->stream, a, gb);
+        else hipLaunchKernelGGL((integrate_kernel<false, false>), grid, dim3(kLanes), lds, view->stream, a, gb);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(view->ev1, view->stream));
@@ -566,14 +841,16 @@ hdr_status hdr_view_synchronize(hdr_view *view) {
     return HDR_OK;
 }
 
-hdr_status hdr_view_last_timing(hdr_view *view, float *integrate_ms, float *frame_ms) {
+hdr_status hdr_view_last_timing(hdr_view *view, float *gbuffer_ms, float *integrate_ms, float *frame_ms) {
     if (!view) return fail(HDR_ERR_INVALID_ARGUMENT, "view is null");
     if (view->timing_pending) {
         HIP_TRY(hipEventSynchronize(view->ev2));
+        HIP_TRY(hipEventElapsedTime(&view->last_gbuffer_ms, view->evg, view->ev0));
         HIP_TRY(hipEventElapsedTime(&view->last_integrate_ms, view->ev0, view->ev1));
-        HIP_TRY(hipEventElapsedTime(&view->last_frame_ms, view->ev0, view->ev2));
+        HIP_TRY(hipEventElapsedTime(&view->last_frame_ms, view->evg, view->ev2));
         view->timing_pending = false;
     }
+    if (gbuffer_ms) *gbuffer_ms = view->last_gbuffer_ms;
     if (integrate_ms) *integrate_ms = view->last_integrate_ms;
     if (frame_ms) *frame_ms = view->last_frame_ms;
     return HDR_OK;

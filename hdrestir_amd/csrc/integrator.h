@@ -46,15 +46,37 @@ struct PixelStats {
     TraceStats ts;
 };
 
-template <bool kCount>
+// Ray queries of the integrator. With kDeferred the shading kernel holds no traversal code: the
+// only query it can meet (the RIS winner's shadow ray, when skipVisibility leaves the candidates
+// untested and nothing BSDF-sampled can reach an emitter) is recorded and answered by the
+// shadow kernel; closest-hit queries cannot occur on that path (the host checks eligibility).
+template <bool kCount, bool kDeferred>
 struct Tracer {
     const DeviceScene &s;
     PixelStats &st;
+    bool has_query = false;
+    V3 qo, qd;
+    float qtmax = 0.0f;
     __device__ bool trace(V3 o, V3 d, HitRec &h) {
-        RawHit rh;
-        if (!trace_closest<kCount>(s, o, d, rh, st.ts)) return false;
-        h = make_hit_record(s, o, d, rh);
-        return true;
+        if constexpr (kDeferred) {
+            return false;
+        } else {
+            RawHit rh;
+            if (!trace_closest_call<kCount>(s, o, d, rh, st.ts)) return false;
+            h = make_hit_record_call(s, o, d, rh);
+            return true;
+        }
+    }
+    __device__ bool occluded(V3 o, V3 d, float tmax) {
+        if constexpr (kDeferred) {
+            has_query = true;
+            qo = o;
+            qd = d;
+            qtmax = tmax;
+            return false;
+        } else {
+            return trace_any_call<kCount>(s, o, d, tmax, st.ts);
+        }
     }
 };
 
@@ -150,12 +172,15 @@ struct Cand {
     float target;
 };
 
-template <bool kCount, bool kDebug>
+template <bool kCountT, bool kDebugT, bool kDeferredT = false>
 struct PixelCtx {
+    static constexpr bool kCount = kCountT;
+    static constexpr bool kDebug = kDebugT;
+    static constexpr bool kDeferred = kDeferredT;
     const FrameArgs &a;
     const SpecTab &tab;
     Mt64 &rng;
-    Tracer<kCount> &tr;
+    Tracer<kCountT, kDeferredT> &tr;
     PixelStats &st;
     Trace &dbg;
     double *wlds;      // this lane's candidate weights: wlds[k * kLanes]
@@ -174,9 +199,12 @@ struct NeeEval {
     bool occluded;
 };
 
-template <bool kCount, bool kDebug>
-__device__ NeeEval eval_light_sample(PixelCtx<kCount, kDebug> &p, const Vertex &v, bool delta, const LSample &ls,
+template <class Ctx>
+__device__ NeeEval eval_light_sample(Ctx &p, const Vertex &v, bool delta, const LSample &ls,
                                      bool skipVis, int occlMode /*0 trace, 1 occluded, 2 visible*/) {
+    constexpr bool kCount = Ctx::kCount;
+    constexpr bool kDebug = Ctx::kDebug;
+    (void)kDebug;
     NeeEval r;
     r.contribution = spec(0.0f);
     r.vis_tested = true;
@@ -201,10 +229,9 @@ __device__ NeeEval eval_light_sample(PixelCtx<kCount, kDebug> &p, const Vertex &
     }
     bool occ;
     if (occlMode == 0) {
-        HitRec sh;
         const V3 so = v.h.pos + v.sn * 1e-4f;
         if (kCount) ++p.st.shadow;
-        occ = p.tr.trace(so, ls.dir, sh) && sh.depth < ls.dist - 1e-3f;
+        occ = p.tr.occluded(so, ls.dir, ls.dist - 1e-3f);
     } else {
         occ = occlMode == 1;
     }
@@ -214,8 +241,11 @@ __device__ NeeEval eval_light_sample(PixelCtx<kCount, kDebug> &p, const Vertex &
 }
 
 // Light pick of UniformLightSampler::ProposeCandidate (uniform_light_sampler.cpp:22-47).
-template <bool kCount, bool kDebug>
-__device__ __forceinline__ int pick_light(PixelCtx<kCount, kDebug> &p, bool record) {
+template <class Ctx>
+__device__ __forceinline__ int pick_light(Ctx &p, bool record) {
+    constexpr bool kCount = Ctx::kCount;
+    constexpr bool kDebug = Ctx::kDebug;
+    (void)kDebug;
     const uint32_t n = uint32_t(p.a.s.num_lights);
     const float u = p.rng.next_float();
     uint32_t idx = uint32_t(u * float(n));
@@ -227,8 +257,11 @@ __device__ __forceinline__ int pick_light(PixelCtx<kCount, kDebug> &p, bool reco
     return int(idx);
 }
 
-template <bool kCount, bool kDebug>
-__device__ __forceinline__ int pick_light_with(PixelCtx<kCount, kDebug> &p, Mt64 &rng, bool record) {
+template <class Ctx>
+__device__ __forceinline__ int pick_light_with(Ctx &p, Mt64 &rng, bool record) {
+    constexpr bool kCount = Ctx::kCount;
+    constexpr bool kDebug = Ctx::kDebug;
+    (void)kDebug;
     const uint32_t n = uint32_t(p.a.s.num_lights);
     const float u = rng.next_float();
     uint32_t idx = uint32_t(u * float(n));
@@ -266,8 +299,11 @@ struct ConnEval {
     bool has_light;
     int light;
 };
-template <bool kCount, bool kDebug>
-__device__ ConnEval eval_connection(PixelCtx<kCount, kDebug> &p, const Conn &cn) {
+template <class Ctx>
+__device__ ConnEval eval_connection(Ctx &p, const Conn &cn) {
+    constexpr bool kCount = Ctx::kCount;
+    constexpr bool kDebug = Ctx::kDebug;
+    (void)kDebug;
     const DeviceScene &s = p.a.s;
     ConnEval r;
     r.emission = spec(0.0f);
@@ -302,8 +338,11 @@ __device__ ConnEval eval_connection(PixelCtx<kCount, kDebug> &p, const Conn &cn)
 }
 
 // RIS weight of the path's own BSDF connection (first BSDF candidate, :263-294).
-template <bool kCount, bool kDebug>
-__device__ double conn_candidate(PixelCtx<kCount, kDebug> &p, const Conn &cn, Cand *out) {
+template <class Ctx>
+__device__ double conn_candidate(Ctx &p, const Conn &cn, Cand *out) {
+    constexpr bool kCount = Ctx::kCount;
+    constexpr bool kDebug = Ctx::kDebug;
+    (void)kDebug;
     const ConnEval e = eval_connection(p, cn);
     const Spec contribution = e.emission * cn.b.imul;
     const float target = spec_lum(p.tab, contribution);
@@ -324,17 +363,22 @@ __device__ double conn_candidate(PixelCtx<kCount, kDebug> &p, const Conn &cn, Ca
 }
 
 // One generated BSDF candidate (:296-402). mode 0: weight, 1: draws only, 2: full candidate.
-template <bool kCount, bool kDebug>
-__device__ double gen_bsdf_candidate(PixelCtx<kCount, kDebug> &p, const Vertex &v, Mt64 &rng, int mode, Cand *out) {
+template <class Ctx>
+__device__ double gen_bsdf_candidate(Ctx &p, const Vertex &v, Mt64 &rng, int mode, Cand *out) {
+    constexpr bool kCount = Ctx::kCount;
+    constexpr bool kDebug = Ctx::kDebug;
+    (void)kDebug;
     const DeviceScene &s = p.a.s;
+    if (mode == 1 || !p.a.trace_generated) {
+        // Only the draws matter: either a replay up to the winner, or a scene without environment
+        // and mesh lights, where the candidate is std::nullopt whatever it samples (:341-368).
+        skip_bounce_draws(p.tab, v.c, v.sn, v.inside, v.rayDir, rng);
+        if (kCount && mode == 0) ++p.st.cand_skipped;
+        return 0.0;
+    }
     Conn cn;
     int refl = 0, refr = 0;
     if (!sample_bounce(p.tab, v.c, v.sn, v.inside, v.h.pos, v.rayDir, 1, 1, refl, refr, rng, cn.b)) return 0.0;
-    if (mode == 1) return 0.0;
-    if (!p.a.trace_generated) {  // no environment, no mesh light: the candidate is std::nullopt
-        if (kCount) ++p.st.cand_skipped;
-        return 0.0;
-    }
     if (kCount) ++p.st.cand;
     cn.has_hit = p.tr.trace(cn.b.o, cn.b.d, cn.hit);
     if (!cn.has_hit) {
@@ -394,9 +438,12 @@ __device__ double gen_bsdf_candidate(PixelCtx<kCount, kDebug> &p, const Vertex &
 }
 
 // One NEE candidate (:407-443). mode 0: weight (+ occlusion bit), 1: draws only, 2: full candidate.
-template <bool kCount, bool kDebug>
-__device__ double nee_candidate(PixelCtx<kCount, kDebug> &p, const Vertex &v, Mt64 &rng, int mode, bool occlBit,
+template <class Ctx>
+__device__ double nee_candidate(Ctx &p, const Vertex &v, Mt64 &rng, int mode, bool occlBit,
                                 bool &occludedOut, Cand *out) {
+    constexpr bool kCount = Ctx::kCount;
+    constexpr bool kDebug = Ctx::kDebug;
+    (void)kDebug;
     const DeviceScene &s = p.a.s;
     occludedOut = false;
     if (s.num_lights <= 0) return 0.0;
@@ -434,9 +481,12 @@ __device__ double nee_candidate(PixelCtx<kCount, kDebug> &p, const Vertex &v, Mt
 
 // Rebuilds candidate `kk` by replaying the generation from `start` (a copy of the RNG taken before
 // candidate 0). occl: per-candidate shadow results recorded during generation.
-template <bool kCount, bool kDebug>
-__device__ void rebuild_candidate(PixelCtx<kCount, kDebug> &p, const Vertex &v, const Conn *conn, bool connCand,
+template <class Ctx>
+__device__ void rebuild_candidate(Ctx &p, const Vertex &v, const Conn *conn, bool connCand,
                                   Mt64 rng, uint64_t occl, int kk, Cand &out) {
+    constexpr bool kCount = Ctx::kCount;
+    constexpr bool kDebug = Ctx::kDebug;
+    (void)kDebug;
     const int nNee = p.a.n_nee;
     bool dummy;
     for (int k = 0; k < nNee; ++k) {
@@ -515,8 +565,11 @@ __device__ __forceinline__ void store_reservoir(hdr_reservoir *dst, bool has, co
 }
 
 // RisDirectLightIntegrator::Li (ris_direct_light_integrator.cpp:84-215).
-template <bool kCount, bool kDebug>
-__device__ Spec ris_direct(PixelCtx<kCount, kDebug> &p, const Vertex &v, const Conn *conn, bool primary, int pixel) {
+template <class Ctx>
+__device__ Spec ris_direct(Ctx &p, const Vertex &v, const Conn *conn, bool primary, int pixel) {
+    constexpr bool kCount = Ctx::kCount;
+    constexpr bool kDebug = Ctx::kDebug;
+    (void)kDebug;
     const FrameArgs &a = p.a;
     const int nNee = a.n_nee;
     const Mt64 start = fork_rng(p.rng, p.slot2);
@@ -668,8 +721,11 @@ __device__ Spec ris_direct(PixelCtx<kCount, kDebug> &p, const Vertex &v, const C
 }
 
 // MisDirectLightIntegrator::Li (mis_direct_light_integrator.cpp:70-271).
-template <bool kCount, bool kDebug>
-__device__ Spec mis_direct(PixelCtx<kCount, kDebug> &p, const Vertex &v, const Conn *conn) {
+template <class Ctx>
+__device__ Spec mis_direct(Ctx &p, const Vertex &v, const Conn *conn) {
+    constexpr bool kCount = Ctx::kCount;
+    constexpr bool kDebug = Ctx::kDebug;
+    (void)kDebug;
     const FrameArgs &a = p.a;
     const DeviceScene &s = a.s;
     const bool useBsdf = conn != nullptr;
@@ -687,9 +743,8 @@ __device__ Spec mis_direct(PixelCtx<kCount, kDebug> &p, const Vertex &v, const C
             if (nDotL > 0.0f && !(cosY <= 0.0f && !delta)) {
                 const float pNee = to_solid_angle(ls.pdf, ls.solid_angle, dist2, smax(cosY, 1e-6f));
                 if (pNee > 0.0f) {
-                    HitRec sh;
                     if (kCount) ++p.st.shadow;
-                    const bool occ = p.tr.trace(v.h.pos + v.sn * 1e-4f, ls.dir, sh) && sh.depth < ls.dist - 1e-3f;
+                    const bool occ = p.tr.occluded(v.h.pos + v.sn * 1e-4f, ls.dir, ls.dist - 1e-3f);
                     if (!occ) {
                         const V3 wo = -v.rayDir;
                         const float bsdfPdf = ggx_pdf(v.c, v.sn, wo, ls.dir);
@@ -758,9 +813,12 @@ __device__ Spec mis_direct(PixelCtx<kCount, kDebug> &p, const Vertex &v, const C
     return total;
 }
 
-template <bool kCount, bool kDebug>
-__device__ __forceinline__ Spec direct_light(PixelCtx<kCount, kDebug> &p, const Vertex &v, const Conn *conn,
+template <class Ctx>
+__device__ __forceinline__ Spec direct_light(Ctx &p, const Vertex &v, const Conn *conn,
                                              bool primary, int pixel) {
+    constexpr bool kCount = Ctx::kCount;
+    constexpr bool kDebug = Ctx::kDebug;
+    (void)kDebug;
     if (p.a.direct == HDR_DIRECT_RIS) return ris_direct(p, v, conn, primary, pixel);
     return mis_direct(p, v, conn);
 }
@@ -768,9 +826,12 @@ __device__ __forceinline__ Spec direct_light(PixelCtx<kCount, kDebug> &p, const 
 // Samples the bounce at a vertex and traces it (Detail::SampleBounceWithConnection,
 // direct_light_integrator_interface.h:18-40). When the ray provably cannot change the result
 // (last vertex, no emissive material, no environment) it is not traced and reads as a miss.
-template <bool kCount, bool kDebug>
-__device__ bool bounce_with_connection(PixelCtx<kCount, kDebug> &p, const Vertex &v, int &refl, int &refr,
+template <class Ctx>
+__device__ bool bounce_with_connection(Ctx &p, const Vertex &v, int &refl, int &refr,
                                        bool last, Conn &cn) {
+    constexpr bool kCount = Ctx::kCount;
+    constexpr bool kDebug = Ctx::kDebug;
+    (void)kDebug;
     if (!sample_bounce(p.tab, v.c, v.sn, v.inside, v.h.pos, v.rayDir, p.a.max_refl, p.a.max_refr, refl, refr, p.rng,
                        cn.b))
         return false;
@@ -797,8 +858,11 @@ __device__ __forceinline__ Vertex make_vertex(const FrameArgs &a, const SpecTab 
 }
 
 // PathIntegrator::Li / _li (source/integrators/path_integrator.cpp:48-209).
-template <bool kCount, bool kDebug>
-__device__ Spec path_li(PixelCtx<kCount, kDebug> &p, V3 rayDir, bool hasHit, HitRec hit, int pixel) {
+template <class Ctx>
+__device__ Spec path_li(Ctx &p, V3 rayDir, bool hasHit, HitRec hit, int pixel) {
+    constexpr bool kCount = Ctx::kCount;
+    constexpr bool kDebug = Ctx::kDebug;
+    (void)kDebug;
     const FrameArgs &a = p.a;
     const DeviceScene &s = a.s;
     Spec throughput = spec(1.0f);

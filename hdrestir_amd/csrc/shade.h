@@ -452,6 +452,74 @@ __device__ bool sample_bounce(const SpecTab &tab, const Clo &c, V3 n, bool isIns
     return true;
 }
 
+// Consumes exactly the RNG draws sample_bounce() would for a fresh bounce state (limits 1 / 1, as
+// for RIS BSDF candidates, ris_direct_light_integrator.cpp:298-302) without building the sample:
+// the lobe decisions use the same float operations, the directions are never needed.
+__device__ void skip_bounce_draws(const SpecTab &tab, const Clo &c, V3 n, bool isInside, V3 rayDir, Mt64 &rng) {
+    const GMaterial &m = *c.m;
+    if (m.is_default) {
+        (void)rng.next_float();
+        (void)rng.next_float();
+        return;
+    }
+    if (m.coat > 0.0f && !isInside) {
+        const float coatFresnel = m.coat * fresnel_dielectric(dot(rayDir, n), m.coat_ior);
+        if (rng.next_float() < coatFresnel) {
+            if (m.coat_roughness > 0.0f) {
+                (void)rng.next_float();
+                (void)rng.next_float();
+            }
+            return;
+        }
+    }
+    if (m.sheen > 0.0f && !isInside) {
+        const float cosTheta = smax(0.0f, dot(-rayDir, n));
+        const float sheenFresnel = m.sheen * pow5_f(1.0f - cosTheta);
+        if (rng.next_float() < sheenFresnel) {
+            if (m.sheen_roughness > 0.0f) {
+                (void)rng.next_float();
+                (void)rng.next_float();
+            }
+            return;
+        }
+    }
+    float localIor = m.ior;
+    {
+        const float C = 10000.0f;
+        const float B = localIor - C / (589.3f * 589.3f);
+        localIor = B + C / (tab.l0 * tab.l0);
+    }
+    const float fresnel = fresnel_dielectric(dot(rayDir, n), localIor);
+    float reflectProb = fresnel * m.specular;
+    if (m.metallic > 0.0f) reflectProb = smax(reflectProb, m.metallic);
+    const float randVal = rng.next_float();
+    if (randVal < reflectProb) {
+        if (m.roughness > 0.0f) {
+            (void)rng.next_float();
+            (void)rng.next_float();
+        }
+        return;
+    }
+    const float remainingProb = (randVal - reflectProb) / (1.0f - reflectProb);
+    if (m.transmission > 1e-6f && remainingProb < m.transmission) {
+        float etaI = 1.0f, etaT = localIor;
+        if (isInside) {
+            etaI = localIor;
+            etaT = 1.0f;
+        }
+        const float eta = etaI / etaT;
+        const float cosThetaI = dot(rayDir, n);
+        const float k = 1.0f - eta * eta * (1.0f - cosThetaI * cosThetaI);
+        if (k >= 0 && m.roughness > 0.0f) {
+            (void)rng.next_float();
+            (void)rng.next_float();
+        }
+        return;
+    }
+    (void)rng.next_float();
+    (void)rng.next_float();
+}
+
 // BeerAbsorption, shading_helpers.h:133-145
 __device__ __forceinline__ void beer(const SpecTab &tab, Spec &tp, const GMaterial &m, float t, bool isInside) {
     if (isInside && m.trans_depth > 0.0f && !m.thin_walled) {

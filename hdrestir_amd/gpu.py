@@ -69,7 +69,7 @@ def lib():
             "hdr_view_read_aovs": [P, P, P, P],
             "hdr_view_color_device_ptr": [P, C.POINTER(P)],
             "hdr_view_synchronize": [P],
-            "hdr_view_last_timing": [P, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+            "hdr_view_last_timing": [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)],
             "hdr_scene_intersect": [P, P, P, I, P, P, P],
             "hdr_rng_raw": [P, P, I, I, P],
             "hdr_rng_float": [P, P, I, I, P],
@@ -179,9 +179,10 @@ class Renderer:
         check(lib().hdr_view_synchronize(self.h))
 
     def last_timing(self):
-        a, b = C.c_float(), C.c_float()
-        check(lib().hdr_view_last_timing(self.h, C.byref(a), C.byref(b)))
-        return float(a.value), float(b.value)
+        """(G-buffer kernel ms, integration kernel ms, whole frame ms) of the last frame."""
+        g, a, b = C.c_float(), C.c_float(), C.c_float()
+        check(lib().hdr_view_last_timing(self.h, C.byref(g), C.byref(a), C.byref(b)))
+        return float(g.value), float(a.value), float(b.value)
 
     def read_color(self):
         check(lib().hdr_view_read_color(self.h, _p(self.color)))

@@ -279,8 +279,9 @@ hdr_status hdr_view_read_aovs(hdr_view *view, float *depth, float *albedo, float
 hdr_status hdr_view_color_device_ptr(hdr_view *view, void **out);
 /* Blocks until all work queued on the view's stream has finished. */
 hdr_status hdr_view_synchronize(hdr_view *view);
-/* Device time (ms) of the last hdr_render_frame's integration kernel and of the whole frame. */
-hdr_status hdr_view_last_timing(hdr_view *view, float *integrate_ms, float *frame_ms);
+/* Device time (ms) of the last hdr_render_frame's kernels: G-buffer (RaycastPass), integration,
+ * and the whole frame (G-buffer through accumulation). */
+hdr_status hdr_view_last_timing(hdr_view *view, float *gbuffer_ms, float *integrate_ms, float *frame_ms);
 
 /* ---- parity / measurement entry points ---- */
 
@@ -307,6 +308,8 @@ typedef struct hdr_frame_counters {
     uint64_t triangle_tests;
     uint64_t instance_visits;
     uint64_t rng_draws;
+    uint64_t primary_node_visits;   /* node visits / triangle tests of the G-buffer (RaycastPass) rays */
+    uint64_t primary_triangle_tests;
 } hdr_frame_counters;
 hdr_status hdr_view_set_counting(hdr_view *view, int32_t enable);
 hdr_status hdr_view_read_counters(hdr_view *view, hdr_frame_counters *out);

@@ -1,0 +1,75 @@
+"""The C-ABI library loads and exports every entry point include/hdrgpu.h declares, and the
+ctypes mirror of its structs matches the C layout. CPU only (no compute calls)."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from hdrestir_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "hdrgpu.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:hdr_status|const char \*|uint32_t)\s*(hdr_\w+)\s*\(", src, re.M)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "hdr_render_frame" in names and "hdr_scene_upload" in names and len(names) >= 20
+
+
+def test_library_exports_every_declared_symbol():
+    from hdrestir_amd import gpu
+    if not os.path.exists(gpu.LIB_PATH):
+        pytest.skip("libhdrgpu.so not built")
+    lib = C.CDLL(gpu.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    lib.hdr_abi_version.restype = C.c_uint32
+    assert lib.hdr_abi_version() == 1
+
+
+STRUCTS = {
+    "hdr_mesh_desc": abi.MeshDesc, "hdr_material_desc": abi.MaterialDesc, "hdr_instance_desc": abi.InstanceDesc,
+    "hdr_light_desc": abi.LightDesc, "hdr_scene_desc": abi.SceneDesc, "hdr_camera_params": abi.CameraParams,
+    "hdr_frame_desc": abi.FrameDesc, "hdr_pipeline_desc": abi.PipelineDesc, "hdr_reservoir": abi.Reservoir,
+    "hdr_pixel_debug": abi.PixelDebug, "hdr_frame_counters": abi.FrameCounters,
+}
+
+
+def test_ctypes_mirror_matches_c_layout():
+    prog = "#include <stdio.h>\n#include \"hdrgpu.h\"\nint main(void){\n"
+    for name in STRUCTS:
+        prog += f'printf("{name} %zu\\n", sizeof({name}));\n'
+    prog += "return 0;}\n"
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "sz.c")
+        exe = os.path.join(d, "sz")
+        open(c, "w").write(prog)
+        subprocess.run(["gcc", "-I", os.path.dirname(HEADER), c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout
+    sizes = dict(line.split() for line in out.strip().splitlines())
+    for name, cls in STRUCTS.items():
+        assert int(sizes[name]) == C.sizeof(cls), (name, sizes[name], C.sizeof(cls))
+    assert C.sizeof(abi.Reservoir) == 80
+
+
+def test_reservoir_and_debug_dtypes_match_structs():
+    assert abi.reservoir_dtype().itemsize == C.sizeof(abi.Reservoir)
+    assert abi.debug_dtype().itemsize == C.sizeof(abi.PixelDebug)
+
+
+def test_product_path_does_not_reference_the_oracle():
+    """libhdrgpu.so and the package must never load the CPU checker."""
+    pkg = os.path.join(ROOT, "hdrestir_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".cpp", ".h")):
+                txt = open(os.path.join(dirpath, f)).read()
+                assert "oracle" not in txt.lower().replace("oracles", ""), f
