@@ -21,6 +21,7 @@ _lib = None
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = ["hdrgpu.hip", "host_scene.cpp"]
+ADAPTER_SMOKE = os.path.join(_HERE, "host", "build", "adapter_smoke")
 
 
 def build(verbose=False):
@@ -32,8 +33,16 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stderr[-4000:])
     os.replace(tmp, LIB_PATH)
+    # the C++20 pass adapter's smoke program, linked against the library just built
+    host = os.path.join(_HERE, "host")
+    os.makedirs(os.path.join(host, "build"), exist_ok=True)
+    r = subprocess.run(["g++", "-std=c++20", "-O2", "-Wall", "-Wextra", "-o", ADAPTER_SMOKE,
+                        os.path.join(host, "adapter_smoke.cpp"), "-L" + _HERE, "-lhdrgpu",
+                        "-Wl,-rpath,$ORIGIN/../.."], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed on the pass adapter:\n" + r.stderr[-4000:])
     if verbose:
-        print("built", LIB_PATH)
+        print("built", LIB_PATH, "and", ADAPTER_SMOKE)
     return LIB_PATH
 
 

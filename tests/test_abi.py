@@ -73,3 +73,15 @@ def test_product_path_does_not_reference_the_oracle():
             if f.endswith((".py", ".hip", ".cpp", ".h")):
                 txt = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in txt.lower().replace("oracles", ""), f
+
+
+def test_cpp_adapter_reports_missing_device():
+    """The C++20 pass adapter turns a failing C-ABI call into hdrgpu::Error (no GPU here)."""
+    from hdrestir_amd import gpu
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU driver is present; the GPU suite runs the adapter for real")
+    if not os.path.exists(gpu.ADAPTER_SMOKE):
+        pytest.skip("adapter not built")
+    r = subprocess.run([gpu.ADAPTER_SMOKE, "--no-device"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "no device" in r.stdout
