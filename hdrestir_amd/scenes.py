@@ -442,3 +442,89 @@ def many_lights(width=1920, height=1080, seed=0, frames=64, n_lights=4096, n_sph
 
 
 CONFIGS = {"box_spheres": box_spheres, "many_lights": many_lights}
+
+
+def _dome_texels(w, h, rng):
+    """Small environment map: log-normal texels (mu 0, sigma 1) times a sky gradient."""
+    tex = np.exp(rng.normal(0.0, 1.0, (h, w, 3))).astype(np.float32)
+    grad = np.linspace(1.5, 0.2, h, dtype=np.float32)[:, None, None]
+    return (tex * grad * np.array([0.6, 0.7, 1.0], np.float32)).astype(np.float32)
+
+
+def features(width=48, height=48, seed=3, frames=4, dome=True, dof=False, emissive=True):
+    """Every shading feature of the path in one small scene: coat / sheen / transmission with Beer
+    absorption / partial opacity / emission / DefaultMaterial with vertex colours; rect, cone-shaped
+    point, distant and dome lights (the dome is the environment); an emissive sphere registered as a
+    mesh light."""
+    rng = np.random.default_rng(seed)
+    sc = Scene("features", width, height)
+    bp, bi = box_mesh(2.0)
+    # open the box's +z wall (camera side) so the dome is visible
+    keep = np.ones(len(bi), bool)
+    keep[2:4] = False
+    m_box = sc.add_mesh(bp, bi[keep])
+    sp, si, sn = icosphere_geodesic(3)
+    colors = np.abs(sp).astype(np.float32)
+    m_sph = sc.add_mesh(sp, si, normals=sn, colors=colors)
+    sc.add_instance(m_box, sc.add_material(diffuse_color=(0.6, 0.6, 0.55), roughness=0.7))
+    mats = [
+        dict(diffuse_color=(0.8, 0.2, 0.2), roughness=0.4, coat=0.8, coat_roughness=0.2, sheen=0.5),
+        dict(diffuse_color=(0.9, 0.9, 0.9), roughness=0.3, transmission=0.9, transmission_depth=0.5,
+             transmission_color=(0.5, 0.8, 0.9), ior=1.45),
+        dict(diffuse_color=(0.2, 0.7, 0.3), roughness=0.6, opacity=0.5),
+        dict(diffuse_color=(0.9, 0.8, 0.3), roughness=0.2, metallic=1.0),
+        dict(diffuse_color=(0.1, 0.1, 0.1), roughness=0.9, emission=4.0 if emissive else 0.0,
+             emission_color=(1.0, 0.6, 0.3)),
+        None,  # DefaultMaterial: vertex colours, Lambert sampling
+    ]
+    pos = [(-1.1, -1.4, -0.8), (0.0, -1.4, -1.0), (1.1, -1.4, -0.8), (-0.6, -1.5, 0.4), (0.7, -1.5, 0.5),
+           (0.0, -1.55, 1.1)]
+    light_idx = {}
+    for k, (m, p) in enumerate(zip(mats, pos)):
+        mat = sc.add_material(**m) if m is not None else -1
+        inst = sc.add_instance(m_sph, mat, _translate_scale(p, 0.45))
+        if k == 4:
+            light_idx["emissive"] = inst
+    rect = sc.add_light(abi.HDR_LIGHT_RECT, blackbody_rgb(5000.0), 15.0, _rect_light_down((0.3, 1.95, -0.2), 0.4),
+                        width=0.6, height=0.4)
+    cone = _translate_scale((-1.2, 1.6, 0.6), 1.0)
+    cone[:3, :3] = np.array([[1, 0, 0], [0, 0, -1], [0, 1, 0]], np.float64)  # local -Z -> world -Y
+    sc.add_light(abi.HDR_LIGHT_POINT, (1.0, 0.9, 0.7), 2.0, cone, cone_angle=50.0, cone_softness=0.3)
+    dist = np.eye(4)
+    d = np.array([0.3, -0.8, -0.5])
+    d /= np.linalg.norm(d)
+    z = -d
+    x = np.cross([0, 1, 0], z)
+    x /= np.linalg.norm(x)
+    y = np.cross(z, x)
+    dist[0, :3], dist[1, :3], dist[2, :3] = x, y, z
+    sc.add_light(abi.HDR_LIGHT_DISTANT, (1.0, 1.0, 0.9), 0.8, dist, exposure=-1.0)
+    # the emissive sphere is the mesh prim of this (small) rect light: GetLightAtHit finds it
+    em = sc.add_light(abi.HDR_LIGHT_RECT, (1.0, 0.6, 0.3), 4.0, _rect_light_down(pos[4], 0.0), width=0.3, height=0.3)
+    if emissive:
+        sc.instances[light_idx["emissive"]]["light"] = em
+    if dome:
+        sc.environment = sc.add_light(abi.HDR_LIGHT_DOME, (1.0, 1.0, 1.0), 0.7, np.eye(4),
+                                      dome_pixels=_dome_texels(32, 16, rng))
+    view = look_at_view((0.2, -0.2, 3.6), (0.0, -0.9, -0.5))
+    proj = perspective(50.0, width / height, 0.05, 100.0)
+    sc.cameras = [(view, proj)] * frames
+    sc.camera_params = None
+    if dof:
+        cp = abi.CameraParams()
+        cp.enable_dof, cp.enable_exposure = 1, 1
+        cp.focal_length, cp.f_stop, cp.focus_distance, cp.bokeh_blades = 35.0, 8.0, 4.0, 6
+        cp.lens_distortion, cp.iso, cp.shutter_speed = 0.05, 200.0, 0.04
+        sc.camera_params = cp
+    return sc
+
+
+def frame_with_camera(sc, frame_index):
+    f = sc.frame(frame_index)
+    if getattr(sc, "camera_params", None) is not None:
+        f.has_camera_params = 1
+        f.camera = sc.camera_params
+    return f
+
+
+CONFIGS["features"] = features
