@@ -24,19 +24,26 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 32   # one BVH node record (6 fp32 bounds + 2 int)
-TRI_BYTES = 36    # three fp32x3 vertices read by the ray-triangle test
-STATE_BYTES = 208  # per pixel: accumulation rd+wr 32, colour wr 16, reservoir rd+wr 160
-GBUF_BYTES = 64    # per pixel G-buffer record, written by the RaycastPass kernel, read by integration
+# Algorithmic bytes: what each kernel must move at least once per unit of work.
+NODE_BYTES = 32    # one BVH node record (6 fp32 bounds + 2 int)
+TRI_BYTES = 36     # three fp32x3 vertices read by the ray-triangle test
+GBUF_BYTES = 64    # per pixel G-buffer record, written by the G-buffer kernel, read once after
+RES_BYTES = 160    # per pixel reservoir (80 B) read + written
+ACCUM_BYTES = 48   # per pixel accumulation rd+wr 32 + colour write 16
+RGB2_BYTES = 32    # deferred path: visible + occluded colour written by shading (one read back: 16)
+FLAG_BYTES = 4     # deferred path: per pixel visibility flag
+QUERY_BYTES = 32   # deferred path: one shadow query (origin, tmax, direction, pixel)
+KERNEL_NAMES = {"gbuffer": "gbuffer_kernel", "integrate": "shade_direct_kernel", "shadow": "shadow_kernel",
+                "accumulate": "accumulate_deferred_kernel", "firefly": "firefly_accumulate_kernel"}
 
 
 def _log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def read_traffic():
-    """HBM bytes per integrate-kernel launch from the committed rocprofv3 --pmc summary, or None."""
-    path = os.path.join(ROOT, "profiles", "integrate_many_lights_pmc.txt")
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_many_lights_pmc.txt")
     if not os.path.exists(path):
         return None
     vals = {}
@@ -75,6 +82,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=8)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES bytes 4 times (the reference "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
     ap.add_argument("--dolly", action="store_true",
                     help="also time the moving-camera variant (each move clears reservoirs and accumulation)")
     args = ap.parse_args()
@@ -82,6 +92,16 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the path shards by pixels and requests, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(args.copy_calibration, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": args.copy_calibration, "launches": 4}), flush=True)
+        return
 
     from hdrestir_amd import abi, gpu, scenes
     _log("building scene")
@@ -117,19 +137,16 @@ def main():
     for i in range(args.warmup):
         r.render(frame_desc(i, False), pipe, read_color=False)
     r.synchronize()
-    gb_ms, integ = [], []
+    r.stage_times()  # clears the per-kernel sums
     t0 = time.perf_counter()
     for i in range(args.steps):
         r.render(frame_desc(args.warmup + i, False), pipe, read_color=False)
-        g_ms, i_ms, _ = r.last_timing()  # HIP events on the view's stream around each kernel
-        gb_ms.append(g_ms)
-        integ.append(i_ms)
     r.synchronize()
     t1 = time.perf_counter()
+    # HIP events on the view's stream around each kernel of the timed frames
+    stages = {k: v for k, v in r.stage_times().items() if v[1] > 0}
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     fps = 1000.0 / ms_per_step
-    kern_ms = sum(integ) / len(integ)
-    gbuf_ms = sum(gb_ms) / len(gb_ms)
     img = r.read_color()
 
     dolly_fps = None
@@ -144,14 +161,31 @@ def main():
 
     rays = counts["primary_rays"] + counts["bounce_rays"] + counts["shadow_rays"] + counts["candidate_rays"]
     ref_rays = rays + counts["candidate_rays_skipped"]
-    # algorithmic bytes per launch: BVH node records and triangle vertices of the rays each kernel
-    # traces, plus the per-pixel state it streams
-    sec_nodes = counts["node_visits"] - counts["primary_node_visits"]
-    sec_tris = counts["triangle_tests"] - counts["primary_triangle_tests"]
-    alg_bytes = NODE_BYTES * sec_nodes + TRI_BYTES * sec_tris + (STATE_BYTES + GBUF_BYTES) * P
-    gbuf_bytes = NODE_BYTES * counts["primary_node_visits"] + TRI_BYTES * counts["primary_triangle_tests"] + \
-        GBUF_BYTES * P
-    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    pn, pt = counts["primary_node_visits"], counts["primary_triangle_tests"]
+    sn, st = counts["node_visits"] - pn, counts["triangle_tests"] - pt
+    q = counts["shadow_rays"]
+    deferred = "shadow" in stages
+    if deferred:
+        alg = {"gbuffer": NODE_BYTES * pn + TRI_BYTES * pt + GBUF_BYTES * P,
+               "integrate": (GBUF_BYTES + RES_BYTES + RGB2_BYTES + FLAG_BYTES) * P + QUERY_BYTES * q,
+               "shadow": QUERY_BYTES * q + NODE_BYTES * sn + TRI_BYTES * st,
+               "accumulate": (RGB2_BYTES // 2 + FLAG_BYTES + ACCUM_BYTES) * P}
+    else:
+        alg = {"gbuffer": NODE_BYTES * pn + TRI_BYTES * pt + GBUF_BYTES * P,
+               "integrate": NODE_BYTES * sn + TRI_BYTES * st + (GBUF_BYTES + RES_BYTES + ACCUM_BYTES) * P}
+    kernels = {}
+    for k, (ms, n) in stages.items():
+        avg = ms / n
+        name = KERNEL_NAMES[k] if (deferred or k != "integrate") else "integrate_kernel"
+        e = {"avg_ms": round(avg, 4), "launches": n}
+        if k in alg:
+            e["algorithmic_bytes_per_launch"] = round(alg[k])
+            e["achieved_GBs"] = round(alg[k] / (avg * 1e-3) / 1e9, 2)
+        kernels[name] = e
+    dom_stage = max(stages, key=lambda k: stages[k][0])
+    dom = KERNEL_NAMES[dom_stage] if (deferred or dom_stage != "integrate") else "integrate_kernel"
+    dom_avg = stages[dom_stage][0] / stages[dom_stage][1]
+    achieved = alg[dom_stage] / (dom_avg * 1e-3) / 1e9
     line = {
         "metric": "ReSTIR frames/s at 1080p (many-lights, M=32, 1 temporal); Mrays/s alongside",
         "value": round(fps, 3),
@@ -175,14 +209,11 @@ def main():
                            "shadow": counts["shadow_rays"], "bsdf_candidate_traced": counts["candidate_rays"],
                            "skipped_cannot_contribute": counts["candidate_rays_skipped"]},
         "dolly_frames_per_s": None if dolly_fps is None else round(dolly_fps, 3),
-        "integrate_kernel_ms": round(kern_ms, 4),
-        "gbuffer_kernel_ms": round(gbuf_ms, 4),
-        "gbuffer_roofline": {"achieved": round(gbuf_bytes / (gbuf_ms * 1e-3) / 1e9, 2), "unit": "GB/s",
-                             "algorithmic_bytes_per_launch": gbuf_bytes},
-        "roofline": {"kernel": "integrate_kernel", "bound": "hbm", "achieved": round(achieved, 2),
+        "kernels": kernels,
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
-                     "algorithmic_bytes_per_launch": round(alg_bytes)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(dom),
+                     "avg_ms": round(dom_avg, 4), "algorithmic_bytes_per_launch": round(alg[dom_stage])},
         "image_mean_rgb": [round(float(v), 3) for v in img[..., :3].reshape(-1, 3).mean(0)],
     }
     r.close()

@@ -12,7 +12,7 @@
 
 #include "../../include/hdrgpu.h"
 #include "host_scene.h"
-#include "integrator.h"
+#include "deferred.h"
 
 using namespace hdr;
 
@@ -64,7 +64,7 @@ __device__ __forceinline__ void gbuffer_pixel(const FrameArgs &a, GRec *gbuf, in
         crng.init(seed, nullptr);
         camera_ray(a, float(x) + 0.5f, float(y) + 0.5f, &crng, co, cd);
     } else {
-        camera_ray(a, float(x) + 0.5f, float(y) + 0.5f, nullptr, co, cd);
+        camera_ray(a, float(x) + 0.5f, float(y) + 0.5f, static_cast<Mt64 *>(nullptr), co, cd);
     }
     RawHit rh;
     HitRec g = {};
@@ -198,29 +198,33 @@ __device__ __forceinline__ void shade_direct_pixel(const FrameArgs &a, const GRe
                                                    bool &query, ShadowQuery &q) {
     const int pixel = y * a.W + x;
     const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
-    Tracer<kCount, true> tracer{a.s, st};
     HitRec g;
     const bool gHit = load_grec(gbuf + pixel, g);
-    Mt64 rng;
-    uint64_t *slot = a.spill ? a.spill + size_t(pixel) * (2 * kMtN) : nullptr;
-    rng.init(seed, slot);
+    Mt64Lean rng;
+    rng.init(seed, nullptr);
     const float px = float(x) + rng.next_float();
     const float py = float(y) + rng.next_float();
     V3 ro, rd;
     camera_ray(a, px, py, &rng, ro, rd);
     const SpecTab tab = make_tab(rng.next_float());
     Trace dbg = {2166136261u, 2166136261u, 0};
-    PixelCtx<kCount, kDebug, true> p{a, tab, rng, tracer, st, dbg, wlane, slot ? slot + kMtN : nullptr};
+    V3 qo = v3s(0.0f), qd = v3s(0.0f);
+    float qtmax = 0.0f;
+    query = false;
     Spec lVis = spec(0.0f), lOcc = spec(0.0f);
     if (gHit) {  // PathIntegrator::Li / _li at depth 1 (path_integrator.cpp:112-177); opacity >= 0.999
         const Clo c = get_closure(a.s, g, a.enable_ss != 0);
         Spec throughput = spec(1.0f);
         const Vertex v0 = make_vertex(a, tab, g, c, rd, throughput);
         const Spec total = rgb2spec(tab, mv3(c.m->emission));
+        // the path's BSDF connection: sampled, never traced (it can reach no emitter)
         int refl = 0, refr = 0;
-        Conn first;
-        const bool ok = bounce_with_connection(p, v0, refl, refr, true, first);
-        const Spec direct = direct_light(p, v0, ok ? &first : nullptr, true, pixel);
+        Bounce first;
+        const bool ok = sample_bounce_inl(tab, v0.c, v0.sn, v0.inside, v0.h.pos, v0.rayDir, a.max_refl, a.max_refr,
+                                          refl, refr, rng, first);
+        if (kCount && ok) ++st.cand_skipped;
+        const Spec direct = ris_direct_lean<kCount, kDebug>(a, tab, v0, ok && first.pdf > 0.0f, rng, wlane, pixel, st,
+                                                           dbg, query, qo, qd, qtmax);
         lVis = throughput * (total + direct);
         lOcc = throughput * (total + spec(0.0f));
     } else if (a.s.environment >= 0 && a.ibl_bg) {
@@ -233,11 +237,10 @@ __device__ __forceinline__ void shade_direct_pixel(const FrameArgs &a, const GRe
     }
     rgb2[2 * pixel] = make_float4(cv.x, cv.y, cv.z, 1.0f);
     rgb2[2 * pixel + 1] = make_float4(co.x, co.y, co.z, 1.0f);
-    query = tracer.has_query;
     flags[pixel] = query ? 1u : 0u;
     if (query) {
-        q.o = make_float4(tracer.qo.x, tracer.qo.y, tracer.qo.z, tracer.qtmax);
-        q.d = make_float4(tracer.qd.x, tracer.qd.y, tracer.qd.z, __int_as_float(pixel));
+        q.o = make_float4(qo.x, qo.y, qo.z, qtmax);
+        q.d = make_float4(qd.x, qd.y, qd.z, __int_as_float(pixel));
     }
     if (kDebug) {
         hdr_pixel_debug d;
@@ -457,12 +460,36 @@ struct hdr_view {
     int W = 0, H = 0;
     hipStream_t stream = nullptr;
     DevBuf res, accum, color, frame, dbg, depth, albedo, normal, counters, spill, gbuf, rgb2, flags, queue, qcount;
-    hipEvent_t evg = nullptr, ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     int counting = 0;
     int frames = 0;
-    float last_gbuffer_ms = 0.0f, last_integrate_ms = 0.0f, last_frame_ms = 0.0f;
-    bool timing_pending = false;
+    // Per-kernel timing: one slot of events per frame in a ring, read back lazily (when the slot is
+    // reused or the sums are requested), so timing never stalls the stream.
+    // ev[k] .. ev[k+1] brackets stage k (HDR_STAGE_*); bit k of `ran` says stage k launched.
+    struct StageSlot {
+        hipEvent_t ev[HDR_STAGE_COUNT + 1] = {};
+        uint32_t ran = 0;
+        bool pending = false;
+    };
+    static constexpr int kRing = 16;
+    StageSlot ring[kRing];
+    int ring_next = 0, ring_last = -1;
+    double stage_ms[HDR_STAGE_COUNT] = {};
+    int32_t stage_n[HDR_STAGE_COUNT] = {};
 };
+
+static hdr_status harvest(hdr_view *v, hdr_view::StageSlot &s) {
+    if (!s.pending) return HDR_OK;
+    HIP_TRY(hipEventSynchronize(s.ev[HDR_STAGE_COUNT]));
+    for (int k = 0; k < HDR_STAGE_COUNT; ++k) {
+        if (!(s.ran & (1u << k))) continue;
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, s.ev[k], s.ev[k + 1]));
+        v->stage_ms[k] += ms;
+        v->stage_n[k] += 1;
+    }
+    s.pending = false;
+    return HDR_OK;
+}
 
 extern "C" {
 
@@ -607,10 +634,8 @@ hdr_status hdr_view_create(hdr_context *ctx, int32_t width, int32_t height, hdr_
     HIP_TRY(v->gbuf.alloc(P * sizeof(GRec)));
     HIP_TRY(v->qcount.alloc(sizeof(unsigned int)));
     HIP_TRY(hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreate(&v->evg));
-    HIP_TRY(hipEventCreate(&v->ev0));
-    HIP_TRY(hipEventCreate(&v->ev1));
-    HIP_TRY(hipEventCreate(&v->ev2));
+    for (auto &slot : v->ring)
+        for (auto &e : slot.ev) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipMemsetAsync(v->res.p, 0, v->res.n, v->stream));
     HIP_TRY(hipMemsetAsync(v->accum.p, 0, v->accum.n, v->stream));
     HIP_TRY(hipMemsetAsync(v->color.p, 0, v->color.n, v->stream));
@@ -624,10 +649,9 @@ hdr_status hdr_view_destroy(hdr_view *view) {
     if (!view) return HDR_OK;
     (void)hipSetDevice(view->ctx->device);
     if (view->stream) (void)hipStreamSynchronize(view->stream);
-    if (view->evg) (void)hipEventDestroy(view->evg);
-    if (view->ev0) (void)hipEventDestroy(view->ev0);
-    if (view->ev1) (void)hipEventDestroy(view->ev1);
-    if (view->ev2) (void)hipEventDestroy(view->ev2);
+    for (auto &slot : view->ring)
+        for (auto &e : slot.ev)
+            if (e) (void)hipEventDestroy(e);
     if (view->stream) (void)hipStreamDestroy(view->stream);
     delete view;
     return HDR_OK;
@@ -760,23 +784,31 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         a.counters = static_cast<unsigned long long *>(view->counters.p);
     }
     const dim3 grid((W + 7) / 8, (H + 7) / 8);
-    const size_t lds = size_t(a.direct == HDR_DIRECT_RIS ? a.n_cand : 1) * kLanes * sizeof(double);
     GRec *gb = static_cast<GRec *>(view->gbuf.p);
     // The deferred-shadow path applies when the winner's shadow ray is the only ray after the
     // G-buffer: RIS at depth 1, skipVisibility, no translucent material (no pass-through rays),
-    // nothing BSDF-sampled can reach an emitter (no emissive material, mesh light or environment).
+    // nothing BSDF-sampled can reach an emitter (no emissive material, mesh light or environment),
+    // and no pixel draws past RNG output 310 (the lean stream of that path has no spill slot).
     const bool deferred = a.direct == HDR_DIRECT_RIS && a.max_depth == 1 && a.skip_vis && !scene->any_translucent &&
-                          !a.trace_generated && !a.trace_last_connection;
+                          !a.trace_generated && !a.trace_last_connection && bound < kMtN - 1;
+    // candidate weights in LDS: the NEE candidates on the deferred path (its BSDF candidates carry
+    // no weight), every candidate on the general path
+    const size_t lds = size_t(deferred ? a.n_nee : (a.direct == HDR_DIRECT_RIS ? a.n_cand : 1)) * kLanes *
+                       sizeof(double);
     if (deferred) {
         if (hdr_status st = ensure(view->rgb2, P * 2 * sizeof(float4))) return st;
         if (hdr_status st = ensure(view->flags, P * sizeof(uint32_t))) return st;
         if (hdr_status st = ensure(view->queue, P * sizeof(ShadowQuery))) return st;
     }
-    HIP_TRY(hipEventRecord(view->evg, view->stream));
+    hdr_view::StageSlot &ts = view->ring[view->ring_next];
+    if (hdr_status st = harvest(view, ts)) return st;
+    ts.ran = 0;
+    HIP_TRY(hipEventRecord(ts.ev[HDR_STAGE_GBUFFER], view->stream));
     if (view->counting) hipLaunchKernelGGL((gbuffer_kernel<true>), grid, dim3(kLanes), 0, view->stream, a, gb);
     else hipLaunchKernelGGL((gbuffer_kernel<false>), grid, dim3(kLanes), 0, view->stream, a, gb);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(view->ev0, view->stream));
+    ts.ran |= 1u << HDR_STAGE_GBUFFER;
+    HIP_TRY(hipEventRecord(ts.ev[HDR_STAGE_INTEGRATE], view->stream));
     if (deferred) {
         float4 *rgb2 = static_cast<float4 *>(view->rgb2.p);
         uint32_t *flags = static_cast<uint32_t *>(view->flags.p);
@@ -794,17 +826,21 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         }
 #undef HDR_SHADE
         HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(ts.ev[HDR_STAGE_SHADOW], view->stream));
         const int sblocks = int(std::min<size_t>((P + kLanes - 1) / kLanes, 8192));
         if (view->counting)
             hipLaunchKernelGGL((shadow_kernel<true>), dim3(sblocks), dim3(kLanes), 0, view->stream, a, queue, qc, flags);
         else
             hipLaunchKernelGGL((shadow_kernel<false>), dim3(sblocks), dim3(kLanes), 0, view->stream, a, queue, qc, flags);
         HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(ts.ev[HDR_STAGE_ACCUMULATE], view->stream));
         const int ablocks = int((P + 255) / 256);
         if (debug_out)
             hipLaunchKernelGGL((accumulate_deferred_kernel<true>), dim3(ablocks), dim3(256), 0, view->stream, a, rgb2, flags);
         else
             hipLaunchKernelGGL((accumulate_deferred_kernel<false>), dim3(ablocks), dim3(256), 0, view->stream, a, rgb2, flags);
+        HIP_TRY(hipGetLastError());
+        ts.ran |= (1u << HDR_STAGE_INTEGRATE) | (1u << HDR_STAGE_SHADOW) | (1u << HDR_STAGE_ACCUMULATE);
     } else if (view->counting) {
         if (debug_out) hipLaunchKernelGGL((integrate_kernel<true, true>), grid, dim3(kLanes), lds, view->stream, a, gb);
         else hipLaunchKernelGGL((integrate_kernel<true, false>), grid, dim3(kLanes), lds, view->stream, a, gb);
@@ -813,15 +849,23 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         else hipLaunchKernelGGL((integrate_kernel<false, false>), grid, dim3(kLanes), lds, view->stream, a, gb);
     }
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(view->ev1, view->stream));
+    if (!deferred) {
+        ts.ran |= 1u << HDR_STAGE_INTEGRATE;
+        HIP_TRY(hipEventRecord(ts.ev[HDR_STAGE_SHADOW], view->stream));
+        HIP_TRY(hipEventRecord(ts.ev[HDR_STAGE_ACCUMULATE], view->stream));
+    }
+    HIP_TRY(hipEventRecord(ts.ev[HDR_STAGE_FIREFLY], view->stream));
     if (a.firefly) {
         const dim3 b2(16, 16), g2((W + 15) / 16, (H + 15) / 16);
         hipLaunchKernelGGL(firefly_accumulate_kernel, g2, b2, 0, view->stream, a.frame_rgb, a.accum, a.color, W, H,
                            a.inv_n);
         HIP_TRY(hipGetLastError());
+        ts.ran |= 1u << HDR_STAGE_FIREFLY;
     }
-    HIP_TRY(hipEventRecord(view->ev2, view->stream));
-    view->timing_pending = true;
+    HIP_TRY(hipEventRecord(ts.ev[HDR_STAGE_COUNT], view->stream));
+    ts.pending = true;
+    view->ring_last = view->ring_next;
+    view->ring_next = (view->ring_next + 1) % hdr_view::kRing;
     view->frames++;
     if (color_out) {
         HIP_TRY(hipMemcpyAsync(color_out, view->color.p, P * sizeof(float4), hipMemcpyDeviceToHost, view->stream));
@@ -843,16 +887,33 @@ hdr_status hdr_view_synchronize(hdr_view *view) {
 
 hdr_status hdr_view_last_timing(hdr_view *view, float *gbuffer_ms, float *integrate_ms, float *frame_ms) {
     if (!view) return fail(HDR_ERR_INVALID_ARGUMENT, "view is null");
-    if (view->timing_pending) {
-        HIP_TRY(hipEventSynchronize(view->ev2));
-        HIP_TRY(hipEventElapsedTime(&view->last_gbuffer_ms, view->evg, view->ev0));
-        HIP_TRY(hipEventElapsedTime(&view->last_integrate_ms, view->ev0, view->ev1));
-        HIP_TRY(hipEventElapsedTime(&view->last_frame_ms, view->evg, view->ev2));
-        view->timing_pending = false;
+    float g = 0.0f, i = 0.0f, f = 0.0f;
+    if (view->ring_last >= 0) {
+        HIP_TRY(hipSetDevice(view->ctx->device));
+        hdr_view::StageSlot &s = view->ring[view->ring_last];
+        HIP_TRY(hipEventSynchronize(s.ev[HDR_STAGE_COUNT]));
+        HIP_TRY(hipEventElapsedTime(&g, s.ev[HDR_STAGE_GBUFFER], s.ev[HDR_STAGE_INTEGRATE]));
+        HIP_TRY(hipEventElapsedTime(&i, s.ev[HDR_STAGE_INTEGRATE], s.ev[HDR_STAGE_FIREFLY]));
+        HIP_TRY(hipEventElapsedTime(&f, s.ev[HDR_STAGE_GBUFFER], s.ev[HDR_STAGE_COUNT]));
     }
-    if (gbuffer_ms) *gbuffer_ms = view->last_gbuffer_ms;
-    if (integrate_ms) *integrate_ms = view->last_integrate_ms;
-    if (frame_ms) *frame_ms = view->last_frame_ms;
+    if (gbuffer_ms) *gbuffer_ms = g;
+    if (integrate_ms) *integrate_ms = i;
+    if (frame_ms) *frame_ms = f;
+    return HDR_OK;
+}
+
+hdr_status hdr_view_read_stage_times(hdr_view *view, double *stage_ms, int32_t *launches) {
+    if (!view || !stage_ms || !launches) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipSetDevice(view->ctx->device));
+    for (int k = 1; k <= hdr_view::kRing; ++k) {  // oldest first
+        if (hdr_status st = harvest(view, view->ring[(view->ring_next + k - 1) % hdr_view::kRing])) return st;
+    }
+    for (int k = 0; k < HDR_STAGE_COUNT; ++k) {
+        stage_ms[k] = view->stage_ms[k];
+        launches[k] = view->stage_n[k];
+        view->stage_ms[k] = 0.0;
+        view->stage_n[k] = 0;
+    }
     return HDR_OK;
 }
 

@@ -81,7 +81,8 @@ struct Tracer {
 };
 
 // GenerateCameraRay (source/scene/camera_ray.cpp:10-76)
-__device__ __forceinline__ void camera_ray(const FrameArgs &a, float px, float py, Mt64 *rng, V3 &o, V3 &d) {
+template <class R>
+__device__ __forceinline__ void camera_ray(const FrameArgs &a, float px, float py, R *rng, V3 &o, V3 &d) {
     float ndcX = (2.0f * px / float(a.W)) - 1.0f;
     float ndcY = (2.0f * py / float(a.H)) - 1.0f;
     if (a.has_cam && a.cam.lens_distortion != 0.0f) {
@@ -451,7 +452,7 @@ __device__ double nee_candidate(Ctx &p, const Vertex &v, Mt64 &rng, int mode, bo
     const GLight &L = s.lights[li];
     if (mode == 1) {
         const int nd = light_draws(s, L);
-        for (int q = 0; q < nd; ++q) (void)rng.next_float();
+        for (int q = 0; q < nd; ++q) rng.discard();
         return 0.0;
     }
     LSample ls;

@@ -127,6 +127,30 @@ struct Mt64 {
         return mt_temper(y);
     }
     HD float next_float() { return mt_canonical(next_raw()); }
+    HD void discard() { (void)next_raw(); }
+};
+
+// The stream for callers that provably never draw past output 310 (the host checks a per-pixel
+// draw bound): the second-twist branches are dropped and a discarded draw only moves the cursors.
+struct Mt64Lean : Mt64 {
+    HD uint64_t next_raw() {
+        uint64_t y;
+        if (k < 156u) {
+            y = hi ^ mt_twist(lo, lo1);
+        } else {
+            if (k == 156u) restart();
+            y = hi ^ mt_twist(lo, lo1) ^ mt_twist(hi, hi1);
+        }
+        advance();
+        ++k;
+        return mt_temper(y);
+    }
+    HD float next_float() { return mt_canonical(next_raw()); }
+    HD void discard() {
+        if (k == 156u) restart();
+        advance();
+        ++k;
+    }
 };
 
 }  // namespace hdr

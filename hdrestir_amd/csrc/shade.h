@@ -268,7 +268,8 @@ struct RoughRefl {
     V3 dir, h;
     float cosH, nDotV, nDotL, nDotH;
 };
-__device__ __forceinline__ RoughRefl rough_reflect(V3 rayDir, V3 n, float roughness, Mt64 &rng) {
+template <class R>
+__device__ __forceinline__ RoughRefl rough_reflect(V3 rayDir, V3 n, float roughness, R &rng) {
     const float u1 = rng.next_float();
     const float u2 = rng.next_float();
     RoughRefl r;
@@ -284,8 +285,9 @@ __device__ __forceinline__ RoughRefl rough_reflect(V3 rayDir, V3 n, float roughn
 
 // Coat / sheen layer of PreviewSurfaceMaterial::SampleBounce (preview_surface.cpp:136-236).
 // Returns 0 = layer not chosen, 1 = sample produced, 2 = bounce-limit error.
-__device__ int layer_bounce(const SpecTab &tab, V3 n, V3 hitPos, V3 rayDir, int maxRefl, int &refl, Mt64 &rng,
-                            float prob, float roughness, V3 tint, Bounce &out) {
+template <class R>
+__device__ __forceinline__ int layer_bounce(const SpecTab &tab, V3 n, V3 hitPos, V3 rayDir, int maxRefl, int &refl,
+                                            R &rng, float prob, float roughness, V3 tint, Bounce &out) {
     if (!(rng.next_float() < prob)) return 0;
     if (refl >= maxRefl) return 2;
     ++refl;
@@ -315,8 +317,10 @@ __device__ int layer_bounce(const SpecTab &tab, V3 n, V3 hitPos, V3 rayDir, int 
 
 // IMaterial::SampleBounce: PreviewSurfaceMaterial (preview_surface.cpp:126-381) or
 // DefaultMaterial (default_material.cpp:65-86). Returns false on a bounce-limit error.
-__device__ bool sample_bounce(const SpecTab &tab, const Clo &c, V3 n, bool isInside, V3 hitPos, V3 rayDir,
-                              int maxRefl, int maxRefr, int &refl, int &refr, Mt64 &rng, Bounce &out) {
+template <class R>
+__device__ __forceinline__ bool sample_bounce_inl(const SpecTab &tab, const Clo &c, V3 n, bool isInside, V3 hitPos,
+                                                  V3 rayDir, int maxRefl, int maxRefr, int &refl, int &refr, R &rng,
+                                                  Bounce &out) {
     const GMaterial &m = *c.m;
     out.skip_roulette = false;
     if (m.is_default) {
@@ -452,22 +456,30 @@ __device__ bool sample_bounce(const SpecTab &tab, const Clo &c, V3 n, bool isIns
     return true;
 }
 
+// Out-of-line form for the integration megakernel (keeps its code size in check).
+__device__ bool sample_bounce(const SpecTab &tab, const Clo &c, V3 n, bool isInside, V3 hitPos, V3 rayDir,
+                              int maxRefl, int maxRefr, int &refl, int &refr, Mt64 &rng, Bounce &out) {
+    return sample_bounce_inl(tab, c, n, isInside, hitPos, rayDir, maxRefl, maxRefr, refl, refr, rng, out);
+}
+
 // Consumes exactly the RNG draws sample_bounce() would for a fresh bounce state (limits 1 / 1, as
 // for RIS BSDF candidates, ris_direct_light_integrator.cpp:298-302) without building the sample:
 // the lobe decisions use the same float operations, the directions are never needed.
-__device__ void skip_bounce_draws(const SpecTab &tab, const Clo &c, V3 n, bool isInside, V3 rayDir, Mt64 &rng) {
+template <class R>
+__device__ __forceinline__ void skip_bounce_draws_inl(const SpecTab &tab, const Clo &c, V3 n, bool isInside,
+                                                      V3 rayDir, R &rng) {
     const GMaterial &m = *c.m;
     if (m.is_default) {
-        (void)rng.next_float();
-        (void)rng.next_float();
+        rng.discard();
+        rng.discard();
         return;
     }
     if (m.coat > 0.0f && !isInside) {
         const float coatFresnel = m.coat * fresnel_dielectric(dot(rayDir, n), m.coat_ior);
         if (rng.next_float() < coatFresnel) {
             if (m.coat_roughness > 0.0f) {
-                (void)rng.next_float();
-                (void)rng.next_float();
+                rng.discard();
+                rng.discard();
             }
             return;
         }
@@ -477,8 +489,8 @@ __device__ void skip_bounce_draws(const SpecTab &tab, const Clo &c, V3 n, bool i
         const float sheenFresnel = m.sheen * pow5_f(1.0f - cosTheta);
         if (rng.next_float() < sheenFresnel) {
             if (m.sheen_roughness > 0.0f) {
-                (void)rng.next_float();
-                (void)rng.next_float();
+                rng.discard();
+                rng.discard();
             }
             return;
         }
@@ -495,8 +507,8 @@ __device__ void skip_bounce_draws(const SpecTab &tab, const Clo &c, V3 n, bool i
     const float randVal = rng.next_float();
     if (randVal < reflectProb) {
         if (m.roughness > 0.0f) {
-            (void)rng.next_float();
-            (void)rng.next_float();
+            rng.discard();
+            rng.discard();
         }
         return;
     }
@@ -511,13 +523,17 @@ __device__ void skip_bounce_draws(const SpecTab &tab, const Clo &c, V3 n, bool i
         const float cosThetaI = dot(rayDir, n);
         const float k = 1.0f - eta * eta * (1.0f - cosThetaI * cosThetaI);
         if (k >= 0 && m.roughness > 0.0f) {
-            (void)rng.next_float();
-            (void)rng.next_float();
+            rng.discard();
+            rng.discard();
         }
         return;
     }
-    (void)rng.next_float();
-    (void)rng.next_float();
+    rng.discard();
+    rng.discard();
+}
+
+__device__ void skip_bounce_draws(const SpecTab &tab, const Clo &c, V3 n, bool isInside, V3 rayDir, Mt64 &rng) {
+    skip_bounce_draws_inl(tab, c, n, isInside, rayDir, rng);
 }
 
 // BeerAbsorption, shading_helpers.h:133-145
@@ -572,7 +588,9 @@ __device__ __forceinline__ int lower_bound_f(const float *a, int n, float v) {
 }
 
 // ILight::SampleLight for each light type. Returns false for std::nullopt.
-__device__ bool sample_light(const DeviceScene &s, const GLight &L, V3 hitPos, Mt64 &rng, LSample &ls) {
+template <class R>
+__device__ __forceinline__ bool sample_light_inl(const DeviceScene &s, const GLight &L, V3 hitPos, R &rng,
+                                                 LSample &ls) {
     switch (L.type) {
         case 2: {  // RectLight, rect_light.cpp:14-37
             const float u = rng.next_float() - 0.5f;
@@ -662,6 +680,10 @@ __device__ bool sample_light(const DeviceScene &s, const GLight &L, V3 hitPos, M
         }
     }
     return false;
+}
+
+__device__ bool sample_light(const DeviceScene &s, const GLight &L, V3 hitPos, Mt64 &rng, LSample &ls) {
+    return sample_light_inl(s, L, hitPos, rng, ls);
 }
 
 // Number of RNG draws SampleLight consumes for a light type (rect / dome: 2; point / distant: 0).

@@ -79,6 +79,7 @@ def lib():
             "hdr_view_color_device_ptr": [P, C.POINTER(P)],
             "hdr_view_synchronize": [P],
             "hdr_view_last_timing": [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)],
+            "hdr_view_read_stage_times": [P, P, P],
             "hdr_scene_intersect": [P, P, P, I, P, P, P],
             "hdr_rng_raw": [P, P, I, I, P],
             "hdr_rng_float": [P, P, I, I, P],
@@ -192,6 +193,16 @@ class Renderer:
         g, a, b = C.c_float(), C.c_float(), C.c_float()
         check(lib().hdr_view_last_timing(self.h, C.byref(g), C.byref(a), C.byref(b)))
         return float(g.value), float(a.value), float(b.value)
+
+    STAGES = ("gbuffer", "integrate", "shadow", "accumulate", "firefly")
+
+    def stage_times(self):
+        """{stage: (summed device ms, launches)} over the frames since the previous call (HIP events
+        on the view's stream around each kernel); clears the sums."""
+        ms = np.zeros(len(self.STAGES), np.float64)
+        n = np.zeros(len(self.STAGES), np.int32)
+        check(lib().hdr_view_read_stage_times(self.h, _p(ms), _p(n)))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.STAGES)}
 
     def read_color(self):
         check(lib().hdr_view_read_color(self.h, _p(self.color)))

@@ -451,11 +451,12 @@ def _dome_texels(w, h, rng):
     return (tex * grad * np.array([0.6, 0.7, 1.0], np.float32)).astype(np.float32)
 
 
-def features(width=48, height=48, seed=3, frames=4, dome=True, dof=False, emissive=True):
+def features(width=48, height=48, seed=3, frames=4, dome=True, dof=False, emissive=True, opaque=False):
     """Every shading feature of the path in one small scene: coat / sheen / transmission with Beer
     absorption / partial opacity / emission / DefaultMaterial with vertex colours; rect, cone-shaped
     point, distant and dome lights (the dome is the environment); an emissive sphere registered as a
-    mesh light."""
+    mesh light. dome / emissive / opaque=True select the variant whose BSDF rays reach no emitter
+    and whose materials are all opaque (the deferred-shadow path at depth 1 with skipVisibility)."""
     rng = np.random.default_rng(seed)
     sc = Scene("features", width, height)
     bp, bi = box_mesh(2.0)
@@ -471,7 +472,7 @@ def features(width=48, height=48, seed=3, frames=4, dome=True, dof=False, emissi
         dict(diffuse_color=(0.8, 0.2, 0.2), roughness=0.4, coat=0.8, coat_roughness=0.2, sheen=0.5),
         dict(diffuse_color=(0.9, 0.9, 0.9), roughness=0.3, transmission=0.9, transmission_depth=0.5,
              transmission_color=(0.5, 0.8, 0.9), ior=1.45),
-        dict(diffuse_color=(0.2, 0.7, 0.3), roughness=0.6, opacity=0.5),
+        dict(diffuse_color=(0.2, 0.7, 0.3), roughness=0.6, opacity=1.0 if opaque else 0.5),
         dict(diffuse_color=(0.9, 0.8, 0.3), roughness=0.2, metallic=1.0),
         dict(diffuse_color=(0.1, 0.1, 0.1), roughness=0.9, emission=4.0 if emissive else 0.0,
              emission_color=(1.0, 0.6, 0.3)),

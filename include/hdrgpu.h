@@ -283,6 +283,21 @@ hdr_status hdr_view_synchronize(hdr_view *view);
  * and the whole frame (G-buffer through accumulation). */
 hdr_status hdr_view_last_timing(hdr_view *view, float *gbuffer_ms, float *integrate_ms, float *frame_ms);
 
+/* Kernel stages of one frame, timed by HIP events on the view's stream. */
+enum {
+    HDR_STAGE_GBUFFER = 0,    /* primary rays -> G-buffer (RaycastPass)                           */
+    HDR_STAGE_INTEGRATE = 1,  /* integration megakernel, or the deferred path's shading kernel    */
+    HDR_STAGE_SHADOW = 2,     /* deferred path: any-hit shadow rays of the winners               */
+    HDR_STAGE_ACCUMULATE = 3, /* deferred path: visibility pick + accumulation                   */
+    HDR_STAGE_FIREFLY = 4,    /* firefly filter + accumulation (enable_firefly_filter)           */
+    HDR_STAGE_COUNT = 5
+};
+/* Per-stage device time (ms) summed over the frames rendered since the previous call, and the
+ * number of launches of each stage; clears the sums. Timing is read back lazily, so frames are
+ * never serialised by it. Plays the role of the reference's per-pass timing metrics
+ * (core/metrics_on_buffers.h, RESTIR_BUFFER_METRIC_SCOPE). */
+hdr_status hdr_view_read_stage_times(hdr_view *view, double *stage_ms, int32_t *launches);
+
 /* ---- parity / measurement entry points ---- */
 
 /* Closest hit for n world-space rays (reference: Scene::IntersectScene, scene.cpp:515-535).
