@@ -4,6 +4,8 @@
 //   nodes      GNode[ ]   32 B: BLAS nodes of every mesh, then TLAS nodes. Internal node:
 //                         left = index of the left child (right = left + 1); leaf:
 //                         left = -(first triangle) - 1, count = triangles (<= 4) or instances (<= 2).
+//                         Every tree starts at an odd index (a padding record precedes it when
+//                         needed), so each sibling pair sits in one 64-byte aligned block.
 //   tris       GTri[ ]    48 B: v0, v1, v2 (+ original triangle index, mesh-local) -- the
 //                         36 bytes the Moller-Trumbore test reads, one 16-B aligned record.
 //   attrs      GTriAttr[] 72 B: shading normals and vertex colours, read once per ray for the
@@ -117,38 +119,8 @@ struct TraceStats {
     uint32_t nodes, tris, instances;
 };
 
-// IntersectAABB (bvh.cpp:32-64 with double thresholds; scene.cpp:13-46 with float thresholds).
-template <bool kFloatThresholds>
-__device__ __forceinline__ bool aabb_hit(const GNode &n, V3 o, V3 d, float &tmin_out) {
-    if (n.mnx > n.mxx || n.mny > n.mxy || n.mnz > n.mxz) return false;  // GfRange3f::IsEmpty
-    float tmin = -1e30f, tmax = 1e30f;
-    const float mn[3] = {n.mnx, n.mny, n.mnz};
-    const float mx[3] = {n.mxx, n.mxy, n.mxz};
-    const float oo[3] = {o.x, o.y, o.z};
-    const float dd[3] = {d.x, d.y, d.z};
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const bool parallel = kFloatThresholds ? (fabsf(dd[i]) < 1e-8f) : (double(fabsf(dd[i])) < 1e-8);
-        if (parallel) {
-            if (oo[i] < mn[i] || oo[i] > mx[i]) return false;
-        } else {
-            const float inv = 1.0f / dd[i];
-            float t1 = (mn[i] - oo[i]) * inv;
-            float t2 = (mx[i] - oo[i]) * inv;
-            if (t1 > t2) {
-                const float s = t1;
-                t1 = t2;
-                t2 = s;
-            }
-            tmin = smax(tmin, t1);
-            tmax = smin(tmax, t2);
-            if (tmin > tmax) return false;
-        }
-    }
-    tmin_out = tmin;
-    if (kFloatThresholds) return tmax > 0.0f && tmax > 1e-4f;
-    return tmax > 0.0f && double(tmax) > 1e-4;
-}
+// IntersectAABB (bvh.cpp:32-64 with double thresholds; scene.cpp:13-46 with float thresholds)
+// is box_hit below, with the per-ray reciprocals hoisted.
 
 // IntersectTriangle, bvh.cpp:8-30 (Moller-Trumbore; reject |det| < 1e-8, accept t > 1e-4).
 __device__ __forceinline__ bool tri_hit(const GTri &t, V3 o, V3 d, float &tt, float &uu, float &vv) {
@@ -195,132 +167,266 @@ struct RawHit {
 
 constexpr int kStackDepth = 64;
 
-// Scene::Intersect (scene.cpp:409-513) + BVH::_IntersectNode (bvh.cpp:251-305) as two explicit
-// stacks: TLAS children pushed right-then-left, BLAS left subtree fully before the right one,
-// every node's box tested against the best t at the time it is reached. Finds the reference's
-// winner: strict t < best, first found among equal t.
+// Per-ray constants of IntersectAABB: origin, reciprocal direction and parallel-axis flags,
+// computed once with the same float operations the per-node test performs, so every box test
+// sees the values the reference computes per node.
+struct RayBox {
+    float o[3], inv[3];
+    bool par[3];
+};
+template <bool kFloatThresholds>
+__device__ __forceinline__ RayBox make_raybox(V3 o, V3 d) {
+    RayBox r;
+    const float oo[3] = {o.x, o.y, o.z};
+    const float dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        r.o[i] = oo[i];
+        r.par[i] = kFloatThresholds ? (fabsf(dd[i]) < 1e-8f) : (double(fabsf(dd[i])) < 1e-8);
+        r.inv[i] = r.par[i] ? 0.0f : 1.0f / dd[i];
+    }
+    return r;
+}
+template <bool kFloatThresholds>
+__device__ __forceinline__ bool box_hit(const RayBox &r, const GNode &n, float &tmin_out) {
+    if (n.mnx > n.mxx || n.mny > n.mxy || n.mnz > n.mxz) return false;  // GfRange3f::IsEmpty
+    float tmin = -1e30f, tmax = 1e30f;
+    const float mn[3] = {n.mnx, n.mny, n.mnz};
+    const float mx[3] = {n.mxx, n.mxy, n.mxz};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (r.par[i]) {
+            if (r.o[i] < mn[i] || r.o[i] > mx[i]) return false;
+        } else {
+            float t1 = (mn[i] - r.o[i]) * r.inv[i];
+            float t2 = (mx[i] - r.o[i]) * r.inv[i];
+            if (t1 > t2) {
+                const float sw = t1;
+                t1 = t2;
+                t2 = sw;
+            }
+            tmin = smax(tmin, t1);
+            tmax = smin(tmax, t2);
+            if (tmin > tmax) return false;
+        }
+    }
+    tmin_out = tmin;
+    if (kFloatThresholds) return tmax > 0.0f && tmax > 1e-4f;
+    return tmax > 0.0f && double(tmax) > 1e-4;
+}
+
+// Stack / cursor form of a node: inner -> index of its left child (the right one follows it);
+// leaf -> -((first << 3) | count) - 1 (count <= 4 triangles or <= 2 instances).
+__device__ __forceinline__ int node_ref(const GNode &n) {
+    return n.left >= 0 ? n.left : -((((-n.left) - 1) << 3) | n.count) - 1;
+}
+
+// Scene::Intersect (scene.cpp:409-513) + BVH::_IntersectNode (bvh.cpp:251-305): left subtree
+// fully before the right one, every box tested against the best t at the time the reference
+// reaches it; strict t < best, first found among equal t -- so the reference's winner. Both
+// children of an inner node are tested together (their records are adjacent: one 64-byte
+// fetch); the left child is entered at once, the right one waits on the stack with its entry
+// distance and is pruned against the best t when popped, which is where and how the reference
+// tests it.
 template <bool kCount>
 __device__ __forceinline__ bool trace_closest(const DeviceScene &s, V3 o, V3 d, RawHit &hit, TraceStats &st) {
     hit.t = 1e30f;
     hit.inst = -1;
     if (s.tlas_root < 0) return false;
-    int tstack[kStackDepth];
-    int bstack[kStackDepth];
+    const RayBox rt = make_raybox<true>(o, d);
+    int tref[kStackDepth];
+    float tta[kStackDepth];
+    int bref[kStackDepth];
+    float bta[kStackDepth];
     int tsp = 0;
-    tstack[tsp++] = s.tlas_root;
     bool was = false;
-    while (tsp > 0) {
-        const int ni = tstack[--tsp];
-        const GNode n = load_node(s.nodes, ni);
+    int cur;
+    {
+        const GNode n = load_node(s.nodes, s.tlas_root);
         if (kCount) ++st.nodes;
-        float tAabb = 0.0f;
-        if (!aabb_hit<true>(n, o, d, tAabb) || tAabb > hit.t) continue;
-        if (n.left >= 0) {
-            tstack[tsp++] = n.left + 1;
-            tstack[tsp++] = n.left;
-            continue;
-        }
-        const int start = -n.left - 1;
-        for (int k = 0; k < n.count; ++k) {
-            const int ii = s.tlas_idx[start + k];
-            const GInstance &inst = s.instances[ii];
-            if (kCount) ++st.instances;
-            if (inst.root < 0) continue;
-            const V3 oo = xform_point(inst.inv, o);
-            const V3 od = xform_dir(inst.inv, d);
-            float instT = hit.t;
-            int bestTri = -1;
-            float bu = 0.f, bv = 0.f;
-            int bsp = 0;
-            bstack[bsp++] = inst.root;
-            while (bsp > 0) {
-                const int bi = bstack[--bsp];
-                const GNode b = load_node(s.nodes, bi);
-                if (kCount) ++st.nodes;
-                float ta;
-                if (!aabb_hit<false>(b, oo, od, ta)) continue;
-                if (ta > instT) continue;
-                if (b.left >= 0) {
-                    bstack[bsp++] = b.left + 1;
-                    bstack[bsp++] = b.left;
-                    continue;
+        float ta;
+        if (!box_hit<true>(rt, n, ta) || ta > hit.t) return false;
+        cur = node_ref(n);
+    }
+    for (;;) {
+        if (cur >= 0) {
+            const GNode l = load_node(s.nodes, cur), r = load_node(s.nodes, cur + 1);
+            if (kCount) st.nodes += 2;
+            float tl, tr;
+            const bool hl = box_hit<true>(rt, l, tl) && !(tl > hit.t);
+            if (box_hit<true>(rt, r, tr)) {
+                tref[tsp] = node_ref(r);
+                tta[tsp] = tr;
+                ++tsp;
+            }
+            if (hl) {
+                cur = node_ref(l);
+                continue;
+            }
+        } else {
+            const int code = -cur - 1;
+            const int start = code >> 3, count = code & 7;
+            for (int k = 0; k < count; ++k) {
+                const int ii = s.tlas_idx[start + k];
+                const GInstance &inst = s.instances[ii];
+                if (kCount) ++st.instances;
+                if (inst.root < 0) continue;
+                const V3 oo = xform_point(inst.inv, o);
+                const V3 od = xform_dir(inst.inv, d);
+                const RayBox rb = make_raybox<false>(oo, od);
+                float instT = hit.t;
+                int bestTri = -1;
+                float bu = 0.f, bv = 0.f;
+                int bsp = 0;
+                int bcur;
+                bool go;
+                {
+                    const GNode b = load_node(s.nodes, inst.root);
+                    if (kCount) ++st.nodes;
+                    float ta;
+                    go = box_hit<false>(rb, b, ta) && !(ta > instT);
+                    bcur = node_ref(b);
                 }
-                const int ts = -b.left - 1;
-                for (int q = 0; q < b.count; ++q) {
-                    const GTri tr = s.tris[ts + q];
-                    if (kCount) ++st.tris;
-                    float tt, uu, vv;
-                    if (tri_hit(tr, oo, od, tt, uu, vv) && tt < instT) {
-                        instT = tt;
-                        bestTri = ts + q;
-                        bu = uu;
-                        bv = vv;
+                while (go) {
+                    if (bcur >= 0) {
+                        const GNode l = load_node(s.nodes, bcur), r = load_node(s.nodes, bcur + 1);
+                        if (kCount) st.nodes += 2;
+                        float tl, tr;
+                        const bool hl = box_hit<false>(rb, l, tl) && !(tl > instT);
+                        if (box_hit<false>(rb, r, tr)) {
+                            bref[bsp] = node_ref(r);
+                            bta[bsp] = tr;
+                            ++bsp;
+                        }
+                        if (hl) {
+                            bcur = node_ref(l);
+                            continue;
+                        }
+                    } else {
+                        const int bcode = -bcur - 1;
+                        const int ts = bcode >> 3, tc = bcode & 7;
+                        for (int q = 0; q < tc; ++q) {
+                            const GTri tr = s.tris[ts + q];
+                            if (kCount) ++st.tris;
+                            float tt, uu, vv;
+                            if (tri_hit(tr, oo, od, tt, uu, vv) && tt < instT) {
+                                instT = tt;
+                                bestTri = ts + q;
+                                bu = uu;
+                                bv = vv;
+                            }
+                        }
+                    }
+                    go = false;
+                    while (bsp > 0) {
+                        --bsp;
+                        if (bta[bsp] > instT) continue;
+                        bcur = bref[bsp];
+                        go = true;
+                        break;
                     }
                 }
+                if (bestTri < 0 || instT >= hit.t) continue;
+                hit.t = instT;
+                hit.inst = ii;
+                hit.tri = bestTri;
+                hit.u = bu;
+                hit.v = bv;
+                was = true;
             }
-            if (bestTri < 0 || instT >= hit.t) continue;
-            hit.t = instT;
-            hit.inst = ii;
-            hit.tri = bestTri;
-            hit.u = bu;
-            hit.v = bv;
-            was = true;
         }
+        bool more = false;
+        while (tsp > 0) {
+            --tsp;
+            if (tta[tsp] > hit.t) continue;
+            cur = tref[tsp];
+            more = true;
+            break;
+        }
+        if (!more) break;
     }
     return was;
 }
 
 // Shadow query: is there a hit with 1e-4 < t < tmax? The reference fires a closest-hit ray and
 // tests Depth < Dist - 1e-3 (ris_direct_light_integrator.cpp:513-519, mis_direct_light_integrator.cpp:100-105);
-// the same traversal started at t = tmax and stopped at the first accepted triangle answers the
-// same question (a box entered before the closest hit cannot be pruned by a larger bound).
+// the same traversal with the bound fixed at tmax, stopped at the first accepted triangle,
+// answers the same question (a box entered before the closest hit cannot be pruned by a
+// larger bound), so subtrees are pruned as soon as their box is tested.
 template <bool kCount>
 __device__ __forceinline__ bool trace_any(const DeviceScene &s, V3 o, V3 d, float tmax, TraceStats &st) {
     if (s.tlas_root < 0) return false;
-    int tstack[kStackDepth];
-    int bstack[kStackDepth];
+    const RayBox rt = make_raybox<true>(o, d);
+    int tref[kStackDepth];
+    int bref[kStackDepth];
     int tsp = 0;
-    tstack[tsp++] = s.tlas_root;
-    while (tsp > 0) {
-        const int ni = tstack[--tsp];
-        const GNode n = load_node(s.nodes, ni);
+    int cur;
+    {
+        const GNode n = load_node(s.nodes, s.tlas_root);
         if (kCount) ++st.nodes;
-        float tAabb = 0.0f;
-        if (!aabb_hit<true>(n, o, d, tAabb) || tAabb > tmax) continue;
-        if (n.left >= 0) {
-            tstack[tsp++] = n.left + 1;
-            tstack[tsp++] = n.left;
-            continue;
-        }
-        const int start = -n.left - 1;
-        for (int k = 0; k < n.count; ++k) {
-            const GInstance &inst = s.instances[s.tlas_idx[start + k]];
-            if (kCount) ++st.instances;
-            if (inst.root < 0) continue;
-            const V3 oo = xform_point(inst.inv, o);
-            const V3 od = xform_dir(inst.inv, d);
-            int bsp = 0;
-            bstack[bsp++] = inst.root;
-            while (bsp > 0) {
-                const int bi = bstack[--bsp];
-                const GNode b = load_node(s.nodes, bi);
-                if (kCount) ++st.nodes;
-                float ta;
-                if (!aabb_hit<false>(b, oo, od, ta) || ta > tmax) continue;
-                if (b.left >= 0) {
-                    bstack[bsp++] = b.left + 1;
-                    bstack[bsp++] = b.left;
-                    continue;
+        float ta;
+        if (!box_hit<true>(rt, n, ta) || ta > tmax) return false;
+        cur = node_ref(n);
+    }
+    for (;;) {
+        if (cur >= 0) {
+            const GNode l = load_node(s.nodes, cur), r = load_node(s.nodes, cur + 1);
+            if (kCount) st.nodes += 2;
+            float tl, tr;
+            const bool hl = box_hit<true>(rt, l, tl) && !(tl > tmax);
+            if (box_hit<true>(rt, r, tr) && !(tr > tmax)) tref[tsp++] = node_ref(r);
+            if (hl) {
+                cur = node_ref(l);
+                continue;
+            }
+        } else {
+            const int code = -cur - 1;
+            const int start = code >> 3, count = code & 7;
+            for (int k = 0; k < count; ++k) {
+                const GInstance &inst = s.instances[s.tlas_idx[start + k]];
+                if (kCount) ++st.instances;
+                if (inst.root < 0) continue;
+                const V3 oo = xform_point(inst.inv, o);
+                const V3 od = xform_dir(inst.inv, d);
+                const RayBox rb = make_raybox<false>(oo, od);
+                int bsp = 0;
+                int bcur;
+                bool go;
+                {
+                    const GNode b = load_node(s.nodes, inst.root);
+                    if (kCount) ++st.nodes;
+                    float ta;
+                    go = box_hit<false>(rb, b, ta) && !(ta > tmax);
+                    bcur = node_ref(b);
                 }
-                const int ts = -b.left - 1;
-                for (int q = 0; q < b.count; ++q) {
-                    const GTri tr = s.tris[ts + q];
-                    if (kCount) ++st.tris;
-                    float tt, uu, vv;
-                    if (tri_hit(tr, oo, od, tt, uu, vv) && tt < tmax) return true;
+                while (go) {
+                    if (bcur >= 0) {
+                        const GNode l = load_node(s.nodes, bcur), r = load_node(s.nodes, bcur + 1);
+                        if (kCount) st.nodes += 2;
+                        float tl, tr;
+                        const bool hl = box_hit<false>(rb, l, tl) && !(tl > tmax);
+                        if (box_hit<false>(rb, r, tr) && !(tr > tmax)) bref[bsp++] = node_ref(r);
+                        if (hl) {
+                            bcur = node_ref(l);
+                            continue;
+                        }
+                    } else {
+                        const int bcode = -bcur - 1;
+                        const int ts = bcode >> 3, tc = bcode & 7;
+                        for (int q = 0; q < tc; ++q) {
+                            const GTri tr = s.tris[ts + q];
+                            if (kCount) ++st.tris;
+                            float tt, uu, vv;
+                            if (tri_hit(tr, oo, od, tt, uu, vv) && tt < tmax) return true;
+                        }
+                    }
+                    go = bsp > 0;
+                    if (go) bcur = bref[--bsp];
                 }
             }
         }
+        if (tsp == 0) break;
+        cur = tref[--tsp];
     }
     return false;
 }

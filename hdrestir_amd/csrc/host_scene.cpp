@@ -60,6 +60,18 @@ GNode make_node(const Box &b, int left, int count) {
     return n;
 }
 
+// Trees are appended at odd indices: a tree's sibling pairs sit at (odd, even) local offsets
+// 1-2, 3-4, ..., so with an odd base every pair starts on a 64-byte boundary of the 256-byte
+// aligned node array and both children arrive in one fetch.
+void pad_to_odd(std::vector<GNode> &nodes) {
+    if (nodes.size() % 2 == 0) {
+        GNode pad{};
+        pad.mnx = pad.mny = pad.mnz = 1.0f;  // empty box; the record is never referenced
+        pad.mxx = pad.mxy = pad.mxz = -1.0f;
+        nodes.push_back(pad);
+    }
+}
+
 struct BuildTri {
     F3 v0, v1, v2, centroid;
     int orig;
@@ -244,6 +256,7 @@ std::string build_host_scene(const hdr_scene_desc &d, HostScene &out) {
         bb.nodes.push_back(GNode{});
         bb.subdivide(0, 0, int(bb.tris.size()), 1);
         out.max_blas_depth = std::max(out.max_blas_depth, bb.depth);
+        pad_to_odd(out.nodes);
         const int nodeBase = int(out.nodes.size());
         const int triBase = int(out.tris.size());
         for (GNode n : bb.nodes) {
@@ -406,6 +419,7 @@ std::string build_host_scene(const hdr_scene_desc &d, HostScene &out) {
         };
         Rec::go(tl, idx, instBounds, instCentroid, 0, 0, d.num_instances, 1, tdepth);
         out.max_tlas_depth = tdepth;
+        pad_to_odd(out.nodes);
         const int base = int(out.nodes.size());
         for (GNode n : tl) {
             if (n.left >= 0 && n.count == 0) n.left += base;
@@ -416,6 +430,8 @@ std::string build_host_scene(const hdr_scene_desc &d, HostScene &out) {
     }
     if (out.max_blas_depth > kStackDepth - 2 || out.max_tlas_depth > kStackDepth - 2)
         return "BVH deeper than the traversal stack";
+    if (out.tris.size() >= (size_t(1) << 28) || out.tlas_idx.size() >= (size_t(1) << 28))
+        return "more than 2^28 triangles or instances (leaf reference range of the traversal)";
 
     // ---- lights (per-sample constants of SampleLight)
     for (int i = 0; i < d.num_lights; ++i) {
