@@ -462,11 +462,20 @@ struct hdr_view {
     DevBuf res, accum, color, frame, dbg, depth, albedo, normal, counters, spill, gbuf, rgb2, flags, queue, qcount;
     int counting = 0;
     int frames = 0;
+    // Frames overlap: the G-buffer kernel of frame N+1 runs on `gstream` as soon as frame N's
+    // shading has read the G-buffer (event `shade_done`), beside frame N's shadow rays and
+    // accumulation on `stream`; frame N+1's shading waits for its G-buffer (event `gbuf_done`).
+    hipStream_t gstream = nullptr;
+    hipEvent_t shade_done = nullptr, gbuf_done = nullptr, serial = nullptr;
+    bool shade_recorded = false;
     // Per-kernel timing: one slot of events per frame in a ring, read back lazily (when the slot is
-    // reused or the sums are requested), so timing never stalls the stream.
-    // ev[k] .. ev[k+1] brackets stage k (HDR_STAGE_*); bit k of `ran` says stage k launched.
+    // reused or the sums are requested), so timing never stalls the streams.
+    // beg[k] .. end[k] brackets stage k (HDR_STAGE_*) on the stream it runs on; bit k of `ran`
+    // says stage k launched; fin marks the end of the frame.
     struct StageSlot {
-        hipEvent_t ev[HDR_STAGE_COUNT + 1] = {};
+        hipEvent_t beg[HDR_STAGE_COUNT] = {};
+        hipEvent_t end[HDR_STAGE_COUNT] = {};
+        hipEvent_t fin = nullptr;
         uint32_t ran = 0;
         bool pending = false;
     };
@@ -479,11 +488,11 @@ struct hdr_view {
 
 static hdr_status harvest(hdr_view *v, hdr_view::StageSlot &s) {
     if (!s.pending) return HDR_OK;
-    HIP_TRY(hipEventSynchronize(s.ev[HDR_STAGE_COUNT]));
+    HIP_TRY(hipEventSynchronize(s.fin));
     for (int k = 0; k < HDR_STAGE_COUNT; ++k) {
         if (!(s.ran & (1u << k))) continue;
         float ms = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&ms, s.ev[k], s.ev[k + 1]));
+        HIP_TRY(hipEventElapsedTime(&ms, s.beg[k], s.end[k]));
         v->stage_ms[k] += ms;
         v->stage_n[k] += 1;
     }
@@ -634,8 +643,15 @@ hdr_status hdr_view_create(hdr_context *ctx, int32_t width, int32_t height, hdr_
     HIP_TRY(v->gbuf.alloc(P * sizeof(GRec)));
     HIP_TRY(v->qcount.alloc(sizeof(unsigned int)));
     HIP_TRY(hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking));
-    for (auto &slot : v->ring)
-        for (auto &e : slot.ev) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipStreamCreateWithFlags(&v->gstream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&v->shade_done, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&v->gbuf_done, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&v->serial, hipEventDisableTiming));
+    for (auto &slot : v->ring) {
+        for (auto &e : slot.beg) HIP_TRY(hipEventCreate(&e));
+        for (auto &e : slot.end) HIP_TRY(hipEventCreate(&e));
+        HIP_TRY(hipEventCreate(&slot.fin));
+    }
     HIP_TRY(hipMemsetAsync(v->res.p, 0, v->res.n, v->stream));
     HIP_TRY(hipMemsetAsync(v->accum.p, 0, v->accum.n, v->stream));
     HIP_TRY(hipMemsetAsync(v->color.p, 0, v->color.n, v->stream));
@@ -649,9 +665,17 @@ hdr_status hdr_view_destroy(hdr_view *view) {
     if (!view) return HDR_OK;
     (void)hipSetDevice(view->ctx->device);
     if (view->stream) (void)hipStreamSynchronize(view->stream);
-    for (auto &slot : view->ring)
-        for (auto &e : slot.ev)
+    if (view->gstream) (void)hipStreamSynchronize(view->gstream);
+    for (auto &slot : view->ring) {
+        for (auto &e : slot.beg)
             if (e) (void)hipEventDestroy(e);
+        for (auto &e : slot.end)
+            if (e) (void)hipEventDestroy(e);
+        if (slot.fin) (void)hipEventDestroy(slot.fin);
+    }
+    for (hipEvent_t e : {view->shade_done, view->gbuf_done, view->serial})
+        if (e) (void)hipEventDestroy(e);
+    if (view->gstream) (void)hipStreamDestroy(view->gstream);
     if (view->stream) (void)hipStreamDestroy(view->stream);
     delete view;
     return HDR_OK;
@@ -803,12 +827,23 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     hdr_view::StageSlot &ts = view->ring[view->ring_next];
     if (hdr_status st = harvest(view, ts)) return st;
     ts.ran = 0;
-    HIP_TRY(hipEventRecord(ts.ev[HDR_STAGE_GBUFFER], view->stream));
-    if (view->counting) hipLaunchKernelGGL((gbuffer_kernel<true>), grid, dim3(kLanes), 0, view->stream, a, gb);
-    else hipLaunchKernelGGL((gbuffer_kernel<false>), grid, dim3(kLanes), 0, view->stream, a, gb);
+    // G-buffer on the second stream: after the previous frame's shading has read the G-buffer, or,
+    // when counting, after everything queued so far (counters are per frame).
+    if (view->counting) {
+        HIP_TRY(hipEventRecord(view->serial, view->stream));
+        HIP_TRY(hipStreamWaitEvent(view->gstream, view->serial, 0));
+    } else if (view->shade_recorded) {
+        HIP_TRY(hipStreamWaitEvent(view->gstream, view->shade_done, 0));
+    }
+    HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_GBUFFER], view->gstream));
+    if (view->counting) hipLaunchKernelGGL((gbuffer_kernel<true>), grid, dim3(kLanes), 0, view->gstream, a, gb);
+    else hipLaunchKernelGGL((gbuffer_kernel<false>), grid, dim3(kLanes), 0, view->gstream, a, gb);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_GBUFFER], view->gstream));
+    HIP_TRY(hipEventRecord(view->gbuf_done, view->gstream));
     ts.ran |= 1u << HDR_STAGE_GBUFFER;
-    HIP_TRY(hipEventRecord(ts.ev[HDR_STAGE_INTEGRATE], view->stream));
+    HIP_TRY(hipStreamWaitEvent(view->stream, view->gbuf_done, 0));
+    HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_INTEGRATE], view->stream));
     if (deferred) {
         float4 *rgb2 = static_cast<float4 *>(view->rgb2.p);
         uint32_t *flags = static_cast<uint32_t *>(view->flags.p);
@@ -826,20 +861,25 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         }
 #undef HDR_SHADE
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(ts.ev[HDR_STAGE_SHADOW], view->stream));
+        HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_INTEGRATE], view->stream));
+        HIP_TRY(hipEventRecord(view->shade_done, view->stream));
+        view->shade_recorded = true;
+        HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_SHADOW], view->stream));
         const int sblocks = int(std::min<size_t>((P + kLanes - 1) / kLanes, 8192));
         if (view->counting)
             hipLaunchKernelGGL((shadow_kernel<true>), dim3(sblocks), dim3(kLanes), 0, view->stream, a, queue, qc, flags);
         else
             hipLaunchKernelGGL((shadow_kernel<false>), dim3(sblocks), dim3(kLanes), 0, view->stream, a, queue, qc, flags);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(ts.ev[HDR_STAGE_ACCUMULATE], view->stream));
+        HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_SHADOW], view->stream));
+        HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_ACCUMULATE], view->stream));
         const int ablocks = int((P + 255) / 256);
         if (debug_out)
             hipLaunchKernelGGL((accumulate_deferred_kernel<true>), dim3(ablocks), dim3(256), 0, view->stream, a, rgb2, flags);
         else
             hipLaunchKernelGGL((accumulate_deferred_kernel<false>), dim3(ablocks), dim3(256), 0, view->stream, a, rgb2, flags);
         HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_ACCUMULATE], view->stream));
         ts.ran |= (1u << HDR_STAGE_INTEGRATE) | (1u << HDR_STAGE_SHADOW) | (1u << HDR_STAGE_ACCUMULATE);
     } else if (view->counting) {
         if (debug_out) hipLaunchKernelGGL((integrate_kernel<true, true>), grid, dim3(kLanes), lds, view->stream, a, gb);
@@ -851,18 +891,20 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     HIP_TRY(hipGetLastError());
     if (!deferred) {
         ts.ran |= 1u << HDR_STAGE_INTEGRATE;
-        HIP_TRY(hipEventRecord(ts.ev[HDR_STAGE_SHADOW], view->stream));
-        HIP_TRY(hipEventRecord(ts.ev[HDR_STAGE_ACCUMULATE], view->stream));
+        HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_INTEGRATE], view->stream));
+        HIP_TRY(hipEventRecord(view->shade_done, view->stream));
+        view->shade_recorded = true;
     }
-    HIP_TRY(hipEventRecord(ts.ev[HDR_STAGE_FIREFLY], view->stream));
     if (a.firefly) {
+        HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_FIREFLY], view->stream));
         const dim3 b2(16, 16), g2((W + 15) / 16, (H + 15) / 16);
         hipLaunchKernelGGL(firefly_accumulate_kernel, g2, b2, 0, view->stream, a.frame_rgb, a.accum, a.color, W, H,
                            a.inv_n);
         HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_FIREFLY], view->stream));
         ts.ran |= 1u << HDR_STAGE_FIREFLY;
     }
-    HIP_TRY(hipEventRecord(ts.ev[HDR_STAGE_COUNT], view->stream));
+    HIP_TRY(hipEventRecord(ts.fin, view->stream));
     ts.pending = true;
     view->ring_last = view->ring_next;
     view->ring_next = (view->ring_next + 1) % hdr_view::kRing;
@@ -882,6 +924,7 @@ hdr_status hdr_view_synchronize(hdr_view *view) {
     if (!view) return fail(HDR_ERR_INVALID_ARGUMENT, "view is null");
     HIP_TRY(hipSetDevice(view->ctx->device));
     HIP_TRY(hipStreamSynchronize(view->stream));
+    HIP_TRY(hipStreamSynchronize(view->gstream));
     return HDR_OK;
 }
 
@@ -891,10 +934,11 @@ hdr_status hdr_view_last_timing(hdr_view *view, float *gbuffer_ms, float *integr
     if (view->ring_last >= 0) {
         HIP_TRY(hipSetDevice(view->ctx->device));
         hdr_view::StageSlot &s = view->ring[view->ring_last];
-        HIP_TRY(hipEventSynchronize(s.ev[HDR_STAGE_COUNT]));
-        HIP_TRY(hipEventElapsedTime(&g, s.ev[HDR_STAGE_GBUFFER], s.ev[HDR_STAGE_INTEGRATE]));
-        HIP_TRY(hipEventElapsedTime(&i, s.ev[HDR_STAGE_INTEGRATE], s.ev[HDR_STAGE_FIREFLY]));
-        HIP_TRY(hipEventElapsedTime(&f, s.ev[HDR_STAGE_GBUFFER], s.ev[HDR_STAGE_COUNT]));
+        HIP_TRY(hipEventSynchronize(s.fin));
+        const int last = (s.ran & (1u << HDR_STAGE_ACCUMULATE)) ? HDR_STAGE_ACCUMULATE : HDR_STAGE_INTEGRATE;
+        HIP_TRY(hipEventElapsedTime(&g, s.beg[HDR_STAGE_GBUFFER], s.end[HDR_STAGE_GBUFFER]));
+        HIP_TRY(hipEventElapsedTime(&i, s.beg[HDR_STAGE_INTEGRATE], s.end[last]));
+        HIP_TRY(hipEventElapsedTime(&f, s.beg[HDR_STAGE_GBUFFER], s.fin));
     }
     if (gbuffer_ms) *gbuffer_ms = g;
     if (integrate_ms) *integrate_ms = i;
