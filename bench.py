@@ -34,7 +34,8 @@ RGB2_BYTES = 32    # deferred path: visible + occluded colour written by shading
 FLAG_BYTES = 4     # deferred path: per pixel visibility flag
 QUERY_BYTES = 32   # deferred path: one shadow query (origin, tmax, direction, pixel)
 KERNEL_NAMES = {"gbuffer": "gbuffer_kernel", "integrate": "shade_direct_kernel", "shadow": "shadow_kernel",
-                "accumulate": "accumulate_deferred_kernel", "firefly": "firefly_accumulate_kernel"}
+                "accumulate": "accumulate_deferred_kernel", "firefly": "firefly_accumulate_kernel",
+                "spatial": "spatial_kernel"}
 
 
 def _log(msg):

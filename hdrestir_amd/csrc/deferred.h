@@ -17,6 +17,34 @@
 
 namespace hdr {
 
+// G-buffer record of one pixel: 4 x float4 = 64 B, written and read coalesced.
+struct GRec {
+    float4 a;  // pos.xyz, depth
+    float4 b;  // ng.xyz, instance (bits; -1 = miss)
+    float4 c;  // ns.xyz, triangle (bits)
+    float4 d;  // albedo.xyz, unused
+};
+
+__device__ __forceinline__ void store_grec(GRec *dst, bool hit, const HitRec &g) {
+    GRec r;
+    r.a = make_float4(g.pos.x, g.pos.y, g.pos.z, g.depth);
+    r.b = make_float4(g.ng.x, g.ng.y, g.ng.z, __int_as_float(hit ? g.inst : -1));
+    r.c = make_float4(g.ns.x, g.ns.y, g.ns.z, __int_as_float(g.tri));
+    r.d = make_float4(g.albedo.x, g.albedo.y, g.albedo.z, 0.0f);
+    *dst = r;
+}
+__device__ __forceinline__ bool load_grec(const GRec *src, HitRec &g) {
+    const GRec r = *src;
+    g.pos = v3(r.a.x, r.a.y, r.a.z);
+    g.depth = r.a.w;
+    g.ng = v3(r.b.x, r.b.y, r.b.z);
+    g.inst = __float_as_int(r.b.w);
+    g.ns = v3(r.c.x, r.c.y, r.c.z);
+    g.tri = __float_as_int(r.c.w);
+    g.albedo = v3(r.d.x, r.d.y, r.d.z);
+    return g.inst >= 0;
+}
+
 // UniformLightSampler::ProposeCandidate light pick (uniform_light_sampler.cpp:22-47).
 template <bool kDebug>
 __device__ __forceinline__ int pick_light_lean(const FrameArgs &a, Mt64Lean &rng, Trace &dbg, bool record) {
@@ -214,6 +242,10 @@ __device__ __forceinline__ Spec ris_direct_lean(const FrameArgs &a, const SpecTa
                 if (!isfinite(W)) W = 0.0f;
             }
         }
+        if (a.spatial_count > 0) {  // spatial reuse follows: it tests the final winner
+            store_reservoir(&a.res_t[pixel], has, ch, wsum, count, W);
+            return spec(0.0f);
+        }
         if (has && !ch.vis_tested && ch.has_cand) {
             ch.tp = winner_contribution_lean(tab, v, ch, query, qo, qd, qtmax);
             ch.vis_tested = true;
@@ -250,6 +282,258 @@ __device__ __forceinline__ Spec ris_direct_lean(const FrameArgs &a, const SpecTa
     Spec tp = ch.tp;
     if (!ch.vis_tested && ch.has_cand) tp = winner_contribution_lean(tab, v, ch, query, qo, qd, qtmax);
     return tp * W;
+}
+
+// ---------------------------------------------------------------- spatial reuse
+// An extension of this build (the reference reuses temporally only), run after every pixel's
+// temporal pass. Pixel q resamples among its own post-temporal reservoir and those of up to
+// spatial_count neighbours picked within spatial_radius pixels that pass a similarity test
+// (depth within 10 %, geometric normals within 25 degrees). Reservoir j (confidence c_j = its
+// candidate count, capped at 20 frames) offers its sample y_j shifted to q, weighted with the
+// generalized balance heuristic
+//     w_j = c_j W_j p_j(y_j) p_q(y_j) / sum_k c_k p_k(y_j -> k) |J_{q->k}|,
+// whose weights sum to one over the domains, so the combination stays unbiased for any BSDF.
+// The winner gets W = sum w / p_q(y) and a (deferred) shadow ray at q, and is stored for the next
+// frame in the reservoir form of reservoir.h (WSum = W * frames * p_q). Neighbour picks and
+// selections draw from a splitmix64 stream seeded by (pixel, frame): integer-only, so results are
+// identical on every target, and the reference's per-pixel mt19937_64 stream is untouched.
+
+constexpr int kMaxSpatial = 8;
+
+struct SpatialRng {
+    uint64_t s;
+    __device__ __forceinline__ SpatialRng(uint32_t pixel, int32_t frame)
+        : s(uint64_t(pixel) * 0x9E3779B97F4A7C15ULL ^ uint64_t(uint32_t(frame)) * 0xC2B2AE3D27D4EB4FULL) {}
+    __device__ __forceinline__ uint64_t next() {
+        s += 0x9E3779B97F4A7C15ULL;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    }
+    __device__ __forceinline__ float next_float() { return float(next() >> 40) * 0x1p-24f; }
+};
+
+__device__ __forceinline__ bool pick_neighbour(SpatialRng &r, int x, int y, int W, int H, int R, int &nx, int &ny) {
+    const uint64_t D = uint64_t(2 * R + 1);
+    for (int t = 0; t < 8; ++t) {
+        const uint64_t z = r.next();
+        const int dx = int((z & 0xffffULL) % D) - R;
+        const int dy = int(((z >> 16) & 0xffffULL) % D) - R;
+        if ((dx == 0 && dy == 0) || dx * dx + dy * dy > R * R) continue;
+        nx = x + dx;
+        ny = y + dy;
+        return nx >= 0 && ny >= 0 && nx < W && ny < H;
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool spatially_similar(const HitRec &q, const HitRec &n) {
+    if (fabsf(n.depth - q.depth) > 0.1f * q.depth) return false;
+    return dot(q.ng, n.ng) >= 0.90630779f;  // cos 25 degrees
+}
+
+// A light sample of a reservoir, as seen from one shading point.
+struct RSample {
+    V3 dir, color, lnormal;
+    float dist, pdf;
+    bool solid, delta;
+    int light;
+};
+
+__device__ __forceinline__ RSample rsample_of(const hdr_reservoir &r) {
+    const uint32_t f = r.flags_light >> 24;
+    RSample y;
+    y.dir = v3(r.dir[0], r.dir[1], r.dir[2]);
+    y.color = v3(r.color[0], r.color[1], r.color[2]);
+    y.lnormal = v3(r.light_normal[0], r.light_normal[1], r.light_normal[2]);
+    y.dist = r.dist;
+    y.pdf = r.pdf;
+    y.solid = (f & HDR_RES_PDF_SOLID_ANGLE) != 0;
+    y.delta = (f & HDR_RES_LIGHT_DELTA) != 0;
+    y.light = int(r.flags_light & 0xffffffu) - 1;
+    return y;
+}
+
+// The sample seen from another shading point: distant and dome samples are directions; a point
+// light is sampled again there (it draws nothing); an area sample keeps its point on the light.
+__device__ __forceinline__ bool retarget(const FrameArgs &a, const RSample &y, V3 from, V3 to, RSample &out) {
+    if (y.light < 0 || y.light >= a.s.num_lights) return false;
+    const GLight &L = a.s.lights[y.light];
+    out = y;
+    if (L.type == HDR_LIGHT_DISTANT || L.type == HDR_LIGHT_DOME) return true;
+    if (L.type == HDR_LIGHT_POINT) {
+        LSample ls;
+        Mt64Lean none;  // point lights draw nothing
+        none.init(0u, nullptr);
+        if (!sample_light_inl(a.s, L, to, none, ls)) return false;
+        out.dir = ls.dir;
+        out.color = ls.color;
+        out.lnormal = ls.lnormal;
+        out.dist = ls.dist;
+        out.pdf = ls.pdf * a.light_select_pdf;
+        out.solid = ls.solid_angle;
+        return true;
+    }
+    const V3 P = from + y.dir * y.dist;
+    const V3 toLight = P - to;
+    const float dist = length(toLight);
+    out.dir = divd(toLight, double(dist));
+    out.dist = dist;
+    return true;
+}
+
+// |d omega_to / d omega_from| of moving a sample: 1 for directions and point lights; for a point on
+// an area light, (cos_to / d_to^2) / (cos_from / d_from^2).
+__device__ __forceinline__ float shift_jacobian(const RSample &from, const RSample &to) {
+    if (from.solid) return 1.0f;
+    const float cosN = smax(0.0f, dot(-from.dir, from.lnormal));
+    const float cosQ = smax(0.0f, dot(-to.dir, to.lnormal));
+    if (cosN <= 0.0f) return 0.0f;
+    return (cosQ / (to.dist * to.dist)) / (cosN / (from.dist * from.dist));
+}
+
+// The unshadowed contribution of a sample at a vertex (_reEvaluateForTemporalReuse, :529-545).
+__device__ __forceinline__ Spec reeval_sample(const SpecTab &tab, const Vertex &v, const RSample &y) {
+    const V3 wo = -v.rayDir;
+    const float nDotL = dot(v.sn, y.dir);
+    const float dist2 = y.dist * y.dist;
+    const float cosY = smax(0.0f, dot(-y.dir, y.lnormal));
+    const float pNee = to_solid_angle(y.pdf, y.solid, dist2, smax(cosY, 1e-6f));
+    if (nDotL <= 0.0f || pNee <= 0.0f || (cosY <= 0.0f && !y.delta)) return spec(0.0f);
+    const V3 bsdf = ggx_eval(v.c, v.sn, wo, y.dir);
+    return rgb2spec(tab, y.color) * rgb2spec(tab, bsdf * nDotL);
+}
+
+// The shading vertex of a pixel for the spatial pass, from its G-buffer record and the temporal
+// pass's record {ray direction, wavelength u} (u < 0: the pixel was not shaded).
+__device__ __forceinline__ bool load_spatial_vertex(const FrameArgs &a, const GRec *gbuf, int pixel, Vertex &v,
+                                                    SpecTab &tab) {
+    const float4 x0 = a.spatial_aux[2 * pixel];
+    if (x0.w < 0.0f) return false;
+    HitRec g;
+    if (!load_grec(gbuf + pixel, g)) return false;
+    tab = make_tab(x0.w);
+    const Clo c = get_closure(a.s, g, a.enable_ss != 0);
+    Spec unused = spec(1.0f);
+    v = make_vertex(a, tab, g, c, v3(x0.x, x0.y, x0.z), unused);
+    return true;
+}
+
+// Spatial reuse at pixel (x, y). dl / cl: this lane's domain pixels and confidences in LDS
+// (stride kLanes). Returns the direct light at q; records the winner's shadow query.
+template <bool kDebug>
+__device__ __forceinline__ Spec spatial_reuse_pixel(const FrameArgs &a, const GRec *gbuf, int x, int y,
+                                                    const Vertex &vq, const SpecTab &tabq, int *dl, float *cl,
+                                                    bool &query, V3 &qo, V3 &qd, float &qtmax, int &ssel) {
+    const int pixel = y * a.W + x;
+    const int K = a.spatial_count < kMaxSpatial ? a.spatial_count : kMaxSpatial;
+    const int R = a.spatial_radius;
+    const int M = a.n_cand > 1 ? a.n_cand : 1;
+    const int cap = 20 * M;
+    int nd = 0;
+    dl[0] = pixel;
+    ++nd;
+    int slotBits = 0;  // slot of domain d (>= 1) in bits 4(d-1) .. 4(d-1)+3
+    SpatialRng srng(uint32_t(pixel), a.frame_index);
+#pragma unroll 1
+    for (int k = 0; k < K; ++k) {
+        int nx, ny;
+        if (!pick_neighbour(srng, x, y, a.W, a.H, R, nx, ny)) continue;
+        const int n = ny * a.W + nx;
+        if (a.spatial_aux[2 * n].w < 0.0f) continue;
+        HitRec gn;
+        if (!load_grec(gbuf + n, gn) || !spatially_similar(vq.h, gn)) continue;
+        dl[nd * kLanes] = n;
+        slotBits |= k << (4 * (nd - 1));
+        ++nd;
+    }
+    int countSum = 0;
+#pragma unroll 1
+    for (int d = 0; d < nd; ++d) {
+        const int c = a.res_t[dl[d * kLanes]].count;
+        const int cc = c < cap ? c : cap;
+        cl[d * kLanes] = float(cc);
+        countSum += cc;
+    }
+    float wsum = 0.0f;
+    bool has = false;
+    Cand ch;
+    ssel = -1;
+#pragma unroll 1
+    for (int j = 0; j < nd; ++j) {
+        const hdr_reservoir rj = a.res_t[dl[j * kLanes]];
+        const uint32_t fj = rj.flags_light >> 24;
+        if (!(fj & HDR_RES_HAS_SAMPLE) || !(fj & HDR_RES_HAS_CANDIDATE) || rj.w <= 0.0f) continue;
+        const float phatJ = rj.target;
+        if (phatJ <= 0.0f) continue;
+        const RSample atJ = rsample_of(rj);
+        const V3 posJ = v3(gbuf[dl[j * kLanes]].a.x, gbuf[dl[j * kLanes]].a.y, gbuf[dl[j * kLanes]].a.z);
+        RSample atQ;
+        if (!retarget(a, atJ, posJ, vq.h.pos, atQ)) continue;
+        const Spec reevalQ = reeval_sample(tabq, vq, atQ);
+        const float phatQ = spec_lum(tabq, reevalQ);
+        if (phatQ <= 0.0f) continue;
+        const float jacJQ = j == 0 ? 1.0f : shift_jacobian(atJ, atQ);
+        if (jacJQ <= 0.0f) continue;
+        // balance-heuristic denominator over every domain, in q's measure
+        float den = 0.0f;
+#pragma unroll 1
+        for (int k = 0; k < nd; ++k) {
+            float pk;
+            if (k == j) {
+                pk = phatJ / jacJQ;
+            } else if (k == 0) {
+                pk = phatQ;
+            } else {
+                Vertex vk;
+                SpecTab tabk;
+                pk = 0.0f;
+                if (load_spatial_vertex(a, gbuf, dl[k * kLanes], vk, tabk)) {
+                    RSample atK;
+                    if (retarget(a, atQ, vq.h.pos, vk.h.pos, atK))
+                        pk = spec_lum(tabk, reeval_sample(tabk, vk, atK)) * shift_jacobian(atQ, atK);
+                }
+            }
+            den += cl[k * kLanes] * pk;
+        }
+        if (!(den > 0.0f)) continue;
+        const float w = cl[j * kLanes] * rj.w * phatJ * phatQ / den;
+        if (!(w > 0.0f)) continue;
+        wsum += w;
+        if (srng.next_float() <= w / wsum) {
+            has = true;
+            ch.has_cand = true;
+            ch.delta = atQ.delta;
+            ch.solid = atQ.solid;
+            ch.vis_tested = j == 0 ? (fj & HDR_RES_VIS_TESTED) != 0 : false;
+            ch.light = atQ.light;
+            ch.dir = atQ.dir;
+            ch.color = atQ.color;
+            ch.lnormal = atQ.lnormal;
+            ch.dist = atQ.dist;
+            ch.pdf = atQ.pdf;
+            ch.tp = reevalQ;
+            ch.target = phatQ;
+            ssel = j == 0 ? -1 : ((slotBits >> (4 * (j - 1))) & 0xf);
+        }
+    }
+    const int count = countSum < cap ? countSum : cap;
+    float W = 0.0f, wstore = 0.0f;
+    if (has) {
+        W = wsum / ch.target;
+        if (!isfinite(W)) W = 0.0f;
+        int frameCount = count / M;
+        if (frameCount < 1) frameCount = 1;
+        wstore = W * float(frameCount) * ch.target;
+    }
+    if (has && !ch.vis_tested && ch.has_cand) {
+        ch.tp = winner_contribution_lean(tabq, vq, ch, query, qo, qd, qtmax);
+        ch.vis_tested = true;
+    }
+    store_reservoir(&a.res[pixel], has, ch, wstore, count, W);
+    if (!has || W <= 0.0f) return spec(0.0f);
+    return ch.tp * W;
 }
 
 }  // namespace hdr

@@ -18,34 +18,6 @@ using namespace hdr;
 
 // ============================================================================ kernels
 
-// G-buffer record of one pixel: 4 x float4 = 64 B, written and read coalesced.
-struct GRec {
-    float4 a;  // pos.xyz, depth
-    float4 b;  // ng.xyz, instance (bits; -1 = miss)
-    float4 c;  // ns.xyz, triangle (bits)
-    float4 d;  // albedo.xyz, unused
-};
-
-__device__ __forceinline__ void store_grec(GRec *dst, bool hit, const HitRec &g) {
-    GRec r;
-    r.a = make_float4(g.pos.x, g.pos.y, g.pos.z, g.depth);
-    r.b = make_float4(g.ng.x, g.ng.y, g.ng.z, __int_as_float(hit ? g.inst : -1));
-    r.c = make_float4(g.ns.x, g.ns.y, g.ns.z, __int_as_float(g.tri));
-    r.d = make_float4(g.albedo.x, g.albedo.y, g.albedo.z, 0.0f);
-    *dst = r;
-}
-__device__ __forceinline__ bool load_grec(const GRec *src, HitRec &g) {
-    const GRec r = *src;
-    g.pos = v3(r.a.x, r.a.y, r.a.z);
-    g.depth = r.a.w;
-    g.ng = v3(r.b.x, r.b.y, r.b.z);
-    g.inst = __float_as_int(r.b.w);
-    g.ns = v3(r.c.x, r.c.y, r.c.z);
-    g.tri = __float_as_int(r.c.w);
-    g.albedo = v3(r.d.x, r.d.y, r.d.z);
-    return g.inst >= 0;
-}
-
 // RaycastPass (raycast_pass.cpp:57-162): closest hit of the pixel-centre ray, one 8x8 tile per wave.
 // Sum over the 64 lanes of the wave (every lane active), added once by lane 0.
 __device__ __forceinline__ void wave_add(unsigned long long *dst, unsigned long long v) {
@@ -206,7 +178,8 @@ __device__ __forceinline__ void shade_direct_pixel(const FrameArgs &a, const GRe
     const float py = float(y) + rng.next_float();
     V3 ro, rd;
     camera_ray(a, px, py, &rng, ro, rd);
-    const SpecTab tab = make_tab(rng.next_float());
+    const float utab = rng.next_float();
+    const SpecTab tab = make_tab(utab);
     Trace dbg = {2166136261u, 2166136261u, 0};
     V3 qo = v3s(0.0f), qd = v3s(0.0f);
     float qtmax = 0.0f;
@@ -227,17 +200,25 @@ __device__ __forceinline__ void shade_direct_pixel(const FrameArgs &a, const GRe
                                                            dbg, query, qo, qd, qtmax);
         lVis = throughput * (total + direct);
         lOcc = throughput * (total + spec(0.0f));
+        if (a.spatial_count > 0) {  // the spatial pass finishes this pixel
+            a.spatial_aux[2 * pixel] = make_float4(rd.x, rd.y, rd.z, utab);
+            a.spatial_aux[2 * pixel + 1] =
+                make_float4(throughput.v[0], throughput.v[1], throughput.v[2], throughput.v[3]);
+        }
     } else if (a.s.environment >= 0 && a.ibl_bg) {
         lVis = lOcc = rgb2spec(tab, env_radiance(a.s, rd));
     }
+    if (a.spatial_count > 0 && !gHit) a.spatial_aux[2 * pixel] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
     V3 cv = spec2rgb(tab, lVis), co = spec2rgb(tab, lOcc);
     if (a.has_cam && a.cam.enable_exposure) {
         cv = cv * a.exposure_mul;
         co = co * a.exposure_mul;
     }
-    rgb2[2 * pixel] = make_float4(cv.x, cv.y, cv.z, 1.0f);
-    rgb2[2 * pixel + 1] = make_float4(co.x, co.y, co.z, 1.0f);
-    flags[pixel] = query ? 1u : 0u;
+    if (!(a.spatial_count > 0 && gHit)) {
+        rgb2[2 * pixel] = make_float4(cv.x, cv.y, cv.z, 1.0f);
+        rgb2[2 * pixel + 1] = make_float4(co.x, co.y, co.z, 1.0f);
+        flags[pixel] = query ? 1u : 0u;
+    }
     if (query) {
         q.o = make_float4(qo.x, qo.y, qo.z, qtmax);
         q.d = make_float4(qd.x, qd.y, qd.z, __int_as_float(pixel));
@@ -291,6 +272,63 @@ __global__ void __launch_bounds__(kLanes) shade_direct_kernel(const FrameArgs a,
         wave_add(c + 4, st.cand_skipped);
         wave_add(c + 8, draws);
     }
+}
+
+// Spatial reuse (deferred.h: spatial_reuse_pixel) and the final colours of the shaded pixels; the
+// winner's shadow query joins the queue as in shade_direct_kernel.
+template <bool kCount, bool kDebug>
+__global__ void __launch_bounds__(kLanes) spatial_kernel(const FrameArgs a, const GRec *gbuf, float4 *rgb2,
+                                                         uint32_t *flags, ShadowQuery *queue,
+                                                         unsigned int *queue_count) {
+    __shared__ int dl[(kMaxSpatial + 1) * kLanes];
+    __shared__ float cl[(kMaxSpatial + 1) * kLanes];
+    const int lane = threadIdx.x;
+    const int x = blockIdx.x * 8 + (lane & 7);
+    const int y = blockIdx.y * 8 + (lane >> 3);
+    bool query = false;
+    ShadowQuery q;
+    if (x < a.W && y < a.H) {
+        const int pixel = y * a.W + x;
+        Vertex vq;
+        SpecTab tab;
+        if (load_spatial_vertex(a, gbuf, pixel, vq, tab)) {
+            V3 qo = v3s(0.0f), qd = v3s(0.0f);
+            float qtmax = 0.0f;
+            int ssel = -1;
+            const Spec direct = spatial_reuse_pixel<kDebug>(a, gbuf, x, y, vq, tab, dl + lane, cl + lane, query, qo,
+                                                            qd, qtmax, ssel);
+            const float4 t4 = a.spatial_aux[2 * pixel + 1];
+            const Spec throughput = Spec{{t4.x, t4.y, t4.z, t4.w}};
+            const Spec total = rgb2spec(tab, mv3(vq.c.m->emission));
+            const Spec lVis = throughput * (total + direct);
+            const Spec lOcc = throughput * (total + spec(0.0f));
+            V3 cv = spec2rgb(tab, lVis), co = spec2rgb(tab, lOcc);
+            if (a.has_cam && a.cam.enable_exposure) {
+                cv = cv * a.exposure_mul;
+                co = co * a.exposure_mul;
+            }
+            rgb2[2 * pixel] = make_float4(cv.x, cv.y, cv.z, 1.0f);
+            rgb2[2 * pixel + 1] = make_float4(co.x, co.y, co.z, 1.0f);
+            flags[pixel] = query ? 1u : 0u;
+            if (query) {
+                q.o = make_float4(qo.x, qo.y, qo.z, qtmax);
+                q.d = make_float4(qd.x, qd.y, qd.z, __int_as_float(pixel));
+            }
+            if (kDebug) a.dbg[pixel].select_hash = fnv(a.dbg[pixel].select_hash, uint32_t(ssel));
+        }
+    }
+    const unsigned long long mask = __ballot(query);
+    const int n = __popcll(mask);
+    if (n > 0) {
+        unsigned int base = 0;
+        if (lane == 0) base = atomicAdd(queue_count, unsigned(n));
+        base = __shfl(base, 0);
+        if (query) {
+            const int rank = __popcll(mask & ((1ULL << lane) - 1ULL));
+            queue[base + rank] = q;
+        }
+    }
+    if (kCount) wave_add(a.counters + 2, query ? 1ULL : 0ULL);
 }
 
 // Any-hit shadow rays of the queue; an occluded winner keeps zero throughput in its reservoir
@@ -460,6 +498,7 @@ struct hdr_view {
     int W = 0, H = 0;
     hipStream_t stream = nullptr;
     DevBuf res, accum, color, frame, dbg, depth, albedo, normal, counters, spill, gbuf, rgb2, flags, queue, qcount;
+    DevBuf res_t, saux;  // spatial reuse: post-temporal reservoirs, per-pixel {ray dir, u}, {throughput}
     int counting = 0;
     int frames = 0;
     // Frames overlap: the G-buffer kernel of frame N+1 runs on `gstream` as soon as frame N's
@@ -693,7 +732,8 @@ hdr_status hdr_view_reset(hdr_view *view) {
 hdr_status hdr_view_device_bytes(const hdr_view *v, uint64_t *out) {
     if (!v || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
     *out = v->res.n + v->accum.n + v->color.n + v->frame.n + v->dbg.n + v->depth.n + v->albedo.n + v->normal.n +
-           v->counters.n + v->spill.n + v->gbuf.n + v->rgb2.n + v->flags.n + v->queue.n + v->qcount.n;
+           v->counters.n + v->spill.n + v->gbuf.n + v->rgb2.n + v->flags.n + v->queue.n + v->qcount.n + v->res_t.n +
+           v->saux.n;
     return HDR_OK;
 }
 
@@ -725,7 +765,10 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     if (pipe->max_depth < 1 || pipe->max_depth > 1024) return fail(HDR_ERR_INVALID_ARGUMENT, "max_depth");
     if (pipe->candidate_count < 1 || pipe->candidate_count > 64)
         return fail(HDR_ERR_INVALID_ARGUMENT, "candidate_count must be in [1, 64]");
-    if (pipe->spatial_count != 0) return fail(HDR_ERR_UNSUPPORTED, "spatial reuse is not available in this build");
+    if (pipe->spatial_count < 0 || pipe->spatial_count > kMaxSpatial)
+        return fail(HDR_ERR_INVALID_ARGUMENT, "spatial_count must be in [0, 8]");
+    if (pipe->spatial_count > 0 && !(pipe->spatial_radius >= 1.0f && pipe->spatial_radius <= 4096.0f))
+        return fail(HDR_ERR_INVALID_ARGUMENT, "spatial_radius must be in [1, 4096] pixels");
     HIP_TRY(hipSetDevice(view->ctx->device));
     const int W = view->W, H = view->H;
     const size_t P = size_t(W) * H;
@@ -815,6 +858,20 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     // and no pixel draws past RNG output 310 (the lean stream of that path has no spill slot).
     const bool deferred = a.direct == HDR_DIRECT_RIS && a.max_depth == 1 && a.skip_vis && !scene->any_translucent &&
                           !a.trace_generated && !a.trace_last_connection && bound < kMtN - 1;
+    // Spatial reuse (an extension) runs on the deferred path with temporal reservoirs.
+    const bool spatial = pipe->spatial_count > 0;
+    if (spatial && !(deferred && a.use_res))
+        return fail(HDR_ERR_UNSUPPORTED,
+                    "spatial reuse needs ReSTIR (use_reservoir) at depth 1 with skip_visibility, opaque materials, "
+                    "no emitters reachable by BSDF rays and no environment");
+    if (spatial) {
+        if (hdr_status st = ensure(view->res_t, P * sizeof(hdr_reservoir))) return st;
+        if (hdr_status st = ensure(view->saux, P * 2 * sizeof(float4))) return st;
+        a.res_t = static_cast<hdr_reservoir *>(view->res_t.p);
+        a.spatial_aux = static_cast<float4 *>(view->saux.p);
+        a.spatial_count = pipe->spatial_count;
+        a.spatial_radius = int(pipe->spatial_radius);
+    }
     // candidate weights in LDS: the NEE candidates on the deferred path (its BSDF candidates carry
     // no weight), every candidate on the general path
     const size_t lds = size_t(deferred ? a.n_nee : (a.direct == HDR_DIRECT_RIS ? a.n_cand : 1)) * kLanes *
@@ -862,6 +919,22 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
 #undef HDR_SHADE
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_INTEGRATE], view->stream));
+        if (spatial) {
+            HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_SPATIAL], view->stream));
+#define HDR_SPATIAL(C, D) \
+    hipLaunchKernelGGL((spatial_kernel<C, D>), grid, dim3(kLanes), 0, view->stream, a, gb, rgb2, flags, queue, qc)
+            if (view->counting) {
+                if (debug_out) HDR_SPATIAL(true, true);
+                else HDR_SPATIAL(true, false);
+            } else {
+                if (debug_out) HDR_SPATIAL(false, true);
+                else HDR_SPATIAL(false, false);
+            }
+#undef HDR_SPATIAL
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_SPATIAL], view->stream));
+            ts.ran |= 1u << HDR_STAGE_SPATIAL;
+        }
         HIP_TRY(hipEventRecord(view->shade_done, view->stream));
         view->shade_recorded = true;
         HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_SHADOW], view->stream));

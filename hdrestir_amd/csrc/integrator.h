@@ -31,7 +31,11 @@ struct FrameArgs {
     float light_select_pdf;            // 1 / number of lights
     int firefly;
     float inv_n;                       // 1 / (frame_index + 1)
-    hdr_reservoir *res;
+    hdr_reservoir *res;                // persistent reservoirs (read by the temporal merge, written last)
+    hdr_reservoir *res_t;              // spatial reuse: this frame's post-temporal reservoirs
+    float4 *spatial_aux;               // spatial reuse: per pixel {ray dir, wavelength u}, {throughput}
+    int spatial_count;                 // spatial reuse: neighbours per pixel (0 = off)
+    int spatial_radius;                // spatial reuse: pick radius in pixels
     float4 *accum;
     float4 *frame_rgb;                 // this frame's colour (firefly path / debugging)
     float4 *color;                     // accumulated colour

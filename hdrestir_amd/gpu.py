@@ -194,7 +194,7 @@ class Renderer:
         check(lib().hdr_view_last_timing(self.h, C.byref(g), C.byref(a), C.byref(b)))
         return float(g.value), float(a.value), float(b.value)
 
-    STAGES = ("gbuffer", "integrate", "shadow", "accumulate", "firefly")
+    STAGES = ("gbuffer", "integrate", "shadow", "accumulate", "firefly", "spatial")
 
     def stage_times(self):
         """{stage: (summed device ms, launches)} over the frames since the previous call (HIP events

@@ -190,9 +190,13 @@ typedef struct hdr_pipeline_desc {
     int32_t  render_ibl_background;
     float    roulette_aggressiveness;
     int32_t  enable_firefly_filter;
-    /* Extension beyond the reference (0 = reference behaviour): after the temporal merge,
-     * merge spatial_count neighbour reservoirs of the previous frame from a disc of
-     * spatial_radius pixels. */
+    /* Extension beyond the reference (0 = reference behaviour): after every pixel's temporal
+     * merge, resample among its reservoir and those of spatial_count (<= 8) neighbours of this
+     * frame picked in a disc of spatial_radius pixels (depth / normal similarity test), with
+     * generalized balance-heuristic weights; the winner gets the shadow ray and is stored for
+     * the next frame. Supported for ReSTIR (use_reservoir) at max_depth 1 with skip_visibility,
+     * opaque materials, no emissive material or mesh light and no environment; otherwise
+     * hdr_render_frame returns HDR_ERR_UNSUPPORTED. */
     int32_t  spatial_count;
     float    spatial_radius;
 } hdr_pipeline_desc;
@@ -290,7 +294,8 @@ enum {
     HDR_STAGE_SHADOW = 2,     /* deferred path: any-hit shadow rays of the winners               */
     HDR_STAGE_ACCUMULATE = 3, /* deferred path: visibility pick + accumulation                   */
     HDR_STAGE_FIREFLY = 4,    /* firefly filter + accumulation (enable_firefly_filter)           */
-    HDR_STAGE_COUNT = 5
+    HDR_STAGE_SPATIAL = 5,    /* deferred path with spatial reuse: neighbour resampling + colour */
+    HDR_STAGE_COUNT = 6
 };
 /* Per-stage device time (ms) summed over the frames rendered since the previous call, and the
  * number of launches of each stage; clears the sums. Timing is read back lazily, so frames are

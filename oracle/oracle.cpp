@@ -1500,6 +1500,18 @@ struct Settings {
     hdr_pipeline_desc p;
 };
 
+// Per-pixel record the temporal pass leaves for the spatial-reuse pass (an extension of this
+// build, see SpatialPass below): the shading vertex and the path weights around its direct light.
+struct SpatialAux {
+    bool valid = false;
+    HitRecord hit;
+    Closure cl;
+    V3 rayDir, sn;
+    bool inside = false;
+    Lambda lam;
+    Spec throughput, emission;
+};
+
 struct PathCtx {  // per-pixel thread-local state passed through the integrators
     const Scene &scene;
     const hdr_pipeline_desc &pipe;
@@ -1509,6 +1521,10 @@ struct PathCtx {  // per-pixel thread-local state passed through the integrators
     Counters &cnt;
     const std::unordered_set<const Light *> *hoisted;
     Reservoir *reservoir;  // persistent buffer entry of this pixel
+    // spatial reuse: the post-temporal reservoir goes here (and the winner's visibility test moves
+    // to the spatial pass); null without spatial reuse
+    Reservoir *reservoirOut = nullptr;
+    SpatialAux *aux = nullptr;
 };
 
 std::optional<HitRecord> Trace(const PathCtx &c, const V3 &o, const V3 &d, uint64_t &classCounter) {
@@ -1931,6 +1947,10 @@ Spec RisLi(const PathCtx &c, const LightSampler &sampler, const ShadingPoint &sp
         }
         res.Finalize(perFrame);
         c.tr.selectHash = Fnv(c.tr.selectHash, uint32_t(sel));
+        if (c.reservoirOut) {  // spatial reuse follows: it tests the final winner
+            *c.reservoirOut = res;
+            return Spec();
+        }
         if (res.ChosenSample && !res.ChosenSample->VisibilityTested && res.ChosenSample->Candidate) {
             RISCandidate &winner = *res.ChosenSample;
             const LightSampler::Candidate &lc = *winner.Candidate;
@@ -2049,6 +2069,18 @@ Spec PathLi(const PathCtx &c, const Ray &ray, std::optional<HitRecord> hit) {  /
             V3 sn = cl.Normal;
             if (isInside) sn = -sn;
             const ShadingPoint sp{&cl, sn, &c.lam, isInside};
+            if (c.aux) {
+                SpatialAux &x = *c.aux;
+                x.valid = true;
+                x.hit = *hit;
+                x.cl = cl;
+                x.rayDir = rayDir;
+                x.sn = sn;
+                x.inside = isInside;
+                x.lam = c.lam;
+                x.throughput = throughput;
+                x.emission = RGBToSpectrum(cl.Emission, c.lam);
+            }
             return throughput * PathLiInner(c, sp, *hit, rayDir);
         }
         hit = Trace(c, hit->Position + rayDir * 1e-4f, rayDir, c.cnt.bounce);
@@ -2125,6 +2157,91 @@ inline uint32_t PixelSeed(size_t i, int frameIndex) {  // integration_pass.cpp:2
 }  // namespace
 
 // ---------------------------------------------------------------- C API
+// ---- Spatial reuse: an extension of this build (the reference reuses temporally only).
+// After every pixel's temporal pass, pixel q resamples among its own post-temporal reservoir and
+// those of up to spatial_count neighbours picked within spatial_radius pixels that pass a
+// similarity test (depth within 10 %, geometric normals within 25 degrees). Each reservoir j
+// (confidence c_j = its candidate count, capped at 20 frames) offers its sample y_j, shifted to q
+// (RetargetSample, Jacobian ShiftJacobian), with the generalized balance heuristic
+//   w_j = c_j W_j p_j(y_j) p_q(y_j) / sum_k c_k p_k(y_j -> k) |J_{q->k}|,
+// whose weights sum to one over the domains, so the combination stays unbiased for any BSDF
+// (the plain count-weighted merge of the temporal step is heavy-tailed on glossy lobes). The
+// winner gets W = sum w / p_q(y) and a shadow ray at q; it is stored for the next frame in the
+// reservoir form of reservoir.h (WSum = W * frames * p_q, count = sum of confidences, capped).
+// Neighbour picks and selections draw from a splitmix64 stream seeded by (pixel, frame),
+// integer-only, so the reference's per-pixel mt19937_64 stream is untouched.
+constexpr int kMaxSpatial = 8;
+struct SpatialRng {
+    uint64_t s;
+    SpatialRng(uint32_t pixel, int32_t frame)
+        : s(uint64_t(pixel) * 0x9E3779B97F4A7C15ULL ^ uint64_t(uint32_t(frame)) * 0xC2B2AE3D27D4EB4FULL) {}
+    uint64_t Next() {
+        s += 0x9E3779B97F4A7C15ULL;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    }
+    float NextFloat() { return float(Next() >> 40) * 0x1p-24f; }
+};
+
+bool PickNeighbour(SpatialRng &r, int x, int y, int W, int H, int R, int &nx, int &ny) {
+    const uint64_t D = uint64_t(2 * R + 1);
+    for (int t = 0; t < 8; ++t) {
+        const uint64_t z = r.Next();
+        const int dx = int((z & 0xffffULL) % D) - R;
+        const int dy = int(((z >> 16) & 0xffffULL) % D) - R;
+        if ((dx == 0 && dy == 0) || dx * dx + dy * dy > R * R) continue;
+        nx = x + dx;
+        ny = y + dy;
+        return nx >= 0 && ny >= 0 && nx < W && ny < H;
+    }
+    return false;
+}
+
+bool SpatiallySimilar(const HitRecord &q, const HitRecord &n) {
+    if (std::fabs(n.Depth - q.Depth) > 0.1f * q.Depth) return false;
+    return Dot(q.Normal, n.Normal) >= 0.90630779f;  // cos 25 degrees
+}
+
+// The neighbour's light sample seen from q: distant and dome samples are directions; a point
+// light is sampled again at q (it draws nothing); an area sample keeps its point on the light.
+std::optional<Light::Sample> RetargetSample(const Light &L, const Light::Sample &ls, const V3 &posN, const V3 &posQ,
+                                            float selectPdf) {
+    if (L.type == HDR_LIGHT_DISTANT || L.type == HDR_LIGHT_DOME) return ls;
+    if (L.type == HDR_LIGHT_POINT) {
+        Rng none(0);
+        auto s = L.SampleLight(posQ, none);
+        if (!s) return std::nullopt;
+        s->pdf.value *= selectPdf;
+        return s;
+    }
+    const V3 P = posN + ls.Dir * ls.Dist;
+    const V3 toLight = P - posQ;
+    const float dist = Length(toLight);
+    Light::Sample o = ls;
+    o.Dir = divd(toLight, double(dist));
+    o.Dist = dist;
+    return o;
+}
+
+// |d omega_q / d omega_n| of moving a light sample from n to q: 1 for directions and point lights;
+// for a point on an area light, (cos_q / d_q^2) / (cos_n / d_n^2).
+float ShiftJacobian(const Light::Sample &atN, const Light::Sample &atQ) {
+    if (atN.pdf.space == PdfSpace::SolidAngle) return 1.0f;
+    const float cosN = std::max(0.0f, Dot(-atN.Dir, atN.LightNormal));
+    const float cosQ = std::max(0.0f, Dot(-atQ.Dir, atQ.LightNormal));
+    if (cosN <= 0.0f) return 0.0f;
+    return (cosQ / (atQ.Dist * atQ.Dist)) / (cosN / (atN.Dist * atN.Dist));
+}
+
+void CapHistory(Reservoir &r, int cap) {
+    if (r.CandidateCount > cap) {
+        r.WSum = r.WSum * (float(cap) / float(r.CandidateCount));
+        r.CandidateCount = cap;
+    }
+}
+
 struct oracle_ctx {
     Scene scene;
     int W, H;
@@ -2132,6 +2249,8 @@ struct oracle_ctx {
     int threads;
     bool fireflyInPlace = true;
     std::vector<Reservoir> reservoirs;      // __ris_reservoir
+    std::vector<Reservoir> resT;            // spatial reuse: post-temporal reservoirs of this frame
+    std::vector<SpatialAux> aux;
     std::vector<float> accum;               // __accum_color (RGBA)
     bool accumValid = false;
     std::vector<std::optional<HitRecord>> gbuf;
@@ -2150,6 +2269,8 @@ oracle_ctx *oracle_create(const hdr_scene_desc *scene, int32_t width, int32_t he
     c->threads = threads > 0 ? threads : int(std::max(1u, std::thread::hardware_concurrency()));
     const size_t n = size_t(width) * height;
     c->reservoirs.assign(n, Reservoir{});
+    c->resT.assign(n, Reservoir{});
+    c->aux.assign(n, SpatialAux{});
     c->accum.assign(n * 4, 0.0f);
     c->gbuf.assign(n, std::nullopt);
     c->depth.assign(n, 1.0f);
@@ -2203,6 +2324,11 @@ double oracle_render_frame(oracle_ctx *c, const hdr_frame_desc *frame, const hdr
     const M4d invView = InverseD(view), invProj = InverseD(proj);
     const bool hasCam = frame->has_camera_params != 0;
     const int frameIndex = frame->frame_index;
+    const bool spatial = pipe->spatial_count > 0;
+    if (spatial && (pipe->direct_light != HDR_DIRECT_RIS || !pipe->use_reservoir || pipe->max_depth != 1))
+        return -1.0;  // spatial reuse is defined for ReSTIR direct lighting at depth 1
+    if (spatial)
+        for (SpatialAux &a : c->aux) a.valid = false;
     constexpr int kTile = 16;
     const int ntx = (W + kTile - 1) / kTile, nty = (H + kTile - 1) / kTile;
     Counters cnt;
@@ -2258,8 +2384,16 @@ double oracle_render_frame(oracle_ctx *c, const hdr_frame_desc *frame, const hdr
                                                   : CameraRay(invView, invProj, px, py, W, H);
                               const Lambda lam = Lambda::SampleUniform(rng.NextFloat());
                               PixelTrace tr;
-                              const PathCtx pc{c->scene, *pipe, rng, lam, tr, lc, c->hoist ? &hoistedSet : nullptr,
-                                               &c->reservoirs[i]};
+                              const PathCtx pc{c->scene,
+                                               *pipe,
+                                               rng,
+                                               lam,
+                                               tr,
+                                               lc,
+                                               c->hoist ? &hoistedSet : nullptr,
+                                               &c->reservoirs[i],
+                                               spatial ? &c->resT[i] : nullptr,
+                                               spatial ? &c->aux[i] : nullptr};
                               const Spec s = PathLi(pc, ray, c->gbuf[i]);
                               V3 L = SpectrumToRGB(s, lam);
                               if (hasCam && frame->camera.enable_exposure) L = L * ExposureMultiplier(frame->camera);
@@ -2283,6 +2417,136 @@ double oracle_render_frame(oracle_ctx *c, const hdr_frame_desc *frame, const hdr
                           }
                   },
                   cnt);
+
+    if (spatial) {  // the spatial-reuse pass (see SpatialRng above)
+        const int K = std::min(pipe->spatial_count, kMaxSpatial);
+        const int R = std::max(1, int(pipe->spatial_radius));
+        const int M = std::max(1, pipe->candidate_count);
+        const int cap = 20 * M;
+        const float selPdf = c->scene.lights.empty() ? 0.0f : 1.0f / float(c->scene.lights.size());
+        ParallelTiles(
+            c->threads, ntx * nty,
+            [&](int t, Counters &lc) {
+                const int sx = (t % ntx) * kTile, sy = (t / ntx) * kTile;
+                for (int y = std::max(sy, y0); y < std::min(sy + kTile, y1); ++y)
+                    for (int x = std::max(sx, x0); x < std::min(sx + kTile, x1); ++x) {
+                        const size_t i = size_t(y) * W + x;
+                        if (!c->aux[i].valid) continue;
+                        // the domains: q itself, then the picked neighbours that pass the similarity test
+                        size_t dom[kMaxSpatial + 1];
+                        int slot[kMaxSpatial + 1];
+                        int nd = 0;
+                        dom[nd] = i;
+                        slot[nd++] = -1;
+                        SpatialRng srng(uint32_t(i), frameIndex);
+                        for (int k = 0; k < K; ++k) {
+                            int nx, ny;
+                            if (!PickNeighbour(srng, x, y, W, H, R, nx, ny)) continue;
+                            const size_t n = size_t(ny) * W + nx;
+                            if (!c->aux[n].valid || !SpatiallySimilar(c->aux[i].hit, c->aux[n].hit)) continue;
+                            dom[nd] = n;
+                            slot[nd++] = k;
+                        }
+                        float conf[kMaxSpatial + 1];
+                        int countSum = 0;
+                        for (int d = 0; d < nd; ++d) {
+                            conf[d] = float(std::min(c->resT[dom[d]].CandidateCount, cap));
+                            countSum += std::min(c->resT[dom[d]].CandidateCount, cap);
+                        }
+                        const SpatialAux &q = c->aux[i];
+                        const ShadingPoint sp{&q.cl, q.sn, &q.lam, q.inside};
+                        std::optional<RISCandidate> chosen;
+                        float wsum = 0.0f;
+                        int ssel = -1;
+                        for (int j = 0; j < nd; ++j) {
+                            const Reservoir &rj = c->resT[dom[j]];
+                            if (!rj.ChosenSample || !rj.ChosenSample->Candidate || rj.W <= 0.0f) continue;
+                            const float phatJ = float(rj.ChosenSample->targetFunction);
+                            if (phatJ <= 0.0f) continue;
+                            const Light &light = *rj.ChosenSample->Candidate->light;
+                            const Light::Sample &atJ = rj.ChosenSample->Candidate->ls;
+                            const auto atQ = RetargetSample(light, atJ, c->aux[dom[j]].hit.Position, q.hit.Position,
+                                                            selPdf);
+                            if (!atQ) continue;
+                            const Spec reevalQ = RisReEvaluate(sp, q.rayDir, LightSampler::Candidate{&light, *atQ});
+                            const float phatQ = SpectrumLuminance(reevalQ, q.lam);
+                            if (phatQ <= 0.0f) continue;
+                            const float jacJQ = j == 0 ? 1.0f : ShiftJacobian(atJ, *atQ);
+                            if (jacJQ <= 0.0f) continue;
+                            // balance-heuristic denominator over every domain, in q's measure
+                            float den = 0.0f;
+                            for (int k = 0; k < nd; ++k) {
+                                float pk;
+                                if (k == j) {
+                                    pk = phatJ / jacJQ;
+                                } else if (k == 0) {
+                                    pk = phatQ;
+                                } else {
+                                    const SpatialAux &ak = c->aux[dom[k]];
+                                    const auto atK = RetargetSample(light, *atQ, q.hit.Position, ak.hit.Position, selPdf);
+                                    pk = 0.0f;
+                                    if (atK) {
+                                        const ShadingPoint spk{&ak.cl, ak.sn, &ak.lam, ak.inside};
+                                        const float ph = SpectrumLuminance(
+                                            RisReEvaluate(spk, ak.rayDir, LightSampler::Candidate{&light, *atK}), ak.lam);
+                                        pk = ph * ShiftJacobian(*atQ, *atK);
+                                    }
+                                }
+                                den += conf[k] * pk;
+                            }
+                            if (!(den > 0.0f)) continue;
+                            const float w = conf[j] * rj.W * phatJ * phatQ / den;
+                            if (!(w > 0.0f)) continue;
+                            wsum += w;
+                            if (srng.NextFloat() <= w / wsum) {
+                                RISCandidate cnd = *rj.ChosenSample;
+                                cnd.Candidate->ls = *atQ;
+                                cnd.Throughput = reevalQ;
+                                cnd.targetFunction = double(phatQ);
+                                if (j != 0) cnd.VisibilityTested = false;
+                                chosen = cnd;
+                                ssel = slot[j];
+                            }
+                        }
+                        Reservoir s;
+                        s.CandidateCount = std::min(countSum, cap);
+                        if (chosen) {
+                            s.ChosenSample = chosen;
+                            s.W = wsum / float(chosen->targetFunction);
+                            if (!std::isfinite(s.W)) s.W = 0.0f;
+                            const int frameCount = std::max(1, s.CandidateCount / M);
+                            s.WSum = s.W * float(frameCount) * float(chosen->targetFunction);
+                        }
+                        if (s.ChosenSample && !s.ChosenSample->VisibilityTested && s.ChosenSample->Candidate) {
+                            RISCandidate &wn = *s.ChosenSample;
+                            PixelTrace tr;
+                            Rng none(0);
+                            const PathCtx pc{c->scene, *pipe, none, q.lam, tr, lc, nullptr, nullptr};
+                            const MISContrib nee = RisEvaluateLightSample(pc, sp, q.hit, q.rayDir,
+                                                                          *wn.Candidate->light, wn.Candidate->ls, false);
+                            wn.Throughput = nee.Emission * nee.ThroughputMul;
+                            wn.VisibilityTested = true;
+                        }
+                        const Spec direct =
+                            (!s.ChosenSample || s.W <= 0.0f) ? Spec() : s.ChosenSample->Throughput * s.W;
+                        c->reservoirs[i] = s;
+                        const Spec Ls = q.throughput * (q.emission + direct);
+                        V3 L = SpectrumToRGB(Ls, q.lam);
+                        if (hasCam && frame->camera.enable_exposure) L = L * ExposureMultiplier(frame->camera);
+                        fb[4 * i + 0] = L[0];
+                        fb[4 * i + 1] = L[1];
+                        fb[4 * i + 2] = L[2];
+                        if (debug_out) {
+                            hdr_pixel_debug &d = debug_out[i];
+                            d.select_hash = Fnv(d.select_hash, uint32_t(ssel));
+                            d.sample_rgb[0] = L[0];
+                            d.sample_rgb[1] = L[1];
+                            d.sample_rgb[2] = L[2];
+                        }
+                    }
+            },
+            cnt);
+    }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 
     // AccumulationPass, accumulation_pass.cpp:64-106 (firefly: :22-60 in place, or out of place)

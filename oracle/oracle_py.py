@@ -79,6 +79,8 @@ class Oracle:
         x0, y0, x1, y1 = rect if rect else (0, 0, self.W, self.H)
         self.last_seconds = lib().oracle_render_frame(self.h, C.byref(frame), C.byref(pipe), x0, y0, x1, y1,
                                                       _p(self.color), _p(self.debug) if debug else None)
+        if self.last_seconds < 0:
+            raise ValueError("the oracle does not define this pipeline (spatial reuse needs ReSTIR at depth 1)")
         return self.color
 
     def reservoirs(self):

@@ -170,3 +170,66 @@ def test_overlapped_frames_match_serial_frames(gpu_ctx, oracle_mod):
     ra.close()
     rb.close()
     gs.close()
+
+
+SPATIAL_PIPES = {
+    "restir16_spatial3_r30": abi.pipeline("ris", candidate_count=16, use_reservoir=True, skip_visibility=True,
+                                          max_depth=1, spatial_count=3, spatial_radius=30.0),
+    "restir8_spatial1_r2": abi.pipeline("ris", candidate_count=8, use_reservoir=True, skip_visibility=True,
+                                        max_depth=1, spatial_count=1, spatial_radius=2.0),
+    "restir4_spatial8_r6": abi.pipeline("ris", candidate_count=4, use_reservoir=True, skip_visibility=True,
+                                        max_depth=1, spatial_count=8, spatial_radius=6.0),
+}
+
+
+@pytest.mark.parametrize("scene", ["features", "box"])
+@pytest.mark.parametrize("pname", list(SPATIAL_PIPES))
+def test_spatial_reuse_parity(gpu_ctx, oracle_mod, pname, scene):
+    """Spatial reuse (an extension of the deferred path) matches the oracle's restatement of the
+    same extension: integer streams (including the spatial selection), reservoirs and colours,
+    over frames that feed the spatially reused reservoirs back into the temporal merge."""
+    from hdrestir_amd import gpu
+    if scene == "features":
+        sc = scenes.features(48, 40, dome=False, emissive=False, opaque=True)
+    else:
+        sc = scenes.box_spheres(48, 40, frames=4)
+    pipe = SPATIAL_PIPES[pname]
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    o = oracle_mod.Oracle(sc, threads=8)
+    r.stage_times()
+    for f in range(4):
+        fr = sc.frame(f)
+        img = r.render(fr, pipe, debug=True).copy()
+        ref = o.render(fr, pipe).copy()
+        same = _match(r, o)
+        print(pname, scene, f, "stream parity", same.mean())
+        assert same.all(), same.mean()
+        a, b = r.debug["sample_rgb"], o.debug["sample_rgb"]
+        exact = (a == b).all(axis=1).mean()
+        rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-3)
+        print("  exact colour fraction", exact, "max rel", rel.max())
+        # each neighbour evaluation rebuilds that pixel's wavelength table (exp), so last-bit
+        # differences of transcendentals touch a few more pixels than without spatial reuse
+        assert exact >= 0.95 and rel.max() < 1e-4, (exact, rel.max())
+        assert np.allclose(img, ref, rtol=1e-4, atol=1e-6)
+        gr, orr = r.reservoirs(), o.reservoirs()
+        assert np.array_equal(gr["flags_light"], orr["flags_light"])
+        assert np.array_equal(gr["count"], orr["count"])
+        for k in ("dir", "dist", "color", "pdf", "light_normal", "target", "throughput", "wsum", "w"):
+            assert np.allclose(gr[k], orr[k], rtol=1e-4, atol=1e-6), k
+    st = r.stage_times()
+    assert st["spatial"][1] == 4 and st["shadow"][1] == 4, st
+    r.close()
+    gs.close()
+
+
+def test_spatial_reuse_unsupported_pipeline_is_refused(gpu_ctx):
+    from hdrestir_amd import gpu
+    sc = scenes.features(16, 16)  # dome and emitters: not the deferred path
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    with pytest.raises(gpu.HdrError):
+        r.render(sc.frame(0), SPATIAL_PIPES["restir16_spatial3_r30"])
+    r.close()
+    gs.close()

@@ -369,3 +369,36 @@ def test_feature_scene_estimators_agree_in_mean(oracle_mod):
     d = np.abs(means["pt"][0] - means["ris"][0])
     se = np.sqrt(means["pt"][1] ** 2 + means["ris"][1] ** 2)
     assert (d <= 2.576 * se + 1e-6).all(), (means, d, se)
+
+
+def test_spatial_reuse_extension_is_unbiased_on_diffuse_scene(oracle_mod):
+    """Spatial reuse (an extension: 3 neighbours within 30 px, balance-heuristic resampling after
+    the reference's temporal merge) keeps the per-object mean radiance of unbiased RIS (no
+    reservoir) on the box scene with diffuse materials, within 2 % over 64 frames."""
+    sc = scenes.box_spheres(64, 64, frames=64)
+    for m in sc.materials:
+        m.update(metallic=0.0, roughness=0.9)
+
+    def run(pipe):
+        o = oracle_mod.Oracle(sc, threads=8)
+        vals = []
+        for f in range(64):
+            o.render(sc.frame(f), pipe)
+            vals.append(o.debug["sample_rgb"].astype(np.float64) @ np.array([0.2126, 0.7152, 0.0722]))
+        return np.array(vals)[16:].mean(0), o.debug["prim_instance"].copy()
+
+    ref, inst = run(abi.pipeline("ris", candidate_count=8, use_reservoir=False, skip_visibility=True, max_depth=1))
+    spa, _ = run(abi.pipeline("ris", candidate_count=8, use_reservoir=True, skip_visibility=True, max_depth=1,
+                              spatial_count=3, spatial_radius=30.0))
+    for k in np.unique(inst[inst >= 0]):
+        m = inst == k
+        assert abs(spa[m].mean() / ref[m].mean() - 1.0) < 0.02, (k, spa[m].mean(), ref[m].mean())
+
+
+def test_spatial_reuse_rejects_unsupported_pipelines(oracle_mod):
+    """The extension is defined for ReSTIR at depth 1; the oracle refuses other pipelines."""
+    sc = scenes.box_spheres(16, 16, frames=1)
+    o = oracle_mod.Oracle(sc, threads=1)
+    pipe = abi.pipeline("ris", candidate_count=8, use_reservoir=False, max_depth=1, spatial_count=3)
+    with pytest.raises(ValueError):
+        o.render(sc.frame(0), pipe)
