@@ -6,7 +6,9 @@ Workload (one step = one frame of the per-pixel hot path over the whole image):
           intensities 10^U[0,4], blackbody colours U[2000,10000] K;
   path    RaycastPass G-buffer ray -> jittered camera ray -> 4 hero wavelengths -> RIS with
           M = 32 candidates (16 light + 16 BSDF), skipVisibility (one shadow ray for the winner),
-          temporal reservoir reuse, max depth 1 -> spectral->RGB -> accumulation;
+          temporal reservoir reuse (the reference's merge), then spatial reuse of 3 neighbours
+          within 30 px (an extension of this build), max depth 1 -> spectral->RGB -> accumulation;
+          the reference's own pipeline (temporal reuse only) is timed beside it;
   camera  static (progressive accumulation with temporal reuse); the linear-dolly variant is
           reported beside it (a camera move clears reservoirs and accumulation, as the renderer does).
 
@@ -33,6 +35,7 @@ ACCUM_BYTES = 48   # per pixel accumulation rd+wr 32 + colour write 16
 RGB2_BYTES = 32    # deferred path: visible + occluded colour written by shading (one read back: 16)
 FLAG_BYTES = 4     # deferred path: per pixel visibility flag
 QUERY_BYTES = 32   # deferred path: one shadow query (origin, tmax, direction, pixel)
+AUX_BYTES = 32     # spatial reuse: per pixel {ray direction, wavelength u}, {throughput}
 KERNEL_NAMES = {"gbuffer": "gbuffer_kernel", "integrate": "shade_direct_kernel", "shadow": "shadow_kernel",
                 "accumulate": "accumulate_deferred_kernel", "firefly": "firefly_accumulate_kernel",
                 "spatial": "spatial_kernel"}
@@ -113,7 +116,10 @@ def main():
     from hdrestir_amd import abi, gpu, scenes
     _log("building scene")
     sc = scenes.many_lights(frames=max(args.steps, 2), dolly=True)
-    pipe = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=1)
+    # 1 temporal (the reference's merge) + 3 spatial neighbours within 30 px (an extension)
+    pipe = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=1,
+                        spatial_count=3, spatial_radius=30.0)
+    pipe_temporal = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=1)
     ctx = gpu.Context(0)
     gs = gpu.GpuScene(ctx, sc)
     r = gpu.Renderer(ctx, gs, sc.width, sc.height)
@@ -156,6 +162,17 @@ def main():
     fps = 1000.0 / ms_per_step
     img = r.read_color()
 
+    # the reference's own pipeline (temporal reuse only), same frames
+    r.reset()
+    for i in range(args.warmup):
+        r.render(frame_desc(i, False), pipe_temporal, read_color=False)
+    r.synchronize()
+    t2 = time.perf_counter()
+    for i in range(args.steps):
+        r.render(frame_desc(args.warmup + i, False), pipe_temporal, read_color=False)
+    r.synchronize()
+    temporal_fps = args.steps / (time.perf_counter() - t2)
+
     dolly_fps = None
     if args.dolly:  # the camera moves every frame, which clears reservoirs and accumulation
         t2 = time.perf_counter()
@@ -172,7 +189,17 @@ def main():
     sn, st = counts["node_visits"] - pn, counts["triangle_tests"] - pt
     q = counts["shadow_rays"]
     deferred = "shadow" in stages
-    if deferred:
+    if deferred and "spatial" in stages:
+        # shading: G-buffer read, reservoir read, post-temporal reservoir and aux record written;
+        # spatial: every pixel's post-temporal reservoir, G-buffer and aux record read once (the
+        # neighbours' re-reads are cache traffic), final reservoir, colour pair and flag written
+        alg = {"gbuffer": NODE_BYTES * pn + TRI_BYTES * pt + GBUF_BYTES * P,
+               "integrate": (GBUF_BYTES + RES_BYTES + AUX_BYTES) * P,
+               "spatial": (RES_BYTES // 2 + GBUF_BYTES + AUX_BYTES + RES_BYTES // 2 + RGB2_BYTES + FLAG_BYTES) * P
+                          + QUERY_BYTES * q,
+               "shadow": QUERY_BYTES * q + NODE_BYTES * sn + TRI_BYTES * st,
+               "accumulate": (RGB2_BYTES // 2 + FLAG_BYTES + ACCUM_BYTES) * P}
+    elif deferred:
         alg = {"gbuffer": NODE_BYTES * pn + TRI_BYTES * pt + GBUF_BYTES * P,
                "integrate": (GBUF_BYTES + RES_BYTES + RGB2_BYTES + FLAG_BYTES) * P + QUERY_BYTES * q,
                "shadow": QUERY_BYTES * q + NODE_BYTES * sn + TRI_BYTES * st,
@@ -195,7 +222,7 @@ def main():
     dom_avg = stages[dom_stage][0] / stages[dom_stage][1]
     achieved = alg[dom_stage] / (dom_avg * 1e-3) / 1e9
     line = {
-        "metric": "ReSTIR frames/s at 1080p (many-lights, M=32, 1 temporal); Mrays/s alongside",
+        "metric": "ReSTIR frames/s at 1080p (many-lights, M=32, 1 temporal + 3 spatial within 30 px); Mrays/s alongside",
         "value": round(fps, 3),
         "unit": "frames/s",
         "n_gpus": 1,
@@ -210,12 +237,14 @@ def main():
         "config": {"workload": "many_lights_1080p_restir", "width": sc.width, "height": sc.height,
                    "triangles_unique": sc.triangle_counts()[0], "triangles_instanced": sc.triangle_counts()[1],
                    "lights": len(sc.lights), "candidate_count": 32, "skip_visibility": True, "max_depth": 1,
-                   "temporal_reuse": True, "spatial_reuse": 0, "camera": "static", "parallelism": "single-gpu"},
+                   "temporal_reuse": True, "spatial_reuse": 3, "spatial_radius_px": 30, "camera": "static",
+                   "parallelism": "single-gpu"},
         "mrays_per_s": round(rays * fps / 1e6, 2),
         "mrays_per_s_reference_equivalent": round(ref_rays * fps / 1e6, 2),
         "rays_per_frame": {"primary": counts["primary_rays"], "bounce": counts["bounce_rays"],
                            "shadow": counts["shadow_rays"], "bsdf_candidate_traced": counts["candidate_rays"],
                            "skipped_cannot_contribute": counts["candidate_rays_skipped"]},
+        "temporal_only_frames_per_s": round(temporal_fps, 3),
         "dolly_frames_per_s": None if dolly_fps is None else round(dolly_fps, 3),
         "kernels": kernels,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2),
