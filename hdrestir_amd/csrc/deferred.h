@@ -45,6 +45,18 @@ __device__ __forceinline__ bool load_grec(const GRec *src, HitRec &g) {
     return g.inst >= 0;
 }
 
+// A shadow query of the deferred path: the winner's shadow ray, answered by the any-hit kernel.
+struct ShadowQuery {
+    float4 o;  // origin.xyz, tmax
+    float4 d;  // dir.xyz, view << 24 | pixel (bits)
+};
+// Launch view and pixel of a shadow query (the host keeps views below 2^24 pixels, batches below
+// 256 views).
+__device__ __forceinline__ uint32_t query_code(const FrameArgs &a, int pixel) {
+    return (uint32_t(a.view) << 24) | uint32_t(pixel);
+}
+
+
 // UniformLightSampler::ProposeCandidate light pick (uniform_light_sampler.cpp:22-47).
 template <bool kDebug>
 __device__ __forceinline__ int pick_light_lean(const FrameArgs &a, Mt64Lean &rng, Trace &dbg, bool record) {

@@ -56,14 +56,35 @@ __device__ __forceinline__ void gbuffer_pixel(const FrameArgs &a, GRec *gbuf, in
     }
 }
 
-template <bool kCount>
-__global__ void __launch_bounds__(kLanes) gbuffer_kernel(const FrameArgs a, GRec *gbuf) {
-    const int lane = threadIdx.x;
-    const int x = blockIdx.x * 8 + (lane & 7);
-    const int y = blockIdx.y * 8 + (lane >> 3);
+// Launch layout: one wave64 per 8x8 pixel tile; the blocks of a launch enumerate the tiles of
+// every view in it, view v owning blocks tile_base[v] .. tile_base[v + 1) (a 1-D grid).
+__device__ __forceinline__ int launch_view(const int *__restrict__ tile_base, int n_views) {
+    const int b = int(blockIdx.x);
+    int lo = 0, hi = n_views - 1;
+    while (lo < hi) {  // last view whose first block is <= b (uniform across the block)
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile_base[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+#define HDR_TILE(args, tile_base, n_views)                                   \
+    const int view_ = launch_view(tile_base, n_views);                        \
+    const FrameArgs &a = (args)[view_];                                       \
+    const int tile_ = int(blockIdx.x) - (tile_base)[view_];                   \
+    const int tw_ = (a.W + 7) >> 3;                                           \
+    const int lane = threadIdx.x;                                             \
+    const int x = (tile_ % tw_) * 8 + (lane & 7);                             \
+    const int y = (tile_ / tw_) * 8 + (lane >> 3);                            \
     const bool valid = x < a.W && y < a.H;
+
+template <bool kCount>
+__global__ void __launch_bounds__(kLanes) gbuffer_kernel(const FrameArgs *__restrict__ args,
+                                                         const int *__restrict__ tile_base, int n_views) {
+    HDR_TILE(args, tile_base, n_views)
     TraceStats ts = {};
-    if (valid) gbuffer_pixel<kCount>(a, gbuf, x, y, ts);
+    if (valid) gbuffer_pixel<kCount>(a, a.gbuf, x, y, ts);
     if (kCount) {
         unsigned long long *c = a.counters;
         wave_add(c + 0, valid ? 1ULL : 0ULL);
@@ -131,15 +152,13 @@ __device__ __forceinline__ void integrate_pixel(const FrameArgs &a, const GRec *
 
 // IntegrationPass + (fused) AccumulationPass for one 8x8 pixel tile per wave.
 template <bool kCount, bool kDebug>
-__global__ void __launch_bounds__(kLanes) integrate_kernel(const FrameArgs a, const GRec *gbuf) {
+__global__ void __launch_bounds__(kLanes) integrate_kernel(const FrameArgs *__restrict__ args,
+                                                           const int *__restrict__ tile_base, int n_views) {
     extern __shared__ double wl[];
-    const int lane = threadIdx.x;
-    const int x = blockIdx.x * 8 + (lane & 7);
-    const int y = blockIdx.y * 8 + (lane >> 3);
-    const bool valid = x < a.W && y < a.H;
+    HDR_TILE(args, tile_base, n_views)
     PixelStats st = {};
     uint32_t draws = 0;
-    if (valid) integrate_pixel<kCount, kDebug>(a, gbuf, x, y, wl + lane, st, draws);
+    if (valid) integrate_pixel<kCount, kDebug>(a, a.gbuf, x, y, wl + lane, st, draws);
     if (kCount) {
         unsigned long long *c = a.counters;
         wave_add(c + 1, st.bounce);
@@ -159,10 +178,6 @@ __global__ void __launch_bounds__(kLanes) integrate_kernel(const FrameArgs a, co
 // code (more waves in flight), the shadow rays are compacted into a queue for an any-hit kernel,
 // and a last pass picks the visible / occluded colour and accumulates.
 
-struct ShadowQuery {
-    float4 o;  // origin.xyz, tmax
-    float4 d;  // dir.xyz, pixel (bits)
-};
 
 template <bool kCount, bool kDebug>
 __device__ __forceinline__ void shade_direct_pixel(const FrameArgs &a, const GRec *gbuf, int x, int y, double *wlane,
@@ -221,7 +236,7 @@ __device__ __forceinline__ void shade_direct_pixel(const FrameArgs &a, const GRe
     }
     if (query) {
         q.o = make_float4(qo.x, qo.y, qo.z, qtmax);
-        q.d = make_float4(qd.x, qd.y, qd.z, __int_as_float(pixel));
+        q.d = make_float4(qd.x, qd.y, qd.z, __uint_as_float(query_code(a, pixel)));
     }
     if (kDebug) {
         hdr_pixel_debug d;
@@ -240,32 +255,33 @@ __device__ __forceinline__ void shade_direct_pixel(const FrameArgs &a, const GRe
     draws = rng.k;
 }
 
-template <bool kCount, bool kDebug>
-__global__ void __launch_bounds__(kLanes) shade_direct_kernel(const FrameArgs a, const GRec *gbuf, float4 *rgb2,
-                                                              uint32_t *flags, ShadowQuery *queue,
-                                                              unsigned int *queue_count) {
-    extern __shared__ double wl[];
-    const int lane = threadIdx.x;
-    const int x = blockIdx.x * 8 + (lane & 7);
-    const int y = blockIdx.y * 8 + (lane >> 3);
-    const bool valid = x < a.W && y < a.H;
-    PixelStats st = {};
-    uint32_t draws = 0;
-    bool query = false;
-    ShadowQuery q;
-    if (valid) shade_direct_pixel<kCount, kDebug>(a, gbuf, x, y, wl + lane, st, draws, rgb2, flags, query, q);
-    // wave-aggregated append to the shadow queue
+// Wave-aggregated append of this lane's shadow query (if any) to the launch's queue.
+__device__ __forceinline__ void append_query(const FrameArgs &a, int lane, bool query, const ShadowQuery &q) {
     const unsigned long long mask = __ballot(query);
     const int n = __popcll(mask);
     if (n > 0) {
         unsigned int base = 0;
-        if (lane == 0) base = atomicAdd(queue_count, unsigned(n));
+        if (lane == 0) base = atomicAdd(a.qcount, unsigned(n));
         base = __shfl(base, 0);
         if (query) {
             const int rank = __popcll(mask & ((1ULL << lane) - 1ULL));
-            queue[base + rank] = q;
+            a.queue[base + rank] = q;
         }
     }
+}
+
+template <bool kCount, bool kDebug>
+__global__ void __launch_bounds__(kLanes) shade_direct_kernel(const FrameArgs *__restrict__ args,
+                                                              const int *__restrict__ tile_base, int n_views) {
+    extern __shared__ double wl[];
+    HDR_TILE(args, tile_base, n_views)
+    PixelStats st = {};
+    uint32_t draws = 0;
+    bool query = false;
+    ShadowQuery q;
+    if (valid)
+        shade_direct_pixel<kCount, kDebug>(a, a.gbuf, x, y, wl + lane, st, draws, a.rgb2, a.flags, query, q);
+    append_query(a, lane, query, q);
     if (kCount) {
         unsigned long long *c = a.counters;
         wave_add(c + 2, query ? 1ULL : 0ULL);
@@ -277,17 +293,17 @@ __global__ void __launch_bounds__(kLanes) shade_direct_kernel(const FrameArgs a,
 // Spatial reuse (deferred.h: spatial_reuse_pixel) and the final colours of the shaded pixels; the
 // winner's shadow query joins the queue as in shade_direct_kernel.
 template <bool kCount, bool kDebug>
-__global__ void __launch_bounds__(kLanes) spatial_kernel(const FrameArgs a, const GRec *gbuf, float4 *rgb2,
-                                                         uint32_t *flags, ShadowQuery *queue,
-                                                         unsigned int *queue_count) {
+__global__ void __launch_bounds__(kLanes) spatial_kernel(const FrameArgs *__restrict__ args,
+                                                         const int *__restrict__ tile_base, int n_views) {
     __shared__ int dl[(kMaxSpatial + 1) * kLanes];
     __shared__ float cl[(kMaxSpatial + 1) * kLanes];
-    const int lane = threadIdx.x;
-    const int x = blockIdx.x * 8 + (lane & 7);
-    const int y = blockIdx.y * 8 + (lane >> 3);
+    HDR_TILE(args, tile_base, n_views)
+    const GRec *gbuf = a.gbuf;
+    float4 *rgb2 = a.rgb2;
+    uint32_t *flags = a.flags;
     bool query = false;
     ShadowQuery q;
-    if (x < a.W && y < a.H) {
+    if (valid) {
         const int pixel = y * a.W + x;
         Vertex vq;
         SpecTab tab;
@@ -312,43 +328,35 @@ __global__ void __launch_bounds__(kLanes) spatial_kernel(const FrameArgs a, cons
             flags[pixel] = query ? 1u : 0u;
             if (query) {
                 q.o = make_float4(qo.x, qo.y, qo.z, qtmax);
-                q.d = make_float4(qd.x, qd.y, qd.z, __int_as_float(pixel));
+                q.d = make_float4(qd.x, qd.y, qd.z, __uint_as_float(query_code(a, pixel)));
             }
             if (kDebug) a.dbg[pixel].select_hash = fnv(a.dbg[pixel].select_hash, uint32_t(ssel));
         }
     }
-    const unsigned long long mask = __ballot(query);
-    const int n = __popcll(mask);
-    if (n > 0) {
-        unsigned int base = 0;
-        if (lane == 0) base = atomicAdd(queue_count, unsigned(n));
-        base = __shfl(base, 0);
-        if (query) {
-            const int rank = __popcll(mask & ((1ULL << lane) - 1ULL));
-            queue[base + rank] = q;
-        }
-    }
+    append_query(a, lane, query, q);
     if (kCount) wave_add(a.counters + 2, query ? 1ULL : 0ULL);
 }
 
 // Any-hit shadow rays of the queue; an occluded winner keeps zero throughput in its reservoir
 // (the visibility fix-up of ris_direct_light_integrator.cpp:163-175).
 template <bool kCount>
-__global__ void __launch_bounds__(kLanes) shadow_kernel(const FrameArgs a, const ShadowQuery *queue,
-                                                        const unsigned int *queue_count, uint32_t *flags) {
-    const unsigned int n = *queue_count;
+__global__ void __launch_bounds__(kLanes) shadow_kernel(const FrameArgs *__restrict__ args) {
+    const FrameArgs &a0 = args[0];  // the queue and its count are shared by the views of the launch
+    const unsigned int n = *a0.qcount;
     TraceStats ts = {};
     for (unsigned int i = blockIdx.x * kLanes + threadIdx.x; i < n; i += gridDim.x * kLanes) {
-        const ShadowQuery q = queue[i];
-        const int pixel = __float_as_int(q.d.w);
+        const ShadowQuery q = a0.queue[i];
+        const uint32_t code = __float_as_uint(q.d.w);
+        const FrameArgs &a = args[code >> 24];
+        const int pixel = int(code & 0xffffffu);
         if (trace_any<kCount>(a.s, v3(q.o.x, q.o.y, q.o.z), v3(q.d.x, q.d.y, q.d.z), q.o.w, ts)) {
-            flags[pixel] = 3u;
+            a.flags[pixel] = 3u;
             hdr_reservoir &r = a.res[pixel];
             r.throughput[0] = r.throughput[1] = r.throughput[2] = r.throughput[3] = 0.0f;
         }
     }
     if (kCount) {
-        unsigned long long *c = a.counters;
+        unsigned long long *c = a0.counters;
         wave_add(c + 5, ts.nodes);
         wave_add(c + 6, ts.tris);
         wave_add(c + 7, ts.instances);
@@ -357,10 +365,13 @@ __global__ void __launch_bounds__(kLanes) shadow_kernel(const FrameArgs a, const
 
 // Picks the visible or occluded colour and accumulates (AccumulationPass, accumulation_pass.cpp:98-106).
 template <bool kDebug>
-__global__ void accumulate_deferred_kernel(const FrameArgs a, const float4 *rgb2, const uint32_t *flags) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.W * a.H) return;
-    const float4 c4 = rgb2[2 * i + (flags[i] == 3u ? 1 : 0)];
+__global__ void __launch_bounds__(kLanes) accumulate_deferred_kernel(const FrameArgs *__restrict__ args,
+                                                                     const int *__restrict__ tile_base,
+                                                                     int n_views) {
+    HDR_TILE(args, tile_base, n_views)
+    if (!valid) return;
+    const int i = y * a.W + x;
+    const float4 c4 = a.rgb2[2 * i + (a.flags[i] == 3u ? 1 : 0)];
     if (kDebug) {
         a.dbg[i].sample_rgb[0] = c4.x;
         a.dbg[i].sample_rgb[1] = c4.y;
@@ -476,6 +487,20 @@ struct DevBuf {
     }
 };
 
+// Pinned host memory (hipHostMalloc'd; freed on destruction).
+template <class T>
+struct PinnedBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    hipError_t alloc(size_t count) {
+        n = count;
+        return hipHostMalloc(reinterpret_cast<void **>(&p), (count ? count : 1) * sizeof(T), hipHostMallocDefault);
+    }
+};
+
 }  // namespace
 
 struct hdr_context {
@@ -499,6 +524,8 @@ struct hdr_view {
     hipStream_t stream = nullptr;
     DevBuf res, accum, color, frame, dbg, depth, albedo, normal, counters, spill, gbuf, rgb2, flags, queue, qcount;
     DevBuf res_t, saux;  // spatial reuse: post-temporal reservoirs, per-pixel {ray dir, u}, {throughput}
+    DevBuf dargs, dtile; // per-frame FrameArgs records (one per timing-ring slot); launch tile table {0, tiles}
+    PinnedBuf<FrameArgs> hargs;  // pinned staging of dargs
     int counting = 0;
     int frames = 0;
     // Frames overlap: the G-buffer kernel of frame N+1 runs on `gstream` as soon as frame N's
@@ -666,8 +693,8 @@ hdr_status hdr_scene_device_bytes(const hdr_scene *scene, uint64_t *out) {
 
 hdr_status hdr_view_create(hdr_context *ctx, int32_t width, int32_t height, hdr_view **out) {
     if (!ctx || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
-    if (width <= 0 || height <= 0 || int64_t(width) * height > (int64_t(1) << 28))
-        return fail(HDR_ERR_INVALID_ARGUMENT, "invalid view size");
+    if (width <= 0 || height <= 0 || int64_t(width) * height >= (int64_t(1) << 24))
+        return fail(HDR_ERR_INVALID_ARGUMENT, "invalid view size (at most 2^24 - 1 pixels)");
     HIP_TRY(hipSetDevice(ctx->device));
     std::unique_ptr<hdr_view> v(new (std::nothrow) hdr_view());
     if (!v) return fail(HDR_ERR_OUT_OF_MEMORY, "host allocation failed");
@@ -681,6 +708,13 @@ hdr_status hdr_view_create(hdr_context *ctx, int32_t width, int32_t height, hdr_
     HIP_TRY(v->counters.alloc(sizeof(hdr_frame_counters)));
     HIP_TRY(v->gbuf.alloc(P * sizeof(GRec)));
     HIP_TRY(v->qcount.alloc(sizeof(unsigned int)));
+    HIP_TRY(v->dargs.alloc(hdr_view::kRing * sizeof(FrameArgs)));
+    HIP_TRY(v->dtile.alloc(2 * sizeof(int)));
+    HIP_TRY(v->hargs.alloc(hdr_view::kRing));
+    {
+        const int tiles[2] = {0, ((width + 7) / 8) * ((height + 7) / 8)};
+        HIP_TRY(hipMemcpy(v->dtile.p, tiles, sizeof tiles, hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking));
     HIP_TRY(hipStreamCreateWithFlags(&v->gstream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&v->shade_done, hipEventDisableTiming));
@@ -850,8 +884,8 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         HIP_TRY(hipMemsetAsync(view->counters.p, 0, view->counters.n, view->stream));
         a.counters = static_cast<unsigned long long *>(view->counters.p);
     }
-    const dim3 grid((W + 7) / 8, (H + 7) / 8);
-    GRec *gb = static_cast<GRec *>(view->gbuf.p);
+    const dim3 grid(unsigned(((W + 7) / 8) * ((H + 7) / 8)));  // one block per 8x8 tile
+    a.gbuf = static_cast<GRec *>(view->gbuf.p);
     // The deferred-shadow path applies when the winner's shadow ray is the only ray after the
     // G-buffer: RIS at depth 1, skipVisibility, no translucent material (no pass-through rays),
     // nothing BSDF-sampled can reach an emitter (no emissive material, mesh light or environment),
@@ -880,10 +914,19 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         if (hdr_status st = ensure(view->rgb2, P * 2 * sizeof(float4))) return st;
         if (hdr_status st = ensure(view->flags, P * sizeof(uint32_t))) return st;
         if (hdr_status st = ensure(view->queue, P * sizeof(ShadowQuery))) return st;
+        a.rgb2 = static_cast<float4 *>(view->rgb2.p);
+        a.flags = static_cast<uint32_t *>(view->flags.p);
+        a.queue = static_cast<ShadowQuery *>(view->queue.p);
+        a.qcount = static_cast<unsigned int *>(view->qcount.p);
     }
+    a.view = 0;
     hdr_view::StageSlot &ts = view->ring[view->ring_next];
-    if (hdr_status st = harvest(view, ts)) return st;
+    if (hdr_status st = harvest(view, ts)) return st;  // the slot's previous frame has finished
     ts.ran = 0;
+    // this frame's arguments: pinned staging -> the slot's device record, ahead of the G-buffer
+    view->hargs.p[view->ring_next] = a;
+    const FrameArgs *da = static_cast<const FrameArgs *>(view->dargs.p) + view->ring_next;
+    const int *dt = static_cast<const int *>(view->dtile.p);
     // G-buffer on the second stream: after the previous frame's shading has read the G-buffer, or,
     // when counting, after everything queued so far (counters are per frame).
     if (view->counting) {
@@ -892,9 +935,11 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     } else if (view->shade_recorded) {
         HIP_TRY(hipStreamWaitEvent(view->gstream, view->shade_done, 0));
     }
+    HIP_TRY(hipMemcpyAsync(const_cast<FrameArgs *>(da), &view->hargs.p[view->ring_next], sizeof(FrameArgs),
+                           hipMemcpyHostToDevice, view->gstream));
     HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_GBUFFER], view->gstream));
-    if (view->counting) hipLaunchKernelGGL((gbuffer_kernel<true>), grid, dim3(kLanes), 0, view->gstream, a, gb);
-    else hipLaunchKernelGGL((gbuffer_kernel<false>), grid, dim3(kLanes), 0, view->gstream, a, gb);
+    if (view->counting) hipLaunchKernelGGL((gbuffer_kernel<true>), grid, dim3(kLanes), 0, view->gstream, da, dt, 1);
+    else hipLaunchKernelGGL((gbuffer_kernel<false>), grid, dim3(kLanes), 0, view->gstream, da, dt, 1);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_GBUFFER], view->gstream));
     HIP_TRY(hipEventRecord(view->gbuf_done, view->gstream));
@@ -902,13 +947,9 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     HIP_TRY(hipStreamWaitEvent(view->stream, view->gbuf_done, 0));
     HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_INTEGRATE], view->stream));
     if (deferred) {
-        float4 *rgb2 = static_cast<float4 *>(view->rgb2.p);
-        uint32_t *flags = static_cast<uint32_t *>(view->flags.p);
-        ShadowQuery *queue = static_cast<ShadowQuery *>(view->queue.p);
-        unsigned int *qc = static_cast<unsigned int *>(view->qcount.p);
-        HIP_TRY(hipMemsetAsync(qc, 0, sizeof(unsigned int), view->stream));
+        HIP_TRY(hipMemsetAsync(a.qcount, 0, sizeof(unsigned int), view->stream));
 #define HDR_SHADE(C, D) \
-    hipLaunchKernelGGL((shade_direct_kernel<C, D>), grid, dim3(kLanes), lds, view->stream, a, gb, rgb2, flags, queue, qc)
+    hipLaunchKernelGGL((shade_direct_kernel<C, D>), grid, dim3(kLanes), lds, view->stream, da, dt, 1)
         if (view->counting) {
             if (debug_out) HDR_SHADE(true, true);
             else HDR_SHADE(true, false);
@@ -922,7 +963,7 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         if (spatial) {
             HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_SPATIAL], view->stream));
 #define HDR_SPATIAL(C, D) \
-    hipLaunchKernelGGL((spatial_kernel<C, D>), grid, dim3(kLanes), 0, view->stream, a, gb, rgb2, flags, queue, qc)
+    hipLaunchKernelGGL((spatial_kernel<C, D>), grid, dim3(kLanes), 0, view->stream, da, dt, 1)
             if (view->counting) {
                 if (debug_out) HDR_SPATIAL(true, true);
                 else HDR_SPATIAL(true, false);
@@ -939,27 +980,24 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         view->shade_recorded = true;
         HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_SHADOW], view->stream));
         const int sblocks = int(std::min<size_t>((P + kLanes - 1) / kLanes, 8192));
-        if (view->counting)
-            hipLaunchKernelGGL((shadow_kernel<true>), dim3(sblocks), dim3(kLanes), 0, view->stream, a, queue, qc, flags);
-        else
-            hipLaunchKernelGGL((shadow_kernel<false>), dim3(sblocks), dim3(kLanes), 0, view->stream, a, queue, qc, flags);
+        if (view->counting) hipLaunchKernelGGL((shadow_kernel<true>), dim3(sblocks), dim3(kLanes), 0, view->stream, da);
+        else hipLaunchKernelGGL((shadow_kernel<false>), dim3(sblocks), dim3(kLanes), 0, view->stream, da);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_SHADOW], view->stream));
         HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_ACCUMULATE], view->stream));
-        const int ablocks = int((P + 255) / 256);
         if (debug_out)
-            hipLaunchKernelGGL((accumulate_deferred_kernel<true>), dim3(ablocks), dim3(256), 0, view->stream, a, rgb2, flags);
+            hipLaunchKernelGGL((accumulate_deferred_kernel<true>), grid, dim3(kLanes), 0, view->stream, da, dt, 1);
         else
-            hipLaunchKernelGGL((accumulate_deferred_kernel<false>), dim3(ablocks), dim3(256), 0, view->stream, a, rgb2, flags);
+            hipLaunchKernelGGL((accumulate_deferred_kernel<false>), grid, dim3(kLanes), 0, view->stream, da, dt, 1);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_ACCUMULATE], view->stream));
         ts.ran |= (1u << HDR_STAGE_INTEGRATE) | (1u << HDR_STAGE_SHADOW) | (1u << HDR_STAGE_ACCUMULATE);
     } else if (view->counting) {
-        if (debug_out) hipLaunchKernelGGL((integrate_kernel<true, true>), grid, dim3(kLanes), lds, view->stream, a, gb);
-        else hipLaunchKernelGGL((integrate_kernel<true, false>), grid, dim3(kLanes), lds, view->stream, a, gb);
+        if (debug_out) hipLaunchKernelGGL((integrate_kernel<true, true>), grid, dim3(kLanes), lds, view->stream, da, dt, 1);
+        else hipLaunchKernelGGL((integrate_kernel<true, false>), grid, dim3(kLanes), lds, view->stream, da, dt, 1);
     } else {
-        if (debug_out) hipLaunchKernelGGL((integrate_kernel<false, true>), grid, dim3(kLanes), lds, view->stream, a, gb);
-        else hipLaunchKernelGGL((integrate_kernel<false, false>), grid, dim3(kLanes), lds, view->stream, a, gb);
+        if (debug_out) hipLaunchKernelGGL((integrate_kernel<false, true>), grid, dim3(kLanes), lds, view->stream, da, dt, 1);
+        else hipLaunchKernelGGL((integrate_kernel<false, false>), grid, dim3(kLanes), lds, view->stream, da, dt, 1);
     }
     HIP_TRY(hipGetLastError());
     if (!deferred) {

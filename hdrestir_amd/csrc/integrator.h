@@ -8,7 +8,11 @@
 
 namespace hdr {
 
-// Frame-constant arguments of the integration kernel.
+struct GRec;
+struct ShadowQuery;
+
+// Frame-constant arguments of one view's frame. Kernels read them from a device array (one
+// record per view of a launch), so one launch can cover the tiles of many views.
 struct FrameArgs {
     DeviceScene s;
     double inv_view[16];
@@ -43,6 +47,13 @@ struct FrameArgs {
     float *aov_depth, *aov_albedo, *aov_normal;
     uint64_t *spill;                   // 2 x 312 words per pixel, or null
     unsigned long long *counters;      // hdr_frame_counters as 9 x u64, or null
+    // per-view work buffers
+    GRec *gbuf;                        // G-buffer, one record per pixel
+    float4 *rgb2;                      // deferred path: visible / occluded colour per pixel
+    uint32_t *flags;                   // deferred path: 0 no shadow ray, 1 visible, 3 occluded
+    ShadowQuery *queue;                // deferred path: shadow queries (shared by the views of a batch)
+    unsigned int *qcount;              // deferred path: number of queries (shared likewise)
+    int view;                          // index of this record in the launch (carried by shadow queries)
 };
 
 struct PixelStats {
