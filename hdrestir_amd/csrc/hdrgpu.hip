@@ -337,6 +337,9 @@ __global__ void __launch_bounds__(kLanes) spatial_kernel(const FrameArgs *__rest
     if (kCount) wave_add(a.counters + 2, query ? 1ULL : 0ULL);
 }
 
+// Clears a counter (a kernel rather than a memset, so the step can sit in a captured graph).
+__global__ void zero_u32_kernel(unsigned int *p) { *p = 0u; }
+
 // Any-hit shadow rays of the queue; an occluded winner keeps zero throughput in its reservoir
 // (the visibility fix-up of ris_direct_light_integrator.cpp:163-175).
 template <bool kCount>
@@ -787,8 +790,17 @@ static hdr_status ensure(DevBuf &b, size_t bytes) {
     return HDR_OK;
 }
 
-hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
-                            const hdr_pipeline_desc *pipe, float *color_out, hdr_pixel_debug *debug_out) {
+// Launch plan of one view's frame.
+struct FramePlan {
+    bool deferred = false;  // the deferred-shadow path (else the integration megakernel)
+    bool spatial = false;   // spatial reuse after the temporal pass (deferred path only)
+    size_t lds = 0;         // dynamic LDS of the shading / integration kernel
+    unsigned tiles = 0;     // 8x8 tiles of the view
+};
+
+// Validates a frame, makes sure the view's buffers for it exist and fills its FrameArgs.
+static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
+                                const hdr_pipeline_desc *pipe, bool debug, FrameArgs &a, FramePlan &plan) {
     if (!view || !scene || !frame || !pipe) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
     if (frame->struct_size != sizeof(hdr_frame_desc)) return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_frame_desc size");
     if (pipe->struct_size != sizeof(hdr_pipeline_desc))
@@ -807,7 +819,6 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     const int W = view->W, H = view->H;
     const size_t P = size_t(W) * H;
 
-    FrameArgs a;
     std::memset(&a, 0, sizeof a);
     a.s = scene->ds;
     double m[4][4], inv[4][4];
@@ -870,7 +881,7 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         if (hdr_status s = ensure(view->frame, P * sizeof(float4))) return s;
         a.frame_rgb = static_cast<float4 *>(view->frame.p);
     }
-    if (debug_out) {
+    if (debug) {
         if (hdr_status s = ensure(view->dbg, P * sizeof(hdr_pixel_debug))) return s;
         a.dbg = static_cast<hdr_pixel_debug *>(view->dbg.p);
     }
@@ -880,11 +891,6 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         a.aov_albedo = static_cast<float *>(view->albedo.p);
         a.aov_normal = static_cast<float *>(view->normal.p);
     }
-    if (view->counting) {
-        HIP_TRY(hipMemsetAsync(view->counters.p, 0, view->counters.n, view->stream));
-        a.counters = static_cast<unsigned long long *>(view->counters.p);
-    }
-    const dim3 grid(unsigned(((W + 7) / 8) * ((H + 7) / 8)));  // one block per 8x8 tile
     a.gbuf = static_cast<GRec *>(view->gbuf.p);
     // The deferred-shadow path applies when the winner's shadow ray is the only ray after the
     // G-buffer: RIS at depth 1, skipVisibility, no translucent material (no pass-through rays),
@@ -920,6 +926,27 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         a.qcount = static_cast<unsigned int *>(view->qcount.p);
     }
     a.view = 0;
+    plan.deferred = deferred;
+    plan.spatial = spatial;
+    plan.lds = lds;
+    plan.tiles = unsigned(((W + 7) / 8) * ((H + 7) / 8));
+    return HDR_OK;
+}
+
+hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
+                            const hdr_pipeline_desc *pipe, float *color_out, hdr_pixel_debug *debug_out) {
+    FrameArgs a;
+    FramePlan plan;
+    if (hdr_status st = prepare_frame(view, scene, frame, pipe, debug_out != nullptr, a, plan)) return st;
+    const int W = view->W, H = view->H;
+    const size_t P = size_t(W) * H;
+    const bool deferred = plan.deferred, spatial = plan.spatial;
+    const size_t lds = plan.lds;
+    const dim3 grid(plan.tiles);  // one block per 8x8 tile
+    if (view->counting) {
+        HIP_TRY(hipMemsetAsync(view->counters.p, 0, view->counters.n, view->stream));
+        a.counters = static_cast<unsigned long long *>(view->counters.p);
+    }
     hdr_view::StageSlot &ts = view->ring[view->ring_next];
     if (hdr_status st = harvest(view, ts)) return st;  // the slot's previous frame has finished
     ts.ran = 0;
@@ -1175,6 +1202,194 @@ hdr_status hdr_rng_raw(hdr_context *ctx, const uint32_t *seeds, int32_t n, int32
 hdr_status hdr_rng_float(hdr_context *ctx, const uint32_t *seeds, int32_t n, int32_t n_draws, float *out) {
     if (!out) return fail(HDR_ERR_INVALID_ARGUMENT, "out is null");
     return rng_common(ctx, seeds, n, n_draws, nullptr, out);
+}
+
+
+// ---------------------------------------------------------------------------- batches of views
+struct hdr_batch {
+    hdr_context *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t beg = nullptr, end = nullptr;
+    bool timed = false;
+    DevBuf queue, qcount;  // shadow queries of every view of a step, and their number
+    // steps in flight: arguments and tile tables staged in pinned memory, copied to the slot's
+    // device buffer; a slot is reused once its step has finished
+    static constexpr int kSlots = 4;
+    struct Slot {
+        DevBuf dev;
+        PinnedBuf<char> host;
+        hipEvent_t done = nullptr;
+        bool pending = false;
+    };
+    Slot slots[kSlots];
+    int next = 0;
+};
+
+hdr_status hdr_batch_create(hdr_context *ctx, hdr_batch **out) {
+    if (!ctx || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<hdr_batch> b(new (std::nothrow) hdr_batch());
+    if (!b) return fail(HDR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    b->ctx = ctx;
+    HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&b->beg));
+    HIP_TRY(hipEventCreate(&b->end));
+    for (auto &sl : b->slots) HIP_TRY(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    HIP_TRY(b->qcount.alloc(sizeof(unsigned int)));
+    *out = b.release();
+    return HDR_OK;
+}
+
+hdr_status hdr_batch_destroy(hdr_batch *b) {
+    if (!b) return HDR_OK;
+    (void)hipSetDevice(b->ctx->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto &sl : b->slots)
+        if (sl.done) (void)hipEventDestroy(sl.done);
+    if (b->beg) (void)hipEventDestroy(b->beg);
+    if (b->end) (void)hipEventDestroy(b->end);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return HDR_OK;
+}
+
+hdr_status hdr_batch_synchronize(hdr_batch *b) {
+    if (!b) return fail(HDR_ERR_INVALID_ARGUMENT, "batch is null");
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return HDR_OK;
+}
+
+hdr_status hdr_batch_last_timing(hdr_batch *b, float *step_ms) {
+    if (!b || !step_ms) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    *step_ms = 0.0f;
+    if (!b->timed) return HDR_OK;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    HIP_TRY(hipEventSynchronize(b->end));
+    HIP_TRY(hipEventElapsedTime(step_ms, b->beg, b->end));
+    return HDR_OK;
+}
+
+hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, const hdr_scene *const *scenes,
+                            const hdr_frame_desc *frames, const hdr_pipeline_desc *pipes) {
+    if (!b || !views || !scenes || !frames || !pipes) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    if (n < 1 || n > 255) return fail(HDR_ERR_INVALID_ARGUMENT, "a batch holds 1 to 255 views");
+    for (int i = 0; i < n; ++i) {
+        if (!views[i] || views[i]->ctx != b->ctx) return fail(HDR_ERR_INVALID_ARGUMENT, "view of another context");
+        if (views[i]->counting) return fail(HDR_ERR_UNSUPPORTED, "counting views render one at a time");
+        if (pipes[i].struct_size == sizeof(hdr_pipeline_desc) && pipes[i].enable_firefly_filter)
+            return fail(HDR_ERR_UNSUPPORTED, "the firefly filter renders one view at a time");
+        for (int j = 0; j < i; ++j)
+            if (views[j] == views[i]) return fail(HDR_ERR_INVALID_ARGUMENT, "a view appears twice in a batch");
+    }
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    std::vector<FrameArgs> args(static_cast<size_t>(n));
+    std::vector<FramePlan> plans(static_cast<size_t>(n));
+    size_t queueCap = 0;
+    for (int i = 0; i < n; ++i) {
+        if (hdr_status st = prepare_frame(views[i], scenes[i], &frames[i], &pipes[i], false, args[i], plans[i]))
+            return st;
+        if (plans[i].deferred) queueCap += size_t(views[i]->W) * views[i]->H;
+    }
+    if (queueCap * sizeof(ShadowQuery) > b->queue.n) {
+        HIP_TRY(hipStreamSynchronize(b->stream));  // earlier steps may still read the old queue
+        if (hdr_status st = ensure(b->queue, queueCap * sizeof(ShadowQuery))) return st;
+    }
+    for (int i = 0; i < n; ++i) {
+        args[i].view = i;
+        args[i].queue = static_cast<ShadowQuery *>(b->queue.p);
+        args[i].qcount = static_cast<unsigned int *>(b->qcount.p);
+    }
+    // launch groups: every view (G-buffer, shadow), deferred views (shading, accumulation), their
+    // spatial-reuse subset, and megakernel views; each gets its own argument array and tile table
+    enum { kAll, kDef, kSpa, kMega, kGroups };
+    std::vector<int> members[kGroups];
+    for (int i = 0; i < n; ++i) {
+        members[kAll].push_back(i);
+        if (plans[i].deferred) members[kDef].push_back(i);
+        if (plans[i].deferred && plans[i].spatial) members[kSpa].push_back(i);
+        if (!plans[i].deferred) members[kMega].push_back(i);
+    }
+    auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
+    size_t off = 0, argOff[kGroups], tileOff[kGroups];
+    unsigned tiles[kGroups];
+    size_t lds[kGroups];
+    for (int g = 0; g < kGroups; ++g) {
+        argOff[g] = off;
+        off = align(off + members[g].size() * sizeof(FrameArgs));
+        tileOff[g] = off;
+        off = align(off + (members[g].size() + 1) * sizeof(int));
+        tiles[g] = 0;
+        lds[g] = 0;
+        for (int i : members[g]) {
+            tiles[g] += plans[i].tiles;
+            lds[g] = std::max(lds[g], plans[i].lds);
+        }
+    }
+    hdr_batch::Slot &sl = b->slots[b->next];
+    if (sl.pending) {
+        HIP_TRY(hipEventSynchronize(sl.done));  // the slot's previous step has finished
+        sl.pending = false;
+    }
+    if (sl.host.n < off) {
+        sl.host = PinnedBuf<char>();
+        HIP_TRY(sl.host.alloc(off));
+    }
+    if (hdr_status st = ensure(sl.dev, off)) return st;
+    char *hp = sl.host.p;
+    const char *dp = static_cast<const char *>(sl.dev.p);
+    for (int g = 0; g < kGroups; ++g) {
+        FrameArgs *ga = reinterpret_cast<FrameArgs *>(hp + argOff[g]);
+        int *tb = reinterpret_cast<int *>(hp + tileOff[g]);
+        int base = 0;
+        for (size_t k = 0; k < members[g].size(); ++k) {
+            ga[k] = args[size_t(members[g][k])];
+            tb[k] = base;
+            base += int(plans[size_t(members[g][k])].tiles);
+        }
+        tb[members[g].size()] = base;
+    }
+    // order after each view's earlier work; stage the arguments
+    for (int i = 0; i < n; ++i) {
+        HIP_TRY(hipEventRecord(views[i]->serial, views[i]->stream));
+        HIP_TRY(hipStreamWaitEvent(b->stream, views[i]->serial, 0));
+    }
+    HIP_TRY(hipEventRecord(b->beg, b->stream));
+    HIP_TRY(hipMemcpyAsync(sl.dev.p, sl.host.p, off, hipMemcpyHostToDevice, b->stream));
+    auto A = [&](int g) { return reinterpret_cast<const FrameArgs *>(dp + argOff[g]); };
+    auto T = [&](int g) { return reinterpret_cast<const int *>(dp + tileOff[g]); };
+    auto N = [&](int g) { return int(members[g].size()); };
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, b->stream, static_cast<unsigned int *>(b->qcount.p));
+    hipLaunchKernelGGL((gbuffer_kernel<false>), dim3(tiles[kAll]), dim3(kLanes), 0, b->stream, A(kAll), T(kAll),
+                       N(kAll));
+    if (N(kDef) > 0)
+        hipLaunchKernelGGL((shade_direct_kernel<false, false>), dim3(tiles[kDef]), dim3(kLanes), lds[kDef], b->stream,
+                           A(kDef), T(kDef), N(kDef));
+    if (N(kSpa) > 0)
+        hipLaunchKernelGGL((spatial_kernel<false, false>), dim3(tiles[kSpa]), dim3(kLanes), 0, b->stream, A(kSpa),
+                           T(kSpa), N(kSpa));
+    if (N(kMega) > 0)
+        hipLaunchKernelGGL((integrate_kernel<false, false>), dim3(tiles[kMega]), dim3(kLanes), lds[kMega], b->stream,
+                           A(kMega), T(kMega), N(kMega));
+    if (N(kDef) > 0) {
+        const int sblocks = int(std::min<size_t>((queueCap + kLanes - 1) / kLanes, 8192));
+        hipLaunchKernelGGL((shadow_kernel<false>), dim3(sblocks), dim3(kLanes), 0, b->stream, A(kAll));
+        hipLaunchKernelGGL((accumulate_deferred_kernel<false>), dim3(tiles[kDef]), dim3(kLanes), 0, b->stream,
+                           A(kDef), T(kDef), N(kDef));
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(b->end, b->stream));
+    HIP_TRY(hipEventRecord(sl.done, b->stream));
+    sl.pending = true;
+    b->timed = true;
+    b->next = (b->next + 1) % hdr_batch::kSlots;
+    // later work of each view (reads, single frames, resets) follows this step
+    for (int i = 0; i < n; ++i) {
+        HIP_TRY(hipStreamWaitEvent(views[i]->stream, sl.done, 0));
+        HIP_TRY(hipStreamWaitEvent(views[i]->gstream, sl.done, 0));
+        views[i]->frames++;
+    }
+    return HDR_OK;
 }
 
 }  // extern "C"

@@ -80,6 +80,11 @@ def lib():
             "hdr_view_synchronize": [P],
             "hdr_view_last_timing": [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)],
             "hdr_view_read_stage_times": [P, P, P],
+            "hdr_batch_create": [P, C.POINTER(P)],
+            "hdr_batch_destroy": [P],
+            "hdr_render_batch": [P, I, P, P, P, P],
+            "hdr_batch_synchronize": [P],
+            "hdr_batch_last_timing": [P, C.POINTER(C.c_float)],
             "hdr_scene_intersect": [P, P, P, I, P, P, P],
             "hdr_rng_raw": [P, P, I, I, P],
             "hdr_rng_float": [P, P, I, I, P],
@@ -232,3 +237,39 @@ class Renderer:
         v = C.c_uint64()
         check(lib().hdr_view_device_bytes(self.h, C.byref(v)))
         return int(v.value)
+
+
+class Batch:
+    """One frame for each of many views in merged launches (hdr_render_batch)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        check(lib().hdr_batch_create(ctx.h, C.byref(self.h)))
+
+    def render(self, renderers, frames, pipes):
+        n = len(renderers)
+        views = (C.c_void_p * n)(*[r.h for r in renderers])
+        scenes = (C.c_void_p * n)(*[r.gscene.h for r in renderers])
+        fr = (abi.FrameDesc * n)(*frames)
+        pp = (abi.PipelineDesc * n)(*pipes)
+        check(lib().hdr_render_batch(self.h, n, views, scenes, fr, pp))
+
+    def synchronize(self):
+        check(lib().hdr_batch_synchronize(self.h))
+
+    def last_timing(self):
+        ms = C.c_float()
+        check(lib().hdr_batch_last_timing(self.h, C.byref(ms)))
+        return float(ms.value)
+
+    def close(self):
+        if self.h:
+            lib().hdr_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

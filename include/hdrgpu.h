@@ -303,6 +303,26 @@ enum {
  * (core/metrics_on_buffers.h, RESTIR_BUFFER_METRIC_SCOPE). */
 hdr_status hdr_view_read_stage_times(hdr_view *view, double *stage_ms, int32_t *launches);
 
+/* ---- batches of views ----
+ * No counterpart in the reference, which renders one view per Renderer::Render call
+ * (renderer.cpp:390-478). A batch renders one frame for each of up to 255 views in merged
+ * launches: one G-buffer launch over every view's 8x8 tiles, one shading launch per kernel path
+ * (deferred shading, spatial reuse, integration megakernel), one shadow launch over the combined
+ * queue and one accumulation launch. Each view gets exactly the frame hdr_render_frame would
+ * give it (same streams of random numbers, same results); views may differ in scene, size,
+ * pipeline and frame. Not available with counting views or the firefly filter
+ * (HDR_ERR_UNSUPPORTED). A view's later calls (reads, resets, single frames) are ordered after
+ * the batch. A batch is used from one host thread at a time. */
+typedef struct hdr_batch hdr_batch;
+hdr_status hdr_batch_create(hdr_context *ctx, hdr_batch **out);
+hdr_status hdr_batch_destroy(hdr_batch *batch);
+hdr_status hdr_render_batch(hdr_batch *batch, int32_t n, hdr_view *const *views, const hdr_scene *const *scenes,
+                            const hdr_frame_desc *frames, const hdr_pipeline_desc *pipes);
+/* Blocks until the batch's queued steps have finished. */
+hdr_status hdr_batch_synchronize(hdr_batch *batch);
+/* Device time (ms) of the last step, from its first launch to its last. */
+hdr_status hdr_batch_last_timing(hdr_batch *batch, float *step_ms);
+
 /* ---- parity / measurement entry points ---- */
 
 /* Closest hit for n world-space rays (reference: Scene::IntersectScene, scene.cpp:515-535).

@@ -1,0 +1,77 @@
+"""Batches of views (hdr_render_batch): one frame for many views in merged launches gives every
+view exactly the frame hdr_render_frame gives it -- across scenes, sizes, kernel paths (deferred
+shading, spatial reuse, integration megakernel) and frames of temporal reuse."""
+import numpy as np
+import pytest
+
+from hdrestir_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    box = scenes.box_spheres(64, 48, frames=5)
+    box_small = scenes.box_spheres(40, 24, frames=5)
+    feat = scenes.features(48, 40, frames=5, dome=False, emissive=False, opaque=True)
+    feat_dome = scenes.features(32, 32, frames=5)
+    return [
+        (box, abi.pipeline("ris", candidate_count=16, use_reservoir=True, skip_visibility=True, max_depth=1)),
+        (feat, abi.pipeline("ris", candidate_count=8, use_reservoir=True, skip_visibility=True, max_depth=1,
+                            spatial_count=3, spatial_radius=10.0)),
+        (box_small, abi.pipeline("mis", max_depth=4)),
+        (feat_dome, abi.pipeline("ris", candidate_count=8, use_reservoir=True, max_depth=3)),
+        (box, abi.pipeline("ris", candidate_count=4, use_reservoir=False, skip_visibility=True, max_depth=1)),
+    ]
+
+
+def test_batch_matches_single_view_frames(gpu_ctx):
+    from hdrestir_amd import gpu
+    cases = _cases()
+    gscenes = {}
+    for sc, _ in cases:
+        if id(sc) not in gscenes:
+            gscenes[id(sc)] = gpu.GpuScene(gpu_ctx, sc)
+    single = [gpu.Renderer(gpu_ctx, gscenes[id(sc)], sc.width, sc.height) for sc, _ in cases]
+    batched = [gpu.Renderer(gpu_ctx, gscenes[id(sc)], sc.width, sc.height) for sc, _ in cases]
+    batch = gpu.Batch(gpu_ctx)
+    for f in range(5):
+        for r, (sc, pipe) in zip(single, cases):
+            r.render(sc.frame(f), pipe, read_color=False)
+        batch.render(batched, [sc.frame(f) for sc, _ in cases], [pipe for _, pipe in cases])
+    batch.synchronize()
+    assert batch.last_timing() > 0.0
+    for k, (rs, rb) in enumerate(zip(single, batched)):
+        a = rs.read_color().copy()
+        b = rb.read_color().copy()
+        assert np.array_equal(a, b), k
+        if cases[k][1].use_reservoir:
+            assert rs.reservoirs().tobytes() == rb.reservoirs().tobytes(), k
+    # a batch of one equals the single-view path too, and later single frames follow the batch
+    for r, (sc, pipe) in zip(batched, cases):
+        r.render(sc.frame(5 % 5), pipe, read_color=False)
+    for r, (sc, pipe) in zip(single, cases):
+        r.render(sc.frame(5 % 5), pipe, read_color=False)
+    for rs, rb in zip(single, batched):
+        assert np.array_equal(rs.read_color(), rb.read_color())
+    batch.close()
+    for r in single + batched:
+        r.close()
+    for g in gscenes.values():
+        g.close()
+
+
+def test_batch_refuses_duplicates_and_counting(gpu_ctx):
+    from hdrestir_amd import gpu
+    sc = scenes.box_spheres(16, 16, frames=1)
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, 16, 16)
+    batch = gpu.Batch(gpu_ctx)
+    pipe = abi.pipeline("mis", max_depth=2)
+    with pytest.raises(gpu.HdrError):
+        batch.render([r, r], [sc.frame(0), sc.frame(0)], [pipe, pipe])
+    r.set_counting(True)
+    with pytest.raises(gpu.HdrError):
+        batch.render([r], [sc.frame(0)], [pipe])
+    batch.close()
+    r.close()
+    gs.close()
