@@ -150,3 +150,24 @@ def reservoir_dtype():
     return np.dtype([("dir", "<f4", 3), ("dist", "<f4"), ("color", "<f4", 3), ("pdf", "<f4"),
                      ("light_normal", "<f4", 3), ("target", "<f4"), ("throughput", "<f4", 4),
                      ("wsum", "<f4"), ("count", "<i4"), ("w", "<f4"), ("flags_light", "<u4")])
+
+
+
+class ServerDesc(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("max_active", C.c_int32), ("hbm_budget_bytes", C.c_uint64),
+                ("batched", C.c_int32), ("_reserved", C.c_int32)]
+
+
+class RequestDesc(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
+                ("scene", C.c_void_p), ("frame", FrameDesc), ("pipe", PipelineDesc)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
+                ("_reserved", C.c_int32), ("queued_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("queued", C.c_int32), ("active", C.c_int32), ("finished", C.c_int32),
+                ("pooled_views", C.c_int32), ("hbm_in_use", C.c_uint64), ("steps", C.c_uint64)]

@@ -20,7 +20,7 @@ _lib = None
 
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["hdrgpu.hip", "host_scene.cpp"]
+SOURCES = ["hdrgpu.hip", "host_scene.cpp", "server.cpp"]
 ADAPTER_SMOKE = os.path.join(_HERE, "host", "build", "adapter_smoke")
 
 
@@ -85,6 +85,13 @@ def lib():
             "hdr_render_batch": [P, I, P, P, P, P],
             "hdr_batch_synchronize": [P],
             "hdr_batch_last_timing": [P, C.POINTER(C.c_float)],
+            "hdr_server_create": [P, C.POINTER(abi.ServerDesc), C.POINTER(P)],
+            "hdr_server_destroy": [P],
+            "hdr_server_submit": [P, C.POINTER(abi.RequestDesc), C.POINTER(C.c_uint64)],
+            "hdr_server_step": [P, C.POINTER(C.c_int32)],
+            "hdr_server_peek_result": [P, C.POINTER(abi.RequestResult), C.POINTER(C.c_int32)],
+            "hdr_server_take_result": [P, C.c_uint64, P],
+            "hdr_server_get_stats": [P, C.POINTER(abi.ServerStats)],
             "hdr_scene_intersect": [P, P, P, I, P, P, P],
             "hdr_rng_raw": [P, P, I, I, P],
             "hdr_rng_float": [P, P, I, I, P],
@@ -266,6 +273,69 @@ class Batch:
     def close(self):
         if self.h:
             lib().hdr_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Render-request server (hdr_server_*): continuous batching (batched=True) or the plain
+    per-request path, with admission by device-memory footprint."""
+
+    def __init__(self, ctx, batched=True, max_active=64, hbm_budget_bytes=64 << 30):
+        self.ctx = ctx
+        d = abi.ServerDesc()
+        d.struct_size = C.sizeof(abi.ServerDesc)
+        d.max_active = max_active
+        d.hbm_budget_bytes = hbm_budget_bytes
+        d.batched = int(batched)
+        self.h = C.c_void_p()
+        check(lib().hdr_server_create(ctx.h, C.byref(d), C.byref(self.h)))
+
+    def submit(self, gscene, width, height, spp, frame, pipe):
+        r = abi.RequestDesc()
+        r.struct_size = C.sizeof(abi.RequestDesc)
+        r.width, r.height, r.spp = width, height, spp
+        r.scene = gscene.h
+        r.frame = frame
+        r.pipe = pipe
+        rid = C.c_uint64()
+        check(lib().hdr_server_submit(self.h, C.byref(r), C.byref(rid)))
+        return int(rid.value)
+
+    def step(self):
+        n = C.c_int32()
+        check(lib().hdr_server_step(self.h, C.byref(n)))
+        return int(n.value)
+
+    def results(self, images=False):
+        """Takes every finished request: list of (result dict, image or None)."""
+        out = []
+        while True:
+            res = abi.RequestResult()
+            avail = C.c_int32()
+            check(lib().hdr_server_peek_result(self.h, C.byref(res), C.byref(avail)))
+            if not avail.value:
+                return out
+            img = None
+            if images:
+                img = np.zeros((res.height, res.width, 4), np.float32)
+            check(lib().hdr_server_take_result(self.h, C.c_uint64(res.id), _p(img) if images else None))
+            out.append(({"id": int(res.id), "width": res.width, "height": res.height, "spp": res.spp,
+                         "queued_ms": res.queued_ms, "latency_ms": res.latency_ms}, img))
+
+    def stats(self):
+        st = abi.ServerStats()
+        check(lib().hdr_server_get_stats(self.h, C.byref(st)))
+        return {k: int(getattr(st, k)) for k, _ in st._fields_}
+
+    def close(self):
+        if self.h:
+            lib().hdr_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

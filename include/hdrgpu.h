@@ -323,6 +323,60 @@ hdr_status hdr_batch_synchronize(hdr_batch *batch);
 /* Device time (ms) of the last step, from its first launch to its last. */
 hdr_status hdr_batch_last_timing(hdr_batch *batch, float *step_ms);
 
+/* ---- serving ----
+ * No counterpart in the reference, whose renderer serves one Hydra render pass at a time
+ * (renderer.cpp:390-478). A server keeps scenes resident (the caller's hdr_scene handles), gives
+ * each admitted request its own view (reservoirs, accumulation) on the device, admits queued
+ * requests first-in first-out while their views fit hbm_budget_bytes and fewer than max_active are
+ * running, and renders either
+ *   batched = 1: continuous batching -- every hdr_server_step renders one frame of every running
+ *                request in one merged launch set (hdr_render_batch); requests join when memory and
+ *                a slot free up and leave when their last frame is done; or
+ *   batched = 0: the plain per-request path -- one request at a time, all its frames through
+ *                hdr_render_frame.
+ * A request renders frames frame.frame_index .. + spp - 1 of its own view, so its image is the
+ * same alone and in any mix. Finished views return to a pool (reused for the same size, freed
+ * oldest-first when memory is needed). Used from one host thread. */
+typedef struct hdr_server hdr_server;
+typedef struct hdr_server_desc {
+    uint32_t struct_size;
+    int32_t  max_active;        /* requests on the device at once, 1..255 (1 when batched = 0) */
+    uint64_t hbm_budget_bytes;  /* device memory the server's views may hold */
+    int32_t  batched;
+    int32_t  _reserved;
+} hdr_server_desc;
+typedef struct hdr_request_desc {
+    uint32_t struct_size;
+    int32_t  width, height;
+    int32_t  spp;               /* frames to accumulate */
+    const hdr_scene *scene;
+    hdr_frame_desc frame;       /* camera; frame_index of the first frame */
+    hdr_pipeline_desc pipe;
+} hdr_request_desc;
+typedef struct hdr_request_result {
+    uint64_t id;
+    int32_t  width, height, spp, _reserved;
+    double   queued_ms;         /* submission -> admission */
+    double   latency_ms;        /* submission -> last frame finished */
+} hdr_request_result;
+typedef struct hdr_server_stats {
+    int32_t  queued, active, finished, pooled_views;
+    uint64_t hbm_in_use;
+    uint64_t steps;
+} hdr_server_stats;
+hdr_status hdr_server_create(hdr_context *ctx, const hdr_server_desc *desc, hdr_server **out);
+hdr_status hdr_server_destroy(hdr_server *server);
+/* Queues a request; HDR_ERR_OUT_OF_MEMORY when its view alone exceeds the budget. */
+hdr_status hdr_server_submit(hdr_server *server, const hdr_request_desc *req, uint64_t *id);
+/* Admits what fits, renders (one merged frame for every running request, or one whole request),
+ * waits for the device and retires finished requests (*n_finished of them). */
+hdr_status hdr_server_step(hdr_server *server, int32_t *n_finished);
+/* The oldest finished request not yet taken (*available = 0 if none). */
+hdr_status hdr_server_peek_result(hdr_server *server, hdr_request_result *out, int32_t *available);
+/* Copies a finished request's image (RGBA32F, width*height*4; rgba may be null) and releases it. */
+hdr_status hdr_server_take_result(hdr_server *server, uint64_t id, float *rgba);
+hdr_status hdr_server_get_stats(hdr_server *server, hdr_server_stats *out);
+
 /* ---- parity / measurement entry points ---- */
 
 /* Closest hit for n world-space rays (reference: Scene::IntersectScene, scene.cpp:515-535).
