@@ -22,6 +22,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -84,6 +86,123 @@ def cpu_baseline(scene, pipe, rows=540, threads=None):
                       f"{scene.height / rows:g} to the full frame"}
 
 
+def look_at_origin_from_sphere(rng, radius):
+    from hdrestir_amd import scenes
+    v = rng.normal(size=3)
+    v[1] = abs(v[1]) + 0.3  # above the ground
+    v = v / np.linalg.norm(v) * radius
+    return scenes.look_at_view(tuple(v), (0.0, 0.0, 0.0))
+
+
+def serving_requests(rng, n, many_view, box_view, only_restir_1080p=False, small_only=False):
+    """The serving stream's request mix: resolution uniform over {256^2, 512^2, 1280x720,
+    1920x1080}, spp = round(2^U[0,6]), PathTracer (MIS, depth 4) or ReSTIR (M = 32, 1 temporal +
+    3 spatial) with equal probability, scene uniform over the resident ones: the box (config 0),
+    the many-lights scene seen from its camera (config 1) and from a random point of a 40 m
+    sphere (config 3's cameras)."""
+    sizes = [(256, 256), (512, 512)] if small_only else [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+    reqs = []
+    for _ in range(n):
+        if only_restir_1080p:
+            w, h, integ, scene = 1920, 1080, "restir", 1
+        else:
+            w, h = sizes[int(rng.integers(len(sizes)))]
+            integ = "pt" if rng.uniform() < 0.5 else "restir"
+            scene = int(rng.integers(3))
+        spp = int(round(2.0 ** rng.uniform(0.0, 6.0)))
+        if scene == 0:
+            view = box_view
+        elif scene == 1:
+            view = many_view
+        else:
+            view = look_at_origin_from_sphere(rng, 40.0)
+        reqs.append(dict(w=w, h=h, spp=spp, integ=integ, scene=scene, view=view))
+    return reqs
+
+
+def run_server(gpu, abi, ctx, gscenes, reqs, batched, arrivals=None, images=False):
+    """Feeds `reqs` to a server (all at t = 0, or at the given arrival times in seconds) and steps
+    it until every request has finished. Returns (makespan s, latencies ms, images by index)."""
+    from hdrestir_amd import scenes
+    pipes = {"pt": abi.pipeline("mis", max_depth=4),
+             "restir": abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=1,
+                                    spatial_count=3, spatial_radius=30.0)}
+    srv = gpu.Server(ctx, batched=batched, max_active=64, hbm_budget_bytes=48 << 30)
+    ids, lat, imgs = {}, [], {}
+    t0 = time.perf_counter()
+    nxt = 0
+    order = list(range(len(reqs)))
+    while len(lat) < len(reqs):
+        now = time.perf_counter() - t0
+        while nxt < len(order) and (arrivals is None or arrivals[nxt] <= now):
+            q = reqs[order[nxt]]
+            proj = scenes.perspective(50.0, q["w"] / q["h"], 0.05, 1000.0)
+            fr = abi.frame(q["view"], proj, 0)
+            ids[srv.submit(gscenes[q["scene"]], q["w"], q["h"], q["spp"], fr, pipes[q["integ"]])] = order[nxt]
+            nxt += 1
+        st = srv.stats()
+        if st["queued"] == 0 and st["active"] == 0 and arrivals is not None and nxt < len(order):
+            time.sleep(max(0.0, min(arrivals[nxt] - (time.perf_counter() - t0), 0.01)))
+            continue
+        srv.step()
+        done_t = time.perf_counter() - t0
+        for res, img in srv.results(images=images):
+            k = ids[res["id"]]
+            arrival = 0.0 if arrivals is None else arrivals[k]
+            lat.append((done_t - arrival) * 1000.0)
+            if images:
+                imgs[k] = img
+    makespan = time.perf_counter() - t0
+    srv.close()
+    return makespan, np.array(lat), imgs
+
+
+def serving_measurement(gpu, abi, ctx, sc_many, g_many, n_requests, seed=1):
+    """Config-2 serving: batched (continuous batching) vs the plain per-request path."""
+    from hdrestir_amd import scenes
+    box = scenes.box_spheres(256, 256, frames=1)
+    g_box = gpu.GpuScene(ctx, box)
+    gscenes = {0: g_box, 1: g_many, 2: g_many}
+    many_view, box_view = sc_many.cameras[0][0], box.cameras[0][0]
+    out = {"requests": n_requests, "scenes": "box (config 0), many-lights (config 1), many-lights from a "
+           "40 m sphere (config 3 cameras); the last two share one resident upload",
+           "mix": "resolution U{256^2, 512^2, 1280x720, 1920x1080}, spp round(2^U[0,6]), PathTracer depth 4 or "
+                  "ReSTIR M=32 1 temporal + 3 spatial (50/50)"}
+    # same images alone and in the mix: small requests, images compared exactly
+    rng = np.random.default_rng(seed + 100)
+    check = [q for q in serving_requests(rng, 40, many_view, box_view) if q["w"] <= 512][:10]
+    _, _, ia = run_server(gpu, abi, ctx, gscenes, check, batched=True, images=True)
+    _, _, ib = run_server(gpu, abi, ctx, gscenes, check, batched=False, images=True)
+    out["identical_images_batched_vs_plain"] = all(np.array_equal(ia[k], ib[k]) for k in range(len(check)))
+    # saturated throughput on the same request list
+    for name, only, small in (("mixed", False, False), ("restir_1080p", True, False), ("small", False, True)):
+        rng = np.random.default_rng(seed)
+        count = max(8, n_requests // 4) if only else (n_requests * 4 if small else n_requests)
+        reqs = serving_requests(rng, count, many_view, box_view, only, small)
+        mb, lb, _ = run_server(gpu, abi, ctx, gscenes, reqs, batched=True)
+        mp, lp, _ = run_server(gpu, abi, ctx, gscenes, reqs, batched=False)
+        out[name] = {"requests": len(reqs), "batched_requests_per_s": round(len(reqs) / mb, 3),
+                     "plain_requests_per_s": round(len(reqs) / mp, 3), "speedup": round(mp / mb, 3),
+                     "batched_p50_ms": round(float(np.percentile(lb, 50)), 2),
+                     "batched_p99_ms": round(float(np.percentile(lb, 99)), 2),
+                     "plain_p50_ms": round(float(np.percentile(lp, 50)), 2),
+                     "plain_p99_ms": round(float(np.percentile(lp, 99)), 2)}
+        _log(f"serving {name}: {out[name]}")
+    # Poisson arrivals
+    out["poisson"] = {}
+    for lam in (50, 200, 800):
+        rng = np.random.default_rng(seed + lam)
+        reqs = serving_requests(rng, n_requests, many_view, box_view)
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, size=len(reqs)))
+        mb, lb, _ = run_server(gpu, abi, ctx, gscenes, reqs, batched=True, arrivals=arrivals)
+        out["poisson"][str(lam)] = {"requests_per_s": round(len(reqs) / mb, 3),
+                                    "p50_ms": round(float(np.percentile(lb, 50)), 2),
+                                    "p99_ms": round(float(np.percentile(lb, 99)), 2)}
+        _log(f"serving poisson {lam}/s: {out['poisson'][str(lam)]}")
+    g_box.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +214,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES bytes 4 times (the reference "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
+    ap.add_argument("--no-serving", action="store_true", help="skip the request-serving measurement")
+    ap.add_argument("--serving-requests", type=int, default=48)
     ap.add_argument("--dolly", action="store_true",
                     help="also time the moving-camera variant (each move clears reservoirs and accumulation)")
     args = ap.parse_args()
@@ -254,6 +375,9 @@ def main():
         "image_mean_rgb": [round(float(v), 3) for v in img[..., :3].reshape(-1, 3).mean(0)],
     }
     r.close()
+    if not args.no_serving:
+        _log("serving")
+        line["serving"] = serving_measurement(gpu, abi, ctx, sc, gs, args.serving_requests)
     gs.close()
     ctx.close()
     if not args.no_cpu_baseline:

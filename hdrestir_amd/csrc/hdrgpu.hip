@@ -12,7 +12,7 @@
 
 #include "../../include/hdrgpu.h"
 #include "host_scene.h"
-#include "deferred.h"
+#include "pathlean.h"
 
 using namespace hdr;
 
@@ -159,6 +159,27 @@ __global__ void __launch_bounds__(kLanes) integrate_kernel(const FrameArgs *__re
     PixelStats st = {};
     uint32_t draws = 0;
     if (valid) integrate_pixel<kCount, kDebug>(a, a.gbuf, x, y, wl + lane, st, draws);
+    if (kCount) {
+        unsigned long long *c = a.counters;
+        wave_add(c + 1, st.bounce);
+        wave_add(c + 2, st.shadow);
+        wave_add(c + 3, st.cand);
+        wave_add(c + 4, st.cand_skipped);
+        wave_add(c + 5, st.ts.nodes);
+        wave_add(c + 6, st.ts.tris);
+        wave_add(c + 7, st.ts.instances);
+        wave_add(c + 8, draws);
+    }
+}
+
+// The path tracer on opaque frames whose pixels stay below RNG output 311 (pathlean.h).
+template <bool kCount, bool kDebug>
+__global__ void __launch_bounds__(kLanes) pt_lean_kernel(const FrameArgs *__restrict__ args,
+                                                         const int *__restrict__ tile_base, int n_views) {
+    HDR_TILE(args, tile_base, n_views)
+    PixelStats st = {};
+    uint32_t draws = 0;
+    if (valid) pt_lean_pixel<kCount, kDebug>(a, x, y, st, draws);
     if (kCount) {
         unsigned long long *c = a.counters;
         wave_add(c + 1, st.bounce);
@@ -793,6 +814,7 @@ static hdr_status ensure(DevBuf &b, size_t bytes) {
 // Launch plan of one view's frame.
 struct FramePlan {
     bool deferred = false;  // the deferred-shadow path (else the integration megakernel)
+    bool lean_pt = false;   // the inlined path tracer (pathlean.h) instead of the megakernel
     bool spatial = false;   // spatial reuse after the temporal pass (deferred path only)
     size_t lds = 0;         // dynamic LDS of the shading / integration kernel
     unsigned tiles = 0;     // 8x8 tiles of the view
@@ -927,6 +949,7 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
     }
     a.view = 0;
     plan.deferred = deferred;
+    plan.lean_pt = !deferred && a.direct == HDR_DIRECT_MIS && !scene->any_translucent && bound < kMtN - 1;
     plan.spatial = spatial;
     plan.lds = lds;
     plan.tiles = unsigned(((W + 7) / 8) * ((H + 7) / 8));
@@ -1019,6 +1042,16 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_ACCUMULATE], view->stream));
         ts.ran |= (1u << HDR_STAGE_INTEGRATE) | (1u << HDR_STAGE_SHADOW) | (1u << HDR_STAGE_ACCUMULATE);
+    } else if (plan.lean_pt) {
+#define HDR_PT(C, D) hipLaunchKernelGGL((pt_lean_kernel<C, D>), grid, dim3(kLanes), 0, view->stream, da, dt, 1)
+        if (view->counting) {
+            if (debug_out) HDR_PT(true, true);
+            else HDR_PT(true, false);
+        } else {
+            if (debug_out) HDR_PT(false, true);
+            else HDR_PT(false, false);
+        }
+#undef HDR_PT
     } else if (view->counting) {
         if (debug_out) hipLaunchKernelGGL((integrate_kernel<true, true>), grid, dim3(kLanes), lds, view->stream, da, dt, 1);
         else hipLaunchKernelGGL((integrate_kernel<true, false>), grid, dim3(kLanes), lds, view->stream, da, dt, 1);
@@ -1302,13 +1335,14 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
     }
     // launch groups: every view (G-buffer, shadow), deferred views (shading, accumulation), their
     // spatial-reuse subset, and megakernel views; each gets its own argument array and tile table
-    enum { kAll, kDef, kSpa, kMega, kGroups };
+    enum { kAll, kDef, kSpa, kMega, kLean, kGroups };
     std::vector<int> members[kGroups];
     for (int i = 0; i < n; ++i) {
         members[kAll].push_back(i);
         if (plans[i].deferred) members[kDef].push_back(i);
         if (plans[i].deferred && plans[i].spatial) members[kSpa].push_back(i);
-        if (!plans[i].deferred) members[kMega].push_back(i);
+        if (!plans[i].deferred && !plans[i].lean_pt) members[kMega].push_back(i);
+        if (plans[i].lean_pt) members[kLean].push_back(i);
     }
     auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
     size_t off = 0, argOff[kGroups], tileOff[kGroups];
@@ -1371,6 +1405,9 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
     if (N(kMega) > 0)
         hipLaunchKernelGGL((integrate_kernel<false, false>), dim3(tiles[kMega]), dim3(kLanes), lds[kMega], b->stream,
                            A(kMega), T(kMega), N(kMega));
+    if (N(kLean) > 0)
+        hipLaunchKernelGGL((pt_lean_kernel<false, false>), dim3(tiles[kLean]), dim3(kLanes), 0, b->stream, A(kLean),
+                           T(kLean), N(kLean));
     if (N(kDef) > 0) {
         const int sblocks = int(std::min<size_t>((queueCap + kLanes - 1) / kLanes, 8192));
         hipLaunchKernelGGL((shadow_kernel<false>), dim3(sblocks), dim3(kLanes), 0, b->stream, A(kAll));
