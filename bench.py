@@ -215,6 +215,8 @@ def main():
                     help="only run a plain device-to-device copy of BYTES bytes 4 times (the reference "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
     ap.add_argument("--no-serving", action="store_true", help="skip the request-serving measurement")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the counting pass and the timed frames of the headline pipeline (for profilers)")
     ap.add_argument("--serving-requests", type=int, default=48)
     ap.add_argument("--dolly", action="store_true",
                     help="also time the moving-camera variant (each move clears reservoirs and accumulation)")
@@ -284,18 +286,20 @@ def main():
     img = r.read_color()
 
     # the reference's own pipeline (temporal reuse only), same frames
-    r.reset()
-    for i in range(args.warmup):
-        r.render(frame_desc(i, False), pipe_temporal, read_color=False)
-    r.synchronize()
-    t2 = time.perf_counter()
-    for i in range(args.steps):
-        r.render(frame_desc(args.warmup + i, False), pipe_temporal, read_color=False)
-    r.synchronize()
-    temporal_fps = args.steps / (time.perf_counter() - t2)
+    temporal_fps = None
+    if not args.headline_only:
+        r.reset()
+        for i in range(args.warmup):
+            r.render(frame_desc(i, False), pipe_temporal, read_color=False)
+        r.synchronize()
+        t2 = time.perf_counter()
+        for i in range(args.steps):
+            r.render(frame_desc(args.warmup + i, False), pipe_temporal, read_color=False)
+        r.synchronize()
+        temporal_fps = args.steps / (time.perf_counter() - t2)
 
     dolly_fps = None
-    if args.dolly:  # the camera moves every frame, which clears reservoirs and accumulation
+    if args.dolly and not args.headline_only:  # a camera move clears reservoirs and accumulation
         t2 = time.perf_counter()
         nd = min(args.steps, len(sc.cameras))
         for i in range(nd):
@@ -365,7 +369,7 @@ def main():
         "rays_per_frame": {"primary": counts["primary_rays"], "bounce": counts["bounce_rays"],
                            "shadow": counts["shadow_rays"], "bsdf_candidate_traced": counts["candidate_rays"],
                            "skipped_cannot_contribute": counts["candidate_rays_skipped"]},
-        "temporal_only_frames_per_s": round(temporal_fps, 3),
+        "temporal_only_frames_per_s": None if temporal_fps is None else round(temporal_fps, 3),
         "dolly_frames_per_s": None if dolly_fps is None else round(dolly_fps, 3),
         "kernels": kernels,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2),
@@ -375,7 +379,7 @@ def main():
         "image_mean_rgb": [round(float(v), 3) for v in img[..., :3].reshape(-1, 3).mean(0)],
     }
     r.close()
-    if not args.no_serving:
+    if not args.no_serving and not args.headline_only:
         _log("serving")
         line["serving"] = serving_measurement(gpu, abi, ctx, sc, gs, args.serving_requests)
     gs.close()
