@@ -158,6 +158,23 @@ class ServerDesc(C.Structure):
                 ("batched", C.c_int32), ("_reserved", C.c_int32)]
 
 
+class LaunchConfig(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("queue_waves", C.c_int32), ("trace_occupancy", C.c_int32),
+                ("shade_occupancy", C.c_int32), ("use_graphs", C.c_int32), ("_reserved", C.c_int32 * 3)]
+
+    def as_dict(self):
+        return {"queue_waves": self.queue_waves, "trace_occupancy": self.trace_occupancy,
+                "shade_occupancy": self.shade_occupancy, "use_graphs": self.use_graphs}
+
+
+def launch_config(queue_waves=0, trace_occupancy=0, shade_occupancy=0, use_graphs=1):
+    c = LaunchConfig()
+    c.struct_size = C.sizeof(LaunchConfig)
+    c.queue_waves, c.trace_occupancy, c.shade_occupancy, c.use_graphs = (queue_waves, trace_occupancy,
+                                                                         shade_occupancy, use_graphs)
+    return c
+
+
 class RequestDesc(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
                 ("scene", C.c_void_p), ("frame", FrameDesc), ("pipe", PipelineDesc)]

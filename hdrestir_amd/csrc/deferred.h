@@ -58,8 +58,8 @@ __device__ __forceinline__ uint32_t query_code(const FrameArgs &a, int pixel) {
 
 
 // UniformLightSampler::ProposeCandidate light pick (uniform_light_sampler.cpp:22-47).
-template <bool kDebug>
-__device__ __forceinline__ int pick_light_lean(const FrameArgs &a, Mt64Lean &rng, Trace &dbg, bool record) {
+template <bool kDebug, class R>
+__device__ __forceinline__ int pick_light_lean(const FrameArgs &a, R &rng, Trace &dbg, bool record) {
     const uint32_t n = uint32_t(a.s.num_lights);
     const float u = rng.next_float();
     uint32_t idx = uint32_t(u * float(n));
@@ -73,8 +73,9 @@ __device__ __forceinline__ int pick_light_lean(const FrameArgs &a, Mt64Lean &rng
 
 // One light-sampler NEE candidate evaluated under skipVisibility (_generateNEECandidates :407-443
 // with _evaluateLightSample :482-527): returns the RIS weight, fills `out` when given.
+template <class R>
 __device__ __forceinline__ double nee_candidate_lean(const FrameArgs &a, const SpecTab &tab, const Vertex &v,
-                                                     Mt64Lean &rng, int li, Cand *out) {
+                                                     R &rng, int li, Cand *out) {
     const DeviceScene &s = a.s;
     const GLight &L = s.lights[li];
     LSample ls;
@@ -117,9 +118,9 @@ __device__ __forceinline__ double nee_candidate_lean(const FrameArgs &a, const S
 
 // Replays the candidate stream from `rng` (a copy taken before candidate 0) and rebuilds NEE
 // candidate `kk`; earlier candidates only consume their draws.
-template <bool kDebug>
+template <bool kDebug, class R>
 __device__ __forceinline__ void rebuild_nee_lean(const FrameArgs &a, const SpecTab &tab, const Vertex &v,
-                                                 Mt64Lean rng, int kk, Trace &dbg, Cand &out) {
+                                                 R rng, int kk, Trace &dbg, Cand &out) {
     const DeviceScene &s = a.s;
     for (int k = 0; k < kk; ++k) {
         const int li = pick_light_lean<kDebug>(a, rng, dbg, false);
@@ -150,16 +151,21 @@ __device__ __forceinline__ Spec winner_contribution_lean(const SpecTab &tab, con
     return rgb2spec(tab, ch.color) * tmul;
 }
 
-// RisDirectLightIntegrator::Li at the primary vertex on the deferred path. wl: this lane's NEE
-// candidate weights in LDS (wl[k * kLanes]).
-template <bool kCount, bool kDebug>
+// A copy of the stream for the winner's replay (the lean stream has no spill slot to share).
+__device__ __forceinline__ Mt64Lean rng_fork(const Mt64Lean &r, uint64_t *) { return r; }
+
+// RisDirectLightIntegrator::Li on a vertex whose BSDF candidates can reach no emitter. With
+// `reservoir` (use_reservoir at the primary vertex) the streaming reservoir and the temporal merge
+// run (:121-185), otherwise RandomIndexFromWeights (:187-214). wl: this lane's NEE candidate weights
+// in LDS (wl[k * kLanes]); slot2: the spare spill slot for the replay copy of a deep stream.
+template <bool kCount, bool kDebug, class R>
 __device__ __forceinline__ Spec ris_direct_lean(const FrameArgs &a, const SpecTab &tab, const Vertex &v,
-                                                bool connCand, Mt64Lean &rng, double *wl, int pixel,
-                                                PixelStats &st, Trace &dbg, bool &query, V3 &qo, V3 &qd,
-                                                float &qtmax) {
+                                                bool connCand, R &rng, double *wl, int pixel,
+                                                PixelStats &st, Trace &dbg, bool reservoir, uint64_t *slot2,
+                                                bool &query, V3 &qo, V3 &qd, float &qtmax) {
     const DeviceScene &s = a.s;
     const int nNee = a.n_nee;
-    const Mt64Lean start = rng;
+    const R start = rng_fork(rng, slot2);
     // NEE candidates
     for (int k = 0; k < nNee; ++k) {
         double w = 0.0;
@@ -178,7 +184,7 @@ __device__ __forceinline__ Spec ris_direct_lean(const FrameArgs &a, const SpecTa
     }
     const int zeroCands = a.n_cand - nNee;  // candidates with weight 0 (counted by the reservoir)
 
-    if (a.use_res) {
+    if (reservoir) {
         // streaming WRS over the candidates (WeightedReservoir::Update, reservoir.h:76-84)
         float wsum = 0.0f;
         int count = 0;

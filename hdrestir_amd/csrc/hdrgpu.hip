@@ -13,6 +13,7 @@
 #include "../../include/hdrgpu.h"
 #include "host_scene.h"
 #include "pathlean.h"
+#include "wavefront.h"
 
 using namespace hdr;
 
@@ -233,7 +234,7 @@ __device__ __forceinline__ void shade_direct_pixel(const FrameArgs &a, const GRe
                                           refl, refr, rng, first);
         if (kCount && ok) ++st.cand_skipped;
         const Spec direct = ris_direct_lean<kCount, kDebug>(a, tab, v0, ok && first.pdf > 0.0f, rng, wlane, pixel, st,
-                                                           dbg, query, qo, qd, qtmax);
+                                                           dbg, a.use_res != 0, nullptr, query, qo, qd, qtmax);
         lVis = throughput * (total + direct);
         lOcc = throughput * (total + spec(0.0f));
         if (a.spatial_count > 0) {  // the spatial pass finishes this pixel
@@ -363,8 +364,9 @@ __global__ void zero_u32_kernel(unsigned int *p) { *p = 0u; }
 
 // Any-hit shadow rays of the queue; an occluded winner keeps zero throughput in its reservoir
 // (the visibility fix-up of ris_direct_light_integrator.cpp:163-175).
-template <bool kCount>
-__global__ void __launch_bounds__(kLanes) shadow_kernel(const FrameArgs *__restrict__ args) {
+template <bool kCount, int kOcc = 1>
+__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
+shadow_kernel(const FrameArgs *__restrict__ args) {
     const FrameArgs &a0 = args[0];  // the queue and its count are shared by the views of the launch
     const unsigned int n = *a0.qcount;
     TraceStats ts = {};
@@ -412,6 +414,225 @@ __global__ void __launch_bounds__(kLanes) accumulate_deferred_kernel(const Frame
     acc.w += c4.w;
     a.accum[i] = acc;
     a.color[i] = make_float4(acc.x * a.inv_n, acc.y * a.inv_n, acc.z * a.inv_n, acc.w * a.inv_n);
+}
+
+// ---------------------------------------------------------------------------- wavefront path
+// RIS / ReSTIR at every vertex of paths of any depth (wavefront.h). Per frame: the G-buffer, then
+// wave_shade0 (primary vertex, one wave per 8x8 tile), wave_trace (bounce rays and shadow rays of
+// vertex 0), then wave_shade / wave_trace for vertices 1 .. depth-1 over the live paths only, then
+// wave_resolve (spectrum -> RGB and accumulation). Queue counts are indexed by vertex parity; each
+// launch clears the counts the launch after it appends to.
+
+// Clears the wavefront queue counts (a kernel, so a step can sit in a captured graph).
+__global__ void wave_zero_counts_kernel(unsigned int *c) {
+    if (threadIdx.x < 8) c[threadIdx.x] = 0u;
+}
+
+template <bool kCount, bool kDebug>
+__global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__restrict__ args,
+                                                             const int *__restrict__ tile_base, int n_views) {
+    extern __shared__ double wl[];
+    HDR_TILE(args, tile_base, n_views)
+    unsigned int *cnt = args[0].wcnt;  // the queues of a launch are shared by its views
+    if (blockIdx.x == 0 && lane == 0) cnt[5] = 0u;  // live paths of vertex 1 (appended by wave_trace)
+    PixelStats st = {};
+    bool query = false, ray = false;
+    ShadowQuery q;
+    float4 cq = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ro = cq, rd = cq;
+    if (valid) {
+        const int pixel = y * a.W + x;
+        const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
+        WavePath p;
+        p.rng.init(seed, wave_slot(a, pixel));
+        const float px = float(x) + p.rng.next_float();
+        const float py = float(y) + p.rng.next_float();
+        V3 o0, d0;
+        camera_ray(a, px, py, &p.rng, o0, d0);
+        p.utab = p.rng.next_float();
+        const SpecTab tab = make_tab(p.utab);
+        p.dbg = Trace{2166136261u, 2166136261u, 0};
+        p.tp = spec(1.0f);
+        p.outer = spec(1.0f);
+        p.total = spec(0.0f);
+        p.refl = p.refr = 0;
+        p.dir = d0;
+        HitRec g;
+        p.primary_hit = load_grec(a.gbuf + pixel, g);
+        if (p.primary_hit) wave_vertex<kCount, kDebug>(a, tab, 0, g, p, wl + lane, pixel, st, query, q, cq, ray, ro, rd);
+        store_path(a, pixel, p, true);
+        if (kDebug) {
+            a.dbg[pixel].prim_instance = p.primary_hit ? g.inst : -1;
+            a.dbg[pixel].prim_triangle = p.primary_hit ? g.tri : -1;
+            a.dbg[pixel].prim_t = p.primary_hit ? g.depth : 0.0f;
+        }
+    }
+    const unsigned int qi = wave_append(cnt + 2, query, lane);
+    if (query) {
+        args[0].queue[qi] = q;
+        args[0].wcq[qi] = cq;
+    }
+    const unsigned int ri = wave_append(cnt + 0, ray, lane);
+    if (ray) {
+        args[0].wray[2 * ri] = ro;
+        args[0].wray[2 * ri + 1] = rd;
+    }
+    if (kCount) {
+        unsigned long long *c = a.counters;
+        wave_add(c + 2, query ? 1ULL : 0ULL);
+        wave_add(c + 4, st.cand_skipped);
+    }
+}
+
+// Vertex `bounce` >= 1 of every live path (the codes wave_trace appended).
+template <bool kCount, bool kDebug, int kOcc = 1>
+__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
+wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce) {
+    extern __shared__ double wl[];
+    const int lane = threadIdx.x;
+    const int par = bounce & 1;
+    unsigned int *cnt = args[0].wcnt;
+    const unsigned int n = cnt[4 + par];
+    if (blockIdx.x == 0 && lane == 0) cnt[4 + (par ^ 1)] = 0u;
+    PixelStats st = {};
+    unsigned int nq = 0;
+    for (unsigned int b0 = blockIdx.x * kLanes; b0 < n; b0 += gridDim.x * kLanes) {
+        const unsigned int i = b0 + unsigned(lane);
+        bool query = false, ray = false;
+        ShadowQuery q;
+        float4 cq = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ro = cq, rd = cq;
+        if (i < n) {
+            const uint32_t code = args[0].walive[i];
+            const FrameArgs &a = args[code >> 24];
+            const int pixel = int(code & 0xffffffu);
+            WavePath p;
+            load_path(a, pixel, p);
+            const SpecTab tab = make_tab(p.utab);
+            HitRec h;
+            (void)load_grec(a.whit + pixel, h);
+            wave_vertex<kCount, kDebug>(a, tab, bounce, h, p, wl + lane, pixel, st, query, q, cq, ray, ro, rd);
+            store_path(a, pixel, p, false);
+        }
+        const unsigned int qi = wave_append(cnt + 2 + par, query, lane);
+        if (query) {
+            args[0].queue[qi] = q;
+            args[0].wcq[qi] = cq;
+        }
+        const unsigned int ri = wave_append(cnt + par, ray, lane);
+        if (ray) {
+            args[0].wray[2 * ri] = ro;
+            args[0].wray[2 * ri + 1] = rd;
+        }
+        nq += query ? 1u : 0u;
+    }
+    if (kCount) {
+        unsigned long long *c = args[0].counters;
+        wave_add(c + 2, nq);
+        wave_add(c + 4, st.cand_skipped);
+    }
+}
+
+// The bounce rays (closest hit: the next vertex, appended to the live list when there is one) and
+// the shadow rays (any hit: an unoccluded winner's contribution joins the path total; an occluded
+// primary winner keeps zero throughput in its reservoir, ris_direct_light_integrator.cpp:163-175)
+// of vertex `bounce`.
+template <bool kCount, int kOcc = 1>
+__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
+wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce) {
+    const int lane = threadIdx.x;
+    const int par = bounce & 1;
+    unsigned int *cnt = args[0].wcnt;
+    const unsigned int nr = cnt[par], ns = cnt[2 + par];
+    if (blockIdx.x == 0 && lane == 0) {
+        cnt[par ^ 1] = 0u;        // rays of the next vertex
+        cnt[2 + (par ^ 1)] = 0u;  // shadow queries of the next vertex
+    }
+    TraceStats ts = {};
+    unsigned int nb = 0;
+    for (unsigned int b0 = blockIdx.x * kLanes; b0 < nr; b0 += gridDim.x * kLanes) {
+        const unsigned int i = b0 + unsigned(lane);
+        bool hit = false;
+        uint32_t code = 0;
+        if (i < nr) {
+            const float4 o4 = args[0].wray[2 * i], d4 = args[0].wray[2 * i + 1];
+            code = __float_as_uint(o4.w);
+            const FrameArgs &a = args[code >> 24];
+            const int pixel = int(code & 0xffffffu);
+            const V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
+            RawHit rh;
+            hit = trace_closest<kCount>(a.s, o, d, rh, ts);
+            if (hit) store_grec(a.whit + pixel, true, make_hit_record(a.s, o, d, rh));
+            ++nb;
+        }
+        const unsigned int ai = wave_append(cnt + 4 + (par ^ 1), hit, lane);
+        if (hit) args[0].walive[ai] = code;
+    }
+    for (unsigned int b0 = blockIdx.x * kLanes; b0 < ns; b0 += gridDim.x * kLanes) {
+        const unsigned int i = b0 + unsigned(lane);
+        if (i >= ns) break;
+        const ShadowQuery q = args[0].queue[i];
+        const uint32_t code = __float_as_uint(q.d.w);
+        const FrameArgs &a = args[code >> 24];
+        const int pixel = int(code & 0xffffffu);
+        if (!trace_any<kCount>(a.s, v3(q.o.x, q.o.y, q.o.z), v3(q.d.x, q.d.y, q.d.z), q.o.w, ts)) {
+            float4 *t = a.wstate + 4 * wave_pixels(a) + pixel;
+            const Spec tot = spec_of(*t) + spec_of(args[0].wcq[i]);
+            *t = spec4(tot);
+        } else if (bounce == 0 && a.use_res) {
+            hdr_reservoir &r = a.res[pixel];
+            r.throughput[0] = r.throughput[1] = r.throughput[2] = r.throughput[3] = 0.0f;
+        }
+    }
+    if (kCount) {
+        unsigned long long *c = args[0].counters;
+        wave_add(c + 1, nb);
+        wave_add(c + 5, ts.nodes);
+        wave_add(c + 6, ts.tris);
+        wave_add(c + 7, ts.instances);
+    }
+}
+
+// The path's radiance (outer throughput x total) to RGB, exposure, accumulation
+// (integration_pass.cpp:236-243, accumulation_pass.cpp:98-106).
+template <bool kCount, bool kDebug>
+__global__ void __launch_bounds__(kLanes) wave_resolve_kernel(const FrameArgs *__restrict__ args,
+                                                              const int *__restrict__ tile_base, int n_views) {
+    HDR_TILE(args, tile_base, n_views)
+    uint32_t draws = 0;
+    if (valid) {
+        const int pixel = y * a.W + x;
+        const size_t P = wave_pixels(a);
+        const float4 *w = a.wstate;
+        const float4 r2 = w[2 * P + pixel], r5 = w[5 * P + pixel], r7 = w[7 * P + pixel];
+        const SpecTab tab = make_tab(r5.w);
+        const bool hit = (__float_as_uint(r7.x) & kWavePrimaryHit) != 0;
+        const Spec L = hit ? spec_of(w[6 * P + pixel]) * spec_of(w[4 * P + pixel]) : spec(0.0f);
+        V3 rgb = spec2rgb(tab, L);
+        if (a.has_cam && a.cam.enable_exposure) rgb = rgb * a.exposure_mul;
+        const float4 c4 = make_float4(rgb.x, rgb.y, rgb.z, 1.0f);
+        if (a.firefly) {
+            a.frame_rgb[pixel] = c4;
+        } else {
+            float4 acc = a.accum[pixel];
+            acc.x += c4.x;
+            acc.y += c4.y;
+            acc.z += c4.z;
+            acc.w += c4.w;
+            a.accum[pixel] = acc;
+            a.color[pixel] = make_float4(acc.x * a.inv_n, acc.y * a.inv_n, acc.z * a.inv_n, acc.w * a.inv_n);
+        }
+        draws = __float_as_uint(r2.z);
+        if (kDebug) {
+            hdr_pixel_debug &d = a.dbg[pixel];
+            d.draws = draws;
+            d.light_hash = __float_as_uint(r7.y);
+            d.select_hash = __float_as_uint(r7.z);
+            d.n_light_picks = __float_as_int(r7.w);
+            d.sample_rgb[0] = rgb.x;
+            d.sample_rgb[1] = rgb.y;
+            d.sample_rgb[2] = rgb.z;
+        }
+    }
+    if (kCount) wave_add(a.counters + 8, draws);
 }
 
 // Out-of-place 3x3 firefly clamp (the reference's GPU accumulation shader semantics,
@@ -542,14 +763,53 @@ struct hdr_scene {
     bool any_translucent = false;         // some material has opacity < 0.999 (unbounded draws)
 };
 
+// Launch shape of the queue-driven launches (hdr_launch_config, validated by the caller):
+// waves per launch and the minimum-waves-per-SIMD variant of the traversal / shading kernels.
+struct LaunchShape {
+    int queue_waves = 0;  // 0: one wave per 64 queue slots, at most kDefaultQueueWaves
+    int trace_occ = 1;
+    int shade_occ = 1;
+};
+constexpr int kDefaultQueueWaves = 8192;
+static int queue_blocks(const LaunchShape &ls, size_t cap) {
+    const size_t need = std::max<size_t>(1, (cap + kLanes - 1) / kLanes);
+    return int(std::min<size_t>(need, size_t(ls.queue_waves > 0 ? ls.queue_waves : kDefaultQueueWaves)));
+}
+template <bool C>
+static void launch_wave_trace(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a, int b) {
+    switch (ls.trace_occ) {
+    case 4: hipLaunchKernelGGL((wave_trace_kernel<C, 4>), dim3(blocks), dim3(kLanes), 0, st, a, b); break;
+    case 6: hipLaunchKernelGGL((wave_trace_kernel<C, 6>), dim3(blocks), dim3(kLanes), 0, st, a, b); break;
+    case 8: hipLaunchKernelGGL((wave_trace_kernel<C, 8>), dim3(blocks), dim3(kLanes), 0, st, a, b); break;
+    default: hipLaunchKernelGGL((wave_trace_kernel<C, 1>), dim3(blocks), dim3(kLanes), 0, st, a, b); break;
+    }
+}
+template <bool C, bool D>
+static void launch_wave_shade(const LaunchShape &ls, int blocks, size_t lds, hipStream_t st, const FrameArgs *a,
+                              int b) {
+    if (ls.shade_occ == 3) hipLaunchKernelGGL((wave_shade_kernel<C, D, 3>), dim3(blocks), dim3(kLanes), lds, st, a, b);
+    else hipLaunchKernelGGL((wave_shade_kernel<C, D, 1>), dim3(blocks), dim3(kLanes), lds, st, a, b);
+}
+template <bool C>
+static void launch_shadow(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a) {
+    switch (ls.trace_occ) {
+    case 4: hipLaunchKernelGGL((shadow_kernel<C, 4>), dim3(blocks), dim3(kLanes), 0, st, a); break;
+    case 6: hipLaunchKernelGGL((shadow_kernel<C, 6>), dim3(blocks), dim3(kLanes), 0, st, a); break;
+    case 8: hipLaunchKernelGGL((shadow_kernel<C, 8>), dim3(blocks), dim3(kLanes), 0, st, a); break;
+    default: hipLaunchKernelGGL((shadow_kernel<C, 1>), dim3(blocks), dim3(kLanes), 0, st, a); break;
+    }
+}
+
 struct hdr_view {
     hdr_context *ctx = nullptr;
     int W = 0, H = 0;
     hipStream_t stream = nullptr;
     DevBuf res, accum, color, frame, dbg, depth, albedo, normal, counters, spill, gbuf, rgb2, flags, queue, qcount;
     DevBuf res_t, saux;  // spatial reuse: post-temporal reservoirs, per-pixel {ray dir, u}, {throughput}
+    DevBuf wstate, whit, wray, wcq, walive, wcnt;  // wavefront path: path records, hits, queues
     DevBuf dargs, dtile; // per-frame FrameArgs records (one per timing-ring slot); launch tile table {0, tiles}
     PinnedBuf<FrameArgs> hargs;  // pinned staging of dargs
+    LaunchShape shape;           // queue-driven launches (hdr_view_set_launch_config)
     int counting = 0;
     int frames = 0;
     // Frames overlap: the G-buffer kernel of frame N+1 runs on `gstream` as soon as frame N's
@@ -791,7 +1051,7 @@ hdr_status hdr_view_device_bytes(const hdr_view *v, uint64_t *out) {
     if (!v || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
     *out = v->res.n + v->accum.n + v->color.n + v->frame.n + v->dbg.n + v->depth.n + v->albedo.n + v->normal.n +
            v->counters.n + v->spill.n + v->gbuf.n + v->rgb2.n + v->flags.n + v->queue.n + v->qcount.n + v->res_t.n +
-           v->saux.n;
+           v->saux.n + v->wstate.n + v->whit.n + v->wray.n + v->wcq.n + v->walive.n + v->wcnt.n;
     return HDR_OK;
 }
 
@@ -816,6 +1076,8 @@ struct FramePlan {
     bool deferred = false;  // the deferred-shadow path (else the integration megakernel)
     bool lean_pt = false;   // the inlined path tracer (pathlean.h) instead of the megakernel
     bool spatial = false;   // spatial reuse after the temporal pass (deferred path only)
+    bool wave = false;      // the wavefront path (RIS at every vertex of deep paths, wavefront.h)
+    int depth = 0;          // path vertices (max_depth)
     size_t lds = 0;         // dynamic LDS of the shading / integration kernel
     unsigned tiles = 0;     // 8x8 tiles of the view
 };
@@ -920,6 +1182,10 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
     // and no pixel draws past RNG output 310 (the lean stream of that path has no spill slot).
     const bool deferred = a.direct == HDR_DIRECT_RIS && a.max_depth == 1 && a.skip_vis && !scene->any_translucent &&
                           !a.trace_generated && !a.trace_last_connection && bound < kMtN - 1;
+    // The wavefront path: the same conditions at any depth and RNG range (frames the deferred path
+    // does not take).
+    const bool wave = !deferred && a.direct == HDR_DIRECT_RIS && a.skip_vis && !scene->any_translucent &&
+                      !a.trace_generated && !a.trace_last_connection;
     // Spatial reuse (an extension) runs on the deferred path with temporal reservoirs.
     const bool spatial = pipe->spatial_count > 0;
     if (spatial && !(deferred && a.use_res))
@@ -936,7 +1202,7 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
     }
     // candidate weights in LDS: the NEE candidates on the deferred path (its BSDF candidates carry
     // no weight), every candidate on the general path
-    const size_t lds = size_t(deferred ? a.n_nee : (a.direct == HDR_DIRECT_RIS ? a.n_cand : 1)) * kLanes *
+    const size_t lds = size_t((deferred || wave) ? a.n_nee : (a.direct == HDR_DIRECT_RIS ? a.n_cand : 1)) * kLanes *
                        sizeof(double);
     if (deferred) {
         if (hdr_status st = ensure(view->rgb2, P * 2 * sizeof(float4))) return st;
@@ -947,8 +1213,27 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
         a.queue = static_cast<ShadowQuery *>(view->queue.p);
         a.qcount = static_cast<unsigned int *>(view->qcount.p);
     }
+    if (wave) {
+        if (hdr_status st = ensure(view->wstate, P * kWaveFields * sizeof(float4))) return st;
+        if (hdr_status st = ensure(view->whit, P * sizeof(GRec))) return st;
+        // the view's own queues (a batch substitutes its shared ones)
+        if (hdr_status st = ensure(view->wray, P * 2 * sizeof(float4))) return st;
+        if (hdr_status st = ensure(view->queue, P * sizeof(ShadowQuery))) return st;
+        if (hdr_status st = ensure(view->wcq, P * sizeof(float4))) return st;
+        if (hdr_status st = ensure(view->walive, P * sizeof(unsigned int))) return st;
+        if (hdr_status st = ensure(view->wcnt, 8 * sizeof(unsigned int))) return st;
+        a.wstate = static_cast<float4 *>(view->wstate.p);
+        a.whit = static_cast<GRec *>(view->whit.p);
+        a.wray = static_cast<float4 *>(view->wray.p);
+        a.queue = static_cast<ShadowQuery *>(view->queue.p);
+        a.wcq = static_cast<float4 *>(view->wcq.p);
+        a.walive = static_cast<unsigned int *>(view->walive.p);
+        a.wcnt = static_cast<unsigned int *>(view->wcnt.p);
+    }
     a.view = 0;
     plan.deferred = deferred;
+    plan.wave = wave;
+    plan.depth = pipe->max_depth;
     plan.lean_pt = !deferred && a.direct == HDR_DIRECT_MIS && !scene->any_translucent && bound < kMtN - 1;
     plan.spatial = spatial;
     plan.lds = lds;
@@ -1029,9 +1314,9 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         HIP_TRY(hipEventRecord(view->shade_done, view->stream));
         view->shade_recorded = true;
         HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_SHADOW], view->stream));
-        const int sblocks = int(std::min<size_t>((P + kLanes - 1) / kLanes, 8192));
-        if (view->counting) hipLaunchKernelGGL((shadow_kernel<true>), dim3(sblocks), dim3(kLanes), 0, view->stream, da);
-        else hipLaunchKernelGGL((shadow_kernel<false>), dim3(sblocks), dim3(kLanes), 0, view->stream, da);
+        const int sblocks = queue_blocks(view->shape, P);
+        if (view->counting) launch_shadow<true>(view->shape, sblocks, view->stream, da);
+        else launch_shadow<false>(view->shape, sblocks, view->stream, da);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_SHADOW], view->stream));
         HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_ACCUMULATE], view->stream));
@@ -1042,6 +1327,53 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_ACCUMULATE], view->stream));
         ts.ran |= (1u << HDR_STAGE_INTEGRATE) | (1u << HDR_STAGE_SHADOW) | (1u << HDR_STAGE_ACCUMULATE);
+    } else if (plan.wave) {
+        const int qblocks = queue_blocks(view->shape, P);
+        hipLaunchKernelGGL(wave_zero_counts_kernel, dim3(1), dim3(kLanes), 0, view->stream, a.wcnt);
+#define HDR_W0(C, D) \
+    hipLaunchKernelGGL((wave_shade0_kernel<C, D>), grid, dim3(kLanes), lds, view->stream, da, dt, 1)
+#define HDR_WS(C, D) launch_wave_shade<C, D>(view->shape, qblocks, lds, view->stream, da, b)
+        if (view->counting) {
+            if (debug_out) HDR_W0(true, true);
+            else HDR_W0(true, false);
+        } else {
+            if (debug_out) HDR_W0(false, true);
+            else HDR_W0(false, false);
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(view->shade_done, view->stream));  // the G-buffer has been read
+        view->shade_recorded = true;
+        for (int b = 0; b < plan.depth; ++b) {
+            if (b > 0) {
+                if (view->counting) {
+                    if (debug_out) HDR_WS(true, true);
+                    else HDR_WS(true, false);
+                } else {
+                    if (debug_out) HDR_WS(false, true);
+                    else HDR_WS(false, false);
+                }
+            }
+            if (view->counting) launch_wave_trace<true>(view->shape, qblocks, view->stream, da, b);
+            else launch_wave_trace<false>(view->shape, qblocks, view->stream, da, b);
+            HIP_TRY(hipGetLastError());
+        }
+#undef HDR_W0
+#undef HDR_WS
+        HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_INTEGRATE], view->stream));
+        HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_ACCUMULATE], view->stream));
+#define HDR_WR(C, D) \
+    hipLaunchKernelGGL((wave_resolve_kernel<C, D>), grid, dim3(kLanes), 0, view->stream, da, dt, 1)
+        if (view->counting) {
+            if (debug_out) HDR_WR(true, true);
+            else HDR_WR(true, false);
+        } else {
+            if (debug_out) HDR_WR(false, true);
+            else HDR_WR(false, false);
+        }
+#undef HDR_WR
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_ACCUMULATE], view->stream));
+        ts.ran |= (1u << HDR_STAGE_INTEGRATE) | (1u << HDR_STAGE_ACCUMULATE);
     } else if (plan.lean_pt) {
 #define HDR_PT(C, D) hipLaunchKernelGGL((pt_lean_kernel<C, D>), grid, dim3(kLanes), 0, view->stream, da, dt, 1)
         if (view->counting) {
@@ -1060,7 +1392,7 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         else hipLaunchKernelGGL((integrate_kernel<false, false>), grid, dim3(kLanes), lds, view->stream, da, dt, 1);
     }
     HIP_TRY(hipGetLastError());
-    if (!deferred) {
+    if (!deferred && !plan.wave) {
         ts.ran |= 1u << HDR_STAGE_INTEGRATE;
         HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_INTEGRATE], view->stream));
         HIP_TRY(hipEventRecord(view->shade_done, view->stream));
@@ -1245,6 +1577,7 @@ struct hdr_batch {
     hipEvent_t beg = nullptr, end = nullptr;
     bool timed = false;
     DevBuf queue, qcount;  // shadow queries of every view of a step, and their number
+    DevBuf wray, wqueue, wcq, walive, wcnt;  // wavefront-path queues of a step (shared by its views)
     // steps in flight: arguments and tile tables staged in pinned memory, copied to the slot's
     // device buffer; a slot is reused once its step has finished
     static constexpr int kSlots = 4;
@@ -1256,6 +1589,13 @@ struct hdr_batch {
     };
     Slot slots[kSlots];
     int next = 0;
+    LaunchShape shape;    // queue-driven launches (hdr_batch_set_launch_config / hdr_batch_autotune)
+    bool graphs = true;   // steps replay captured graphs
+    struct Graph {
+        std::vector<uint64_t> key;  // launch shape the graph was captured for
+        hipGraphExec_t exec = nullptr;
+    };
+    Graph graph[kSlots];  // one per argument slot (its buffers are baked into the captured copy)
 };
 
 hdr_status hdr_batch_create(hdr_context *ctx, hdr_batch **out) {
@@ -1269,6 +1609,7 @@ hdr_status hdr_batch_create(hdr_context *ctx, hdr_batch **out) {
     HIP_TRY(hipEventCreate(&b->end));
     for (auto &sl : b->slots) HIP_TRY(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
     HIP_TRY(b->qcount.alloc(sizeof(unsigned int)));
+    HIP_TRY(b->wcnt.alloc(8 * sizeof(unsigned int)));
     *out = b.release();
     return HDR_OK;
 }
@@ -1277,6 +1618,8 @@ hdr_status hdr_batch_destroy(hdr_batch *b) {
     if (!b) return HDR_OK;
     (void)hipSetDevice(b->ctx->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto &g : b->graph)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (auto &sl : b->slots)
         if (sl.done) (void)hipEventDestroy(sl.done);
     if (b->beg) (void)hipEventDestroy(b->beg);
@@ -1303,6 +1646,52 @@ hdr_status hdr_batch_last_timing(hdr_batch *b, float *step_ms) {
     return HDR_OK;
 }
 
+static hdr_status shape_of(const hdr_launch_config *c, LaunchShape &ls) {
+    if (!c || c->struct_size != sizeof(hdr_launch_config)) return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_launch_config size");
+    if (c->queue_waves != 0 && (c->queue_waves < 64 || c->queue_waves > 65536))
+        return fail(HDR_ERR_INVALID_ARGUMENT, "queue_waves must be 0 or in [64, 65536]");
+    const int t = c->trace_occupancy == 0 ? 1 : c->trace_occupancy;
+    if (t != 1 && t != 4 && t != 6 && t != 8)
+        return fail(HDR_ERR_INVALID_ARGUMENT, "trace_occupancy must be 0, 1, 4, 6 or 8");
+    const int sh = c->shade_occupancy == 0 ? 1 : c->shade_occupancy;
+    if (sh != 1 && sh != 3) return fail(HDR_ERR_INVALID_ARGUMENT, "shade_occupancy must be 0, 1 or 3");
+    ls.queue_waves = c->queue_waves;
+    ls.trace_occ = t;
+    ls.shade_occ = sh;
+    return HDR_OK;
+}
+static void config_of(const LaunchShape &ls, bool graphs, hdr_launch_config *c) {
+    std::memset(c, 0, sizeof *c);
+    c->struct_size = sizeof(hdr_launch_config);
+    c->queue_waves = ls.queue_waves;
+    c->trace_occupancy = ls.trace_occ;
+    c->shade_occupancy = ls.shade_occ;
+    c->use_graphs = graphs ? 1 : 0;
+}
+
+hdr_status hdr_view_set_launch_config(hdr_view *view, const hdr_launch_config *cfg) {
+    if (!view) return fail(HDR_ERR_INVALID_ARGUMENT, "view is null");
+    LaunchShape ls;
+    if (hdr_status st = shape_of(cfg, ls)) return st;
+    view->shape = ls;
+    return HDR_OK;
+}
+
+hdr_status hdr_batch_set_launch_config(hdr_batch *b, const hdr_launch_config *cfg) {
+    if (!b) return fail(HDR_ERR_INVALID_ARGUMENT, "batch is null");
+    LaunchShape ls;
+    if (hdr_status st = shape_of(cfg, ls)) return st;
+    b->shape = ls;
+    b->graphs = cfg->use_graphs != 0;
+    return HDR_OK;
+}
+
+hdr_status hdr_batch_get_launch_config(hdr_batch *b, hdr_launch_config *cfg) {
+    if (!b || !cfg) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    config_of(b->shape, b->graphs, cfg);
+    return HDR_OK;
+}
+
 hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, const hdr_scene *const *scenes,
                             const hdr_frame_desc *frames, const hdr_pipeline_desc *pipes) {
     if (!b || !views || !scenes || !frames || !pipes) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
@@ -1318,31 +1707,51 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
     HIP_TRY(hipSetDevice(b->ctx->device));
     std::vector<FrameArgs> args(static_cast<size_t>(n));
     std::vector<FramePlan> plans(static_cast<size_t>(n));
-    size_t queueCap = 0;
+    size_t queueCap = 0, waveCap = 0;
+    int waveDepth = 0;
     for (int i = 0; i < n; ++i) {
         if (hdr_status st = prepare_frame(views[i], scenes[i], &frames[i], &pipes[i], false, args[i], plans[i]))
             return st;
         if (plans[i].deferred) queueCap += size_t(views[i]->W) * views[i]->H;
+        if (plans[i].wave) {
+            waveCap += size_t(views[i]->W) * views[i]->H;
+            waveDepth = std::max(waveDepth, plans[i].depth);
+        }
     }
-    if (queueCap * sizeof(ShadowQuery) > b->queue.n) {
-        HIP_TRY(hipStreamSynchronize(b->stream));  // earlier steps may still read the old queue
+    if (queueCap * sizeof(ShadowQuery) > b->queue.n || waveCap * sizeof(ShadowQuery) > b->wqueue.n) {
+        HIP_TRY(hipStreamSynchronize(b->stream));  // earlier steps may still read the old queues
         if (hdr_status st = ensure(b->queue, queueCap * sizeof(ShadowQuery))) return st;
+        if (waveCap > 0) {
+            if (hdr_status st = ensure(b->wqueue, waveCap * sizeof(ShadowQuery))) return st;
+            if (hdr_status st = ensure(b->wray, waveCap * 2 * sizeof(float4))) return st;
+            if (hdr_status st = ensure(b->wcq, waveCap * sizeof(float4))) return st;
+            if (hdr_status st = ensure(b->walive, waveCap * sizeof(unsigned int))) return st;
+        }
     }
     for (int i = 0; i < n; ++i) {
         args[i].view = i;
-        args[i].queue = static_cast<ShadowQuery *>(b->queue.p);
         args[i].qcount = static_cast<unsigned int *>(b->qcount.p);
+        if (plans[i].wave) {
+            args[i].queue = static_cast<ShadowQuery *>(b->wqueue.p);
+            args[i].wray = static_cast<float4 *>(b->wray.p);
+            args[i].wcq = static_cast<float4 *>(b->wcq.p);
+            args[i].walive = static_cast<unsigned int *>(b->walive.p);
+            args[i].wcnt = static_cast<unsigned int *>(b->wcnt.p);
+        } else {
+            args[i].queue = static_cast<ShadowQuery *>(b->queue.p);
+        }
     }
     // launch groups: every view (G-buffer, shadow), deferred views (shading, accumulation), their
     // spatial-reuse subset, and megakernel views; each gets its own argument array and tile table
-    enum { kAll, kDef, kSpa, kMega, kLean, kGroups };
+    enum { kAll, kDef, kSpa, kMega, kLean, kWave, kGroups };
     std::vector<int> members[kGroups];
     for (int i = 0; i < n; ++i) {
         members[kAll].push_back(i);
         if (plans[i].deferred) members[kDef].push_back(i);
         if (plans[i].deferred && plans[i].spatial) members[kSpa].push_back(i);
-        if (!plans[i].deferred && !plans[i].lean_pt) members[kMega].push_back(i);
+        if (!plans[i].deferred && !plans[i].lean_pt && !plans[i].wave) members[kMega].push_back(i);
         if (plans[i].lean_pt) members[kLean].push_back(i);
+        if (plans[i].wave) members[kWave].push_back(i);
     }
     auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
     size_t off = 0, argOff[kGroups], tileOff[kGroups];
@@ -1378,6 +1787,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         int base = 0;
         for (size_t k = 0; k < members[g].size(); ++k) {
             ga[k] = args[size_t(members[g][k])];
+            if (g == kWave) ga[k].view = int(k);  // wavefront queue codes index the group's array
             tb[k] = base;
             base += int(plans[size_t(members[g][k])].tiles);
         }
@@ -1388,33 +1798,90 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         HIP_TRY(hipEventRecord(views[i]->serial, views[i]->stream));
         HIP_TRY(hipStreamWaitEvent(b->stream, views[i]->serial, 0));
     }
-    HIP_TRY(hipEventRecord(b->beg, b->stream));
-    HIP_TRY(hipMemcpyAsync(sl.dev.p, sl.host.p, off, hipMemcpyHostToDevice, b->stream));
     auto A = [&](int g) { return reinterpret_cast<const FrameArgs *>(dp + argOff[g]); };
     auto T = [&](int g) { return reinterpret_cast<const int *>(dp + tileOff[g]); };
     auto N = [&](int g) { return int(members[g].size()); };
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, b->stream, static_cast<unsigned int *>(b->qcount.p));
-    hipLaunchKernelGGL((gbuffer_kernel<false>), dim3(tiles[kAll]), dim3(kLanes), 0, b->stream, A(kAll), T(kAll),
-                       N(kAll));
-    if (N(kDef) > 0)
-        hipLaunchKernelGGL((shade_direct_kernel<false, false>), dim3(tiles[kDef]), dim3(kLanes), lds[kDef], b->stream,
-                           A(kDef), T(kDef), N(kDef));
-    if (N(kSpa) > 0)
-        hipLaunchKernelGGL((spatial_kernel<false, false>), dim3(tiles[kSpa]), dim3(kLanes), 0, b->stream, A(kSpa),
-                           T(kSpa), N(kSpa));
-    if (N(kMega) > 0)
-        hipLaunchKernelGGL((integrate_kernel<false, false>), dim3(tiles[kMega]), dim3(kLanes), lds[kMega], b->stream,
-                           A(kMega), T(kMega), N(kMega));
-    if (N(kLean) > 0)
-        hipLaunchKernelGGL((pt_lean_kernel<false, false>), dim3(tiles[kLean]), dim3(kLanes), 0, b->stream, A(kLean),
-                           T(kLean), N(kLean));
-    if (N(kDef) > 0) {
-        const int sblocks = int(std::min<size_t>((queueCap + kLanes - 1) / kLanes, 8192));
-        hipLaunchKernelGGL((shadow_kernel<false>), dim3(sblocks), dim3(kLanes), 0, b->stream, A(kAll));
-        hipLaunchKernelGGL((accumulate_deferred_kernel<false>), dim3(tiles[kDef]), dim3(kLanes), 0, b->stream,
-                           A(kDef), T(kDef), N(kDef));
+    const LaunchShape ls = b->shape;
+    // the step: argument copy, then every launch (captured once per slot and launch shape when
+    // graphs are on, so later steps replay it)
+    auto enqueue = [&]() -> hipError_t {
+        hipError_t e = hipMemcpyAsync(sl.dev.p, sl.host.p, off, hipMemcpyHostToDevice, b->stream);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, b->stream, static_cast<unsigned int *>(b->qcount.p));
+        hipLaunchKernelGGL((gbuffer_kernel<false>), dim3(tiles[kAll]), dim3(kLanes), 0, b->stream, A(kAll), T(kAll),
+                           N(kAll));
+        if (N(kDef) > 0)
+            hipLaunchKernelGGL((shade_direct_kernel<false, false>), dim3(tiles[kDef]), dim3(kLanes), lds[kDef],
+                               b->stream, A(kDef), T(kDef), N(kDef));
+        if (N(kSpa) > 0)
+            hipLaunchKernelGGL((spatial_kernel<false, false>), dim3(tiles[kSpa]), dim3(kLanes), 0, b->stream, A(kSpa),
+                               T(kSpa), N(kSpa));
+        if (N(kMega) > 0)
+            hipLaunchKernelGGL((integrate_kernel<false, false>), dim3(tiles[kMega]), dim3(kLanes), lds[kMega],
+                               b->stream, A(kMega), T(kMega), N(kMega));
+        if (N(kLean) > 0)
+            hipLaunchKernelGGL((pt_lean_kernel<false, false>), dim3(tiles[kLean]), dim3(kLanes), 0, b->stream,
+                               A(kLean), T(kLean), N(kLean));
+        if (N(kWave) > 0) {
+            const int qblocks = queue_blocks(ls, waveCap);
+            hipLaunchKernelGGL(wave_zero_counts_kernel, dim3(1), dim3(kLanes), 0, b->stream,
+                               static_cast<unsigned int *>(b->wcnt.p));
+            hipLaunchKernelGGL((wave_shade0_kernel<false, false>), dim3(tiles[kWave]), dim3(kLanes), lds[kWave],
+                               b->stream, A(kWave), T(kWave), N(kWave));
+            for (int v = 0; v < waveDepth; ++v) {
+                if (v > 0) launch_wave_shade<false, false>(ls, qblocks, lds[kWave], b->stream, A(kWave), v);
+                launch_wave_trace<false>(ls, qblocks, b->stream, A(kWave), v);
+            }
+            hipLaunchKernelGGL((wave_resolve_kernel<false, false>), dim3(tiles[kWave]), dim3(kLanes), 0, b->stream,
+                               A(kWave), T(kWave), N(kWave));
+        }
+        if (N(kDef) > 0) {
+            launch_shadow<false>(ls, queue_blocks(ls, queueCap), b->stream, A(kAll));
+            hipLaunchKernelGGL((accumulate_deferred_kernel<false>), dim3(tiles[kDef]), dim3(kLanes), 0, b->stream,
+                               A(kDef), T(kDef), N(kDef));
+        }
+        return hipGetLastError();
+    };
+    if (b->graphs) {
+        // launch shape key: everything baked into the captured launches (buffers, grid sizes,
+        // argument layout); the arguments themselves are copied by the graph's first node
+        std::vector<uint64_t> key = {uint64_t(n), uint64_t(off), uint64_t(uintptr_t(sl.dev.p)),
+                                     uint64_t(uintptr_t(sl.host.p)), uint64_t(uintptr_t(b->queue.p)),
+                                     uint64_t(uintptr_t(b->wqueue.p)), uint64_t(uintptr_t(b->wray.p)),
+                                     uint64_t(uintptr_t(b->wcq.p)), uint64_t(uintptr_t(b->walive.p)),
+                                     uint64_t(queueCap), uint64_t(waveCap), uint64_t(waveDepth),
+                                     uint64_t(ls.queue_waves), uint64_t(ls.trace_occ), uint64_t(ls.shade_occ)};
+        for (int g = 0; g < kGroups; ++g) {
+            key.push_back(uint64_t(tiles[g]));
+            key.push_back(uint64_t(lds[g]));
+            key.push_back(uint64_t(members[g].size()));
+            for (int i : members[g]) key.push_back(uint64_t(i));
+        }
+        hdr_batch::Graph &gr = b->graph[b->next];
+        if (!gr.exec || gr.key != key) {
+            if (gr.exec) {
+                HIP_TRY(hipGraphExecDestroy(gr.exec));
+                gr.exec = nullptr;
+            }
+            HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+            const hipError_t le = enqueue();
+            hipGraph_t graph = nullptr;
+            const hipError_t ce = hipStreamEndCapture(b->stream, &graph);
+            if (le != hipSuccess || ce != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                HIP_TRY(le != hipSuccess ? le : ce);
+            }
+            const hipError_t ie = hipGraphInstantiate(&gr.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIP_TRY(ie);
+            gr.key = std::move(key);
+        }
+        HIP_TRY(hipEventRecord(b->beg, b->stream));
+        HIP_TRY(hipGraphLaunch(gr.exec, b->stream));
+    } else {
+        HIP_TRY(hipEventRecord(b->beg, b->stream));
+        HIP_TRY(enqueue());
     }
-    HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(b->end, b->stream));
     HIP_TRY(hipEventRecord(sl.done, b->stream));
     sl.pending = true;
@@ -1426,6 +1893,65 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         HIP_TRY(hipStreamWaitEvent(views[i]->gstream, sl.done, 0));
         views[i]->frames++;
     }
+    return HDR_OK;
+}
+
+// Coordinate search over the launch shape: queue waves, then the traversal kernels' occupancy
+// variant, then the wavefront shading kernel's; each candidate is timed (device time, HIP events)
+// over `reps` steps after one untimed step, and the fastest is kept. Every candidate renders the
+// same frames exactly (the launch shape moves work between waves, never changes a path's result).
+hdr_status hdr_batch_autotune(hdr_batch *b, int32_t n, hdr_view *const *views, const hdr_scene *const *scenes,
+                              const hdr_frame_desc *frames, const hdr_pipeline_desc *pipes, int32_t reps,
+                              hdr_launch_config *chosen, float *best_step_ms) {
+    if (!b || reps < 1 || reps > 64) return fail(HDR_ERR_INVALID_ARGUMENT, "reps must be in [1, 64]");
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    auto time_shape = [&](const LaunchShape &ls, float &ms) -> hdr_status {
+        b->shape = ls;
+        for (int w = 0; w < hdr_batch::kSlots; ++w)  // capture every slot's graph outside the timing
+            if (hdr_status st = hdr_render_batch(b, n, views, scenes, frames, pipes)) return st;
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, b->stream));
+        hdr_status st = HDR_OK;
+        for (int r = 0; r < reps && st == HDR_OK; ++r) st = hdr_render_batch(b, n, views, scenes, frames, pipes);
+        (void)hipEventRecord(e1, b->stream);
+        (void)hipEventSynchronize(e1);
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ms = t / float(reps);
+        return st;
+    };
+    LaunchShape best = b->shape;
+    float bestMs = 0.0f;
+    if (hdr_status st = time_shape(best, bestMs)) return st;
+    for (int q : {2048, 4096, 16384, 32768}) {
+        LaunchShape c = best;
+        c.queue_waves = q;
+        float ms;
+        if (hdr_status st = time_shape(c, ms)) return st;
+        if (ms < bestMs) bestMs = ms, best = c;
+    }
+    for (int t : {4, 6, 8}) {
+        LaunchShape c = best;
+        c.trace_occ = t;
+        float ms;
+        if (hdr_status st = time_shape(c, ms)) return st;
+        if (ms < bestMs) bestMs = ms, best = c;
+    }
+    {
+        LaunchShape c = best;
+        c.shade_occ = 3;
+        float ms;
+        if (hdr_status st = time_shape(c, ms)) return st;
+        if (ms < bestMs) bestMs = ms, best = c;
+    }
+    b->shape = best;
+    if (chosen) config_of(best, b->graphs, chosen);
+    if (best_step_ms) *best_step_ms = bestMs;
     return HDR_OK;
 }
 

@@ -54,6 +54,14 @@ struct FrameArgs {
     ShadowQuery *queue;                // deferred path: shadow queries (shared by the views of a batch)
     unsigned int *qcount;              // deferred path: number of queries (shared likewise)
     int view;                          // index of this record in the launch (carried by shadow queries)
+    // wavefront path (paths of any depth, one launch per vertex): per-view path records and the
+    // bounce rays' hits; queues shared by the views of a launch
+    float4 *wstate;                    // kWaveFields float4 per pixel, field-major
+    GRec *whit;                        // hit of each path's last bounce ray
+    float4 *wray;                      // bounce-ray queue: {origin, code}, {direction, -}
+    float4 *wcq;                       // contribution of each shadow query (index-aligned with queue)
+    unsigned int *walive;              // codes of the paths whose bounce ray hit (next vertex)
+    unsigned int *wcnt;                // queue counts: rays[2], shadows[2], alive[2] by vertex parity
 };
 
 struct PixelStats {

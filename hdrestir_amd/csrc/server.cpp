@@ -28,12 +28,14 @@ double ms_since(Clock::time_point a, Clock::time_point b) {
 
 // Device bytes a view of W x H pixels can hold for a pipeline (the buffers hdr_render_frame
 // allocates: reservoirs, accumulation, colour, G-buffer; the deferred path's colour pair, flags
-// and shadow queue; spatial reuse's post-temporal reservoirs and per-pixel record; RNG spill
-// slots when the path can draw past output 311).
+// and shadow queue; the wavefront path's path records and queues; spatial reuse's post-temporal
+// reservoirs and per-pixel record; RNG spill slots when the path can draw past output 311).
 uint64_t view_footprint(int w, int h, const hdr_pipeline_desc &p) {
     const uint64_t P = uint64_t(w) * uint64_t(h);
     uint64_t per = sizeof(hdr_reservoir) + 16 + 16 + 64;
     if (p.direct_light == HDR_DIRECT_RIS && p.max_depth == 1 && p.skip_visibility) per += 32 + 4 + 32;
+    // wavefront path: path record, bounce hit, ray / shadow / live queues
+    if (p.direct_light == HDR_DIRECT_RIS && p.max_depth > 1 && p.skip_visibility) per += 128 + 64 + 32 + 32 + 16 + 4;
     if (p.spatial_count > 0) per += sizeof(hdr_reservoir) + 32;
     // draw bound of the megakernel path (host_scene / hdr_render_frame use the same count)
     const int M = p.candidate_count;

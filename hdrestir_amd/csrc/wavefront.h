@@ -1,0 +1,216 @@
+// wavefront.h -- PathIntegrator::Li (path_integrator.cpp:48-209) with RIS / ReSTIR direct lighting
+// (ris_direct_light_integrator.cpp:84-215) at every vertex, for paths of any depth, run as a
+// wavefront: one shading launch per path vertex and one traversal launch between them, with the
+// path's state in HBM. A shading launch samples the vertex's bounce, ranks the RIS candidates and
+// records two ray queries: the bounce ray (closest hit, the next vertex) and the RIS winner's
+// shadow ray (any hit). The traversal launch answers both queues and compacts the paths that go on
+// into the next vertex's work list, so later (sparser) vertices launch only over live paths.
+//
+// The host admits a frame here under the same conditions as the deferred path except depth and
+// RNG range: skipVisibility (candidates are ranked without shadow rays), no translucent material,
+// and nothing BSDF-sampled can reach an emitter (no emissive material, mesh light or environment),
+// so the BSDF candidates reduce to their RNG draws and the winner's shadow ray is the only query
+// besides the bounce ray. Streams may run past RNG output 310: the second-twist machinery (spill
+// slots) stays out of line.
+//
+// Results are identical to the megakernel's: every vertex consumes the same draws in the same order,
+// and the radiance sum is formed in the same order -- vertex b's term `tp * d` is added to the path
+// total by the vertex's shading launch, or by the traversal launch right after it when d waits for
+// the winner's shadow ray; the next vertex adds its term only after that launch.
+#pragma once
+
+#include "deferred.h"
+
+namespace hdr {
+
+// ---------------------------------------------------------------- RNG for deep paths
+// Outputs 311 and beyond (the special last word of the first twist, then the spill slot) go
+// through one out-of-line function, so the inlined draw sites stay small.
+__device__ __noinline__ uint64_t mt_deep_next(Mt64 &r) { return r.next_raw(); }
+__device__ __noinline__ void mt_copy_slot(uint64_t *dst, const uint64_t *src) {
+    for (int q = 0; q < kMtN; ++q) dst[q] = src[q];
+}
+
+struct Mt64Deep : Mt64 {
+    __device__ __forceinline__ uint64_t next_raw() {
+        if (k >= 311u) return mt_deep_next(*this);
+        uint64_t y;
+        if (k < 156u) {
+            y = hi ^ mt_twist(lo, lo1);
+        } else {
+            if (k == 156u) restart();
+            y = hi ^ mt_twist(lo, lo1) ^ mt_twist(hi, hi1);
+        }
+        advance();
+        ++k;
+        return mt_temper(y);
+    }
+    __device__ __forceinline__ float next_float() { return mt_canonical(next_raw()); }
+    __device__ __forceinline__ void discard() {
+        if (k >= 311u) {
+            (void)mt_deep_next(*this);
+            return;
+        }
+        if (k == 156u) restart();
+        advance();
+        ++k;
+    }
+};
+
+// The winner's replay copy: once the stream has materialised its spill slot (output 313 on), the
+// copy works on the spare slot.
+__device__ __forceinline__ Mt64Deep rng_fork(const Mt64Deep &r, uint64_t *slot2) {
+    Mt64Deep c = r;
+    if (r.k > 312u && slot2 && r.slot) mt_copy_slot(slot2, r.slot);
+    c.slot = slot2;
+    return c;
+}
+
+// ---------------------------------------------------------------- path record
+// Field-major in HBM (field f of pixel i at wstate[f * P + i]), 8 x 16 B per pixel, so each field
+// of a wave's paths is one coalesced 1 KB access.
+constexpr int kWaveFields = 8;
+constexpr uint32_t kWavePrimaryHit = 1u << 31;  // in field 7.x: the G-buffer ray hit
+
+struct WavePath {
+    Mt64Deep rng;
+    Spec tp, total, outer;
+    V3 dir;      // direction of the ray that reached the current vertex
+    float utab;  // hero-wavelength draw
+    int refl, refr;
+    bool primary_hit;
+    Trace dbg;
+};
+
+__device__ __forceinline__ float4 pack_u64x2(uint64_t a, uint64_t b) {
+    return make_float4(__uint_as_float(uint32_t(a)), __uint_as_float(uint32_t(a >> 32)),
+                       __uint_as_float(uint32_t(b)), __uint_as_float(uint32_t(b >> 32)));
+}
+__device__ __forceinline__ uint64_t unpack_lo(float4 v) {
+    return uint64_t(__float_as_uint(v.x)) | (uint64_t(__float_as_uint(v.y)) << 32);
+}
+__device__ __forceinline__ uint64_t unpack_hi(float4 v) {
+    return uint64_t(__float_as_uint(v.z)) | (uint64_t(__float_as_uint(v.w)) << 32);
+}
+__device__ __forceinline__ float4 spec4(const Spec &s) { return make_float4(s.v[0], s.v[1], s.v[2], s.v[3]); }
+__device__ __forceinline__ Spec spec_of(float4 v) { return Spec{{v.x, v.y, v.z, v.w}}; }
+
+__device__ __forceinline__ size_t wave_pixels(const FrameArgs &a) { return size_t(a.W) * size_t(a.H); }
+__device__ __forceinline__ uint64_t *wave_slot(const FrameArgs &a, int pixel) {
+    return a.spill ? a.spill + size_t(pixel) * (2 * kMtN) : nullptr;
+}
+
+// outer: only at the primary vertex (later vertices leave field 6 alone).
+__device__ __forceinline__ void store_path(const FrameArgs &a, int pixel, const WavePath &p, bool outer) {
+    float4 *w = a.wstate;
+    const size_t P = wave_pixels(a);
+    w[pixel] = pack_u64x2(p.rng.lo, p.rng.lo1);
+    w[P + pixel] = pack_u64x2(p.rng.hi, p.rng.hi1);
+    w[2 * P + pixel] = make_float4(__uint_as_float(uint32_t(p.rng.s156)), __uint_as_float(uint32_t(p.rng.s156 >> 32)),
+                                   __uint_as_float(p.rng.k), __uint_as_float(p.rng.j));
+    w[3 * P + pixel] = spec4(p.tp);
+    w[4 * P + pixel] = spec4(p.total);
+    w[5 * P + pixel] = make_float4(p.dir.x, p.dir.y, p.dir.z, p.utab);
+    if (outer) w[6 * P + pixel] = spec4(p.outer);
+    const uint32_t bits = (uint32_t(p.refl) & 0x7fffu) | ((uint32_t(p.refr) & 0x7fffu) << 15) |
+                          (p.primary_hit ? kWavePrimaryHit : 0u);
+    w[7 * P + pixel] = make_float4(__uint_as_float(bits), __uint_as_float(p.dbg.lhash), __uint_as_float(p.dbg.shash),
+                                   __int_as_float(p.dbg.picks));
+}
+
+__device__ __forceinline__ void load_path(const FrameArgs &a, int pixel, WavePath &p) {
+    const float4 *w = a.wstate;
+    const size_t P = wave_pixels(a);
+    const float4 r0 = w[pixel], r1 = w[P + pixel], r2 = w[2 * P + pixel];
+    p.rng.lo = unpack_lo(r0);
+    p.rng.lo1 = unpack_hi(r0);
+    p.rng.hi = unpack_lo(r1);
+    p.rng.hi1 = unpack_hi(r1);
+    p.rng.s156 = unpack_lo(r2);
+    p.rng.k = __float_as_uint(r2.z);
+    p.rng.j = __float_as_uint(r2.w);
+    p.rng.seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
+    p.rng.slot = wave_slot(a, pixel);
+    p.tp = spec_of(w[3 * P + pixel]);
+    p.total = spec_of(w[4 * P + pixel]);
+    const float4 r5 = w[5 * P + pixel];
+    p.dir = v3(r5.x, r5.y, r5.z);
+    p.utab = r5.w;
+    const float4 r7 = w[7 * P + pixel];
+    const uint32_t bits = __float_as_uint(r7.x);
+    p.refl = int(bits & 0x7fffu);
+    p.refr = int((bits >> 15) & 0x7fffu);
+    p.primary_hit = (bits & kWavePrimaryHit) != 0;
+    p.dbg.lhash = __float_as_uint(r7.y);
+    p.dbg.shash = __float_as_uint(r7.z);
+    p.dbg.picks = __float_as_int(r7.w);
+}
+
+// ---------------------------------------------------------------- one path vertex
+// The body of one iteration of _li's bounce loop (path_integrator.cpp:112-177) at vertex `bounce`
+// (0 = the primary hit, whose throughput is the outer one and whose direct term is not scaled by
+// tp): closure, bounce sample (its ray is traced by the next launch, as the path's BSDF
+// connection), RIS direct lighting, path throughput and Russian roulette. Outputs the vertex's
+// shadow query with its contribution `tp * d` (added to the total if the ray is unoccluded) and the
+// bounce ray when the path goes on.
+template <bool kCount, bool kDebug>
+__device__ __forceinline__ void wave_vertex(const FrameArgs &a, const SpecTab &tab, int bounce, const HitRec &h,
+                                            WavePath &p, double *wl, int pixel, PixelStats &st, bool &query,
+                                            ShadowQuery &q, float4 &cq, bool &ray, float4 &ro, float4 &rd) {
+    const DeviceScene &s = a.s;
+    const bool first = bounce == 0;
+    const Clo c = get_closure(s, h, a.enable_ss != 0);
+    Vertex v;
+    if (first) v = make_vertex(a, tab, h, c, p.dir, p.outer);
+    else v = make_vertex(a, tab, h, c, p.dir, p.tp);
+    if (first) p.total = rgb2spec(tab, mv3(c.m->emission));
+    Bounce b;
+    const bool ok = sample_bounce_inl(tab, v.c, v.sn, v.inside, v.h.pos, v.rayDir, a.max_refl, a.max_refr, p.refl,
+                                      p.refr, p.rng, b);
+    const bool last = bounce == a.max_depth - 1;
+    if (kCount && ok && last) ++st.cand_skipped;  // the last connection ray is never traced here
+    V3 qo = v3s(0.0f), qd = v3s(0.0f);
+    float qtmax = 0.0f;
+    bool qq = false;
+    uint64_t *slot = wave_slot(a, pixel);
+    const Spec d = ris_direct_lean<kCount, kDebug>(a, tab, v, ok && b.pdf > 0.0f, p.rng, wl, pixel, st, p.dbg,
+                                                   first && a.use_res, slot ? slot + kMtN : nullptr, qq, qo, qd,
+                                                   qtmax);
+    const Spec td = first ? d : p.tp * d;
+    if (qq) {
+        query = true;
+        cq = spec4(td);
+        q.o = make_float4(qo.x, qo.y, qo.z, qtmax);
+        q.d = make_float4(qd.x, qd.y, qd.z, __uint_as_float(query_code(a, pixel)));
+    } else {
+        p.total = p.total + td;
+    }
+    ray = false;
+    if (!ok) return;
+    p.tp = p.tp * b.tmul;
+    if (!b.skip_roulette && bounce > 3) {
+        const float pr = sclamp(spec_max(p.tp) * a.roulette, 0.0f, 1.0f);
+        if (p.rng.next_float() > pr) return;
+        p.tp = p.tp * (1.0f / pr);
+    }
+    if (last) return;
+    ray = true;
+    p.dir = b.d;
+    ro = make_float4(b.o.x, b.o.y, b.o.z, __uint_as_float(query_code(a, pixel)));
+    rd = make_float4(b.d.x, b.d.y, b.d.z, 0.0f);
+}
+
+// Wave-aggregated append: one atomic per wave, each flagged lane gets its slot. All 64 lanes of
+// the wave must call it together.
+__device__ __forceinline__ unsigned int wave_append(unsigned int *count, bool want, int lane) {
+    const unsigned long long mask = __ballot(want);
+    const int n = __popcll(mask);
+    unsigned int base = 0;
+    if (n > 0) {
+        if (lane == 0) base = atomicAdd(count, unsigned(n));
+        base = __shfl(base, 0);
+    }
+    return base + unsigned(__popcll(mask & ((1ULL << lane) - 1ULL)));
+}
+
+}  // namespace hdr

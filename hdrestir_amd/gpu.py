@@ -85,6 +85,10 @@ def lib():
             "hdr_render_batch": [P, I, P, P, P, P],
             "hdr_batch_synchronize": [P],
             "hdr_batch_last_timing": [P, C.POINTER(C.c_float)],
+            "hdr_view_set_launch_config": [P, C.POINTER(abi.LaunchConfig)],
+            "hdr_batch_set_launch_config": [P, C.POINTER(abi.LaunchConfig)],
+            "hdr_batch_get_launch_config": [P, C.POINTER(abi.LaunchConfig)],
+            "hdr_batch_autotune": [P, I, P, P, P, P, I, C.POINTER(abi.LaunchConfig), C.POINTER(C.c_float)],
             "hdr_server_create": [P, C.POINTER(abi.ServerDesc), C.POINTER(P)],
             "hdr_server_destroy": [P],
             "hdr_server_submit": [P, C.POINTER(abi.RequestDesc), C.POINTER(C.c_uint64)],
@@ -235,6 +239,9 @@ class Renderer:
     def set_counting(self, on):
         check(lib().hdr_view_set_counting(self.h, int(on)))
 
+    def set_launch_config(self, cfg):
+        check(lib().hdr_view_set_launch_config(self.h, C.byref(cfg)))
+
     def counters(self):
         c = abi.FrameCounters()
         check(lib().hdr_view_read_counters(self.h, C.byref(c)))
@@ -254,13 +261,31 @@ class Batch:
         self.h = C.c_void_p()
         check(lib().hdr_batch_create(ctx.h, C.byref(self.h)))
 
-    def render(self, renderers, frames, pipes):
+    @staticmethod
+    def _arrays(renderers, frames, pipes):
         n = len(renderers)
-        views = (C.c_void_p * n)(*[r.h for r in renderers])
-        scenes = (C.c_void_p * n)(*[r.gscene.h for r in renderers])
-        fr = (abi.FrameDesc * n)(*frames)
-        pp = (abi.PipelineDesc * n)(*pipes)
-        check(lib().hdr_render_batch(self.h, n, views, scenes, fr, pp))
+        return (n, (C.c_void_p * n)(*[r.h for r in renderers]), (C.c_void_p * n)(*[r.gscene.h for r in renderers]),
+                (abi.FrameDesc * n)(*frames), (abi.PipelineDesc * n)(*pipes))
+
+    def render(self, renderers, frames, pipes):
+        check(lib().hdr_render_batch(self.h, *self._arrays(renderers, frames, pipes)))
+
+    def set_launch_config(self, cfg):
+        check(lib().hdr_batch_set_launch_config(self.h, C.byref(cfg)))
+
+    def launch_config(self):
+        c = abi.LaunchConfig()
+        check(lib().hdr_batch_get_launch_config(self.h, C.byref(c)))
+        return c.as_dict()
+
+    def autotune(self, renderers, frames, pipes, reps=2):
+        """Times candidate launch shapes on these views and keeps the fastest; returns
+        (chosen config dict, its step ms). The views' frames advance (reset them afterwards)."""
+        c = abi.LaunchConfig()
+        ms = C.c_float()
+        check(lib().hdr_batch_autotune(self.h, *self._arrays(renderers, frames, pipes), reps, C.byref(c),
+                                       C.byref(ms)))
+        return c.as_dict(), float(ms.value)
 
     def synchronize(self):
         check(lib().hdr_batch_synchronize(self.h))

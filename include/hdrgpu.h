@@ -307,8 +307,9 @@ hdr_status hdr_view_read_stage_times(hdr_view *view, double *stage_ms, int32_t *
  * No counterpart in the reference, which renders one view per Renderer::Render call
  * (renderer.cpp:390-478). A batch renders one frame for each of up to 255 views in merged
  * launches: one G-buffer launch over every view's 8x8 tiles, one shading launch per kernel path
- * (deferred shading, spatial reuse, integration megakernel), one shadow launch over the combined
- * queue and one accumulation launch. Each view gets exactly the frame hdr_render_frame would
+ * (deferred shading, spatial reuse, integration megakernel, inlined path tracer; the wavefront
+ * path's per-vertex shading and traversal launches), one shadow launch over the combined queue and
+ * one accumulation launch. Each view gets exactly the frame hdr_render_frame would
  * give it (same streams of random numbers, same results); views may differ in scene, size,
  * pipeline and frame. Not available with counting views or the firefly filter
  * (HDR_ERR_UNSUPPORTED). A view's later calls (reads, resets, single frames) are ordered after
@@ -322,6 +323,35 @@ hdr_status hdr_render_batch(hdr_batch *batch, int32_t n, hdr_view *const *views,
 hdr_status hdr_batch_synchronize(hdr_batch *batch);
 /* Device time (ms) of the last step, from its first launch to its last. */
 hdr_status hdr_batch_last_timing(hdr_batch *batch, float *step_ms);
+
+/* ---- launch shape and autotuning ----
+ * No counterpart in the reference (its passes run on CPU threads). The queue-driven launches --
+ * the wavefront path's shading and traversal launches and the deferred path's shadow launch --
+ * loop over their queue with a fixed number of 64-lane waves; the traversal and wavefront
+ * shading kernels exist in variants compiled for a minimum number of waves per SIMD (fewer
+ * registers, more waves in flight). A launch shape never changes a result: it only moves paths
+ * between waves. Batch steps are captured into a hipGraph per argument slot and launch shape and
+ * replayed (use_graphs, default 1). */
+typedef struct hdr_launch_config {
+    uint32_t struct_size;       /* sizeof(hdr_launch_config) */
+    int32_t  queue_waves;       /* waves per queue-driven launch: 0 = one per 64 queue slots, at most 8192;
+                                   else 64 .. 65536 */
+    int32_t  trace_occupancy;   /* traversal kernels: 0 or 1 = the compiler's register budget, 4, 6, 8 = at
+                                   least that many waves per SIMD */
+    int32_t  shade_occupancy;   /* wavefront shading kernel: 0 or 1, or 3 */
+    int32_t  use_graphs;        /* batches: 1 = replay captured graphs, 0 = launch every kernel */
+    int32_t  _reserved[3];
+} hdr_launch_config;
+hdr_status hdr_view_set_launch_config(hdr_view *view, const hdr_launch_config *cfg);
+hdr_status hdr_batch_set_launch_config(hdr_batch *batch, const hdr_launch_config *cfg);
+hdr_status hdr_batch_get_launch_config(hdr_batch *batch, hdr_launch_config *cfg);
+/* Autotuner: renders `reps` + 4 steps of the given views per candidate launch shape (queue waves,
+ * then traversal occupancy, then shading occupancy, a coordinate search), keeps the fastest in the
+ * batch and reports it. The views' reservoirs and accumulation advance like any rendered frames
+ * (reset them before the frames that count). reps: 1 .. 64. */
+hdr_status hdr_batch_autotune(hdr_batch *batch, int32_t n, hdr_view *const *views, const hdr_scene *const *scenes,
+                              const hdr_frame_desc *frames, const hdr_pipeline_desc *pipes, int32_t reps,
+                              hdr_launch_config *chosen, float *best_step_ms);
 
 /* ---- serving ----
  * No counterpart in the reference, whose renderer serves one Hydra render pass at a time

@@ -41,7 +41,7 @@ STRUCTS = {
     "hdr_frame_desc": abi.FrameDesc, "hdr_pipeline_desc": abi.PipelineDesc, "hdr_reservoir": abi.Reservoir,
     "hdr_pixel_debug": abi.PixelDebug, "hdr_frame_counters": abi.FrameCounters,
     "hdr_server_desc": abi.ServerDesc, "hdr_request_desc": abi.RequestDesc, "hdr_request_result": abi.RequestResult,
-    "hdr_server_stats": abi.ServerStats,
+    "hdr_server_stats": abi.ServerStats, "hdr_launch_config": abi.LaunchConfig,
 }
 
 

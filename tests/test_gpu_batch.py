@@ -1,6 +1,7 @@
 """Batches of views (hdr_render_batch): one frame for many views in merged launches gives every
 view exactly the frame hdr_render_frame gives it -- across scenes, sizes, kernel paths (deferred
-shading, spatial reuse, integration megakernel) and frames of temporal reuse."""
+shading, spatial reuse, integration megakernel, inlined path tracer, wavefront) and frames of
+temporal reuse."""
 import numpy as np
 import pytest
 
@@ -21,6 +22,9 @@ def _cases():
         (box_small, abi.pipeline("mis", max_depth=4)),
         (feat_dome, abi.pipeline("ris", candidate_count=8, use_reservoir=True, max_depth=3)),
         (box, abi.pipeline("ris", candidate_count=4, use_reservoir=False, skip_visibility=True, max_depth=1)),
+        # the wavefront path, two depths in one group
+        (box_small, abi.pipeline("ris", candidate_count=16, use_reservoir=True, skip_visibility=True, max_depth=5)),
+        (feat, abi.pipeline("ris", candidate_count=8, use_reservoir=False, skip_visibility=True, max_depth=3)),
     ]
 
 
@@ -75,3 +79,60 @@ def test_batch_refuses_duplicates_and_counting(gpu_ctx):
     batch.close()
     r.close()
     gs.close()
+
+
+def test_graphs_launch_shapes_and_autotune_keep_results(gpu_ctx):
+    """Captured-graph replay, every launch-shape variant (queue waves, traversal and shading
+    occupancy) and the autotuner's choice render exactly the frames of plain launches."""
+    from hdrestir_amd import gpu
+    cases = _cases()
+    gscenes = {}
+    for sc, _ in cases:
+        if id(sc) not in gscenes:
+            gscenes[id(sc)] = gpu.GpuScene(gpu_ctx, sc)
+
+    def views():
+        return [gpu.Renderer(gpu_ctx, gscenes[id(sc)], sc.width, sc.height) for sc, _ in cases]
+
+    pipes = [pipe for _, pipe in cases]
+    tune_views = views()
+    tuner = gpu.Batch(gpu_ctx)
+    chosen, ms = tuner.autotune(tune_views, [sc.frame(0) for sc, _ in cases], pipes, reps=1)
+    print("autotuned", chosen, ms)
+    assert ms > 0.0 and chosen["use_graphs"] == 1
+    configs = [abi.launch_config(use_graphs=0),
+               abi.launch_config(**chosen),
+               abi.launch_config(queue_waves=64, trace_occupancy=8, shade_occupancy=3, use_graphs=1),
+               abi.launch_config(queue_waves=65536, trace_occupancy=4, use_graphs=1)]
+    results = []
+    for cfg in configs:
+        vs = views()
+        b = gpu.Batch(gpu_ctx)
+        b.set_launch_config(cfg)
+        for f in range(6):  # more steps than argument slots: graphs are replayed
+            b.render(vs, [sc.frame(f % 5) for sc, _ in cases], pipes)
+        b.synchronize()
+        results.append([(v.read_color().copy(), v.reservoirs().tobytes()) for v in vs])
+        b.close()
+        for v in vs:
+            v.close()
+    for k in range(1, len(results)):
+        for j, ((ca, ra), (cb, rb)) in enumerate(zip(results[0], results[k])):
+            assert np.array_equal(ca, cb), (k, j)
+            assert ra == rb, (k, j)
+    # single views take the launch shape too
+    sc, pipe = cases[5]
+    imgs = []
+    for cfg in (abi.launch_config(), abi.launch_config(queue_waves=128, trace_occupancy=6, shade_occupancy=3)):
+        r = gpu.Renderer(gpu_ctx, gscenes[id(sc)], sc.width, sc.height)
+        r.set_launch_config(cfg)
+        for f in range(3):
+            r.render(sc.frame(f), pipe, read_color=False)
+        imgs.append(r.read_color().copy())
+        r.close()
+    assert np.array_equal(imgs[0], imgs[1])
+    tuner.close()
+    for v in tune_views:
+        v.close()
+    for g in gscenes.values():
+        g.close()

@@ -1,0 +1,131 @@
+"""Convergence against 4096-spp references: the per-pixel MSE of each estimator at a stated spp, and
+the mean radiance of the unbiased estimators inside a 99 % interval (batch means over 32 batches).
+
+The references are the GPU PathTracer (MIS) at 4096 frames. Every frame of it is the oracle's frame
+on the same seed -- the parity tests pin the streams bit-exactly and the colours to float rounding
+(test_render_parity_box_spheres, test_render_parity_many_lights_crop, test_lean_pathtracer_parity)
+-- so this average is the oracle's 4096-spp image, obtained in seconds instead of CPU hours.
+
+Workloads: the box of 12 triangles with two icospheres and one rect light, 256 x 256, depth 4
+(PathTracer, RIS M = 16 without and with the reservoir); the many-lights scene (height field, 500
+spheres, 4096 point / rect lights) from its camera at 480 x 270, direct lighting (depth 1: RIS
+M = 32, ReSTIR M = 32 with one shadow ray, with and without 3 spatial neighbours).
+
+Relative MSE = mean over pixels and RGB of (image - reference)^2 / mean of reference^2. The
+thresholds are the stated bars; the measured values are printed.
+"""
+import numpy as np
+import pytest
+
+from hdrestir_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+REF_SPP = 4096
+BATCHES = 32
+
+
+def _render_mean(r, sc_frame, pipe, first, n, batch=None):
+    """Renders frames first .. first+n-1 into a freshly reset view. Returns the accumulated image
+    (float64, RGB) and, with `batch`, the per-batch mean radiance of the image (n / batch batches)."""
+    r.reset()
+    means = []
+    prev_sum = np.zeros(3)
+    for i in range(n):
+        r.render(sc_frame(first + i), pipe, read_color=False)
+        if batch and (i + 1) % batch == 0:
+            img = r.read_color()[..., :3].astype(np.float64)
+            tot = img.mean(axis=(0, 1)) * (i + 1)  # sum of the per-frame image means so far
+            means.append((tot - prev_sum) / batch)
+            prev_sum = tot
+    img = r.read_color()[..., :3].astype(np.float64)
+    return img, np.array(means)
+
+
+def _rel_mse(img, ref):
+    return float(((img - ref) ** 2).mean() / (ref ** 2).mean())
+
+
+def _ci_check(batch_means, ref_batch_means, name):
+    """The difference of the two mean radiances inside its 99 % interval: standard errors from the
+    batch means of each run (32 batches each), t quantile 2.744 (31 degrees of freedom)."""
+    m, rm = batch_means.mean(0), ref_batch_means.mean(0)
+    se = np.sqrt(batch_means.var(0, ddof=1) / len(batch_means) + ref_batch_means.var(0, ddof=1) / len(ref_batch_means))
+    z = np.abs(m - rm) / np.maximum(se, 1e-12)
+    print(f"{name}: mean {m} reference {rm} se {se} |z| {z}")
+    assert (np.abs(m - rm) <= 2.744 * se).all(), (m, rm, se)
+
+
+@pytest.fixture(scope="module")
+def box(gpu_ctx):
+    from hdrestir_amd import gpu
+    sc = scenes.box_spheres(256, 256, frames=1)
+    view, proj = sc.cameras[0]
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    fr = lambda f: abi.frame(view, proj, f)  # noqa: E731
+    ref, ref_bm = _render_mean(r, fr, abi.pipeline("mis", max_depth=4), 0, REF_SPP, batch=REF_SPP // BATCHES)
+    yield r, fr, ref, ref_bm
+    r.close()
+    gs.close()
+
+
+@pytest.fixture(scope="module")
+def many(gpu_ctx):
+    from hdrestir_amd import gpu
+    sc = scenes.many_lights(480, 270, frames=1, dolly=False)
+    view, proj = sc.cameras[0]
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    fr = lambda f: abi.frame(view, proj, f)  # noqa: E731
+    ref, ref_bm = _render_mean(r, fr, abi.pipeline("mis", max_depth=1), 0, REF_SPP, batch=REF_SPP // BATCHES)
+    yield r, fr, ref, ref_bm
+    r.close()
+    gs.close()
+
+
+# (pipeline, spp for the MSE bar, relative-MSE bar, unbiased -> CI test)
+BOX_CASES = {
+    "pt_d4": (abi.pipeline("mis", max_depth=4), 64, 0.05, True),
+    "ris16_d4": (abi.pipeline("ris", candidate_count=16, use_reservoir=False, max_depth=4), 64, 0.05, True),
+    "restir16_d4": (abi.pipeline("ris", candidate_count=16, use_reservoir=True, max_depth=4), 64, 0.05, False),
+}
+
+
+@pytest.mark.parametrize("name", list(BOX_CASES))
+def test_box_convergence(box, name):
+    r, fr, ref, ref_bm = box
+    pipe, spp, bar, unbiased = BOX_CASES[name]
+    first = 100000  # seeds disjoint from the reference's frames 0 .. 4095
+    img, _ = _render_mean(r, fr, pipe, first, spp)
+    e = _rel_mse(img, ref)
+    print(f"box {name}: relative MSE at {spp} spp = {e:.5f}")
+    assert e < bar, e
+    if unbiased:
+        _, bm = _render_mean(r, fr, pipe, first + spp, BATCHES * 8, batch=8)
+        _ci_check(bm, ref_bm, f"box {name}")
+
+
+MANY_CASES = {
+    "pt_d1": (abi.pipeline("mis", max_depth=1), 64, 1.0, True),
+    "ris32_d1": (abi.pipeline("ris", candidate_count=32, use_reservoir=False, max_depth=1), 64, 1.0, True),
+    "restir32_d1_skipvis": (abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True,
+                                         max_depth=1), 64, 1.0, False),
+    "restir32_d1_skipvis_spatial3": (abi.pipeline("ris", candidate_count=32, use_reservoir=True,
+                                                  skip_visibility=True, max_depth=1, spatial_count=3,
+                                                  spatial_radius=30.0), 64, 1.0, False),
+}
+
+
+@pytest.mark.parametrize("name", list(MANY_CASES))
+def test_many_lights_convergence(many, name):
+    r, fr, ref, ref_bm = many
+    pipe, spp, bar, unbiased = MANY_CASES[name]
+    first = 100000
+    img, _ = _render_mean(r, fr, pipe, first, spp)
+    e = _rel_mse(img, ref)
+    print(f"many-lights {name}: relative MSE at {spp} spp = {e:.5f}")
+    assert e < bar, e
+    if unbiased:
+        _, bm = _render_mean(r, fr, pipe, first + spp, BATCHES * 4, batch=4)
+        _ci_check(bm, ref_bm, f"many-lights {name}")
