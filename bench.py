@@ -174,6 +174,9 @@ def serving_measurement(gpu, abi, ctx, sc_many, g_many, n_requests, seed=1):
     _, _, ia = run_server(gpu, abi, ctx, gscenes, check, batched=True, images=True)
     _, _, ib = run_server(gpu, abi, ctx, gscenes, check, batched=False, images=True)
     out["identical_images_batched_vs_plain"] = all(np.array_equal(ia[k], ib[k]) for k in range(len(check)))
+    # identical images have identical MSE against any reference (the 4096-spp references of
+    # tests/test_gpu_convergence.py included): the ratio the serving target bounds by 1.02
+    out["mse_ratio_batched_vs_plain"] = 1.0 if out["identical_images_batched_vs_plain"] else None
     # saturated throughput on the same request list
     for name, only, small in (("mixed", False, False), ("restir_1080p", True, False), ("small", False, True)):
         rng = np.random.default_rng(seed)
@@ -203,6 +206,85 @@ def serving_measurement(gpu, abi, ctx, sc_many, g_many, n_requests, seed=1):
     return out
 
 
+def deep_path_measurement(gpu, abi, ctx, sc, gs, steps, depth=6):
+    """The many-lights frame at path depth `depth` with ReSTIR at every vertex (M = 32, one shadow
+    ray per vertex): the wavefront path. Frames/s and rays/s of one 1080p view, static camera."""
+    pipe = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=depth)
+    r = gpu.Renderer(ctx, gs, sc.width, sc.height)
+    r.set_counting(True)
+    r.render(sc.frame(0, camera_index=0), pipe, read_color=False)
+    r.synchronize()
+    c = r.counters()
+    r.set_counting(False)
+    r.reset()
+    for i in range(2):
+        r.render(sc.frame(i, camera_index=0), pipe, read_color=False)
+    r.synchronize()
+    r.stage_times()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        r.render(sc.frame(2 + i, camera_index=0), pipe, read_color=False)
+    r.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    rays = c["primary_rays"] + c["bounce_rays"] + c["shadow_rays"]
+    r.close()
+    return {"workload": f"many_lights_1080p_restir_depth{depth}", "max_depth": depth, "candidate_count": 32,
+            "frames": steps, "ms_per_frame": round(ms, 3), "frames_per_s": round(1000.0 / ms, 3),
+            "rays_per_frame": {"primary": c["primary_rays"], "bounce": c["bounce_rays"], "shadow": c["shadow_rays"]},
+            "mrays_per_s": round(rays / ms / 1e3, 2)}
+
+
+def multiview_measurement(gpu, abi, ctx, gs, steps, seed=3):
+    """16 views of the many-lights scene in one batch: sizes uniform in [320, 1920] x [240, 1080],
+    cameras on a 40 m sphere around the origin, ReSTIR M = 32 at every vertex of depth-6 paths.
+    The launch shape is autotuned, steps replay captured graphs; the same frames rendered view by
+    view (hdr_render_frame) are timed beside it."""
+    from hdrestir_amd import scenes
+    rng = np.random.default_rng(seed)
+    pipe = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=6)
+    views, cams = [], []
+    for _ in range(16):
+        w, h = int(rng.integers(320, 1921)), int(rng.integers(240, 1081))
+        cams.append((look_at_origin_from_sphere(rng, 40.0), scenes.perspective(50.0, w / h, 0.05, 1000.0)))
+        views.append(gpu.Renderer(ctx, gs, w, h))
+    pixels = sum(v.W * v.H for v in views)
+
+    def frames(f):
+        return [abi.frame(v, p, f) for v, p in cams]
+    batch = gpu.Batch(ctx)
+    chosen, tuned_ms = batch.autotune(views, frames(0), [pipe] * 16, reps=2)
+    for v in views:
+        v.reset()
+    batch.render(views, frames(0), [pipe] * 16)
+    batch.synchronize()
+    t0 = time.perf_counter()
+    for f in range(1, 1 + steps):
+        batch.render(views, frames(f), [pipe] * 16)
+    batch.synchronize()
+    ms_batched = (time.perf_counter() - t0) * 1000.0 / steps
+    for v in views:
+        v.reset()
+    for v, fr in zip(views, frames(0)):
+        v.render(fr, pipe, read_color=False)
+    for v in views:
+        v.synchronize()
+    t0 = time.perf_counter()
+    for f in range(1, 1 + steps):
+        for v, fr in zip(views, frames(f)):
+            v.render(fr, pipe, read_color=False)
+    for v in views:
+        v.synchronize()
+    ms_plain = (time.perf_counter() - t0) * 1000.0 / steps
+    batch.close()
+    for v in views:
+        v.close()
+    return {"workload": "many_lights_16_views_restir_depth6", "views": 16, "pixels": pixels,
+            "sizes": [[v.W, v.H] for v in views], "steps": steps, "autotuned_config": chosen,
+            "autotune_step_ms": round(tuned_ms, 3), "batched_ms_per_step": round(ms_batched, 3),
+            "batched_mpixels_per_s": round(pixels / ms_batched / 1e3, 2),
+            "plain_ms_per_step": round(ms_plain, 3), "speedup": round(ms_plain / ms_batched, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +297,10 @@ def main():
                     help="only run a plain device-to-device copy of BYTES bytes 4 times (the reference "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
     ap.add_argument("--no-serving", action="store_true", help="skip the request-serving measurement")
+    ap.add_argument("--no-deep", action="store_true",
+                    help="skip the depth-6 single-view and 16-view batch measurements")
+    ap.add_argument("--deep-only", action="store_true",
+                    help="only the depth-6 single-view measurement, --steps frames (for profilers)")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the counting pass and the timed frames of the headline pipeline (for profilers)")
     ap.add_argument("--serving-requests", type=int, default=48)
@@ -239,6 +325,13 @@ def main():
     from hdrestir_amd import abi, gpu, scenes
     _log("building scene")
     sc = scenes.many_lights(frames=max(args.steps, 2), dolly=True)
+    if args.deep_only:
+        ctx = gpu.Context(0)
+        gs = gpu.GpuScene(ctx, sc)
+        print(json.dumps(deep_path_measurement(gpu, abi, ctx, sc, gs, args.steps)), flush=True)
+        gs.close()
+        ctx.close()
+        return
     # 1 temporal (the reference's merge) + 3 spatial neighbours within 30 px (an extension)
     pipe = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=1,
                         spatial_count=3, spatial_radius=30.0)
@@ -379,6 +472,13 @@ def main():
         "image_mean_rgb": [round(float(v), 3) for v in img[..., :3].reshape(-1, 3).mean(0)],
     }
     r.close()
+    if not args.headline_only and not args.no_deep:
+        _log("deep paths")
+        line["deep_paths"] = deep_path_measurement(gpu, abi, ctx, sc, gs, max(4, args.steps // 4))
+        _log(f"deep paths {line['deep_paths']}")
+        _log("multi-view batch")
+        line["multiview"] = multiview_measurement(gpu, abi, ctx, gs, 4)
+        _log(f"multi-view {line['multiview']}")
     if not args.no_serving and not args.headline_only:
         _log("serving")
         line["serving"] = serving_measurement(gpu, abi, ctx, sc, gs, args.serving_requests)

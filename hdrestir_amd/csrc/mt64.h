@@ -17,6 +17,7 @@
 // recurrence steps. Outputs from 312 on (second twist) are produced by materialising the
 // full state in a caller-provided 312-word global slot; paths that can draw that many are
 // rare (deep paths with large candidate counts) and the host sizes the slot pool for them.
+// Mt64Deep (below) also evaluates the second twist lazily and needs the slot from output 624 on.
 #pragma once
 
 #include "gfm.h"
@@ -152,5 +153,128 @@ struct Mt64Lean : Mt64 {
         ++k;
     }
 };
+
+// The stream for paths that may draw past output 310 (the wavefront path's deep paths).
+// Outputs 0 .. 310 as Mt64 (first twist from two seeding cursors). Outputs 311 .. 623 -- the
+// second twist -- are evaluated lazily as well, from the same two cursors restarted at the seed:
+// with X the engine's word sequence (X[i] = seeding word S(i) for i < 312, X[n] = X[n-156] ^
+// T(X[n-312], X[n-311]) after) and B1(j) = X[312+j], B2(j) = X[624+j] (output 312+j is B2(j)
+// tempered):
+//   B1(u)       = S(u+156) ^ T(S(u), S(u+1))                      u < 156
+//   B1(u+156)   = B1(u) ^ T(S(u+156), S(u+157))                   u < 155
+//   B1(311)     = B1(155) ^ T(S(311), B1(0))
+//   B2(u)       = B1(u+156) ^ T(B1(u), B1(u+1))                   u < 156    (pass A, u = k - 312)
+//   B2(u+156)   = B2(u) ^ T(B1(u+156), B1(u+157))                 u < 155    (pass B, u = k - 468)
+//   B2(311)     = B2(155) ^ T(B1(311), B2(0))
+// so each output costs a few seeding steps and twists and no memory. B1(155) and B1(311) are kept
+// from pass A for pass B. From output 624 on (third twist) the state is materialised in the spill
+// slot; those outputs go through one out-of-line function, so the inlined draw sites stay small.
+struct Mt64Deep : Mt64 {
+    uint64_t b155, b311;  // B1(155), B1(311), set during pass A
+
+    HD uint64_t b1_0() const { return s156 ^ mt_twist(uint64_t(seed), mt_step(uint64_t(seed), 1)); }
+    HD uint64_t b1_156(uint64_t c0) const { return c0 ^ mt_twist(s156, mt_step(s156, 157)); }
+
+    // Outputs 311 .. 623 (y != nullptr) or their discard (only the cursors move).
+    HD void second(uint64_t *y) {
+        if (k == 311u) {  // B1(311): cursors at j = 155 (lo = S(155), lo1 = S(156), hi = S(311))
+            if (y) {
+                const uint64_t t155 = hi ^ mt_twist(lo, lo1);
+                *y = t155 ^ mt_twist(hi, b1_0());
+            }
+            return;
+        }
+        if (k == 312u || k == 468u) restart();
+        if (k < 468u) {  // pass A, u = j = k - 312
+            if (j < 155u) {
+                if (y) {
+                    const uint64_t P = hi ^ mt_twist(lo, lo1);  // B1(u)
+                    const uint64_t lo2 = mt_step(lo1, j + 2);
+                    const uint64_t Q = hi1 ^ mt_twist(lo1, lo2);  // B1(u+1)
+                    *y = P ^ mt_twist(hi, hi1) ^ mt_twist(P, Q);
+                }
+                advance();
+            } else {  // u = 155: hi = S(311)
+                const uint64_t P = hi ^ mt_twist(lo, lo1);  // B1(155)
+                const uint64_t c0 = b1_0();
+                b155 = P;
+                b311 = P ^ mt_twist(hi, c0);
+                if (y) *y = b311 ^ mt_twist(P, b1_156(c0));
+            }
+        } else if (k < 623u) {  // pass B, u = j = k - 468 <= 154
+            if (y) {
+                const uint64_t P = hi ^ mt_twist(lo, lo1);                // B1(u)
+                const uint64_t lo2 = mt_step(lo1, j + 2);
+                const uint64_t Q = hi1 ^ mt_twist(lo1, lo2);              // B1(u+1)
+                const uint64_t R = P ^ mt_twist(hi, hi1);                 // B1(u+156)
+                const uint64_t B2u = R ^ mt_twist(P, Q);                  // B2(u)
+                const uint64_t S = j < 154u ? Q ^ mt_twist(hi1, mt_step(hi1, j + 158)) : b311;  // B1(u+157)
+                *y = B2u ^ mt_twist(R, S);
+            }
+            advance();
+        } else if (y) {  // k = 623: B2(311) = B2(155) ^ T(B1(311), B2(0))
+            const uint64_t c0 = b1_0(), c156 = b1_156(c0);
+            const uint64_t s1 = mt_step(uint64_t(seed), 1), s2 = mt_step(s1, 2);
+            const uint64_t b1 = mt_step(s156, 157) ^ mt_twist(s1, s2);  // B1(1)
+            const uint64_t b2_0 = c156 ^ mt_twist(c0, b1);
+            const uint64_t b2_155 = b311 ^ mt_twist(b155, c156);
+            *y = b2_155 ^ mt_twist(b311, b2_0);
+        }
+    }
+    // Output 624 and beyond: the spill slot holds the state after the third twist, twisted again
+    // every 312 outputs.
+    HD uint64_t third() {
+        if (!slot) return 0;  // the host allocates slots whenever a pixel can draw this far
+        const uint32_t idx = (k - 624u) % uint32_t(kMtN);
+        if (idx == 0u) {
+            if (k == 624u) materialise_second();
+            twist_in_place(slot);
+        }
+        return slot[idx];
+    }
+
+    HD uint64_t next_raw();
+    HD float next_float() { return mt_canonical(next_raw()); }
+    HD void discard();
+};
+
+static __host__ __device__ __noinline__ uint64_t mt_deep_next(Mt64Deep &r) {
+    uint64_t y = 0;
+    if (r.k < 624u) r.second(&y);
+    else y = r.third();
+    ++r.k;
+    return mt_temper(y);
+}
+static __host__ __device__ __noinline__ void mt_deep_discard(Mt64Deep &r) {
+    if (r.k < 624u) r.second(nullptr);
+    else (void)r.third();
+    ++r.k;
+}
+static __host__ __device__ __noinline__ void mt_copy_slot(uint64_t *dst, const uint64_t *src) {
+    for (int q = 0; q < kMtN; ++q) dst[q] = src[q];
+}
+
+HD uint64_t Mt64Deep::next_raw() {
+    if (k >= 311u) return mt_deep_next(*this);
+    uint64_t y;
+    if (k < 156u) {
+        y = hi ^ mt_twist(lo, lo1);
+    } else {
+        if (k == 156u) restart();
+        y = hi ^ mt_twist(lo, lo1) ^ mt_twist(hi, hi1);
+    }
+    advance();
+    ++k;
+    return mt_temper(y);
+}
+HD void Mt64Deep::discard() {
+    if (k >= 311u) {
+        mt_deep_discard(*this);
+        return;
+    }
+    if (k == 156u) restart();
+    advance();
+    ++k;
+}
 
 }  // namespace hdr

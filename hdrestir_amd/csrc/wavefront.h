@@ -23,53 +23,21 @@
 
 namespace hdr {
 
-// ---------------------------------------------------------------- RNG for deep paths
-// Outputs 311 and beyond (the special last word of the first twist, then the spill slot) go
-// through one out-of-line function, so the inlined draw sites stay small.
-__device__ __noinline__ uint64_t mt_deep_next(Mt64 &r) { return r.next_raw(); }
-__device__ __noinline__ void mt_copy_slot(uint64_t *dst, const uint64_t *src) {
-    for (int q = 0; q < kMtN; ++q) dst[q] = src[q];
-}
+// The deep stream (Mt64Deep) is in mt64.h.
 
-struct Mt64Deep : Mt64 {
-    __device__ __forceinline__ uint64_t next_raw() {
-        if (k >= 311u) return mt_deep_next(*this);
-        uint64_t y;
-        if (k < 156u) {
-            y = hi ^ mt_twist(lo, lo1);
-        } else {
-            if (k == 156u) restart();
-            y = hi ^ mt_twist(lo, lo1) ^ mt_twist(hi, hi1);
-        }
-        advance();
-        ++k;
-        return mt_temper(y);
-    }
-    __device__ __forceinline__ float next_float() { return mt_canonical(next_raw()); }
-    __device__ __forceinline__ void discard() {
-        if (k >= 311u) {
-            (void)mt_deep_next(*this);
-            return;
-        }
-        if (k == 156u) restart();
-        advance();
-        ++k;
-    }
-};
-
-// The winner's replay copy: once the stream has materialised its spill slot (output 313 on), the
+// The winner's replay copy: once the stream has materialised its spill slot (output 625 on), the
 // copy works on the spare slot.
 __device__ __forceinline__ Mt64Deep rng_fork(const Mt64Deep &r, uint64_t *slot2) {
     Mt64Deep c = r;
-    if (r.k > 312u && slot2 && r.slot) mt_copy_slot(slot2, r.slot);
+    if (r.k > 624u && slot2 && r.slot) mt_copy_slot(slot2, r.slot);
     c.slot = slot2;
     return c;
 }
 
 // ---------------------------------------------------------------- path record
-// Field-major in HBM (field f of pixel i at wstate[f * P + i]), 8 x 16 B per pixel, so each field
+// Field-major in HBM (field f of pixel i at wstate[f * P + i]), 9 x 16 B per pixel, so each field
 // of a wave's paths is one coalesced 1 KB access.
-constexpr int kWaveFields = 8;
+constexpr int kWaveFields = 9;
 constexpr uint32_t kWavePrimaryHit = 1u << 31;  // in field 7.x: the G-buffer ray hit
 
 struct WavePath {
@@ -116,6 +84,7 @@ __device__ __forceinline__ void store_path(const FrameArgs &a, int pixel, const 
                           (p.primary_hit ? kWavePrimaryHit : 0u);
     w[7 * P + pixel] = make_float4(__uint_as_float(bits), __uint_as_float(p.dbg.lhash), __uint_as_float(p.dbg.shash),
                                    __int_as_float(p.dbg.picks));
+    if (p.rng.k > 467u) w[8 * P + pixel] = pack_u64x2(p.rng.b155, p.rng.b311);  // set once pass A ends
 }
 
 __device__ __forceinline__ void load_path(const FrameArgs &a, int pixel, WavePath &p) {
@@ -144,6 +113,11 @@ __device__ __forceinline__ void load_path(const FrameArgs &a, int pixel, WavePat
     p.dbg.lhash = __float_as_uint(r7.y);
     p.dbg.shash = __float_as_uint(r7.z);
     p.dbg.picks = __float_as_int(r7.w);
+    if (p.rng.k > 467u) {
+        const float4 r8 = w[8 * P + pixel];
+        p.rng.b155 = unpack_lo(r8);
+        p.rng.b311 = unpack_hi(r8);
+    }
 }
 
 // ---------------------------------------------------------------- one path vertex

@@ -26,8 +26,10 @@ BATCHES = 32
 
 
 def _render_mean(r, sc_frame, pipe, first, n, batch=None):
-    """Renders frames first .. first+n-1 into a freshly reset view. Returns the accumulated image
-    (float64, RGB) and, with `batch`, the per-batch mean radiance of the image (n / batch batches)."""
+    """Renders frames first .. first+n-1 into a freshly reset view. Returns the mean image of those
+    frames (float64, RGB) and, with `batch`, the per-batch mean radiance (n / batch batches). The
+    view's colour is its accumulated sum divided by frame_index + 1 (the renderer's progressive
+    normalisation), so the sum is recovered by multiplying back."""
     r.reset()
     means = []
     prev_sum = np.zeros(3)
@@ -35,10 +37,10 @@ def _render_mean(r, sc_frame, pipe, first, n, batch=None):
         r.render(sc_frame(first + i), pipe, read_color=False)
         if batch and (i + 1) % batch == 0:
             img = r.read_color()[..., :3].astype(np.float64)
-            tot = img.mean(axis=(0, 1)) * (i + 1)  # sum of the per-frame image means so far
+            tot = img.mean(axis=(0, 1)) * (first + i + 1)  # sum of the per-frame image means so far
             means.append((tot - prev_sum) / batch)
             prev_sum = tot
-    img = r.read_color()[..., :3].astype(np.float64)
+    img = r.read_color()[..., :3].astype(np.float64) * (first + n) / n
     return img, np.array(means)
 
 
@@ -87,8 +89,8 @@ def many(gpu_ctx):
 # (pipeline, spp for the MSE bar, relative-MSE bar, unbiased -> CI test)
 BOX_CASES = {
     "pt_d4": (abi.pipeline("mis", max_depth=4), 64, 0.05, True),
-    "ris16_d4": (abi.pipeline("ris", candidate_count=16, use_reservoir=False, max_depth=4), 64, 0.05, True),
-    "restir16_d4": (abi.pipeline("ris", candidate_count=16, use_reservoir=True, max_depth=4), 64, 0.05, False),
+    "ris16_d4": (abi.pipeline("ris", candidate_count=16, use_reservoir=False, max_depth=4), 64, 0.03, True),
+    "restir16_d4": (abi.pipeline("ris", candidate_count=16, use_reservoir=True, max_depth=4), 64, 0.03, False),
 }
 
 
@@ -108,12 +110,12 @@ def test_box_convergence(box, name):
 
 MANY_CASES = {
     "pt_d1": (abi.pipeline("mis", max_depth=1), 64, 1.0, True),
-    "ris32_d1": (abi.pipeline("ris", candidate_count=32, use_reservoir=False, max_depth=1), 64, 1.0, True),
+    "ris32_d1": (abi.pipeline("ris", candidate_count=32, use_reservoir=False, max_depth=1), 64, 0.11, True),
     "restir32_d1_skipvis": (abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True,
                                          max_depth=1), 64, 1.0, False),
     "restir32_d1_skipvis_spatial3": (abi.pipeline("ris", candidate_count=32, use_reservoir=True,
                                                   skip_visibility=True, max_depth=1, spatial_count=3,
-                                                  spatial_radius=30.0), 64, 1.0, False),
+                                                  spatial_radius=30.0), 64, 0.75, False),
 }
 
 
