@@ -160,18 +160,21 @@ class ServerDesc(C.Structure):
 
 class LaunchConfig(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("queue_waves", C.c_int32), ("trace_occupancy", C.c_int32),
-                ("shade_occupancy", C.c_int32), ("use_graphs", C.c_int32), ("_reserved", C.c_int32 * 3)]
+                ("shade_occupancy", C.c_int32), ("use_graphs", C.c_int32), ("lds_stack", C.c_int32),
+                ("_reserved", C.c_int32 * 2)]
 
     def as_dict(self):
         return {"queue_waves": self.queue_waves, "trace_occupancy": self.trace_occupancy,
-                "shade_occupancy": self.shade_occupancy, "use_graphs": self.use_graphs}
+                "shade_occupancy": self.shade_occupancy, "use_graphs": self.use_graphs,
+                "lds_stack": self.lds_stack}
 
 
-def launch_config(queue_waves=0, trace_occupancy=0, shade_occupancy=0, use_graphs=1):
+def launch_config(queue_waves=0, trace_occupancy=0, shade_occupancy=0, use_graphs=1, lds_stack=0):
     c = LaunchConfig()
     c.struct_size = C.sizeof(LaunchConfig)
     c.queue_waves, c.trace_occupancy, c.shade_occupancy, c.use_graphs = (queue_waves, trace_occupancy,
                                                                          shade_occupancy, use_graphs)
+    c.lds_stack = lds_stack
     return c
 
 

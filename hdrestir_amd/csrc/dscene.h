@@ -129,7 +129,9 @@ __device__ __forceinline__ bool tri_hit(const GTri &t, V3 o, V3 d, float &tt, fl
     const V3 e2 = v3(t.v2x, t.v2y, t.v2z) - v0;
     const V3 pvec = cross(d, e2);
     const float det = dot(e1, pvec);
-    if (double(fabsf(det)) < 1e-8) return false;
+    // double(|det|) < 1e-8 and double(tt) > 1e-4 as float compares: 1e-8f and 1e-4f are the
+    // largest floats below 1e-8 and 1e-4 (no float lies between), so the outcomes are the same
+    if (fabsf(det) <= 1e-8f) return false;
     const float invDet = 1.0f / det;
     const V3 tvec = o - v0;
     const float u = dot(tvec, pvec) * invDet;
@@ -140,7 +142,7 @@ __device__ __forceinline__ bool tri_hit(const GTri &t, V3 o, V3 d, float &tt, fl
     tt = dot(e2, qvec) * invDet;
     uu = u;
     vv = v;
-    return double(tt) > 1e-4;
+    return tt > 1e-4f;
 }
 
 __device__ __forceinline__ GNode load_node(const GNode *nodes, int i) {
@@ -167,6 +169,71 @@ struct RawHit {
 
 constexpr int kStackDepth = 64;
 
+// Traversal stacks. An entry is a node reference and, for closest-hit queries, the entry distance
+// of its box (kTa). PrivStack keeps every entry in private (scratch) memory. RingStack keeps the
+// top kRing entries of each lane in LDS -- the lane's column of a wave-shared array, slot i at
+// lds[i * 64] -- and moves older entries to private memory only when the ring overflows, so the
+// pops on the traversal's critical path read LDS.
+template <bool kTa>
+struct PrivStack {
+    int ref[kStackDepth];
+    float ta[kStackDepth];
+    int sp;
+    __device__ __forceinline__ void reset() { sp = 0; }
+    __device__ __forceinline__ bool empty() const { return sp == 0; }
+    __device__ __forceinline__ void push(int r, float t) {
+        ref[sp] = r;
+        if (kTa) ta[sp] = t;
+        ++sp;
+    }
+    __device__ __forceinline__ void pop(int &r, float &t) {
+        --sp;
+        r = ref[sp];
+        t = kTa ? ta[sp] : 0.0f;
+    }
+};
+
+template <int kRing, bool kTa>
+struct RingStack {
+    static_assert((kRing & (kRing - 1)) == 0, "ring size must be a power of two");
+    int *lr;    // this lane's LDS column of references (slot i at lr[i * 64])
+    float *lt;  // and of entry distances (kTa)
+    int ref[kStackDepth];
+    float ta[kStackDepth];
+    int sp, valid;  // entries on the stack; how many of the top ones are in the ring
+    __device__ __forceinline__ void bind(int *r, float *t) {
+        lr = r;
+        lt = t;
+        sp = valid = 0;
+    }
+    __device__ __forceinline__ void reset() { sp = valid = 0; }
+    __device__ __forceinline__ bool empty() const { return sp == 0; }
+    __device__ __forceinline__ void push(int r, float t) {
+        const int slot = (sp & (kRing - 1)) * 64;
+        if (valid == kRing) {  // evict the oldest ring entry (stack position sp - kRing)
+            ref[sp - kRing] = lr[slot];
+            if (kTa) ta[sp - kRing] = lt[slot];
+        } else {
+            ++valid;
+        }
+        lr[slot] = r;
+        if (kTa) lt[slot] = t;
+        ++sp;
+    }
+    __device__ __forceinline__ void pop(int &r, float &t) {
+        --sp;
+        if (valid > 0) {
+            const int slot = (sp & (kRing - 1)) * 64;
+            r = lr[slot];
+            t = kTa ? lt[slot] : 0.0f;
+            --valid;
+        } else {
+            r = ref[sp];
+            t = kTa ? ta[sp] : 0.0f;
+        }
+    }
+};
+
 // Per-ray constants of IntersectAABB: origin, reciprocal direction and parallel-axis flags,
 // computed once with the same float operations the per-node test performs, so every box test
 // sees the values the reference computes per node.
@@ -187,32 +254,30 @@ __device__ __forceinline__ RayBox make_raybox(V3 o, V3 d) {
     }
     return r;
 }
+// Branch-free form of the reference's test, with the same outcome: the per-axis early exits
+// (tmin > tmax after an axis; a parallel axis outside its slab) are folded into one verdict at the
+// end -- tmin only grows and tmax only shrinks from axis to axis (NaN operands are ignored by
+// smax / smin), so a box rejected after some axis is still rejected after the last one. The
+// reference's double threshold double(tmax) > 1e-4 equals tmax > 1e-4f (1e-4f is the largest
+// float below 1e-4, and no float lies between it and 1e-4), so both variants share it.
 template <bool kFloatThresholds>
 __device__ __forceinline__ bool box_hit(const RayBox &r, const GNode &n, float &tmin_out) {
-    if (n.mnx > n.mxx || n.mny > n.mxy || n.mnz > n.mxz) return false;  // GfRange3f::IsEmpty
+    bool miss = n.mnx > n.mxx || n.mny > n.mxy || n.mnz > n.mxz;  // GfRange3f::IsEmpty
     float tmin = -1e30f, tmax = 1e30f;
     const float mn[3] = {n.mnx, n.mny, n.mnz};
     const float mx[3] = {n.mxx, n.mxy, n.mxz};
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        if (r.par[i]) {
-            if (r.o[i] < mn[i] || r.o[i] > mx[i]) return false;
-        } else {
-            float t1 = (mn[i] - r.o[i]) * r.inv[i];
-            float t2 = (mx[i] - r.o[i]) * r.inv[i];
-            if (t1 > t2) {
-                const float sw = t1;
-                t1 = t2;
-                t2 = sw;
-            }
-            tmin = smax(tmin, t1);
-            tmax = smin(tmax, t2);
-            if (tmin > tmax) return false;
-        }
+        const float t1 = (mn[i] - r.o[i]) * r.inv[i];
+        const float t2 = (mx[i] - r.o[i]) * r.inv[i];
+        const bool sw = t1 > t2;
+        const float lo = sw ? t2 : t1, hi = sw ? t1 : t2;
+        miss = miss || (r.par[i] && (r.o[i] < mn[i] || r.o[i] > mx[i]));
+        tmin = r.par[i] ? tmin : smax(tmin, lo);
+        tmax = r.par[i] ? tmax : smin(tmax, hi);
     }
     tmin_out = tmin;
-    if (kFloatThresholds) return tmax > 0.0f && tmax > 1e-4f;
-    return tmax > 0.0f && double(tmax) > 1e-4;
+    return !miss && !(tmin > tmax) && tmax > 0.0f && tmax > 1e-4f;
 }
 
 // Stack / cursor form of a node: inner -> index of its left child (the right one follows it);
@@ -228,17 +293,14 @@ __device__ __forceinline__ int node_ref(const GNode &n) {
 // fetch); the left child is entered at once, the right one waits on the stack with its entry
 // distance and is pruned against the best t when popped, which is where and how the reference
 // tests it.
-template <bool kCount>
-__device__ __forceinline__ bool trace_closest(const DeviceScene &s, V3 o, V3 d, RawHit &hit, TraceStats &st) {
+template <bool kCount, class TS, class BS>
+__device__ __forceinline__ bool trace_closest_s(const DeviceScene &s, V3 o, V3 d, RawHit &hit, TraceStats &st,
+                                                TS &tstk, BS &bstk) {
     hit.t = 1e30f;
     hit.inst = -1;
     if (s.tlas_root < 0) return false;
     const RayBox rt = make_raybox<true>(o, d);
-    int tref[kStackDepth];
-    float tta[kStackDepth];
-    int bref[kStackDepth];
-    float bta[kStackDepth];
-    int tsp = 0;
+    tstk.reset();
     bool was = false;
     int cur;
     {
@@ -254,11 +316,7 @@ __device__ __forceinline__ bool trace_closest(const DeviceScene &s, V3 o, V3 d, 
             if (kCount) st.nodes += 2;
             float tl, tr;
             const bool hl = box_hit<true>(rt, l, tl) && !(tl > hit.t);
-            if (box_hit<true>(rt, r, tr)) {
-                tref[tsp] = node_ref(r);
-                tta[tsp] = tr;
-                ++tsp;
-            }
+            if (box_hit<true>(rt, r, tr)) tstk.push(node_ref(r), tr);
             if (hl) {
                 cur = node_ref(l);
                 continue;
@@ -277,7 +335,7 @@ __device__ __forceinline__ bool trace_closest(const DeviceScene &s, V3 o, V3 d, 
                 float instT = hit.t;
                 int bestTri = -1;
                 float bu = 0.f, bv = 0.f;
-                int bsp = 0;
+                bstk.reset();
                 int bcur;
                 bool go;
                 {
@@ -293,11 +351,7 @@ __device__ __forceinline__ bool trace_closest(const DeviceScene &s, V3 o, V3 d, 
                         if (kCount) st.nodes += 2;
                         float tl, tr;
                         const bool hl = box_hit<false>(rb, l, tl) && !(tl > instT);
-                        if (box_hit<false>(rb, r, tr)) {
-                            bref[bsp] = node_ref(r);
-                            bta[bsp] = tr;
-                            ++bsp;
-                        }
+                        if (box_hit<false>(rb, r, tr)) bstk.push(node_ref(r), tr);
                         if (hl) {
                             bcur = node_ref(l);
                             continue;
@@ -318,10 +372,12 @@ __device__ __forceinline__ bool trace_closest(const DeviceScene &s, V3 o, V3 d, 
                         }
                     }
                     go = false;
-                    while (bsp > 0) {
-                        --bsp;
-                        if (bta[bsp] > instT) continue;
-                        bcur = bref[bsp];
+                    while (!bstk.empty()) {
+                        int rr;
+                        float ta;
+                        bstk.pop(rr, ta);
+                        if (ta > instT) continue;
+                        bcur = rr;
                         go = true;
                         break;
                     }
@@ -336,10 +392,12 @@ __device__ __forceinline__ bool trace_closest(const DeviceScene &s, V3 o, V3 d, 
             }
         }
         bool more = false;
-        while (tsp > 0) {
-            --tsp;
-            if (tta[tsp] > hit.t) continue;
-            cur = tref[tsp];
+        while (!tstk.empty()) {
+            int rr;
+            float ta;
+            tstk.pop(rr, ta);
+            if (ta > hit.t) continue;
+            cur = rr;
             more = true;
             break;
         }
@@ -347,19 +405,23 @@ __device__ __forceinline__ bool trace_closest(const DeviceScene &s, V3 o, V3 d, 
     }
     return was;
 }
+template <bool kCount>
+__device__ __forceinline__ bool trace_closest(const DeviceScene &s, V3 o, V3 d, RawHit &hit, TraceStats &st) {
+    PrivStack<true> t, b;
+    return trace_closest_s<kCount>(s, o, d, hit, st, t, b);
+}
 
 // Shadow query: is there a hit with 1e-4 < t < tmax? The reference fires a closest-hit ray and
 // tests Depth < Dist - 1e-3 (ris_direct_light_integrator.cpp:513-519, mis_direct_light_integrator.cpp:100-105);
 // the same traversal with the bound fixed at tmax, stopped at the first accepted triangle,
 // answers the same question (a box entered before the closest hit cannot be pruned by a
 // larger bound), so subtrees are pruned as soon as their box is tested.
-template <bool kCount>
-__device__ __forceinline__ bool trace_any(const DeviceScene &s, V3 o, V3 d, float tmax, TraceStats &st) {
+template <bool kCount, class TS, class BS>
+__device__ __forceinline__ bool trace_any_s(const DeviceScene &s, V3 o, V3 d, float tmax, TraceStats &st, TS &tstk,
+                                            BS &bstk) {
     if (s.tlas_root < 0) return false;
     const RayBox rt = make_raybox<true>(o, d);
-    int tref[kStackDepth];
-    int bref[kStackDepth];
-    int tsp = 0;
+    tstk.reset();
     int cur;
     {
         const GNode n = load_node(s.nodes, s.tlas_root);
@@ -374,7 +436,7 @@ __device__ __forceinline__ bool trace_any(const DeviceScene &s, V3 o, V3 d, floa
             if (kCount) st.nodes += 2;
             float tl, tr;
             const bool hl = box_hit<true>(rt, l, tl) && !(tl > tmax);
-            if (box_hit<true>(rt, r, tr) && !(tr > tmax)) tref[tsp++] = node_ref(r);
+            if (box_hit<true>(rt, r, tr) && !(tr > tmax)) tstk.push(node_ref(r), 0.0f);
             if (hl) {
                 cur = node_ref(l);
                 continue;
@@ -389,7 +451,7 @@ __device__ __forceinline__ bool trace_any(const DeviceScene &s, V3 o, V3 d, floa
                 const V3 oo = xform_point(inst.inv, o);
                 const V3 od = xform_dir(inst.inv, d);
                 const RayBox rb = make_raybox<false>(oo, od);
-                int bsp = 0;
+                bstk.reset();
                 int bcur;
                 bool go;
                 {
@@ -405,7 +467,7 @@ __device__ __forceinline__ bool trace_any(const DeviceScene &s, V3 o, V3 d, floa
                         if (kCount) st.nodes += 2;
                         float tl, tr;
                         const bool hl = box_hit<false>(rb, l, tl) && !(tl > tmax);
-                        if (box_hit<false>(rb, r, tr) && !(tr > tmax)) bref[bsp++] = node_ref(r);
+                        if (box_hit<false>(rb, r, tr) && !(tr > tmax)) bstk.push(node_ref(r), 0.0f);
                         if (hl) {
                             bcur = node_ref(l);
                             continue;
@@ -420,15 +482,24 @@ __device__ __forceinline__ bool trace_any(const DeviceScene &s, V3 o, V3 d, floa
                             if (tri_hit(tr, oo, od, tt, uu, vv) && tt < tmax) return true;
                         }
                     }
-                    go = bsp > 0;
-                    if (go) bcur = bref[--bsp];
+                    go = !bstk.empty();
+                    if (go) {
+                        float unused;
+                        bstk.pop(bcur, unused);
+                    }
                 }
             }
         }
-        if (tsp == 0) break;
-        cur = tref[--tsp];
+        if (tstk.empty()) break;
+        float unused;
+        tstk.pop(cur, unused);
     }
     return false;
+}
+template <bool kCount>
+__device__ __forceinline__ bool trace_any(const DeviceScene &s, V3 o, V3 d, float tmax, TraceStats &st) {
+    PrivStack<false> t, b;
+    return trace_any_s<kCount>(s, o, d, tmax, st, t, b);
 }
 
 // HitRecord of the winner (Scene::Intersect :456-497 and IntersectScene :515-535): world

@@ -27,8 +27,43 @@ __device__ __forceinline__ void wave_add(unsigned long long *dst, unsigned long 
     if ((threadIdx.x & 63) == 0) atomicAdd(dst, v);
 }
 
-template <bool kCount>
-__device__ __forceinline__ void gbuffer_pixel(const FrameArgs &a, GRec *gbuf, int x, int y, TraceStats &ts) {
+// LDS rings of the traversal stacks (RingStack): the top 4 TLAS and 8 BLAS entries of each lane.
+constexpr int kRingT = 4, kRingB = 8;
+#define HDR_RING_LDS                                \
+    __shared__ int ring_tr[kRingT * kLanes];        \
+    __shared__ float ring_tt[kRingT * kLanes];      \
+    __shared__ int ring_br[kRingB * kLanes];        \
+    __shared__ float ring_bt[kRingB * kLanes];
+
+// Closest hit with the stacks in private memory or with LDS rings (the lane's columns).
+template <bool kCount, bool kLds>
+__device__ __forceinline__ bool closest_hit(const DeviceScene &s, V3 o, V3 d, RawHit &rh, TraceStats &ts, int *tr,
+                                            float *tt, int *br, float *bt) {
+    if (kLds) {
+        RingStack<kRingT, true> t;
+        RingStack<kRingB, true> b;
+        t.bind(tr, tt);
+        b.bind(br, bt);
+        return trace_closest_s<kCount>(s, o, d, rh, ts, t, b);
+    }
+    return trace_closest<kCount>(s, o, d, rh, ts);
+}
+template <bool kCount, bool kLds>
+__device__ __forceinline__ bool any_hit(const DeviceScene &s, V3 o, V3 d, float tmax, TraceStats &ts, int *tr,
+                                        int *br) {
+    if (kLds) {
+        RingStack<kRingT, false> t;
+        RingStack<kRingB, false> b;
+        t.bind(tr, nullptr);
+        b.bind(br, nullptr);
+        return trace_any_s<kCount>(s, o, d, tmax, ts, t, b);
+    }
+    return trace_any<kCount>(s, o, d, tmax, ts);
+}
+
+template <bool kCount, bool kLds = false>
+__device__ __forceinline__ void gbuffer_pixel(const FrameArgs &a, GRec *gbuf, int x, int y, TraceStats &ts, int *tr,
+                                              float *tt, int *br, float *bt) {
     const int pixel = y * a.W + x;
     V3 co, cd;
     if (a.has_cam) {
@@ -41,7 +76,7 @@ __device__ __forceinline__ void gbuffer_pixel(const FrameArgs &a, GRec *gbuf, in
     }
     RawHit rh;
     HitRec g = {};
-    const bool hit = trace_closest<kCount>(a.s, co, cd, rh, ts);
+    const bool hit = closest_hit<kCount, kLds>(a.s, co, cd, rh, ts, tr, tt, br, bt);
     if (hit) g = make_hit_record(a.s, co, cd, rh);
     store_grec(gbuf + pixel, hit, g);
     if (a.aov_depth) {
@@ -80,12 +115,14 @@ __device__ __forceinline__ int launch_view(const int *__restrict__ tile_base, in
     const int y = (tile_ / tw_) * 8 + (lane >> 3);                            \
     const bool valid = x < a.W && y < a.H;
 
-template <bool kCount>
+template <bool kCount, bool kLds = false>
 __global__ void __launch_bounds__(kLanes) gbuffer_kernel(const FrameArgs *__restrict__ args,
                                                          const int *__restrict__ tile_base, int n_views) {
+    HDR_RING_LDS
     HDR_TILE(args, tile_base, n_views)
     TraceStats ts = {};
-    if (valid) gbuffer_pixel<kCount>(a, a.gbuf, x, y, ts);
+    if (valid) gbuffer_pixel<kCount, kLds>(a, a.gbuf, x, y, ts, ring_tr + lane, ring_tt + lane, ring_br + lane,
+                                           ring_bt + lane);
     if (kCount) {
         unsigned long long *c = a.counters;
         wave_add(c + 0, valid ? 1ULL : 0ULL);
@@ -364,9 +401,11 @@ __global__ void zero_u32_kernel(unsigned int *p) { *p = 0u; }
 
 // Any-hit shadow rays of the queue; an occluded winner keeps zero throughput in its reservoir
 // (the visibility fix-up of ris_direct_light_integrator.cpp:163-175).
-template <bool kCount, int kOcc = 1>
+template <bool kCount, int kOcc = 1, bool kLds = false>
 __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
 shadow_kernel(const FrameArgs *__restrict__ args) {
+    HDR_RING_LDS
+    const int lane = threadIdx.x;
     const FrameArgs &a0 = args[0];  // the queue and its count are shared by the views of the launch
     const unsigned int n = *a0.qcount;
     TraceStats ts = {};
@@ -375,7 +414,8 @@ shadow_kernel(const FrameArgs *__restrict__ args) {
         const uint32_t code = __float_as_uint(q.d.w);
         const FrameArgs &a = args[code >> 24];
         const int pixel = int(code & 0xffffffu);
-        if (trace_any<kCount>(a.s, v3(q.o.x, q.o.y, q.o.z), v3(q.d.x, q.d.y, q.d.z), q.o.w, ts)) {
+        if (any_hit<kCount, kLds>(a.s, v3(q.o.x, q.o.y, q.o.z), v3(q.d.x, q.d.y, q.d.z), q.o.w, ts, ring_tr + lane,
+                                  ring_br + lane)) {
             a.flags[pixel] = 3u;
             hdr_reservoir &r = a.res[pixel];
             r.throughput[0] = r.throughput[1] = r.throughput[2] = r.throughput[3] = 0.0f;
@@ -535,9 +575,10 @@ wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce) {
 // the shadow rays (any hit: an unoccluded winner's contribution joins the path total; an occluded
 // primary winner keeps zero throughput in its reservoir, ris_direct_light_integrator.cpp:163-175)
 // of vertex `bounce`.
-template <bool kCount, int kOcc = 1>
+template <bool kCount, int kOcc = 1, bool kLds = false>
 __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
 wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce) {
+    HDR_RING_LDS
     const int lane = threadIdx.x;
     const int par = bounce & 1;
     unsigned int *cnt = args[0].wcnt;
@@ -559,7 +600,8 @@ wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce) {
             const int pixel = int(code & 0xffffffu);
             const V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
             RawHit rh;
-            hit = trace_closest<kCount>(a.s, o, d, rh, ts);
+            hit = closest_hit<kCount, kLds>(a.s, o, d, rh, ts, ring_tr + lane, ring_tt + lane, ring_br + lane,
+                                            ring_bt + lane);
             if (hit) store_grec(a.whit + pixel, true, make_hit_record(a.s, o, d, rh));
             ++nb;
         }
@@ -573,7 +615,8 @@ wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce) {
         const uint32_t code = __float_as_uint(q.d.w);
         const FrameArgs &a = args[code >> 24];
         const int pixel = int(code & 0xffffffu);
-        if (!trace_any<kCount>(a.s, v3(q.o.x, q.o.y, q.o.z), v3(q.d.x, q.d.y, q.d.z), q.o.w, ts)) {
+        if (!any_hit<kCount, kLds>(a.s, v3(q.o.x, q.o.y, q.o.z), v3(q.d.x, q.d.y, q.d.z), q.o.w, ts, ring_tr + lane,
+                                   ring_br + lane)) {
             float4 *t = a.wstate + 4 * wave_pixels(a) + pixel;
             const Spec tot = spec_of(*t) + spec_of(args[0].wcq[i]);
             *t = spec4(tot);
@@ -769,20 +812,26 @@ struct LaunchShape {
     int queue_waves = 0;  // 0: one wave per 64 queue slots, at most kDefaultQueueWaves
     int trace_occ = 1;
     int shade_occ = 1;
+    bool lds_stack = false;  // traversal stacks with LDS rings (RingStack) instead of private memory
 };
 constexpr int kDefaultQueueWaves = 8192;
 static int queue_blocks(const LaunchShape &ls, size_t cap) {
     const size_t need = std::max<size_t>(1, (cap + kLanes - 1) / kLanes);
     return int(std::min<size_t>(need, size_t(ls.queue_waves > 0 ? ls.queue_waves : kDefaultQueueWaves)));
 }
+template <bool C, bool L>
+static void launch_wave_trace_l(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a, int b) {
+    switch (ls.trace_occ) {
+    case 4: hipLaunchKernelGGL((wave_trace_kernel<C, 4, L>), dim3(blocks), dim3(kLanes), 0, st, a, b); break;
+    case 6: hipLaunchKernelGGL((wave_trace_kernel<C, 6, L>), dim3(blocks), dim3(kLanes), 0, st, a, b); break;
+    case 8: hipLaunchKernelGGL((wave_trace_kernel<C, 8, L>), dim3(blocks), dim3(kLanes), 0, st, a, b); break;
+    default: hipLaunchKernelGGL((wave_trace_kernel<C, 1, L>), dim3(blocks), dim3(kLanes), 0, st, a, b); break;
+    }
+}
 template <bool C>
 static void launch_wave_trace(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a, int b) {
-    switch (ls.trace_occ) {
-    case 4: hipLaunchKernelGGL((wave_trace_kernel<C, 4>), dim3(blocks), dim3(kLanes), 0, st, a, b); break;
-    case 6: hipLaunchKernelGGL((wave_trace_kernel<C, 6>), dim3(blocks), dim3(kLanes), 0, st, a, b); break;
-    case 8: hipLaunchKernelGGL((wave_trace_kernel<C, 8>), dim3(blocks), dim3(kLanes), 0, st, a, b); break;
-    default: hipLaunchKernelGGL((wave_trace_kernel<C, 1>), dim3(blocks), dim3(kLanes), 0, st, a, b); break;
-    }
+    if (ls.lds_stack) launch_wave_trace_l<C, true>(ls, blocks, st, a, b);
+    else launch_wave_trace_l<C, false>(ls, blocks, st, a, b);
 }
 template <bool C, bool D>
 static void launch_wave_shade(const LaunchShape &ls, int blocks, size_t lds, hipStream_t st, const FrameArgs *a,
@@ -790,14 +839,25 @@ static void launch_wave_shade(const LaunchShape &ls, int blocks, size_t lds, hip
     if (ls.shade_occ == 3) hipLaunchKernelGGL((wave_shade_kernel<C, D, 3>), dim3(blocks), dim3(kLanes), lds, st, a, b);
     else hipLaunchKernelGGL((wave_shade_kernel<C, D, 1>), dim3(blocks), dim3(kLanes), lds, st, a, b);
 }
+template <bool C, bool L>
+static void launch_shadow_l(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a) {
+    switch (ls.trace_occ) {
+    case 4: hipLaunchKernelGGL((shadow_kernel<C, 4, L>), dim3(blocks), dim3(kLanes), 0, st, a); break;
+    case 6: hipLaunchKernelGGL((shadow_kernel<C, 6, L>), dim3(blocks), dim3(kLanes), 0, st, a); break;
+    case 8: hipLaunchKernelGGL((shadow_kernel<C, 8, L>), dim3(blocks), dim3(kLanes), 0, st, a); break;
+    default: hipLaunchKernelGGL((shadow_kernel<C, 1, L>), dim3(blocks), dim3(kLanes), 0, st, a); break;
+    }
+}
 template <bool C>
 static void launch_shadow(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a) {
-    switch (ls.trace_occ) {
-    case 4: hipLaunchKernelGGL((shadow_kernel<C, 4>), dim3(blocks), dim3(kLanes), 0, st, a); break;
-    case 6: hipLaunchKernelGGL((shadow_kernel<C, 6>), dim3(blocks), dim3(kLanes), 0, st, a); break;
-    case 8: hipLaunchKernelGGL((shadow_kernel<C, 8>), dim3(blocks), dim3(kLanes), 0, st, a); break;
-    default: hipLaunchKernelGGL((shadow_kernel<C, 1>), dim3(blocks), dim3(kLanes), 0, st, a); break;
-    }
+    if (ls.lds_stack) launch_shadow_l<C, true>(ls, blocks, st, a);
+    else launch_shadow_l<C, false>(ls, blocks, st, a);
+}
+template <bool C>
+static void launch_gbuffer(const LaunchShape &ls, unsigned tiles, hipStream_t st, const FrameArgs *a, const int *t,
+                           int n) {
+    if (ls.lds_stack) hipLaunchKernelGGL((gbuffer_kernel<C, true>), dim3(tiles), dim3(kLanes), 0, st, a, t, n);
+    else hipLaunchKernelGGL((gbuffer_kernel<C, false>), dim3(tiles), dim3(kLanes), 0, st, a, t, n);
 }
 
 struct hdr_view {
@@ -1273,8 +1333,8 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     HIP_TRY(hipMemcpyAsync(const_cast<FrameArgs *>(da), &view->hargs.p[view->ring_next], sizeof(FrameArgs),
                            hipMemcpyHostToDevice, view->gstream));
     HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_GBUFFER], view->gstream));
-    if (view->counting) hipLaunchKernelGGL((gbuffer_kernel<true>), grid, dim3(kLanes), 0, view->gstream, da, dt, 1);
-    else hipLaunchKernelGGL((gbuffer_kernel<false>), grid, dim3(kLanes), 0, view->gstream, da, dt, 1);
+    if (view->counting) launch_gbuffer<true>(view->shape, plan.tiles, view->gstream, da, dt, 1);
+    else launch_gbuffer<false>(view->shape, plan.tiles, view->gstream, da, dt, 1);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_GBUFFER], view->gstream));
     HIP_TRY(hipEventRecord(view->gbuf_done, view->gstream));
@@ -1655,9 +1715,11 @@ static hdr_status shape_of(const hdr_launch_config *c, LaunchShape &ls) {
         return fail(HDR_ERR_INVALID_ARGUMENT, "trace_occupancy must be 0, 1, 4, 6 or 8");
     const int sh = c->shade_occupancy == 0 ? 1 : c->shade_occupancy;
     if (sh != 1 && sh != 3) return fail(HDR_ERR_INVALID_ARGUMENT, "shade_occupancy must be 0, 1 or 3");
+    if (c->lds_stack != 0 && c->lds_stack != 1) return fail(HDR_ERR_INVALID_ARGUMENT, "lds_stack must be 0 or 1");
     ls.queue_waves = c->queue_waves;
     ls.trace_occ = t;
     ls.shade_occ = sh;
+    ls.lds_stack = c->lds_stack != 0;
     return HDR_OK;
 }
 static void config_of(const LaunchShape &ls, bool graphs, hdr_launch_config *c) {
@@ -1667,6 +1729,7 @@ static void config_of(const LaunchShape &ls, bool graphs, hdr_launch_config *c) 
     c->trace_occupancy = ls.trace_occ;
     c->shade_occupancy = ls.shade_occ;
     c->use_graphs = graphs ? 1 : 0;
+    c->lds_stack = ls.lds_stack ? 1 : 0;
 }
 
 hdr_status hdr_view_set_launch_config(hdr_view *view, const hdr_launch_config *cfg) {
@@ -1808,8 +1871,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         hipError_t e = hipMemcpyAsync(sl.dev.p, sl.host.p, off, hipMemcpyHostToDevice, b->stream);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, b->stream, static_cast<unsigned int *>(b->qcount.p));
-        hipLaunchKernelGGL((gbuffer_kernel<false>), dim3(tiles[kAll]), dim3(kLanes), 0, b->stream, A(kAll), T(kAll),
-                           N(kAll));
+        launch_gbuffer<false>(ls, tiles[kAll], b->stream, A(kAll), T(kAll), N(kAll));
         if (N(kDef) > 0)
             hipLaunchKernelGGL((shade_direct_kernel<false, false>), dim3(tiles[kDef]), dim3(kLanes), lds[kDef],
                                b->stream, A(kDef), T(kDef), N(kDef));
@@ -1850,7 +1912,8 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
                                      uint64_t(uintptr_t(b->wqueue.p)), uint64_t(uintptr_t(b->wray.p)),
                                      uint64_t(uintptr_t(b->wcq.p)), uint64_t(uintptr_t(b->walive.p)),
                                      uint64_t(queueCap), uint64_t(waveCap), uint64_t(waveDepth),
-                                     uint64_t(ls.queue_waves), uint64_t(ls.trace_occ), uint64_t(ls.shade_occ)};
+                                     uint64_t(ls.queue_waves), uint64_t(ls.trace_occ), uint64_t(ls.shade_occ),
+                                     uint64_t(ls.lds_stack)};
         for (int g = 0; g < kGroups; ++g) {
             key.push_back(uint64_t(tiles[g]));
             key.push_back(uint64_t(lds[g]));
@@ -1896,8 +1959,9 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
     return HDR_OK;
 }
 
-// Coordinate search over the launch shape: queue waves, then the traversal kernels' occupancy
-// variant, then the wavefront shading kernel's; each candidate is timed (device time, HIP events)
+// Coordinate search over the launch shape: the traversal stacks' home (private memory or LDS
+// rings), queue waves, then the traversal kernels' occupancy variant, then the wavefront shading
+// kernel's; each candidate is timed (device time, HIP events)
 // over `reps` steps after one untimed step, and the fastest is kept. Every candidate renders the
 // same frames exactly (the launch shape moves work between waves, never changes a path's result).
 hdr_status hdr_batch_autotune(hdr_batch *b, int32_t n, hdr_view *const *views, const hdr_scene *const *scenes,
@@ -1928,6 +1992,13 @@ hdr_status hdr_batch_autotune(hdr_batch *b, int32_t n, hdr_view *const *views, c
     LaunchShape best = b->shape;
     float bestMs = 0.0f;
     if (hdr_status st = time_shape(best, bestMs)) return st;
+    {
+        LaunchShape c = best;
+        c.lds_stack = !c.lds_stack;
+        float ms;
+        if (hdr_status st = time_shape(c, ms)) return st;
+        if (ms < bestMs) bestMs = ms, best = c;
+    }
     for (int q : {2048, 4096, 16384, 32768}) {
         LaunchShape c = best;
         c.queue_waves = q;

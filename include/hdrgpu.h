@@ -340,13 +340,15 @@ typedef struct hdr_launch_config {
                                    least that many waves per SIMD */
     int32_t  shade_occupancy;   /* wavefront shading kernel: 0 or 1, or 3 */
     int32_t  use_graphs;        /* batches: 1 = replay captured graphs, 0 = launch every kernel */
-    int32_t  _reserved[3];
+    int32_t  lds_stack;         /* traversal kernels (G-buffer, shadow, wavefront traversal): 1 = the top
+                                   4 TLAS / 8 BLAS stack entries of each lane in LDS, 0 = private memory */
+    int32_t  _reserved[2];
 } hdr_launch_config;
 hdr_status hdr_view_set_launch_config(hdr_view *view, const hdr_launch_config *cfg);
 hdr_status hdr_batch_set_launch_config(hdr_batch *batch, const hdr_launch_config *cfg);
 hdr_status hdr_batch_get_launch_config(hdr_batch *batch, hdr_launch_config *cfg);
-/* Autotuner: renders `reps` + 4 steps of the given views per candidate launch shape (queue waves,
- * then traversal occupancy, then shading occupancy, a coordinate search), keeps the fastest in the
+/* Autotuner: renders `reps` + 4 steps of the given views per candidate launch shape (stack home,
+ * queue waves, traversal occupancy, shading occupancy: a coordinate search), keeps the fastest in the
  * batch and reports it. The views' reservoirs and accumulation advance like any rendered frames
  * (reset them before the frames that count). reps: 1 .. 64. */
 hdr_status hdr_batch_autotune(hdr_batch *batch, int32_t n, hdr_view *const *views, const hdr_scene *const *scenes,
