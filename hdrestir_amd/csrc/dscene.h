@@ -18,6 +18,8 @@
 // hits -- is the reference's.
 #pragma once
 
+#include <climits>
+
 #include "gfm.h"
 
 namespace hdr {
@@ -240,6 +242,7 @@ struct RingStack {
 struct RayBox {
     float o[3], inv[3];
     bool par[3];
+    bool anypar;  // some axis is parallel (the general slab test is needed)
 };
 template <bool kFloatThresholds>
 __device__ __forceinline__ RayBox make_raybox(V3 o, V3 d) {
@@ -252,6 +255,7 @@ __device__ __forceinline__ RayBox make_raybox(V3 o, V3 d) {
         r.par[i] = kFloatThresholds ? (fabsf(dd[i]) < 1e-8f) : (double(fabsf(dd[i])) < 1e-8);
         r.inv[i] = r.par[i] ? 0.0f : 1.0f / dd[i];
     }
+    r.anypar = r.par[0] || r.par[1] || r.par[2];
     return r;
 }
 // Branch-free form of the reference's test, with the same outcome: the per-axis early exits
@@ -260,12 +264,29 @@ __device__ __forceinline__ RayBox make_raybox(V3 o, V3 d) {
 // smax / smin), so a box rejected after some axis is still rejected after the last one. The
 // reference's double threshold double(tmax) > 1e-4 equals tmax > 1e-4f (1e-4f is the largest
 // float below 1e-4, and no float lies between it and 1e-4), so both variants share it.
+//
+// Rays without a parallel axis (nearly all) take a shorter form: the slab ends are finite there
+// (|1/d| <= 1e8 on finite boxes), so the swap is a min / max pair, and v_max / v_min stand for the
+// reference's std::max / std::min (they differ only for NaN operands, which finite slabs do not
+// produce, and in the sign of a zero result, which no comparison sees).
 template <bool kFloatThresholds>
 __device__ __forceinline__ bool box_hit(const RayBox &r, const GNode &n, float &tmin_out) {
     bool miss = n.mnx > n.mxx || n.mny > n.mxy || n.mnz > n.mxz;  // GfRange3f::IsEmpty
-    float tmin = -1e30f, tmax = 1e30f;
     const float mn[3] = {n.mnx, n.mny, n.mnz};
     const float mx[3] = {n.mxx, n.mxy, n.mxz};
+    if (!r.anypar) {
+        float tmin = -1e30f, tmax = 1e30f;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float t1 = (mn[i] - r.o[i]) * r.inv[i];
+            const float t2 = (mx[i] - r.o[i]) * r.inv[i];
+            tmin = fmaxf(tmin, fminf(t1, t2));
+            tmax = fminf(tmax, fmaxf(t1, t2));
+        }
+        tmin_out = tmin;
+        return !miss && !(tmin > tmax) && tmax > 1e-4f;
+    }
+    float tmin = -1e30f, tmax = 1e30f;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const float t1 = (mn[i] - r.o[i]) * r.inv[i];
@@ -286,103 +307,94 @@ __device__ __forceinline__ int node_ref(const GNode &n) {
     return n.left >= 0 ? n.left : -((((-n.left) - 1) << 3) | n.count) - 1;
 }
 
+// No node to process: the stack is popped next (leaf codes stay above it: fewer than 2^28
+// triangles and instance slots, checked on the host).
+constexpr int kNoNode = INT_MIN;
+
 // Scene::Intersect (scene.cpp:409-513) + BVH::_IntersectNode (bvh.cpp:251-305): left subtree
 // fully before the right one, every box tested against the best t at the time the reference
 // reaches it; strict t < best, first found among equal t -- so the reference's winner. Both
 // children of an inner node are tested together (their records are adjacent: one 64-byte
 // fetch); the left child is entered at once, the right one waits on the stack with its entry
 // distance and is pruned against the best t when popped, which is where and how the reference
-// tests it.
+// tests it. The instances of a TLAS leaf are traversed in order, each from the best t so far, and
+// a closer hit replaces the best after its instance.
+//
+// One loop serves both levels: a lane is either in the TLAS (world ray, best t) or in an
+// instance's BLAS (object ray, the instance's t), and an inner node of either level runs the same
+// code, so the lanes of a wave do not wait at level boundaries for each other (a nested
+// BLAS loop would hold every lane still in the TLAS until the slowest lane's instance is done).
 template <bool kCount, class TS, class BS>
 __device__ __forceinline__ bool trace_closest_s(const DeviceScene &s, V3 o, V3 d, RawHit &hit, TraceStats &st,
                                                 TS &tstk, BS &bstk) {
     hit.t = 1e30f;
     hit.inst = -1;
     if (s.tlas_root < 0) return false;
-    const RayBox rt = make_raybox<true>(o, d);
+    RayBox R = make_raybox<true>(o, d);  // the current level's ray
     tstk.reset();
-    bool was = false;
     int cur;
     {
         const GNode n = load_node(s.nodes, s.tlas_root);
         if (kCount) ++st.nodes;
         float ta;
-        if (!box_hit<true>(rt, n, ta) || ta > hit.t) return false;
+        if (!box_hit<true>(R, n, ta) || ta > hit.t) return false;
         cur = node_ref(n);
     }
+    bool was = false, blas = false, world = true;
+    int lnext = 0, lend = 0;  // instance slots of the current TLAS leaf still to traverse
+    int ii = -1, bestTri = -1;
+    float instT = 0.0f, bu = 0.0f, bv = 0.0f;
+    V3 oo = o, od = d;
     for (;;) {
-        if (cur >= 0) {
+        if (cur >= 0) {  // inner node of either level
             const GNode l = load_node(s.nodes, cur), r = load_node(s.nodes, cur + 1);
             if (kCount) st.nodes += 2;
+            const float lim = blas ? instT : hit.t;
             float tl, tr;
-            const bool hl = box_hit<true>(rt, l, tl) && !(tl > hit.t);
-            if (box_hit<true>(rt, r, tr)) tstk.push(node_ref(r), tr);
+            const bool hl = box_hit<true>(R, l, tl) && !(tl > lim);
+            if (box_hit<true>(R, r, tr)) {
+                if (blas) bstk.push(node_ref(r), tr);
+                else tstk.push(node_ref(r), tr);
+            }
             if (hl) {
                 cur = node_ref(l);
                 continue;
             }
-        } else {
+        } else if (cur != kNoNode) {  // leaf
             const int code = -cur - 1;
             const int start = code >> 3, count = code & 7;
-            for (int k = 0; k < count; ++k) {
-                const int ii = s.tlas_idx[start + k];
-                const GInstance &inst = s.instances[ii];
-                if (kCount) ++st.instances;
-                if (inst.root < 0) continue;
-                const V3 oo = xform_point(inst.inv, o);
-                const V3 od = xform_dir(inst.inv, d);
-                const RayBox rb = make_raybox<false>(oo, od);
-                float instT = hit.t;
-                int bestTri = -1;
-                float bu = 0.f, bv = 0.f;
-                bstk.reset();
-                int bcur;
-                bool go;
-                {
-                    const GNode b = load_node(s.nodes, inst.root);
-                    if (kCount) ++st.nodes;
-                    float ta;
-                    go = box_hit<false>(rb, b, ta) && !(ta > instT);
-                    bcur = node_ref(b);
-                }
-                while (go) {
-                    if (bcur >= 0) {
-                        const GNode l = load_node(s.nodes, bcur), r = load_node(s.nodes, bcur + 1);
-                        if (kCount) st.nodes += 2;
-                        float tl, tr;
-                        const bool hl = box_hit<false>(rb, l, tl) && !(tl > instT);
-                        if (box_hit<false>(rb, r, tr)) bstk.push(node_ref(r), tr);
-                        if (hl) {
-                            bcur = node_ref(l);
-                            continue;
-                        }
-                    } else {
-                        const int bcode = -bcur - 1;
-                        const int ts = bcode >> 3, tc = bcode & 7;
-                        for (int q = 0; q < tc; ++q) {
-                            const GTri tr = s.tris[ts + q];
-                            if (kCount) ++st.tris;
-                            float tt, uu, vv;
-                            if (tri_hit(tr, oo, od, tt, uu, vv) && tt < instT) {
-                                instT = tt;
-                                bestTri = ts + q;
-                                bu = uu;
-                                bv = vv;
-                            }
-                        }
-                    }
-                    go = false;
-                    while (!bstk.empty()) {
-                        int rr;
-                        float ta;
-                        bstk.pop(rr, ta);
-                        if (ta > instT) continue;
-                        bcur = rr;
-                        go = true;
-                        break;
+            if (blas) {
+                for (int q = 0; q < count; ++q) {
+                    const GTri tri = s.tris[start + q];
+                    if (kCount) ++st.tris;
+                    float tt, uu, vv;
+                    if (tri_hit(tri, oo, od, tt, uu, vv) && tt < instT) {
+                        instT = tt;
+                        bestTri = start + q;
+                        bu = uu;
+                        bv = vv;
                     }
                 }
-                if (bestTri < 0 || instT >= hit.t) continue;
+            } else {
+                lnext = start;
+                lend = start + count;
+            }
+        }
+        cur = kNoNode;
+        if (blas) {
+            bool more = false;
+            while (!bstk.empty()) {
+                int rr;
+                float ta;
+                bstk.pop(rr, ta);
+                if (ta > instT) continue;
+                cur = rr;
+                more = true;
+                break;
+            }
+            if (more) continue;
+            blas = false;  // the instance is done
+            if (bestTri >= 0) {
                 hit.t = instT;
                 hit.inst = ii;
                 hit.tri = bestTri;
@@ -390,6 +402,33 @@ __device__ __forceinline__ bool trace_closest_s(const DeviceScene &s, V3 o, V3 d
                 hit.v = bv;
                 was = true;
             }
+        }
+        while (lnext < lend) {  // the leaf's next instance
+            ii = s.tlas_idx[lnext++];
+            const GInstance &inst = s.instances[ii];
+            if (kCount) ++st.instances;
+            if (inst.root < 0) continue;
+            oo = xform_point(inst.inv, o);
+            od = xform_dir(inst.inv, d);
+            R = make_raybox<false>(oo, od);
+            world = false;
+            instT = hit.t;
+            bestTri = -1;
+            bu = bv = 0.0f;
+            bstk.reset();
+            const GNode b = load_node(s.nodes, inst.root);
+            if (kCount) ++st.nodes;
+            float ta;
+            if (box_hit<true>(R, b, ta) && !(ta > instT)) {
+                cur = node_ref(b);
+                blas = true;
+                break;
+            }
+        }
+        if (blas) continue;
+        if (!world) {
+            R = make_raybox<true>(o, d);
+            world = true;
         }
         bool more = false;
         while (!tstk.empty()) {
@@ -415,80 +454,85 @@ __device__ __forceinline__ bool trace_closest(const DeviceScene &s, V3 o, V3 d, 
 // tests Depth < Dist - 1e-3 (ris_direct_light_integrator.cpp:513-519, mis_direct_light_integrator.cpp:100-105);
 // the same traversal with the bound fixed at tmax, stopped at the first accepted triangle,
 // answers the same question (a box entered before the closest hit cannot be pruned by a
-// larger bound), so subtrees are pruned as soon as their box is tested.
+// larger bound), so subtrees are pruned as soon as their box is tested. One loop for both levels,
+// as in trace_closest_s.
 template <bool kCount, class TS, class BS>
 __device__ __forceinline__ bool trace_any_s(const DeviceScene &s, V3 o, V3 d, float tmax, TraceStats &st, TS &tstk,
                                             BS &bstk) {
     if (s.tlas_root < 0) return false;
-    const RayBox rt = make_raybox<true>(o, d);
+    RayBox R = make_raybox<true>(o, d);
     tstk.reset();
     int cur;
     {
         const GNode n = load_node(s.nodes, s.tlas_root);
         if (kCount) ++st.nodes;
         float ta;
-        if (!box_hit<true>(rt, n, ta) || ta > tmax) return false;
+        if (!box_hit<true>(R, n, ta) || ta > tmax) return false;
         cur = node_ref(n);
     }
+    bool blas = false, world = true;
+    int lnext = 0, lend = 0;
+    V3 oo = o, od = d;
     for (;;) {
         if (cur >= 0) {
             const GNode l = load_node(s.nodes, cur), r = load_node(s.nodes, cur + 1);
             if (kCount) st.nodes += 2;
             float tl, tr;
-            const bool hl = box_hit<true>(rt, l, tl) && !(tl > tmax);
-            if (box_hit<true>(rt, r, tr) && !(tr > tmax)) tstk.push(node_ref(r), 0.0f);
+            const bool hl = box_hit<true>(R, l, tl) && !(tl > tmax);
+            if (box_hit<true>(R, r, tr) && !(tr > tmax)) {
+                if (blas) bstk.push(node_ref(r), 0.0f);
+                else tstk.push(node_ref(r), 0.0f);
+            }
             if (hl) {
                 cur = node_ref(l);
                 continue;
             }
-        } else {
+        } else if (cur != kNoNode) {
             const int code = -cur - 1;
             const int start = code >> 3, count = code & 7;
-            for (int k = 0; k < count; ++k) {
-                const GInstance &inst = s.instances[s.tlas_idx[start + k]];
-                if (kCount) ++st.instances;
-                if (inst.root < 0) continue;
-                const V3 oo = xform_point(inst.inv, o);
-                const V3 od = xform_dir(inst.inv, d);
-                const RayBox rb = make_raybox<false>(oo, od);
-                bstk.reset();
-                int bcur;
-                bool go;
-                {
-                    const GNode b = load_node(s.nodes, inst.root);
-                    if (kCount) ++st.nodes;
-                    float ta;
-                    go = box_hit<false>(rb, b, ta) && !(ta > tmax);
-                    bcur = node_ref(b);
+            if (blas) {
+                for (int q = 0; q < count; ++q) {
+                    const GTri tri = s.tris[start + q];
+                    if (kCount) ++st.tris;
+                    float tt, uu, vv;
+                    if (tri_hit(tri, oo, od, tt, uu, vv) && tt < tmax) return true;
                 }
-                while (go) {
-                    if (bcur >= 0) {
-                        const GNode l = load_node(s.nodes, bcur), r = load_node(s.nodes, bcur + 1);
-                        if (kCount) st.nodes += 2;
-                        float tl, tr;
-                        const bool hl = box_hit<false>(rb, l, tl) && !(tl > tmax);
-                        if (box_hit<false>(rb, r, tr) && !(tr > tmax)) bstk.push(node_ref(r), 0.0f);
-                        if (hl) {
-                            bcur = node_ref(l);
-                            continue;
-                        }
-                    } else {
-                        const int bcode = -bcur - 1;
-                        const int ts = bcode >> 3, tc = bcode & 7;
-                        for (int q = 0; q < tc; ++q) {
-                            const GTri tr = s.tris[ts + q];
-                            if (kCount) ++st.tris;
-                            float tt, uu, vv;
-                            if (tri_hit(tr, oo, od, tt, uu, vv) && tt < tmax) return true;
-                        }
-                    }
-                    go = !bstk.empty();
-                    if (go) {
-                        float unused;
-                        bstk.pop(bcur, unused);
-                    }
-                }
+            } else {
+                lnext = start;
+                lend = start + count;
             }
+        }
+        cur = kNoNode;
+        if (blas) {
+            if (!bstk.empty()) {
+                float unused;
+                bstk.pop(cur, unused);
+                continue;
+            }
+            blas = false;
+        }
+        while (lnext < lend) {
+            const GInstance &inst = s.instances[s.tlas_idx[lnext++]];
+            if (kCount) ++st.instances;
+            if (inst.root < 0) continue;
+            oo = xform_point(inst.inv, o);
+            od = xform_dir(inst.inv, d);
+            R = make_raybox<false>(oo, od);
+            world = false;
+            bstk.reset();
+            const GNode b = load_node(s.nodes, inst.root);
+            if (kCount) ++st.nodes;
+            float ta;
+            if (box_hit<true>(R, b, ta) && !(ta > tmax)) {
+                cur = node_ref(b);
+                blas = true;
+                break;
+            }
+        }
+        if (blas) continue;
+        if (!world) {
+            R = make_raybox<true>(o, d);
+            world = true;
         }
         if (tstk.empty()) break;
         float unused;
