@@ -26,15 +26,19 @@ struct GRec {
 };
 
 __device__ __forceinline__ void store_grec(GRec *dst, bool hit, const HitRec &g) {
-    GRec r;
-    r.a = make_float4(g.pos.x, g.pos.y, g.pos.z, g.depth);
-    r.b = make_float4(g.ng.x, g.ng.y, g.ng.z, __int_as_float(hit ? g.inst : -1));
-    r.c = make_float4(g.ns.x, g.ns.y, g.ns.z, __int_as_float(g.tri));
-    r.d = make_float4(g.albedo.x, g.albedo.y, g.albedo.z, 0.0f);
-    *dst = r;
+    float4 *p = reinterpret_cast<float4 *>(dst);
+    gstore4(p, make_float4(g.pos.x, g.pos.y, g.pos.z, g.depth));
+    gstore4(p + 1, make_float4(g.ng.x, g.ng.y, g.ng.z, __int_as_float(hit ? g.inst : -1)));
+    gstore4(p + 2, make_float4(g.ns.x, g.ns.y, g.ns.z, __int_as_float(g.tri)));
+    gstore4(p + 3, make_float4(g.albedo.x, g.albedo.y, g.albedo.z, 0.0f));
 }
 __device__ __forceinline__ bool load_grec(const GRec *src, HitRec &g) {
-    const GRec r = *src;
+    const float4 *p = reinterpret_cast<const float4 *>(src);
+    GRec r;
+    r.a = gload4(p);
+    r.b = gload4(p + 1);
+    r.c = gload4(p + 2);
+    r.d = gload4(p + 3);
     g.pos = v3(r.a.x, r.a.y, r.a.z);
     g.depth = r.a.w;
     g.ng = v3(r.b.x, r.b.y, r.b.z);

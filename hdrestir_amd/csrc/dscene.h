@@ -147,9 +147,59 @@ __device__ __forceinline__ bool tri_hit(const GTri &t, V3 o, V3 d, float &tt, fl
     return tt > 1e-4f;
 }
 
+// Loads and stores through the global address space: the scene's and the views' arrays are device
+// allocations, so global_load / global_store (a generic pointer would become flat_load, whose
+// completion the wave must also wait for with the LDS and scalar-load counter).
+typedef float GF4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 gload4(const void *p) {
+    const GF4 v = *(const __attribute__((address_space(1))) GF4 *)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gstore4(void *p, float4 v) {
+    GF4 w;
+    w.x = v.x;
+    w.y = v.y;
+    w.z = v.z;
+    w.w = v.w;
+    *(__attribute__((address_space(1))) GF4 *)p = w;
+}
+__device__ __forceinline__ int gload_i(const int *p) { return *(const __attribute__((address_space(1))) int *)p; }
+
+__device__ __forceinline__ GTri load_tri(const GTri *tris, int i) {
+    const float4 *p = reinterpret_cast<const float4 *>(tris + i);
+    const float4 a = gload4(p), b = gload4(p + 1), c = gload4(p + 2);
+    GTri t;
+    t.v0x = a.x;
+    t.v0y = a.y;
+    t.v0z = a.z;
+    t.orig = __float_as_int(a.w);
+    t.v1x = b.x;
+    t.v1y = b.y;
+    t.v1z = b.z;
+    t._p1 = __float_as_int(b.w);
+    t.v2x = c.x;
+    t.v2y = c.y;
+    t.v2z = c.z;
+    t._p2 = __float_as_int(c.w);
+    return t;
+}
+// An instance's inverse transform and BLAS root (the traversal's view of a GInstance).
+__device__ __forceinline__ int load_inst_inv(const GInstance *insts, int ii, float m[16]) {
+    const float4 *p = reinterpret_cast<const float4 *>(insts + ii);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 v = gload4(p + q);
+        m[4 * q] = v.x;
+        m[4 * q + 1] = v.y;
+        m[4 * q + 2] = v.z;
+        m[4 * q + 3] = v.w;
+    }
+    return __float_as_int(gload4(p + 8).x);  // root: the first int after inv[16] and xf[16]
+}
+
 __device__ __forceinline__ GNode load_node(const GNode *nodes, int i) {
     const float4 *p = reinterpret_cast<const float4 *>(nodes + i);
-    const float4 a = p[0], b = p[1];
+    const float4 a = gload4(p), b = gload4(p + 1);
     GNode n;
     n.mnx = a.x;
     n.mny = a.y;
@@ -365,7 +415,7 @@ __device__ __forceinline__ bool trace_closest_s(const DeviceScene &s, V3 o, V3 d
             const int start = code >> 3, count = code & 7;
             if (blas) {
                 for (int q = 0; q < count; ++q) {
-                    const GTri tri = s.tris[start + q];
+                    const GTri tri = load_tri(s.tris, start + q);
                     if (kCount) ++st.tris;
                     float tt, uu, vv;
                     if (tri_hit(tri, oo, od, tt, uu, vv) && tt < instT) {
@@ -404,19 +454,20 @@ __device__ __forceinline__ bool trace_closest_s(const DeviceScene &s, V3 o, V3 d
             }
         }
         while (lnext < lend) {  // the leaf's next instance
-            ii = s.tlas_idx[lnext++];
-            const GInstance &inst = s.instances[ii];
+            ii = gload_i(s.tlas_idx + lnext++);
+            float minv[16];
+            const int root = load_inst_inv(s.instances, ii, minv);
             if (kCount) ++st.instances;
-            if (inst.root < 0) continue;
-            oo = xform_point(inst.inv, o);
-            od = xform_dir(inst.inv, d);
+            if (root < 0) continue;
+            oo = xform_point(minv, o);
+            od = xform_dir(minv, d);
             R = make_raybox<false>(oo, od);
             world = false;
             instT = hit.t;
             bestTri = -1;
             bu = bv = 0.0f;
             bstk.reset();
-            const GNode b = load_node(s.nodes, inst.root);
+            const GNode b = load_node(s.nodes, root);
             if (kCount) ++st.nodes;
             float ta;
             if (box_hit<true>(R, b, ta) && !(ta > instT)) {
@@ -492,7 +543,7 @@ __device__ __forceinline__ bool trace_any_s(const DeviceScene &s, V3 o, V3 d, fl
             const int start = code >> 3, count = code & 7;
             if (blas) {
                 for (int q = 0; q < count; ++q) {
-                    const GTri tri = s.tris[start + q];
+                    const GTri tri = load_tri(s.tris, start + q);
                     if (kCount) ++st.tris;
                     float tt, uu, vv;
                     if (tri_hit(tri, oo, od, tt, uu, vv) && tt < tmax) return true;
@@ -512,15 +563,16 @@ __device__ __forceinline__ bool trace_any_s(const DeviceScene &s, V3 o, V3 d, fl
             blas = false;
         }
         while (lnext < lend) {
-            const GInstance &inst = s.instances[s.tlas_idx[lnext++]];
+            float minv[16];
+            const int root = load_inst_inv(s.instances, gload_i(s.tlas_idx + lnext++), minv);
             if (kCount) ++st.instances;
-            if (inst.root < 0) continue;
-            oo = xform_point(inst.inv, o);
-            od = xform_dir(inst.inv, d);
+            if (root < 0) continue;
+            oo = xform_point(minv, o);
+            od = xform_dir(minv, d);
             R = make_raybox<false>(oo, od);
             world = false;
             bstk.reset();
-            const GNode b = load_node(s.nodes, inst.root);
+            const GNode b = load_node(s.nodes, root);
             if (kCount) ++st.nodes;
             float ta;
             if (box_hit<true>(R, b, ta) && !(ta > tmax)) {
