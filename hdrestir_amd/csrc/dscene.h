@@ -505,8 +505,9 @@ __device__ __forceinline__ bool trace_closest(const DeviceScene &s, V3 o, V3 d, 
 // tests Depth < Dist - 1e-3 (ris_direct_light_integrator.cpp:513-519, mis_direct_light_integrator.cpp:100-105);
 // the same traversal with the bound fixed at tmax, stopped at the first accepted triangle,
 // answers the same question (a box entered before the closest hit cannot be pruned by a
-// larger bound), so subtrees are pruned as soon as their box is tested. One loop for both levels,
-// as in trace_closest_s.
+// larger bound), so subtrees are pruned as soon as their box is tested. The answer does not depend
+// on the order subtrees are visited in, so the nearer child goes first (occluders are met sooner).
+// One loop for both levels, as in trace_closest_s.
 template <bool kCount, class TS, class BS>
 __device__ __forceinline__ bool trace_any_s(const DeviceScene &s, V3 o, V3 d, float tmax, TraceStats &st, TS &tstk,
                                             BS &bstk) {
@@ -525,17 +526,21 @@ __device__ __forceinline__ bool trace_any_s(const DeviceScene &s, V3 o, V3 d, fl
     int lnext = 0, lend = 0;
     V3 oo = o, od = d;
     for (;;) {
-        if (cur >= 0) {
+        if (cur >= 0) {  // nearer child first: any occluder ends the query, so order is free
             const GNode l = load_node(s.nodes, cur), r = load_node(s.nodes, cur + 1);
             if (kCount) st.nodes += 2;
             float tl, tr;
             const bool hl = box_hit<true>(R, l, tl) && !(tl > tmax);
-            if (box_hit<true>(R, r, tr) && !(tr > tmax)) {
-                if (blas) bstk.push(node_ref(r), 0.0f);
-                else tstk.push(node_ref(r), 0.0f);
-            }
-            if (hl) {
-                cur = node_ref(l);
+            const bool hr = box_hit<true>(R, r, tr) && !(tr > tmax);
+            if (hl || hr) {
+                const bool leftFirst = hl && (!hr || !(tr < tl));
+                const int first = leftFirst ? node_ref(l) : node_ref(r);
+                if (hl && hr) {
+                    const int second = leftFirst ? node_ref(r) : node_ref(l);
+                    if (blas) bstk.push(second, 0.0f);
+                    else tstk.push(second, 0.0f);
+                }
+                cur = first;
                 continue;
             }
         } else if (cur != kNoNode) {
