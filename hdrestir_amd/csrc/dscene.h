@@ -361,19 +361,19 @@ __device__ __forceinline__ int node_ref(const GNode &n) {
 // triangles and instance slots, checked on the host).
 constexpr int kNoNode = INT_MIN;
 
-// Scene::Intersect (scene.cpp:409-513) + BVH::_IntersectNode (bvh.cpp:251-305): left subtree
-// fully before the right one, every box tested against the best t at the time the reference
-// reaches it; strict t < best, first found among equal t -- so the reference's winner. Both
-// children of an inner node are tested together (their records are adjacent: one 64-byte
-// fetch); the left child is entered at once, the right one waits on the stack with its entry
-// distance and is pruned against the best t when popped, which is where and how the reference
-// tests it. The instances of a TLAS leaf are traversed in order, each from the best t so far, and
-// a closer hit replaces the best after its instance.
+// Scene::Intersect (scene.cpp:409-513) + BVH::_IntersectNode (bvh.cpp:251-305) return the closest
+// hit; among hits at the same t the reference keeps the one it meets first in its left-first
+// depth-first order (strict t < best at every level, instances in TLAS-leaf order). That order is
+// the order of the arrays: both builds partition a node's range into [left child, right child),
+// so TLAS slots and each BLAS's triangle records are laid out in left-first DFS order. This
+// traversal visits the nearer child first (a closer hit is found sooner and prunes more) and
+// keeps the reference's winner by breaking t ties on (TLAS slot, triangle record): the earliest
+// in the reference's order wins, whatever order the candidates are met in. A box is pruned when
+// its entry distance exceeds the best t (ties are still visited), at push and again at pop.
 //
-// One loop serves both levels: a lane is either in the TLAS (world ray, best t) or in an
-// instance's BLAS (object ray, the instance's t), and an inner node of either level runs the same
-// code, so the lanes of a wave do not wait at level boundaries for each other (a nested
-// BLAS loop would hold every lane still in the TLAS until the slowest lane's instance is done).
+// One loop serves both levels: a lane is either in the TLAS (world ray) or in an instance's BLAS
+// (object ray), and an inner node of either level runs the same code, so the lanes of a wave do not
+// wait at level boundaries for each other.
 template <bool kCount, class TS, class BS>
 __device__ __forceinline__ bool trace_closest_s(const DeviceScene &s, V3 o, V3 d, RawHit &hit, TraceStats &st,
                                                 TS &tstk, BS &bstk) {
@@ -392,22 +392,23 @@ __device__ __forceinline__ bool trace_closest_s(const DeviceScene &s, V3 o, V3 d
     }
     bool was = false, blas = false, world = true;
     int lnext = 0, lend = 0;  // instance slots of the current TLAS leaf still to traverse
-    int ii = -1, bestTri = -1;
-    float instT = 0.0f, bu = 0.0f, bv = 0.0f;
+    int ii = -1, slot = -1;   // current instance and its TLAS slot
+    int bslot = 0x7fffffff;   // TLAS slot of the best hit (tie-break key with hit.tri)
     V3 oo = o, od = d;
     for (;;) {
         if (cur >= 0) {  // inner node of either level
             const GNode l = load_node(s.nodes, cur), r = load_node(s.nodes, cur + 1);
             if (kCount) st.nodes += 2;
-            const float lim = blas ? instT : hit.t;
             float tl, tr;
-            const bool hl = box_hit<true>(R, l, tl) && !(tl > lim);
-            if (box_hit<true>(R, r, tr)) {
-                if (blas) bstk.push(node_ref(r), tr);
-                else tstk.push(node_ref(r), tr);
-            }
-            if (hl) {
-                cur = node_ref(l);
+            const bool hl = box_hit<true>(R, l, tl) && !(tl > hit.t);
+            const bool hr = box_hit<true>(R, r, tr) && !(tr > hit.t);
+            if (hl || hr) {
+                const bool leftFirst = hl && (!hr || !(tr < tl));
+                if (hl && hr) {
+                    if (blas) bstk.push(leftFirst ? node_ref(r) : node_ref(l), leftFirst ? tr : tl);
+                    else tstk.push(leftFirst ? node_ref(r) : node_ref(l), leftFirst ? tr : tl);
+                }
+                cur = leftFirst ? node_ref(l) : node_ref(r);
                 continue;
             }
         } else if (cur != kNoNode) {  // leaf
@@ -418,11 +419,15 @@ __device__ __forceinline__ bool trace_closest_s(const DeviceScene &s, V3 o, V3 d
                     const GTri tri = load_tri(s.tris, start + q);
                     if (kCount) ++st.tris;
                     float tt, uu, vv;
-                    if (tri_hit(tri, oo, od, tt, uu, vv) && tt < instT) {
-                        instT = tt;
-                        bestTri = start + q;
-                        bu = uu;
-                        bv = vv;
+                    if (tri_hit(tri, oo, od, tt, uu, vv) &&
+                        (tt < hit.t || (tt == hit.t && (slot < bslot || (slot == bslot && start + q < hit.tri))))) {
+                        hit.t = tt;
+                        hit.inst = ii;
+                        hit.tri = start + q;
+                        hit.u = uu;
+                        hit.v = vv;
+                        bslot = slot;
+                        was = true;
                     }
                 }
             } else {
@@ -437,24 +442,17 @@ __device__ __forceinline__ bool trace_closest_s(const DeviceScene &s, V3 o, V3 d
                 int rr;
                 float ta;
                 bstk.pop(rr, ta);
-                if (ta > instT) continue;
+                if (ta > hit.t) continue;
                 cur = rr;
                 more = true;
                 break;
             }
             if (more) continue;
             blas = false;  // the instance is done
-            if (bestTri >= 0) {
-                hit.t = instT;
-                hit.inst = ii;
-                hit.tri = bestTri;
-                hit.u = bu;
-                hit.v = bv;
-                was = true;
-            }
         }
         while (lnext < lend) {  // the leaf's next instance
-            ii = gload_i(s.tlas_idx + lnext++);
+            slot = lnext++;
+            ii = gload_i(s.tlas_idx + slot);
             float minv[16];
             const int root = load_inst_inv(s.instances, ii, minv);
             if (kCount) ++st.instances;
@@ -463,14 +461,11 @@ __device__ __forceinline__ bool trace_closest_s(const DeviceScene &s, V3 o, V3 d
             od = xform_dir(minv, d);
             R = make_raybox<false>(oo, od);
             world = false;
-            instT = hit.t;
-            bestTri = -1;
-            bu = bv = 0.0f;
             bstk.reset();
             const GNode b = load_node(s.nodes, root);
             if (kCount) ++st.nodes;
             float ta;
-            if (box_hit<true>(R, b, ta) && !(ta > instT)) {
+            if (box_hit<true>(R, b, ta) && !(ta > hit.t)) {
                 cur = node_ref(b);
                 blas = true;
                 break;
