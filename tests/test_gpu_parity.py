@@ -189,3 +189,53 @@ def test_render_parity_many_lights_crop_deep(gpu_ctx, oracle_mod):
         assert np.quantile(rel, 0.999) < 1e-3
     r.close()
     gs.close()
+
+
+def test_closest_hit_ties_keep_the_references_winner(gpu_ctx, oracle_mod):
+    """Exact ties: a flat 16x16-quad grid instanced twice at the same place (every hit has an equal-t
+    twin in the other instance) plus a third copy 0.5 below, hit by vertical rays aimed at
+    grid vertices, edge midpoints and quad centres (a vertex or edge hit also ties between the
+    triangles that share it) and by random rays. The traversal visits nearer children first; the
+    winner must still be the reference's (first in its left-first order): instance and triangle
+    identical on every ray."""
+    gpu = _gpu()
+    sc = scenes.Scene("ties", 16, 16)
+    n = 16
+    xs = np.linspace(-1.0, 1.0, n + 1, dtype=np.float32)
+    gx, gz = np.meshgrid(xs, xs)
+    pts = np.stack([gx.ravel(), np.zeros(gx.size, np.float32), gz.ravel()], 1)
+    idx = []
+    for j in range(n):
+        for i in range(n):
+            a = j * (n + 1) + i
+            idx += [a, a + 1, a + n + 2, a, a + n + 2, a + n + 1]
+    m = sc.add_mesh(pts, np.array(idx, np.int32).reshape(-1, 3))
+    mat = sc.add_material(diffuse_color=(0.5, 0.5, 0.5))
+    lift = np.eye(4)
+    lift[3, 1] = -0.5
+    for t in (np.eye(4), np.eye(4), lift):
+        sc.add_instance(m, mat, t)
+    sc.add_light(abi.HDR_LIGHT_DISTANT, (1.0, 1.0, 1.0), 1.0, np.eye(4))
+    sc.cameras = [(scenes.look_at_view((0.0, 3.0, 0.1), (0.0, 0.0, 0.0)), scenes.perspective(60.0, 1.0))]
+    h = np.linspace(-1.0, 1.0, 2 * n + 1, dtype=np.float32)  # vertices and edge midpoints
+    c = (xs[:-1] + xs[1:]) / 2  # quad centres
+    targets = [(x, z) for x in h for z in h] + [(x, z) for x in c for z in c]
+    o = np.array([(x, 2.0, z) for x, z in targets], np.float32)
+    d = np.tile(np.array([[0.0, -1.0, 0.0]], np.float32), (len(o), 1))
+    rs = np.random.RandomState(5)
+    o2 = rs.uniform(-1.2, 1.2, (4000, 3)).astype(np.float32)
+    o2[:, 1] = 2.0
+    d2 = rs.normal(size=(4000, 3))
+    d2[:, 1] = -np.abs(d2[:, 1]) - 0.5
+    d2 = (d2 / np.linalg.norm(d2, axis=1, keepdims=True)).astype(np.float32)
+    o, d = np.concatenate([o, o2]), np.concatenate([d, d2]).astype(np.float32)
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    gi, gt, gtt = gs.intersect(o, d)
+    orc = oracle_mod.Oracle(sc, threads=8)
+    oi, ot, ott = orc.intersect(o, d)
+    hit = oi >= 0
+    assert hit.sum() > 2000  # every vertical ray and about a fifth of the random ones
+    assert np.array_equal(gi, oi), f"instance mismatch on {(gi != oi).sum()} rays"
+    assert np.array_equal(gt[hit], ot[hit]), f"triangle mismatch on {(gt[hit] != ot[hit]).sum()} rays"
+    assert np.array_equal(gtt[hit], ott[hit])
+    gs.close()
