@@ -468,7 +468,7 @@ __global__ void wave_zero_counts_kernel(unsigned int *c) {
     if (threadIdx.x < 8) c[threadIdx.x] = 0u;
 }
 
-template <bool kCount, bool kDebug>
+template <bool kCount, bool kDebug, bool kLean = false>
 __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__restrict__ args,
                                                              const int *__restrict__ tile_base, int n_views) {
     extern __shared__ double wl[];
@@ -482,7 +482,7 @@ __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__
     if (valid) {
         const int pixel = y * a.W + x;
         const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
-        WavePath p;
+        WavePath<typename std::conditional<kLean, Mt64Lean, Mt64Deep>::type> p;
         p.rng.init(seed, wave_slot(a, pixel));
         const float px = float(x) + p.rng.next_float();
         const float py = float(y) + p.rng.next_float();
@@ -544,7 +544,7 @@ wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce) {
             const uint32_t code = args[0].walive[i];
             const FrameArgs &a = args[code >> 24];
             const int pixel = int(code & 0xffffffu);
-            WavePath p;
+            WavePath<Mt64Deep> p;
             load_path(a, pixel, p);
             const SpecTab tab = make_tab(p.utab);
             HitRec h;
@@ -1137,6 +1137,7 @@ struct FramePlan {
     bool lean_pt = false;   // the inlined path tracer (pathlean.h) instead of the megakernel
     bool spatial = false;   // spatial reuse after the temporal pass (deferred path only)
     bool wave = false;      // the wavefront path (RIS at every vertex of deep paths, wavefront.h)
+    bool wave_lean0 = false;  // its primary vertex cannot draw past RNG output 310 (lean stream there)
     int depth = 0;          // path vertices (max_depth)
     size_t lds = 0;         // dynamic LDS of the shading / integration kernel
     unsigned tiles = 0;     // 8x8 tiles of the view
@@ -1209,13 +1210,14 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
     a.color = static_cast<float4 *>(view->color.p);
 
     // upper bound of RNG draws per pixel; past 311 the engine needs its spill slots
-    int bound = 5;
+    int bound = 5, bound0 = 5;  // bound0: through the primary vertex only
     {
         const int sb = scene->max_sample_draws_per_vertex;
         int perVertex = 1 + sb + 1;  // opacity, bounce sample, roulette
         if (a.direct == HDR_DIRECT_MIS) perVertex += 3;
         else perVertex += a.n_nee * 3 + std::max(0, a.n_bsdf - 1) * sb + a.n_cand + 1;
         bound += pipe->max_depth * perVertex;
+        bound0 += perVertex;
     }
     if (scene->any_translucent || bound >= kMtN) {
         if (hdr_status s = ensure(view->spill, P * 2 * kMtN * sizeof(uint64_t))) return s;
@@ -1293,6 +1295,7 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
     a.view = 0;
     plan.deferred = deferred;
     plan.wave = wave;
+    plan.wave_lean0 = wave && bound0 < kMtN - 1;
     plan.depth = pipe->max_depth;
     plan.lean_pt = !deferred && a.direct == HDR_DIRECT_MIS && !scene->any_translucent && bound < kMtN - 1;
     plan.spatial = spatial;
@@ -1390,8 +1393,11 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     } else if (plan.wave) {
         const int qblocks = queue_blocks(view->shape, P);
         hipLaunchKernelGGL(wave_zero_counts_kernel, dim3(1), dim3(kLanes), 0, view->stream, a.wcnt);
-#define HDR_W0(C, D) \
-    hipLaunchKernelGGL((wave_shade0_kernel<C, D>), grid, dim3(kLanes), lds, view->stream, da, dt, 1)
+#define HDR_W0(C, D)                                                                                        \
+    if (plan.wave_lean0)                                                                                    \
+        hipLaunchKernelGGL((wave_shade0_kernel<C, D, true>), grid, dim3(kLanes), lds, view->stream, da, dt, 1); \
+    else                                                                                                    \
+        hipLaunchKernelGGL((wave_shade0_kernel<C, D, false>), grid, dim3(kLanes), lds, view->stream, da, dt, 1)
 #define HDR_WS(C, D) launch_wave_shade<C, D>(view->shape, qblocks, lds, view->stream, da, b)
         if (view->counting) {
             if (debug_out) HDR_W0(true, true);
@@ -1772,6 +1778,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
     std::vector<FramePlan> plans(static_cast<size_t>(n));
     size_t queueCap = 0, waveCap = 0;
     int waveDepth = 0;
+    bool waveLean0 = true;  // every wavefront view's primary vertex stays below RNG output 311
     for (int i = 0; i < n; ++i) {
         if (hdr_status st = prepare_frame(views[i], scenes[i], &frames[i], &pipes[i], false, args[i], plans[i]))
             return st;
@@ -1779,6 +1786,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         if (plans[i].wave) {
             waveCap += size_t(views[i]->W) * views[i]->H;
             waveDepth = std::max(waveDepth, plans[i].depth);
+            waveLean0 = waveLean0 && plans[i].wave_lean0;
         }
     }
     if (queueCap * sizeof(ShadowQuery) > b->queue.n || waveCap * sizeof(ShadowQuery) > b->wqueue.n) {
@@ -1888,8 +1896,12 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
             const int qblocks = queue_blocks(ls, waveCap);
             hipLaunchKernelGGL(wave_zero_counts_kernel, dim3(1), dim3(kLanes), 0, b->stream,
                                static_cast<unsigned int *>(b->wcnt.p));
-            hipLaunchKernelGGL((wave_shade0_kernel<false, false>), dim3(tiles[kWave]), dim3(kLanes), lds[kWave],
-                               b->stream, A(kWave), T(kWave), N(kWave));
+            if (waveLean0)
+                hipLaunchKernelGGL((wave_shade0_kernel<false, false, true>), dim3(tiles[kWave]), dim3(kLanes),
+                                   lds[kWave], b->stream, A(kWave), T(kWave), N(kWave));
+            else
+                hipLaunchKernelGGL((wave_shade0_kernel<false, false, false>), dim3(tiles[kWave]), dim3(kLanes),
+                                   lds[kWave], b->stream, A(kWave), T(kWave), N(kWave));
             for (int v = 0; v < waveDepth; ++v) {
                 if (v > 0) launch_wave_shade<false, false>(ls, qblocks, lds[kWave], b->stream, A(kWave), v);
                 launch_wave_trace<false>(ls, qblocks, b->stream, A(kWave), v);
@@ -1911,7 +1923,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
                                      uint64_t(uintptr_t(sl.host.p)), uint64_t(uintptr_t(b->queue.p)),
                                      uint64_t(uintptr_t(b->wqueue.p)), uint64_t(uintptr_t(b->wray.p)),
                                      uint64_t(uintptr_t(b->wcq.p)), uint64_t(uintptr_t(b->walive.p)),
-                                     uint64_t(queueCap), uint64_t(waveCap), uint64_t(waveDepth),
+                                     uint64_t(queueCap), uint64_t(waveCap), uint64_t(waveDepth), uint64_t(waveLean0),
                                      uint64_t(ls.queue_waves), uint64_t(ls.trace_occ), uint64_t(ls.shade_occ),
                                      uint64_t(ls.lds_stack)};
         for (int g = 0; g < kGroups; ++g) {

@@ -19,6 +19,8 @@
 // the winner's shadow ray; the next vertex adds its term only after that launch.
 #pragma once
 
+#include <type_traits>
+
 #include "deferred.h"
 
 namespace hdr {
@@ -40,8 +42,11 @@ __device__ __forceinline__ Mt64Deep rng_fork(const Mt64Deep &r, uint64_t *slot2)
 constexpr int kWaveFields = 9;
 constexpr uint32_t kWavePrimaryHit = 1u << 31;  // in field 7.x: the G-buffer ray hit
 
+// R: Mt64Deep, or Mt64Lean on the primary vertex when the host has checked that no pixel can draw
+// past output 310 there (the two share Mt64's cursor state, so the record is the same).
+template <class R = Mt64Deep>
 struct WavePath {
-    Mt64Deep rng;
+    R rng;
     Spec tp, total, outer;
     V3 dir;      // direction of the ray that reached the current vertex
     float utab;  // hero-wavelength draw
@@ -69,7 +74,8 @@ __device__ __forceinline__ uint64_t *wave_slot(const FrameArgs &a, int pixel) {
 }
 
 // outer: only at the primary vertex (later vertices leave field 6 alone).
-__device__ __forceinline__ void store_path(const FrameArgs &a, int pixel, const WavePath &p, bool outer) {
+template <class R>
+__device__ __forceinline__ void store_path(const FrameArgs &a, int pixel, const WavePath<R> &p, bool outer) {
     float4 *w = a.wstate;
     const size_t P = wave_pixels(a);
     w[pixel] = pack_u64x2(p.rng.lo, p.rng.lo1);
@@ -84,10 +90,11 @@ __device__ __forceinline__ void store_path(const FrameArgs &a, int pixel, const 
                           (p.primary_hit ? kWavePrimaryHit : 0u);
     w[7 * P + pixel] = make_float4(__uint_as_float(bits), __uint_as_float(p.dbg.lhash), __uint_as_float(p.dbg.shash),
                                    __int_as_float(p.dbg.picks));
-    if (p.rng.k > 467u) w[8 * P + pixel] = pack_u64x2(p.rng.b155, p.rng.b311);  // set once pass A ends
+    if constexpr (std::is_same<R, Mt64Deep>::value)
+        if (p.rng.k > 467u) w[8 * P + pixel] = pack_u64x2(p.rng.b155, p.rng.b311);  // set once pass A ends
 }
 
-__device__ __forceinline__ void load_path(const FrameArgs &a, int pixel, WavePath &p) {
+__device__ __forceinline__ void load_path(const FrameArgs &a, int pixel, WavePath<Mt64Deep> &p) {
     const float4 *w = a.wstate;
     const size_t P = wave_pixels(a);
     const float4 r0 = w[pixel], r1 = w[P + pixel], r2 = w[2 * P + pixel];
@@ -127,9 +134,9 @@ __device__ __forceinline__ void load_path(const FrameArgs &a, int pixel, WavePat
 // connection), RIS direct lighting, path throughput and Russian roulette. Outputs the vertex's
 // shadow query with its contribution `tp * d` (added to the total if the ray is unoccluded) and the
 // bounce ray when the path goes on.
-template <bool kCount, bool kDebug>
+template <bool kCount, bool kDebug, class R>
 __device__ __forceinline__ void wave_vertex(const FrameArgs &a, const SpecTab &tab, int bounce, const HitRec &h,
-                                            WavePath &p, double *wl, int pixel, PixelStats &st, bool &query,
+                                            WavePath<R> &p, double *wl, int pixel, PixelStats &st, bool &query,
                                             ShadowQuery &q, float4 &cq, bool &ray, float4 &ro, float4 &rd) {
     const DeviceScene &s = a.s;
     const bool first = bounce == 0;
