@@ -234,6 +234,41 @@ def deep_path_measurement(gpu, abi, ctx, sc, gs, steps, depth=6):
             "mrays_per_s": round(rays / ms / 1e3, 2)}
 
 
+def large_scene_measurement(gpu, abi, ctx, frames=3):
+    """The large-scene workload at 1080p: 2,093,058-triangle displaced grid + 4000 instances of a
+    256-triangle sphere (3.1 M instanced triangles), 65,536 point / rect lights over 5 decades, a
+    2048x1024 environment map with log-normal texels (dome CDFs built on the device), ReSTIR M = 32
+    with one shadow ray at every vertex of depth-8 paths. Spatial reuse is not available at depth 8
+    (HDR_ERR_UNSUPPORTED), so the reference's temporal reuse runs alone; with an environment the
+    generated BSDF candidates are traced, which the general megakernel does."""
+    from hdrestir_amd import scenes
+    t0 = time.perf_counter()
+    sc = scenes.large_scene(frames=1)
+    built = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gs = gpu.GpuScene(ctx, sc)
+    uploaded = time.perf_counter() - t0
+    view, proj = sc.cameras[0]
+    pipe = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=8)
+    r = gpu.Renderer(ctx, gs, sc.width, sc.height)
+    r.render(abi.frame(view, proj, 0), pipe, read_color=False)
+    r.synchronize()
+    t0 = time.perf_counter()
+    for f in range(1, 1 + frames):
+        r.render(abi.frame(view, proj, f), pipe, read_color=False)
+    r.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / frames
+    out = {"workload": "large_scene_1080p_restir_depth8", "triangles_unique": sc.triangle_counts()[0],
+           "triangles_instanced": sc.triangle_counts()[1], "lights": len(sc.lights), "dome": [2048, 1024],
+           "max_depth": 8, "candidate_count": 32, "spatial_reuse": "not available at depth > 1",
+           "scene_generate_s": round(built, 2), "scene_upload_s": round(uploaded, 2),
+           "scene_device_bytes": gs.device_bytes(), "frames": frames, "ms_per_frame": round(ms, 2),
+           "frames_per_s": round(1000.0 / ms, 3)}
+    r.close()
+    gs.close()
+    return out
+
+
 def multiview_measurement(gpu, abi, ctx, gs, steps, seed=3):
     """16 views of the many-lights scene in one batch: sizes uniform in [320, 1920] x [240, 1080],
     cameras on a 40 m sphere around the origin, ReSTIR M = 32 at every vertex of depth-6 paths.
@@ -298,7 +333,7 @@ def main():
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
     ap.add_argument("--no-serving", action="store_true", help="skip the request-serving measurement")
     ap.add_argument("--no-deep", action="store_true",
-                    help="skip the depth-6 single-view and 16-view batch measurements")
+                    help="skip the depth-6 single-view, 16-view batch and large-scene measurements")
     ap.add_argument("--deep-only", action="store_true",
                     help="only the depth-6 single-view measurement, --steps frames (for profilers)")
     ap.add_argument("--headline-only", action="store_true",
@@ -479,6 +514,9 @@ def main():
         _log("multi-view batch")
         line["multiview"] = multiview_measurement(gpu, abi, ctx, gs, 4)
         _log(f"multi-view {line['multiview']}")
+        _log("large scene")
+        line["large_scene"] = large_scene_measurement(gpu, abi, ctx)
+        _log(f"large scene {line['large_scene']}")
     if not args.no_serving and not args.headline_only:
         _log("serving")
         line["serving"] = serving_measurement(gpu, abi, ctx, sc, gs, args.serving_requests)

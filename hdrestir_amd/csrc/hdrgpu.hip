@@ -396,6 +396,45 @@ __global__ void __launch_bounds__(kLanes) spatial_kernel(const FrameArgs *__rest
     if (kCount) wave_add(a.counters + 2, query ? 1ULL : 0ULL);
 }
 
+// DomeLight::Prepare (dome_light.cpp:49-76) at scene upload: per row y, the luminance of each texel
+// weighted by sin(theta_y), its running sum as the column CDF and the row's total, summed texel by
+// texel in the reference's order (one thread per row keeps that order), then normalised by the
+// row total; the row totals go to row[y + 1] for dome_cdf_total_kernel.
+__global__ void dome_cdf_rows_kernel(const float *__restrict__ pix, const float *__restrict__ sin_theta, int w, int h,
+                                     float *row, float *col) {
+    const int y = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (y >= h) return;
+    const float sinTheta = sin_theta[y];
+    float *c = col + size_t(y) * size_t(w + 1);
+    const float *p = pix + size_t(y) * size_t(w) * 3;
+    float rowLum = 0.0f;
+    c[0] = 0.0f;
+    for (int x = 0; x < w; ++x) {
+        float lum = 0.2126f * p[3 * x] + 0.7152f * p[3 * x + 1] + 0.0722f * p[3 * x + 2];
+        lum *= sinTheta;
+        rowLum += lum;
+        c[x + 1] = c[x] + lum;
+    }
+    if (rowLum > 0)
+        for (int x = 0; x <= w; ++x) c[x] /= rowLum;
+    row[y + 1] = rowLum;
+}
+// The row CDF (running sum of the row totals, normalised by their sum) and the dome's total
+// luminance and pdf scale (totalLum * (pi/h) * (2pi/w)), in the reference's order on one thread.
+__global__ void dome_cdf_total_kernel(float *row, int w, int h, GDome *rec) {
+    float total = 0.0f;
+    row[0] = 0.0f;
+    for (int y = 0; y < h; ++y) {
+        const float rowLum = row[y + 1];
+        total += rowLum;
+        row[y + 1] = row[y] + rowLum;
+    }
+    if (total > 0)
+        for (int y = 0; y <= h; ++y) row[y] /= total;
+    rec->total_lum = total;
+    rec->pdf_scale = total * (float(M_PI) / float(h)) * (2.0f * float(M_PI) / float(w));
+}
+
 // Clears a counter (a kernel rather than a memset, so the step can sit in a captured graph).
 __global__ void zero_u32_kernel(unsigned int *p) { *p = 0u; }
 
@@ -971,8 +1010,8 @@ hdr_status hdr_scene_upload(hdr_context *ctx, const hdr_scene_desc *desc, hdr_sc
     std::vector<size_t> oDomePix(h.domes.size()), oDomeRow(h.domes.size()), oDomeCol(h.domes.size());
     for (size_t k = 0; k < h.domes.size(); ++k) {
         oDomePix[k] = place(h.domes[k].pixels.size() * sizeof(float));
-        oDomeRow[k] = place(h.domes[k].row_cdf.size() * sizeof(float));
-        oDomeCol[k] = place(h.domes[k].col_cdf.size() * sizeof(float));
+        oDomeRow[k] = place(size_t(h.domes[k].h + 1) * sizeof(float));
+        oDomeCol[k] = place(size_t(h.domes[k].h) * size_t(h.domes[k].w + 1) * sizeof(float));
     }
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(sc->mem.alloc(off));
@@ -990,17 +1029,31 @@ hdr_status hdr_scene_upload(hdr_context *ctx, const hdr_scene_desc *desc, hdr_sc
     std::vector<GDome> gd(h.domes.size());
     for (size_t k = 0; k < h.domes.size(); ++k) {
         HIP_TRY(up(oDomePix[k], h.domes[k].pixels.data(), h.domes[k].pixels.size() * sizeof(float)));
-        HIP_TRY(up(oDomeRow[k], h.domes[k].row_cdf.data(), h.domes[k].row_cdf.size() * sizeof(float)));
-        HIP_TRY(up(oDomeCol[k], h.domes[k].col_cdf.data(), h.domes[k].col_cdf.size() * sizeof(float)));
         gd[k].pixels = reinterpret_cast<const float *>(base + oDomePix[k]);
         gd[k].row_cdf = reinterpret_cast<const float *>(base + oDomeRow[k]);
         gd[k].col_cdf = reinterpret_cast<const float *>(base + oDomeCol[k]);
         gd[k].w = h.domes[k].w;
         gd[k].h = h.domes[k].h;
-        gd[k].total_lum = h.domes[k].total_lum;
-        gd[k].pdf_scale = h.domes[k].pdf_scale;
+        gd[k].total_lum = 0.0f;  // set by dome_cdf_total_kernel
+        gd[k].pdf_scale = 0.0f;
     }
     HIP_TRY(up(oDomes, gd.data(), gd.size() * sizeof(GDome)));
+    // DomeLight::Prepare on the device: column CDFs one row per thread, then the row CDF
+    for (size_t k = 0; k < h.domes.size(); ++k) {
+        const HostScene::Dome &dm = h.domes[k];
+        DevBuf sinT;
+        HIP_TRY(sinT.alloc(size_t(dm.h) * sizeof(float)));
+        HIP_TRY(hipMemcpy(sinT.p, dm.sin_theta.data(), size_t(dm.h) * sizeof(float), hipMemcpyHostToDevice));
+        float *row = reinterpret_cast<float *>(base + oDomeRow[k]);
+        float *col = reinterpret_cast<float *>(base + oDomeCol[k]);
+        hipLaunchKernelGGL(dome_cdf_rows_kernel, dim3((dm.h + 63) / 64), dim3(64), 0, 0,
+                           reinterpret_cast<const float *>(base + oDomePix[k]), static_cast<const float *>(sinT.p),
+                           dm.w, dm.h, row, col);
+        hipLaunchKernelGGL(dome_cdf_total_kernel, dim3(1), dim3(1), 0, 0, row, dm.w, dm.h,
+                           reinterpret_cast<GDome *>(base + oDomes) + k);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipDeviceSynchronize());
+    }
     DeviceScene &ds = sc->ds;
     ds.nodes = reinterpret_cast<const GNode *>(base + oNodes);
     ds.tris = reinterpret_cast<const GTri *>(base + oTris);
@@ -1026,6 +1079,21 @@ hdr_status hdr_scene_upload(hdr_context *ctx, const hdr_scene_desc *desc, hdr_sc
 hdr_status hdr_scene_destroy(hdr_scene *scene) {
     if (scene) (void)hipSetDevice(scene->ctx->device);
     delete scene;
+    return HDR_OK;
+}
+
+hdr_status hdr_scene_read_dome(const hdr_scene *scene, int32_t light, float *row_cdf, float *col_cdf,
+                                float *total_lum) {
+    if (!scene || !row_cdf || !col_cdf || !total_lum) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    if (light < 0 || light >= int32_t(scene->hs.lights.size()) || scene->hs.lights[size_t(light)].dome < 0)
+        return fail(HDR_ERR_INVALID_ARGUMENT, "not a dome light with texels");
+    HIP_TRY(hipSetDevice(scene->ctx->device));
+    const int k = scene->hs.lights[size_t(light)].dome;
+    GDome gd;
+    HIP_TRY(hipMemcpy(&gd, scene->ds.domes + k, sizeof(GDome), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(row_cdf, gd.row_cdf, size_t(gd.h + 1) * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(col_cdf, gd.col_cdf, size_t(gd.h) * size_t(gd.w + 1) * sizeof(float), hipMemcpyDeviceToHost));
+    *total_lum = gd.total_lum;
     return HDR_OK;
 }
 

@@ -470,29 +470,12 @@ std::string build_host_scene(const hdr_scene_desc &d, HostScene &out) {
             dm.h = s.dome_height;
             const int w = dm.w, h = dm.h;
             dm.pixels.assign(s.dome_pixels, s.dome_pixels + size_t(w) * h * 3);
-            dm.row_cdf.assign(h + 1, 0.0f);
-            dm.col_cdf.assign(size_t(h) * (w + 1), 0.0f);
-            // DomeLight::Prepare, dome_light.cpp:49-76
-            float total = 0.0f;
-            for (int y = 0; y < h; ++y) {
-                float rowLum = 0.0f;
-                const float sinTheta = float(std::sin(M_PI * double(float(y + 0.5f)) / double(float(h))));
-                for (int x = 0; x < w; ++x) {
-                    const size_t p = size_t(y * w + x) * 3;
-                    float lum = 0.2126f * dm.pixels[p] + 0.7152f * dm.pixels[p + 1] + 0.0722f * dm.pixels[p + 2];
-                    lum *= sinTheta;
-                    rowLum += lum;
-                    dm.col_cdf[size_t(y) * (w + 1) + x + 1] = dm.col_cdf[size_t(y) * (w + 1) + x] + lum;
-                }
-                if (rowLum > 0)
-                    for (int x = 0; x <= w; ++x) dm.col_cdf[size_t(y) * (w + 1) + x] /= rowLum;
-                total += rowLum;
-                dm.row_cdf[y + 1] = dm.row_cdf[y] + rowLum;
-            }
-            if (total > 0)
-                for (int y = 0; y <= h; ++y) dm.row_cdf[y] /= total;
-            dm.total_lum = total;
-            dm.pdf_scale = total * (float(M_PI) / float(h)) * (2.0f * float(M_PI) / float(w));
+            // DomeLight::Prepare (dome_light.cpp:49-76) runs on the device at upload (the row and
+            // column CDFs over every texel, dome_cdf_rows_kernel / dome_cdf_total_kernel); the host
+            // keeps the per-row sin(theta) weights, evaluated with the host libm as the reference does.
+            dm.sin_theta.resize(size_t(h));
+            for (int y = 0; y < h; ++y)
+                dm.sin_theta[size_t(y)] = float(std::sin(M_PI * double(float(y + 0.5f)) / double(float(h))));
             g.dome = int(out.domes.size());
             out.domes.push_back(std::move(dm));
         }

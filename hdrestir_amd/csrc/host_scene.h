@@ -18,9 +18,9 @@ struct HostScene {
     std::vector<GMaterial> materials; // num_materials records + the default material
     std::vector<GLight> lights;
     struct Dome {
-        std::vector<float> pixels, row_cdf, col_cdf;
-        int w = 0, h = 0;
-        float total_lum = 0.0f, pdf_scale = 0.0f;
+        std::vector<float> pixels;     // w*h*3 texels
+        std::vector<float> sin_theta;  // per-row weight of DomeLight::Prepare
+        int w = 0, h = 0;              // the CDFs (h+1 rows, h*(w+1) columns) are built on the device
     };
     std::vector<Dome> domes;
     int tlas_root = -1;

@@ -68,6 +68,7 @@ def lib():
             "hdr_scene_upload": [P, C.POINTER(abi.SceneDesc), C.POINTER(P)],
             "hdr_scene_destroy": [P],
             "hdr_scene_device_bytes": [P, C.POINTER(C.c_uint64)],
+            "hdr_scene_read_dome": [P, I, P, P, C.POINTER(C.c_float)],
             "hdr_view_create": [P, I, I, C.POINTER(P)],
             "hdr_view_destroy": [P],
             "hdr_view_reset": [P],
@@ -159,6 +160,14 @@ class GpuScene:
         v = C.c_uint64()
         check(lib().hdr_scene_device_bytes(self.h, C.byref(v)))
         return int(v.value)
+
+    def dome_cdf(self, light, w, h):
+        """(row_cdf, col_cdf (h, w+1), total luminance) of dome light `light`, as built on the device."""
+        row = np.zeros(h + 1, np.float32)
+        col = np.zeros((h, w + 1), np.float32)
+        tot = C.c_float()
+        check(lib().hdr_scene_read_dome(self.h, light, _p(row), _p(col), C.byref(tot)))
+        return row, col, float(tot.value)
 
     def intersect(self, origins, dirs):
         origins = np.ascontiguousarray(origins, np.float32)

@@ -399,15 +399,16 @@ def box_spheres(width=256, height=256, seed=0, frames=16):
 
 
 def many_lights(width=1920, height=1080, seed=0, frames=64, n_lights=4096, n_spheres=500, grid=512,
-                dolly=True):
-    """Height field + instanced spheres + many point/rect lights (the 1080p ReSTIR config)."""
+                dolly=True, sphere_rings=11, intensity_decades=4.0, dome=None, name="many_lights"):
+    """Height field + instanced spheres + many point/rect lights (the 1080p ReSTIR config).
+    dome=(w, h, sigma) adds an environment map with log-normal texels exp(N(0, sigma))."""
     rng = np.random.default_rng(seed)
-    sc = Scene("many_lights", width, height)
+    sc = Scene(name, width, height)
     size = 100.0
     hp, hi, hn, hgt = value_noise_heightfield(grid, size, 4, 6.0, rng)
     m_h = sc.add_mesh(hp, hi, normals=hn)
     sc.add_instance(m_h, _random_material(sc, rng))
-    up, ui, un = uv_sphere(16, 11)
+    up, ui, un = uv_sphere(16, sphere_rings)
     m_s = sc.add_mesh(up, ui, normals=un)
     xs = np.linspace(-size / 2, size / 2, grid)
 
@@ -423,7 +424,7 @@ def many_lights(width=1920, height=1080, seed=0, frames=64, n_lights=4096, n_sph
     for k in range(n_lights):
         x, z = (float(v) for v in rng.uniform(-size * 0.48, size * 0.48, 2))
         y = height_at(x, z) + float(rng.uniform(2.0, 10.0))
-        inten = float(10.0 ** rng.uniform(0.0, 4.0))
+        inten = float(10.0 ** rng.uniform(0.0, intensity_decades))
         col = blackbody_rgb(float(rng.uniform(2000.0, 10000.0)))
         if k < n_point:
             sc.add_light(abi.HDR_LIGHT_POINT, col, inten, _translate_scale((x, y, z), 1.0))
@@ -431,6 +432,10 @@ def many_lights(width=1920, height=1080, seed=0, frames=64, n_lights=4096, n_sph
             side = float(rng.uniform(0.05, 0.5))
             sc.add_light(abi.HDR_LIGHT_RECT, col, inten, _rect_light_down((x, y, z), float(rng.uniform(0, math.pi))),
                          width=side, height=side)
+    if dome is not None:
+        dw, dh, sigma = dome
+        tex = np.exp(rng.normal(0.0, sigma, (dh, dw, 3))).astype(np.float32)
+        sc.environment = sc.add_light(abi.HDR_LIGHT_DOME, (1.0, 1.0, 1.0), 1.0, np.eye(4), dome_pixels=tex)
     proj = perspective(55.0, width / height, 0.1, 1000.0)
     cams = []
     for f in range(frames):
@@ -441,7 +446,16 @@ def many_lights(width=1920, height=1080, seed=0, frames=64, n_lights=4096, n_sph
     return sc
 
 
-CONFIGS = {"box_spheres": box_spheres, "many_lights": many_lights}
+def large_scene(width=1920, height=1080, seed=0, frames=8, n_lights=65536, n_meshes=4000, grid=1024,
+                dolly=False, dome=(2048, 1024, 2.0)):
+    """The large-scene workload: a 1024x1024-vertex displaced grid (2,093,058 triangles) plus 4000
+    instances of a 256-triangle sphere, 65,536 point / rect lights with intensities over 5 decades,
+    and a 2048x1024 environment map with log-normal texels exp(N(0, 2))."""
+    return many_lights(width, height, seed, frames, n_lights, n_meshes, grid, dolly, sphere_rings=9,
+                       intensity_decades=5.0, dome=dome, name="large")
+
+
+CONFIGS = {"box_spheres": box_spheres, "many_lights": many_lights, "large": large_scene}
 
 
 def _dome_texels(w, h, rng):

@@ -252,11 +252,18 @@ hdr_status hdr_context_set_cancel(hdr_context *ctx, int32_t cancelled);
 
 /* Builds BLAS/TLAS (midpoint-split, as Scene::BuildRenderState / BVH::Build), light and
  * material tables on the host and uploads them (reference: Scene::BuildRenderState,
- * source/scene/scene.cpp:118-271). The scene stays resident until hdr_scene_destroy. */
+ * source/scene/scene.cpp:118-271); a dome light's importance-sampling CDFs
+ * (DomeLight::Prepare, source/scene/dome_light.cpp:49-76) are built on the device from the
+ * uploaded texels. The scene stays resident until hdr_scene_destroy. */
 hdr_status hdr_scene_upload(hdr_context *ctx, const hdr_scene_desc *desc, hdr_scene **out);
 hdr_status hdr_scene_destroy(hdr_scene *scene);
 /* device bytes held by the scene */
 hdr_status hdr_scene_device_bytes(const hdr_scene *scene, uint64_t *out);
+/* Reads back the dome CDFs of light `light` as built on the device (DomeLight::Prepare,
+ * dome_light.cpp:49-76): row_cdf h+1 floats, col_cdf h*(w+1) floats (row-major), the total
+ * sin-weighted luminance. HDR_ERR_INVALID_ARGUMENT for a light without texels. */
+hdr_status hdr_scene_read_dome(const hdr_scene *scene, int32_t light, float *row_cdf, float *col_cdf,
+                               float *total_lum);
 
 /* A view owns the device-resident persistent buffers of one pipeline instance:
  * reservoirs (__ris_reservoir) and accumulation (__accum_color), plus the G-buffer. */

@@ -2235,13 +2235,6 @@ float ShiftJacobian(const Light::Sample &atN, const Light::Sample &atQ) {
     return (cosQ / (atQ.Dist * atQ.Dist)) / (cosN / (atN.Dist * atN.Dist));
 }
 
-void CapHistory(Reservoir &r, int cap) {
-    if (r.CandidateCount > cap) {
-        r.WSum = r.WSum * (float(cap) / float(r.CandidateCount));
-        r.CandidateCount = cap;
-    }
-}
-
 struct oracle_ctx {
     Scene scene;
     int W, H;
@@ -2635,6 +2628,17 @@ void oracle_read_counters(oracle_ctx *c, hdr_frame_counters *out) {
     out->node_visits = c->last.nodes;
     out->triangle_tests = c->last.tris;
     out->instance_visits = c->last.instances;
+}
+
+// The dome CDFs of light `light` (DomeLight::Prepare): row_cdf h+1 floats, col_cdf h*(w+1), total.
+int32_t oracle_dome_cdf(oracle_ctx *c, int32_t light, float *row_cdf, float *col_cdf, float *total_lum) {
+    if (light < 0 || light >= int32_t(c->scene.lights.size())) return -1;
+    const Light &l = c->scene.lights[size_t(light)];
+    if (l.pixels.empty()) return -1;
+    std::copy(l.rowCdf.begin(), l.rowCdf.end(), row_cdf);
+    std::copy(l.colCdf.begin(), l.colCdf.end(), col_cdf);
+    *total_lum = l.totalLum;
+    return 0;
 }
 
 void oracle_intersect(oracle_ctx *c, const float *origins, const float *dirs, int32_t n, int32_t *inst, int32_t *tri,

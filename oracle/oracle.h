@@ -46,6 +46,7 @@ void oracle_read_aovs(oracle_ctx *ctx, float *depth, float *albedo, float *norma
 void oracle_read_counters(oracle_ctx *ctx, hdr_frame_counters *out);
 
 /* Scene::IntersectScene on arbitrary rays; triangle index is within its mesh. */
+int32_t oracle_dome_cdf(oracle_ctx *ctx, int32_t light, float *row_cdf, float *col_cdf, float *total_lum);
 void oracle_intersect(oracle_ctx *ctx, const float *origins, const float *dirs, int32_t n,
                       int32_t *instance_out, int32_t *triangle_out, float *t_out);
 

@@ -39,6 +39,8 @@ def lib():
         L.oracle_read_aovs.argtypes = [P, P, P, P]
         L.oracle_read_counters.argtypes = [P, C.POINTER(abi.FrameCounters)]
         L.oracle_intersect.argtypes = [P, P, P, C.c_int32, P, P, P]
+        L.oracle_dome_cdf.argtypes = [P, C.c_int32, P, P, P]
+        L.oracle_dome_cdf.restype = C.c_int32
         L.oracle_rng_raw.argtypes = [P, C.c_int32, C.c_int32, P]
         L.oracle_rng_float.argtypes = [P, C.c_int32, C.c_int32, P]
         L.oracle_camera_rays.argtypes = [C.POINTER(abi.FrameDesc), C.c_int32, C.c_int32, C.c_int32, P, P]
@@ -99,6 +101,15 @@ class Oracle:
         c = abi.FrameCounters()
         lib().oracle_read_counters(self.h, C.byref(c))
         return c.as_dict()
+
+    def dome_cdf(self, light, w, h):
+        """(row_cdf, col_cdf (h, w+1), total luminance) of dome light `light`."""
+        row = np.zeros(h + 1, np.float32)
+        col = np.zeros((h, w + 1), np.float32)
+        tot = C.c_float()
+        if lib().oracle_dome_cdf(self.h, light, _p(row), _p(col), C.byref(tot)) != 0:
+            raise ValueError("not a dome light with texels")
+        return row, col, float(tot.value)
 
     def intersect(self, origins, dirs):
         origins = np.ascontiguousarray(origins, np.float32)
