@@ -306,6 +306,277 @@ __device__ __forceinline__ Spec ris_direct_lean(const FrameArgs &a, const SpecTa
     return tp * W;
 }
 
+// ---------------------------------------------------------------- traced candidates
+// RisDirectLightIntegrator::Li on a vertex whose BSDF-sampled candidates can see an emitter: the
+// environment (escaped generated candidates) or an emissive material (the path's own connection).
+// No mesh light is present (the host checks), so a generated candidate that hits geometry has zero
+// weight (:341-368) and only needs to know whether its ray escapes -- an any-hit query with no bound,
+// which accepts exactly the hits the reference's closest-hit query would find. The connection
+// candidate uses the bounce ray's closest hit, traced by the caller before this call.
+
+// _evaluateBSDFConnection (:547-634) via the connection's hit, with the candidate's RIS weight
+// (:263-294).
+__device__ __forceinline__ double conn_weight_lean(const FrameArgs &a, const SpecTab &tab, const Bounce &b, bool hasHit,
+                                                   const HitRec &hit, Cand *out) {
+    const DeviceScene &s = a.s;
+    Spec emission = spec(0.0f);
+    float pNee = 0.0f;
+    if (!hasHit) {
+        if (s.environment >= 0) {
+            const float sel = a.light_select_pdf;
+            const float envPdf = env_pdf(s, b.d);
+            pNee = sel > 0.0f ? sel * envPdf : (a.n_sky > 0 ? envPdf : 0.0f);
+            emission = rgb2spec(tab, env_radiance(s, b.d));
+        }
+    } else {
+        emission = rgb2spec(tab, hit_emission(s, hit));
+        const int hl = s.instances[hit.inst].light;
+        if (hl >= 0) {
+            const float sel = a.light_select_pdf;
+            const V3 off = hit.pos - b.o;
+            const float dist2 = dot(off, off);
+            const float cosY = smax(0.0f, dot(-b.d, hit.ng));
+            if (cosY > 0.0f && sel > 0.0f) {
+                bool solid;
+                const float ap = light_eval_pdf(s, s.lights[hl], b.d, solid);
+                pNee = sel * to_solid_angle(ap, solid, dist2, cosY);
+            }
+        }
+    }
+    const Spec contribution = emission * b.imul;
+    const float target = spec_lum(tab, contribution);
+    float pNeeSampler = pNee, pNeeSky = 0.0f;
+    if (!hasHit && (s.environment < 0 || a.light_select_pdf <= 0.0f)) {
+        pNeeSampler = 0.0f;
+        pNeeSky = pNee;
+    }
+    const double mis = double(power3(b.pdf, a.n_bsdf, pNeeSampler, a.n_nee, pNeeSky, a.n_sky));
+    if (out) {
+        out->has_cand = false;
+        out->vis_tested = true;
+        out->light = -1;
+        out->tp = contribution;
+        out->target = target;
+    }
+    return mis * double(target) / (double(b.pdf) + 1e-9);
+}
+
+// A generated BSDF candidate whose ray escaped to the environment (:296-340).
+__device__ __forceinline__ double gen_env_weight(const FrameArgs &a, const SpecTab &tab, const Bounce &b, Cand *out) {
+    const DeviceScene &s = a.s;
+    if (s.environment < 0) return 0.0;
+    const V3 wi = b.d;
+    const V3 envRGB = env_radiance(s, wi);
+    const Spec envEm = rgb2spec(tab, envRGB);
+    const float pBsdf = b.pdf;
+    const float sel = a.light_select_pdf;
+    const float envPdf = env_pdf(s, wi);
+    const float pNeeSampler = sel * envPdf;
+    const float pNeeSky = (sel <= 0.0f && a.n_sky > 0) ? envPdf : 0.0f;
+    const float pNee = pNeeSampler + pNeeSky;
+    const Spec tp = envEm * b.imul;
+    const float target = spec_lum(tab, tp);
+    const double mis = double(power3(pBsdf, a.n_bsdf, pNeeSampler, a.n_nee, pNeeSky, a.n_sky));
+    if (out) {
+        out->has_cand = pNee > 0.0f;
+        out->delta = false;
+        out->solid = true;
+        out->vis_tested = true;
+        out->light = s.environment;
+        out->dir = wi;
+        out->color = envRGB;
+        out->lnormal = -wi;
+        out->dist = 1e30f;
+        out->pdf = pNee;
+        out->tp = tp;
+        out->target = target;
+    }
+    return mis * double(target) / (double(pBsdf) + 1e-9);
+}
+
+// Rebuilds candidate kk by replaying the stream from `rng` (taken before candidate 0): NEE
+// candidates as rebuild_nee_lean, the connection from its hit, a generated candidate by replaying
+// the draws before it and sampling it again (esc: which generated candidates escaped).
+template <bool kDebug, class R>
+__device__ __forceinline__ void rebuild_traced(const FrameArgs &a, const SpecTab &tab, const Vertex &v,
+                                               const Bounce *conn, bool connHit, const HitRec &connRec,
+                                               bool connCand, R rng, uint32_t esc, int kk, Trace &dbg, Cand &out) {
+    const DeviceScene &s = a.s;
+    const int nNee = a.n_nee;
+    if (kk < nNee) {
+        rebuild_nee_lean<kDebug>(a, tab, v, rng, kk, dbg, out);
+        return;
+    }
+    for (int k = 0; k < nNee; ++k) {
+        const int li = pick_light_lean<kDebug>(a, rng, dbg, false);
+        const int nd = light_draws(s, s.lights[li]);
+        for (int q = 0; q < nd; ++q) rng.discard();
+    }
+    int k = nNee + a.n_sky;
+    if (connCand) {
+        if (k == kk) {
+            (void)conn_weight_lean(a, tab, *conn, connHit, connRec, &out);
+            return;
+        }
+        ++k;
+    }
+    for (; k < kk; ++k) skip_bounce_draws_inl(tab, v.c, v.sn, v.inside, v.rayDir, rng);
+    Bounce b;
+    int refl = 0, refr = 0;
+    (void)sample_bounce_inl(tab, v.c, v.sn, v.inside, v.h.pos, v.rayDir, 1, 1, refl, refr, rng, b);
+    if ((esc >> kk) & 1u) (void)gen_env_weight(a, tab, b, &out);
+}
+
+// RisDirectLightIntegrator::Li (:84-215) with traced BSDF candidates; weights of every candidate in
+// LDS (wl[k * kLanes], k < n_cand). Shadow query and return value as ris_direct_lean.
+template <bool kCount, bool kDebug, class R>
+__device__ __forceinline__ Spec ris_direct_traced(const FrameArgs &a, const SpecTab &tab, const Vertex &v,
+                                                  const Bounce *conn, bool connHit, const HitRec &connRec, R &rng,
+                                                  double *wl, int pixel, PixelStats &st, Trace &dbg, bool reservoir,
+                                                  uint64_t *slot2, bool &query, V3 &qo, V3 &qd, float &qtmax) {
+    const DeviceScene &s = a.s;
+    const int nNee = a.n_nee, nCand = a.n_cand;
+    const R start = rng_fork(rng, slot2);
+    for (int k = 0; k < nNee; ++k) {
+        double w = 0.0;
+        if (s.num_lights > 0) {
+            const int li = pick_light_lean<kDebug>(a, rng, dbg, true);
+            w = nee_candidate_lean(a, tab, v, rng, li, nullptr);
+        }
+        wl[k * kLanes] = w;
+    }
+    int k = nNee + a.n_sky;
+    const bool connCand = conn != nullptr && conn->pdf > 0.0f;
+    if (connCand) {
+        wl[k * kLanes] = conn_weight_lean(a, tab, *conn, connHit, connRec, nullptr);
+        ++k;
+    }
+    uint32_t esc = 0;
+    for (; k < nCand; ++k) {
+        double w = 0.0;
+        if (!a.trace_generated) {  // nothing a generated candidate can reach: only its draws
+            skip_bounce_draws_inl(tab, v.c, v.sn, v.inside, v.rayDir, rng);
+            if (kCount) ++st.cand_skipped;
+        } else {
+            Bounce b;
+            int refl = 0, refr = 0;
+            if (sample_bounce_inl(tab, v.c, v.sn, v.inside, v.h.pos, v.rayDir, 1, 1, refl, refr, rng, b)) {
+                if (kCount) ++st.cand;
+                if (!trace_any<kCount>(s, b.o, b.d, 1e30f, st.ts)) {
+                    esc |= 1u << k;
+                    w = gen_env_weight(a, tab, b, nullptr);
+                }
+            }
+        }
+        wl[k * kLanes] = w;
+    }
+
+    Cand ch;
+    bool has = false;
+    if (reservoir) {
+        float wsum = 0.0f;
+        int count = 0;
+        int sel = -1;
+        for (int q = 0; q < nCand; ++q) {  // WeightedReservoir::Update (reservoir.h:76-84)
+            ++count;
+            const float w = float(wl[q * kLanes]);
+            if (w <= 0.0f) continue;
+            wsum += w;
+            if (rng.next_float() <= w / wsum) sel = q;
+        }
+        const hdr_reservoir prev = a.res[pixel];  // temporal merge (:127-155)
+        const uint32_t pflags = prev.flags_light >> 24;
+        Cand prevC;
+        if ((pflags & HDR_RES_HAS_SAMPLE) && (pflags & HDR_RES_HAS_CANDIDATE)) {
+            prevC.has_cand = true;
+            prevC.delta = (pflags & HDR_RES_LIGHT_DELTA) != 0;
+            prevC.solid = (pflags & HDR_RES_PDF_SOLID_ANGLE) != 0;
+            prevC.vis_tested = (pflags & HDR_RES_VIS_TESTED) != 0;
+            prevC.light = int(prev.flags_light & 0xffffffu) - 1;
+            prevC.dir = v3(prev.dir[0], prev.dir[1], prev.dir[2]);
+            prevC.color = v3(prev.color[0], prev.color[1], prev.color[2]);
+            prevC.lnormal = v3(prev.light_normal[0], prev.light_normal[1], prev.light_normal[2]);
+            prevC.dist = prev.dist;
+            prevC.pdf = prev.pdf;
+            Spec reeval = spec(0.0f);  // _reEvaluateForTemporalReuse (:529-545)
+            {
+                const V3 wo = -v.rayDir;
+                const float nDotL = dot(v.sn, prevC.dir);
+                const float dist2 = prevC.dist * prevC.dist;
+                const float cosY = smax(0.0f, dot(-prevC.dir, prevC.lnormal));
+                const float pNee = to_solid_angle(prevC.pdf, prevC.solid, dist2, smax(cosY, 1e-6f));
+                if (!(nDotL <= 0.0f || pNee <= 0.0f || (cosY <= 0.0f && !prevC.delta))) {
+                    const V3 bsdf = ggx_eval(v.c, v.sn, wo, prevC.dir);
+                    reeval = rgb2spec(tab, prevC.color) * rgb2spec(tab, bsdf * nDotL);
+                }
+            }
+            const float phatCurr = spec_lum(tab, reeval);
+            const float phatGen = prev.target;
+            if (phatGen > 0.0f) {
+                prevC.target = phatCurr;
+                prevC.tp = reeval;
+                const float mergeWeight = prev.wsum * (phatCurr / phatGen);
+                count += prev.count;
+                if (mergeWeight > 0.0f) {
+                    wsum += mergeWeight;
+                    if (rng.next_float() <= mergeWeight / wsum) sel = -2;
+                }
+            }
+        }
+        if (kDebug) dbg.shash = fnv(dbg.shash, uint32_t(sel));
+        if (sel >= 0) {
+            rebuild_traced<kDebug>(a, tab, v, conn, connHit, connRec, connCand, start, esc, sel, dbg, ch);
+            has = true;
+        } else if (sel == -2) {
+            ch = prevC;
+            has = true;
+        }
+        float W = 0.0f;  // Finalize (reservoir.h:108-125)
+        if (has && count > 0) {
+            const float phat = ch.target;
+            if (phat > 0.0f) {
+                const int perFrame = nCand > 1 ? nCand : 1;
+                int frameCount = count / perFrame;
+                if (frameCount < 1) frameCount = 1;
+                W = wsum / (float(frameCount) * phat);
+                if (!isfinite(W)) W = 0.0f;
+            }
+        }
+        if (has && !ch.vis_tested && ch.has_cand) {
+            ch.tp = winner_contribution_lean(tab, v, ch, query, qo, qd, qtmax);
+            ch.vis_tested = true;
+        }
+        store_reservoir(&a.res[pixel], has, ch, wsum, count, W);
+        if (!has || W <= 0.0f) return spec(0.0f);
+        return ch.tp * W;
+    }
+    double sum = 0.0;  // RIS::RandomIndexFromWeights (random_index.h:27-63)
+    for (int q = 0; q < nCand; ++q) {
+        const double w = wl[q * kLanes];
+        if (w > 0.0) sum += w;
+    }
+    int chosen = -1;
+    if (sum > 0.0) {
+        double rem = double(rng.next_float()) * sum;
+        for (int q = 0; q < nCand; ++q) {
+            const double w = wl[q * kLanes];
+            if (w <= 0.0) continue;
+            rem -= w;
+            if (rem <= 0.0) {
+                chosen = q;
+                break;
+            }
+        }
+    }
+    if (kDebug) dbg.shash = fnv(dbg.shash, uint32_t(chosen));
+    if (chosen < 0) return spec(0.0f);
+    rebuild_traced<kDebug>(a, tab, v, conn, connHit, connRec, connCand, start, esc, chosen, dbg, ch);
+    const float W = float(sum / double(ch.target));
+    Spec tp = ch.tp;
+    if (!ch.vis_tested && ch.has_cand) tp = winner_contribution_lean(tab, v, ch, query, qo, qd, qtmax);
+    return tp * W;
+}
+
 // ---------------------------------------------------------------- spatial reuse
 // An extension of this build (the reference reuses temporally only), run after every pixel's
 // temporal pass. Pixel q resamples among its own post-temporal reservoir and those of up to

@@ -502,22 +502,32 @@ __global__ void __launch_bounds__(kLanes) accumulate_deferred_kernel(const Frame
 // wave_resolve (spectrum -> RGB and accumulation). Queue counts are indexed by vertex parity; each
 // launch clears the counts the launch after it appends to.
 
+// Live-path list of vertex v (two lists, by vertex parity, so a launch can read one and append to
+// the other).
+__device__ __forceinline__ unsigned int *alive_list(const FrameArgs &a0, int v) {
+    return (v & 1) ? a0.walive2 : a0.walive;
+}
+
 // Clears the wavefront queue counts (a kernel, so a step can sit in a captured graph).
 __global__ void wave_zero_counts_kernel(unsigned int *c) {
     if (threadIdx.x < 8) c[threadIdx.x] = 0u;
 }
 
-template <bool kCount, bool kDebug, bool kLean = false>
+template <bool kCount, bool kDebug, bool kLean = false, bool kTraced = false>
 __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__restrict__ args,
                                                              const int *__restrict__ tile_base, int n_views) {
     extern __shared__ double wl[];
     HDR_TILE(args, tile_base, n_views)
     unsigned int *cnt = args[0].wcnt;  // the queues of a launch are shared by its views
-    if (blockIdx.x == 0 && lane == 0) cnt[5] = 0u;  // live paths of vertex 1 (appended by wave_trace)
+    // live paths of vertex 1: appended by wave_trace (or, traced, by this launch after the
+    // frame-start clear)
+    if (!kTraced && blockIdx.x == 0 && lane == 0) cnt[5] = 0u;
     PixelStats st = {};
     bool query = false, ray = false;
     ShadowQuery q;
     float4 cq = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ro = cq, rd = cq;
+    HitRec next;
+    uint32_t code = 0;
     if (valid) {
         const int pixel = y * a.W + x;
         const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
@@ -537,8 +547,14 @@ __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__
         p.dir = d0;
         HitRec g;
         p.primary_hit = load_grec(a.gbuf + pixel, g);
-        if (p.primary_hit) wave_vertex<kCount, kDebug>(a, tab, 0, g, p, wl + lane, pixel, st, query, q, cq, ray, ro, rd);
+        if (p.primary_hit)
+            wave_vertex<kCount, kDebug, kTraced>(a, tab, 0, g, p, wl + lane, pixel, st, query, q, cq, ray, ro, rd,
+                                                 next);
+        else if (a.s.environment >= 0 && a.ibl_bg)  // a miss sees the environment (PathIntegrator::Li, path_integrator.cpp:48-209)
+            p.total = rgb2spec(tab, env_radiance(a.s, d0));
         store_path(a, pixel, p, true);
+        if (kTraced && ray) store_grec(a.whit + pixel, true, next);
+        code = query_code(a, pixel);
         if (kDebug) {
             a.dbg[pixel].prim_instance = p.primary_hit ? g.inst : -1;
             a.dbg[pixel].prim_triangle = p.primary_hit ? g.tri : -1;
@@ -550,20 +566,30 @@ __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__
         args[0].queue[qi] = q;
         args[0].wcq[qi] = cq;
     }
-    const unsigned int ri = wave_append(cnt + 0, ray, lane);
-    if (ray) {
-        args[0].wray[2 * ri] = ro;
-        args[0].wray[2 * ri + 1] = rd;
+    if (kTraced) {
+        const unsigned int ai = wave_append(cnt + 5, ray, lane);
+        if (ray) alive_list(args[0], 1)[ai] = code;
+    } else {
+        const unsigned int ri = wave_append(cnt + 0, ray, lane);
+        if (ray) {
+            args[0].wray[2 * ri] = ro;
+            args[0].wray[2 * ri + 1] = rd;
+        }
     }
     if (kCount) {
         unsigned long long *c = a.counters;
+        wave_add(c + 1, st.bounce);
         wave_add(c + 2, query ? 1ULL : 0ULL);
+        wave_add(c + 3, st.cand);
         wave_add(c + 4, st.cand_skipped);
+        wave_add(c + 5, st.ts.nodes);
+        wave_add(c + 6, st.ts.tris);
+        wave_add(c + 7, st.ts.instances);
     }
 }
 
 // Vertex `bounce` >= 1 of every live path (the codes wave_trace appended).
-template <bool kCount, bool kDebug, int kOcc = 1>
+template <bool kCount, bool kDebug, int kOcc = 1, bool kTraced = false>
 __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
 wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce) {
     extern __shared__ double wl[];
@@ -571,7 +597,9 @@ wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce) {
     const int par = bounce & 1;
     unsigned int *cnt = args[0].wcnt;
     const unsigned int n = cnt[4 + par];
-    if (blockIdx.x == 0 && lane == 0) cnt[4 + (par ^ 1)] = 0u;
+    // live paths of the next vertex: appended by wave_trace (traced: by this launch, cleared by the
+    // traversal launch before it)
+    if (!kTraced && blockIdx.x == 0 && lane == 0) cnt[4 + (par ^ 1)] = 0u;
     PixelStats st = {};
     unsigned int nq = 0;
     for (unsigned int b0 = blockIdx.x * kLanes; b0 < n; b0 += gridDim.x * kLanes) {
@@ -579,34 +607,47 @@ wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce) {
         bool query = false, ray = false;
         ShadowQuery q;
         float4 cq = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ro = cq, rd = cq;
+        uint32_t code = 0;
         if (i < n) {
-            const uint32_t code = args[0].walive[i];
+            code = alive_list(args[0], bounce)[i];
             const FrameArgs &a = args[code >> 24];
             const int pixel = int(code & 0xffffffu);
             WavePath<Mt64Deep> p;
             load_path(a, pixel, p);
             const SpecTab tab = make_tab(p.utab);
-            HitRec h;
+            HitRec h, next;
             (void)load_grec(a.whit + pixel, h);
-            wave_vertex<kCount, kDebug>(a, tab, bounce, h, p, wl + lane, pixel, st, query, q, cq, ray, ro, rd);
+            wave_vertex<kCount, kDebug, kTraced>(a, tab, bounce, h, p, wl + lane, pixel, st, query, q, cq, ray, ro,
+                                                 rd, next);
             store_path(a, pixel, p, false);
+            if (kTraced && ray) store_grec(a.whit + pixel, true, next);
         }
         const unsigned int qi = wave_append(cnt + 2 + par, query, lane);
         if (query) {
             args[0].queue[qi] = q;
             args[0].wcq[qi] = cq;
         }
-        const unsigned int ri = wave_append(cnt + par, ray, lane);
-        if (ray) {
-            args[0].wray[2 * ri] = ro;
-            args[0].wray[2 * ri + 1] = rd;
+        if (kTraced) {
+            const unsigned int ai = wave_append(cnt + 4 + (par ^ 1), ray, lane);
+            if (ray) alive_list(args[0], bounce + 1)[ai] = code;
+        } else {
+            const unsigned int ri = wave_append(cnt + par, ray, lane);
+            if (ray) {
+                args[0].wray[2 * ri] = ro;
+                args[0].wray[2 * ri + 1] = rd;
+            }
         }
         nq += query ? 1u : 0u;
     }
     if (kCount) {
         unsigned long long *c = args[0].counters;
+        wave_add(c + 1, st.bounce);
         wave_add(c + 2, nq);
+        wave_add(c + 3, st.cand);
         wave_add(c + 4, st.cand_skipped);
+        wave_add(c + 5, st.ts.nodes);
+        wave_add(c + 6, st.ts.tris);
+        wave_add(c + 7, st.ts.instances);
     }
 }
 
@@ -616,7 +657,7 @@ wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce) {
 // of vertex `bounce`.
 template <bool kCount, int kOcc = 1, bool kLds = false>
 __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
-wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce) {
+wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce, int traced) {
     HDR_RING_LDS
     const int lane = threadIdx.x;
     const int par = bounce & 1;
@@ -625,6 +666,7 @@ wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce) {
     if (blockIdx.x == 0 && lane == 0) {
         cnt[par ^ 1] = 0u;        // rays of the next vertex
         cnt[2 + (par ^ 1)] = 0u;  // shadow queries of the next vertex
+        if (traced) cnt[4 + par] = 0u;  // live paths the next shading launch appends (its input is the other list)
     }
     TraceStats ts = {};
     unsigned int nb = 0;
@@ -645,7 +687,7 @@ wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce) {
             ++nb;
         }
         const unsigned int ai = wave_append(cnt + 4 + (par ^ 1), hit, lane);
-        if (hit) args[0].walive[ai] = code;
+        if (hit) alive_list(args[0], bounce + 1)[ai] = code;
     }
     for (unsigned int b0 = blockIdx.x * kLanes; b0 < ns; b0 += gridDim.x * kLanes) {
         const unsigned int i = b0 + unsigned(lane);
@@ -687,7 +729,7 @@ __global__ void __launch_bounds__(kLanes) wave_resolve_kernel(const FrameArgs *_
         const float4 r2 = w[2 * P + pixel], r5 = w[5 * P + pixel], r7 = w[7 * P + pixel];
         const SpecTab tab = make_tab(r5.w);
         const bool hit = (__float_as_uint(r7.x) & kWavePrimaryHit) != 0;
-        const Spec L = hit ? spec_of(w[6 * P + pixel]) * spec_of(w[4 * P + pixel]) : spec(0.0f);
+        const Spec L = hit ? spec_of(w[6 * P + pixel]) * spec_of(w[4 * P + pixel]) : spec_of(w[4 * P + pixel]);
         V3 rgb = spec2rgb(tab, L);
         if (a.has_cam && a.cam.enable_exposure) rgb = rgb * a.exposure_mul;
         const float4 c4 = make_float4(rgb.x, rgb.y, rgb.z, 1.0f);
@@ -859,24 +901,44 @@ static int queue_blocks(const LaunchShape &ls, size_t cap) {
     return int(std::min<size_t>(need, size_t(ls.queue_waves > 0 ? ls.queue_waves : kDefaultQueueWaves)));
 }
 template <bool C, bool L>
-static void launch_wave_trace_l(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a, int b) {
+static void launch_wave_trace_l(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a, int b, int t) {
     switch (ls.trace_occ) {
-    case 4: hipLaunchKernelGGL((wave_trace_kernel<C, 4, L>), dim3(blocks), dim3(kLanes), 0, st, a, b); break;
-    case 6: hipLaunchKernelGGL((wave_trace_kernel<C, 6, L>), dim3(blocks), dim3(kLanes), 0, st, a, b); break;
-    case 8: hipLaunchKernelGGL((wave_trace_kernel<C, 8, L>), dim3(blocks), dim3(kLanes), 0, st, a, b); break;
-    default: hipLaunchKernelGGL((wave_trace_kernel<C, 1, L>), dim3(blocks), dim3(kLanes), 0, st, a, b); break;
+    case 4: hipLaunchKernelGGL((wave_trace_kernel<C, 4, L>), dim3(blocks), dim3(kLanes), 0, st, a, b, t); break;
+    case 6: hipLaunchKernelGGL((wave_trace_kernel<C, 6, L>), dim3(blocks), dim3(kLanes), 0, st, a, b, t); break;
+    case 8: hipLaunchKernelGGL((wave_trace_kernel<C, 8, L>), dim3(blocks), dim3(kLanes), 0, st, a, b, t); break;
+    default: hipLaunchKernelGGL((wave_trace_kernel<C, 1, L>), dim3(blocks), dim3(kLanes), 0, st, a, b, t); break;
     }
 }
+// traced: the frames' shading launches trace their own bounce rays (wave_vertex kTraced).
 template <bool C>
-static void launch_wave_trace(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a, int b) {
-    if (ls.lds_stack) launch_wave_trace_l<C, true>(ls, blocks, st, a, b);
-    else launch_wave_trace_l<C, false>(ls, blocks, st, a, b);
+static void launch_wave_trace(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a, int b,
+                              bool traced) {
+    if (ls.lds_stack) launch_wave_trace_l<C, true>(ls, blocks, st, a, b, traced ? 1 : 0);
+    else launch_wave_trace_l<C, false>(ls, blocks, st, a, b, traced ? 1 : 0);
+}
+template <bool C, bool D, bool T>
+static void launch_wave_shade_t(const LaunchShape &ls, int blocks, size_t lds, hipStream_t st, const FrameArgs *a,
+                                int b) {
+    if (ls.shade_occ == 3)
+        hipLaunchKernelGGL((wave_shade_kernel<C, D, 3, T>), dim3(blocks), dim3(kLanes), lds, st, a, b);
+    else hipLaunchKernelGGL((wave_shade_kernel<C, D, 1, T>), dim3(blocks), dim3(kLanes), lds, st, a, b);
 }
 template <bool C, bool D>
-static void launch_wave_shade(const LaunchShape &ls, int blocks, size_t lds, hipStream_t st, const FrameArgs *a,
-                              int b) {
-    if (ls.shade_occ == 3) hipLaunchKernelGGL((wave_shade_kernel<C, D, 3>), dim3(blocks), dim3(kLanes), lds, st, a, b);
-    else hipLaunchKernelGGL((wave_shade_kernel<C, D, 1>), dim3(blocks), dim3(kLanes), lds, st, a, b);
+static void launch_wave_shade(const LaunchShape &ls, int blocks, size_t lds, hipStream_t st, const FrameArgs *a, int b,
+                              bool traced) {
+    if (traced) launch_wave_shade_t<C, D, true>(ls, blocks, lds, st, a, b);
+    else launch_wave_shade_t<C, D, false>(ls, blocks, lds, st, a, b);
+}
+// The primary-vertex launch: lean RNG stream when no pixel can pass output 310 there; traced
+// candidates when BSDF-sampled rays can see an emitter.
+template <bool C, bool D>
+static void launch_wave_shade0(bool lean, bool traced, unsigned tiles, size_t lds, hipStream_t st, const FrameArgs *a,
+                               const int *t, int n) {
+    const dim3 g(tiles), b(kLanes);
+    if (lean && traced) hipLaunchKernelGGL((wave_shade0_kernel<C, D, true, true>), g, b, lds, st, a, t, n);
+    else if (lean) hipLaunchKernelGGL((wave_shade0_kernel<C, D, true, false>), g, b, lds, st, a, t, n);
+    else if (traced) hipLaunchKernelGGL((wave_shade0_kernel<C, D, false, true>), g, b, lds, st, a, t, n);
+    else hipLaunchKernelGGL((wave_shade0_kernel<C, D, false, false>), g, b, lds, st, a, t, n);
 }
 template <bool C, bool L>
 static void launch_shadow_l(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a) {
@@ -1206,6 +1268,7 @@ struct FramePlan {
     bool spatial = false;   // spatial reuse after the temporal pass (deferred path only)
     bool wave = false;      // the wavefront path (RIS at every vertex of deep paths, wavefront.h)
     bool wave_lean0 = false;  // its primary vertex cannot draw past RNG output 310 (lean stream there)
+    bool wave_traced = false; // BSDF-sampled candidates can see an emitter: traced in the shading launches
     int depth = 0;          // path vertices (max_depth)
     size_t lds = 0;         // dynamic LDS of the shading / integration kernel
     unsigned tiles = 0;     // 8x8 tiles of the view
@@ -1315,7 +1378,10 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
     // The wavefront path: the same conditions at any depth and RNG range (frames the deferred path
     // does not take).
     const bool wave = !deferred && a.direct == HDR_DIRECT_RIS && a.skip_vis && !scene->any_translucent &&
-                      !a.trace_generated && !a.trace_last_connection;
+                      !scene->ds.has_mesh_lights;
+    // With an environment or emissive materials, BSDF-sampled candidates can see an emitter: the
+    // wavefront's shading launches then trace the bounce and the generated candidates themselves.
+    const bool waveTraced = wave && (a.trace_generated || a.trace_last_connection);
     // Spatial reuse (an extension) runs on the deferred path with temporal reservoirs.
     const bool spatial = pipe->spatial_count > 0;
     if (spatial && !(deferred && a.use_res))
@@ -1332,8 +1398,9 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
     }
     // candidate weights in LDS: the NEE candidates on the deferred path (its BSDF candidates carry
     // no weight), every candidate on the general path
-    const size_t lds = size_t((deferred || wave) ? a.n_nee : (a.direct == HDR_DIRECT_RIS ? a.n_cand : 1)) * kLanes *
-                       sizeof(double);
+    const size_t lds = size_t((deferred || (wave && !waveTraced)) ? a.n_nee
+                                                                  : (a.direct == HDR_DIRECT_RIS ? a.n_cand : 1)) *
+                       kLanes * sizeof(double);
     if (deferred) {
         if (hdr_status st = ensure(view->rgb2, P * 2 * sizeof(float4))) return st;
         if (hdr_status st = ensure(view->flags, P * sizeof(uint32_t))) return st;
@@ -1350,7 +1417,7 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
         if (hdr_status st = ensure(view->wray, P * 2 * sizeof(float4))) return st;
         if (hdr_status st = ensure(view->queue, P * sizeof(ShadowQuery))) return st;
         if (hdr_status st = ensure(view->wcq, P * sizeof(float4))) return st;
-        if (hdr_status st = ensure(view->walive, P * sizeof(unsigned int))) return st;
+        if (hdr_status st = ensure(view->walive, 2 * P * sizeof(unsigned int))) return st;  // two parities
         if (hdr_status st = ensure(view->wcnt, 8 * sizeof(unsigned int))) return st;
         a.wstate = static_cast<float4 *>(view->wstate.p);
         a.whit = static_cast<GRec *>(view->whit.p);
@@ -1358,12 +1425,14 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
         a.queue = static_cast<ShadowQuery *>(view->queue.p);
         a.wcq = static_cast<float4 *>(view->wcq.p);
         a.walive = static_cast<unsigned int *>(view->walive.p);
+        a.walive2 = a.walive + P;
         a.wcnt = static_cast<unsigned int *>(view->wcnt.p);
     }
     a.view = 0;
     plan.deferred = deferred;
     plan.wave = wave;
     plan.wave_lean0 = wave && bound0 < kMtN - 1;
+    plan.wave_traced = waveTraced;
     plan.depth = pipe->max_depth;
     plan.lean_pt = !deferred && a.direct == HDR_DIRECT_MIS && !scene->any_translucent && bound < kMtN - 1;
     plan.spatial = spatial;
@@ -1461,12 +1530,9 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     } else if (plan.wave) {
         const int qblocks = queue_blocks(view->shape, P);
         hipLaunchKernelGGL(wave_zero_counts_kernel, dim3(1), dim3(kLanes), 0, view->stream, a.wcnt);
-#define HDR_W0(C, D)                                                                                        \
-    if (plan.wave_lean0)                                                                                    \
-        hipLaunchKernelGGL((wave_shade0_kernel<C, D, true>), grid, dim3(kLanes), lds, view->stream, da, dt, 1); \
-    else                                                                                                    \
-        hipLaunchKernelGGL((wave_shade0_kernel<C, D, false>), grid, dim3(kLanes), lds, view->stream, da, dt, 1)
-#define HDR_WS(C, D) launch_wave_shade<C, D>(view->shape, qblocks, lds, view->stream, da, b)
+#define HDR_W0(C, D) \
+    launch_wave_shade0<C, D>(plan.wave_lean0, plan.wave_traced, plan.tiles, lds, view->stream, da, dt, 1)
+#define HDR_WS(C, D) launch_wave_shade<C, D>(view->shape, qblocks, lds, view->stream, da, b, plan.wave_traced)
         if (view->counting) {
             if (debug_out) HDR_W0(true, true);
             else HDR_W0(true, false);
@@ -1487,8 +1553,8 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
                     else HDR_WS(false, false);
                 }
             }
-            if (view->counting) launch_wave_trace<true>(view->shape, qblocks, view->stream, da, b);
-            else launch_wave_trace<false>(view->shape, qblocks, view->stream, da, b);
+            if (view->counting) launch_wave_trace<true>(view->shape, qblocks, view->stream, da, b, plan.wave_traced);
+            else launch_wave_trace<false>(view->shape, qblocks, view->stream, da, b, plan.wave_traced);
             HIP_TRY(hipGetLastError());
         }
 #undef HDR_W0
@@ -1845,16 +1911,19 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
     std::vector<FrameArgs> args(static_cast<size_t>(n));
     std::vector<FramePlan> plans(static_cast<size_t>(n));
     size_t queueCap = 0, waveCap = 0;
-    int waveDepth = 0;
-    bool waveLean0 = true;  // every wavefront view's primary vertex stays below RNG output 311
+    // wavefront views in two groups (bounce rays traced by the traversal launch / by the shading
+    // launches), each run as its own launch sequence over the shared queues
+    int waveDepth[2] = {0, 0};
+    bool waveLean0[2] = {true, true};  // every view's primary vertex stays below RNG output 311
     for (int i = 0; i < n; ++i) {
         if (hdr_status st = prepare_frame(views[i], scenes[i], &frames[i], &pipes[i], false, args[i], plans[i]))
             return st;
         if (plans[i].deferred) queueCap += size_t(views[i]->W) * views[i]->H;
         if (plans[i].wave) {
+            const int t = plans[i].wave_traced ? 1 : 0;
             waveCap += size_t(views[i]->W) * views[i]->H;
-            waveDepth = std::max(waveDepth, plans[i].depth);
-            waveLean0 = waveLean0 && plans[i].wave_lean0;
+            waveDepth[t] = std::max(waveDepth[t], plans[i].depth);
+            waveLean0[t] = waveLean0[t] && plans[i].wave_lean0;
         }
     }
     if (queueCap * sizeof(ShadowQuery) > b->queue.n || waveCap * sizeof(ShadowQuery) > b->wqueue.n) {
@@ -1864,7 +1933,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
             if (hdr_status st = ensure(b->wqueue, waveCap * sizeof(ShadowQuery))) return st;
             if (hdr_status st = ensure(b->wray, waveCap * 2 * sizeof(float4))) return st;
             if (hdr_status st = ensure(b->wcq, waveCap * sizeof(float4))) return st;
-            if (hdr_status st = ensure(b->walive, waveCap * sizeof(unsigned int))) return st;
+            if (hdr_status st = ensure(b->walive, 2 * waveCap * sizeof(unsigned int))) return st;
         }
     }
     for (int i = 0; i < n; ++i) {
@@ -1875,6 +1944,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
             args[i].wray = static_cast<float4 *>(b->wray.p);
             args[i].wcq = static_cast<float4 *>(b->wcq.p);
             args[i].walive = static_cast<unsigned int *>(b->walive.p);
+            args[i].walive2 = args[i].walive + waveCap;
             args[i].wcnt = static_cast<unsigned int *>(b->wcnt.p);
         } else {
             args[i].queue = static_cast<ShadowQuery *>(b->queue.p);
@@ -1882,7 +1952,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
     }
     // launch groups: every view (G-buffer, shadow), deferred views (shading, accumulation), their
     // spatial-reuse subset, and megakernel views; each gets its own argument array and tile table
-    enum { kAll, kDef, kSpa, kMega, kLean, kWave, kGroups };
+    enum { kAll, kDef, kSpa, kMega, kLean, kWave, kWaveT, kGroups };
     std::vector<int> members[kGroups];
     for (int i = 0; i < n; ++i) {
         members[kAll].push_back(i);
@@ -1890,7 +1960,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         if (plans[i].deferred && plans[i].spatial) members[kSpa].push_back(i);
         if (!plans[i].deferred && !plans[i].lean_pt && !plans[i].wave) members[kMega].push_back(i);
         if (plans[i].lean_pt) members[kLean].push_back(i);
-        if (plans[i].wave) members[kWave].push_back(i);
+        if (plans[i].wave) members[plans[i].wave_traced ? kWaveT : kWave].push_back(i);
     }
     auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
     size_t off = 0, argOff[kGroups], tileOff[kGroups];
@@ -1926,7 +1996,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         int base = 0;
         for (size_t k = 0; k < members[g].size(); ++k) {
             ga[k] = args[size_t(members[g][k])];
-            if (g == kWave) ga[k].view = int(k);  // wavefront queue codes index the group's array
+            if (g == kWave || g == kWaveT) ga[k].view = int(k);  // wavefront queue codes index the group's array
             tb[k] = base;
             base += int(plans[size_t(members[g][k])].tiles);
         }
@@ -1960,22 +2030,19 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         if (N(kLean) > 0)
             hipLaunchKernelGGL((pt_lean_kernel<false, false>), dim3(tiles[kLean]), dim3(kLanes), 0, b->stream,
                                A(kLean), T(kLean), N(kLean));
-        if (N(kWave) > 0) {
+        for (int wg : {int(kWave), int(kWaveT)}) {
+            if (N(wg) == 0) continue;
+            const int t = wg == kWaveT ? 1 : 0;
             const int qblocks = queue_blocks(ls, waveCap);
             hipLaunchKernelGGL(wave_zero_counts_kernel, dim3(1), dim3(kLanes), 0, b->stream,
                                static_cast<unsigned int *>(b->wcnt.p));
-            if (waveLean0)
-                hipLaunchKernelGGL((wave_shade0_kernel<false, false, true>), dim3(tiles[kWave]), dim3(kLanes),
-                                   lds[kWave], b->stream, A(kWave), T(kWave), N(kWave));
-            else
-                hipLaunchKernelGGL((wave_shade0_kernel<false, false, false>), dim3(tiles[kWave]), dim3(kLanes),
-                                   lds[kWave], b->stream, A(kWave), T(kWave), N(kWave));
-            for (int v = 0; v < waveDepth; ++v) {
-                if (v > 0) launch_wave_shade<false, false>(ls, qblocks, lds[kWave], b->stream, A(kWave), v);
-                launch_wave_trace<false>(ls, qblocks, b->stream, A(kWave), v);
+            launch_wave_shade0<false, false>(waveLean0[t], t == 1, tiles[wg], lds[wg], b->stream, A(wg), T(wg), N(wg));
+            for (int v = 0; v < waveDepth[t]; ++v) {
+                if (v > 0) launch_wave_shade<false, false>(ls, qblocks, lds[wg], b->stream, A(wg), v, t == 1);
+                launch_wave_trace<false>(ls, qblocks, b->stream, A(wg), v, t == 1);
             }
-            hipLaunchKernelGGL((wave_resolve_kernel<false, false>), dim3(tiles[kWave]), dim3(kLanes), 0, b->stream,
-                               A(kWave), T(kWave), N(kWave));
+            hipLaunchKernelGGL((wave_resolve_kernel<false, false>), dim3(tiles[wg]), dim3(kLanes), 0, b->stream,
+                               A(wg), T(wg), N(wg));
         }
         if (N(kDef) > 0) {
             launch_shadow<false>(ls, queue_blocks(ls, queueCap), b->stream, A(kAll));
@@ -1991,7 +2058,8 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
                                      uint64_t(uintptr_t(sl.host.p)), uint64_t(uintptr_t(b->queue.p)),
                                      uint64_t(uintptr_t(b->wqueue.p)), uint64_t(uintptr_t(b->wray.p)),
                                      uint64_t(uintptr_t(b->wcq.p)), uint64_t(uintptr_t(b->walive.p)),
-                                     uint64_t(queueCap), uint64_t(waveCap), uint64_t(waveDepth), uint64_t(waveLean0),
+                                     uint64_t(queueCap), uint64_t(waveCap), uint64_t(waveDepth[0]),
+                                     uint64_t(waveDepth[1]), uint64_t(waveLean0[0]), uint64_t(waveLean0[1]),
                                      uint64_t(ls.queue_waves), uint64_t(ls.trace_occ), uint64_t(ls.shade_occ),
                                      uint64_t(ls.lds_stack)};
         for (int g = 0; g < kGroups; ++g) {

@@ -60,7 +60,8 @@ struct FrameArgs {
     GRec *whit;                        // hit of each path's last bounce ray
     float4 *wray;                      // bounce-ray queue: {origin, code}, {direction, -}
     float4 *wcq;                       // contribution of each shadow query (index-aligned with queue)
-    unsigned int *walive;              // codes of the paths whose bounce ray hit (next vertex)
+    unsigned int *walive;              // codes of the live paths of even vertices (bounce ray hit)
+    unsigned int *walive2;             // and of odd vertices
     unsigned int *wcnt;                // queue counts: rays[2], shadows[2], alive[2] by vertex parity
 };
 

@@ -134,10 +134,15 @@ __device__ __forceinline__ void load_path(const FrameArgs &a, int pixel, WavePat
 // connection), RIS direct lighting, path throughput and Russian roulette. Outputs the vertex's
 // shadow query with its contribution `tp * d` (added to the total if the ray is unoccluded) and the
 // bounce ray when the path goes on.
-template <bool kCount, bool kDebug, class R>
+// kTraced: frames whose BSDF-sampled candidates can see an emitter (environment, emissive material;
+// no mesh light): the bounce ray is traced here, before the RIS candidates (the connection candidate
+// needs its hit), generated candidates are traced by ris_direct_traced, and the bounce hit -- the
+// next vertex -- is returned in `next` (ray = a next vertex exists) instead of a bounce ray.
+template <bool kCount, bool kDebug, bool kTraced, class R>
 __device__ __forceinline__ void wave_vertex(const FrameArgs &a, const SpecTab &tab, int bounce, const HitRec &h,
                                             WavePath<R> &p, double *wl, int pixel, PixelStats &st, bool &query,
-                                            ShadowQuery &q, float4 &cq, bool &ray, float4 &ro, float4 &rd) {
+                                            ShadowQuery &q, float4 &cq, bool &ray, float4 &ro, float4 &rd,
+                                            HitRec &next) {
     const DeviceScene &s = a.s;
     const bool first = bounce == 0;
     const Clo c = get_closure(s, h, a.enable_ss != 0);
@@ -149,14 +154,32 @@ __device__ __forceinline__ void wave_vertex(const FrameArgs &a, const SpecTab &t
     const bool ok = sample_bounce_inl(tab, v.c, v.sn, v.inside, v.h.pos, v.rayDir, a.max_refl, a.max_refr, p.refl,
                                       p.refr, p.rng, b);
     const bool last = bounce == a.max_depth - 1;
-    if (kCount && ok && last) ++st.cand_skipped;  // the last connection ray is never traced here
+    bool cHit = false;
+    if (kTraced) {  // SampleBounceWithConnection (direct_light_integrator_interface.h:18-40)
+        if (ok) {
+            if (last && !a.trace_last_connection) {
+                if (kCount) ++st.cand_skipped;
+            } else {
+                if (kCount) ++st.bounce;
+                RawHit rh;
+                cHit = trace_closest<kCount>(s, b.o, b.d, rh, st.ts);
+                if (cHit) next = make_hit_record(s, b.o, b.d, rh);
+            }
+        }
+    } else if (kCount && ok && last) {
+        ++st.cand_skipped;  // the last connection ray is never traced here
+    }
     V3 qo = v3s(0.0f), qd = v3s(0.0f);
     float qtmax = 0.0f;
     bool qq = false;
     uint64_t *slot = wave_slot(a, pixel);
-    const Spec d = ris_direct_lean<kCount, kDebug>(a, tab, v, ok && b.pdf > 0.0f, p.rng, wl, pixel, st, p.dbg,
-                                                   first && a.use_res, slot ? slot + kMtN : nullptr, qq, qo, qd,
-                                                   qtmax);
+    Spec d;
+    if (kTraced)
+        d = ris_direct_traced<kCount, kDebug>(a, tab, v, ok ? &b : nullptr, cHit, next, p.rng, wl, pixel, st, p.dbg,
+                                              first && a.use_res, slot ? slot + kMtN : nullptr, qq, qo, qd, qtmax);
+    else
+        d = ris_direct_lean<kCount, kDebug>(a, tab, v, ok && b.pdf > 0.0f, p.rng, wl, pixel, st, p.dbg,
+                                            first && a.use_res, slot ? slot + kMtN : nullptr, qq, qo, qd, qtmax);
     const Spec td = first ? d : p.tp * d;
     if (qq) {
         query = true;
@@ -175,8 +198,12 @@ __device__ __forceinline__ void wave_vertex(const FrameArgs &a, const SpecTab &t
         p.tp = p.tp * (1.0f / pr);
     }
     if (last) return;
-    ray = true;
     p.dir = b.d;
+    if (kTraced) {
+        ray = cHit;
+        return;
+    }
+    ray = true;
     ro = make_float4(b.o.x, b.o.y, b.o.z, __uint_as_float(query_code(a, pixel)));
     rd = make_float4(b.d.x, b.d.y, b.d.z, 0.0f);
 }

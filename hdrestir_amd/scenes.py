@@ -465,12 +465,14 @@ def _dome_texels(w, h, rng):
     return (tex * grad * np.array([0.6, 0.7, 1.0], np.float32)).astype(np.float32)
 
 
-def features(width=48, height=48, seed=3, frames=4, dome=True, dof=False, emissive=True, opaque=False):
+def features(width=48, height=48, seed=3, frames=4, dome=True, dof=False, emissive=True, opaque=False,
+             mesh_light=None):
     """Every shading feature of the path in one small scene: coat / sheen / transmission with Beer
     absorption / partial opacity / emission / DefaultMaterial with vertex colours; rect, cone-shaped
     point, distant and dome lights (the dome is the environment); an emissive sphere registered as a
     mesh light. dome / emissive / opaque=True select the variant whose BSDF rays reach no emitter
-    and whose materials are all opaque (the deferred-shadow path at depth 1 with skipVisibility)."""
+    and whose materials are all opaque (the deferred-shadow path at depth 1 with skipVisibility).
+    mesh_light=False keeps the emissive material but registers no mesh light (default: emissive)."""
     rng = np.random.default_rng(seed)
     sc = Scene("features", width, height)
     bp, bi = box_mesh(2.0)
@@ -516,7 +518,7 @@ def features(width=48, height=48, seed=3, frames=4, dome=True, dof=False, emissi
     sc.add_light(abi.HDR_LIGHT_DISTANT, (1.0, 1.0, 0.9), 0.8, dist, exposure=-1.0)
     # the emissive sphere is the mesh prim of this (small) rect light: GetLightAtHit finds it
     em = sc.add_light(abi.HDR_LIGHT_RECT, (1.0, 0.6, 0.3), 4.0, _rect_light_down(pos[4], 0.0), width=0.3, height=0.3)
-    if emissive:
+    if emissive if mesh_light is None else mesh_light:
         sc.instances[light_idx["emissive"]]["light"] = em
     if dome:
         sc.environment = sc.add_light(abi.HDR_LIGHT_DOME, (1.0, 1.0, 1.0), 0.7, np.eye(4),
