@@ -238,9 +238,10 @@ def large_scene_measurement(gpu, abi, ctx, frames=3):
     """The large-scene workload at 1080p: 2,093,058-triangle displaced grid + 4000 instances of a
     256-triangle sphere (3.1 M instanced triangles), 65,536 point / rect lights over 5 decades, a
     2048x1024 environment map with log-normal texels (dome CDFs built on the device), ReSTIR M = 32
-    with one shadow ray at every vertex of depth-8 paths. Spatial reuse is not available at depth 8
-    (HDR_ERR_UNSUPPORTED), so the reference's temporal reuse runs alone; with an environment the
-    generated BSDF candidates are traced, which the general megakernel does."""
+    with one shadow ray at every vertex of depth-8 paths, the reference's temporal reuse plus 5
+    spatial neighbours within 30 px at the primary vertex. With an environment the generated BSDF
+    candidates are traced (the wavefront's traced shading launches). The same frames without
+    spatial reuse are timed beside it."""
     from hdrestir_amd import scenes
     t0 = time.perf_counter()
     sc = scenes.large_scene(frames=1)
@@ -249,21 +250,31 @@ def large_scene_measurement(gpu, abi, ctx, frames=3):
     gs = gpu.GpuScene(ctx, sc)
     uploaded = time.perf_counter() - t0
     view, proj = sc.cameras[0]
-    pipe = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=8)
+    pipe = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=8,
+                        spatial_count=5, spatial_radius=30.0)
+    pipe_t = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=8)
     r = gpu.Renderer(ctx, gs, sc.width, sc.height)
-    r.render(abi.frame(view, proj, 0), pipe, read_color=False)
-    r.synchronize()
-    t0 = time.perf_counter()
-    for f in range(1, 1 + frames):
-        r.render(abi.frame(view, proj, f), pipe, read_color=False)
-    r.synchronize()
-    ms = (time.perf_counter() - t0) * 1000.0 / frames
-    out = {"workload": "large_scene_1080p_restir_depth8", "triangles_unique": sc.triangle_counts()[0],
+
+    def timed(p):
+        r.reset()
+        r.render(abi.frame(view, proj, 0), p, read_color=False)
+        r.synchronize()
+        r.stage_times()
+        t0 = time.perf_counter()
+        for f in range(1, 1 + frames):
+            r.render(abi.frame(view, proj, f), p, read_color=False)
+        r.synchronize()
+        return (time.perf_counter() - t0) * 1000.0 / frames, r.stage_times()
+    ms, st = timed(pipe)
+    ms_t, _ = timed(pipe_t)
+    out = {"workload": "large_scene_1080p_restir_depth8_spatial5", "triangles_unique": sc.triangle_counts()[0],
            "triangles_instanced": sc.triangle_counts()[1], "lights": len(sc.lights), "dome": [2048, 1024],
-           "max_depth": 8, "candidate_count": 32, "spatial_reuse": "not available at depth > 1",
+           "max_depth": 8, "candidate_count": 32, "spatial_reuse": {"neighbours": 5, "radius_px": 30},
            "scene_generate_s": round(built, 2), "scene_upload_s": round(uploaded, 2),
            "scene_device_bytes": gs.device_bytes(), "frames": frames, "ms_per_frame": round(ms, 2),
-           "frames_per_s": round(1000.0 / ms, 3)}
+           "frames_per_s": round(1000.0 / ms, 3),
+           "spatial_stage_ms_per_frame": round(st["spatial"][0] / max(1, st["spatial"][1]), 3),
+           "temporal_only_ms_per_frame": round(ms_t, 2)}
     r.close()
     gs.close()
     return out
@@ -336,6 +347,8 @@ def main():
                     help="skip the depth-6 single-view, 16-view batch and large-scene measurements")
     ap.add_argument("--deep-only", action="store_true",
                     help="only the depth-6 single-view measurement, --steps frames (for profilers)")
+    ap.add_argument("--large-only", action="store_true",
+                    help="only the large-scene measurement (for profilers)")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the counting pass and the timed frames of the headline pipeline (for profilers)")
     ap.add_argument("--serving-requests", type=int, default=48)
@@ -358,6 +371,11 @@ def main():
         return
 
     from hdrestir_amd import abi, gpu, scenes
+    if args.large_only:
+        ctx = gpu.Context(0)
+        print(json.dumps(large_scene_measurement(gpu, abi, ctx)), flush=True)
+        ctx.close()
+        return
     _log("building scene")
     sc = scenes.many_lights(frames=max(args.steps, 2), dolly=True)
     if args.deep_only:

@@ -542,6 +542,10 @@ __device__ __forceinline__ Spec ris_direct_traced(const FrameArgs &a, const Spec
                 if (!isfinite(W)) W = 0.0f;
             }
         }
+        if (a.spatial_count > 0) {  // spatial reuse follows: it tests the final winner
+            store_reservoir(&a.res_t[pixel], has, ch, wsum, count, W);
+            return spec(0.0f);
+        }
         if (has && !ch.vis_tested && ch.has_cand) {
             ch.tp = winner_contribution_lean(tab, v, ch, query, qo, qd, qtmax);
             ch.vis_tested = true;

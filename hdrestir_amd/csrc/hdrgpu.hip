@@ -552,6 +552,10 @@ __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__
                                                  next);
         else if (a.s.environment >= 0 && a.ibl_bg)  // a miss sees the environment (PathIntegrator::Li, path_integrator.cpp:48-209)
             p.total = rgb2spec(tab, env_radiance(a.s, d0));
+        if (a.spatial_count > 0) {  // the spatial launch's record of the primary vertex
+            a.spatial_aux[2 * pixel] = make_float4(d0.x, d0.y, d0.z, p.primary_hit ? p.utab : -1.0f);
+            if (p.primary_hit) a.spatial_aux[2 * pixel + 1] = spec4(p.outer);
+        }
         store_path(a, pixel, p, true);
         if (kTraced && ray) store_grec(a.whit + pixel, true, next);
         code = query_code(a, pixel);
@@ -586,6 +590,45 @@ __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__
         wave_add(c + 6, st.ts.tris);
         wave_add(c + 7, st.ts.instances);
     }
+}
+
+// Spatial reuse of the primary vertex (deferred.h: spatial_reuse_pixel), after wave_shade0 has
+// stored every pixel's post-temporal reservoir. Per shaded pixel: the primary vertex's emission
+// plus the reused direct light (rgb2[2p]) and plus nothing (rgb2[2p+1]), as spectra, and flags[p]
+// = (selected domain slot + 1) << 1; the winner's shadow ray (none when it was tested in an
+// earlier frame) joins vertex 0's shadow queue, whose traversal sets bit 0 of the flags when it
+// is occluded.
+template <bool kCount>
+__global__ void __launch_bounds__(kLanes) wave_spatial_kernel(const FrameArgs *__restrict__ args,
+                                                              const int *__restrict__ tile_base, int n_views) {
+    __shared__ int dl[(kMaxSpatial + 1) * kLanes];
+    __shared__ float cl[(kMaxSpatial + 1) * kLanes];
+    HDR_TILE(args, tile_base, n_views)
+    bool query = false;
+    ShadowQuery q;
+    if (valid && a.spatial_count > 0) {
+        const int pixel = y * a.W + x;
+        Vertex vq;
+        SpecTab tab;
+        if (load_spatial_vertex(a, a.gbuf, pixel, vq, tab)) {
+            V3 qo = v3s(0.0f), qd = v3s(0.0f);
+            float qtmax = 0.0f;
+            int ssel = -1;
+            const Spec direct = spatial_reuse_pixel<false>(a, a.gbuf, x, y, vq, tab, dl + lane, cl + lane, query, qo,
+                                                           qd, qtmax, ssel);
+            const Spec e = rgb2spec(tab, mv3(vq.c.m->emission));
+            a.rgb2[2 * pixel] = spec4(e + direct);
+            a.rgb2[2 * pixel + 1] = spec4(e + spec(0.0f));
+            a.flags[pixel] = uint32_t(ssel + 1) << 1;
+            if (query) {
+                q.o = make_float4(qo.x, qo.y, qo.z, qtmax);
+                q.d = make_float4(qd.x, qd.y, qd.z, __uint_as_float(query_code(a, pixel)));
+            }
+        }
+    }
+    const unsigned int qi = wave_append(args[0].wcnt + 2, query, lane);
+    if (query) args[0].queue[qi] = q;
+    if (kCount) wave_add(a.counters + 2, query ? 1ULL : 0ULL);
 }
 
 // Vertex `bounce` >= 1 of every live path (the codes wave_trace appended).
@@ -698,10 +741,13 @@ wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce, int traced) {
         const int pixel = int(code & 0xffffffu);
         if (!any_hit<kCount, kLds>(a.s, v3(q.o.x, q.o.y, q.o.z), v3(q.d.x, q.d.y, q.d.z), q.o.w, ts, ring_tr + lane,
                                    ring_br + lane)) {
-            float4 *t = a.wstate + 4 * wave_pixels(a) + pixel;
-            const Spec tot = spec_of(*t) + spec_of(args[0].wcq[i]);
-            *t = spec4(tot);
+            if (!(bounce == 0 && a.spatial_count > 0)) {  // (a spatial winner's colour is already staged)
+                float4 *t = a.wstate + 4 * wave_pixels(a) + pixel;
+                const Spec tot = spec_of(*t) + spec_of(args[0].wcq[i]);
+                *t = spec4(tot);
+            }
         } else if (bounce == 0 && a.use_res) {
+            if (a.spatial_count > 0) a.flags[pixel] |= 1u;  // the spatial winner is occluded
             hdr_reservoir &r = a.res[pixel];
             r.throughput[0] = r.throughput[1] = r.throughput[2] = r.throughput[3] = 0.0f;
         }
@@ -729,7 +775,16 @@ __global__ void __launch_bounds__(kLanes) wave_resolve_kernel(const FrameArgs *_
         const float4 r2 = w[2 * P + pixel], r5 = w[5 * P + pixel], r7 = w[7 * P + pixel];
         const SpecTab tab = make_tab(r5.w);
         const bool hit = (__float_as_uint(r7.x) & kWavePrimaryHit) != 0;
-        const Spec L = hit ? spec_of(w[6 * P + pixel]) * spec_of(w[4 * P + pixel]) : spec_of(w[4 * P + pixel]);
+        uint32_t shash = __float_as_uint(r7.z);
+        Spec L;
+        if (hit && a.spatial_count > 0) {  // outer * ((emission + reused direct) + later vertices)
+            const uint32_t f = a.flags[pixel];
+            const Spec e = spec_of(a.rgb2[2 * pixel + (f & 1u)]);
+            L = spec_of(w[6 * P + pixel]) * (e + spec_of(w[4 * P + pixel]));
+            if (kDebug) shash = fnv(shash, uint32_t(int(f >> 1) - 1));
+        } else {
+            L = hit ? spec_of(w[6 * P + pixel]) * spec_of(w[4 * P + pixel]) : spec_of(w[4 * P + pixel]);
+        }
         V3 rgb = spec2rgb(tab, L);
         if (a.has_cam && a.cam.enable_exposure) rgb = rgb * a.exposure_mul;
         const float4 c4 = make_float4(rgb.x, rgb.y, rgb.z, 1.0f);
@@ -749,7 +804,7 @@ __global__ void __launch_bounds__(kLanes) wave_resolve_kernel(const FrameArgs *_
             hdr_pixel_debug &d = a.dbg[pixel];
             d.draws = draws;
             d.light_hash = __float_as_uint(r7.y);
-            d.select_hash = __float_as_uint(r7.z);
+            d.select_hash = shash;
             d.n_light_picks = __float_as_int(r7.w);
             d.sample_rgb[0] = rgb.x;
             d.sample_rgb[1] = rgb.y;
@@ -1382,12 +1437,13 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
     // With an environment or emissive materials, BSDF-sampled candidates can see an emitter: the
     // wavefront's shading launches then trace the bounce and the generated candidates themselves.
     const bool waveTraced = wave && (a.trace_generated || a.trace_last_connection);
-    // Spatial reuse (an extension) runs on the deferred path with temporal reservoirs.
+    // Spatial reuse (an extension) runs on the deferred and wavefront paths with temporal
+    // reservoirs.
     const bool spatial = pipe->spatial_count > 0;
-    if (spatial && !(deferred && a.use_res))
+    if (spatial && !((deferred || wave) && a.use_res))
         return fail(HDR_ERR_UNSUPPORTED,
-                    "spatial reuse needs ReSTIR (use_reservoir) at depth 1 with skip_visibility, opaque materials, "
-                    "no emitters reachable by BSDF rays and no environment");
+                    "spatial reuse needs ReSTIR (use_reservoir) with skip_visibility, opaque materials and no mesh "
+                    "light");
     if (spatial) {
         if (hdr_status st = ensure(view->res_t, P * sizeof(hdr_reservoir))) return st;
         if (hdr_status st = ensure(view->saux, P * 2 * sizeof(float4))) return st;
@@ -1427,6 +1483,12 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
         a.walive = static_cast<unsigned int *>(view->walive.p);
         a.walive2 = a.walive + P;
         a.wcnt = static_cast<unsigned int *>(view->wcnt.p);
+        if (spatial) {  // per pixel: emission + reused direct, visible / occluded; visibility + slot
+            if (hdr_status st = ensure(view->rgb2, P * 2 * sizeof(float4))) return st;
+            if (hdr_status st = ensure(view->flags, P * sizeof(uint32_t))) return st;
+            a.rgb2 = static_cast<float4 *>(view->rgb2.p);
+            a.flags = static_cast<uint32_t *>(view->flags.p);
+        }
     }
     a.view = 0;
     plan.deferred = deferred;
@@ -1541,6 +1603,16 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
             else HDR_W0(false, false);
         }
         HIP_TRY(hipGetLastError());
+        if (spatial) {  // inside the integration stage's span on this path
+            HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_SPATIAL], view->stream));
+            if (view->counting)
+                hipLaunchKernelGGL((wave_spatial_kernel<true>), grid, dim3(kLanes), 0, view->stream, da, dt, 1);
+            else
+                hipLaunchKernelGGL((wave_spatial_kernel<false>), grid, dim3(kLanes), 0, view->stream, da, dt, 1);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_SPATIAL], view->stream));
+            ts.ran |= 1u << HDR_STAGE_SPATIAL;
+        }
         HIP_TRY(hipEventRecord(view->shade_done, view->stream));  // the G-buffer has been read
         view->shade_recorded = true;
         for (int b = 0; b < plan.depth; ++b) {
@@ -1915,6 +1987,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
     // launches), each run as its own launch sequence over the shared queues
     int waveDepth[2] = {0, 0};
     bool waveLean0[2] = {true, true};  // every view's primary vertex stays below RNG output 311
+    bool waveSpatial[2] = {false, false};  // some view of the group reuses spatially
     for (int i = 0; i < n; ++i) {
         if (hdr_status st = prepare_frame(views[i], scenes[i], &frames[i], &pipes[i], false, args[i], plans[i]))
             return st;
@@ -1924,6 +1997,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
             waveCap += size_t(views[i]->W) * views[i]->H;
             waveDepth[t] = std::max(waveDepth[t], plans[i].depth);
             waveLean0[t] = waveLean0[t] && plans[i].wave_lean0;
+            waveSpatial[t] = waveSpatial[t] || plans[i].spatial;
         }
     }
     if (queueCap * sizeof(ShadowQuery) > b->queue.n || waveCap * sizeof(ShadowQuery) > b->wqueue.n) {
@@ -2037,6 +2111,9 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
             hipLaunchKernelGGL(wave_zero_counts_kernel, dim3(1), dim3(kLanes), 0, b->stream,
                                static_cast<unsigned int *>(b->wcnt.p));
             launch_wave_shade0<false, false>(waveLean0[t], t == 1, tiles[wg], lds[wg], b->stream, A(wg), T(wg), N(wg));
+            if (waveSpatial[t])
+                hipLaunchKernelGGL((wave_spatial_kernel<false>), dim3(tiles[wg]), dim3(kLanes), 0, b->stream, A(wg),
+                                   T(wg), N(wg));
             for (int v = 0; v < waveDepth[t]; ++v) {
                 if (v > 0) launch_wave_shade<false, false>(ls, qblocks, lds[wg], b->stream, A(wg), v, t == 1);
                 launch_wave_trace<false>(ls, qblocks, b->stream, A(wg), v, t == 1);
@@ -2060,6 +2137,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
                                      uint64_t(uintptr_t(b->wcq.p)), uint64_t(uintptr_t(b->walive.p)),
                                      uint64_t(queueCap), uint64_t(waveCap), uint64_t(waveDepth[0]),
                                      uint64_t(waveDepth[1]), uint64_t(waveLean0[0]), uint64_t(waveLean0[1]),
+                                     uint64_t(waveSpatial[0]), uint64_t(waveSpatial[1]),
                                      uint64_t(ls.queue_waves), uint64_t(ls.trace_occ), uint64_t(ls.shade_occ),
                                      uint64_t(ls.lds_stack)};
         for (int g = 0; g < kGroups; ++g) {

@@ -33,10 +33,12 @@ double ms_since(Clock::time_point a, Clock::time_point b) {
 uint64_t view_footprint(int w, int h, const hdr_pipeline_desc &p) {
     const uint64_t P = uint64_t(w) * uint64_t(h);
     uint64_t per = sizeof(hdr_reservoir) + 16 + 16 + 64;
-    if (p.direct_light == HDR_DIRECT_RIS && p.max_depth == 1 && p.skip_visibility) per += 32 + 4 + 32;
-    // wavefront path: path record, bounce hit, ray / shadow / live queues
-    if (p.direct_light == HDR_DIRECT_RIS && p.max_depth > 1 && p.skip_visibility) per += 128 + 64 + 32 + 32 + 16 + 4;
-    if (p.spatial_count > 0) per += sizeof(hdr_reservoir) + 32;
+    // RIS with skipVisibility: the deferred path (depth 1; colour pair, flags, shadow queue) or,
+    // at any depth, the wavefront path (path record, bounce hit, ray / shadow / contribution
+    // queues, two live lists); the scene decides which, so the larger one is counted
+    if (p.direct_light == HDR_DIRECT_RIS && p.skip_visibility) per += 144 + 64 + 32 + 32 + 16 + 8;
+    // spatial reuse: post-temporal reservoirs, per-pixel record, colour pair and flags
+    if (p.spatial_count > 0) per += sizeof(hdr_reservoir) + 32 + 32 + 4;
     // draw bound of the megakernel path (host_scene / hdr_render_frame use the same count)
     const int M = p.candidate_count;
     const int nNee = std::max(1, M / 2), nBsdf = std::max(0, M - nNee);

@@ -6,6 +6,12 @@
 // shadow ray (any hit). The traversal launch answers both queues and compacts the paths that go on
 // into the next vertex's work list, so later (sparser) vertices launch only over live paths.
 //
+// Spatial reuse (an extension, deferred.h: spatial_reuse_pixel): the primary vertex stores its
+// post-temporal reservoir and the spatial launch after wave_shade0 resamples it with the
+// neighbours'; its winner's shadow ray joins vertex 0's shadow queue and only sets the pixel's
+// occlusion flag. The path total then holds the later vertices from zero, and the pixel's
+// radiance is outer * ((emission + reused direct) + total).
+//
 // The host admits a frame here under the same conditions as the deferred path except depth and
 // RNG range: skipVisibility (candidates are ranked without shadow rays), no translucent material,
 // and nothing BSDF-sampled can reach an emitter (no emissive material, mesh light or environment),
@@ -149,7 +155,9 @@ __device__ __forceinline__ void wave_vertex(const FrameArgs &a, const SpecTab &t
     Vertex v;
     if (first) v = make_vertex(a, tab, h, c, p.dir, p.outer);
     else v = make_vertex(a, tab, h, c, p.dir, p.tp);
-    if (first) p.total = rgb2spec(tab, mv3(c.m->emission));
+    // with spatial reuse the primary vertex's emission and direct light join in the spatial pass;
+    // the total then holds the later vertices only
+    if (first) p.total = a.spatial_count > 0 ? spec(0.0f) : rgb2spec(tab, mv3(c.m->emission));
     Bounce b;
     const bool ok = sample_bounce_inl(tab, v.c, v.sn, v.inside, v.h.pos, v.rayDir, a.max_refl, a.max_refr, p.refl,
                                       p.refr, p.rng, b);

@@ -194,8 +194,10 @@ typedef struct hdr_pipeline_desc {
      * merge, resample among its reservoir and those of spatial_count (<= 8) neighbours of this
      * frame picked in a disc of spatial_radius pixels (depth / normal similarity test), with
      * generalized balance-heuristic weights; the winner gets the shadow ray and is stored for
-     * the next frame. Supported for ReSTIR (use_reservoir) at max_depth 1 with skip_visibility,
-     * opaque materials, no emissive material or mesh light and no environment; otherwise
+     * the next frame. At max_depth > 1 the reuse replaces the primary vertex's direct light;
+     * the later vertices are rendered as without it and their radiance is added after it.
+     * Supported for ReSTIR (use_reservoir) with skip_visibility, opaque materials and no mesh
+     * light, at any depth, with or without an environment or emissive materials; otherwise
      * hdr_render_frame returns HDR_ERR_UNSUPPORTED. */
     int32_t  spatial_count;
     float    spatial_radius;
@@ -301,7 +303,7 @@ enum {
     HDR_STAGE_SHADOW = 2,     /* deferred path: any-hit shadow rays of the winners               */
     HDR_STAGE_ACCUMULATE = 3, /* deferred path: visibility pick + accumulation                   */
     HDR_STAGE_FIREFLY = 4,    /* firefly filter + accumulation (enable_firefly_filter)           */
-    HDR_STAGE_SPATIAL = 5,    /* deferred path with spatial reuse: neighbour resampling + colour */
+    HDR_STAGE_SPATIAL = 5,    /* spatial reuse: neighbour resampling + colour                    */
     HDR_STAGE_COUNT = 6
 };
 /* Per-stage device time (ms) summed over the frames rendered since the previous call, and the

@@ -1510,6 +1510,9 @@ struct SpatialAux {
     bool inside = false;
     Lambda lam;
     Spec throughput, emission;
+    // the radiance of the path's later vertices (1 .. maxDepth-1), summed in path order from zero;
+    // the spatial pass adds it after the primary vertex's emission and reused direct light
+    Spec rest;
 };
 
 struct PathCtx {  // per-pixel thread-local state passed through the integrators
@@ -2005,16 +2008,22 @@ Spec PathLiInner(const PathCtx &c, const ShadingPoint &sp0, const HitRecord &hit
     const Scene &scene = c.scene;
     const BounceConfig cfg{c.pipe.max_reflection_bounces, c.pipe.max_refraction_bounces};
     const LightSampler sampler(scene.lightPtrs, c.hoisted);  // _factory->Create (:99)
-    Spec total = RGBToSpectrum(sp0.c->Emission, *sp0.lambda);
+    // With spatial reuse (an extension) the primary vertex's emission and direct light are added by
+    // the spatial pass, so the sum here starts from zero and holds the later vertices only.
+    Spec total = c.aux ? Spec(0.0f) : RGBToSpectrum(sp0.c->Emission, *sp0.lambda);
     Spec throughput(1.0f);
     BounceState st{};
+    auto done = [&]() -> Spec {
+        if (c.aux) c.aux->rest = total;
+        return total;
+    };
     const MaterialParams *firstMat = scene.GetMaterial(hit0.MatId);
     Connection first;
     const bool firstOk = SampleBounceWithConnection(c, firstMat, sp0, hit0, ray0Dir, cfg, st, first, 0);
     std::optional<Connection> firstConn;
     if (firstOk) firstConn = first;
     total += DirectLi(c, sampler, sp0, firstMat, hit0, ray0Dir, firstConn, true);
-    if (!firstOk) return total;
+    if (!firstOk) return done();
     throughput *= first.Bounce.ThroughputMul;
     Ray cur = first.Bounce.NextRay;
     std::optional<HitRecord> next = first.Hit;
@@ -2051,7 +2060,7 @@ Spec PathLiInner(const PathCtx &c, const ShadingPoint &sp0, const HitRecord &hit
             throughput *= 1.0f / p;
         }
     }
-    return total;
+    return done();
 }
 
 Spec PathLi(const PathCtx &c, const Ray &ray, std::optional<HitRecord> hit) {  // Li :48-90
@@ -2170,6 +2179,9 @@ inline uint32_t PixelSeed(size_t i, int frameIndex) {  // integration_pass.cpp:2
 // reservoir form of reservoir.h (WSum = W * frames * p_q, count = sum of confidences, capped).
 // Neighbour picks and selections draw from a splitmix64 stream seeded by (pixel, frame),
 // integer-only, so the reference's per-pixel mt19937_64 stream is untouched.
+// Any depth: the reuse replaces the primary vertex's direct light only. The path's later vertices
+// run as in the reference (same draws) and sum, from zero, into SpatialAux::rest; the pixel's
+// radiance is throughput * ((emission + reused direct) + rest).
 constexpr int kMaxSpatial = 8;
 struct SpatialRng {
     uint64_t s;
@@ -2318,8 +2330,8 @@ double oracle_render_frame(oracle_ctx *c, const hdr_frame_desc *frame, const hdr
     const bool hasCam = frame->has_camera_params != 0;
     const int frameIndex = frame->frame_index;
     const bool spatial = pipe->spatial_count > 0;
-    if (spatial && (pipe->direct_light != HDR_DIRECT_RIS || !pipe->use_reservoir || pipe->max_depth != 1))
-        return -1.0;  // spatial reuse is defined for ReSTIR direct lighting at depth 1
+    if (spatial && (pipe->direct_light != HDR_DIRECT_RIS || !pipe->use_reservoir))
+        return -1.0;  // spatial reuse is defined for ReSTIR direct lighting (at the primary vertex)
     if (spatial)
         for (SpatialAux &a : c->aux) a.valid = false;
     constexpr int kTile = 16;
@@ -2523,7 +2535,7 @@ double oracle_render_frame(oracle_ctx *c, const hdr_frame_desc *frame, const hdr
                         const Spec direct =
                             (!s.ChosenSample || s.W <= 0.0f) ? Spec() : s.ChosenSample->Throughput * s.W;
                         c->reservoirs[i] = s;
-                        const Spec Ls = q.throughput * (q.emission + direct);
+                        const Spec Ls = q.throughput * ((q.emission + direct) + q.rest);
                         V3 L = SpectrumToRGB(Ls, q.lam);
                         if (hasCam && frame->camera.enable_exposure) L = L * ExposureMultiplier(frame->camera);
                         fb[4 * i + 0] = L[0];

@@ -82,7 +82,7 @@ class Oracle:
         self.last_seconds = lib().oracle_render_frame(self.h, C.byref(frame), C.byref(pipe), x0, y0, x1, y1,
                                                       _p(self.color), _p(self.debug) if debug else None)
         if self.last_seconds < 0:
-            raise ValueError("the oracle does not define this pipeline (spatial reuse needs ReSTIR at depth 1)")
+            raise ValueError("the oracle does not define this pipeline (spatial reuse needs ReSTIR: RIS with the reservoir)")
         return self.color
 
     def reservoirs(self):

@@ -15,6 +15,7 @@ def _cases():
     box_small = scenes.box_spheres(40, 24, frames=5)
     feat = scenes.features(48, 40, frames=5, dome=False, emissive=False, opaque=True)
     feat_dome = scenes.features(32, 32, frames=5)
+    feat_dome_opaque = scenes.features(32, 32, frames=5, dome=True, emissive=False, mesh_light=False, opaque=True)
     return [
         (box, abi.pipeline("ris", candidate_count=16, use_reservoir=True, skip_visibility=True, max_depth=1)),
         (feat, abi.pipeline("ris", candidate_count=8, use_reservoir=True, skip_visibility=True, max_depth=1,
@@ -25,6 +26,11 @@ def _cases():
         # the wavefront path, two depths in one group
         (box_small, abi.pipeline("ris", candidate_count=16, use_reservoir=True, skip_visibility=True, max_depth=5)),
         (feat, abi.pipeline("ris", candidate_count=8, use_reservoir=False, skip_visibility=True, max_depth=3)),
+        # spatial reuse on the wavefront path, with and without traced candidates
+        (box, abi.pipeline("ris", candidate_count=8, use_reservoir=True, skip_visibility=True, max_depth=4,
+                           spatial_count=3, spatial_radius=12.0)),
+        (feat_dome_opaque, abi.pipeline("ris", candidate_count=8, use_reservoir=True, skip_visibility=True,
+                                        max_depth=3, spatial_count=2, spatial_radius=8.0)),
     ]
 
 

@@ -395,8 +395,51 @@ def test_spatial_reuse_extension_is_unbiased_on_diffuse_scene(oracle_mod):
         assert abs(spa[m].mean() / ref[m].mean() - 1.0) < 0.02, (k, spa[m].mean(), ref[m].mean())
 
 
+def _fnv(h, v):
+    for b in range(4):
+        h ^= (v >> (8 * b)) & 0xFF
+        h = (h * 16777619) & 0xFFFFFFFF
+    return h
+
+
+def test_spatial_reuse_any_depth_reduces_to_the_temporal_result_without_neighbours(oracle_mod):
+    """At depth > 1 the spatial pass replaces the primary vertex's direct light only; the later
+    vertices run as without it and their radiance is added after it. A pixel rendered alone (a
+    one-pixel rect: no neighbour is shaded) has one reuse domain, its own reservoir, so the pass
+    returns the temporal winner with the same weight: the colour equals the one without spatial
+    reuse up to the order of the float sums, the RNG streams are the same, and the selection hash
+    gains the slot -1. Rendering at depth 1 as well shows that the later vertices carry radiance
+    here, so a dropped (or doubled) later-vertex sum fails."""
+    W, H = 48, 40
+    sc = scenes.box_spheres(W, H, frames=1)
+    base = dict(candidate_count=8, use_reservoir=True, skip_visibility=True)
+    rng = np.random.default_rng(5)
+    pix = [(int(x), int(y)) for x, y in zip(rng.integers(0, W, 24), rng.integers(0, H, 24))]
+    o0, o1, o2 = (oracle_mod.Oracle(sc, threads=1) for _ in range(3))
+    p0 = abi.pipeline("ris", max_depth=4, **base)
+    p1 = abi.pipeline("ris", max_depth=4, spatial_count=3, spatial_radius=30.0, **base)
+    p2 = abi.pipeline("ris", max_depth=1, spatial_count=3, spatial_radius=30.0, **base)
+    shaded = indirect = 0
+    for x, y in pix:
+        i = y * W + x
+        rect = (x, y, x + 1, y + 1)
+        o0.render(sc.frame(0), p0, rect=rect)
+        o1.render(sc.frame(0), p1, rect=rect)
+        o2.render(sc.frame(0), p2, rect=rect)
+        d0, d1, d2 = o0.debug[i], o1.debug[i], o2.debug[i]
+        assert d0["draws"] == d1["draws"] and d0["light_hash"] == d1["light_hash"]
+        c0, c1, c2 = (np.array(d["sample_rgb"], np.float64) for d in (d0, d1, d2))
+        assert np.allclose(c1, c0, rtol=1e-4, atol=1e-7), (x, y, c0, c1)
+        if d0["prim_instance"] >= 0:
+            shaded += 1
+            assert int(d1["select_hash"]) == _fnv(int(d0["select_hash"]), 0xFFFFFFFF)
+            indirect += int(np.abs(c1 - c2).max() > 1e-3 * max(np.abs(c1).max(), 1e-6))
+    print("shaded", shaded, "with indirect radiance", indirect)
+    assert shaded >= 16 and indirect >= shaded // 2, (shaded, indirect)
+
+
 def test_spatial_reuse_rejects_unsupported_pipelines(oracle_mod):
-    """The extension is defined for ReSTIR at depth 1; the oracle refuses other pipelines."""
+    """The extension is defined for ReSTIR; the oracle refuses other pipelines."""
     sc = scenes.box_spheres(16, 16, frames=1)
     o = oracle_mod.Oracle(sc, threads=1)
     pipe = abi.pipeline("ris", candidate_count=8, use_reservoir=False, max_depth=1, spatial_count=3)
