@@ -32,6 +32,10 @@ __device__ __forceinline__ void store_grec(GRec *dst, bool hit, const HitRec &g)
     gstore4(p + 2, make_float4(g.ns.x, g.ns.y, g.ns.z, __int_as_float(g.tri)));
     gstore4(p + 3, make_float4(g.albedo.x, g.albedo.y, g.albedo.z, 0.0f));
 }
+// A record that says "no hit" (load_grec returns false for it).
+__device__ __forceinline__ void store_grec_miss(GRec *dst) {
+    gstore4(reinterpret_cast<float4 *>(dst) + 1, make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1)));
+}
 __device__ __forceinline__ bool load_grec(const GRec *src, HitRec &g) {
     const float4 *p = reinterpret_cast<const float4 *>(src);
     GRec r;
@@ -428,12 +432,15 @@ __device__ __forceinline__ void rebuild_traced(const FrameArgs &a, const SpecTab
 }
 
 // RisDirectLightIntegrator::Li (:84-215) with traced BSDF candidates; weights of every candidate in
-// LDS (wl[k * kLanes], k < n_cand). Shadow query and return value as ris_direct_lean.
+// LDS (wl[k * kLanes], k < n_cand). Shadow query and return value as ris_direct_lean. The
+// generated candidates' escape tests (any hit up to 1e30, from the sampled ray) were answered
+// beforehand: bit k of `escm` is set when candidate k's ray escapes.
 template <bool kCount, bool kDebug, class R>
 __device__ __forceinline__ Spec ris_direct_traced(const FrameArgs &a, const SpecTab &tab, const Vertex &v,
                                                   const Bounce *conn, bool connHit, const HitRec &connRec, R &rng,
                                                   double *wl, int pixel, PixelStats &st, Trace &dbg, bool reservoir,
-                                                  uint64_t *slot2, bool &query, V3 &qo, V3 &qd, float &qtmax) {
+                                                  uint64_t *slot2, uint32_t escm, bool &query, V3 &qo, V3 &qd,
+                                                  float &qtmax) {
     const DeviceScene &s = a.s;
     const int nNee = a.n_nee, nCand = a.n_cand;
     const R start = rng_fork(rng, slot2);
@@ -462,7 +469,7 @@ __device__ __forceinline__ Spec ris_direct_traced(const FrameArgs &a, const Spec
             int refl = 0, refr = 0;
             if (sample_bounce_inl(tab, v.c, v.sn, v.inside, v.h.pos, v.rayDir, 1, 1, refl, refr, rng, b)) {
                 if (kCount) ++st.cand;
-                if (!trace_any<kCount>(s, b.o, b.d, 1e30f, st.ts)) {
+                if ((escm >> k) & 1u) {
                     esc |= 1u << k;
                     w = gen_env_weight(a, tab, b, nullptr);
                 }

@@ -519,26 +519,31 @@ __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__
     extern __shared__ double wl[];
     HDR_TILE(args, tile_base, n_views)
     unsigned int *cnt = args[0].wcnt;  // the queues of a launch are shared by its views
-    // live paths of vertex 1: appended by wave_trace (or, traced, by this launch after the
-    // frame-start clear)
+    // live paths of vertex 1: appended by wave_trace (traced frames: this launch appends vertex
+    // 0's live list after the frame-start clear)
     if (!kTraced && blockIdx.x == 0 && lane == 0) cnt[5] = 0u;
     PixelStats st = {};
-    bool query = false, ray = false;
+    bool query = false, ray = false, bray = false;
     ShadowQuery q;
     float4 cq = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ro = cq, rd = cq;
     HitRec next;
     uint32_t code = 0;
+    using R = typename std::conditional<kLean, Mt64Lean, Mt64Deep>::type;
+    R gr;           // traced: the stream copy wave_gen leaves at the first generated candidate
+    Vertex gv;      // and the vertex
+    SpecTab tab;
+    int k0 = 0, nc = 0;
     if (valid) {
         const int pixel = y * a.W + x;
         const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
-        WavePath<typename std::conditional<kLean, Mt64Lean, Mt64Deep>::type> p;
+        WavePath<R> p;
         p.rng.init(seed, wave_slot(a, pixel));
         const float px = float(x) + p.rng.next_float();
         const float py = float(y) + p.rng.next_float();
         V3 o0, d0;
         camera_ray(a, px, py, &p.rng, o0, d0);
         p.utab = p.rng.next_float();
-        const SpecTab tab = make_tab(p.utab);
+        tab = make_tab(p.utab);
         p.dbg = Trace{2166136261u, 2166136261u, 0};
         p.tp = spec(1.0f);
         p.outer = spec(1.0f);
@@ -547,17 +552,25 @@ __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__
         p.dir = d0;
         HitRec g;
         p.primary_hit = load_grec(a.gbuf + pixel, g);
-        if (p.primary_hit)
+        if (kTraced) {  // vertex 0's ray queries; its shading runs in the next shading launch
+            ray = p.primary_hit;
+            if (ray) {
+                gr = rng_fork(p.rng, wave_slot(a, pixel) ? wave_slot(a, pixel) + kMtN : nullptr);
+                wave_gen(a, tab, 0, g, p, pixel, gr, gv, bray, ro, rd, k0);
+                nc = a.n_cand;
+            }
+        } else if (p.primary_hit) {
             wave_vertex<kCount, kDebug, kTraced>(a, tab, 0, g, p, wl + lane, pixel, st, query, q, cq, ray, ro, rd,
                                                  next);
-        else if (a.s.environment >= 0 && a.ibl_bg)  // a miss sees the environment (PathIntegrator::Li, path_integrator.cpp:48-209)
+        }
+        if (!p.primary_hit && a.s.environment >= 0 && a.ibl_bg)  // a miss sees the environment (PathIntegrator::Li, path_integrator.cpp:48-209)
             p.total = rgb2spec(tab, env_radiance(a.s, d0));
         if (a.spatial_count > 0) {  // the spatial launch's record of the primary vertex
             a.spatial_aux[2 * pixel] = make_float4(d0.x, d0.y, d0.z, p.primary_hit ? p.utab : -1.0f);
-            if (p.primary_hit) a.spatial_aux[2 * pixel + 1] = spec4(p.outer);
+            // traced frames: the throughput is recorded where vertex 0 is shaded
+            if (!kTraced && p.primary_hit) a.spatial_aux[2 * pixel + 1] = spec4(p.outer);
         }
         store_path(a, pixel, p, true);
-        if (kTraced && ray) store_grec(a.whit + pixel, true, next);
         code = query_code(a, pixel);
         if (kDebug) {
             a.dbg[pixel].prim_instance = p.primary_hit ? g.inst : -1;
@@ -570,15 +583,16 @@ __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__
         args[0].queue[qi] = q;
         args[0].wcq[qi] = cq;
     }
-    if (kTraced) {
-        const unsigned int ai = wave_append(cnt + 5, ray, lane);
-        if (ray) alive_list(args[0], 1)[ai] = code;
-    } else {
-        const unsigned int ri = wave_append(cnt + 0, ray, lane);
-        if (ray) {
-            args[0].wray[2 * ri] = ro;
-            args[0].wray[2 * ri + 1] = rd;
-        }
+    if (kTraced) {  // vertex 0's live list, bounce rays and escape rays
+        const unsigned int ai = wave_append(cnt + 4, ray, lane);
+        if (ray) alive_list(args[0], 0)[ai] = code;
+        wave_emit_escapes(cnt + 6, args[0].wesq, tab, gv, gr, k0, nc, code, lane);
+        ray = bray;
+    }
+    const unsigned int ri = wave_append(cnt + 0, ray, lane);
+    if (ray) {
+        args[0].wray[2 * ri] = ro;
+        args[0].wray[2 * ri + 1] = rd;
     }
     if (kCount) {
         unsigned long long *c = a.counters;
@@ -632,7 +646,7 @@ __global__ void __launch_bounds__(kLanes) wave_spatial_kernel(const FrameArgs *_
 }
 
 // Vertex `bounce` >= 1 of every live path (the codes wave_trace appended).
-template <bool kCount, bool kDebug, int kOcc = 1, bool kTraced = false>
+template <bool kCount, bool kDebug, int kOcc = 1>
 __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
 wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce) {
     extern __shared__ double wl[];
@@ -640,9 +654,7 @@ wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce) {
     const int par = bounce & 1;
     unsigned int *cnt = args[0].wcnt;
     const unsigned int n = cnt[4 + par];
-    // live paths of the next vertex: appended by wave_trace (traced: by this launch, cleared by the
-    // traversal launch before it)
-    if (!kTraced && blockIdx.x == 0 && lane == 0) cnt[4 + (par ^ 1)] = 0u;
+    if (blockIdx.x == 0 && lane == 0) cnt[4 + (par ^ 1)] = 0u;  // live paths of the next vertex: appended by wave_trace
     PixelStats st = {};
     unsigned int nq = 0;
     for (unsigned int b0 = blockIdx.x * kLanes; b0 < n; b0 += gridDim.x * kLanes) {
@@ -650,9 +662,8 @@ wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce) {
         bool query = false, ray = false;
         ShadowQuery q;
         float4 cq = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ro = cq, rd = cq;
-        uint32_t code = 0;
         if (i < n) {
-            code = alive_list(args[0], bounce)[i];
+            const uint32_t code = alive_list(args[0], bounce)[i];
             const FrameArgs &a = args[code >> 24];
             const int pixel = int(code & 0xffffffu);
             WavePath<Mt64Deep> p;
@@ -660,25 +671,19 @@ wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce) {
             const SpecTab tab = make_tab(p.utab);
             HitRec h, next;
             (void)load_grec(a.whit + pixel, h);
-            wave_vertex<kCount, kDebug, kTraced>(a, tab, bounce, h, p, wl + lane, pixel, st, query, q, cq, ray, ro,
-                                                 rd, next);
+            wave_vertex<kCount, kDebug, false>(a, tab, bounce, h, p, wl + lane, pixel, st, query, q, cq, ray, ro, rd,
+                                               next);
             store_path(a, pixel, p, false);
-            if (kTraced && ray) store_grec(a.whit + pixel, true, next);
         }
         const unsigned int qi = wave_append(cnt + 2 + par, query, lane);
         if (query) {
             args[0].queue[qi] = q;
             args[0].wcq[qi] = cq;
         }
-        if (kTraced) {
-            const unsigned int ai = wave_append(cnt + 4 + (par ^ 1), ray, lane);
-            if (ray) alive_list(args[0], bounce + 1)[ai] = code;
-        } else {
-            const unsigned int ri = wave_append(cnt + par, ray, lane);
-            if (ray) {
-                args[0].wray[2 * ri] = ro;
-                args[0].wray[2 * ri + 1] = rd;
-            }
+        const unsigned int ri = wave_append(cnt + par, ray, lane);
+        if (ray) {
+            args[0].wray[2 * ri] = ro;
+            args[0].wray[2 * ri + 1] = rd;
         }
         nq += query ? 1u : 0u;
     }
@@ -694,13 +699,103 @@ wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce) {
     }
 }
 
+// Traced frames (wavefront.h: wave_gen): launch v (1 .. depth) shades vertex u = v - 1 of every
+// path in vertex u's live list -- its ray queries were answered by the traversal launch before --
+// and, for the paths that go on, emits vertex v's ray queries (bounce ray, escape rays) and
+// appends the path to vertex v's live list. Vertex u's shadow query joins the queue of parity u.
+template <bool kCount, bool kDebug, int kOcc = 1>
+__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
+wave_shade_traced_kernel(const FrameArgs *__restrict__ args, int v) {
+    extern __shared__ double wl[];
+    const int lane = threadIdx.x;
+    const int u = v - 1;
+    unsigned int *cnt = args[0].wcnt;
+    const unsigned int n = cnt[4 + (u & 1)];
+    PixelStats st = {};
+    unsigned int nq = 0;
+    for (unsigned int b0 = blockIdx.x * kLanes; b0 < n; b0 += gridDim.x * kLanes) {
+        const unsigned int i = b0 + unsigned(lane);
+        bool query = false, cont = false, bray = false;
+        ShadowQuery q;
+        float4 cq = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ro = cq, rd = cq, bo = cq, bd = cq;
+        uint32_t code = 0;
+        Mt64Deep gr;
+        Vertex gv;
+        SpecTab tab;
+        int k0 = 0, nc = 0;
+        if (i < n) {
+            code = alive_list(args[0], u)[i];
+            const FrameArgs &a = args[code >> 24];
+            const int pixel = int(code & 0xffffffu);
+            WavePath<Mt64Deep> p;
+            load_path(a, pixel, p, u == 0);
+            tab = make_tab(p.utab);
+            HitRec h, next;
+            (void)load_grec((u == 0 ? a.gbuf : traced_hits(a, u)) + pixel, h);
+            wave_vertex<kCount, kDebug, true>(a, tab, u, h, p, wl + lane, pixel, st, query, q, cq, cont, ro, rd,
+                                              next);
+            if (u == 0 && a.spatial_count > 0) a.spatial_aux[2 * pixel + 1] = spec4(p.outer);
+            if (cont) {
+                uint64_t *slot = wave_slot(a, pixel);
+                gr = rng_fork(p.rng, slot ? slot + kMtN : nullptr);
+                wave_gen(a, tab, v, next, p, pixel, gr, gv, bray, bo, bd, k0);
+                nc = a.n_cand;
+            }
+            store_path(a, pixel, p, u == 0);
+        }
+        const unsigned int qi = wave_append(cnt + 2 + (u & 1), query, lane);
+        if (query) {
+            args[0].queue[qi] = q;
+            args[0].wcq[qi] = cq;
+        }
+        wave_emit_escapes(cnt + 6 + (v & 1), args[0].wesq, tab, gv, gr, k0, nc, code, lane);
+        const unsigned int ri = wave_append(cnt + (v & 1), bray, lane);
+        if (bray) {
+            args[0].wray[2 * ri] = bo;
+            args[0].wray[2 * ri + 1] = bd;
+        }
+        const unsigned int ai = wave_append(cnt + 4 + (v & 1), cont, lane);
+        if (cont) alive_list(args[0], v)[ai] = code;
+        nq += query ? 1u : 0u;
+    }
+    if (kCount) {
+        unsigned long long *c = args[0].counters;
+        wave_add(c + 2, nq);
+        wave_add(c + 3, st.cand);
+        wave_add(c + 4, st.cand_skipped);
+    }
+}
+
+// Shadow ray i of the queue (vertex `vtx`): an unoccluded winner's contribution joins the path
+// total (a spatial winner's colour is already staged); an occluded primary winner keeps zero
+// throughput in its reservoir (ris_direct_light_integrator.cpp:163-175).
+template <bool kCount, bool kLds>
+__device__ __forceinline__ void wave_shadow(const FrameArgs *__restrict__ args, unsigned int i, int vtx,
+                                            TraceStats &ts, int *ring_tr, int *ring_br) {
+    const ShadowQuery q = args[0].queue[i];
+    const uint32_t code = __float_as_uint(q.d.w);
+    const FrameArgs &a = args[code >> 24];
+    const int pixel = int(code & 0xffffffu);
+    if (!any_hit<kCount, kLds>(a.s, v3(q.o.x, q.o.y, q.o.z), v3(q.d.x, q.d.y, q.d.z), q.o.w, ts, ring_tr, ring_br)) {
+        if (!(vtx == 0 && a.spatial_count > 0)) {
+            float4 *t = a.wstate + 4 * wave_pixels(a) + pixel;
+            const Spec tot = spec_of(*t) + spec_of(args[0].wcq[i]);
+            *t = spec4(tot);
+        }
+    } else if (vtx == 0 && a.use_res) {
+        if (a.spatial_count > 0) a.flags[pixel] |= 1u;  // the spatial winner is occluded
+        hdr_reservoir &r = a.res[pixel];
+        r.throughput[0] = r.throughput[1] = r.throughput[2] = r.throughput[3] = 0.0f;
+    }
+}
+
 // The bounce rays (closest hit: the next vertex, appended to the live list when there is one) and
 // the shadow rays (any hit: an unoccluded winner's contribution joins the path total; an occluded
 // primary winner keeps zero throughput in its reservoir, ris_direct_light_integrator.cpp:163-175)
 // of vertex `bounce`.
 template <bool kCount, int kOcc = 1, bool kLds = false>
 __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
-wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce, int traced) {
+wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce) {
     HDR_RING_LDS
     const int lane = threadIdx.x;
     const int par = bounce & 1;
@@ -709,7 +804,6 @@ wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce, int traced) {
     if (blockIdx.x == 0 && lane == 0) {
         cnt[par ^ 1] = 0u;        // rays of the next vertex
         cnt[2 + (par ^ 1)] = 0u;  // shadow queries of the next vertex
-        if (traced) cnt[4 + par] = 0u;  // live paths the next shading launch appends (its input is the other list)
     }
     TraceStats ts = {};
     unsigned int nb = 0;
@@ -732,26 +826,62 @@ wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce, int traced) {
         const unsigned int ai = wave_append(cnt + 4 + (par ^ 1), hit, lane);
         if (hit) alive_list(args[0], bounce + 1)[ai] = code;
     }
-    for (unsigned int b0 = blockIdx.x * kLanes; b0 < ns; b0 += gridDim.x * kLanes) {
-        const unsigned int i = b0 + unsigned(lane);
-        if (i >= ns) break;
-        const ShadowQuery q = args[0].queue[i];
-        const uint32_t code = __float_as_uint(q.d.w);
+    for (unsigned int i = blockIdx.x * kLanes + unsigned(lane); i < ns; i += gridDim.x * kLanes)
+        wave_shadow<kCount, kLds>(args, i, bounce, ts, ring_tr + lane, ring_br + lane);
+    if (kCount) {
+        unsigned long long *c = args[0].counters;
+        wave_add(c + 1, nb);
+        wave_add(c + 5, ts.nodes);
+        wave_add(c + 6, ts.tris);
+        wave_add(c + 7, ts.instances);
+    }
+}
+
+// Traced frames: traversal launch v (0 .. depth) after shading launch v -- vertex v's bounce rays
+// (closest hit: vertex v+1's hit record, or a miss record) and escape rays (any hit up to 1e30:
+// the candidate's bit of the pixel's escape mask when none), and vertex v-1's shadow rays.
+template <bool kCount, int kOcc = 1, bool kLds = false>
+__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
+wave_trace_traced_kernel(const FrameArgs *__restrict__ args, int v) {
+    HDR_RING_LDS
+    const int lane = threadIdx.x;
+    const int par = v & 1;
+    unsigned int *cnt = args[0].wcnt;
+    const unsigned int nr = cnt[par], ne = cnt[6 + par], ns = v > 0 ? cnt[2 + (par ^ 1)] : 0u;
+    if (blockIdx.x == 0 && lane == 0) {  // the counts the next shading launch appends to
+        cnt[par ^ 1] = 0u;        // bounce rays of vertex v+1
+        cnt[6 + (par ^ 1)] = 0u;  // escape rays of vertex v+1
+        cnt[2 + par] = 0u;        // shadow queries of vertex v
+        cnt[4 + (par ^ 1)] = 0u;  // live paths of vertex v+1
+    }
+    TraceStats ts = {};
+    unsigned int nb = 0;
+    for (unsigned int i = blockIdx.x * kLanes + unsigned(lane); i < nr; i += gridDim.x * kLanes) {
+        const float4 o4 = args[0].wray[2 * i], d4 = args[0].wray[2 * i + 1];
+        const uint32_t code = __float_as_uint(o4.w);
         const FrameArgs &a = args[code >> 24];
         const int pixel = int(code & 0xffffffu);
-        if (!any_hit<kCount, kLds>(a.s, v3(q.o.x, q.o.y, q.o.z), v3(q.d.x, q.d.y, q.d.z), q.o.w, ts, ring_tr + lane,
-                                   ring_br + lane)) {
-            if (!(bounce == 0 && a.spatial_count > 0)) {  // (a spatial winner's colour is already staged)
-                float4 *t = a.wstate + 4 * wave_pixels(a) + pixel;
-                const Spec tot = spec_of(*t) + spec_of(args[0].wcq[i]);
-                *t = spec4(tot);
-            }
-        } else if (bounce == 0 && a.use_res) {
-            if (a.spatial_count > 0) a.flags[pixel] |= 1u;  // the spatial winner is occluded
-            hdr_reservoir &r = a.res[pixel];
-            r.throughput[0] = r.throughput[1] = r.throughput[2] = r.throughput[3] = 0.0f;
-        }
+        const V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
+        RawHit rh;
+        GRec *dst = traced_hits(a, v + 1) + pixel;
+        if (closest_hit<kCount, kLds>(a.s, o, d, rh, ts, ring_tr + lane, ring_tt + lane, ring_br + lane,
+                                      ring_bt + lane))
+            store_grec(dst, true, make_hit_record(a.s, o, d, rh));
+        else
+            store_grec_miss(dst);
+        ++nb;
     }
+    for (unsigned int i = blockIdx.x * kLanes + unsigned(lane); i < ne; i += gridDim.x * kLanes) {
+        const float4 o4 = args[0].wesq[2 * i], d4 = args[0].wesq[2 * i + 1];
+        const uint32_t code = __float_as_uint(o4.w);
+        const FrameArgs &a = args[code >> 24];
+        const int pixel = int(code & 0xffffffu);
+        if (!any_hit<kCount, kLds>(a.s, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), 1e30f, ts, ring_tr + lane,
+                                   ring_br + lane))
+            atomicOr(a.wesc + pixel, 1u << __float_as_int(d4.w));
+    }
+    for (unsigned int i = blockIdx.x * kLanes + unsigned(lane); i < ns; i += gridDim.x * kLanes)
+        wave_shadow<kCount, kLds>(args, i, v - 1, ts, ring_tr + lane, ring_br + lane);
     if (kCount) {
         unsigned long long *c = args[0].counters;
         wave_add(c + 1, nb);
@@ -955,34 +1085,43 @@ static int queue_blocks(const LaunchShape &ls, size_t cap) {
     const size_t need = std::max<size_t>(1, (cap + kLanes - 1) / kLanes);
     return int(std::min<size_t>(need, size_t(ls.queue_waves > 0 ? ls.queue_waves : kDefaultQueueWaves)));
 }
-template <bool C, bool L>
-static void launch_wave_trace_l(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a, int b, int t) {
+template <bool C, bool L, bool T>
+static void launch_wave_trace_l(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a, int b) {
+#define HDR_WT(O)                                                                                          \
+    if (T) hipLaunchKernelGGL((wave_trace_traced_kernel<C, O, L>), dim3(blocks), dim3(kLanes), 0, st, a, b); \
+    else hipLaunchKernelGGL((wave_trace_kernel<C, O, L>), dim3(blocks), dim3(kLanes), 0, st, a, b);
     switch (ls.trace_occ) {
-    case 4: hipLaunchKernelGGL((wave_trace_kernel<C, 4, L>), dim3(blocks), dim3(kLanes), 0, st, a, b, t); break;
-    case 6: hipLaunchKernelGGL((wave_trace_kernel<C, 6, L>), dim3(blocks), dim3(kLanes), 0, st, a, b, t); break;
-    case 8: hipLaunchKernelGGL((wave_trace_kernel<C, 8, L>), dim3(blocks), dim3(kLanes), 0, st, a, b, t); break;
-    default: hipLaunchKernelGGL((wave_trace_kernel<C, 1, L>), dim3(blocks), dim3(kLanes), 0, st, a, b, t); break;
+    case 4: HDR_WT(4) break;
+    case 6: HDR_WT(6) break;
+    case 8: HDR_WT(8) break;
+    default: HDR_WT(1) break;
     }
+#undef HDR_WT
 }
-// traced: the frames' shading launches trace their own bounce rays (wave_vertex kTraced).
+// Traversal launch b; traced frames: wave_trace_traced_kernel (bounce, escape and shadow rays).
 template <bool C>
 static void launch_wave_trace(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a, int b,
                               bool traced) {
-    if (ls.lds_stack) launch_wave_trace_l<C, true>(ls, blocks, st, a, b, traced ? 1 : 0);
-    else launch_wave_trace_l<C, false>(ls, blocks, st, a, b, traced ? 1 : 0);
+    if (traced) {
+        if (ls.lds_stack) launch_wave_trace_l<C, true, true>(ls, blocks, st, a, b);
+        else launch_wave_trace_l<C, false, true>(ls, blocks, st, a, b);
+    } else {
+        if (ls.lds_stack) launch_wave_trace_l<C, true, false>(ls, blocks, st, a, b);
+        else launch_wave_trace_l<C, false, false>(ls, blocks, st, a, b);
+    }
 }
-template <bool C, bool D, bool T>
-static void launch_wave_shade_t(const LaunchShape &ls, int blocks, size_t lds, hipStream_t st, const FrameArgs *a,
-                                int b) {
-    if (ls.shade_occ == 3)
-        hipLaunchKernelGGL((wave_shade_kernel<C, D, 3, T>), dim3(blocks), dim3(kLanes), lds, st, a, b);
-    else hipLaunchKernelGGL((wave_shade_kernel<C, D, 1, T>), dim3(blocks), dim3(kLanes), lds, st, a, b);
-}
+// Shading launch b >= 1; traced frames: wave_shade_traced_kernel.
 template <bool C, bool D>
 static void launch_wave_shade(const LaunchShape &ls, int blocks, size_t lds, hipStream_t st, const FrameArgs *a, int b,
                               bool traced) {
-    if (traced) launch_wave_shade_t<C, D, true>(ls, blocks, lds, st, a, b);
-    else launch_wave_shade_t<C, D, false>(ls, blocks, lds, st, a, b);
+    const dim3 g(blocks), l(kLanes);
+    if (traced) {
+        if (ls.shade_occ == 3) hipLaunchKernelGGL((wave_shade_traced_kernel<C, D, 3>), g, l, lds, st, a, b);
+        else hipLaunchKernelGGL((wave_shade_traced_kernel<C, D, 1>), g, l, lds, st, a, b);
+    } else {
+        if (ls.shade_occ == 3) hipLaunchKernelGGL((wave_shade_kernel<C, D, 3>), g, l, lds, st, a, b);
+        else hipLaunchKernelGGL((wave_shade_kernel<C, D, 1>), g, l, lds, st, a, b);
+    }
 }
 // The primary-vertex launch: lean RNG stream when no pixel can pass output 310 there; traced
 // candidates when BSDF-sampled rays can see an emitter.
@@ -1023,6 +1162,7 @@ struct hdr_view {
     DevBuf res, accum, color, frame, dbg, depth, albedo, normal, counters, spill, gbuf, rgb2, flags, queue, qcount;
     DevBuf res_t, saux;  // spatial reuse: post-temporal reservoirs, per-pixel {ray dir, u}, {throughput}
     DevBuf wstate, whit, wray, wcq, walive, wcnt;  // wavefront path: path records, hits, queues
+    DevBuf wesc, wesq;                             // traced frames: escape masks, escape-ray queue
     DevBuf dargs, dtile; // per-frame FrameArgs records (one per timing-ring slot); launch tile table {0, tiles}
     PinnedBuf<FrameArgs> hargs;  // pinned staging of dargs
     LaunchShape shape;           // queue-driven launches (hdr_view_set_launch_config)
@@ -1296,7 +1436,8 @@ hdr_status hdr_view_device_bytes(const hdr_view *v, uint64_t *out) {
     if (!v || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
     *out = v->res.n + v->accum.n + v->color.n + v->frame.n + v->dbg.n + v->depth.n + v->albedo.n + v->normal.n +
            v->counters.n + v->spill.n + v->gbuf.n + v->rgb2.n + v->flags.n + v->queue.n + v->qcount.n + v->res_t.n +
-           v->saux.n + v->wstate.n + v->whit.n + v->wray.n + v->wcq.n + v->walive.n + v->wcnt.n;
+           v->saux.n + v->wstate.n + v->whit.n + v->wray.n + v->wcq.n + v->walive.n + v->wcnt.n + v->wesc.n +
+           v->wesq.n;
     return HDR_OK;
 }
 
@@ -1320,18 +1461,21 @@ static hdr_status ensure(DevBuf &b, size_t bytes) {
 struct FramePlan {
     bool deferred = false;  // the deferred-shadow path (else the integration megakernel)
     bool lean_pt = false;   // the inlined path tracer (pathlean.h) instead of the megakernel
-    bool spatial = false;   // spatial reuse after the temporal pass (deferred path only)
+    bool spatial = false;   // spatial reuse after the temporal pass (deferred and wavefront paths)
     bool wave = false;      // the wavefront path (RIS at every vertex of deep paths, wavefront.h)
     bool wave_lean0 = false;  // its primary vertex cannot draw past RNG output 310 (lean stream there)
-    bool wave_traced = false; // BSDF-sampled candidates can see an emitter: traced in the shading launches
+    bool wave_traced = false; // BSDF-sampled candidates can see an emitter: bounce and escape rays per vertex
+    size_t escapes = 0;       // traced frames: escape-queue capacity (rays)
     int depth = 0;          // path vertices (max_depth)
     size_t lds = 0;         // dynamic LDS of the shading / integration kernel
     unsigned tiles = 0;     // 8x8 tiles of the view
 };
 
 // Validates a frame, makes sure the view's buffers for it exist and fills its FrameArgs.
+// shared_queues: a batch substitutes its own wavefront queues, so the view's are not allocated.
 static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
-                                const hdr_pipeline_desc *pipe, bool debug, FrameArgs &a, FramePlan &plan) {
+                                const hdr_pipeline_desc *pipe, bool debug, FrameArgs &a, FramePlan &plan,
+                                bool shared_queues = false) {
     if (!view || !scene || !frame || !pipe) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
     if (frame->struct_size != sizeof(hdr_frame_desc)) return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_frame_desc size");
     if (pipe->struct_size != sizeof(hdr_pipeline_desc))
@@ -1432,11 +1576,13 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
                           !a.trace_generated && !a.trace_last_connection && bound < kMtN - 1;
     // The wavefront path: the same conditions at any depth and RNG range (frames the deferred path
     // does not take).
-    const bool wave = !deferred && a.direct == HDR_DIRECT_RIS && a.skip_vis && !scene->any_translucent &&
-                      !scene->ds.has_mesh_lights;
     // With an environment or emissive materials, BSDF-sampled candidates can see an emitter: the
-    // wavefront's shading launches then trace the bounce and the generated candidates themselves.
-    const bool waveTraced = wave && (a.trace_generated || a.trace_last_connection);
+    // wavefront then emits each vertex's bounce ray and generated candidates' escape rays before
+    // its RIS (traced frames; at most 32 candidates, one bit each in the escape mask).
+    const bool needTraced = a.trace_generated || a.trace_last_connection;
+    const bool wave = !deferred && a.direct == HDR_DIRECT_RIS && a.skip_vis && !scene->any_translucent &&
+                      !scene->ds.has_mesh_lights && !(needTraced && a.n_cand > 32);
+    const bool waveTraced = wave && needTraced;
     // Spatial reuse (an extension) runs on the deferred and wavefront paths with temporal
     // reservoirs.
     const bool spatial = pipe->spatial_count > 0;
@@ -1468,21 +1614,34 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
     }
     if (wave) {
         if (hdr_status st = ensure(view->wstate, P * kWaveFields * sizeof(float4))) return st;
-        if (hdr_status st = ensure(view->whit, P * sizeof(GRec))) return st;
-        // the view's own queues (a batch substitutes its shared ones)
-        if (hdr_status st = ensure(view->wray, P * 2 * sizeof(float4))) return st;
-        if (hdr_status st = ensure(view->queue, P * sizeof(ShadowQuery))) return st;
-        if (hdr_status st = ensure(view->wcq, P * sizeof(float4))) return st;
-        if (hdr_status st = ensure(view->walive, 2 * P * sizeof(unsigned int))) return st;  // two parities
-        if (hdr_status st = ensure(view->wcnt, 8 * sizeof(unsigned int))) return st;
+        // traced frames: hit records of odd and even vertices, escape masks and the escape queue
+        if (hdr_status st = ensure(view->whit, (waveTraced ? 2 : 1) * P * sizeof(GRec))) return st;
+        if (waveTraced) {
+            plan.escapes = a.trace_generated ? P * size_t(a.n_cand - a.n_nee) : 0;
+            if (hdr_status st = ensure(view->wesc, P * sizeof(unsigned int))) return st;
+            a.whit2 = static_cast<GRec *>(view->whit.p) + P;
+            a.wesc = static_cast<unsigned int *>(view->wesc.p);
+        }
         a.wstate = static_cast<float4 *>(view->wstate.p);
         a.whit = static_cast<GRec *>(view->whit.p);
-        a.wray = static_cast<float4 *>(view->wray.p);
-        a.queue = static_cast<ShadowQuery *>(view->queue.p);
-        a.wcq = static_cast<float4 *>(view->wcq.p);
-        a.walive = static_cast<unsigned int *>(view->walive.p);
-        a.walive2 = a.walive + P;
-        a.wcnt = static_cast<unsigned int *>(view->wcnt.p);
+        if (!shared_queues) {  // the view's own queues
+            if (hdr_status st = ensure(view->wray, P * 2 * sizeof(float4))) return st;
+            if (hdr_status st = ensure(view->queue, P * sizeof(ShadowQuery))) return st;
+            if (hdr_status st = ensure(view->wcq, P * sizeof(float4))) return st;
+            if (hdr_status st = ensure(view->walive, 2 * P * sizeof(unsigned int))) return st;  // two parities
+            if (hdr_status st = ensure(view->wcnt, 8 * sizeof(unsigned int))) return st;
+            if (waveTraced) {
+                if (hdr_status st = ensure(view->wesq, std::max<size_t>(1, plan.escapes) * 2 * sizeof(float4)))
+                    return st;
+                a.wesq = static_cast<float4 *>(view->wesq.p);
+            }
+            a.wray = static_cast<float4 *>(view->wray.p);
+            a.queue = static_cast<ShadowQuery *>(view->queue.p);
+            a.wcq = static_cast<float4 *>(view->wcq.p);
+            a.walive = static_cast<unsigned int *>(view->walive.p);
+            a.walive2 = a.walive + P;
+            a.wcnt = static_cast<unsigned int *>(view->wcnt.p);
+        }
         if (spatial) {  // per pixel: emission + reused direct, visible / occluded; visibility + slot
             if (hdr_status st = ensure(view->rgb2, P * 2 * sizeof(float4))) return st;
             if (hdr_status st = ensure(view->flags, P * sizeof(uint32_t))) return st;
@@ -1603,7 +1762,10 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
             else HDR_W0(false, false);
         }
         HIP_TRY(hipGetLastError());
-        if (spatial) {  // inside the integration stage's span on this path
+        // spatial reuse after the launch that stores the post-temporal reservoirs: wave_shade0, or
+        // on traced frames the shading launch of vertex 0 (launch 1); inside the integration
+        // stage's span on this path
+        auto spatial_launch = [&]() -> hdr_status {
             HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_SPATIAL], view->stream));
             if (view->counting)
                 hipLaunchKernelGGL((wave_spatial_kernel<true>), grid, dim3(kLanes), 0, view->stream, da, dt, 1);
@@ -1612,10 +1774,11 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_SPATIAL], view->stream));
             ts.ran |= 1u << HDR_STAGE_SPATIAL;
-        }
-        HIP_TRY(hipEventRecord(view->shade_done, view->stream));  // the G-buffer has been read
-        view->shade_recorded = true;
-        for (int b = 0; b < plan.depth; ++b) {
+            return HDR_OK;
+        };
+        // traced frames: shading launch v finishes vertex v-1, so there is one more launch pair
+        const int launches = plan.wave_traced ? plan.depth + 1 : plan.depth;
+        for (int b = 0; b < launches; ++b) {
             if (b > 0) {
                 if (view->counting) {
                     if (debug_out) HDR_WS(true, true);
@@ -1624,6 +1787,13 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
                     if (debug_out) HDR_WS(false, true);
                     else HDR_WS(false, false);
                 }
+                HIP_TRY(hipGetLastError());
+            }
+            if (spatial && b == (plan.wave_traced ? 1 : 0))
+                if (hdr_status st = spatial_launch()) return st;
+            if (b == (plan.wave_traced ? 1 : 0)) {  // the G-buffer (vertex 0's hits) has been read
+                HIP_TRY(hipEventRecord(view->shade_done, view->stream));
+                view->shade_recorded = true;
             }
             if (view->counting) launch_wave_trace<true>(view->shape, qblocks, view->stream, da, b, plan.wave_traced);
             else launch_wave_trace<false>(view->shape, qblocks, view->stream, da, b, plan.wave_traced);
@@ -1850,6 +2020,7 @@ struct hdr_batch {
     bool timed = false;
     DevBuf queue, qcount;  // shadow queries of every view of a step, and their number
     DevBuf wray, wqueue, wcq, walive, wcnt;  // wavefront-path queues of a step (shared by its views)
+    DevBuf wesq;                             // and the traced frames' escape-ray queue
     // steps in flight: arguments and tile tables staged in pinned memory, copied to the slot's
     // device buffer; a slot is reused once its step has finished
     static constexpr int kSlots = 4;
@@ -1982,25 +2153,28 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
     HIP_TRY(hipSetDevice(b->ctx->device));
     std::vector<FrameArgs> args(static_cast<size_t>(n));
     std::vector<FramePlan> plans(static_cast<size_t>(n));
-    size_t queueCap = 0, waveCap = 0;
+    size_t queueCap = 0, waveCap = 0, escCap = 0;
     // wavefront views in two groups (bounce rays traced by the traversal launch / by the shading
     // launches), each run as its own launch sequence over the shared queues
     int waveDepth[2] = {0, 0};
     bool waveLean0[2] = {true, true};  // every view's primary vertex stays below RNG output 311
     bool waveSpatial[2] = {false, false};  // some view of the group reuses spatially
     for (int i = 0; i < n; ++i) {
-        if (hdr_status st = prepare_frame(views[i], scenes[i], &frames[i], &pipes[i], false, args[i], plans[i]))
+        if (hdr_status st =
+                prepare_frame(views[i], scenes[i], &frames[i], &pipes[i], false, args[i], plans[i], true))
             return st;
         if (plans[i].deferred) queueCap += size_t(views[i]->W) * views[i]->H;
         if (plans[i].wave) {
             const int t = plans[i].wave_traced ? 1 : 0;
             waveCap += size_t(views[i]->W) * views[i]->H;
+            escCap += plans[i].escapes;
             waveDepth[t] = std::max(waveDepth[t], plans[i].depth);
             waveLean0[t] = waveLean0[t] && plans[i].wave_lean0;
             waveSpatial[t] = waveSpatial[t] || plans[i].spatial;
         }
     }
-    if (queueCap * sizeof(ShadowQuery) > b->queue.n || waveCap * sizeof(ShadowQuery) > b->wqueue.n) {
+    if (queueCap * sizeof(ShadowQuery) > b->queue.n || waveCap * sizeof(ShadowQuery) > b->wqueue.n ||
+        escCap * 2 * sizeof(float4) > b->wesq.n) {
         HIP_TRY(hipStreamSynchronize(b->stream));  // earlier steps may still read the old queues
         if (hdr_status st = ensure(b->queue, queueCap * sizeof(ShadowQuery))) return st;
         if (waveCap > 0) {
@@ -2009,6 +2183,8 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
             if (hdr_status st = ensure(b->wcq, waveCap * sizeof(float4))) return st;
             if (hdr_status st = ensure(b->walive, 2 * waveCap * sizeof(unsigned int))) return st;
         }
+        if (escCap > 0)
+            if (hdr_status st = ensure(b->wesq, escCap * 2 * sizeof(float4))) return st;
     }
     for (int i = 0; i < n; ++i) {
         args[i].view = i;
@@ -2020,6 +2196,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
             args[i].walive = static_cast<unsigned int *>(b->walive.p);
             args[i].walive2 = args[i].walive + waveCap;
             args[i].wcnt = static_cast<unsigned int *>(b->wcnt.p);
+            args[i].wesq = static_cast<float4 *>(b->wesq.p);
         } else {
             args[i].queue = static_cast<ShadowQuery *>(b->queue.p);
         }
@@ -2111,11 +2288,14 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
             hipLaunchKernelGGL(wave_zero_counts_kernel, dim3(1), dim3(kLanes), 0, b->stream,
                                static_cast<unsigned int *>(b->wcnt.p));
             launch_wave_shade0<false, false>(waveLean0[t], t == 1, tiles[wg], lds[wg], b->stream, A(wg), T(wg), N(wg));
-            if (waveSpatial[t])
-                hipLaunchKernelGGL((wave_spatial_kernel<false>), dim3(tiles[wg]), dim3(kLanes), 0, b->stream, A(wg),
-                                   T(wg), N(wg));
-            for (int v = 0; v < waveDepth[t]; ++v) {
+            // traced frames: shading launch v finishes vertex v-1 (one more launch pair), and the
+            // post-temporal reservoirs are stored by launch 1
+            const int launches = t == 1 ? waveDepth[t] + 1 : waveDepth[t];
+            for (int v = 0; v < launches; ++v) {
                 if (v > 0) launch_wave_shade<false, false>(ls, qblocks, lds[wg], b->stream, A(wg), v, t == 1);
+                if (waveSpatial[t] && v == t)
+                    hipLaunchKernelGGL((wave_spatial_kernel<false>), dim3(tiles[wg]), dim3(kLanes), 0, b->stream,
+                                       A(wg), T(wg), N(wg));
                 launch_wave_trace<false>(ls, qblocks, b->stream, A(wg), v, t == 1);
             }
             hipLaunchKernelGGL((wave_resolve_kernel<false, false>), dim3(tiles[wg]), dim3(kLanes), 0, b->stream,
@@ -2134,6 +2314,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         std::vector<uint64_t> key = {uint64_t(n), uint64_t(off), uint64_t(uintptr_t(sl.dev.p)),
                                      uint64_t(uintptr_t(sl.host.p)), uint64_t(uintptr_t(b->queue.p)),
                                      uint64_t(uintptr_t(b->wqueue.p)), uint64_t(uintptr_t(b->wray.p)),
+                                     uint64_t(uintptr_t(b->wesq.p)), uint64_t(escCap),
                                      uint64_t(uintptr_t(b->wcq.p)), uint64_t(uintptr_t(b->walive.p)),
                                      uint64_t(queueCap), uint64_t(waveCap), uint64_t(waveDepth[0]),
                                      uint64_t(waveDepth[1]), uint64_t(waveLean0[0]), uint64_t(waveLean0[1]),

@@ -57,12 +57,15 @@ struct FrameArgs {
     // wavefront path (paths of any depth, one launch per vertex): per-view path records and the
     // bounce rays' hits; queues shared by the views of a launch
     float4 *wstate;                    // kWaveFields float4 per pixel, field-major
-    GRec *whit;                        // hit of each path's last bounce ray
+    GRec *whit;                        // hit of each path's last bounce ray (traced frames: of odd vertices)
+    GRec *whit2;                       // traced frames: hit of the vertex, even vertices >= 2
     float4 *wray;                      // bounce-ray queue: {origin, code}, {direction, -}
     float4 *wcq;                       // contribution of each shadow query (index-aligned with queue)
+    float4 *wesq;                      // traced frames: escape-ray queue {origin, code}, {direction, k}
+    unsigned int *wesc;                // traced frames: per pixel, bit k = candidate k's ray escaped
     unsigned int *walive;              // codes of the live paths of even vertices (bounce ray hit)
     unsigned int *walive2;             // and of odd vertices
-    unsigned int *wcnt;                // queue counts: rays[2], shadows[2], alive[2] by vertex parity
+    unsigned int *wcnt;                // queue counts by vertex parity: rays[2], shadows[2], alive[2], escapes[2]
 };
 
 struct PixelStats {

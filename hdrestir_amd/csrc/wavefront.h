@@ -100,9 +100,11 @@ __device__ __forceinline__ void store_path(const FrameArgs &a, int pixel, const 
         if (p.rng.k > 467u) w[8 * P + pixel] = pack_u64x2(p.rng.b155, p.rng.b311);  // set once pass A ends
 }
 
-__device__ __forceinline__ void load_path(const FrameArgs &a, int pixel, WavePath<Mt64Deep> &p) {
+// outer: also the outer throughput (field 6), for a launch that shades the primary vertex.
+__device__ __forceinline__ void load_path(const FrameArgs &a, int pixel, WavePath<Mt64Deep> &p, bool outer = false) {
     const float4 *w = a.wstate;
     const size_t P = wave_pixels(a);
+    if (outer) p.outer = spec_of(w[6 * P + pixel]);
     const float4 r0 = w[pixel], r1 = w[P + pixel], r2 = w[2 * P + pixel];
     p.rng.lo = unpack_lo(r0);
     p.rng.lo1 = unpack_hi(r0);
@@ -133,6 +135,11 @@ __device__ __forceinline__ void load_path(const FrameArgs &a, int pixel, WavePat
     }
 }
 
+// Traced frames: hit records of vertex v >= 1, double-buffered by parity (the shading launch of
+// vertex v reads vertex v's and vertex v+1's records; the traversal launch after it writes vertex
+// v+2's into the first buffer). Vertex 0's record is the G-buffer's.
+__device__ __forceinline__ GRec *traced_hits(const FrameArgs &a, int v) { return (v & 1) ? a.whit : a.whit2; }
+
 // ---------------------------------------------------------------- one path vertex
 // The body of one iteration of _li's bounce loop (path_integrator.cpp:112-177) at vertex `bounce`
 // (0 = the primary hit, whose throughput is the outer one and whose direct term is not scaled by
@@ -141,9 +148,11 @@ __device__ __forceinline__ void load_path(const FrameArgs &a, int pixel, WavePat
 // shadow query with its contribution `tp * d` (added to the total if the ray is unoccluded) and the
 // bounce ray when the path goes on.
 // kTraced: frames whose BSDF-sampled candidates can see an emitter (environment, emissive material;
-// no mesh light): the bounce ray is traced here, before the RIS candidates (the connection candidate
-// needs its hit), generated candidates are traced by ris_direct_traced, and the bounce hit -- the
-// next vertex -- is returned in `next` (ray = a next vertex exists) instead of a bounce ray.
+// no mesh light): the vertex's ray queries were emitted by wave_gen and answered by a traversal
+// launch -- the bounce ray's hit (the connection candidate's hit and the next vertex) is read from
+// the hit buffer of vertex bounce + 1, the generated candidates' escape tests from the pixel's
+// escape mask -- and the next vertex's hit is returned in `next` (ray = a next vertex exists)
+// instead of a bounce ray.
 template <bool kCount, bool kDebug, bool kTraced, class R>
 __device__ __forceinline__ void wave_vertex(const FrameArgs &a, const SpecTab &tab, int bounce, const HitRec &h,
                                             WavePath<R> &p, double *wl, int pixel, PixelStats &st, bool &query,
@@ -167,11 +176,8 @@ __device__ __forceinline__ void wave_vertex(const FrameArgs &a, const SpecTab &t
         if (ok) {
             if (last && !a.trace_last_connection) {
                 if (kCount) ++st.cand_skipped;
-            } else {
-                if (kCount) ++st.bounce;
-                RawHit rh;
-                cHit = trace_closest<kCount>(s, b.o, b.d, rh, st.ts);
-                if (cHit) next = make_hit_record(s, b.o, b.d, rh);
+            } else {  // traced (and counted) by the traversal launch
+                cHit = load_grec(traced_hits(a, bounce + 1) + pixel, next);
             }
         }
     } else if (kCount && ok && last) {
@@ -184,7 +190,8 @@ __device__ __forceinline__ void wave_vertex(const FrameArgs &a, const SpecTab &t
     Spec d;
     if (kTraced)
         d = ris_direct_traced<kCount, kDebug>(a, tab, v, ok ? &b : nullptr, cHit, next, p.rng, wl, pixel, st, p.dbg,
-                                              first && a.use_res, slot ? slot + kMtN : nullptr, qq, qo, qd, qtmax);
+                                              first && a.use_res, slot ? slot + kMtN : nullptr,
+                                              a.trace_generated ? a.wesc[pixel] : 0u, qq, qo, qd, qtmax);
     else
         d = ris_direct_lean<kCount, kDebug>(a, tab, v, ok && b.pdf > 0.0f, p.rng, wl, pixel, st, p.dbg,
                                             first && a.use_res, slot ? slot + kMtN : nullptr, qq, qo, qd, qtmax);
@@ -216,6 +223,47 @@ __device__ __forceinline__ void wave_vertex(const FrameArgs &a, const SpecTab &t
     rd = make_float4(b.d.x, b.d.y, b.d.z, 0.0f);
 }
 
+// ---------------------------------------------------------------- traced frames: ray queries
+// Before wave_vertex runs vertex `bounce` of a traced frame, wave_gen draws what it will draw up to
+// its generated candidates -- the bounce sample and the NEE candidates' draws -- on a copy of the
+// path's stream (the record keeps the position before the vertex) and emits the rays whose
+// answers the vertex needs: the bounce ray (closest hit; not at the last vertex unless the last
+// connection is traced) and, through wave_emit_escapes, each sampled generated candidate's escape
+// ray (any hit up to 1e30). r and vx are left at the first generated candidate k0 (k0 = n_cand when
+// nothing generated can see an emitter). Clears the pixel's escape mask.
+template <class R>
+__device__ __forceinline__ void wave_gen(const FrameArgs &a, const SpecTab &tab, int bounce, const HitRec &h,
+                                         const WavePath<R> &p, int pixel, R &r, Vertex &vx, bool &bray, float4 &bo,
+                                         float4 &bd, int &k0) {
+    const DeviceScene &s = a.s;
+    const Clo c = get_closure(s, h, a.enable_ss != 0);
+    Spec thr = bounce == 0 ? p.outer : p.tp;
+    vx = make_vertex(a, tab, h, c, p.dir, thr);
+    int refl = p.refl, refr = p.refr;
+    Bounce b;
+    const bool ok = sample_bounce_inl(tab, vx.c, vx.sn, vx.inside, vx.h.pos, vx.rayDir, a.max_refl, a.max_refr, refl,
+                                      refr, r, b);
+    const bool last = bounce == a.max_depth - 1;
+    bray = ok && !(last && !a.trace_last_connection);
+    const uint32_t code = query_code(a, pixel);
+    if (bray) {
+        bo = make_float4(b.o.x, b.o.y, b.o.z, __uint_as_float(code));
+        bd = make_float4(b.d.x, b.d.y, b.d.z, 0.0f);
+    }
+    k0 = a.n_cand;
+    if (!a.trace_generated) return;
+    a.wesc[pixel] = 0u;
+    if (s.num_lights > 0) {  // the NEE candidates' draws (ris_direct_traced's order)
+        Trace none;
+        for (int k = 0; k < a.n_nee; ++k) {
+            const int li = pick_light_lean<false>(a, r, none, false);
+            const int nd = light_draws(s, s.lights[li]);
+            for (int q = 0; q < nd; ++q) r.discard();
+        }
+    }
+    k0 = a.n_nee + a.n_sky + ((ok && b.pdf > 0.0f) ? 1 : 0);  // after the connection candidate
+}
+
 // Wave-aggregated append: one atomic per wave, each flagged lane gets its slot. All 64 lanes of
 // the wave must call it together.
 __device__ __forceinline__ unsigned int wave_append(unsigned int *count, bool want, int lane) {
@@ -227,6 +275,32 @@ __device__ __forceinline__ unsigned int wave_append(unsigned int *count, bool wa
         base = __shfl(base, 0);
     }
     return base + unsigned(__popcll(mask & ((1ULL << lane) - 1ULL)));
+}
+
+// Escape rays of this lane's generated candidates k0 .. n_cand-1 (wave_gen's state), appended to
+// the escape queue {origin, code}, {direction, k}. All 64 lanes of the wave call it together
+// (a lane with nothing to emit passes k0 >= n); it loops until every lane is done.
+template <class R>
+__device__ __forceinline__ void wave_emit_escapes(unsigned int *count, float4 *queue, const SpecTab &tab,
+                                                  const Vertex &vx, R &r, int k0, int n, uint32_t code, int lane) {
+    int k = k0;
+    for (;;) {
+        const bool live = k < n;
+        if (__ballot(live) == 0ULL) break;
+        bool want = false;
+        Bounce b;
+        const int kc = k;
+        if (live) {
+            int refl = 0, refr = 0;
+            want = sample_bounce_inl(tab, vx.c, vx.sn, vx.inside, vx.h.pos, vx.rayDir, 1, 1, refl, refr, r, b);
+            ++k;
+        }
+        const unsigned int qi = wave_append(count, want, lane);
+        if (want) {
+            queue[2 * qi] = make_float4(b.o.x, b.o.y, b.o.z, __uint_as_float(code));
+            queue[2 * qi + 1] = make_float4(b.d.x, b.d.y, b.d.z, __int_as_float(kc));
+        }
+    }
 }
 
 }  // namespace hdr
