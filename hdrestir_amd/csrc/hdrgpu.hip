@@ -1076,11 +1076,11 @@ struct hdr_scene {
 // waves per launch and the minimum-waves-per-SIMD variant of the traversal / shading kernels.
 struct LaunchShape {
     int queue_waves = 0;  // 0: one wave per 64 queue slots, at most kDefaultQueueWaves
-    int trace_occ = 1;
-    int shade_occ = 1;
+    int trace_occ = 8;    // at least 8 waves per SIMD (1: the compiler's register budget)
+    int shade_occ = 3;
     bool lds_stack = false;  // traversal stacks with LDS rings (RingStack) instead of private memory
 };
-constexpr int kDefaultQueueWaves = 8192;
+constexpr int kDefaultQueueWaves = 32768;
 static int queue_blocks(const LaunchShape &ls, size_t cap) {
     const size_t need = std::max<size_t>(1, (cap + kLanes - 1) / kLanes);
     return int(std::min<size_t>(need, size_t(ls.queue_waves > 0 ? ls.queue_waves : kDefaultQueueWaves)));
@@ -2093,10 +2093,10 @@ static hdr_status shape_of(const hdr_launch_config *c, LaunchShape &ls) {
     if (!c || c->struct_size != sizeof(hdr_launch_config)) return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_launch_config size");
     if (c->queue_waves != 0 && (c->queue_waves < 64 || c->queue_waves > 65536))
         return fail(HDR_ERR_INVALID_ARGUMENT, "queue_waves must be 0 or in [64, 65536]");
-    const int t = c->trace_occupancy == 0 ? 1 : c->trace_occupancy;
+    const int t = c->trace_occupancy == 0 ? 8 : c->trace_occupancy;
     if (t != 1 && t != 4 && t != 6 && t != 8)
         return fail(HDR_ERR_INVALID_ARGUMENT, "trace_occupancy must be 0, 1, 4, 6 or 8");
-    const int sh = c->shade_occupancy == 0 ? 1 : c->shade_occupancy;
+    const int sh = c->shade_occupancy == 0 ? 3 : c->shade_occupancy;
     if (sh != 1 && sh != 3) return fail(HDR_ERR_INVALID_ARGUMENT, "shade_occupancy must be 0, 1 or 3");
     if (c->lds_stack != 0 && c->lds_stack != 1) return fail(HDR_ERR_INVALID_ARGUMENT, "lds_stack must be 0 or 1");
     ls.queue_waves = c->queue_waves;
@@ -2406,14 +2406,14 @@ hdr_status hdr_batch_autotune(hdr_batch *b, int32_t n, hdr_view *const *views, c
         if (hdr_status st = time_shape(c, ms)) return st;
         if (ms < bestMs) bestMs = ms, best = c;
     }
-    for (int q : {2048, 4096, 16384, 32768}) {
+    for (int q : {4096, 8192, 16384, 65536}) {
         LaunchShape c = best;
         c.queue_waves = q;
         float ms;
         if (hdr_status st = time_shape(c, ms)) return st;
         if (ms < bestMs) bestMs = ms, best = c;
     }
-    for (int t : {4, 6, 8}) {
+    for (int t : {1, 4, 6, 8}) {
         LaunchShape c = best;
         c.trace_occ = t;
         float ms;
@@ -2422,7 +2422,7 @@ hdr_status hdr_batch_autotune(hdr_batch *b, int32_t n, hdr_view *const *views, c
     }
     {
         LaunchShape c = best;
-        c.shade_occ = 3;
+        c.shade_occ = c.shade_occ == 3 ? 1 : 3;
         float ms;
         if (hdr_status st = time_shape(c, ms)) return st;
         if (ms < bestMs) bestMs = ms, best = c;

@@ -343,11 +343,12 @@ hdr_status hdr_batch_last_timing(hdr_batch *batch, float *step_ms);
  * replayed (use_graphs, default 1). */
 typedef struct hdr_launch_config {
     uint32_t struct_size;       /* sizeof(hdr_launch_config) */
-    int32_t  queue_waves;       /* waves per queue-driven launch: 0 = one per 64 queue slots, at most 8192;
-                                   else 64 .. 65536 */
-    int32_t  trace_occupancy;   /* traversal kernels: 0 or 1 = the compiler's register budget, 4, 6, 8 = at
-                                   least that many waves per SIMD */
-    int32_t  shade_occupancy;   /* wavefront shading kernel: 0 or 1, or 3 */
+    int32_t  queue_waves;       /* waves per queue-driven launch: 0 = one per 64 queue slots, at most
+                                   32768; else 64 .. 65536 */
+    int32_t  trace_occupancy;   /* traversal kernels: 4, 6, 8 = at least that many waves per SIMD, 1 = the
+                                   compiler's register budget; 0 = 8 */
+    int32_t  shade_occupancy;   /* wavefront shading kernels: 3 = at least 3 waves per SIMD, 1 = the
+                                   compiler's register budget; 0 = 3 */
     int32_t  use_graphs;        /* batches: 1 = replay captured graphs, 0 = launch every kernel */
     int32_t  lds_stack;         /* traversal kernels (G-buffer, shadow, wavefront traversal): 1 = the top
                                    4 TLAS / 8 BLAS stack entries of each lane in LDS, 0 = private memory */

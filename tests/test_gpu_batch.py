@@ -109,7 +109,8 @@ def test_graphs_launch_shapes_and_autotune_keep_results(gpu_ctx):
     configs = [abi.launch_config(use_graphs=0),
                abi.launch_config(**chosen),
                abi.launch_config(queue_waves=64, trace_occupancy=8, shade_occupancy=3, use_graphs=1),
-               abi.launch_config(queue_waves=65536, trace_occupancy=4, use_graphs=1)]
+               abi.launch_config(queue_waves=65536, trace_occupancy=4, use_graphs=1),
+               abi.launch_config(trace_occupancy=1, shade_occupancy=1, use_graphs=1)]
     results = []
     for cfg in configs:
         vs = views()
