@@ -490,6 +490,210 @@ __device__ __forceinline__ bool trace_closest_s(const DeviceScene &s, V3 o, V3 d
     }
     return was;
 }
+// ---------------------------------------------------------------- wave-coherent closest hit
+// The 64 rays of one wave (the primary rays of an 8x8 tile, coherent) traverse together: the wave
+// walks the union of the nodes its rays visit, with one stack of {node record, lane mask} per level
+// in LDS, and loads every node, triangle and instance once for all lanes, from uniform addresses
+// (scalar loads). Each lane keeps its own decisions: the box test and the prune against its own
+// best t at every node, again when an entry is popped (the box is tested again: same operands,
+// so the same entry distance as at the push), and the tie rule of trace_closest_s in the leaves.
+// So each lane tests the triangles it would test alone and returns the same hit; only the visiting
+// order differs (the child that more lanes meet first goes first), and the tie rule makes the
+// winner independent of the order. All 64 lanes of the wave call it together; lanes without a ray
+// pass act = false.
+struct WaveStack {
+    int rec[kStackDepth];                  // node record index (the box to test again at the pop)
+    unsigned long long mask[kStackDepth];  // lanes that entered it
+};
+
+__device__ __forceinline__ float4 sload4(const void *p) {  // uniform address: a scalar load
+    const GF4 v = *(const __attribute__((address_space(4))) GF4 *)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ GNode load_node_u(const GNode *nodes, int i) {
+    const float4 *p = reinterpret_cast<const float4 *>(nodes + i);
+    const float4 a = sload4(p), b = sload4(p + 1);
+    GNode n;
+    n.mnx = a.x;
+    n.mny = a.y;
+    n.mnz = a.z;
+    n.left = __float_as_int(a.w);
+    n.mxx = b.x;
+    n.mxy = b.y;
+    n.mxz = b.z;
+    n.count = __float_as_int(b.w);
+    return n;
+}
+__device__ __forceinline__ GTri load_tri_u(const GTri *tris, int i) {
+    const float4 *p = reinterpret_cast<const float4 *>(tris + i);
+    const float4 a = sload4(p), b = sload4(p + 1), c = sload4(p + 2);
+    GTri t;
+    t.v0x = a.x;
+    t.v0y = a.y;
+    t.v0z = a.z;
+    t.orig = __float_as_int(a.w);
+    t.v1x = b.x;
+    t.v1y = b.y;
+    t.v1z = b.z;
+    t._p1 = __float_as_int(b.w);
+    t.v2x = c.x;
+    t.v2y = c.y;
+    t.v2z = c.z;
+    t._p2 = __float_as_int(c.w);
+    return t;
+}
+__device__ __forceinline__ int load_inst_inv_u(const GInstance *insts, int ii, float m[16]) {
+    const float4 *p = reinterpret_cast<const float4 *>(insts + ii);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 v = sload4(p + q);
+        m[4 * q] = v.x;
+        m[4 * q + 1] = v.y;
+        m[4 * q + 2] = v.z;
+        m[4 * q + 3] = v.w;
+    }
+    return __float_as_int(sload4(p + 8).x);
+}
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
+    const unsigned lo = unsigned(__builtin_amdgcn_readfirstlane(int(unsigned(v)))),
+                   hi = unsigned(__builtin_amdgcn_readfirstlane(int(unsigned(v >> 32))));
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+// Pops the top entries of a wave stack until one still holds a lane whose box test passes against
+// its current best t; returns false when the stack empties.
+__device__ __forceinline__ bool wave_pop(const DeviceScene &s, WaveStack &S, int &sp, const RayBox &R, float best,
+                                         int lane, int &cur, unsigned long long &cm) {
+    while (sp > 0) {
+        --sp;
+        const int rec = uniform_i(S.rec[sp]);
+        const unsigned long long m = uniform_u64(S.mask[sp]);
+        const GNode n = load_node_u(s.nodes, rec);
+        float ta;
+        const bool h = ((m >> lane) & 1ULL) && box_hit<true>(R, n, ta) && !(ta > best);
+        const unsigned long long mm = __ballot(h);
+        if (mm != 0ULL) {
+            cur = node_ref(n);
+            cm = mm;
+            return true;
+        }
+    }
+    return false;
+}
+__device__ __forceinline__ void wave_push(WaveStack &S, int &sp, int rec, unsigned long long m) {
+    S.rec[sp] = rec;  // every lane writes the same value
+    S.mask[sp] = m;
+    ++sp;
+}
+
+__device__ __forceinline__ bool trace_closest_wave(const DeviceScene &s, bool act, V3 o, V3 d, RawHit &hit,
+                                                   WaveStack &tstk, WaveStack &bstk) {
+    const int lane = int(threadIdx.x & 63u);
+    hit.t = 1e30f;
+    hit.inst = -1;
+    hit.tri = -1;
+    hit.u = hit.v = 0.0f;
+    if (s.tlas_root < 0) return false;
+    RayBox R = make_raybox<true>(o, d);
+    int tsp = 0, bsp = 0;
+    unsigned long long cm;  // lanes in the current node
+    int cur;
+    {
+        const GNode n = load_node_u(s.nodes, s.tlas_root);
+        float ta;
+        const bool h = act && box_hit<true>(R, n, ta) && !(ta > hit.t);
+        cm = __ballot(h);
+        if (cm == 0ULL) return false;
+        cur = node_ref(n);
+    }
+    bool was = false, blas = false, world = true;
+    int lnext = 0, lend = 0;    // instance slots of the current TLAS leaf still to traverse
+    unsigned long long lm = 0;  // and the lanes that reached that leaf
+    int ii = -1, slot = -1;
+    int bslot = 0x7fffffff;
+    V3 oo = o, od = d;
+    for (;;) {
+        if (cur >= 0) {  // inner node of either level
+            const GNode l = load_node_u(s.nodes, cur), r = load_node_u(s.nodes, cur + 1);
+            const bool in = (cm >> lane) & 1ULL;
+            float tl = 0.0f, tr = 0.0f;
+            const bool hl = in && box_hit<true>(R, l, tl) && !(tl > hit.t);
+            const bool hr = in && box_hit<true>(R, r, tr) && !(tr > hit.t);
+            const unsigned long long ml = __ballot(hl), mr = __ballot(hr);
+            if ((ml | mr) != 0ULL) {
+                const unsigned long long mf = __ballot(hl && (!hr || !(tr < tl)));
+                const bool leftFirst = ml != 0ULL && (mr == 0ULL || 2 * __popcll(mf) >= __popcll(ml | mr));
+                if (ml != 0ULL && mr != 0ULL) {
+                    if (blas) wave_push(bstk, bsp, leftFirst ? cur + 1 : cur, leftFirst ? mr : ml);
+                    else wave_push(tstk, tsp, leftFirst ? cur + 1 : cur, leftFirst ? mr : ml);
+                }
+                cur = node_ref(leftFirst ? l : r);
+                cm = leftFirst ? ml : mr;
+                continue;
+            }
+        } else if (cur != kNoNode) {  // leaf
+            const int code = -cur - 1;
+            const int start = code >> 3, count = code & 7;
+            if (blas) {
+                const bool in = (cm >> lane) & 1ULL;
+                for (int q = 0; q < count; ++q) {
+                    const GTri tri = load_tri_u(s.tris, start + q);
+                    float tt, uu, vv;
+                    if (in && tri_hit(tri, oo, od, tt, uu, vv) &&
+                        (tt < hit.t || (tt == hit.t && (slot < bslot || (slot == bslot && start + q < hit.tri))))) {
+                        hit.t = tt;
+                        hit.inst = ii;
+                        hit.tri = start + q;
+                        hit.u = uu;
+                        hit.v = vv;
+                        bslot = slot;
+                        was = true;
+                    }
+                }
+            } else {
+                lnext = start;
+                lend = start + count;
+                lm = cm;
+            }
+        }
+        cur = kNoNode;
+        if (blas) {
+            if (wave_pop(s, bstk, bsp, R, hit.t, lane, cur, cm)) continue;
+            blas = false;  // the instance is done
+        }
+        while (lnext < lend) {  // the leaf's next instance
+            slot = lnext++;
+            ii = *(const __attribute__((address_space(4))) int *)(s.tlas_idx + slot);
+            float minv[16];
+            const int root = load_inst_inv_u(s.instances, ii, minv);
+            if (root < 0) continue;
+            oo = xform_point(minv, o);
+            od = xform_dir(minv, d);
+            R = make_raybox<false>(oo, od);
+            world = false;
+            bsp = 0;
+            const GNode b = load_node_u(s.nodes, root);
+            float ta;
+            const bool h = ((lm >> lane) & 1ULL) && box_hit<true>(R, b, ta) && !(ta > hit.t);
+            const unsigned long long m = __ballot(h);
+            if (m != 0ULL) {
+                cur = node_ref(b);
+                cm = m;
+                blas = true;
+                break;
+            }
+        }
+        if (blas) continue;
+        if (!world) {
+            R = make_raybox<true>(o, d);
+            world = true;
+        }
+        if (!wave_pop(s, tstk, tsp, R, hit.t, lane, cur, cm)) break;
+    }
+    return was;
+}
+
 template <bool kCount>
 __device__ __forceinline__ bool trace_closest(const DeviceScene &s, V3 o, V3 d, RawHit &hit, TraceStats &st) {
     PrivStack<true> t, b;

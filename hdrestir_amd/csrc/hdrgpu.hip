@@ -61,11 +61,10 @@ __device__ __forceinline__ bool any_hit(const DeviceScene &s, V3 o, V3 d, float 
     return trace_any<kCount>(s, o, d, tmax, ts);
 }
 
-template <bool kCount, bool kLds = false>
-__device__ __forceinline__ void gbuffer_pixel(const FrameArgs &a, GRec *gbuf, int x, int y, TraceStats &ts, int *tr,
-                                              float *tt, int *br, float *bt) {
+// The G-buffer ray of a pixel (RaycastPass: pixel centre; the camera's lens draws when it has
+// camera parameters).
+__device__ __forceinline__ void gbuffer_ray(const FrameArgs &a, int x, int y, V3 &co, V3 &cd) {
     const int pixel = y * a.W + x;
-    V3 co, cd;
     if (a.has_cam) {
         const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
         Mt64 crng;
@@ -74,9 +73,11 @@ __device__ __forceinline__ void gbuffer_pixel(const FrameArgs &a, GRec *gbuf, in
     } else {
         camera_ray(a, float(x) + 0.5f, float(y) + 0.5f, static_cast<Mt64 *>(nullptr), co, cd);
     }
-    RawHit rh;
+}
+// The hit record and AOVs of a pixel.
+__device__ __forceinline__ void gbuffer_store(const FrameArgs &a, GRec *gbuf, int pixel, V3 co, V3 cd, bool hit,
+                                              const RawHit &rh) {
     HitRec g = {};
-    const bool hit = closest_hit<kCount, kLds>(a.s, co, cd, rh, ts, tr, tt, br, bt);
     if (hit) g = make_hit_record(a.s, co, cd, rh);
     store_grec(gbuf + pixel, hit, g);
     if (a.aov_depth) {
@@ -90,6 +91,15 @@ __device__ __forceinline__ void gbuffer_pixel(const FrameArgs &a, GRec *gbuf, in
         a.aov_normal[3 * pixel + 1] = nn.y;
         a.aov_normal[3 * pixel + 2] = nn.z;
     }
+}
+template <bool kCount, bool kLds = false>
+__device__ __forceinline__ void gbuffer_pixel(const FrameArgs &a, GRec *gbuf, int x, int y, TraceStats &ts, int *tr,
+                                              float *tt, int *br, float *bt) {
+    V3 co, cd;
+    gbuffer_ray(a, x, y, co, cd);
+    RawHit rh;
+    const bool hit = closest_hit<kCount, kLds>(a.s, co, cd, rh, ts, tr, tt, br, bt);
+    gbuffer_store(a, gbuf, y * a.W + x, co, cd, hit, rh);
 }
 
 // Launch layout: one wave64 per 8x8 pixel tile; the blocks of a launch enumerate the tiles of
@@ -121,8 +131,18 @@ __global__ void __launch_bounds__(kLanes) gbuffer_kernel(const FrameArgs *__rest
     HDR_RING_LDS
     HDR_TILE(args, tile_base, n_views)
     TraceStats ts = {};
-    if (valid) gbuffer_pixel<kCount, kLds>(a, a.gbuf, x, y, ts, ring_tr + lane, ring_tt + lane, ring_br + lane,
-                                           ring_bt + lane);
+    if (kCount || kLds) {  // per-lane traversal (the counting pass counts each ray's own work)
+        if (valid)
+            gbuffer_pixel<kCount, kLds>(a, a.gbuf, x, y, ts, ring_tr + lane, ring_tt + lane, ring_br + lane,
+                                        ring_bt + lane);
+    } else {  // the tile's 64 primary rays traverse together
+        __shared__ WaveStack wt, wb;
+        V3 co = v3s(0.0f), cd = v3(0.0f, 0.0f, 1.0f);
+        if (valid) gbuffer_ray(a, x, y, co, cd);
+        RawHit rh;
+        const bool hit = trace_closest_wave(a.s, valid, co, cd, rh, wt, wb);
+        if (valid) gbuffer_store(a, a.gbuf, y * a.W + x, co, cd, hit, rh);
+    }
     if (kCount) {
         unsigned long long *c = a.counters;
         wave_add(c + 0, valid ? 1ULL : 0ULL);
@@ -1168,12 +1188,14 @@ struct hdr_view {
     LaunchShape shape;           // queue-driven launches (hdr_view_set_launch_config)
     int counting = 0;
     int frames = 0;
-    // Frames overlap: the G-buffer kernel of frame N+1 runs on `gstream` as soon as frame N's
-    // shading has read the G-buffer (event `shade_done`), beside frame N's shadow rays and
-    // accumulation on `stream`; frame N+1's shading waits for its G-buffer (event `gbuf_done`).
+    // Frames overlap: the G-buffer is double-buffered by frame parity, and the G-buffer kernel of
+    // frame N+1 runs on `gstream` as soon as frame N-1's shading has read that buffer (event
+    // `shade_done[parity]`), beside frame N's shading, shadow rays and accumulation on `stream`;
+    // frame N+1's shading waits for its G-buffer (event `gbuf_done`).
     hipStream_t gstream = nullptr;
-    hipEvent_t shade_done = nullptr, gbuf_done = nullptr, serial = nullptr;
-    bool shade_recorded = false;
+    hipEvent_t shade_done[2] = {nullptr, nullptr}, gbuf_done = nullptr, serial = nullptr;
+    bool shade_recorded[2] = {false, false};
+    int gpar = 0;  // G-buffer half of the next frame
     // Per-kernel timing: one slot of events per frame in a ring, read back lazily (when the slot is
     // reused or the sums are requested), so timing never stalls the streams.
     // beg[k] .. end[k] brackets stage k (HDR_STAGE_*) on the stream it runs on; bit k of `ran`
@@ -1375,7 +1397,7 @@ hdr_status hdr_view_create(hdr_context *ctx, int32_t width, int32_t height, hdr_
     HIP_TRY(v->accum.alloc(P * sizeof(float4)));
     HIP_TRY(v->color.alloc(P * sizeof(float4)));
     HIP_TRY(v->counters.alloc(sizeof(hdr_frame_counters)));
-    HIP_TRY(v->gbuf.alloc(P * sizeof(GRec)));
+    HIP_TRY(v->gbuf.alloc(2 * P * sizeof(GRec)));  // two halves, by frame parity
     HIP_TRY(v->qcount.alloc(sizeof(unsigned int)));
     HIP_TRY(v->dargs.alloc(hdr_view::kRing * sizeof(FrameArgs)));
     HIP_TRY(v->dtile.alloc(2 * sizeof(int)));
@@ -1386,7 +1408,7 @@ hdr_status hdr_view_create(hdr_context *ctx, int32_t width, int32_t height, hdr_
     }
     HIP_TRY(hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking));
     HIP_TRY(hipStreamCreateWithFlags(&v->gstream, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreateWithFlags(&v->shade_done, hipEventDisableTiming));
+    for (hipEvent_t &e : v->shade_done) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&v->gbuf_done, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&v->serial, hipEventDisableTiming));
     for (auto &slot : v->ring) {
@@ -1415,7 +1437,7 @@ hdr_status hdr_view_destroy(hdr_view *view) {
             if (e) (void)hipEventDestroy(e);
         if (slot.fin) (void)hipEventDestroy(slot.fin);
     }
-    for (hipEvent_t e : {view->shade_done, view->gbuf_done, view->serial})
+    for (hipEvent_t e : {view->shade_done[0], view->shade_done[1], view->gbuf_done, view->serial})
         if (e) (void)hipEventDestroy(e);
     if (view->gstream) (void)hipStreamDestroy(view->gstream);
     if (view->stream) (void)hipStreamDestroy(view->stream);
@@ -1567,7 +1589,7 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
         a.aov_albedo = static_cast<float *>(view->albedo.p);
         a.aov_normal = static_cast<float *>(view->normal.p);
     }
-    a.gbuf = static_cast<GRec *>(view->gbuf.p);
+    a.gbuf = static_cast<GRec *>(view->gbuf.p) + size_t(view->gpar & 1) * P;  // this frame's half
     // The deferred-shadow path applies when the winner's shadow ray is the only ray after the
     // G-buffer: RIS at depth 1, skipVisibility, no translucent material (no pass-through rays),
     // nothing BSDF-sampled can reach an emitter (no emissive material, mesh light or environment),
@@ -1671,6 +1693,7 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     const size_t P = size_t(W) * H;
     const bool deferred = plan.deferred, spatial = plan.spatial;
     const size_t lds = plan.lds;
+    const int gp = view->gpar & 1;  // this frame's G-buffer half (prepare_frame chose it)
     const dim3 grid(plan.tiles);  // one block per 8x8 tile
     if (view->counting) {
         HIP_TRY(hipMemsetAsync(view->counters.p, 0, view->counters.n, view->stream));
@@ -1688,8 +1711,8 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     if (view->counting) {
         HIP_TRY(hipEventRecord(view->serial, view->stream));
         HIP_TRY(hipStreamWaitEvent(view->gstream, view->serial, 0));
-    } else if (view->shade_recorded) {
-        HIP_TRY(hipStreamWaitEvent(view->gstream, view->shade_done, 0));
+    } else if (view->shade_recorded[gp]) {
+        HIP_TRY(hipStreamWaitEvent(view->gstream, view->shade_done[gp], 0));
     }
     HIP_TRY(hipMemcpyAsync(const_cast<FrameArgs *>(da), &view->hargs.p[view->ring_next], sizeof(FrameArgs),
                            hipMemcpyHostToDevice, view->gstream));
@@ -1732,8 +1755,8 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
             HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_SPATIAL], view->stream));
             ts.ran |= 1u << HDR_STAGE_SPATIAL;
         }
-        HIP_TRY(hipEventRecord(view->shade_done, view->stream));
-        view->shade_recorded = true;
+        HIP_TRY(hipEventRecord(view->shade_done[gp], view->stream));
+        view->shade_recorded[gp] = true;
         HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_SHADOW], view->stream));
         const int sblocks = queue_blocks(view->shape, P);
         if (view->counting) launch_shadow<true>(view->shape, sblocks, view->stream, da);
@@ -1792,8 +1815,8 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
             if (spatial && b == (plan.wave_traced ? 1 : 0))
                 if (hdr_status st = spatial_launch()) return st;
             if (b == (plan.wave_traced ? 1 : 0)) {  // the G-buffer (vertex 0's hits) has been read
-                HIP_TRY(hipEventRecord(view->shade_done, view->stream));
-                view->shade_recorded = true;
+                HIP_TRY(hipEventRecord(view->shade_done[gp], view->stream));
+                view->shade_recorded[gp] = true;
             }
             if (view->counting) launch_wave_trace<true>(view->shape, qblocks, view->stream, da, b, plan.wave_traced);
             else launch_wave_trace<false>(view->shape, qblocks, view->stream, da, b, plan.wave_traced);
@@ -1837,8 +1860,8 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     if (!deferred && !plan.wave) {
         ts.ran |= 1u << HDR_STAGE_INTEGRATE;
         HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_INTEGRATE], view->stream));
-        HIP_TRY(hipEventRecord(view->shade_done, view->stream));
-        view->shade_recorded = true;
+        HIP_TRY(hipEventRecord(view->shade_done[gp], view->stream));
+        view->shade_recorded[gp] = true;
     }
     if (a.firefly) {
         HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_FIREFLY], view->stream));
@@ -1854,6 +1877,7 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     view->ring_last = view->ring_next;
     view->ring_next = (view->ring_next + 1) % hdr_view::kRing;
     view->frames++;
+    view->gpar ^= 1;
     if (color_out) {
         HIP_TRY(hipMemcpyAsync(color_out, view->color.p, P * sizeof(float4), hipMemcpyDeviceToHost, view->stream));
     }
@@ -2362,6 +2386,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         HIP_TRY(hipStreamWaitEvent(views[i]->stream, sl.done, 0));
         HIP_TRY(hipStreamWaitEvent(views[i]->gstream, sl.done, 0));
         views[i]->frames++;
+        views[i]->gpar ^= 1;
     }
     return HDR_OK;
 }

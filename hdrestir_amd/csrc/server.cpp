@@ -32,7 +32,7 @@ double ms_since(Clock::time_point a, Clock::time_point b) {
 // reservoirs and per-pixel record; RNG spill slots when the path can draw past output 311).
 uint64_t view_footprint(int w, int h, const hdr_pipeline_desc &p) {
     const uint64_t P = uint64_t(w) * uint64_t(h);
-    uint64_t per = sizeof(hdr_reservoir) + 16 + 16 + 64;
+    uint64_t per = sizeof(hdr_reservoir) + 16 + 16 + 2 * 64;  // G-buffer: two halves
     // RIS with skipVisibility: the deferred path (depth 1; colour pair, flags, shadow queue) or,
     // at any depth, the wavefront path (path record, bounce hit, ray / shadow / contribution
     // queues, two live lists); the scene decides which, so the larger one is counted

@@ -303,7 +303,8 @@ enum {
     HDR_STAGE_SHADOW = 2,     /* deferred path: any-hit shadow rays of the winners               */
     HDR_STAGE_ACCUMULATE = 3, /* deferred path: visibility pick + accumulation                   */
     HDR_STAGE_FIREFLY = 4,    /* firefly filter + accumulation (enable_firefly_filter)           */
-    HDR_STAGE_SPATIAL = 5,    /* spatial reuse: neighbour resampling + colour                    */
+    HDR_STAGE_SPATIAL = 5,    /* spatial reuse: neighbour resampling + colour (on the wavefront
+                                 path this launch lies inside the HDR_STAGE_INTEGRATE span)      */
     HDR_STAGE_COUNT = 6
 };
 /* Per-stage device time (ms) summed over the frames rendered since the previous call, and the
