@@ -1195,7 +1195,8 @@ struct hdr_view {
     hipStream_t gstream = nullptr;
     hipEvent_t shade_done[2] = {nullptr, nullptr}, gbuf_done = nullptr, serial = nullptr;
     bool shade_recorded[2] = {false, false};
-    int gpar = 0;  // G-buffer half of the next frame
+    int gpar = 0;                    // G-buffer half of the next frame
+    bool single_since_batch = true;  // a single frame used the G-buffer after the last batch step
     // Per-kernel timing: one slot of events per frame in a ring, read back lazily (when the slot is
     // reused or the sums are requested), so timing never stalls the streams.
     // beg[k] .. end[k] brackets stage k (HDR_STAGE_*) on the stream it runs on; bit k of `ran`
@@ -1878,6 +1879,7 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     view->ring_next = (view->ring_next + 1) % hdr_view::kRing;
     view->frames++;
     view->gpar ^= 1;
+    view->single_since_batch = true;
     if (color_out) {
         HIP_TRY(hipMemcpyAsync(color_out, view->color.p, P * sizeof(float4), hipMemcpyDeviceToHost, view->stream));
     }
@@ -2040,6 +2042,11 @@ hdr_status hdr_rng_float(hdr_context *ctx, const uint32_t *seeds, int32_t n, int
 struct hdr_batch {
     hdr_context *ctx = nullptr;
     hipStream_t stream = nullptr;
+    // A step's argument copy and G-buffer launch run on `gstream`, beside the previous step's
+    // shading on `stream` (the views' G-buffers are double-buffered by frame parity); the step's
+    // shading waits for them (event `gdone`).
+    hipStream_t gstream = nullptr;
+    hipEvent_t gdone = nullptr;
     hipEvent_t beg = nullptr, end = nullptr;
     bool timed = false;
     DevBuf queue, qcount;  // shadow queries of every view of a step, and their number
@@ -2072,6 +2079,8 @@ hdr_status hdr_batch_create(hdr_context *ctx, hdr_batch **out) {
     if (!b) return fail(HDR_ERR_OUT_OF_MEMORY, "host allocation failed");
     b->ctx = ctx;
     HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&b->gstream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&b->gdone, hipEventDisableTiming));
     HIP_TRY(hipEventCreate(&b->beg));
     HIP_TRY(hipEventCreate(&b->end));
     for (auto &sl : b->slots) HIP_TRY(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
@@ -2084,6 +2093,7 @@ hdr_status hdr_batch_create(hdr_context *ctx, hdr_batch **out) {
 hdr_status hdr_batch_destroy(hdr_batch *b) {
     if (!b) return HDR_OK;
     (void)hipSetDevice(b->ctx->device);
+    if (b->gstream) (void)hipStreamSynchronize(b->gstream);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (auto &g : b->graph)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -2091,6 +2101,8 @@ hdr_status hdr_batch_destroy(hdr_batch *b) {
         if (sl.done) (void)hipEventDestroy(sl.done);
     if (b->beg) (void)hipEventDestroy(b->beg);
     if (b->end) (void)hipEventDestroy(b->end);
+    if (b->gdone) (void)hipEventDestroy(b->gdone);
+    if (b->gstream) (void)hipStreamDestroy(b->gstream);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return HDR_OK;
@@ -2099,6 +2111,7 @@ hdr_status hdr_batch_destroy(hdr_batch *b) {
 hdr_status hdr_batch_synchronize(hdr_batch *b) {
     if (!b) return fail(HDR_ERR_INVALID_ARGUMENT, "batch is null");
     HIP_TRY(hipSetDevice(b->ctx->device));
+    HIP_TRY(hipStreamSynchronize(b->gstream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     return HDR_OK;
 }
@@ -2277,22 +2290,31 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         }
         tb[members[g].size()] = base;
     }
-    // order after each view's earlier work; stage the arguments
+    // order after each view's earlier work; stage the arguments. The G-buffer halves this step
+    // writes were last read two steps ago, or by a single frame of the view since its last step.
     for (int i = 0; i < n; ++i) {
         HIP_TRY(hipEventRecord(views[i]->serial, views[i]->stream));
         HIP_TRY(hipStreamWaitEvent(b->stream, views[i]->serial, 0));
+        if (views[i]->single_since_batch) HIP_TRY(hipStreamWaitEvent(b->gstream, views[i]->serial, 0));
+    }
+    {
+        const hdr_batch::Slot &prev2 = b->slots[(b->next + hdr_batch::kSlots - 2) % hdr_batch::kSlots];
+        if (prev2.pending) HIP_TRY(hipStreamWaitEvent(b->gstream, prev2.done, 0));
     }
     auto A = [&](int g) { return reinterpret_cast<const FrameArgs *>(dp + argOff[g]); };
     auto T = [&](int g) { return reinterpret_cast<const int *>(dp + tileOff[g]); };
     auto N = [&](int g) { return int(members[g].size()); };
     const LaunchShape ls = b->shape;
-    // the step: argument copy, then every launch (captured once per slot and launch shape when
-    // graphs are on, so later steps replay it)
-    auto enqueue = [&]() -> hipError_t {
-        hipError_t e = hipMemcpyAsync(sl.dev.p, sl.host.p, off, hipMemcpyHostToDevice, b->stream);
+    // the step: argument copy and G-buffer on gstream, then every other launch on stream (captured
+    // once per slot and launch shape when graphs are on, so later steps replay it)
+    auto enqueue_g = [&]() -> hipError_t {
+        const hipError_t e = hipMemcpyAsync(sl.dev.p, sl.host.p, off, hipMemcpyHostToDevice, b->gstream);
         if (e != hipSuccess) return e;
+        launch_gbuffer<false>(ls, tiles[kAll], b->gstream, A(kAll), T(kAll), N(kAll));
+        return hipGetLastError();
+    };
+    auto enqueue = [&]() -> hipError_t {
         hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, b->stream, static_cast<unsigned int *>(b->qcount.p));
-        launch_gbuffer<false>(ls, tiles[kAll], b->stream, A(kAll), T(kAll), N(kAll));
         if (N(kDef) > 0)
             hipLaunchKernelGGL((shade_direct_kernel<false, false>), dim3(tiles[kDef]), dim3(kLanes), lds[kDef],
                                b->stream, A(kDef), T(kDef), N(kDef));
@@ -2332,9 +2354,12 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         }
         return hipGetLastError();
     };
+    HIP_TRY(enqueue_g());
+    HIP_TRY(hipEventRecord(b->gdone, b->gstream));
+    HIP_TRY(hipStreamWaitEvent(b->stream, b->gdone, 0));
     if (b->graphs) {
         // launch shape key: everything baked into the captured launches (buffers, grid sizes,
-        // argument layout); the arguments themselves are copied by the graph's first node
+        // argument layout); the arguments themselves are copied before the graph runs
         std::vector<uint64_t> key = {uint64_t(n), uint64_t(off), uint64_t(uintptr_t(sl.dev.p)),
                                      uint64_t(uintptr_t(sl.host.p)), uint64_t(uintptr_t(b->queue.p)),
                                      uint64_t(uintptr_t(b->wqueue.p)), uint64_t(uintptr_t(b->wray.p)),
@@ -2387,6 +2412,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         HIP_TRY(hipStreamWaitEvent(views[i]->gstream, sl.done, 0));
         views[i]->frames++;
         views[i]->gpar ^= 1;
+        views[i]->single_since_batch = false;
     }
     return HDR_OK;
 }

@@ -211,8 +211,12 @@ hdr_status hdr_server_step(hdr_server *s, int32_t *n_finished) {
         if (hdr_status st = hdr_render_batch(s->batch, int32_t(n), views.data(), scenes.data(), frames.data(),
                                              pipes.data()))
             return st;
-        for (Request &r : s->active) ++r.done;
-        if (hdr_status st = hdr_batch_synchronize(s->batch)) return st;
+        bool finishing = false;
+        for (Request &r : s->active) finishing = ++r.done >= r.spp || finishing;
+        // steps run ahead on the device; the host waits only when a request completes here (its
+        // completion time is taken after the wait, and its image is read after it)
+        if (finishing)
+            if (hdr_status st = hdr_batch_synchronize(s->batch)) return st;
     } else {
         Request &r = s->active.front();  // plain path: the whole request, frame by frame
         while (r.done < r.spp) {
