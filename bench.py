@@ -120,6 +120,9 @@ def serving_requests(rng, n, many_view, box_view, only_restir_1080p=False, small
     return reqs
 
 
+SERVER_MAX_ACTIVE = 64  # requests on the device at once when batched
+
+
 def run_server(gpu, abi, ctx, gscenes, reqs, batched, arrivals=None, images=False):
     """Feeds `reqs` to a server (all at t = 0, or at the given arrival times in seconds) and steps
     it until every request has finished. Returns (makespan s, latencies ms, images by index)."""
@@ -127,7 +130,7 @@ def run_server(gpu, abi, ctx, gscenes, reqs, batched, arrivals=None, images=Fals
     pipes = {"pt": abi.pipeline("mis", max_depth=4),
              "restir": abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=1,
                                     spatial_count=3, spatial_radius=30.0)}
-    srv = gpu.Server(ctx, batched=batched, max_active=64, hbm_budget_bytes=48 << 30)
+    srv = gpu.Server(ctx, batched=batched, max_active=SERVER_MAX_ACTIVE, hbm_budget_bytes=48 << 30)
     ids, lat, imgs = {}, [], {}
     t0 = time.perf_counter()
     nxt = 0
