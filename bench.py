@@ -360,7 +360,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by pixels and requests, see DESIGN.md)", file=sys.stderr)
+              "(the path would shard by pixels and by requests across GPUs)", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_calibration:

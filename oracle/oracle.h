@@ -4,8 +4,8 @@
  * TEST INFRASTRUCTURE ONLY. The oracle is a faithful C++20 CPU restatement of the
  * hdRestir per-pixel hot path (RaycastPass -> IntegrationPass with PathIntegrator and
  * MIS / RIS direct lighting -> AccumulationPass). Only tests/, __graft_entry__.smoke()
- * and bench.py's CPU-baseline leg load it, as the checker; the product library
- * (libhdrgpu.so) never links, loads or calls it.
+ * and bench.py's CPU-baseline leg load it, to check results against and to time the CPU
+ * baseline; the product library (libhdrgpu.so) never links, loads or calls it.
  *
  * Inputs use the same POD descriptors as include/hdrgpu.h so both sides see identical
  * scenes, frames and pipeline settings.

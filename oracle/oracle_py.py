@@ -1,7 +1,7 @@
 """ctypes front end of the CPU oracle (oracle/build/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by bench.py's
-CPU-baseline leg, as the checker. The product library never imports it.
+CPU-baseline leg, to check results against. The product library never imports it.
 """
 import ctypes as C
 import os

@@ -5,7 +5,7 @@ Bars (stated per test):
   * closest hits: instance id exact, triangle id on >= 99.99 % of rays, t within 1e-5 relative;
   * per-pixel integer streams (draw count, FNV hash of every light index and every RIS /
     reservoir selection): identical on >= 99.9 % of pixels (a pixel can only diverge when a
-    float decision lands within an ulp of its threshold, see DESIGN.md);
+    float decision lands within an ulp of its threshold);
   * colours of stream-identical pixels: within 1e-3 relative (float transcendental ulps).
 """
 import numpy as np
