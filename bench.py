@@ -7,7 +7,7 @@ Workload (one step = one frame of the per-pixel hot path over the whole image):
   path    RaycastPass G-buffer ray -> jittered camera ray -> 4 hero wavelengths -> RIS with
           M = 32 candidates (16 light + 16 BSDF), skipVisibility (one shadow ray for the winner),
           temporal reservoir reuse (the reference's merge), then spatial reuse of 3 neighbours
-          within 30 px (an extension of this build), max depth 1 -> spectral->RGB -> accumulation;
+          within 30 px (an extension beyond the reference), max depth 1 -> spectral->RGB -> accumulation;
           the reference's own pipeline (temporal reuse only) is timed beside it;
   camera  static (progressive accumulation with temporal reuse); the linear-dolly variant is
           reported beside it (a camera move clears reservoirs and accumulation, as the renderer does).
@@ -359,7 +359,7 @@ def main():
                     help="also time the moving-camera variant (each move clears reservoirs and accumulation)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the path runs on one GPU; --gpus {args.gpus} is not supported "
               "(the path would shard by pixels and by requests across GPUs)", file=sys.stderr)
         sys.exit(2)
 

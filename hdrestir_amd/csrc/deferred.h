@@ -589,7 +589,7 @@ __device__ __forceinline__ Spec ris_direct_traced(const FrameArgs &a, const Spec
 }
 
 // ---------------------------------------------------------------- spatial reuse
-// An extension of this build (the reference reuses temporally only), run after every pixel's
+// An extension beyond the reference (which reuses temporally only), run after every pixel's
 // temporal pass. Pixel q resamples among its own post-temporal reservoir and those of up to
 // spatial_count neighbours picked within spatial_radius pixels that pass a similarity test
 // (depth within 10 %, geometric normals within 25 degrees). Reservoir j (confidence c_j = its

@@ -1500,8 +1500,9 @@ struct Settings {
     hdr_pipeline_desc p;
 };
 
-// Per-pixel record the temporal pass leaves for the spatial-reuse pass (an extension of this
-// build, see SpatialPass below): the shading vertex and the path weights around its direct light.
+// Per-pixel record the temporal pass leaves for the spatial-reuse pass (an extension beyond the
+// reference, see the spatial pass below): the shading vertex and the path weights around its direct
+// light.
 struct SpatialAux {
     bool valid = false;
     HitRecord hit;
@@ -2166,7 +2167,7 @@ inline uint32_t PixelSeed(size_t i, int frameIndex) {  // integration_pass.cpp:2
 }  // namespace
 
 // ---------------------------------------------------------------- C API
-// ---- Spatial reuse: an extension of this build (the reference reuses temporally only).
+// ---- Spatial reuse: an extension beyond the reference (which reuses temporally only).
 // After every pixel's temporal pass, pixel q resamples among its own post-temporal reservoir and
 // those of up to spatial_count neighbours picked within spatial_radius pixels that pass a
 // similarity test (depth within 10 %, geometric normals within 25 degrees). Each reservoir j
