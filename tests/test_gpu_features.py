@@ -390,3 +390,58 @@ def test_wavefront_traced_candidates_parity(gpu_ctx, oracle_mod, variant, reserv
         assert np.array_equal(gr["count"][m], orr["count"][m])
     r.close()
     gs.close()
+
+
+COUNTER_CASES = {
+    # inlined path tracer (MIS): the last connection ray is skipped, not traced
+    "pt_d4": (lambda: scenes.box_spheres(48, 40, frames=2), abi.pipeline("mis", max_depth=4), False),
+    # general megakernel, RIS with visibility: BSDF candidates that can reach no emitter are skipped
+    "ris16_d4": (lambda: scenes.box_spheres(48, 40, frames=2),
+                 abi.pipeline("ris", candidate_count=16, use_reservoir=True, max_depth=4), False),
+    # deferred-shadow path
+    "restir16_skipvis_d1": (lambda: scenes.box_spheres(48, 40, frames=2),
+                            abi.pipeline("ris", candidate_count=16, use_reservoir=True, skip_visibility=True,
+                                         max_depth=1), False),
+    # wavefront, with spatial reuse
+    "restir8_skipvis_d4_spatial3": (lambda: scenes.box_spheres(48, 40, frames=2),
+                                    abi.pipeline("ris", candidate_count=8, use_reservoir=True, skip_visibility=True,
+                                                 max_depth=4, spatial_count=3, spatial_radius=10.0), False),
+    # wavefront with traced candidates (environment): every ray is traced, class by class
+    "dome_restir8_skipvis_d4": (lambda: scenes.features(48, 40, dome=True, emissive=False, mesh_light=False,
+                                                        opaque=True),
+                                abi.pipeline("ris", candidate_count=8, use_reservoir=True, skip_visibility=True,
+                                             max_depth=4), True),
+}
+
+
+@pytest.mark.parametrize("case", list(COUNTER_CASES))
+def test_ray_counters_match_the_oracle(gpu_ctx, oracle_mod, case):
+    """The per-frame ray counters (the benchmark's ray counts and reference-equivalent rays) against
+    the oracle's: primary and shadow rays equal; a ray the GPU path skips because it can reach no
+    emitter (its result is known without tracing) is counted as skipped, so bounce + candidate +
+    skipped equals the oracle's bounce + candidate rays; frames that trace every ray match class by
+    class."""
+    from hdrestir_amd import gpu
+    make, pipe, traced = COUNTER_CASES[case]
+    sc = make()
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    r.set_counting(True)
+    o = oracle_mod.Oracle(sc, threads=8)
+    for f in range(2):
+        fr = scenes.frame_with_camera(sc, f)
+        r.render(fr, pipe, read_color=False)
+        r.synchronize()
+        g = r.counters()
+        o.render(fr, pipe)
+        c = o.counters()
+        print(case, f, {k: g[k] for k in g if k.endswith("rays") or k.endswith("skipped")}, c)
+        assert g["primary_rays"] == c["primary_rays"] == sc.width * sc.height
+        assert g["shadow_rays"] == c["shadow_rays"]
+        assert (g["bounce_rays"] + g["candidate_rays"] + g["candidate_rays_skipped"] ==
+                c["bounce_rays"] + c["candidate_rays"])
+        if traced:
+            assert g["candidate_rays_skipped"] == 0
+            assert g["bounce_rays"] == c["bounce_rays"] and g["candidate_rays"] == c["candidate_rays"]
+    r.close()
+    gs.close()
