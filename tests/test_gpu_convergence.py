@@ -7,7 +7,8 @@ on the same seed -- the parity tests pin the streams bit-exactly and the colours
 -- so this average is the oracle's 4096-spp image, obtained in seconds instead of CPU hours.
 
 Workloads: the box of 12 triangles with two icospheres and one rect light, 256 x 256, depth 4
-(PathTracer, RIS M = 16 without and with the reservoir); the many-lights scene (height field, 500
+(PathTracer, RIS M = 16 without and with the reservoir, ReSTIR with one shadow ray with and without
+3 spatial neighbours); the many-lights scene (height field, 500
 spheres, 4096 point / rect lights) from its camera at 480 x 270, direct lighting (depth 1: RIS
 M = 32, ReSTIR M = 32 with one shadow ray, with and without 3 spatial neighbours).
 
@@ -91,6 +92,13 @@ BOX_CASES = {
     "pt_d4": (abi.pipeline("mis", max_depth=4), 64, 0.05, True),
     "ris16_d4": (abi.pipeline("ris", candidate_count=16, use_reservoir=False, max_depth=4), 64, 0.03, True),
     "restir16_d4": (abi.pipeline("ris", candidate_count=16, use_reservoir=True, max_depth=4), 64, 0.03, False),
+    # spatial reuse (an extension) at depth 4: the primary vertex's direct light resampled with 3
+    # neighbours, the later vertices as without it
+    "restir16_d4_skipvis_spatial3": (abi.pipeline("ris", candidate_count=16, use_reservoir=True,
+                                                  skip_visibility=True, max_depth=4, spatial_count=3,
+                                                  spatial_radius=30.0), 64, 0.06, False),
+    "restir16_d4_skipvis": (abi.pipeline("ris", candidate_count=16, use_reservoir=True, skip_visibility=True,
+                                         max_depth=4), 64, 0.03, False),
 }
 
 
