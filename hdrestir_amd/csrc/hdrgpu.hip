@@ -1,4 +1,6 @@
-// hdrgpu.hip -- kernels and C-ABI of libhdrgpu.so (declared in include/hdrgpu.h).
+// hdrgpu.hip -- host side and C-ABI of libhdrgpu.so (declared in include/hdrgpu.h), plus the small
+// kernels of scene upload, queue clears, the firefly filter and the RNG parity entry point. The
+// rendering kernels live in k_*.hip (declared in kernels.h).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -12,409 +14,11 @@
 
 #include "../../include/hdrgpu.h"
 #include "host_scene.h"
-#include "pathlean.h"
+#include "kernels.h"
 #include "wavefront.h"
 
 using namespace hdr;
 
-// ============================================================================ kernels
-
-// RaycastPass (raycast_pass.cpp:57-162): closest hit of the pixel-centre ray, one 8x8 tile per wave.
-// Sum over the 64 lanes of the wave (every lane active), added once by lane 0.
-__device__ __forceinline__ void wave_add(unsigned long long *dst, unsigned long long v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(dst, v);
-}
-
-// LDS rings of the traversal stacks (RingStack): the top 4 TLAS and 8 BLAS entries of each lane.
-constexpr int kRingT = 4, kRingB = 8;
-#define HDR_RING_LDS                                \
-    __shared__ int ring_tr[kRingT * kLanes];        \
-    __shared__ float ring_tt[kRingT * kLanes];      \
-    __shared__ int ring_br[kRingB * kLanes];        \
-    __shared__ float ring_bt[kRingB * kLanes];
-
-// Closest hit with the stacks in private memory or with LDS rings (the lane's columns).
-template <bool kCount, bool kLds>
-__device__ __forceinline__ bool closest_hit(const DeviceScene &s, V3 o, V3 d, RawHit &rh, TraceStats &ts, int *tr,
-                                            float *tt, int *br, float *bt) {
-    if (kLds) {
-        RingStack<kRingT, true> t;
-        RingStack<kRingB, true> b;
-        t.bind(tr, tt);
-        b.bind(br, bt);
-        return trace_closest_s<kCount>(s, o, d, rh, ts, t, b);
-    }
-    return trace_closest<kCount>(s, o, d, rh, ts);
-}
-template <bool kCount, bool kLds>
-__device__ __forceinline__ bool any_hit(const DeviceScene &s, V3 o, V3 d, float tmax, TraceStats &ts, int *tr,
-                                        int *br) {
-    if (kLds) {
-        RingStack<kRingT, false> t;
-        RingStack<kRingB, false> b;
-        t.bind(tr, nullptr);
-        b.bind(br, nullptr);
-        return trace_any_s<kCount>(s, o, d, tmax, ts, t, b);
-    }
-    return trace_any<kCount>(s, o, d, tmax, ts);
-}
-
-// The G-buffer ray of a pixel (RaycastPass: pixel centre; the camera's lens draws when it has
-// camera parameters).
-__device__ __forceinline__ void gbuffer_ray(const FrameArgs &a, int x, int y, V3 &co, V3 &cd) {
-    const int pixel = y * a.W + x;
-    if (a.has_cam) {
-        const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
-        Mt64 crng;
-        crng.init(seed, nullptr);
-        camera_ray(a, float(x) + 0.5f, float(y) + 0.5f, &crng, co, cd);
-    } else {
-        camera_ray(a, float(x) + 0.5f, float(y) + 0.5f, static_cast<Mt64 *>(nullptr), co, cd);
-    }
-}
-// The hit record and AOVs of a pixel.
-__device__ __forceinline__ void gbuffer_store(const FrameArgs &a, GRec *gbuf, int pixel, V3 co, V3 cd, bool hit,
-                                              const RawHit &rh) {
-    HitRec g = {};
-    if (hit) g = make_hit_record(a.s, co, cd, rh);
-    store_grec(gbuf + pixel, hit, g);
-    if (a.aov_depth) {
-        a.aov_depth[pixel] = hit ? g.depth : 1.0f;
-        const V3 al = hit ? g.albedo : v3s(0.0f);
-        const V3 nn = hit ? g.ng : v3s(0.0f);
-        a.aov_albedo[3 * pixel + 0] = al.x;
-        a.aov_albedo[3 * pixel + 1] = al.y;
-        a.aov_albedo[3 * pixel + 2] = al.z;
-        a.aov_normal[3 * pixel + 0] = nn.x;
-        a.aov_normal[3 * pixel + 1] = nn.y;
-        a.aov_normal[3 * pixel + 2] = nn.z;
-    }
-}
-template <bool kCount, bool kLds = false>
-__device__ __forceinline__ void gbuffer_pixel(const FrameArgs &a, GRec *gbuf, int x, int y, TraceStats &ts, int *tr,
-                                              float *tt, int *br, float *bt) {
-    V3 co, cd;
-    gbuffer_ray(a, x, y, co, cd);
-    RawHit rh;
-    const bool hit = closest_hit<kCount, kLds>(a.s, co, cd, rh, ts, tr, tt, br, bt);
-    gbuffer_store(a, gbuf, y * a.W + x, co, cd, hit, rh);
-}
-
-// Launch layout: one wave64 per 8x8 pixel tile; the blocks of a launch enumerate the tiles of
-// every view in it, view v owning blocks tile_base[v] .. tile_base[v + 1) (a 1-D grid).
-__device__ __forceinline__ int launch_view(const int *__restrict__ tile_base, int n_views) {
-    const int b = int(blockIdx.x);
-    int lo = 0, hi = n_views - 1;
-    while (lo < hi) {  // last view whose first block is <= b (uniform across the block)
-        const int mid = (lo + hi + 1) >> 1;
-        if (tile_base[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-#define HDR_TILE(args, tile_base, n_views)                                   \
-    const int view_ = launch_view(tile_base, n_views);                        \
-    const FrameArgs &a = (args)[view_];                                       \
-    const int tile_ = int(blockIdx.x) - (tile_base)[view_];                   \
-    const int tw_ = (a.W + 7) >> 3;                                           \
-    const int lane = threadIdx.x;                                             \
-    const int x = (tile_ % tw_) * 8 + (lane & 7);                             \
-    const int y = (tile_ / tw_) * 8 + (lane >> 3);                            \
-    const bool valid = x < a.W && y < a.H;
-
-template <bool kCount, bool kLds = false>
-__global__ void __launch_bounds__(kLanes) gbuffer_kernel(const FrameArgs *__restrict__ args,
-                                                         const int *__restrict__ tile_base, int n_views) {
-    HDR_RING_LDS
-    HDR_TILE(args, tile_base, n_views)
-    TraceStats ts = {};
-    if (kCount || kLds) {  // per-lane traversal (the counting pass counts each ray's own work)
-        if (valid)
-            gbuffer_pixel<kCount, kLds>(a, a.gbuf, x, y, ts, ring_tr + lane, ring_tt + lane, ring_br + lane,
-                                        ring_bt + lane);
-    } else {  // the tile's 64 primary rays traverse together
-        __shared__ WaveStack wt, wb;
-        V3 co = v3s(0.0f), cd = v3(0.0f, 0.0f, 1.0f);
-        if (valid) gbuffer_ray(a, x, y, co, cd);
-        RawHit rh;
-        const bool hit = trace_closest_wave(a.s, valid, co, cd, rh, wt, wb);
-        if (valid) gbuffer_store(a, a.gbuf, y * a.W + x, co, cd, hit, rh);
-    }
-    if (kCount) {
-        unsigned long long *c = a.counters;
-        wave_add(c + 0, valid ? 1ULL : 0ULL);
-        wave_add(c + 5, ts.nodes);
-        wave_add(c + 6, ts.tris);
-        wave_add(c + 7, ts.instances);
-        wave_add(c + 9, ts.nodes);
-        wave_add(c + 10, ts.tris);
-    }
-}
-
-// IntegrationPass (integration_pass.cpp:208-243) + fused accumulation for one pixel.
-template <bool kCount, bool kDebug>
-__device__ __forceinline__ void integrate_pixel(const FrameArgs &a, const GRec *gbuf, int x, int y, double *wlane,
-                                                PixelStats &st, uint32_t &draws) {
-    const int pixel = y * a.W + x;
-    const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
-    Tracer<kCount, false> tracer{a.s, st};
-    HitRec g;
-    const bool gHit = load_grec(gbuf + pixel, g);
-
-    // IntegrationPass (integration_pass.cpp:208-243)
-    Mt64 rng;
-    uint64_t *slot = a.spill ? a.spill + size_t(pixel) * (2 * kMtN) : nullptr;
-    rng.init(seed, slot);
-    const float px = float(x) + rng.next_float();
-    const float py = float(y) + rng.next_float();
-    V3 ro, rd;
-    camera_ray(a, px, py, &rng, ro, rd);
-    const SpecTab tab = make_tab(rng.next_float());
-    Trace dbg = {2166136261u, 2166136261u, 0};
-    PixelCtx<kCount, kDebug, false> p{a, tab, rng, tracer, st, dbg, wlane, slot ? slot + kMtN : nullptr};
-    const Spec L = path_li(p, rd, gHit, g, pixel);
-    V3 rgb = spec2rgb(tab, L);
-    if (a.has_cam && a.cam.enable_exposure) rgb = rgb * a.exposure_mul;
-    const float4 c4 = make_float4(rgb.x, rgb.y, rgb.z, 1.0f);
-    if (a.firefly) {
-        a.frame_rgb[pixel] = c4;
-    } else {
-        // AccumulationPass (accumulation_pass.cpp:98-106): accum += c; out = accum * invN
-        float4 acc = a.accum[pixel];
-        acc.x += c4.x;
-        acc.y += c4.y;
-        acc.z += c4.z;
-        acc.w += c4.w;
-        a.accum[pixel] = acc;
-        a.color[pixel] = make_float4(acc.x * a.inv_n, acc.y * a.inv_n, acc.z * a.inv_n, acc.w * a.inv_n);
-    }
-    if (kDebug) {
-        hdr_pixel_debug d;
-        d.draws = rng.k;
-        d.light_hash = dbg.lhash;
-        d.select_hash = dbg.shash;
-        d.n_light_picks = dbg.picks;
-        d.prim_instance = gHit ? g.inst : -1;
-        d.prim_triangle = gHit ? g.tri : -1;
-        d.prim_t = gHit ? g.depth : 0.0f;
-        d.sample_rgb[0] = rgb.x;
-        d.sample_rgb[1] = rgb.y;
-        d.sample_rgb[2] = rgb.z;
-        a.dbg[pixel] = d;
-    }
-    draws = rng.k;
-}
-
-// IntegrationPass + (fused) AccumulationPass for one 8x8 pixel tile per wave.
-template <bool kCount, bool kDebug>
-__global__ void __launch_bounds__(kLanes) integrate_kernel(const FrameArgs *__restrict__ args,
-                                                           const int *__restrict__ tile_base, int n_views) {
-    extern __shared__ double wl[];
-    HDR_TILE(args, tile_base, n_views)
-    PixelStats st = {};
-    uint32_t draws = 0;
-    if (valid) integrate_pixel<kCount, kDebug>(a, a.gbuf, x, y, wl + lane, st, draws);
-    if (kCount) {
-        unsigned long long *c = a.counters;
-        wave_add(c + 1, st.bounce);
-        wave_add(c + 2, st.shadow);
-        wave_add(c + 3, st.cand);
-        wave_add(c + 4, st.cand_skipped);
-        wave_add(c + 5, st.ts.nodes);
-        wave_add(c + 6, st.ts.tris);
-        wave_add(c + 7, st.ts.instances);
-        wave_add(c + 8, draws);
-    }
-}
-
-// The path tracer on opaque frames whose pixels stay below RNG output 311 (pathlean.h).
-template <bool kCount, bool kDebug>
-__global__ void __launch_bounds__(kLanes) pt_lean_kernel(const FrameArgs *__restrict__ args,
-                                                         const int *__restrict__ tile_base, int n_views) {
-    HDR_TILE(args, tile_base, n_views)
-    PixelStats st = {};
-    uint32_t draws = 0;
-    if (valid) pt_lean_pixel<kCount, kDebug>(a, x, y, st, draws);
-    if (kCount) {
-        unsigned long long *c = a.counters;
-        wave_add(c + 1, st.bounce);
-        wave_add(c + 2, st.shadow);
-        wave_add(c + 3, st.cand);
-        wave_add(c + 4, st.cand_skipped);
-        wave_add(c + 5, st.ts.nodes);
-        wave_add(c + 6, st.ts.tris);
-        wave_add(c + 7, st.ts.instances);
-        wave_add(c + 8, draws);
-    }
-}
-
-// ---------------------------------------------------------------------------- deferred-shadow path
-// RIS at depth 1 with skipVisibility in a scene where nothing BSDF-sampled can reach an emitter:
-// the only ray after the G-buffer is the RIS winner's shadow ray. Shading runs without traversal
-// code (more waves in flight), the shadow rays are compacted into a queue for an any-hit kernel,
-// and a last pass picks the visible / occluded colour and accumulates.
-
-
-template <bool kCount, bool kDebug>
-__device__ __forceinline__ void shade_direct_pixel(const FrameArgs &a, const GRec *gbuf, int x, int y, double *wlane,
-                                                   PixelStats &st, uint32_t &draws, float4 *rgb2, uint32_t *flags,
-                                                   bool &query, ShadowQuery &q) {
-    const int pixel = y * a.W + x;
-    const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
-    HitRec g;
-    const bool gHit = load_grec(gbuf + pixel, g);
-    Mt64Lean rng;
-    rng.init(seed, nullptr);
-    const float px = float(x) + rng.next_float();
-    const float py = float(y) + rng.next_float();
-    V3 ro, rd;
-    camera_ray(a, px, py, &rng, ro, rd);
-    const float utab = rng.next_float();
-    const SpecTab tab = make_tab(utab);
-    Trace dbg = {2166136261u, 2166136261u, 0};
-    V3 qo = v3s(0.0f), qd = v3s(0.0f);
-    float qtmax = 0.0f;
-    query = false;
-    Spec lVis = spec(0.0f), lOcc = spec(0.0f);
-    if (gHit) {  // PathIntegrator::Li / _li at depth 1 (path_integrator.cpp:112-177); opacity >= 0.999
-        const Clo c = get_closure(a.s, g, a.enable_ss != 0);
-        Spec throughput = spec(1.0f);
-        const Vertex v0 = make_vertex(a, tab, g, c, rd, throughput);
-        const Spec total = rgb2spec(tab, mv3(c.m->emission));
-        // the path's BSDF connection: sampled, never traced (it can reach no emitter)
-        int refl = 0, refr = 0;
-        Bounce first;
-        const bool ok = sample_bounce_inl(tab, v0.c, v0.sn, v0.inside, v0.h.pos, v0.rayDir, a.max_refl, a.max_refr,
-                                          refl, refr, rng, first);
-        if (kCount && ok) ++st.cand_skipped;
-        const Spec direct = ris_direct_lean<kCount, kDebug>(a, tab, v0, ok && first.pdf > 0.0f, rng, wlane, pixel, st,
-                                                           dbg, a.use_res != 0, nullptr, query, qo, qd, qtmax);
-        lVis = throughput * (total + direct);
-        lOcc = throughput * (total + spec(0.0f));
-        if (a.spatial_count > 0) {  // the spatial pass finishes this pixel
-            a.spatial_aux[2 * pixel] = make_float4(rd.x, rd.y, rd.z, utab);
-            a.spatial_aux[2 * pixel + 1] =
-                make_float4(throughput.v[0], throughput.v[1], throughput.v[2], throughput.v[3]);
-        }
-    } else if (a.s.environment >= 0 && a.ibl_bg) {
-        lVis = lOcc = rgb2spec(tab, env_radiance(a.s, rd));
-    }
-    if (a.spatial_count > 0 && !gHit) a.spatial_aux[2 * pixel] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-    V3 cv = spec2rgb(tab, lVis), co = spec2rgb(tab, lOcc);
-    if (a.has_cam && a.cam.enable_exposure) {
-        cv = cv * a.exposure_mul;
-        co = co * a.exposure_mul;
-    }
-    if (!(a.spatial_count > 0 && gHit)) {
-        rgb2[2 * pixel] = make_float4(cv.x, cv.y, cv.z, 1.0f);
-        rgb2[2 * pixel + 1] = make_float4(co.x, co.y, co.z, 1.0f);
-        flags[pixel] = query ? 1u : 0u;
-    }
-    if (query) {
-        q.o = make_float4(qo.x, qo.y, qo.z, qtmax);
-        q.d = make_float4(qd.x, qd.y, qd.z, __uint_as_float(query_code(a, pixel)));
-    }
-    if (kDebug) {
-        hdr_pixel_debug d;
-        d.draws = rng.k;
-        d.light_hash = dbg.lhash;
-        d.select_hash = dbg.shash;
-        d.n_light_picks = dbg.picks;
-        d.prim_instance = gHit ? g.inst : -1;
-        d.prim_triangle = gHit ? g.tri : -1;
-        d.prim_t = gHit ? g.depth : 0.0f;
-        d.sample_rgb[0] = cv.x;
-        d.sample_rgb[1] = cv.y;
-        d.sample_rgb[2] = cv.z;
-        a.dbg[pixel] = d;
-    }
-    draws = rng.k;
-}
-
-// Wave-aggregated append of this lane's shadow query (if any) to the launch's queue.
-__device__ __forceinline__ void append_query(const FrameArgs &a, int lane, bool query, const ShadowQuery &q) {
-    const unsigned long long mask = __ballot(query);
-    const int n = __popcll(mask);
-    if (n > 0) {
-        unsigned int base = 0;
-        if (lane == 0) base = atomicAdd(a.qcount, unsigned(n));
-        base = __shfl(base, 0);
-        if (query) {
-            const int rank = __popcll(mask & ((1ULL << lane) - 1ULL));
-            a.queue[base + rank] = q;
-        }
-    }
-}
-
-template <bool kCount, bool kDebug>
-__global__ void __launch_bounds__(kLanes) shade_direct_kernel(const FrameArgs *__restrict__ args,
-                                                              const int *__restrict__ tile_base, int n_views) {
-    extern __shared__ double wl[];
-    HDR_TILE(args, tile_base, n_views)
-    PixelStats st = {};
-    uint32_t draws = 0;
-    bool query = false;
-    ShadowQuery q;
-    if (valid)
-        shade_direct_pixel<kCount, kDebug>(a, a.gbuf, x, y, wl + lane, st, draws, a.rgb2, a.flags, query, q);
-    append_query(a, lane, query, q);
-    if (kCount) {
-        unsigned long long *c = a.counters;
-        wave_add(c + 2, query ? 1ULL : 0ULL);
-        wave_add(c + 4, st.cand_skipped);
-        wave_add(c + 8, draws);
-    }
-}
-
-// Spatial reuse (deferred.h: spatial_reuse_pixel) and the final colours of the shaded pixels; the
-// winner's shadow query joins the queue as in shade_direct_kernel.
-template <bool kCount, bool kDebug>
-__global__ void __launch_bounds__(kLanes) spatial_kernel(const FrameArgs *__restrict__ args,
-                                                         const int *__restrict__ tile_base, int n_views) {
-    __shared__ int dl[(kMaxSpatial + 1) * kLanes];
-    __shared__ float cl[(kMaxSpatial + 1) * kLanes];
-    HDR_TILE(args, tile_base, n_views)
-    const GRec *gbuf = a.gbuf;
-    float4 *rgb2 = a.rgb2;
-    uint32_t *flags = a.flags;
-    bool query = false;
-    ShadowQuery q;
-    if (valid) {
-        const int pixel = y * a.W + x;
-        Vertex vq;
-        SpecTab tab;
-        if (load_spatial_vertex(a, gbuf, pixel, vq, tab)) {
-            V3 qo = v3s(0.0f), qd = v3s(0.0f);
-            float qtmax = 0.0f;
-            int ssel = -1;
-            const Spec direct = spatial_reuse_pixel<kDebug>(a, gbuf, x, y, vq, tab, dl + lane, cl + lane, query, qo,
-                                                            qd, qtmax, ssel);
-            const float4 t4 = a.spatial_aux[2 * pixel + 1];
-            const Spec throughput = Spec{{t4.x, t4.y, t4.z, t4.w}};
-            const Spec total = rgb2spec(tab, mv3(vq.c.m->emission));
-            const Spec lVis = throughput * (total + direct);
-            const Spec lOcc = throughput * (total + spec(0.0f));
-            V3 cv = spec2rgb(tab, lVis), co = spec2rgb(tab, lOcc);
-            if (a.has_cam && a.cam.enable_exposure) {
-                cv = cv * a.exposure_mul;
-                co = co * a.exposure_mul;
-            }
-            rgb2[2 * pixel] = make_float4(cv.x, cv.y, cv.z, 1.0f);
-            rgb2[2 * pixel + 1] = make_float4(co.x, co.y, co.z, 1.0f);
-            flags[pixel] = query ? 1u : 0u;
-            if (query) {
-                q.o = make_float4(qo.x, qo.y, qo.z, qtmax);
-                q.d = make_float4(qd.x, qd.y, qd.z, __uint_as_float(query_code(a, pixel)));
-            }
-            if (kDebug) a.dbg[pixel].select_hash = fnv(a.dbg[pixel].select_hash, uint32_t(ssel));
-        }
-    }
-    append_query(a, lane, query, q);
-    if (kCount) wave_add(a.counters + 2, query ? 1ULL : 0ULL);
-}
 
 // DomeLight::Prepare (dome_light.cpp:49-76) at scene upload: per row y, the luminance of each texel
 // weighted by sin(theta_y), its running sum as the column CDF and the row's total, summed texel by
@@ -458,63 +62,6 @@ __global__ void dome_cdf_total_kernel(float *row, int w, int h, GDome *rec) {
 // Clears a counter (a kernel rather than a memset, so the step can sit in a captured graph).
 __global__ void zero_u32_kernel(unsigned int *p) { *p = 0u; }
 
-// Any-hit shadow rays of the queue; an occluded winner keeps zero throughput in its reservoir
-// (the visibility fix-up of ris_direct_light_integrator.cpp:163-175).
-template <bool kCount, int kOcc = 1, bool kLds = false>
-__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
-shadow_kernel(const FrameArgs *__restrict__ args) {
-    HDR_RING_LDS
-    const int lane = threadIdx.x;
-    const FrameArgs &a0 = args[0];  // the queue and its count are shared by the views of the launch
-    const unsigned int n = *a0.qcount;
-    TraceStats ts = {};
-    for (unsigned int i = blockIdx.x * kLanes + threadIdx.x; i < n; i += gridDim.x * kLanes) {
-        const ShadowQuery q = a0.queue[i];
-        const uint32_t code = __float_as_uint(q.d.w);
-        const FrameArgs &a = args[code >> 24];
-        const int pixel = int(code & 0xffffffu);
-        if (any_hit<kCount, kLds>(a.s, v3(q.o.x, q.o.y, q.o.z), v3(q.d.x, q.d.y, q.d.z), q.o.w, ts, ring_tr + lane,
-                                  ring_br + lane)) {
-            a.flags[pixel] = 3u;
-            hdr_reservoir &r = a.res[pixel];
-            r.throughput[0] = r.throughput[1] = r.throughput[2] = r.throughput[3] = 0.0f;
-        }
-    }
-    if (kCount) {
-        unsigned long long *c = a0.counters;
-        wave_add(c + 5, ts.nodes);
-        wave_add(c + 6, ts.tris);
-        wave_add(c + 7, ts.instances);
-    }
-}
-
-// Picks the visible or occluded colour and accumulates (AccumulationPass, accumulation_pass.cpp:98-106).
-template <bool kDebug>
-__global__ void __launch_bounds__(kLanes) accumulate_deferred_kernel(const FrameArgs *__restrict__ args,
-                                                                     const int *__restrict__ tile_base,
-                                                                     int n_views) {
-    HDR_TILE(args, tile_base, n_views)
-    if (!valid) return;
-    const int i = y * a.W + x;
-    const float4 c4 = a.rgb2[2 * i + (a.flags[i] == 3u ? 1 : 0)];
-    if (kDebug) {
-        a.dbg[i].sample_rgb[0] = c4.x;
-        a.dbg[i].sample_rgb[1] = c4.y;
-        a.dbg[i].sample_rgb[2] = c4.z;
-    }
-    if (a.firefly) {
-        a.frame_rgb[i] = c4;
-        return;
-    }
-    float4 acc = a.accum[i];
-    acc.x += c4.x;
-    acc.y += c4.y;
-    acc.z += c4.z;
-    acc.w += c4.w;
-    a.accum[i] = acc;
-    a.color[i] = make_float4(acc.x * a.inv_n, acc.y * a.inv_n, acc.z * a.inv_n, acc.w * a.inv_n);
-}
-
 // ---------------------------------------------------------------------------- wavefront path
 // RIS / ReSTIR at every vertex of paths of any depth (wavefront.h). Per frame: the G-buffer, then
 // wave_shade0 (primary vertex, one wave per 8x8 tile), wave_trace (bounce rays and shadow rays of
@@ -522,446 +69,10 @@ __global__ void __launch_bounds__(kLanes) accumulate_deferred_kernel(const Frame
 // wave_resolve (spectrum -> RGB and accumulation). Queue counts are indexed by vertex parity; each
 // launch clears the counts the launch after it appends to.
 
-// Live-path list of vertex v (two lists, by vertex parity, so a launch can read one and append to
-// the other).
-__device__ __forceinline__ unsigned int *alive_list(const FrameArgs &a0, int v) {
-    return (v & 1) ? a0.walive2 : a0.walive;
-}
 
 // Clears the wavefront queue counts (a kernel, so a step can sit in a captured graph).
 __global__ void wave_zero_counts_kernel(unsigned int *c) {
     if (threadIdx.x < 8) c[threadIdx.x] = 0u;
-}
-
-template <bool kCount, bool kDebug, bool kLean = false, bool kTraced = false>
-__global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__restrict__ args,
-                                                             const int *__restrict__ tile_base, int n_views) {
-    extern __shared__ double wl[];
-    HDR_TILE(args, tile_base, n_views)
-    unsigned int *cnt = args[0].wcnt;  // the queues of a launch are shared by its views
-    // live paths of vertex 1: appended by wave_trace (traced frames: this launch appends vertex
-    // 0's live list after the frame-start clear)
-    if (!kTraced && blockIdx.x == 0 && lane == 0) cnt[5] = 0u;
-    PixelStats st = {};
-    bool query = false, ray = false, bray = false;
-    ShadowQuery q;
-    float4 cq = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ro = cq, rd = cq;
-    HitRec next;
-    uint32_t code = 0;
-    using R = typename std::conditional<kLean, Mt64Lean, Mt64Deep>::type;
-    R gr;           // traced: the stream copy wave_gen leaves at the first generated candidate
-    Vertex gv;      // and the vertex
-    SpecTab tab;
-    int k0 = 0, nc = 0;
-    if (valid) {
-        const int pixel = y * a.W + x;
-        const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
-        WavePath<R> p;
-        p.rng.init(seed, wave_slot(a, pixel));
-        const float px = float(x) + p.rng.next_float();
-        const float py = float(y) + p.rng.next_float();
-        V3 o0, d0;
-        camera_ray(a, px, py, &p.rng, o0, d0);
-        p.utab = p.rng.next_float();
-        tab = make_tab(p.utab);
-        p.dbg = Trace{2166136261u, 2166136261u, 0};
-        p.tp = spec(1.0f);
-        p.outer = spec(1.0f);
-        p.total = spec(0.0f);
-        p.refl = p.refr = 0;
-        p.dir = d0;
-        HitRec g;
-        p.primary_hit = load_grec(a.gbuf + pixel, g);
-        if (kTraced) {  // vertex 0's ray queries; its shading runs in the next shading launch
-            ray = p.primary_hit;
-            if (ray) {
-                gr = rng_fork(p.rng, wave_slot(a, pixel) ? wave_slot(a, pixel) + kMtN : nullptr);
-                wave_gen(a, tab, 0, g, p, pixel, gr, gv, bray, ro, rd, k0);
-                nc = a.n_cand;
-            }
-        } else if (p.primary_hit) {
-            wave_vertex<kCount, kDebug, kTraced>(a, tab, 0, g, p, wl + lane, pixel, st, query, q, cq, ray, ro, rd,
-                                                 next);
-        }
-        if (!p.primary_hit && a.s.environment >= 0 && a.ibl_bg)  // a miss sees the environment (PathIntegrator::Li, path_integrator.cpp:48-209)
-            p.total = rgb2spec(tab, env_radiance(a.s, d0));
-        if (a.spatial_count > 0) {  // the spatial launch's record of the primary vertex
-            a.spatial_aux[2 * pixel] = make_float4(d0.x, d0.y, d0.z, p.primary_hit ? p.utab : -1.0f);
-            // traced frames: the throughput is recorded where vertex 0 is shaded
-            if (!kTraced && p.primary_hit) a.spatial_aux[2 * pixel + 1] = spec4(p.outer);
-        }
-        store_path(a, pixel, p, true);
-        code = query_code(a, pixel);
-        if (kDebug) {
-            a.dbg[pixel].prim_instance = p.primary_hit ? g.inst : -1;
-            a.dbg[pixel].prim_triangle = p.primary_hit ? g.tri : -1;
-            a.dbg[pixel].prim_t = p.primary_hit ? g.depth : 0.0f;
-        }
-    }
-    const unsigned int qi = wave_append(cnt + 2, query, lane);
-    if (query) {
-        args[0].queue[qi] = q;
-        args[0].wcq[qi] = cq;
-    }
-    if (kTraced) {  // vertex 0's live list, bounce rays and escape rays
-        const unsigned int ai = wave_append(cnt + 4, ray, lane);
-        if (ray) alive_list(args[0], 0)[ai] = code;
-        wave_emit_escapes(cnt + 6, args[0].wesq, tab, gv, gr, k0, nc, code, lane);
-        ray = bray;
-    }
-    const unsigned int ri = wave_append(cnt + 0, ray, lane);
-    if (ray) {
-        args[0].wray[2 * ri] = ro;
-        args[0].wray[2 * ri + 1] = rd;
-    }
-    if (kCount) {
-        unsigned long long *c = a.counters;
-        wave_add(c + 1, st.bounce);
-        wave_add(c + 2, query ? 1ULL : 0ULL);
-        wave_add(c + 3, st.cand);
-        wave_add(c + 4, st.cand_skipped);
-        wave_add(c + 5, st.ts.nodes);
-        wave_add(c + 6, st.ts.tris);
-        wave_add(c + 7, st.ts.instances);
-    }
-}
-
-// Spatial reuse of the primary vertex (deferred.h: spatial_reuse_pixel), after wave_shade0 has
-// stored every pixel's post-temporal reservoir. Per shaded pixel: the primary vertex's emission
-// plus the reused direct light (rgb2[2p]) and plus nothing (rgb2[2p+1]), as spectra, and flags[p]
-// = (selected domain slot + 1) << 1; the winner's shadow ray (none when it was tested in an
-// earlier frame) joins vertex 0's shadow queue, whose traversal sets bit 0 of the flags when it
-// is occluded.
-template <bool kCount>
-__global__ void __launch_bounds__(kLanes) wave_spatial_kernel(const FrameArgs *__restrict__ args,
-                                                              const int *__restrict__ tile_base, int n_views) {
-    __shared__ int dl[(kMaxSpatial + 1) * kLanes];
-    __shared__ float cl[(kMaxSpatial + 1) * kLanes];
-    HDR_TILE(args, tile_base, n_views)
-    bool query = false;
-    ShadowQuery q;
-    if (valid && a.spatial_count > 0) {
-        const int pixel = y * a.W + x;
-        Vertex vq;
-        SpecTab tab;
-        if (load_spatial_vertex(a, a.gbuf, pixel, vq, tab)) {
-            V3 qo = v3s(0.0f), qd = v3s(0.0f);
-            float qtmax = 0.0f;
-            int ssel = -1;
-            const Spec direct = spatial_reuse_pixel<false>(a, a.gbuf, x, y, vq, tab, dl + lane, cl + lane, query, qo,
-                                                           qd, qtmax, ssel);
-            const Spec e = rgb2spec(tab, mv3(vq.c.m->emission));
-            a.rgb2[2 * pixel] = spec4(e + direct);
-            a.rgb2[2 * pixel + 1] = spec4(e + spec(0.0f));
-            a.flags[pixel] = uint32_t(ssel + 1) << 1;
-            if (query) {
-                q.o = make_float4(qo.x, qo.y, qo.z, qtmax);
-                q.d = make_float4(qd.x, qd.y, qd.z, __uint_as_float(query_code(a, pixel)));
-            }
-        }
-    }
-    const unsigned int qi = wave_append(args[0].wcnt + 2, query, lane);
-    if (query) args[0].queue[qi] = q;
-    if (kCount) wave_add(a.counters + 2, query ? 1ULL : 0ULL);
-}
-
-// Vertex `bounce` >= 1 of every live path (the codes wave_trace appended).
-template <bool kCount, bool kDebug, int kOcc = 1>
-__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
-wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce) {
-    extern __shared__ double wl[];
-    const int lane = threadIdx.x;
-    const int par = bounce & 1;
-    unsigned int *cnt = args[0].wcnt;
-    const unsigned int n = cnt[4 + par];
-    if (blockIdx.x == 0 && lane == 0) cnt[4 + (par ^ 1)] = 0u;  // live paths of the next vertex: appended by wave_trace
-    PixelStats st = {};
-    unsigned int nq = 0;
-    for (unsigned int b0 = blockIdx.x * kLanes; b0 < n; b0 += gridDim.x * kLanes) {
-        const unsigned int i = b0 + unsigned(lane);
-        bool query = false, ray = false;
-        ShadowQuery q;
-        float4 cq = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ro = cq, rd = cq;
-        if (i < n) {
-            const uint32_t code = alive_list(args[0], bounce)[i];
-            const FrameArgs &a = args[code >> 24];
-            const int pixel = int(code & 0xffffffu);
-            WavePath<Mt64Deep> p;
-            load_path(a, pixel, p);
-            const SpecTab tab = make_tab(p.utab);
-            HitRec h, next;
-            (void)load_grec(a.whit + pixel, h);
-            wave_vertex<kCount, kDebug, false>(a, tab, bounce, h, p, wl + lane, pixel, st, query, q, cq, ray, ro, rd,
-                                               next);
-            store_path(a, pixel, p, false);
-        }
-        const unsigned int qi = wave_append(cnt + 2 + par, query, lane);
-        if (query) {
-            args[0].queue[qi] = q;
-            args[0].wcq[qi] = cq;
-        }
-        const unsigned int ri = wave_append(cnt + par, ray, lane);
-        if (ray) {
-            args[0].wray[2 * ri] = ro;
-            args[0].wray[2 * ri + 1] = rd;
-        }
-        nq += query ? 1u : 0u;
-    }
-    if (kCount) {
-        unsigned long long *c = args[0].counters;
-        wave_add(c + 1, st.bounce);
-        wave_add(c + 2, nq);
-        wave_add(c + 3, st.cand);
-        wave_add(c + 4, st.cand_skipped);
-        wave_add(c + 5, st.ts.nodes);
-        wave_add(c + 6, st.ts.tris);
-        wave_add(c + 7, st.ts.instances);
-    }
-}
-
-// Traced frames (wavefront.h: wave_gen): launch v (1 .. depth) shades vertex u = v - 1 of every
-// path in vertex u's live list -- its ray queries were answered by the traversal launch before --
-// and, for the paths that go on, emits vertex v's ray queries (bounce ray, escape rays) and
-// appends the path to vertex v's live list. Vertex u's shadow query joins the queue of parity u.
-template <bool kCount, bool kDebug, int kOcc = 1>
-__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
-wave_shade_traced_kernel(const FrameArgs *__restrict__ args, int v) {
-    extern __shared__ double wl[];
-    const int lane = threadIdx.x;
-    const int u = v - 1;
-    unsigned int *cnt = args[0].wcnt;
-    const unsigned int n = cnt[4 + (u & 1)];
-    PixelStats st = {};
-    unsigned int nq = 0;
-    for (unsigned int b0 = blockIdx.x * kLanes; b0 < n; b0 += gridDim.x * kLanes) {
-        const unsigned int i = b0 + unsigned(lane);
-        bool query = false, cont = false, bray = false;
-        ShadowQuery q;
-        float4 cq = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ro = cq, rd = cq, bo = cq, bd = cq;
-        uint32_t code = 0;
-        Mt64Deep gr;
-        Vertex gv;
-        SpecTab tab;
-        int k0 = 0, nc = 0;
-        if (i < n) {
-            code = alive_list(args[0], u)[i];
-            const FrameArgs &a = args[code >> 24];
-            const int pixel = int(code & 0xffffffu);
-            WavePath<Mt64Deep> p;
-            load_path(a, pixel, p, u == 0);
-            tab = make_tab(p.utab);
-            HitRec h, next;
-            (void)load_grec((u == 0 ? a.gbuf : traced_hits(a, u)) + pixel, h);
-            wave_vertex<kCount, kDebug, true>(a, tab, u, h, p, wl + lane, pixel, st, query, q, cq, cont, ro, rd,
-                                              next);
-            if (u == 0 && a.spatial_count > 0) a.spatial_aux[2 * pixel + 1] = spec4(p.outer);
-            if (cont) {
-                uint64_t *slot = wave_slot(a, pixel);
-                gr = rng_fork(p.rng, slot ? slot + kMtN : nullptr);
-                wave_gen(a, tab, v, next, p, pixel, gr, gv, bray, bo, bd, k0);
-                nc = a.n_cand;
-            }
-            store_path(a, pixel, p, u == 0);
-        }
-        const unsigned int qi = wave_append(cnt + 2 + (u & 1), query, lane);
-        if (query) {
-            args[0].queue[qi] = q;
-            args[0].wcq[qi] = cq;
-        }
-        wave_emit_escapes(cnt + 6 + (v & 1), args[0].wesq, tab, gv, gr, k0, nc, code, lane);
-        const unsigned int ri = wave_append(cnt + (v & 1), bray, lane);
-        if (bray) {
-            args[0].wray[2 * ri] = bo;
-            args[0].wray[2 * ri + 1] = bd;
-        }
-        const unsigned int ai = wave_append(cnt + 4 + (v & 1), cont, lane);
-        if (cont) alive_list(args[0], v)[ai] = code;
-        nq += query ? 1u : 0u;
-    }
-    if (kCount) {
-        unsigned long long *c = args[0].counters;
-        wave_add(c + 2, nq);
-        wave_add(c + 3, st.cand);
-        wave_add(c + 4, st.cand_skipped);
-    }
-}
-
-// Shadow ray i of the queue (vertex `vtx`): an unoccluded winner's contribution joins the path
-// total (a spatial winner's colour is already staged); an occluded primary winner keeps zero
-// throughput in its reservoir (ris_direct_light_integrator.cpp:163-175).
-template <bool kCount, bool kLds>
-__device__ __forceinline__ void wave_shadow(const FrameArgs *__restrict__ args, unsigned int i, int vtx,
-                                            TraceStats &ts, int *ring_tr, int *ring_br) {
-    const ShadowQuery q = args[0].queue[i];
-    const uint32_t code = __float_as_uint(q.d.w);
-    const FrameArgs &a = args[code >> 24];
-    const int pixel = int(code & 0xffffffu);
-    if (!any_hit<kCount, kLds>(a.s, v3(q.o.x, q.o.y, q.o.z), v3(q.d.x, q.d.y, q.d.z), q.o.w, ts, ring_tr, ring_br)) {
-        if (!(vtx == 0 && a.spatial_count > 0)) {
-            float4 *t = a.wstate + 4 * wave_pixels(a) + pixel;
-            const Spec tot = spec_of(*t) + spec_of(args[0].wcq[i]);
-            *t = spec4(tot);
-        }
-    } else if (vtx == 0 && a.use_res) {
-        if (a.spatial_count > 0) a.flags[pixel] |= 1u;  // the spatial winner is occluded
-        hdr_reservoir &r = a.res[pixel];
-        r.throughput[0] = r.throughput[1] = r.throughput[2] = r.throughput[3] = 0.0f;
-    }
-}
-
-// The bounce rays (closest hit: the next vertex, appended to the live list when there is one) and
-// the shadow rays (any hit: an unoccluded winner's contribution joins the path total; an occluded
-// primary winner keeps zero throughput in its reservoir, ris_direct_light_integrator.cpp:163-175)
-// of vertex `bounce`.
-template <bool kCount, int kOcc = 1, bool kLds = false>
-__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
-wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce) {
-    HDR_RING_LDS
-    const int lane = threadIdx.x;
-    const int par = bounce & 1;
-    unsigned int *cnt = args[0].wcnt;
-    const unsigned int nr = cnt[par], ns = cnt[2 + par];
-    if (blockIdx.x == 0 && lane == 0) {
-        cnt[par ^ 1] = 0u;        // rays of the next vertex
-        cnt[2 + (par ^ 1)] = 0u;  // shadow queries of the next vertex
-    }
-    TraceStats ts = {};
-    unsigned int nb = 0;
-    for (unsigned int b0 = blockIdx.x * kLanes; b0 < nr; b0 += gridDim.x * kLanes) {
-        const unsigned int i = b0 + unsigned(lane);
-        bool hit = false;
-        uint32_t code = 0;
-        if (i < nr) {
-            const float4 o4 = args[0].wray[2 * i], d4 = args[0].wray[2 * i + 1];
-            code = __float_as_uint(o4.w);
-            const FrameArgs &a = args[code >> 24];
-            const int pixel = int(code & 0xffffffu);
-            const V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
-            RawHit rh;
-            hit = closest_hit<kCount, kLds>(a.s, o, d, rh, ts, ring_tr + lane, ring_tt + lane, ring_br + lane,
-                                            ring_bt + lane);
-            if (hit) store_grec(a.whit + pixel, true, make_hit_record(a.s, o, d, rh));
-            ++nb;
-        }
-        const unsigned int ai = wave_append(cnt + 4 + (par ^ 1), hit, lane);
-        if (hit) alive_list(args[0], bounce + 1)[ai] = code;
-    }
-    for (unsigned int i = blockIdx.x * kLanes + unsigned(lane); i < ns; i += gridDim.x * kLanes)
-        wave_shadow<kCount, kLds>(args, i, bounce, ts, ring_tr + lane, ring_br + lane);
-    if (kCount) {
-        unsigned long long *c = args[0].counters;
-        wave_add(c + 1, nb);
-        wave_add(c + 5, ts.nodes);
-        wave_add(c + 6, ts.tris);
-        wave_add(c + 7, ts.instances);
-    }
-}
-
-// Traced frames: traversal launch v (0 .. depth) after shading launch v -- vertex v's bounce rays
-// (closest hit: vertex v+1's hit record, or a miss record) and escape rays (any hit up to 1e30:
-// the candidate's bit of the pixel's escape mask when none), and vertex v-1's shadow rays.
-template <bool kCount, int kOcc = 1, bool kLds = false>
-__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
-wave_trace_traced_kernel(const FrameArgs *__restrict__ args, int v) {
-    HDR_RING_LDS
-    const int lane = threadIdx.x;
-    const int par = v & 1;
-    unsigned int *cnt = args[0].wcnt;
-    const unsigned int nr = cnt[par], ne = cnt[6 + par], ns = v > 0 ? cnt[2 + (par ^ 1)] : 0u;
-    if (blockIdx.x == 0 && lane == 0) {  // the counts the next shading launch appends to
-        cnt[par ^ 1] = 0u;        // bounce rays of vertex v+1
-        cnt[6 + (par ^ 1)] = 0u;  // escape rays of vertex v+1
-        cnt[2 + par] = 0u;        // shadow queries of vertex v
-        cnt[4 + (par ^ 1)] = 0u;  // live paths of vertex v+1
-    }
-    TraceStats ts = {};
-    unsigned int nb = 0;
-    for (unsigned int i = blockIdx.x * kLanes + unsigned(lane); i < nr; i += gridDim.x * kLanes) {
-        const float4 o4 = args[0].wray[2 * i], d4 = args[0].wray[2 * i + 1];
-        const uint32_t code = __float_as_uint(o4.w);
-        const FrameArgs &a = args[code >> 24];
-        const int pixel = int(code & 0xffffffu);
-        const V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
-        RawHit rh;
-        GRec *dst = traced_hits(a, v + 1) + pixel;
-        if (closest_hit<kCount, kLds>(a.s, o, d, rh, ts, ring_tr + lane, ring_tt + lane, ring_br + lane,
-                                      ring_bt + lane))
-            store_grec(dst, true, make_hit_record(a.s, o, d, rh));
-        else
-            store_grec_miss(dst);
-        ++nb;
-    }
-    for (unsigned int i = blockIdx.x * kLanes + unsigned(lane); i < ne; i += gridDim.x * kLanes) {
-        const float4 o4 = args[0].wesq[2 * i], d4 = args[0].wesq[2 * i + 1];
-        const uint32_t code = __float_as_uint(o4.w);
-        const FrameArgs &a = args[code >> 24];
-        const int pixel = int(code & 0xffffffu);
-        if (!any_hit<kCount, kLds>(a.s, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), 1e30f, ts, ring_tr + lane,
-                                   ring_br + lane))
-            atomicOr(a.wesc + pixel, 1u << __float_as_int(d4.w));
-    }
-    for (unsigned int i = blockIdx.x * kLanes + unsigned(lane); i < ns; i += gridDim.x * kLanes)
-        wave_shadow<kCount, kLds>(args, i, v - 1, ts, ring_tr + lane, ring_br + lane);
-    if (kCount) {
-        unsigned long long *c = args[0].counters;
-        wave_add(c + 1, nb);
-        wave_add(c + 5, ts.nodes);
-        wave_add(c + 6, ts.tris);
-        wave_add(c + 7, ts.instances);
-    }
-}
-
-// The path's radiance (outer throughput x total) to RGB, exposure, accumulation
-// (integration_pass.cpp:236-243, accumulation_pass.cpp:98-106).
-template <bool kCount, bool kDebug>
-__global__ void __launch_bounds__(kLanes) wave_resolve_kernel(const FrameArgs *__restrict__ args,
-                                                              const int *__restrict__ tile_base, int n_views) {
-    HDR_TILE(args, tile_base, n_views)
-    uint32_t draws = 0;
-    if (valid) {
-        const int pixel = y * a.W + x;
-        const size_t P = wave_pixels(a);
-        const float4 *w = a.wstate;
-        const float4 r2 = w[2 * P + pixel], r5 = w[5 * P + pixel], r7 = w[7 * P + pixel];
-        const SpecTab tab = make_tab(r5.w);
-        const bool hit = (__float_as_uint(r7.x) & kWavePrimaryHit) != 0;
-        uint32_t shash = __float_as_uint(r7.z);
-        Spec L;
-        if (hit && a.spatial_count > 0) {  // outer * ((emission + reused direct) + later vertices)
-            const uint32_t f = a.flags[pixel];
-            const Spec e = spec_of(a.rgb2[2 * pixel + (f & 1u)]);
-            L = spec_of(w[6 * P + pixel]) * (e + spec_of(w[4 * P + pixel]));
-            if (kDebug) shash = fnv(shash, uint32_t(int(f >> 1) - 1));
-        } else {
-            L = hit ? spec_of(w[6 * P + pixel]) * spec_of(w[4 * P + pixel]) : spec_of(w[4 * P + pixel]);
-        }
-        V3 rgb = spec2rgb(tab, L);
-        if (a.has_cam && a.cam.enable_exposure) rgb = rgb * a.exposure_mul;
-        const float4 c4 = make_float4(rgb.x, rgb.y, rgb.z, 1.0f);
-        if (a.firefly) {
-            a.frame_rgb[pixel] = c4;
-        } else {
-            float4 acc = a.accum[pixel];
-            acc.x += c4.x;
-            acc.y += c4.y;
-            acc.z += c4.z;
-            acc.w += c4.w;
-            a.accum[pixel] = acc;
-            a.color[pixel] = make_float4(acc.x * a.inv_n, acc.y * a.inv_n, acc.z * a.inv_n, acc.w * a.inv_n);
-        }
-        draws = __float_as_uint(r2.z);
-        if (kDebug) {
-            hdr_pixel_debug &d = a.dbg[pixel];
-            d.draws = draws;
-            d.light_hash = __float_as_uint(r7.y);
-            d.select_hash = shash;
-            d.n_light_picks = __float_as_int(r7.w);
-            d.sample_rgb[0] = rgb.x;
-            d.sample_rgb[1] = rgb.y;
-            d.sample_rgb[2] = rgb.z;
-        }
-    }
-    if (kCount) wave_add(a.counters + 8, draws);
 }
 
 // Out-of-place 3x3 firefly clamp (the reference's GPU accumulation shader semantics,
@@ -1010,25 +121,6 @@ __global__ void rng_kernel(const uint32_t *seeds, int n, int nd, uint64_t *raw, 
     }
 }
 
-__global__ void intersect_kernel(DeviceScene s, const float *o, const float *d, int n, int *inst, int *tri,
-                                 float *t) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    TraceStats ts = {};
-    RawHit h;
-    const V3 oo = v3(o[3 * i], o[3 * i + 1], o[3 * i + 2]);
-    const V3 dd = v3(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
-    if (trace_closest<false>(s, oo, dd, h, ts)) {
-        const HitRec r = make_hit_record(s, oo, dd, h);
-        inst[i] = r.inst;
-        tri[i] = r.tri;
-        t[i] = r.depth;
-    } else {
-        inst[i] = -1;
-        tri[i] = -1;
-        t[i] = 0.0f;
-    }
-}
 
 // ============================================================================ host side
 
@@ -2485,3 +1577,4 @@ hdr_status hdr_batch_autotune(hdr_batch *b, int32_t n, hdr_view *const *views, c
 }
 
 }  // extern "C"
+

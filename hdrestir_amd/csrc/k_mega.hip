@@ -1,0 +1,89 @@
+// k_mega.hip -- The integration megakernel (IntegrationPass + fused accumulation).
+#include "kcommon.h"
+
+namespace hdr {
+
+// IntegrationPass (integration_pass.cpp:208-243) + fused accumulation for one pixel.
+template <bool kCount, bool kDebug>
+__device__ __forceinline__ void integrate_pixel(const FrameArgs &a, const GRec *gbuf, int x, int y, double *wlane,
+                                                PixelStats &st, uint32_t &draws) {
+    const int pixel = y * a.W + x;
+    const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
+    Tracer<kCount, false> tracer{a.s, st};
+    HitRec g;
+    const bool gHit = load_grec(gbuf + pixel, g);
+
+    // IntegrationPass (integration_pass.cpp:208-243)
+    Mt64 rng;
+    uint64_t *slot = a.spill ? a.spill + size_t(pixel) * (2 * kMtN) : nullptr;
+    rng.init(seed, slot);
+    const float px = float(x) + rng.next_float();
+    const float py = float(y) + rng.next_float();
+    V3 ro, rd;
+    camera_ray(a, px, py, &rng, ro, rd);
+    const SpecTab tab = make_tab(rng.next_float());
+    Trace dbg = {2166136261u, 2166136261u, 0};
+    PixelCtx<kCount, kDebug, false> p{a, tab, rng, tracer, st, dbg, wlane, slot ? slot + kMtN : nullptr};
+    const Spec L = path_li(p, rd, gHit, g, pixel);
+    V3 rgb = spec2rgb(tab, L);
+    if (a.has_cam && a.cam.enable_exposure) rgb = rgb * a.exposure_mul;
+    const float4 c4 = make_float4(rgb.x, rgb.y, rgb.z, 1.0f);
+    if (a.firefly) {
+        a.frame_rgb[pixel] = c4;
+    } else {
+        // AccumulationPass (accumulation_pass.cpp:98-106): accum += c; out = accum * invN
+        float4 acc = a.accum[pixel];
+        acc.x += c4.x;
+        acc.y += c4.y;
+        acc.z += c4.z;
+        acc.w += c4.w;
+        a.accum[pixel] = acc;
+        a.color[pixel] = make_float4(acc.x * a.inv_n, acc.y * a.inv_n, acc.z * a.inv_n, acc.w * a.inv_n);
+    }
+    if (kDebug) {
+        hdr_pixel_debug d;
+        d.draws = rng.k;
+        d.light_hash = dbg.lhash;
+        d.select_hash = dbg.shash;
+        d.n_light_picks = dbg.picks;
+        d.prim_instance = gHit ? g.inst : -1;
+        d.prim_triangle = gHit ? g.tri : -1;
+        d.prim_t = gHit ? g.depth : 0.0f;
+        d.sample_rgb[0] = rgb.x;
+        d.sample_rgb[1] = rgb.y;
+        d.sample_rgb[2] = rgb.z;
+        a.dbg[pixel] = d;
+    }
+    draws = rng.k;
+}
+
+// IntegrationPass + (fused) AccumulationPass for one 8x8 pixel tile per wave.
+template <bool kCount, bool kDebug>
+__global__ void __launch_bounds__(kLanes) integrate_kernel(const FrameArgs *__restrict__ args,
+                                                           const int *__restrict__ tile_base, int n_views) {
+    extern __shared__ double wl[];
+    HDR_TILE(args, tile_base, n_views)
+    PixelStats st = {};
+    uint32_t draws = 0;
+    if (valid) integrate_pixel<kCount, kDebug>(a, a.gbuf, x, y, wl + lane, st, draws);
+    if (kCount) {
+        unsigned long long *c = a.counters;
+        wave_add(c + 1, st.bounce);
+        wave_add(c + 2, st.shadow);
+        wave_add(c + 3, st.cand);
+        wave_add(c + 4, st.cand_skipped);
+        wave_add(c + 5, st.ts.nodes);
+        wave_add(c + 6, st.ts.tris);
+        wave_add(c + 7, st.ts.instances);
+        wave_add(c + 8, draws);
+    }
+}
+
+// ---------------------------------------------------------------------------- instantiations
+// (launched from hdrgpu.hip, which sees the declarations in kernels.h)
+template __global__ void integrate_kernel<true, true>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void integrate_kernel<true, false>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void integrate_kernel<false, true>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void integrate_kernel<false, false>(const FrameArgs *__restrict__, const int *__restrict__, int);
+
+}  // namespace hdr

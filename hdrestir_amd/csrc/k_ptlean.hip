@@ -1,0 +1,34 @@
+// k_ptlean.hip -- The inlined path tracer of opaque frames.
+#include "kcommon.h"
+
+namespace hdr {
+
+// The path tracer on opaque frames whose pixels stay below RNG output 311 (pathlean.h).
+template <bool kCount, bool kDebug>
+__global__ void __launch_bounds__(kLanes) pt_lean_kernel(const FrameArgs *__restrict__ args,
+                                                         const int *__restrict__ tile_base, int n_views) {
+    HDR_TILE(args, tile_base, n_views)
+    PixelStats st = {};
+    uint32_t draws = 0;
+    if (valid) pt_lean_pixel<kCount, kDebug>(a, x, y, st, draws);
+    if (kCount) {
+        unsigned long long *c = a.counters;
+        wave_add(c + 1, st.bounce);
+        wave_add(c + 2, st.shadow);
+        wave_add(c + 3, st.cand);
+        wave_add(c + 4, st.cand_skipped);
+        wave_add(c + 5, st.ts.nodes);
+        wave_add(c + 6, st.ts.tris);
+        wave_add(c + 7, st.ts.instances);
+        wave_add(c + 8, draws);
+    }
+}
+
+// ---------------------------------------------------------------------------- instantiations
+// (launched from hdrgpu.hip, which sees the declarations in kernels.h)
+template __global__ void pt_lean_kernel<true, true>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void pt_lean_kernel<true, false>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void pt_lean_kernel<false, true>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void pt_lean_kernel<false, false>(const FrameArgs *__restrict__, const int *__restrict__, int);
+
+}  // namespace hdr

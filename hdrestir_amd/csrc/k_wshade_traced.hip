@@ -1,0 +1,84 @@
+// k_wshade_traced.hip -- The wavefront's shading launches of traced frames.
+#include "kcommon.h"
+
+namespace hdr {
+
+// Traced frames (wavefront.h: wave_gen): launch v (1 .. depth) shades vertex u = v - 1 of every
+// path in vertex u's live list -- its ray queries were answered by the traversal launch before --
+// and, for the paths that go on, emits vertex v's ray queries (bounce ray, escape rays) and
+// appends the path to vertex v's live list. Vertex u's shadow query joins the queue of parity u.
+template <bool kCount, bool kDebug, int kOcc = 1>
+__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
+wave_shade_traced_kernel(const FrameArgs *__restrict__ args, int v) {
+    extern __shared__ double wl[];
+    const int lane = threadIdx.x;
+    const int u = v - 1;
+    unsigned int *cnt = args[0].wcnt;
+    const unsigned int n = cnt[4 + (u & 1)];
+    PixelStats st = {};
+    unsigned int nq = 0;
+    for (unsigned int b0 = blockIdx.x * kLanes; b0 < n; b0 += gridDim.x * kLanes) {
+        const unsigned int i = b0 + unsigned(lane);
+        bool query = false, cont = false, bray = false;
+        ShadowQuery q;
+        float4 cq = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ro = cq, rd = cq, bo = cq, bd = cq;
+        uint32_t code = 0;
+        Mt64Deep gr;
+        Vertex gv;
+        SpecTab tab;
+        int k0 = 0, nc = 0;
+        if (i < n) {
+            code = alive_list(args[0], u)[i];
+            const FrameArgs &a = args[code >> 24];
+            const int pixel = int(code & 0xffffffu);
+            WavePath<Mt64Deep> p;
+            load_path(a, pixel, p, u == 0);
+            tab = make_tab(p.utab);
+            HitRec h, next;
+            (void)load_grec((u == 0 ? a.gbuf : traced_hits(a, u)) + pixel, h);
+            wave_vertex<kCount, kDebug, true>(a, tab, u, h, p, wl + lane, pixel, st, query, q, cq, cont, ro, rd,
+                                              next);
+            if (u == 0 && a.spatial_count > 0) a.spatial_aux[2 * pixel + 1] = spec4(p.outer);
+            if (cont) {
+                uint64_t *slot = wave_slot(a, pixel);
+                gr = rng_fork(p.rng, slot ? slot + kMtN : nullptr);
+                wave_gen(a, tab, v, next, p, pixel, gr, gv, bray, bo, bd, k0);
+                nc = a.n_cand;
+            }
+            store_path(a, pixel, p, u == 0);
+        }
+        const unsigned int qi = wave_append(cnt + 2 + (u & 1), query, lane);
+        if (query) {
+            args[0].queue[qi] = q;
+            args[0].wcq[qi] = cq;
+        }
+        wave_emit_escapes(cnt + 6 + (v & 1), args[0].wesq, tab, gv, gr, k0, nc, code, lane);
+        const unsigned int ri = wave_append(cnt + (v & 1), bray, lane);
+        if (bray) {
+            args[0].wray[2 * ri] = bo;
+            args[0].wray[2 * ri + 1] = bd;
+        }
+        const unsigned int ai = wave_append(cnt + 4 + (v & 1), cont, lane);
+        if (cont) alive_list(args[0], v)[ai] = code;
+        nq += query ? 1u : 0u;
+    }
+    if (kCount) {
+        unsigned long long *c = args[0].counters;
+        wave_add(c + 2, nq);
+        wave_add(c + 3, st.cand);
+        wave_add(c + 4, st.cand_skipped);
+    }
+}
+
+// ---------------------------------------------------------------------------- instantiations
+// (launched from hdrgpu.hip, which sees the declarations in kernels.h)
+template __global__ void wave_shade_traced_kernel<true, true, 1>(const FrameArgs *__restrict__, int);
+template __global__ void wave_shade_traced_kernel<true, true, 3>(const FrameArgs *__restrict__, int);
+template __global__ void wave_shade_traced_kernel<true, false, 1>(const FrameArgs *__restrict__, int);
+template __global__ void wave_shade_traced_kernel<true, false, 3>(const FrameArgs *__restrict__, int);
+template __global__ void wave_shade_traced_kernel<false, true, 1>(const FrameArgs *__restrict__, int);
+template __global__ void wave_shade_traced_kernel<false, true, 3>(const FrameArgs *__restrict__, int);
+template __global__ void wave_shade_traced_kernel<false, false, 1>(const FrameArgs *__restrict__, int);
+template __global__ void wave_shade_traced_kernel<false, false, 3>(const FrameArgs *__restrict__, int);
+
+}  // namespace hdr

@@ -18,20 +18,42 @@ LIB_PATH = os.path.join(_HERE, "libhdrgpu.so")
 CSRC = os.path.join(_HERE, "csrc")
 _lib = None
 
-HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["hdrgpu.hip", "host_scene.cpp", "server.cpp"]
+# the rendering kernels are split over several translation units so they compile in parallel
+SOURCES = ["k_wshade.hip", "k_wshade_traced.hip", "k_mega.hip", "k_ptlean.hip", "k_wave0.hip", "k_trace.hip",
+           "k_deferred.hip", "hdrgpu.hip",
+           "host_scene.cpp", "server.cpp"]
+OBJ_DIR = os.path.join(_HERE, "_obj")
 ADAPTER_SMOKE = os.path.join(_HERE, "host", "build", "adapter_smoke")
 
 
-def build(verbose=False):
-    """Compiles libhdrgpu.so for gfx950 in-tree (hipcc cross-compiles without a GPU)."""
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+def build(verbose=False, jobs=None):
+    """Compiles libhdrgpu.so for gfx950 in-tree (hipcc cross-compiles without a GPU): every source
+    to an object in parallel, then one link."""
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    jobs = jobs or max(1, min(len(SOURCES), os.cpu_count() or 1))
+    pending = list(SOURCES)
+    running, objs = [], []
+    while pending or running:
+        while pending and len(running) < jobs:
+            s = pending.pop(0)
+            obj = os.path.join(OBJ_DIR, os.path.splitext(s)[0] + ".o")
+            objs.append(obj)
+            cmd = ["/opt/rocm/bin/hipcc"] + HIPCC_FLAGS + ["-c", "-o", obj, os.path.join(CSRC, s)]
+            running.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+        s, p = running.pop(0)
+        _, err = p.communicate()
+        if p.returncode != 0:
+            for _, q in running:
+                q.kill()
+                q.wait()
+            raise RuntimeError(f"hipcc failed on {s}:\n" + err[-4000:])
     tmp = LIB_PATH + ".tmp"
-    cmd = ["/opt/rocm/bin/hipcc"] + HIPCC_FLAGS + ["-o", tmp] + srcs
-    r = subprocess.run(cmd, capture_output=True, text=True)
+    r = subprocess.run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-shared", "-o", tmp] + objs,
+                       capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stderr[-4000:])
+        raise RuntimeError("hipcc link failed:\n" + r.stderr[-4000:])
     os.replace(tmp, LIB_PATH)
     # the C++20 pass adapter's smoke program, linked against the library just built
     host = os.path.join(_HERE, "host")
