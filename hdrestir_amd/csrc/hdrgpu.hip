@@ -4,10 +4,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -121,12 +123,25 @@ __global__ void rng_kernel(const uint32_t *seeds, int n, int nd, uint64_t *raw, 
     }
 }
 
+// Scratch of the allocation-free entry point hdr_rng_raw_noalloc: the 312-word spill slots of the
+// streams drawn past output 311, one slot per seed (at most kRngScratchBytes / 2496 = 13443 seeds).
+// Each device holds its own copy until the process ends. Host code reaches it only through the
+// address hipGetSymbolAddress returns on that device (g_scratch, set by hdr_context_create), never
+// through this name.
+constexpr size_t kRngScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) uint64_t scratch[kRngScratchBytes / sizeof(uint64_t)];
+
 
 // ============================================================================ host side
 
 namespace {
 
 thread_local std::string g_err;
+
+// Per-device address of `scratch` (set once per device, with that device current).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+std::atomic<uint64_t *> g_scratch[kMaxDevices];
 
 hdr_status fail(hdr_status s, const std::string &msg) {
     g_err = msg;
@@ -332,6 +347,15 @@ hdr_status hdr_context_create(int32_t device, hdr_context **out) {
     HIP_TRY(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(HDR_ERR_INVALID_ARGUMENT, "invalid device index");
     HIP_TRY(hipSetDevice(device));
+    if (device < kMaxDevices) {  // the device address of hdr_rng_raw_noalloc's scratch, once per device
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch[device].load()) {
+            void *p = nullptr;
+            HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            if (!p) return fail(HDR_ERR_DEVICE, "hipGetSymbolAddress returned a null scratch address");
+            g_scratch[device].store(static_cast<uint64_t *>(p));
+        }
+    }
     auto *c = new (std::nothrow) hdr_context();
     if (!c) return fail(HDR_ERR_OUT_OF_MEMORY, "host allocation failed");
     c->device = device;
@@ -1127,6 +1151,26 @@ hdr_status hdr_rng_raw(hdr_context *ctx, const uint32_t *seeds, int32_t n, int32
 hdr_status hdr_rng_float(hdr_context *ctx, const uint32_t *seeds, int32_t n, int32_t n_draws, float *out) {
     if (!out) return fail(HDR_ERR_INVALID_ARGUMENT, "out is null");
     return rng_common(ctx, seeds, n, n_draws, nullptr, out);
+}
+
+// hdr_rng_raw on device buffers, with the spill slots in the static scratch: no allocation, no
+// copy, no wait (the launch is queued on `stream`).
+hdr_status hdr_rng_raw_noalloc(hdr_context *ctx, const uint32_t *seeds, int32_t n, int32_t n_draws, uint64_t *out,
+                               void *stream) {
+    if (!ctx || n < 0 || n_draws < 0) return fail(HDR_ERR_INVALID_ARGUMENT, "invalid argument");
+    if (n == 0 || n_draws == 0) return HDR_OK;
+    if (!seeds || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null device pointer");
+    uint64_t *spill = ctx->device < kMaxDevices ? g_scratch[ctx->device].load() : nullptr;
+    if (!spill) return fail(HDR_ERR_UNSUPPORTED, "no scratch was set up for this device");
+    const bool needSpill = n_draws > 311;  // outputs from 312 on come from a materialised state
+    if (needSpill && size_t(n) * kMtN * sizeof(uint64_t) > kRngScratchBytes)
+        return fail(HDR_ERR_CAPACITY, "more than " + std::to_string(kRngScratchBytes / (kMtN * sizeof(uint64_t))) +
+                                          " seeds with n_draws > 311 exceed the 32 MiB scratch");
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(rng_kernel, dim3((n + 63) / 64), dim3(64), 0, static_cast<hipStream_t>(stream), seeds, n,
+                       n_draws, out, nullptr, needSpill ? spill : nullptr);
+    HIP_TRY(hipGetLastError());
+    return HDR_OK;
 }
 
 

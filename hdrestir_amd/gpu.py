@@ -122,6 +122,7 @@ def lib():
             "hdr_scene_intersect": [P, P, P, I, P, P, P],
             "hdr_rng_raw": [P, P, I, I, P],
             "hdr_rng_float": [P, P, I, I, P],
+            "hdr_rng_raw_noalloc": [P, P, I, I, P, P],
             "hdr_view_set_counting": [P, I],
             "hdr_view_read_counters": [P, C.POINTER(abi.FrameCounters)],
         }
@@ -162,6 +163,11 @@ class Context:
         out = np.zeros((len(seeds), n_draws), np.uint64)
         check(lib().hdr_rng_raw(self.h, _p(seeds), len(seeds), n_draws, _p(out)))
         return out
+
+    def rng_raw_noalloc(self, seeds_ptr, n, n_draws, out_ptr, stream=0):
+        """hdr_rng_raw_noalloc on device pointers (queued on `stream`; the caller synchronises)."""
+        check(lib().hdr_rng_raw_noalloc(self.h, C.c_void_p(seeds_ptr), int(n), int(n_draws), C.c_void_p(out_ptr),
+                                        C.c_void_p(stream)))
 
     def rng_float(self, seeds, n_draws):
         seeds = np.ascontiguousarray(seeds, np.uint32)

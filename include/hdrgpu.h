@@ -433,6 +433,17 @@ hdr_status hdr_rng_raw(hdr_context *ctx, const uint32_t *seeds, int32_t n, int32
 /* Same, mapped through uniform_real_distribution<float> (Rng::NextFloat). */
 hdr_status hdr_rng_float(hdr_context *ctx, const uint32_t *seeds, int32_t n, int32_t n_draws, float *out);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc): hdr_rng_raw on device buffers.
+ * seeds (n uint32) and out (n * n_draws uint64, row per seed) are device pointers; the launch is
+ * queued on `stream` (a hipStream_t, NULL = the null stream) and the call returns without waiting.
+ * Streams drawn past output 311 keep their state in a static device scratch of the library, 32 MiB
+ * per device (312 words per seed): with n_draws > 311, n may be at most 13443, else
+ * HDR_ERR_CAPACITY before any launch. Allocates nothing; the scratch is set up by
+ * hdr_context_create. Calls that overlap on one device share that scratch and race. */
+hdr_status hdr_rng_raw_noalloc(hdr_context *ctx, const uint32_t *seeds, int32_t n, int32_t n_draws, uint64_t *out,
+                               void *stream);
+
 /* Work counters of the last frame (rays by class, BVH node visits, triangle tests), filled
  * only when counting is enabled on the view (costs atomics; off for timing runs). */
 typedef struct hdr_frame_counters {
