@@ -13,6 +13,29 @@
 // evaluation order of the reference's primary platform, clang); the reference leaves it
 // unspecified (e.g. SampleGGX(rng.NextFloat(), rng.NextFloat(), r)).
 // Compiled with -ffp-contract=off so no multiply-add is fused.
+//
+// What pins each part (tests/test_oracle.py, tests/test_oracle_pins.py; a planted mistake in any
+// of these functions fails its pin):
+//   Rng / NextFloat ............ the mt19937_64 KAT and the exact canonical mapping
+//   CameraRay .................. pinhole rays in closed form (identity and look-at); CameraRayParams:
+//                                the DoF centre ray meets the focus plane (test_camera.cpp:40-60)
+//   Lambda, RGB<->spectrum ..... the round-trip expectation invariant, flat spectra E-balanced
+//   GGXEval / GGXPdfFull ....... reciprocity, pdf integral <= 1, energy bound
+//   PreviewSampleBounce ........ direction histogram vs the GGX / cosine mixture at normal
+//                                incidence, reported pdf per lobe, unit directions
+//   GetClosure ................. parameter mapping (PreviewSurface and DefaultMaterial)
+//   Light::SampleLight ......... rect: irradiance vs the rectangle view factor, uniform points,
+//                                back faces; point: no 1/d^2 and cone smoothstep; distant: the -Z
+//                                axis; dome: texel histogram vs lum sin(theta), pdf, EvalPdf,
+//                                E[L/pdf] vs quadrature
+//   LightIndex ................. pinned values and equal-width intervals
+//   Reservoir, RandomIndex ..... the KATs of test_reservoir.cpp:18-106, proportional selection,
+//                                the RIS-1D closed form, PowerHeuristic normalisation
+//   Scene::Intersect ........... float64 brute force over every instanced triangle
+//   whole estimators ........... exact linearity in the emitted radiance (exposure + 1 doubles
+//                                every pixel, streams unchanged); MIS and RIS agree in mean
+// Parity unpinned: the last-ulp rounding of real Gf (GetInverse, GfRange3f) and of the reference
+// platforms' libm; the reference's own images (not available).
 
 #include "oracle.h"
 
@@ -937,6 +960,53 @@ struct Light {
     }
 };
 
+// PreviewSurfaceParams (preview_surface_params.h:12-52, textures excluded) from a material desc.
+MaterialParams MaterialFromDesc(const hdr_material_desc &s) {
+    MaterialParams p;
+    auto v3 = [](const float *a) { return V3(a[0], a[1], a[2]); };
+    p.DiffuseColor = v3(s.diffuse_color);
+    p.Metallic = s.metallic;
+    p.Roughness = s.roughness;
+    p.SpecularColor = v3(s.specular_color);
+    p.Specular = s.specular;
+    p.Opacity = s.opacity;
+    p.Ior = s.ior;
+    p.Transmission = s.transmission;
+    p.TransmissionColor = v3(s.transmission_color);
+    p.TransmissionDepth = s.transmission_depth;
+    p.EmissionColor = v3(s.emission_color);
+    p.Emission = s.emission;
+    p.Coat = s.coat;
+    p.CoatColor = v3(s.coat_color);
+    p.CoatRoughness = s.coat_roughness;
+    p.CoatIor = s.coat_ior;
+    p.Sheen = s.sheen;
+    p.SheenColor = v3(s.sheen_color);
+    p.SheenRoughness = s.sheen_roughness;
+    p.Subsurface = s.subsurface;
+    p.SubsurfaceColor = v3(s.subsurface_color);
+    p.ThinWalled = s.thin_walled != 0;
+    return p;
+}
+
+// LightFactory (light_factory.cpp:11-25) + LightParams (light_params.h:15-31) from a light desc.
+void MakeLight(const hdr_light_desc &s, Light &l) {
+    l.type = s.type;
+    l.Color = V3(s.color[0], s.color[1], s.color[2]);
+    l.Intensity = s.intensity;
+    l.Exposure = s.exposure;
+    l.T = M4d::From(s.transform);
+    l.Width = s.width;
+    l.Height = s.height;
+    l.ConeAngle = s.shaping_cone_angle;
+    l.ConeSoftness = s.shaping_cone_softness;
+    if (s.type == HDR_LIGHT_DOME && s.dome_pixels && s.dome_width > 0 && s.dome_height > 0) {
+        l.w = s.dome_width;
+        l.h = s.dome_height;
+        l.PrepareDome(s.dome_pixels);
+    }
+}
+
 // ---------------------------------------------------------------- scene (source/scene/scene.cpp)
 struct Instance {
     int mesh;
@@ -986,52 +1056,9 @@ struct Scene {
         blas.resize(d.num_meshes);
         for (int i = 0; i < d.num_meshes; ++i) blas[i].Build(d.meshes[i]);
         materials.resize(d.num_materials);
-        for (int i = 0; i < d.num_materials; ++i) {
-            const hdr_material_desc &s = d.materials[i];
-            MaterialParams &p = materials[i];
-            auto v3 = [](const float *a) { return V3(a[0], a[1], a[2]); };
-            p.DiffuseColor = v3(s.diffuse_color);
-            p.Metallic = s.metallic;
-            p.Roughness = s.roughness;
-            p.SpecularColor = v3(s.specular_color);
-            p.Specular = s.specular;
-            p.Opacity = s.opacity;
-            p.Ior = s.ior;
-            p.Transmission = s.transmission;
-            p.TransmissionColor = v3(s.transmission_color);
-            p.TransmissionDepth = s.transmission_depth;
-            p.EmissionColor = v3(s.emission_color);
-            p.Emission = s.emission;
-            p.Coat = s.coat;
-            p.CoatColor = v3(s.coat_color);
-            p.CoatRoughness = s.coat_roughness;
-            p.CoatIor = s.coat_ior;
-            p.Sheen = s.sheen;
-            p.SheenColor = v3(s.sheen_color);
-            p.SheenRoughness = s.sheen_roughness;
-            p.Subsurface = s.subsurface;
-            p.SubsurfaceColor = v3(s.subsurface_color);
-            p.ThinWalled = s.thin_walled != 0;
-        }
+        for (int i = 0; i < d.num_materials; ++i) materials[i] = MaterialFromDesc(d.materials[i]);
         lights.resize(d.num_lights);
-        for (int i = 0; i < d.num_lights; ++i) {
-            const hdr_light_desc &s = d.lights[i];
-            Light &l = lights[i];
-            l.type = s.type;
-            l.Color = V3(s.color[0], s.color[1], s.color[2]);
-            l.Intensity = s.intensity;
-            l.Exposure = s.exposure;
-            l.T = M4d::From(s.transform);
-            l.Width = s.width;
-            l.Height = s.height;
-            l.ConeAngle = s.shaping_cone_angle;
-            l.ConeSoftness = s.shaping_cone_softness;
-            if (s.type == HDR_LIGHT_DOME && s.dome_pixels && s.dome_width > 0 && s.dome_height > 0) {
-                l.w = s.dome_width;
-                l.h = s.dome_height;
-                l.PrepareDome(s.dome_pixels);
-            }
-        }
+        for (int i = 0; i < d.num_lights; ++i) MakeLight(d.lights[i], lights[i]);
         for (const Light &l : lights) lightPtrs.push_back(&l);
         environment = d.environment;
         // appendMeshInstance, scene.cpp:178-207
@@ -1406,6 +1433,9 @@ bool DefaultSampleBounce(const ShadingPoint &sp, const V3 &hitPos, Rng &rng, Bou
 }
 
 // ---------------------------------------------------------------- light sampler (uniform_light_sampler.cpp:12-52)
+// UniformLightSampler::ProposeCandidate's index (uniform_light_sampler.cpp:30-33): float product u * n.
+inline size_t LightIndex(float u, size_t n) { return std::min(size_t(u * float(n)), n - 1u); }
+
 struct LightSampler {
     const std::vector<const Light *> *lights;
     std::unordered_set<const Light *> ownSet;
@@ -1428,7 +1458,7 @@ struct LightSampler {
     std::optional<Candidate> Propose(const V3 &hitPos, Rng &rng, PixelTrace &tr) const {
         if (lights->empty()) return std::nullopt;
         const size_t n = lights->size();
-        const size_t idx = std::min(size_t(rng.NextFloat() * float(n)), n - 1u);
+        const size_t idx = LightIndex(rng.NextFloat(), n);
         tr.lightHash = Fnv(tr.lightHash, uint32_t(idx));
         ++tr.lightPicks;
         const Light &light = *(*lights)[idx];
@@ -2825,6 +2855,56 @@ void oracle_sample_bounce(const float *base, float metallic, float roughness, fl
         pdfs[k] = ok ? b.BsdfPdf.value : -1.0f;
         draws[k] = int32_t(rng.draws);
     }
+}
+
+// ILight::SampleLight of one light built from `desc` (as the scene builds it) at hitPos, once per
+// seed (a fresh mt19937_64(seed) each). Per sample: ok (0 = nullopt), Dir, Color, LightNormal, Dist,
+// the pdf value and its space (0 = area, 1 = solid angle), the draws used, and EvalPdf(Dir).
+void oracle_sample_light(const hdr_light_desc *desc, const float *hitPos, const uint32_t *seeds, int32_t n,
+                         int32_t *ok, float *dir, float *color, float *normal, float *dist, float *pdf,
+                         int32_t *pdf_space, int32_t *draws, float *eval_pdf) {
+    Light l;
+    MakeLight(*desc, l);
+    const V3 p(hitPos[0], hitPos[1], hitPos[2]);
+    for (int k = 0; k < n; ++k) {
+        Rng rng(seeds[k]);
+        const auto s = l.SampleLight(p, rng);
+        ok[k] = s ? 1 : 0;
+        draws[k] = int32_t(rng.draws);
+        const Light::Sample z = s ? *s : Light::Sample{};
+        for (int i = 0; i < 3; ++i) {
+            dir[3 * k + i] = z.Dir[i];
+            color[3 * k + i] = z.Color[i];
+            normal[3 * k + i] = z.LightNormal[i];
+        }
+        dist[k] = z.Dist;
+        pdf[k] = z.pdf.value;
+        pdf_space[k] = z.pdf.space == PdfSpace::SolidAngle ? 1 : 0;
+        eval_pdf[k] = s ? l.EvalPdf(z.Dir).value : 0.0f;
+    }
+}
+
+// UniformLightSampler's light index for uniform draws u over n lights.
+void oracle_light_index(const float *u, int32_t count, int64_t n_lights, int64_t *out) {
+    for (int k = 0; k < count; ++k) out[k] = int64_t(LightIndex(u[k], size_t(n_lights)));
+}
+
+// GetClosure (PreviewSurface, or DefaultMaterial when mat is null) at a hit with geometric normal
+// ng, smooth normal ns and vertex colour albedo. Output floats: BaseColor 3, Emission 3, Normal 3,
+// Metallic, Roughness, Specular, Ior, Opacity, Transmission.
+void oracle_closure(const hdr_material_desc *mat, const float *ng, const float *ns, const float *albedo,
+                    float *out) {
+    MaterialParams p;
+    if (mat) p = MaterialFromDesc(*mat);
+    HitRecord h;
+    h.Normal = V3(ng[0], ng[1], ng[2]);
+    h.SmoothNormal = V3(ns[0], ns[1], ns[2]);
+    h.Albedo = V3(albedo[0], albedo[1], albedo[2]);
+    const Closure c = GetClosure(mat ? &p : nullptr, h);
+    const float v[15] = {c.BaseColor[0], c.BaseColor[1], c.BaseColor[2], c.Emission[0], c.Emission[1],
+                         c.Emission[2], c.Normal[0], c.Normal[1], c.Normal[2], c.Metallic, c.Roughness,
+                         c.Specular, c.Ior, c.Opacity, c.Transmission};
+    for (int i = 0; i < 15; ++i) out[i] = v[i];
 }
 
 // Inverse of a 4x4 double matrix (the GfMatrix4d::GetInverse stand-in).

@@ -54,6 +54,11 @@ void oracle_intersect(oracle_ctx *ctx, const float *origins, const float *dirs, 
 void oracle_rng_raw(const uint32_t *seeds, int32_t n, int32_t n_draws, uint64_t *out);
 void oracle_rng_float(const uint32_t *seeds, int32_t n, int32_t n_draws, float *out);
 
+/* Unit-level exports used by the pin tests (declared in the tests' ctypes bindings):
+ * oracle_reservoir_run, oracle_random_index, oracle_spectrum_roundtrip, oracle_spectrum_to_rgb,
+ * oracle_ggx, oracle_power_heuristic, oracle_sample_bounce, oracle_sample_light,
+ * oracle_light_index, oracle_closure, oracle_inverse4. */
+
 /* Jittered and pixel-centre camera rays (GenerateCameraRay) for a whole frame:
  * origins/dirs are width*height*3 floats. */
 void oracle_camera_rays(const hdr_frame_desc *frame, int32_t width, int32_t height, int32_t centre,

@@ -1,0 +1,447 @@
+"""Pins the oracle's camera, light, light-selection, closure and bounce-sampling functions against
+what the mathematics fixes. CPU only; each test names the closed form or statistical law it uses:
+
+  * pinhole rays (camera_ray.cpp:10-21): ndc = 2 (p + 0.5) / size - 1 with row 0 at NDC -1, through
+    a symmetric perspective frustum, for an identity and a look-at camera (direction in closed form
+    from the frustum's half-angles and the camera basis);
+  * the uniform light index min(floor(u N), N - 1) (uniform_light_sampler.cpp:30-33): pinned values
+    and equal-width index intervals;
+  * rect light (rect_light.cpp:14-37): Monte Carlo irradiance E[L cos_p cos_l / d^2 / pdf_A] against
+    the view factor of a parallel rectangle (differential area under a corner:
+    F = 1/(2 pi) [X/sqrt(1+X^2) atan(Y/sqrt(1+X^2)) + Y/sqrt(1+Y^2) atan(X/sqrt(1+Y^2))], E = pi L F);
+    uniform sample points over the rectangle (chi-square); back faces give no sample;
+  * point light (point_light.cpp:16-55): no 1/d^2, pdf 1 in solid angle, cone shaping with its
+    smoothstep falloff; distant light (distant_light.cpp:10-17): the transformed -Z axis;
+  * dome light (dome_light.cpp:79-148): sampled texels follow lum * sin(theta) (chi-square), the pdf
+    equals lum / (total * dtheta * dphi), EvalPdf of the sampled direction returns it, and
+    E[L / pdf] equals the quadrature of the map;
+  * PreviewSurface SampleBounce (preview_surface.cpp:126-381) at normal incidence: the histogram of
+    directions against the mixture of the textbook GGX half-vector law (CDF tan^2 / (alpha^2 + tan^2),
+    reflected at twice the angle, below-horizon reflections mirrored) and the cosine law, with the
+    reflect probability from the normal-incidence Fresnel term; the reported pdf is the full
+    mixture density on the reflect lobe and the cosine density on the diffuse lobe (the
+    reference's pdf-reporting asymmetry, :279-281 vs :362-377);
+  * GetClosure (preview_surface.cpp:61-124, default_material.cpp:55-63): parameter mapping;
+  * linearity of the renderer in the emitted radiance (tests/test_mis_direct_light_integrator.cpp:
+    205-256 checks that the BSDF connection scales linearly): every light's exposure + 1 and every
+    emission x 2 doubles every pixel exactly -- a power of two commutes with float rounding, so the
+    draws and selections stay identical -- for MIS, RIS and ReSTIR (with spatial reuse).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from hdrestir_amd import abi, scenes
+
+
+def _L(o):
+    L = o.lib()
+    P = C.c_void_p
+    L.oracle_sample_light.argtypes = [C.POINTER(abi.LightDesc), P, P, C.c_int32, P, P, P, P, P, P, P, P, P]
+    L.oracle_light_index.argtypes = [P, C.c_int32, C.c_int64, P]
+    L.oracle_closure.argtypes = [C.POINTER(abi.MaterialDesc), P, P, P, P]
+    L.oracle_sample_bounce.argtypes = [P, C.c_float, C.c_float, C.c_float, C.c_float, P, P, C.c_int32, P, P, P]
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _light_desc(type_, color=(1.0, 1.0, 1.0), intensity=1.0, exposure=0.0, transform=None, width=1.0, height=1.0,
+                cone_angle=180.0, cone_softness=0.0, dome=None):
+    d = abi.LightDesc()
+    d.type = type_
+    for i in range(3):
+        d.color[i] = color[i]
+    d.intensity, d.exposure = intensity, exposure
+    t = np.eye(4) if transform is None else np.asarray(transform, np.float64)
+    for i in range(16):
+        d.transform[i] = float(t.flat[i])
+    d.width, d.height = width, height
+    d.shaping_cone_angle, d.shaping_cone_softness = cone_angle, cone_softness
+    keep = None
+    if dome is not None:
+        keep = np.ascontiguousarray(dome, np.float32)
+        d.dome_pixels = keep.ctypes.data_as(C.POINTER(C.c_float))
+        d.dome_height, d.dome_width = keep.shape[0], keep.shape[1]
+    return d, keep
+
+
+def _sample_light(o, desc, hit, n, seed0=1):
+    L = _L(o)
+    seeds = (np.arange(n, dtype=np.uint64) * 2654435761 + seed0).astype(np.uint32)
+    ok = np.zeros(n, np.int32)
+    dr = np.zeros(n, np.int32)
+    sp = np.zeros(n, np.int32)
+    v = {k: np.zeros((n, 3), np.float32) for k in ("dir", "color", "normal")}
+    s = {k: np.zeros(n, np.float32) for k in ("dist", "pdf", "eval_pdf")}
+    hp = np.asarray(hit, np.float32)
+    L.oracle_sample_light(C.byref(desc), _p(hp), _p(seeds), n, _p(ok), _p(v["dir"]), _p(v["color"]),
+                          _p(v["normal"]), _p(s["dist"]), _p(s["pdf"]), _p(sp), _p(dr), _p(s["eval_pdf"]))
+    out = dict(v, **s)
+    out.update(ok=ok.astype(bool), draws=dr, solid_angle=sp.astype(bool))
+    return out
+
+
+# light frame: local x -> world x, local y -> world -z, local z -> world +y (the light's -Z axis,
+# its emitting side, points down); row-vector convention, translation in the last row
+def _down_facing(pos):
+    t = np.zeros((4, 4))
+    t[0, :3] = (1, 0, 0)
+    t[1, :3] = (0, 0, -1)
+    t[2, :3] = (0, 1, 0)
+    t[3, :3] = pos
+    t[3, 3] = 1.0
+    return t
+
+
+# ------------------------------------------------------------------ camera
+
+
+def _centre_rays(o, view, proj, W, H):
+    fr = abi.frame(view, proj, 0)
+    return o.camera_rays(fr, W, H, True)
+
+
+@pytest.mark.parametrize("look_at", [False, True])
+def test_pinhole_rays_closed_form(oracle_mod, look_at):
+    W, H, fovy = 8, 6, 50.0
+    proj = scenes.perspective(fovy, W / H)
+    if look_at:
+        eye, target = np.array([3.0, 2.0, 5.0]), np.array([0.5, 0.0, -1.0])
+        view = scenes.look_at_view(eye, target)
+        back = (eye - target) / np.linalg.norm(eye - target)
+        right = np.cross([0.0, 1.0, 0.0], back)
+        right /= np.linalg.norm(right)
+        up = np.cross(back, right)
+    else:
+        eye, view = np.zeros(3), np.eye(4)
+        right, up, back = np.eye(3)
+    o, d = _centre_rays(oracle_mod, view, proj, W, H)
+    ty = math.tan(math.radians(fovy) / 2)
+    tx = ty * W / H
+    ys, xs = np.mgrid[0:H, 0:W]
+    ndc_x = 2.0 * (xs + 0.5) / W - 1.0  # row 0 at NDC -1, no y flip
+    ndc_y = 2.0 * (ys + 0.5) / H - 1.0
+    want = (ndc_x * tx)[..., None] * right + (ndc_y * ty)[..., None] * up - back
+    want = (want / np.linalg.norm(want, axis=-1, keepdims=True)).reshape(-1, 3)
+    assert np.abs(o - eye).max() < 1e-5
+    assert np.abs(d - want).max() < 2e-6, np.abs(d - want).max()
+    # the first row looks down the camera's -up side, the last row up
+    assert (d[:W] @ up < 0).all() and (d[-W:] @ up > 0).all()
+
+
+# ------------------------------------------------------------------ light selection
+
+
+def _light_index(o, u, n):
+    u = np.ascontiguousarray(u, np.float32)
+    out = np.zeros(len(u), np.int64)
+    _L(o).oracle_light_index(_p(u), len(u), n, _p(out))
+    return out
+
+
+def test_light_index_pinned_values(oracle_mod):
+    below_1 = np.nextafter(np.float32(1), np.float32(0))
+    below_q = np.nextafter(np.float32(0.25), np.float32(0))
+    cases = [(0.0, 4096, 0), (below_1, 4096, 4095), (0.5, 4096, 2048), (below_1, 65536, 65535), (0.25, 4, 1),
+             (below_q, 4, 0), (0.5, 3, 1), (0.999, 1, 0), (0.75, 2, 1), (1.0 / 4096, 4096, 1)]
+    got = _light_index(oracle_mod, [c[0] for c in cases], 1)  # warm the binding
+    for u, n, want in cases:
+        got = _light_index(oracle_mod, [u], n)[0]
+        assert got == want, (u, n, got, want)
+
+
+def test_light_index_intervals_have_equal_width(oracle_mod):
+    n = 7
+    u = (np.arange(1 << 20, dtype=np.float64) + 0.5) / (1 << 20)  # a uniform grid of draws
+    idx = _light_index(oracle_mod, u.astype(np.float32), n)
+    counts = np.bincount(idx, minlength=n)
+    assert counts.min() >= (1 << 20) // n - 2 and counts.max() <= (1 << 20) // n + 2, counts
+    assert (np.diff(idx) >= 0).all()  # monotone in u
+
+
+# ------------------------------------------------------------------ rect light
+
+
+def _corner_view_factor(a, b, h):
+    X, Y = a / h, b / h
+    return (X / math.sqrt(1 + X * X) * math.atan(Y / math.sqrt(1 + X * X)) +
+            Y / math.sqrt(1 + Y * Y) * math.atan(X / math.sqrt(1 + Y * Y))) / (2 * math.pi)
+
+
+@pytest.mark.parametrize("where", ["centre", "corner"])
+def test_rect_light_irradiance_closed_form(oracle_mod, where):
+    W, Hh, h = 0.5, 0.3, 0.7
+    L_rgb = np.array([2.0, 1.0, 0.5])
+    desc, _ = _light_desc(abi.HDR_LIGHT_RECT, color=L_rgb / 4.0, intensity=2.0, exposure=1.0,
+                          transform=_down_facing((0.0, h, 0.0)), width=W, height=Hh)
+    # receiver at the origin, normal +y; "corner": the receiver under the rectangle's corner
+    P = (0.0, 0.0, 0.0) if where == "centre" else (W / 2, 0.0, Hh / 2)
+    n = 200000
+    s = _sample_light(oracle_mod, desc, P, n)
+    assert s["ok"].all() and (s["draws"] == 2).all() and not s["solid_angle"].any()
+    assert np.allclose(s["pdf"], 1.0 / (W * Hh), rtol=1e-6)
+    assert np.allclose(s["color"], L_rgb, rtol=1e-6)  # Color * Intensity * 2^Exposure
+    assert np.allclose(s["normal"], [0.0, -1.0, 0.0], atol=1e-6)
+    d, dist = s["dir"].astype(np.float64), s["dist"].astype(np.float64)
+    cos_p = d[:, 1]
+    cos_l = (s["normal"] * -s["dir"]).sum(1)
+    est = (cos_p * cos_l / dist ** 2 / s["pdf"])[:, None] * s["color"]
+    F = 4 * _corner_view_factor(W / 2, Hh / 2, h) if where == "centre" else _corner_view_factor(W, Hh, h)
+    want = math.pi * L_rgb * F
+    se = est.std(0) / math.sqrt(n)
+    assert (np.abs(est.mean(0) - want) < 4.5 * se).all(), (est.mean(0), want, se)
+    # the sample points cover the rectangle uniformly
+    pts = np.asarray(P) + d * dist[:, None]
+    assert np.allclose(pts[:, 1], h, atol=1e-5)
+    gx = np.floor((pts[:, 0] + W / 2) / W * 4).clip(0, 3)
+    gz = np.floor((pts[:, 2] + Hh / 2) / Hh * 4).clip(0, 3)
+    assert (np.abs(pts[:, 0]) <= W / 2 + 1e-5).all() and (np.abs(pts[:, 2]) <= Hh / 2 + 1e-5).all()
+    counts = np.bincount((gx * 4 + gz).astype(int), minlength=16)
+    assert stats.chisquare(counts).pvalue > 1e-3
+
+
+def test_rect_light_back_face_gives_no_sample(oracle_mod):
+    desc, _ = _light_desc(abi.HDR_LIGHT_RECT, transform=_down_facing((0.0, 0.7, 0.0)), width=0.5, height=0.3)
+    s = _sample_light(oracle_mod, desc, (0.0, 1.5, 0.0), 1000)  # above the down-facing light
+    assert not s["ok"].any() and (s["draws"] == 2).all()
+
+
+# ------------------------------------------------------------------ point and distant lights
+
+
+def test_point_light_without_cone(oracle_mod):
+    pos = np.array([1.0, 3.0, -2.0])
+    desc, _ = _light_desc(abi.HDR_LIGHT_POINT, color=(0.25, 0.5, 1.0), intensity=3.0, exposure=2.0,
+                          transform=_down_facing(pos))
+    P = np.array([0.5, -1.0, 0.25])
+    s = _sample_light(oracle_mod, desc, P, 4)
+    to = pos - P
+    assert s["ok"].all() and (s["draws"] == 0).all() and s["solid_angle"].all()
+    assert np.allclose(s["dir"], to / np.linalg.norm(to), atol=1e-6)
+    assert np.allclose(s["dist"], np.linalg.norm(to), rtol=1e-6)
+    assert np.allclose(s["color"], np.array([0.25, 0.5, 1.0]) * 12.0, rtol=1e-6)  # no 1/d^2
+    assert np.allclose(s["pdf"], 1.0) and np.allclose(s["normal"], -s["dir"], atol=1e-6)
+
+
+def test_point_light_cone_shaping(oracle_mod):
+    cone, soft = 30.0, 0.5
+    desc, _ = _light_desc(abi.HDR_LIGHT_POINT, color=(1.0, 1.0, 1.0), intensity=1.0,
+                          transform=_down_facing((0.0, 2.0, 0.0)), cone_angle=cone, cone_softness=soft)
+    inner = cone * (1 - soft)
+    for ang in (0.0, 10.0, 14.0, 16.0, 20.0, 25.0, 29.0, 31.0, 60.0):
+        a = math.radians(ang)
+        P = np.array([2.0 * math.tan(a), 0.0, 0.0])  # the receiver at `ang` off the cone axis
+        s = _sample_light(oracle_mod, desc, P, 1)
+        if ang >= cone:
+            assert not s["ok"][0], ang
+            continue
+        f = 1.0
+        if ang > inner:
+            f = (math.cos(a) - math.cos(math.radians(cone))) / (math.cos(math.radians(inner)) - math.cos(math.radians(cone)))
+            f = f * f * (3 - 2 * f)
+        assert s["ok"][0] and abs(s["color"][0, 0] - f) < 2e-5 * max(f, 1e-3) + 1e-6, (ang, s["color"][0], f)
+
+
+def test_distant_light(oracle_mod):
+    rot = _down_facing((5.0, 6.0, 7.0))  # translation must not matter
+    desc, _ = _light_desc(abi.HDR_LIGHT_DISTANT, color=(1.0, 0.5, 0.25), intensity=4.0, exposure=-1.0,
+                          transform=rot)
+    s = _sample_light(oracle_mod, desc, (0.3, 0.2, 0.1), 3)
+    axis = np.array([0.0, 0.0, -1.0]) @ rot[:3, :3]  # TransformDir of -Z
+    assert s["ok"].all() and (s["draws"] == 0).all()
+    assert np.allclose(s["dir"], axis / np.linalg.norm(axis), atol=1e-6)
+    assert np.allclose(s["dist"], 1e30) and np.allclose(s["pdf"], 1.0) and s["solid_angle"].all()
+    assert np.allclose(s["color"], [2.0, 1.0, 0.5], rtol=1e-6)
+
+
+# ------------------------------------------------------------------ dome light
+
+
+def test_dome_sampling_follows_luminance_and_pdf(oracle_mod):
+    w, h = 16, 8
+    rs = np.random.default_rng(5)
+    tex = np.exp(rs.normal(0.0, 1.0, (h, w, 3))).astype(np.float32)
+    desc, keep = _light_desc(abi.HDR_LIGHT_DOME, color=(1.0, 1.0, 1.0), intensity=1.0, dome=tex)
+    n = 300000
+    s = _sample_light(oracle_mod, desc, (0.0, 0.0, 0.0), n)
+    assert s["ok"].all() and (s["draws"] == 2).all() and s["solid_angle"].all()
+    d = s["dir"].astype(np.float64)
+    theta = np.arccos(np.clip(d[:, 1], -1, 1))
+    phi = np.mod(np.arctan2(d[:, 2], d[:, 0]), 2 * np.pi)
+    x = np.floor(phi / (2 * np.pi) * w).astype(int).clip(0, w - 1)
+    y = np.floor(theta / np.pi * h).astype(int).clip(0, h - 1)
+    lum = (0.2126 * tex[..., 0] + 0.7152 * tex[..., 1] + 0.0722 * tex[..., 2]).astype(np.float64)
+    sin_t = np.sin(np.pi * (np.arange(h) + 0.5) / h)
+    p_texel = lum * sin_t[:, None]
+    p_texel /= p_texel.sum()
+    counts = np.bincount(y * w + x, minlength=w * h)
+    assert stats.chisquare(counts, p_texel.ravel() * n).pvalue > 1e-3
+    dth, dph = np.pi / h, 2 * np.pi / w
+    total = (lum * sin_t[:, None]).sum()
+    want_pdf = lum[y, x] / (total * dth * dph)
+    assert np.allclose(s["pdf"], want_pdf, rtol=2e-5)
+    assert (np.abs(s["eval_pdf"] - s["pdf"]) <= 1e-6 * s["pdf"]).mean() > 0.999
+    assert np.allclose(s["color"], tex[y, x], rtol=1e-6)
+    # E[L / pdf] = integral of L over the sphere = quadrature of the map
+    quad = (tex.astype(np.float64) * (sin_t * dth * dph)[:, None, None]).sum(axis=(0, 1))
+    est = s["color"] / s["pdf"][:, None]
+    se = est.std(0) / math.sqrt(n)
+    assert (np.abs(est.mean(0) - quad) < 4.5 * se + 1e-9).all(), (est.mean(0), quad, se)
+
+
+# ------------------------------------------------------------------ SampleBounce
+
+
+def _ggx_alpha(r):
+    return max(0.001, r * r)
+
+
+def _ggx_theta_cdf(theta, alpha):
+    t2 = np.tan(np.minimum(theta, np.pi / 2 - 1e-12)) ** 2
+    return t2 / (alpha * alpha + t2)
+
+
+def _ggx_d(cos_h, alpha):
+    a2 = alpha * alpha
+    den = cos_h * cos_h * (a2 - 1) + 1
+    return a2 / (np.pi * den * den)
+
+
+def _fresnel_normal(ior):
+    return ((ior - 1) / (ior + 1)) ** 2
+
+
+def _sample_bounce(o, base, metal, rough, spec, ior, n):
+    seeds = (np.arange(n, dtype=np.uint64) * 2246822519 + 7).astype(np.uint32)
+    dirs = np.zeros((n, 3), np.float32)
+    pdfs = np.zeros(n, np.float32)
+    draws = np.zeros(n, np.int32)
+    b = np.asarray(base, np.float32)
+    rd = np.array([0.0, -1.0, 0.0], np.float32)
+    _L(o).oracle_sample_bounce(_p(b), metal, rough, spec, ior, _p(rd), _p(seeds), n, _p(dirs), _p(pdfs), _p(draws))
+    return dirs.astype(np.float64), pdfs.astype(np.float64), draws
+
+
+@pytest.mark.parametrize("rough,metal", [(0.3, 0.0), (0.8, 0.0), (0.3, 1.0), (0.6, 1.0), (0.15, 1.0)])
+def test_sample_bounce_direction_histogram(oracle_mod, rough, metal):
+    ior, spec, n = 1.5, 1.0, 200000
+    dirs, pdfs, draws = _sample_bounce(oracle_mod, (0.5, 0.5, 0.5), metal, rough, spec, ior, n)
+    assert (draws == 3).all()  # randVal + two for the chosen lobe
+    assert np.abs(np.linalg.norm(dirs, axis=1) - 1.0).max() < 2e-6
+    lam0 = 360.0 + 0.5 * 470.0  # the hero wavelength of u = 0.5; dispersion uses it alone
+    local_ior = ior - 10000.0 / 589.3 ** 2 + 10000.0 / lam0 ** 2
+    p_r = max(_fresnel_normal(local_ior) * spec, metal) if metal > 0 else _fresnel_normal(local_ior) * spec
+    alpha = _ggx_alpha(rough)
+    theta = np.arccos(np.clip(dirs[:, 1], -1, 1))
+    phi = np.mod(np.arctan2(dirs[:, 2], dirs[:, 0]), 2 * np.pi)
+    tb = np.linspace(0, np.pi / 2, 10)  # 9 polar bands
+    nphi = 8
+    # reflect lobe: the reflection of the normal-incidence ray about h leaves at 2 theta_h; those
+    # past the horizon (theta_h > 45 deg) are replaced by the mirror direction (theta = 0)
+    refl = np.diff(_ggx_theta_cdf(tb / 2, alpha))
+    mirror_p = p_r * (1.0 - _ggx_theta_cdf(np.pi / 4, alpha))  # a point mass at the zenith
+    diff = np.diff(np.sin(tb) ** 2)  # cosine law: P(theta <= t) = sin^2 t
+    band = p_r * refl + (1 - p_r) * diff
+    expected = np.append(np.repeat(band / nphi, nphi), mirror_p) * n
+    mirror = (dirs[:, 0] == 0.0) & (dirs[:, 2] == 0.0)
+    ti = np.clip(np.searchsorted(tb, theta, side="right") - 1, 0, len(tb) - 2)
+    pj = np.floor(phi / (2 * np.pi) * nphi).astype(int).clip(0, nphi - 1)
+    cell = np.where(mirror, (len(tb) - 1) * nphi, ti * nphi + pj)
+    counts = np.bincount(cell, minlength=(len(tb) - 1) * nphi + 1)
+    keep = expected > 20  # the sparse cells are pooled into one
+    obs = np.append(counts[keep], counts[~keep].sum())
+    exp = np.append(expected[keep], expected[~keep].sum())
+    chi2 = ((obs - exp) ** 2 / np.maximum(exp, 1.0)).sum()
+    crit = stats.chi2.ppf(0.999, len(obs) - 1)
+    assert chi2 < crit, (chi2, crit)
+    # reported pdf: the full mixture on the reflect lobe (Fresnel at the material's IOR), the
+    # cosine density on the diffuse lobe
+    cos_l = np.maximum(0.001, dirs[:, 1])
+    hv = dirs + np.array([0.0, 1.0, 0.0])
+    hv /= np.linalg.norm(hv, axis=1, keepdims=True)
+    cos_h = np.maximum(0.001, hv[:, 1])
+    l_h = np.maximum(0.001, (dirs * hv).sum(1))
+    p_mix_r = max(_fresnel_normal(ior) * spec, metal) if metal > 0 else _fresnel_normal(ior) * spec
+    mix = p_mix_r * _ggx_d(cos_h, alpha) * cos_h / (4 * l_h) + (1 - p_mix_r) * cos_l / np.pi
+    cosine = np.maximum(dirs[:, 1] / np.pi, 1e-6)
+    # (near a peaked lobe's axis the float32 half vector carries a few 1e-3 of relative error)
+    is_mix = np.abs(pdfs - mix) <= 1e-2 * mix + 1e-7
+    is_cos = np.abs(pdfs - cosine) <= 2e-5 * cosine + 1e-9
+    assert (is_mix | is_cos).all(), np.flatnonzero(~(is_mix | is_cos))[:5]
+    if metal == 0.0:
+        frac = (is_mix & ~is_cos).mean()
+        se = math.sqrt(p_r * (1 - p_r) / n)
+        assert abs(frac - p_r) < 5 * se + 1e-3, (frac, p_r)
+    else:
+        assert is_mix.all()
+
+
+# ------------------------------------------------------------------ GetClosure
+
+
+def _closure(o, mat, ng=(0.0, 1.0, 0.0), ns=(0.0, 0.8, 0.6), albedo=(0.2, 0.3, 0.4)):
+    out = np.zeros(15, np.float32)
+    a = [np.asarray(v, np.float32) for v in (ng, ns, albedo)]
+    _L(o).oracle_closure(C.byref(mat) if mat is not None else None, _p(a[0]), _p(a[1]), _p(a[2]), _p(out))
+    return out
+
+
+def test_closure_mapping(oracle_mod):
+    sc = scenes.Scene("m", 1, 1)
+    sc.add_material(diffuse_color=(0.7, 0.6, 0.5), metallic=0.25, roughness=0.0, emission_color=(1.0, 0.5, 0.25),
+                    emission=4.0, ior=1.33, opacity=0.5, transmission=0.1)
+    sc.add_material(roughness=0.4)
+    d = sc.desc
+    c0 = _closure(oracle_mod, d.materials[0])
+    assert np.allclose(c0[0:3], [0.7, 0.6, 0.5]) and np.allclose(c0[3:6], [4.0, 2.0, 1.0])
+    assert np.allclose(c0[6:9], [0.0, 0.8, 0.6])  # the smooth normal
+    assert np.allclose(c0[9:15], [0.25, 0.001, 1.0, 1.33, 0.5, 0.1], rtol=1e-6)  # roughness floor 0.001
+    assert abs(_closure(oracle_mod, d.materials[1])[10] - 0.4) < 1e-7
+    cd = _closure(oracle_mod, None)  # DefaultMaterial: vertex colour, geometric normal, rough dielectric
+    assert np.allclose(cd[0:3], [0.2, 0.3, 0.4]) and np.allclose(cd[6:9], [0.0, 1.0, 0.0])
+    assert cd[9] == 0.0 and cd[10] == 1.0 and np.allclose(cd[3:6], 0.0)
+
+
+# ------------------------------------------------------------------ linearity in the emitted radiance
+
+
+def _double_radiance(sc):
+    for l in sc.lights:
+        l["exposure"] += 1.0
+    for m in sc.materials:
+        m["emission"] *= 2.0
+    if hasattr(sc, "_desc"):
+        del sc._desc
+    return sc
+
+
+@pytest.mark.parametrize("name,make,pipe", [
+    ("box_mis_d4", lambda: scenes.box_spheres(24, 24), abi.pipeline("mis", max_depth=4)),
+    ("box_ris16_d4", lambda: scenes.box_spheres(24, 24), abi.pipeline("ris", candidate_count=16, use_reservoir=False)),
+    ("box_restir16_d4", lambda: scenes.box_spheres(24, 24), abi.pipeline("ris", candidate_count=16)),
+    ("features_mis_d5", lambda: scenes.features(24, 24), abi.pipeline("mis", max_depth=5)),
+    ("features_restir_spatial_d3", lambda: scenes.features(24, 24, opaque=True),
+     abi.pipeline("ris", candidate_count=8, skip_visibility=True, max_depth=3, spatial_count=3, spatial_radius=6.0)),
+])
+def test_image_is_linear_in_emitted_radiance(oracle_mod, name, make, pipe):
+    imgs, dbgs = [], []
+    for k in range(2):
+        sc = make()
+        if k:
+            _double_radiance(sc)
+        o = oracle_mod.Oracle(sc, threads=4)
+        for f in range(2):
+            img = o.render(sc.frame(f), pipe).copy()
+        imgs.append(img)
+        dbgs.append(o.debug.copy())
+        o.close()
+    for fld in ("draws", "light_hash", "select_hash"):
+        assert np.array_equal(dbgs[0][fld], dbgs[1][fld]), fld
+    assert np.array_equal(imgs[1][..., :3], 2.0 * imgs[0][..., :3]), np.abs(imgs[1][..., :3] - 2 * imgs[0][..., :3]).max()
+    assert imgs[0][..., :3].max() > 0.0
