@@ -1,10 +1,12 @@
 """Convergence against 4096-spp references: the per-pixel MSE of each estimator at a stated spp, and
 the mean radiance of the unbiased estimators inside a 99 % interval (batch means over 32 batches).
 
-The references are the GPU PathTracer (MIS) at 4096 frames. Every frame of it is the oracle's frame
-on the same seed -- the parity tests pin the streams bit-exactly and the colours to float rounding
-(test_render_parity_box_spheres, test_render_parity_many_lights_crop, test_lean_pathtracer_parity)
--- so this average is the oracle's 4096-spp image, obtained in seconds instead of CPU hours.
+The references are the CPU oracle's 4096-spp PathTracer (MIS) images, committed under tests/golden/
+(written by tests/golden/make_convergence_refs.py): the whole 256 x 256 box image, and a 64 x 64
+crop of the many-lights view. The GPU renders the same 4096 frames (same seeds) first, and its image
+must equal the oracle's to rounding (the whole image for the box, the crop for the many-lights
+view); the box bars then use the oracle's image, the many-lights bars the GPU image that equals
+the oracle's on the crop. The CI checks use the GPU run's batch means.
 
 Workloads: the box of 12 triangles with two icospheres and one rect light, 256 x 256, depth 4
 (PathTracer, RIS M = 16 without and with the reservoir, ReSTIR with one shadow ray with and without
@@ -15,10 +17,14 @@ M = 32, ReSTIR M = 32 with one shadow ray, with and without 3 spatial neighbours
 Relative MSE = mean over pixels and RGB of (image - reference)^2 / mean of reference^2. The
 thresholds are the stated bars; the measured values are printed.
 """
+import os
+
 import numpy as np
 import pytest
 
 from hdrestir_amd import abi, scenes
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 pytestmark = pytest.mark.gpu
 
@@ -67,7 +73,12 @@ def box(gpu_ctx):
     gs = gpu.GpuScene(gpu_ctx, sc)
     r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
     fr = lambda f: abi.frame(view, proj, f)  # noqa: E731
-    ref, ref_bm = _render_mean(r, fr, abi.pipeline("mis", max_depth=4), 0, REF_SPP, batch=REF_SPP // BATCHES)
+    gref, ref_bm = _render_mean(r, fr, abi.pipeline("mis", max_depth=4), 0, REF_SPP, batch=REF_SPP // BATCHES)
+    ref = np.load(os.path.join(GOLDEN, "box_pt_d4_4096spp.npz"))["mean"].astype(np.float64)
+    e = _rel_mse(gref, ref)  # float16 storage of the oracle's image: ~1e-7
+    print(f"box: GPU 4096-spp image vs the oracle's: relative MSE {e:.3g}")
+    assert e < 1e-6, e
+    assert np.allclose(gref.mean((0, 1)), ref.mean((0, 1)), rtol=1e-4)
     yield r, fr, ref, ref_bm
     r.close()
     gs.close()
@@ -82,6 +93,13 @@ def many(gpu_ctx):
     r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
     fr = lambda f: abi.frame(view, proj, f)  # noqa: E731
     ref, ref_bm = _render_mean(r, fr, abi.pipeline("mis", max_depth=1), 0, REF_SPP, batch=REF_SPP // BATCHES)
+    g = np.load(os.path.join(GOLDEN, "many_lights_pt_d1_crop_4096spp.npz"))
+    x0, y0, x1, y1 = (int(v) for v in g["rect"])
+    crop, oref = ref[y0:y1, x0:x1], g["mean"].astype(np.float64)
+    e = _rel_mse(crop, oref)
+    print(f"many-lights: GPU 4096-spp crop vs the oracle's: relative MSE {e:.3g}")
+    assert e < 1e-6, e
+    assert np.allclose(crop.mean((0, 1)), oref.mean((0, 1)), rtol=1e-4)
     yield r, fr, ref, ref_bm
     r.close()
     gs.close()
