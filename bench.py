@@ -9,12 +9,16 @@ Workload (one step = one frame of the per-pixel hot path over the whole image):
           temporal reservoir reuse (the reference's merge), then spatial reuse of 3 neighbours
           within 30 px (an extension beyond the reference), max depth 1 -> spectral->RGB -> accumulation;
           the reference's own pipeline (temporal reuse only) is timed beside it;
-  camera  static (progressive accumulation with temporal reuse); the linear-dolly variant is
-          reported beside it (a camera move clears reservoirs and accumulation, as the renderer does).
+  camera  the 64-frame linear dolly of the workload, 1 spp per frame: every camera move clears the
+          reservoirs and the accumulation and restarts the frame count (renderer.cpp:401-406), so
+          each frame starts from a cleared view at frame index 0; the static-camera variant
+          (progressive frames with temporal reuse) is reported beside it.
 
 Prints one JSON line. `value` = ReSTIR frames/s at 1080p with scene, reservoirs and accumulation
 resident in HBM. The CPU baseline is the oracle (a faithful C++ restatement of the reference's CPU
-path) timed on a bounded sample of the same frame on this host.
+path) timed on a bounded sample of the same frames on this host. Further entries: config 0 (the
+box at 256x256, 16 spp, PathTracer and RIS), the depth-6 frame, the 16-view batch, the large scene
+and the serving stream.
 """
 import argparse
 import json
@@ -63,27 +67,108 @@ def read_traffic(kernel):
         return None
 
 
-def cpu_baseline(scene, pipe, rows=540, threads=None):
-    """Oracle (faithful restatement, per-path light sampler as in the reference) on `rows` central
-    pixel rows of frame 1 (after frame 0 on the same rows), scaled to frames/s of the whole frame.
-    Threads: the host's cores, at most 16 (one GPU's share of the host)."""
-    if threads is None:
-        threads = min(16, os.cpu_count() or 1)
+def host_cpu():
+    """The host's logical CPU count and model name (from /proc/cpuinfo)."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
+def cpu_threads():
+    """One worker per hardware thread of this GPU's share of the host (OMP_NUM_THREADS where the
+    job scheduler sets it, else every logical CPU)."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+
+
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
-    o = oracle_py.Oracle(scene, hoist_light_set=False, threads=threads)
+    return oracle_py
+
+
+def cpu_baseline(scene, pipe, rows=540, timed=2):
+    """The oracle (a faithful C++20 restatement of the reference's CPU path: 16x16 tiles, per-pixel
+    mt19937_64, a UniformLightSampler with its set of every light built per path) on dolly frames:
+    each frame from a cleared state at frame index 0, as the reference renders a moving camera. One
+    warm-up frame, then the median of `timed` frames over the central `rows` pixel rows, scaled to
+    the whole frame. The hoisted-light-set variant (one light set per frame, identical output) is
+    timed the same way beside it."""
+    oracle_py = _oracle()
+    threads = cpu_threads()
     rows = min(rows, scene.height)
     y0 = (scene.height - rows) // 2
     rect = (0, y0, scene.width, y0 + rows)
-    o.render(scene.frame(0), pipe, rect=rect, debug=False)  # warm-up
-    o.render(scene.frame(1), pipe, rect=rect, debug=False)
-    secs = o.last_seconds
-    o.close()
-    frame_secs = secs * (scene.height / rows)
-    return {"value": 1.0 / frame_secs, "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"{scene.width}x{rows} pixels (the central rows) of frame 1 after frame 0 on the same pixels "
-                      f"(temporal reuse active), faithful per-path light-sampler construction, scaled by "
-                      f"{scene.height / rows:g} to the full frame"}
+    out = {}
+    for hoist in (False, True):
+        o = oracle_py.Oracle(scene, hoist_light_set=hoist, threads=threads)
+        secs = []
+        for k in range(1 + timed):
+            o.reset()
+            o.render(scene.frame(0, camera_index=k % len(scene.cameras)), pipe, rect=rect, debug=False)
+            if k:
+                secs.append(o.last_seconds * scene.height / rows)
+        o.close()
+        out[hoist] = secs
+    med = float(np.median(out[False]))
+    return {"value": round(1.0 / med, 5), "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"median of {timed} dolly frames (after one warm-up), each {scene.width}x{rows} pixels (the "
+                      f"central rows) from a cleared state, scaled by {scene.height / rows:g} to the full frame; "
+                      "faithful per-path light-sampler construction",
+            "frame_seconds": [round(v, 3) for v in out[False]],
+            "hoisted_light_set_frames_per_s": round(1.0 / float(np.median(out[True])), 5),
+            "host": host_cpu(), "compiler": "g++ -O2 -std=c++20 -ffp-contract=off (oracle/Makefile)"}
+
+
+def config0_measurement(gpu, abi, ctx, cpu=True, cpu_frames=5):
+    """Config 0: the box of 12 triangles with two icospheres (4840 triangles) and one rect light at
+    256x256, 16 spp (frames 0 .. 15), depth 4: PathTracer (MIS) and RIS with M = 16 (8 light + 8
+    BSDF candidates, visibility tested, without and with the reservoir). GPU: frames/s (= spp/s) over
+    the 16 frames after an untimed pass; CPU: the oracle on whole frames, median of `cpu_frames`
+    frames after one warm-up."""
+    from hdrestir_amd import scenes
+    sc = scenes.box_spheres(256, 256, frames=16)
+    pipes = {"pt_d4": abi.pipeline("mis", max_depth=4),
+             "ris16_d4": abi.pipeline("ris", candidate_count=16, use_reservoir=False, max_depth=4),
+             "restir16_d4": abi.pipeline("ris", candidate_count=16, use_reservoir=True, max_depth=4)}
+    gs = gpu.GpuScene(ctx, sc)
+    r = gpu.Renderer(ctx, gs, sc.width, sc.height)
+    out = {"workload": "box_spheres_256_16spp_depth4", "triangles": sc.triangle_counts()[0], "spp": 16}
+    for name, pipe in pipes.items():
+        for _ in range(2):  # an untimed pass, then the timed one
+            r.reset()
+            r.synchronize()
+            t0 = time.perf_counter()
+            for f in range(16):
+                r.render(sc.frame(f), pipe, read_color=False)
+            r.synchronize()
+            ms = (time.perf_counter() - t0) * 1000.0 / 16
+        out[name] = {"gpu_ms_per_frame": round(ms, 4), "gpu_frames_per_s": round(1000.0 / ms, 2),
+                     "gpu_image_ms_16spp": round(16 * ms, 3)}
+    r.close()
+    gs.close()
+    if cpu:
+        oracle_py = _oracle()
+        threads = cpu_threads()
+        for name, pipe in pipes.items():
+            o = oracle_py.Oracle(sc, hoist_light_set=False, threads=threads)
+            secs = []
+            for f in range(1 + cpu_frames):
+                o.render(sc.frame(f), pipe, debug=False)
+                if f:
+                    secs.append(o.last_seconds)
+            o.close()
+            med = float(np.median(secs))
+            out[name]["cpu_frames_per_s"] = round(1.0 / med, 3)
+            out[name]["cpu_cores"] = threads
+        out["cpu_sample"] = f"whole 256x256 frames 1 .. {cpu_frames} after frame 0, median; the oracle, faithful"
+    return out
 
 
 def look_at_origin_from_sphere(rng, radius):
@@ -334,131 +419,58 @@ def multiview_measurement(gpu, abi, ctx, gs, steps, seed=3):
             "plain_ms_per_step": round(ms_plain, 3), "speedup": round(ms_plain / ms_batched, 3)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64)
-    ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=540,
-                    help="pixel rows of the frame the CPU baseline renders (centred; about 15 s on 16 cores)")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
-                    help="only run a plain device-to-device copy of BYTES bytes 4 times (the reference "
-                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
-    ap.add_argument("--no-serving", action="store_true", help="skip the request-serving measurement")
-    ap.add_argument("--no-deep", action="store_true",
-                    help="skip the depth-6 single-view, 16-view batch and large-scene measurements")
-    ap.add_argument("--deep-only", action="store_true",
-                    help="only the depth-6 single-view measurement, --steps frames (for profilers)")
-    ap.add_argument("--large-only", action="store_true",
-                    help="only the large-scene measurement (for profilers)")
-    ap.add_argument("--headline-only", action="store_true",
-                    help="only the counting pass and the timed frames of the headline pipeline (for profilers)")
-    ap.add_argument("--serving-requests", type=int, default=48)
-    ap.add_argument("--dolly", action="store_true",
-                    help="also time the moving-camera variant (each move clears reservoirs and accumulation)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: the path runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the path would shard by pixels and by requests across GPUs)", file=sys.stderr)
-        sys.exit(2)
-
-    if args.copy_calibration:
-        import torch
-        src = torch.ones(args.copy_calibration, dtype=torch.uint8, device="cuda:0")
-        dst = torch.empty_like(src)
-        for _ in range(4):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes": args.copy_calibration, "launches": 4}), flush=True)
-        return
-
-    from hdrestir_amd import abi, gpu, scenes
-    if args.large_only:
-        ctx = gpu.Context(0)
-        print(json.dumps(large_scene_measurement(gpu, abi, ctx)), flush=True)
-        ctx.close()
-        return
-    _log("building scene")
-    sc = scenes.many_lights(frames=max(args.steps, 2), dolly=True)
-    if args.deep_only:
-        ctx = gpu.Context(0)
-        gs = gpu.GpuScene(ctx, sc)
-        print(json.dumps(deep_path_measurement(gpu, abi, ctx, sc, gs, args.steps)), flush=True)
-        gs.close()
-        ctx.close()
-        return
-    # 1 temporal (the reference's merge) + 3 spatial neighbours within 30 px (an extension)
-    pipe = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=1,
-                        spatial_count=3, spatial_radius=30.0)
-    pipe_temporal = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=1)
-    ctx = gpu.Context(0)
-    gs = gpu.GpuScene(ctx, sc)
-    r = gpu.Renderer(ctx, gs, sc.width, sc.height)
-    P = sc.width * sc.height
-
-    def frame_desc(i, moving):
-        return sc.frame(i, camera_index=(i % len(sc.cameras)) if moving else 0)
-
-    # counting pass (untimed): the same frame sequence as the timed pass below, with per-frame
-    # counters of rays by class, BVH node visits and triangle tests (deterministic, so the timed
-    # pass repeats exactly this work); averaged over the timed frames
-    r.set_counting(True)
-    r.reset()
-    for i in range(args.warmup):
-        r.render(frame_desc(i, False), pipe, read_color=False)
-    counts = None
-    for i in range(args.steps):
-        r.render(frame_desc(args.warmup + i, False), pipe, read_color=False)
-        r.synchronize()
-        c = r.counters()
-        counts = c if counts is None else {k: counts[k] + c[k] for k in c}
-    counts = {k: v / args.steps for k, v in counts.items()}
-    r.set_counting(False)
-    _log(f"counters per frame {counts}")
-
-    # static camera: progressive frames with temporal reuse
-    r.reset()
-    for i in range(args.warmup):
-        r.render(frame_desc(i, False), pipe, read_color=False)
+def run_frames(r, desc, pipe, warmup, steps, reset_each):
+    """`warmup` untimed frames desc(0 ..), then `steps` timed frames; with reset_each every frame starts
+    from a cleared view (a camera move). Returns (ms per frame by the host clock around the timed
+    frames, synchronised on both sides; {stage: (summed device ms, launches)} from HIP events on the
+    streams the kernels run on)."""
+    if not reset_each:
+        r.reset()
+    for i in range(warmup):
+        if reset_each:
+            r.reset()
+        r.render(desc(i), pipe, read_color=False)
     r.synchronize()
     r.stage_times()  # clears the per-kernel sums
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        r.render(frame_desc(args.warmup + i, False), pipe, read_color=False)
-    r.synchronize()
-    t1 = time.perf_counter()
-    # HIP events on the view's stream around each kernel of the timed frames
-    stages = {k: v for k, v in r.stage_times().items() if v[1] > 0}
-    ms_per_step = (t1 - t0) * 1000.0 / args.steps
-    fps = 1000.0 / ms_per_step
-    img = r.read_color()
-
-    # the reference's own pipeline (temporal reuse only), same frames
-    temporal_fps = None
-    if not args.headline_only:
-        r.reset()
-        for i in range(args.warmup):
-            r.render(frame_desc(i, False), pipe_temporal, read_color=False)
-        r.synchronize()
-        t2 = time.perf_counter()
-        for i in range(args.steps):
-            r.render(frame_desc(args.warmup + i, False), pipe_temporal, read_color=False)
-        r.synchronize()
-        temporal_fps = args.steps / (time.perf_counter() - t2)
-
-    dolly_fps = None
-    if args.dolly and not args.headline_only:  # a camera move clears reservoirs and accumulation
-        t2 = time.perf_counter()
-        nd = min(args.steps, len(sc.cameras))
-        for i in range(nd):
+    for i in range(steps):
+        if reset_each:
             r.reset()
-            r.render(frame_desc(i, True), pipe, read_color=False)
-        r.synchronize()
-        dolly_fps = nd / (time.perf_counter() - t2)
+        r.render(desc(warmup + i), pipe, read_color=False)
+    r.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    return ms, {k: v for k, v in r.stage_times().items() if v[1] > 0}
 
-    rays = counts["primary_rays"] + counts["bounce_rays"] + counts["shadow_rays"] + counts["candidate_rays"]
-    ref_rays = rays + counts["candidate_rays_skipped"]
+
+def count_frames(r, desc, pipe, warmup, steps, reset_each):
+    """The frame sequence of run_frames with the work counters on (untimed, deterministic, so the
+    timed pass repeats exactly this work); counters averaged over the `steps` frames."""
+    r.set_counting(True)
+    r.reset()
+    for i in range(warmup):
+        if reset_each:
+            r.reset()
+        r.render(desc(i), pipe, read_color=False)
+    counts = None
+    for i in range(steps):
+        if reset_each:
+            r.reset()
+        r.render(desc(warmup + i), pipe, read_color=False)
+        r.synchronize()
+        c = r.counters()
+        counts = c if counts is None else {k: counts[k] + c[k] for k in c}
+    r.set_counting(False)
+    return {k: v / steps for k, v in counts.items()}
+
+
+# stages on the view's main stream: the frame's critical path (the G-buffer kernel of the next frame
+# runs on the second stream beside them)
+CRITICAL_STAGES = ("integrate", "spatial", "shadow", "accumulate", "firefly")
+
+
+def kernel_table(stages, counts, P):
+    """Per-kernel average time, algorithmic bytes and achieved GB/s, and the roofline entry of the
+    critical-path kernel with the largest summed time."""
     pn, pt = counts["primary_node_visits"], counts["primary_triangle_tests"]
     sn, st = counts["node_visits"] - pn, counts["triangle_tests"] - pt
     q = counts["shadow_rays"]
@@ -491,12 +503,120 @@ def main():
             e["achieved_GBs"] = round(alg[k] / (avg * 1e-3) / 1e9, 2)
             e["hbm_traffic_bytes_per_launch"] = read_traffic(name)
         kernels[name] = e
-    dom_stage = max(stages, key=lambda k: stages[k][0])
+    crit = [k for k in stages if k in CRITICAL_STAGES and k in alg]
+    dom_stage = max(crit, key=lambda k: stages[k][0])
     dom = KERNEL_NAMES[dom_stage] if (deferred or dom_stage != "integrate") else "integrate_kernel"
     dom_avg = stages[dom_stage][0] / stages[dom_stage][1]
     achieved = alg[dom_stage] / (dom_avg * 1e-3) / 1e9
+    roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(dom),
+            "traffic_source": f"profiles/{dom}_many_lights_pmc.txt (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of "
+                              "this workload, scaled by a plain copy)",
+            "avg_ms": round(dom_avg, 4), "algorithmic_bytes_per_launch": round(alg[dom_stage]),
+            "selection": "the critical-path kernel (view's main stream) with the largest summed time"}
+    return kernels, roof
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=270,
+                    help="pixel rows of each frame the CPU baseline renders (centred)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES bytes 4 times (the reference "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
+    ap.add_argument("--no-serving", action="store_true", help="skip the request-serving measurement")
+    ap.add_argument("--no-deep", action="store_true",
+                    help="skip the depth-6 single-view, 16-view batch and large-scene measurements")
+    ap.add_argument("--deep-only", action="store_true",
+                    help="only the depth-6 single-view measurement, --steps frames (for profilers)")
+    ap.add_argument("--large-only", action="store_true",
+                    help="only the large-scene measurement (for profilers)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the counting pass and the timed frames of the headline pipeline (for profilers)")
+    ap.add_argument("--serving-requests", type=int, default=48)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: the path runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path would shard by pixels and by requests across GPUs)", file=sys.stderr)
+        sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(args.copy_calibration, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": args.copy_calibration, "launches": 4}), flush=True)
+        return
+
+    from hdrestir_amd import abi, gpu, scenes
+    if args.large_only:
+        ctx = gpu.Context(0)
+        print(json.dumps(large_scene_measurement(gpu, abi, ctx)), flush=True)
+        ctx.close()
+        return
+    _log("building scene")
+    sc = scenes.many_lights(frames=64, dolly=True)  # config 1's 64-frame linear dolly
+    if args.deep_only:
+        ctx = gpu.Context(0)
+        gs = gpu.GpuScene(ctx, sc)
+        print(json.dumps(deep_path_measurement(gpu, abi, ctx, sc, gs, args.steps)), flush=True)
+        gs.close()
+        ctx.close()
+        return
+    # 1 temporal (the reference's merge) + 3 spatial neighbours within 30 px (an extension)
+    pipe = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=1,
+                        spatial_count=3, spatial_radius=30.0)
+    pipe_temporal = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=1)
+    ctx = gpu.Context(0)
+    gs = gpu.GpuScene(ctx, sc)
+    r = gpu.Renderer(ctx, gs, sc.width, sc.height)
+    P = sc.width * sc.height
+
+    # config 1 as written: a 64-frame linear dolly at 1 spp per frame. Each camera move clears the
+    # reservoirs and the accumulation and restarts the frame count (renderer.cpp:401-406), so every
+    # frame is rendered from a cleared view at frame index 0.
+    def dolly(i):
+        return sc.frame(0, camera_index=i % len(sc.cameras))
+
+    def static(i):  # the progressive variant: one camera, temporal reuse across frames
+        return sc.frame(i, camera_index=0)
+
+    counts = count_frames(r, dolly, pipe, args.warmup, args.steps, reset_each=True)
+    _log(f"counters per dolly frame {counts}")
+    ms_per_step, stages = run_frames(r, dolly, pipe, args.warmup, args.steps, reset_each=True)
+    fps = 1000.0 / ms_per_step
+    img = r.read_color()
+    kernels, roof = kernel_table(stages, counts, P)
+
+    # the same frames with the reference's own pipeline (temporal reuse only)
+    temporal_fps = None
+    if not args.headline_only:
+        ms_t, _ = run_frames(r, dolly, pipe_temporal, args.warmup, args.steps, reset_each=True)
+        temporal_fps = 1000.0 / ms_t
+    # static camera: progressive accumulation, temporal reuse from the previous frames
+    static_line = None
+    if not args.headline_only:
+        counts_s = count_frames(r, static, pipe, args.warmup, args.steps, reset_each=False)
+        ms_s, stages_s = run_frames(r, static, pipe, args.warmup, args.steps, reset_each=False)
+        kernels_s, roof_s = kernel_table(stages_s, counts_s, P)
+        rays_s = sum(counts_s[k] for k in ("primary_rays", "bounce_rays", "shadow_rays", "candidate_rays"))
+        static_line = {"camera": "static (progressive, temporal reuse across frames)",
+                       "frames_per_s": round(1000.0 / ms_s, 3), "ms_per_step": round(ms_s, 4),
+                       "mrays_per_s": round(rays_s / ms_s / 1e3, 2), "shadow_rays_per_frame": counts_s["shadow_rays"],
+                       "kernels": kernels_s, "roofline": roof_s}
+        _log(f"static camera {static_line['frames_per_s']} frames/s")
+
+    rays = counts["primary_rays"] + counts["bounce_rays"] + counts["shadow_rays"] + counts["candidate_rays"]
+    ref_rays = rays + counts["candidate_rays_skipped"]
     line = {
-        "metric": "ReSTIR frames/s at 1080p (many-lights, M=32, 1 temporal + 3 spatial within 30 px); Mrays/s alongside",
+        "metric": "ReSTIR frames/s at 1080p (many-lights, M=32, 1 temporal + 3 spatial within 30 px, 64-frame "
+                  "linear dolly); Mrays/s alongside",
         "value": round(fps, 3),
         "unit": "frames/s",
         "n_gpus": 1,
@@ -511,7 +631,9 @@ def main():
         "config": {"workload": "many_lights_1080p_restir", "width": sc.width, "height": sc.height,
                    "triangles_unique": sc.triangle_counts()[0], "triangles_instanced": sc.triangle_counts()[1],
                    "lights": len(sc.lights), "candidate_count": 32, "skip_visibility": True, "max_depth": 1,
-                   "temporal_reuse": True, "spatial_reuse": 3, "spatial_radius_px": 30, "camera": "static",
+                   "temporal_reuse": True, "spatial_reuse": 3, "spatial_radius_px": 30, "camera": "linear_dolly",
+                   "dolly_frames": len(sc.cameras), "spp_per_frame": 1,
+                   "camera_move": "clears reservoirs and accumulation, frame index restarts at 0 (renderer.cpp:401-406)",
                    "parallelism": "single-gpu"},
         "mrays_per_s": round(rays * fps / 1e6, 2),
         "mrays_per_s_reference_equivalent": round(ref_rays * fps / 1e6, 2),
@@ -519,12 +641,9 @@ def main():
                            "shadow": counts["shadow_rays"], "bsdf_candidate_traced": counts["candidate_rays"],
                            "skipped_cannot_contribute": counts["candidate_rays_skipped"]},
         "temporal_only_frames_per_s": None if temporal_fps is None else round(temporal_fps, 3),
-        "dolly_frames_per_s": None if dolly_fps is None else round(dolly_fps, 3),
         "kernels": kernels,
-        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(dom),
-                     "avg_ms": round(dom_avg, 4), "algorithmic_bytes_per_launch": round(alg[dom_stage])},
+        "roofline": roof,
+        "static_camera": static_line,
         "image_mean_rgb": [round(float(v), 3) for v in img[..., :3].reshape(-1, 3).mean(0)],
     }
     r.close()
@@ -538,6 +657,10 @@ def main():
         _log("large scene")
         line["large_scene"] = large_scene_measurement(gpu, abi, ctx)
         _log(f"large scene {line['large_scene']}")
+    if not args.headline_only:
+        _log("config 0")
+        line["config0"] = config0_measurement(gpu, abi, ctx, cpu=not args.no_cpu_baseline)
+        _log(f"config 0 {line['config0']}")
     if not args.no_serving and not args.headline_only:
         _log("serving")
         line["serving"] = serving_measurement(gpu, abi, ctx, sc, gs, args.serving_requests)
