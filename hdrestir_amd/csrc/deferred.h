@@ -603,6 +603,9 @@ __device__ __forceinline__ Spec ris_direct_traced(const FrameArgs &a, const Spec
 // identical on every target, and the reference's per-pixel mt19937_64 stream is untouched.
 
 constexpr int kMaxSpatial = 8;
+// Domains of a pixel's spatial reuse (itself + up to spatial_count neighbours), rounded up to the
+// register-array sizes the spatial kernels are compiled for.
+constexpr int spatial_domains(int spatial_count) { return spatial_count <= 3 ? 4 : (spatial_count <= 5 ? 6 : 9); }
 
 struct SpatialRng {
     uint64_t s;
@@ -725,13 +728,20 @@ __device__ __forceinline__ bool load_spatial_vertex(const FrameArgs &a, const GR
 }
 
 // Spatial reuse at pixel (x, y). dl / cl: this lane's domain pixels and confidences in LDS
-// (stride kLanes). Returns the direct light at q; records the winner's shadow query.
-template <bool kDebug>
+// (stride kLanes). Returns the direct light at q; records the winner's shadow query. kND: the most
+// domains the launch's pixels can have (1 + spatial_count, rounded up to 4, 6 or 9).
+//
+// Domain-major evaluation of the balance-heuristic weights: pass 1 moves every domain's sample to q
+// (kept in registers), pass 2 visits each neighbour domain once -- its G-buffer record, closure and
+// wavelength table -- and adds its term to every sample's denominator, pass 3 forms the weights and
+// resamples. Each denominator is summed over the domains in the same order k = 0 .. nd-1 as the
+// sample-by-sample formulation, so every weight, draw and selection is the same.
+template <bool kDebug, int kND>
 __device__ __forceinline__ Spec spatial_reuse_pixel(const FrameArgs &a, const GRec *gbuf, int x, int y,
                                                     const Vertex &vq, const SpecTab &tabq, int *dl, float *cl,
                                                     bool &query, V3 &qo, V3 &qd, float &qtmax, int &ssel) {
     const int pixel = y * a.W + x;
-    const int K = a.spatial_count < kMaxSpatial ? a.spatial_count : kMaxSpatial;
+    const int K = a.spatial_count < kND - 1 ? a.spatial_count : kND - 1;
     const int R = a.spatial_radius;
     const int M = a.n_cand > 1 ? a.n_cand : 1;
     const int cap = 20 * M;
@@ -760,67 +770,97 @@ __device__ __forceinline__ Spec spatial_reuse_pixel(const FrameArgs &a, const GR
         cl[d * kLanes] = float(cc);
         countSum += cc;
     }
-    float wsum = 0.0f;
-    bool has = false;
-    Cand ch;
-    ssel = -1;
-#pragma unroll 1
-    for (int j = 0; j < nd; ++j) {
-        const hdr_reservoir rj = a.res_t[dl[j * kLanes]];
+    // pass 1: each domain's sample moved to q
+    RSample yq[kND];
+    float phJ[kND], jac[kND], phQ[kND], wJ[kND], den[kND];
+    bool ok[kND];
+    bool vis0 = false;
+    const float cl0 = cl[0];
+#pragma unroll
+    for (int j = 0; j < kND; ++j) {
+        ok[j] = false;
+        phJ[j] = jac[j] = phQ[j] = wJ[j] = den[j] = 0.0f;
+        if (j >= nd) continue;
+        const int dj = dl[j * kLanes];
+        const hdr_reservoir rj = a.res_t[dj];
         const uint32_t fj = rj.flags_light >> 24;
+        if (j == 0) vis0 = (fj & HDR_RES_VIS_TESTED) != 0;
         if (!(fj & HDR_RES_HAS_SAMPLE) || !(fj & HDR_RES_HAS_CANDIDATE) || rj.w <= 0.0f) continue;
         const float phatJ = rj.target;
         if (phatJ <= 0.0f) continue;
         const RSample atJ = rsample_of(rj);
-        const V3 posJ = v3(gbuf[dl[j * kLanes]].a.x, gbuf[dl[j * kLanes]].a.y, gbuf[dl[j * kLanes]].a.z);
-        RSample atQ;
-        if (!retarget(a, atJ, posJ, vq.h.pos, atQ)) continue;
-        const Spec reevalQ = reeval_sample(tabq, vq, atQ);
-        const float phatQ = spec_lum(tabq, reevalQ);
+        const V3 posJ = v3(gbuf[dj].a.x, gbuf[dj].a.y, gbuf[dj].a.z);
+        if (!retarget(a, atJ, posJ, vq.h.pos, yq[j])) continue;
+        const float phatQ = spec_lum(tabq, reeval_sample(tabq, vq, yq[j]));
         if (phatQ <= 0.0f) continue;
-        const float jacJQ = j == 0 ? 1.0f : shift_jacobian(atJ, atQ);
+        const float jacJQ = j == 0 ? 1.0f : shift_jacobian(atJ, yq[j]);
         if (jacJQ <= 0.0f) continue;
-        // balance-heuristic denominator over every domain, in q's measure
-        float den = 0.0f;
-#pragma unroll 1
-        for (int k = 0; k < nd; ++k) {
+        ok[j] = true;
+        phJ[j] = phatJ;
+        jac[j] = jacJQ;
+        phQ[j] = phatQ;
+        wJ[j] = rj.w;
+        den[j] = cl0 * (j == 0 ? phatJ / jacJQ : phatQ);  // the k = 0 term of its denominator
+    }
+    // pass 2: the neighbour domains' terms, domain by domain
+#pragma unroll
+    for (int k = 1; k < kND; ++k) {
+        if (k >= nd) continue;
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < kND; ++j) any = any || (ok[j] && j != k);
+        Vertex vk;
+        SpecTab tabk;
+        const bool loaded = any && load_spatial_vertex(a, gbuf, dl[k * kLanes], vk, tabk);
+        const float ck = cl[k * kLanes];
+#pragma unroll
+        for (int j = 0; j < kND; ++j) {
+            if (!ok[j]) continue;
             float pk;
             if (k == j) {
-                pk = phatJ / jacJQ;
-            } else if (k == 0) {
-                pk = phatQ;
+                pk = phJ[j] / jac[j];
             } else {
-                Vertex vk;
-                SpecTab tabk;
                 pk = 0.0f;
-                if (load_spatial_vertex(a, gbuf, dl[k * kLanes], vk, tabk)) {
-                    RSample atK;
-                    if (retarget(a, atQ, vq.h.pos, vk.h.pos, atK))
-                        pk = spec_lum(tabk, reeval_sample(tabk, vk, atK)) * shift_jacobian(atQ, atK);
-                }
+                RSample atK;
+                if (loaded && retarget(a, yq[j], vq.h.pos, vk.h.pos, atK))
+                    pk = spec_lum(tabk, reeval_sample(tabk, vk, atK)) * shift_jacobian(yq[j], atK);
             }
-            den += cl[k * kLanes] * pk;
+            den[j] += ck * pk;
         }
-        if (!(den > 0.0f)) continue;
-        const float w = cl[j * kLanes] * rj.w * phatJ * phatQ / den;
+    }
+    // pass 3: weights and resampling, in domain order
+    float wsum = 0.0f, bestPhQ = 0.0f;
+    int chosen = -1;
+    RSample best = yq[0];
+#pragma unroll
+    for (int j = 0; j < kND; ++j) {
+        if (!ok[j] || !(den[j] > 0.0f)) continue;
+        const float w = cl[j * kLanes] * wJ[j] * phJ[j] * phQ[j] / den[j];
         if (!(w > 0.0f)) continue;
         wsum += w;
         if (srng.next_float() <= w / wsum) {
-            has = true;
-            ch.has_cand = true;
-            ch.delta = atQ.delta;
-            ch.solid = atQ.solid;
-            ch.vis_tested = j == 0 ? (fj & HDR_RES_VIS_TESTED) != 0 : false;
-            ch.light = atQ.light;
-            ch.dir = atQ.dir;
-            ch.color = atQ.color;
-            ch.lnormal = atQ.lnormal;
-            ch.dist = atQ.dist;
-            ch.pdf = atQ.pdf;
-            ch.tp = reevalQ;
-            ch.target = phatQ;
-            ssel = j == 0 ? -1 : ((slotBits >> (4 * (j - 1))) & 0xf);
+            chosen = j;
+            best = yq[j];
+            bestPhQ = phQ[j];
         }
+    }
+    const bool has = chosen >= 0;
+    Cand ch;
+    ssel = -1;
+    if (has) {
+        ch.has_cand = true;
+        ch.delta = best.delta;
+        ch.solid = best.solid;
+        ch.vis_tested = chosen == 0 ? vis0 : false;
+        ch.light = best.light;
+        ch.dir = best.dir;
+        ch.color = best.color;
+        ch.lnormal = best.lnormal;
+        ch.dist = best.dist;
+        ch.pdf = best.pdf;
+        ch.tp = reeval_sample(tabq, vq, best);  // the same evaluation as in pass 1
+        ch.target = bestPhQ;
+        ssel = chosen == 0 ? -1 : ((slotBits >> (4 * (chosen - 1))) & 0xf);
     }
     const int count = countSum < cap ? countSum : cap;
     float W = 0.0f, wstore = 0.0f;

@@ -282,6 +282,20 @@ static void launch_gbuffer(const LaunchShape &ls, unsigned tiles, hipStream_t st
     else hipLaunchKernelGGL((gbuffer_kernel<C, false>), dim3(tiles), dim3(kLanes), 0, st, a, t, n);
 }
 
+// Spatial-reuse launches, compiled for 4, 6 or 9 domains per pixel (spatial_domains).
+template <bool C, bool D>
+static void launch_spatial(int nd, dim3 grid, hipStream_t st, const FrameArgs *a, const int *t, int n) {
+    if (nd <= 4) hipLaunchKernelGGL((spatial_kernel<C, D, 4>), grid, dim3(kLanes), 0, st, a, t, n);
+    else if (nd <= 6) hipLaunchKernelGGL((spatial_kernel<C, D, 6>), grid, dim3(kLanes), 0, st, a, t, n);
+    else hipLaunchKernelGGL((spatial_kernel<C, D, 9>), grid, dim3(kLanes), 0, st, a, t, n);
+}
+template <bool C>
+static void launch_wave_spatial(int nd, dim3 grid, hipStream_t st, const FrameArgs *a, const int *t, int n) {
+    if (nd <= 4) hipLaunchKernelGGL((wave_spatial_kernel<C, 4>), grid, dim3(kLanes), 0, st, a, t, n);
+    else if (nd <= 6) hipLaunchKernelGGL((wave_spatial_kernel<C, 6>), grid, dim3(kLanes), 0, st, a, t, n);
+    else hipLaunchKernelGGL((wave_spatial_kernel<C, 9>), grid, dim3(kLanes), 0, st, a, t, n);
+}
+
 struct hdr_view {
     hdr_context *ctx = nullptr;
     int W = 0, H = 0;
@@ -601,6 +615,7 @@ struct FramePlan {
     bool deferred = false;  // the deferred-shadow path (else the integration megakernel)
     bool lean_pt = false;   // the inlined path tracer (pathlean.h) instead of the megakernel
     bool spatial = false;   // spatial reuse after the temporal pass (deferred and wavefront paths)
+    int spatial_nd = 0;     // its domains per pixel, rounded up to a compiled size (spatial_domains)
     bool wave = false;      // the wavefront path (RIS at every vertex of deep paths, wavefront.h)
     bool wave_lean0 = false;  // its primary vertex cannot draw past RNG output 310 (lean stream there)
     bool wave_traced = false; // BSDF-sampled candidates can see an emitter: bounce and escape rays per vertex
@@ -796,6 +811,7 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
     plan.depth = pipe->max_depth;
     plan.lean_pt = !deferred && a.direct == HDR_DIRECT_MIS && !scene->any_translucent && bound < kMtN - 1;
     plan.spatial = spatial;
+    plan.spatial_nd = spatial ? spatial_domains(pipe->spatial_count) : 0;
     plan.lds = lds;
     plan.tiles = unsigned(((W + 7) / 8) * ((H + 7) / 8));
     return HDR_OK;
@@ -858,8 +874,7 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_INTEGRATE], view->stream));
         if (spatial) {
             HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_SPATIAL], view->stream));
-#define HDR_SPATIAL(C, D) \
-    hipLaunchKernelGGL((spatial_kernel<C, D>), grid, dim3(kLanes), 0, view->stream, da, dt, 1)
+#define HDR_SPATIAL(C, D) launch_spatial<C, D>(plan.spatial_nd, grid, view->stream, da, dt, 1)
             if (view->counting) {
                 if (debug_out) HDR_SPATIAL(true, true);
                 else HDR_SPATIAL(true, false);
@@ -907,10 +922,8 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         // stage's span on this path
         auto spatial_launch = [&]() -> hdr_status {
             HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_SPATIAL], view->stream));
-            if (view->counting)
-                hipLaunchKernelGGL((wave_spatial_kernel<true>), grid, dim3(kLanes), 0, view->stream, da, dt, 1);
-            else
-                hipLaunchKernelGGL((wave_spatial_kernel<false>), grid, dim3(kLanes), 0, view->stream, da, dt, 1);
+            if (view->counting) launch_wave_spatial<true>(plan.spatial_nd, grid, view->stream, da, dt, 1);
+            else launch_wave_spatial<false>(plan.spatial_nd, grid, view->stream, da, dt, 1);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_SPATIAL], view->stream));
             ts.ran |= 1u << HDR_STAGE_SPATIAL;
@@ -1390,6 +1403,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
     size_t off = 0, argOff[kGroups], tileOff[kGroups];
     unsigned tiles[kGroups];
     size_t lds[kGroups];
+    int spatialNd[kGroups];  // spatial-reuse launches: the group's largest domain count
     for (int g = 0; g < kGroups; ++g) {
         argOff[g] = off;
         off = align(off + members[g].size() * sizeof(FrameArgs));
@@ -1397,9 +1411,11 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         off = align(off + (members[g].size() + 1) * sizeof(int));
         tiles[g] = 0;
         lds[g] = 0;
+        spatialNd[g] = 0;
         for (int i : members[g]) {
             tiles[g] += plans[i].tiles;
             lds[g] = std::max(lds[g], plans[i].lds);
+            spatialNd[g] = std::max(spatialNd[g], plans[i].spatial_nd);
         }
     }
     hdr_batch::Slot &sl = b->slots[b->next];
@@ -1454,9 +1470,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         if (N(kDef) > 0)
             hipLaunchKernelGGL((shade_direct_kernel<false, false>), dim3(tiles[kDef]), dim3(kLanes), lds[kDef],
                                b->stream, A(kDef), T(kDef), N(kDef));
-        if (N(kSpa) > 0)
-            hipLaunchKernelGGL((spatial_kernel<false, false>), dim3(tiles[kSpa]), dim3(kLanes), 0, b->stream, A(kSpa),
-                               T(kSpa), N(kSpa));
+        if (N(kSpa) > 0) launch_spatial<false, false>(spatialNd[kSpa], dim3(tiles[kSpa]), b->stream, A(kSpa), T(kSpa), N(kSpa));
         if (N(kMega) > 0)
             hipLaunchKernelGGL((integrate_kernel<false, false>), dim3(tiles[kMega]), dim3(kLanes), lds[kMega],
                                b->stream, A(kMega), T(kMega), N(kMega));
@@ -1476,8 +1490,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
             for (int v = 0; v < launches; ++v) {
                 if (v > 0) launch_wave_shade<false, false>(ls, qblocks, lds[wg], b->stream, A(wg), v, t == 1);
                 if (waveSpatial[t] && v == t)
-                    hipLaunchKernelGGL((wave_spatial_kernel<false>), dim3(tiles[wg]), dim3(kLanes), 0, b->stream,
-                                       A(wg), T(wg), N(wg));
+                    launch_wave_spatial<false>(spatialNd[wg], dim3(tiles[wg]), b->stream, A(wg), T(wg), N(wg));
                 launch_wave_trace<false>(ls, qblocks, b->stream, A(wg), v, t == 1);
             }
             hipLaunchKernelGGL((wave_resolve_kernel<false, false>), dim3(tiles[wg]), dim3(kLanes), 0, b->stream,
@@ -1510,6 +1523,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
             key.push_back(uint64_t(tiles[g]));
             key.push_back(uint64_t(lds[g]));
             key.push_back(uint64_t(members[g].size()));
+            key.push_back(uint64_t(spatialNd[g]));
             for (int i : members[g]) key.push_back(uint64_t(i));
         }
         hdr_batch::Graph &gr = b->graph[b->next];

@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(kLanes) shade_direct_kernel(const FrameArgs *_
 
 // Spatial reuse (deferred.h: spatial_reuse_pixel) and the final colours of the shaded pixels; the
 // winner's shadow query joins the queue as in shade_direct_kernel.
-template <bool kCount, bool kDebug>
+template <bool kCount, bool kDebug, int kND>
 __global__ void __launch_bounds__(kLanes) spatial_kernel(const FrameArgs *__restrict__ args,
                                                          const int *__restrict__ tile_base, int n_views) {
     __shared__ int dl[(kMaxSpatial + 1) * kLanes];
@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(kLanes) spatial_kernel(const FrameArgs *__rest
             V3 qo = v3s(0.0f), qd = v3s(0.0f);
             float qtmax = 0.0f;
             int ssel = -1;
-            const Spec direct = spatial_reuse_pixel<kDebug>(a, gbuf, x, y, vq, tab, dl + lane, cl + lane, query, qo,
+            const Spec direct = spatial_reuse_pixel<kDebug, kND>(a, gbuf, x, y, vq, tab, dl + lane, cl + lane, query, qo,
                                                             qd, qtmax, ssel);
             const float4 t4 = a.spatial_aux[2 * pixel + 1];
             const Spec throughput = Spec{{t4.x, t4.y, t4.z, t4.w}};
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(kLanes) accumulate_deferred_kernel(const Frame
 // = (selected domain slot + 1) << 1; the winner's shadow ray (none when it was tested in an
 // earlier frame) joins vertex 0's shadow queue, whose traversal sets bit 0 of the flags when it
 // is occluded.
-template <bool kCount>
+template <bool kCount, int kND>
 __global__ void __launch_bounds__(kLanes) wave_spatial_kernel(const FrameArgs *__restrict__ args,
                                                               const int *__restrict__ tile_base, int n_views) {
     __shared__ int dl[(kMaxSpatial + 1) * kLanes];
@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(kLanes) wave_spatial_kernel(const FrameArgs *_
             V3 qo = v3s(0.0f), qd = v3s(0.0f);
             float qtmax = 0.0f;
             int ssel = -1;
-            const Spec direct = spatial_reuse_pixel<false>(a, a.gbuf, x, y, vq, tab, dl + lane, cl + lane, query, qo,
+            const Spec direct = spatial_reuse_pixel<false, kND>(a, a.gbuf, x, y, vq, tab, dl + lane, cl + lane, query, qo,
                                                            qd, qtmax, ssel);
             const Spec e = rgb2spec(tab, mv3(vq.c.m->emission));
             a.rgb2[2 * pixel] = spec4(e + direct);
@@ -279,14 +279,26 @@ template __global__ void shade_direct_kernel<true, true>(const FrameArgs *__rest
 template __global__ void shade_direct_kernel<true, false>(const FrameArgs *__restrict__, const int *__restrict__, int);
 template __global__ void shade_direct_kernel<false, true>(const FrameArgs *__restrict__, const int *__restrict__, int);
 template __global__ void shade_direct_kernel<false, false>(const FrameArgs *__restrict__, const int *__restrict__, int);
-template __global__ void spatial_kernel<true, true>(const FrameArgs *__restrict__, const int *__restrict__, int);
-template __global__ void spatial_kernel<true, false>(const FrameArgs *__restrict__, const int *__restrict__, int);
-template __global__ void spatial_kernel<false, true>(const FrameArgs *__restrict__, const int *__restrict__, int);
-template __global__ void spatial_kernel<false, false>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void spatial_kernel<true, true, 4>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void spatial_kernel<true, true, 6>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void spatial_kernel<true, true, 9>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void spatial_kernel<true, false, 4>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void spatial_kernel<true, false, 6>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void spatial_kernel<true, false, 9>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void spatial_kernel<false, true, 4>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void spatial_kernel<false, true, 6>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void spatial_kernel<false, true, 9>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void spatial_kernel<false, false, 4>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void spatial_kernel<false, false, 6>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void spatial_kernel<false, false, 9>(const FrameArgs *__restrict__, const int *__restrict__, int);
 template __global__ void accumulate_deferred_kernel<true>(const FrameArgs *__restrict__, const int *__restrict__, int);
 template __global__ void accumulate_deferred_kernel<false>(const FrameArgs *__restrict__, const int *__restrict__, int);
-template __global__ void wave_spatial_kernel<true>(const FrameArgs *__restrict__, const int *__restrict__, int);
-template __global__ void wave_spatial_kernel<false>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void wave_spatial_kernel<true, 4>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void wave_spatial_kernel<true, 6>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void wave_spatial_kernel<true, 9>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void wave_spatial_kernel<false, 4>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void wave_spatial_kernel<false, 6>(const FrameArgs *__restrict__, const int *__restrict__, int);
+template __global__ void wave_spatial_kernel<false, 9>(const FrameArgs *__restrict__, const int *__restrict__, int);
 template __global__ void wave_resolve_kernel<true, true>(const FrameArgs *__restrict__, const int *__restrict__, int);
 template __global__ void wave_resolve_kernel<true, false>(const FrameArgs *__restrict__, const int *__restrict__, int);
 template __global__ void wave_resolve_kernel<false, true>(const FrameArgs *__restrict__, const int *__restrict__, int);

@@ -30,12 +30,12 @@ __global__ void pt_lean_kernel(const FrameArgs *__restrict__ args, const int *__
 template <bool kCount, bool kDebug>
 __global__ void shade_direct_kernel(const FrameArgs *__restrict__ args, const int *__restrict__ tile_base,
                                     int n_views);
-template <bool kCount, bool kDebug>
+template <bool kCount, bool kDebug, int kND>
 __global__ void spatial_kernel(const FrameArgs *__restrict__ args, const int *__restrict__ tile_base, int n_views);
 template <bool kDebug>
 __global__ void accumulate_deferred_kernel(const FrameArgs *__restrict__ args, const int *__restrict__ tile_base,
                                            int n_views);
-template <bool kCount>
+template <bool kCount, int kND>
 __global__ void wave_spatial_kernel(const FrameArgs *__restrict__ args, const int *__restrict__ tile_base,
                                     int n_views);
 template <bool kCount, bool kDebug>
