@@ -229,7 +229,7 @@ constexpr int kStackDepth = 64;
 template <bool kTa>
 struct PrivStack {
     int ref[kStackDepth];
-    float ta[kStackDepth];
+    float ta[kTa ? kStackDepth : 1];  // any-hit queries keep no entry distances
     int sp;
     __device__ __forceinline__ void reset() { sp = 0; }
     __device__ __forceinline__ bool empty() const { return sp == 0; }

@@ -7,6 +7,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -301,6 +302,7 @@ struct hdr_view {
     int W = 0, H = 0;
     hipStream_t stream = nullptr;
     DevBuf res, accum, color, frame, dbg, depth, albedo, normal, counters, spill, gbuf, rgb2, flags, queue, qcount;
+    DevBuf status;  // device status word (kStatusRngOverrun, ...), read at the view's synchronisations
     DevBuf res_t, saux;  // spatial reuse: post-temporal reservoirs, per-pixel {ray dir, u}, {throughput}
     DevBuf wstate, whit, wray, wcq, walive, wcnt;  // wavefront path: path records, hits, queues
     DevBuf wesc, wesq;                             // traced frames: escape masks, escape-ray queue
@@ -335,6 +337,18 @@ struct hdr_view {
     double stage_ms[HDR_STAGE_COUNT] = {};
     int32_t stage_n[HDR_STAGE_COUNT] = {};
 };
+
+// The view's device status word, read after its streams have been synchronised: a set
+// kStatusRngOverrun bit means some frame since the last hdr_view_reset drew past the RNG outputs its
+// stream was set up for, so its results are wrong.
+static hdr_status view_status(hdr_view *v) {
+    unsigned int word = 0;
+    HIP_TRY(hipMemcpy(&word, v->status.p, sizeof word, hipMemcpyDeviceToHost));
+    if (word & kStatusRngOverrun)
+        return fail(HDR_ERR_CAPACITY, "a pixel drew past the random numbers its stream was set up for (RNG spill "
+                                      "slot missing); the view's frames since its last reset are wrong");
+    return HDR_OK;
+}
 
 static hdr_status harvest(hdr_view *v, hdr_view::StageSlot &s) {
     if (!s.pending) return HDR_OK;
@@ -528,6 +542,7 @@ hdr_status hdr_view_create(hdr_context *ctx, int32_t width, int32_t height, hdr_
     HIP_TRY(v->accum.alloc(P * sizeof(float4)));
     HIP_TRY(v->color.alloc(P * sizeof(float4)));
     HIP_TRY(v->counters.alloc(sizeof(hdr_frame_counters)));
+    HIP_TRY(v->status.alloc(sizeof(unsigned int)));
     HIP_TRY(v->gbuf.alloc(2 * P * sizeof(GRec)));  // two halves, by frame parity
     HIP_TRY(v->qcount.alloc(sizeof(unsigned int)));
     HIP_TRY(v->dargs.alloc(hdr_view::kRing * sizeof(FrameArgs)));
@@ -551,6 +566,7 @@ hdr_status hdr_view_create(hdr_context *ctx, int32_t width, int32_t height, hdr_
     HIP_TRY(hipMemsetAsync(v->accum.p, 0, v->accum.n, v->stream));
     HIP_TRY(hipMemsetAsync(v->color.p, 0, v->color.n, v->stream));
     HIP_TRY(hipMemsetAsync(v->counters.p, 0, v->counters.n, v->stream));
+    HIP_TRY(hipMemsetAsync(v->status.p, 0, v->status.n, v->stream));
     HIP_TRY(hipStreamSynchronize(v->stream));
     *out = v.release();
     return HDR_OK;
@@ -581,6 +597,7 @@ hdr_status hdr_view_reset(hdr_view *view) {
     HIP_TRY(hipSetDevice(view->ctx->device));
     HIP_TRY(hipMemsetAsync(view->res.p, 0, view->res.n, view->stream));
     HIP_TRY(hipMemsetAsync(view->accum.p, 0, view->accum.n, view->stream));
+    HIP_TRY(hipMemsetAsync(view->status.p, 0, view->status.n, view->stream));
     view->frames = 0;
     return HDR_OK;
 }
@@ -690,6 +707,7 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
     a.firefly = pipe->enable_firefly_filter != 0;
     a.inv_n = 1.0f / float(frame->frame_index + 1);
     a.res = static_cast<hdr_reservoir *>(view->res.p);
+    a.status = static_cast<unsigned int *>(view->status.p);
     a.accum = static_cast<float4 *>(view->accum.p);
     a.color = static_cast<float4 *>(view->color.p);
 
@@ -703,7 +721,11 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
         bound += pipe->max_depth * perVertex;
         bound0 += perVertex;
     }
-    if (scene->any_translucent || bound >= kMtN) {
+    // HDRGPU_TEST_WITHHOLD_SPILL=1: a test hook that withholds the spill slots a frame needs, to check
+    // that the kernels report the overrun (their draws past the slot return 0; no memory is touched)
+    const char *withholdEnv = std::getenv("HDRGPU_TEST_WITHHOLD_SPILL");
+    const bool withholdSpill = withholdEnv && withholdEnv[0] == '1';
+    if ((scene->any_translucent || bound >= kMtN) && !withholdSpill) {
         if (hdr_status s = ensure(view->spill, P * 2 * kMtN * sizeof(uint64_t))) return s;
         a.spill = static_cast<uint64_t *>(view->spill.p);
     }
@@ -1016,7 +1038,10 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         HIP_TRY(hipMemcpyAsync(debug_out, view->dbg.p, P * sizeof(hdr_pixel_debug), hipMemcpyDeviceToHost,
                                view->stream));
     }
-    if (color_out || debug_out) HIP_TRY(hipStreamSynchronize(view->stream));
+    if (color_out || debug_out) {
+        HIP_TRY(hipStreamSynchronize(view->stream));
+        return view_status(view);
+    }
     return HDR_OK;
 }
 
@@ -1025,7 +1050,7 @@ hdr_status hdr_view_synchronize(hdr_view *view) {
     HIP_TRY(hipSetDevice(view->ctx->device));
     HIP_TRY(hipStreamSynchronize(view->stream));
     HIP_TRY(hipStreamSynchronize(view->gstream));
-    return HDR_OK;
+    return view_status(view);
 }
 
 hdr_status hdr_view_last_timing(hdr_view *view, float *gbuffer_ms, float *integrate_ms, float *frame_ms) {
@@ -1066,7 +1091,7 @@ hdr_status hdr_view_read_color(hdr_view *view, float *rgba_out) {
     HIP_TRY(hipSetDevice(view->ctx->device));
     HIP_TRY(hipMemcpyAsync(rgba_out, view->color.p, view->color.n, hipMemcpyDeviceToHost, view->stream));
     HIP_TRY(hipStreamSynchronize(view->stream));
-    return HDR_OK;
+    return view_status(view);
 }
 
 hdr_status hdr_view_read_reservoirs(hdr_view *view, hdr_reservoir *out) {
@@ -1074,7 +1099,7 @@ hdr_status hdr_view_read_reservoirs(hdr_view *view, hdr_reservoir *out) {
     HIP_TRY(hipSetDevice(view->ctx->device));
     HIP_TRY(hipMemcpyAsync(out, view->res.p, view->res.n, hipMemcpyDeviceToHost, view->stream));
     HIP_TRY(hipStreamSynchronize(view->stream));
-    return HDR_OK;
+    return view_status(view);
 }
 
 hdr_status hdr_view_read_aovs(hdr_view *view, float *depth, float *albedo, float *normal) {

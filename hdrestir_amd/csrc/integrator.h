@@ -47,6 +47,7 @@ struct FrameArgs {
     float *aov_depth, *aov_albedo, *aov_normal;
     uint64_t *spill;                   // 2 x 312 words per pixel, or null
     unsigned long long *counters;      // hdr_frame_counters as 9 x u64, or null
+    unsigned int *status;              // the view's status word: kStatusRngOverrun, ... (atomicOr)
     // per-view work buffers
     GRec *gbuf;                        // G-buffer, one record per pixel
     float4 *rgb2;                      // deferred path: visible / occluded colour per pixel
@@ -214,6 +215,15 @@ struct PixelCtx {
     double *wlds;      // this lane's candidate weights: wlds[k * kLanes]
     uint64_t *slot2;   // RNG fork spill slot
 };
+
+// Bits of a view's device status word. kStatusRngOverrun: a pixel drew past the RNG outputs its
+// stream was set up for (the lean stream past output 310, a stream without a spill slot past
+// 311 or 623); such draws return 0 instead of the engine's value, and the host reports the view as
+// failed (HDR_ERR_CAPACITY) at its next synchronisation.
+constexpr unsigned int kStatusRngOverrun = 1u;
+__device__ __forceinline__ void flag_rng_overrun(const FrameArgs &a, bool overrun) {
+    if (overrun && a.status) atomicOr(a.status, kStatusRngOverrun);
+}
 
 constexpr int kLanes = 64;  // threads per block; candidate weights live at wlds[k * kLanes + lane]
 

@@ -62,8 +62,8 @@ __device__ __forceinline__ void shade_direct_pixel(const FrameArgs &a, const GRe
         co = co * a.exposure_mul;
     }
     if (!(a.spatial_count > 0 && gHit)) {
-        rgb2[2 * pixel] = make_float4(cv.x, cv.y, cv.z, 1.0f);
-        rgb2[2 * pixel + 1] = make_float4(co.x, co.y, co.z, 1.0f);
+        rgb2[pixel] = make_float4(cv.x, cv.y, cv.z, 1.0f);
+        rgb2[size_t(a.W) * a.H + pixel] = make_float4(co.x, co.y, co.z, 1.0f);
         flags[pixel] = query ? 1u : 0u;
     }
     if (query) {
@@ -85,6 +85,7 @@ __device__ __forceinline__ void shade_direct_pixel(const FrameArgs &a, const GRe
         a.dbg[pixel] = d;
     }
     draws = rng.k;
+    flag_rng_overrun(a, rng.k > 311u);  // the lean stream stops at output 310
 }
 
 template <bool kCount, bool kDebug>
@@ -140,8 +141,8 @@ __global__ void __launch_bounds__(kLanes) spatial_kernel(const FrameArgs *__rest
                 cv = cv * a.exposure_mul;
                 co = co * a.exposure_mul;
             }
-            rgb2[2 * pixel] = make_float4(cv.x, cv.y, cv.z, 1.0f);
-            rgb2[2 * pixel + 1] = make_float4(co.x, co.y, co.z, 1.0f);
+            rgb2[pixel] = make_float4(cv.x, cv.y, cv.z, 1.0f);
+            rgb2[size_t(a.W) * a.H + pixel] = make_float4(co.x, co.y, co.z, 1.0f);
             flags[pixel] = query ? 1u : 0u;
             if (query) {
                 q.o = make_float4(qo.x, qo.y, qo.z, qtmax);
@@ -162,7 +163,7 @@ __global__ void __launch_bounds__(kLanes) accumulate_deferred_kernel(const Frame
     HDR_TILE(args, tile_base, n_views)
     if (!valid) return;
     const int i = y * a.W + x;
-    const float4 c4 = a.rgb2[2 * i + (a.flags[i] == 3u ? 1 : 0)];
+    const float4 c4 = a.rgb2[(a.flags[i] == 3u ? size_t(a.W) * a.H : size_t(0)) + i];
     if (kDebug) {
         a.dbg[i].sample_rgb[0] = c4.x;
         a.dbg[i].sample_rgb[1] = c4.y;
@@ -183,7 +184,7 @@ __global__ void __launch_bounds__(kLanes) accumulate_deferred_kernel(const Frame
 
 // Spatial reuse of the primary vertex (deferred.h: spatial_reuse_pixel), after wave_shade0 has
 // stored every pixel's post-temporal reservoir. Per shaded pixel: the primary vertex's emission
-// plus the reused direct light (rgb2[2p]) and plus nothing (rgb2[2p+1]), as spectra, and flags[p]
+// plus the reused direct light (rgb2[p]) and plus nothing (rgb2[P + p]), as spectra, and flags[p]
 // = (selected domain slot + 1) << 1; the winner's shadow ray (none when it was tested in an
 // earlier frame) joins vertex 0's shadow queue, whose traversal sets bit 0 of the flags when it
 // is occluded.
@@ -206,8 +207,8 @@ __global__ void __launch_bounds__(kLanes) wave_spatial_kernel(const FrameArgs *_
             const Spec direct = spatial_reuse_pixel<false, kND>(a, a.gbuf, x, y, vq, tab, dl + lane, cl + lane, query, qo,
                                                            qd, qtmax, ssel);
             const Spec e = rgb2spec(tab, mv3(vq.c.m->emission));
-            a.rgb2[2 * pixel] = spec4(e + direct);
-            a.rgb2[2 * pixel + 1] = spec4(e + spec(0.0f));
+            a.rgb2[pixel] = spec4(e + direct);
+            a.rgb2[size_t(a.W) * a.H + pixel] = spec4(e + spec(0.0f));
             a.flags[pixel] = uint32_t(ssel + 1) << 1;
             if (query) {
                 q.o = make_float4(qo.x, qo.y, qo.z, qtmax);
@@ -238,7 +239,7 @@ __global__ void __launch_bounds__(kLanes) wave_resolve_kernel(const FrameArgs *_
         Spec L;
         if (hit && a.spatial_count > 0) {  // outer * ((emission + reused direct) + later vertices)
             const uint32_t f = a.flags[pixel];
-            const Spec e = spec_of(a.rgb2[2 * pixel + (f & 1u)]);
+            const Spec e = spec_of(a.rgb2[((f & 1u) ? P : size_t(0)) + pixel]);
             L = spec_of(w[6 * P + pixel]) * (e + spec_of(w[4 * P + pixel]));
             if (kDebug) shash = fnv(shash, uint32_t(int(f >> 1) - 1));
         } else {
@@ -259,6 +260,7 @@ __global__ void __launch_bounds__(kLanes) wave_resolve_kernel(const FrameArgs *_
             a.color[pixel] = make_float4(acc.x * a.inv_n, acc.y * a.inv_n, acc.z * a.inv_n, acc.w * a.inv_n);
         }
         draws = __float_as_uint(r2.z);
+        flag_rng_overrun(a, !wave_slot(a, pixel) && draws > 624u);  // outputs from 624 on need the slot
         if (kDebug) {
             hdr_pixel_debug &d = a.dbg[pixel];
             d.draws = draws;

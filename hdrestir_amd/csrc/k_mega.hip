@@ -55,6 +55,7 @@ __device__ __forceinline__ void integrate_pixel(const FrameArgs &a, const GRec *
         a.dbg[pixel] = d;
     }
     draws = rng.k;
+    flag_rng_overrun(a, !slot && rng.k > 312u);  // outputs from 312 on need the spill slot
 }
 
 // IntegrationPass + (fused) AccumulationPass for one 8x8 pixel tile per wave.

@@ -60,6 +60,7 @@ __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__
             // traced frames: the throughput is recorded where vertex 0 is shaded
             if (!kTraced && p.primary_hit) a.spatial_aux[2 * pixel + 1] = spec4(p.outer);
         }
+        if (kLean) flag_rng_overrun(a, p.rng.k > 311u);  // the lean stream stops at output 310
         store_path(a, pixel, p, true);
         code = query_code(a, pixel);
         if (kDebug) {
@@ -77,6 +78,7 @@ __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__
         const unsigned int ai = wave_append(cnt + 4, ray, lane);
         if (ray) alive_list(args[0], 0)[ai] = code;
         wave_emit_escapes(cnt + 6, args[0].wesq, tab, gv, gr, k0, nc, code, lane);
+        if (kLean && valid && ray) flag_rng_overrun(a, gr.k > 311u);  // the generation's lean stream too
         ray = bray;
     }
     const unsigned int ri = wave_append(cnt + 0, ray, lane);

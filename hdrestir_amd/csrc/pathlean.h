@@ -196,6 +196,7 @@ __device__ __forceinline__ void pt_lean_pixel(const FrameArgs &a, int x, int y, 
         a.dbg[pixel] = d;
     }
     draws = rng.k;
+    flag_rng_overrun(a, rng.k > 311u);  // the lean stream stops at output 310
 }
 
 }  // namespace hdr
