@@ -24,7 +24,7 @@ IP = C.POINTER(C.c_int32)
 
 
 class MeshDesc(C.Structure):
-    _fields_ = [("points", FP), ("num_points", C.c_int32),
+    _fields_ = [("struct_size", C.c_uint32), ("points", FP), ("num_points", C.c_int32),
                 ("indices", IP), ("num_triangles", C.c_int32),
                 ("normals", FP), ("num_normals", C.c_int32),
                 ("uvs", FP), ("num_uvs", C.c_int32),
@@ -32,8 +32,8 @@ class MeshDesc(C.Structure):
 
 
 class MaterialDesc(C.Structure):
-    _fields_ = [("diffuse_color", F3), ("metallic", C.c_float), ("roughness", C.c_float),
-                ("specular_color", F3), ("specular", C.c_float), ("opacity", C.c_float),
+    _fields_ = [("struct_size", C.c_uint32), ("diffuse_color", F3), ("metallic", C.c_float),
+                ("roughness", C.c_float), ("specular_color", F3), ("specular", C.c_float), ("opacity", C.c_float),
                 ("ior", C.c_float), ("transmission", C.c_float), ("transmission_color", F3),
                 ("transmission_depth", C.c_float), ("emission_color", F3), ("emission", C.c_float),
                 ("coat", C.c_float), ("coat_color", F3), ("coat_roughness", C.c_float),
@@ -43,13 +43,13 @@ class MaterialDesc(C.Structure):
 
 
 class InstanceDesc(C.Structure):
-    _fields_ = [("transform", C.c_float * 16), ("mesh", C.c_int32), ("material", C.c_int32),
-                ("light", C.c_int32), ("_pad", C.c_int32)]
+    _fields_ = [("struct_size", C.c_uint32), ("transform", C.c_float * 16), ("mesh", C.c_int32),
+                ("material", C.c_int32), ("light", C.c_int32), ("_pad", C.c_int32)]
 
 
 class LightDesc(C.Structure):
-    _fields_ = [("type", C.c_int32), ("color", F3), ("intensity", C.c_float), ("exposure", C.c_float),
-                ("transform", C.c_double * 16), ("width", C.c_float), ("height", C.c_float),
+    _fields_ = [("struct_size", C.c_uint32), ("type", C.c_int32), ("color", F3), ("intensity", C.c_float),
+                ("exposure", C.c_float), ("transform", C.c_double * 16), ("width", C.c_float), ("height", C.c_float),
                 ("shaping_cone_angle", C.c_float), ("shaping_cone_softness", C.c_float),
                 ("dome_pixels", FP), ("dome_width", C.c_int32), ("dome_height", C.c_int32)]
 
@@ -155,7 +155,7 @@ def reservoir_dtype():
 
 class ServerDesc(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("max_active", C.c_int32), ("hbm_budget_bytes", C.c_uint64),
-                ("batched", C.c_int32), ("_reserved", C.c_int32)]
+                ("batched", C.c_int32), ("own_thread", C.c_int32)]
 
 
 class LaunchConfig(C.Structure):
@@ -185,7 +185,7 @@ class RequestDesc(C.Structure):
 
 class RequestResult(C.Structure):
     _fields_ = [("id", C.c_uint64), ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
-                ("_reserved", C.c_int32), ("queued_ms", C.c_double), ("latency_ms", C.c_double)]
+                ("status", C.c_int32), ("queued_ms", C.c_double), ("latency_ms", C.c_double)]
 
 
 class ServerStats(C.Structure):

@@ -636,22 +636,38 @@ struct FramePlan {
     bool wave = false;      // the wavefront path (RIS at every vertex of deep paths, wavefront.h)
     bool wave_lean0 = false;  // its primary vertex cannot draw past RNG output 310 (lean stream there)
     bool wave_traced = false; // BSDF-sampled candidates can see an emitter: bounce and escape rays per vertex
+    bool spill = false;       // RNG spill slots (a pixel can draw past output 311)
     size_t escapes = 0;       // traced frames: escape-queue capacity (rays)
     int depth = 0;          // path vertices (max_depth)
     size_t lds = 0;         // dynamic LDS of the shading / integration kernel
     unsigned tiles = 0;     // 8x8 tiles of the view
 };
 
-// Validates a frame, makes sure the view's buffers for it exist and fills its FrameArgs.
-// shared_queues: a batch substitutes its own wavefront queues, so the view's are not allocated.
-static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
-                                const hdr_pipeline_desc *pipe, bool debug, FrameArgs &a, FramePlan &plan,
-                                bool shared_queues = false) {
-    if (!view || !scene || !frame || !pipe) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
-    if (frame->struct_size != sizeof(hdr_frame_desc)) return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_frame_desc size");
+// Device bytes of the buffers a frame's plan needs beyond those every view holds (0: not needed).
+struct FrameNeeds {
+    size_t spill = 0, frame = 0, dbg = 0, res_t = 0, saux = 0, rgb2 = 0, flags = 0, queue = 0;
+    size_t wstate = 0, whit = 0, wesc = 0;            // the wavefront's per-view state
+    size_t wray = 0, wcq = 0, walive = 0, wesq = 0;   // and its queues (a batch's are shared)
+    size_t sum() const {
+        return spill + frame + dbg + res_t + saux + rgb2 + flags + queue + wstate + whit + wesc + wray + wcq +
+               walive + wesq;
+    }
+};
+
+// Device bytes every view holds from hdr_view_create on: reservoirs, accumulation, colour, the two
+// G-buffer halves, counters, status word, argument ring and tile table.
+static size_t view_base_bytes(size_t P) {
+    return P * (sizeof(hdr_reservoir) + 2 * sizeof(float4) + 2 * sizeof(GRec)) + sizeof(hdr_frame_counters) +
+           2 * sizeof(unsigned int) + 16 * sizeof(FrameArgs) + 2 * sizeof(int);
+}
+
+// Validates a pipeline on a scene and decides the frame's kernel path and the buffers it needs.
+// Host-only: no device call; hdr_view_plan_bytes and the server's admission use it too.
+static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_pipeline_desc *pipe, bool debug,
+                                FramePlan &plan, FrameNeeds &need) {
+    if (!scene || !pipe) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
     if (pipe->struct_size != sizeof(hdr_pipeline_desc))
         return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_pipeline_desc size");
-    if (view->ctx->cancelled) return fail(HDR_ERR_CANCELLED, "render cancelled");
     if (pipe->direct_light != HDR_DIRECT_MIS && pipe->direct_light != HDR_DIRECT_RIS)
         return fail(HDR_ERR_INVALID_ARGUMENT, "unknown direct_light kind");
     if (pipe->max_depth < 1 || pipe->max_depth > 1024) return fail(HDR_ERR_INVALID_ARGUMENT, "max_depth");
@@ -661,9 +677,106 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
         return fail(HDR_ERR_INVALID_ARGUMENT, "spatial_count must be in [0, 8]");
     if (pipe->spatial_count > 0 && !(pipe->spatial_radius >= 1.0f && pipe->spatial_radius <= 4096.0f))
         return fail(HDR_ERR_INVALID_ARGUMENT, "spatial_radius must be in [1, 4096] pixels");
-    HIP_TRY(hipSetDevice(view->ctx->device));
+    const size_t P = size_t(W) * H;
+    plan = FramePlan();
+    need = FrameNeeds();
+    // technique split (ris_direct_light_integrator.cpp:93-100); the environment is always one of the
+    // lights, so the light sampler covers it and no separate sky technique exists
+    const bool ris = pipe->direct_light == HDR_DIRECT_RIS;
+    const int M = pipe->candidate_count;
+    const int nNee = std::max(1, M / 2), nBsdf = std::max(0, M - nNee), nCand = nNee + nBsdf;
+    const bool traceGenerated = scene->ds.has_mesh_lights || scene->ds.environment >= 0;
+    const bool traceLastConnection = scene->ds.has_emission || scene->ds.environment >= 0;
+    // upper bound of RNG draws per pixel; past 311 the engine needs its spill slots
+    int bound = 5, bound0 = 5;  // bound0: through the primary vertex only
+    {
+        const int sb = scene->max_sample_draws_per_vertex;
+        int perVertex = 1 + sb + 1;  // opacity, bounce sample, roulette
+        if (!ris) perVertex += 3;
+        else perVertex += nNee * 3 + std::max(0, nBsdf - 1) * sb + nCand + 1;
+        bound += pipe->max_depth * perVertex;
+        bound0 += perVertex;
+    }
+    plan.spill = scene->any_translucent || bound >= kMtN;
+    if (plan.spill) need.spill = P * 2 * kMtN * sizeof(uint64_t);
+    if (pipe->enable_firefly_filter) need.frame = P * sizeof(float4);
+    if (debug) need.dbg = P * sizeof(hdr_pixel_debug);
+    // The deferred-shadow path applies when the winner's shadow ray is the only ray after the
+    // G-buffer: RIS at depth 1, skipVisibility, no translucent material (no pass-through rays),
+    // nothing BSDF-sampled can reach an emitter (no emissive material, mesh light or environment),
+    // and no pixel draws past RNG output 310 (the lean stream of that path has no spill slot).
+    const bool skipVis = pipe->skip_visibility != 0;
+    const bool deferred = ris && pipe->max_depth == 1 && skipVis && !scene->any_translucent && !traceGenerated &&
+                          !traceLastConnection && bound < kMtN - 1;
+    // The wavefront path: the same conditions at any depth and RNG range (frames the deferred path
+    // does not take). With an environment or emissive materials, BSDF-sampled candidates can see an
+    // emitter: the wavefront then emits each vertex's bounce ray and generated candidates' escape
+    // rays before its RIS (traced frames; at most 32 candidates, one bit each in the escape mask).
+    const bool needTraced = traceGenerated || traceLastConnection;
+    const bool wave = !deferred && ris && skipVis && !scene->any_translucent && !scene->ds.has_mesh_lights &&
+                      !(needTraced && nCand > 32);
+    const bool waveTraced = wave && needTraced;
+    // Spatial reuse (an extension) runs on the deferred and wavefront paths with temporal reservoirs.
+    const bool spatial = pipe->spatial_count > 0;
+    if (spatial && !((deferred || wave) && pipe->use_reservoir))
+        return fail(HDR_ERR_UNSUPPORTED,
+                    "spatial reuse needs ReSTIR (use_reservoir) with skip_visibility, opaque materials and no mesh "
+                    "light");
+    if (spatial) {
+        need.res_t = P * sizeof(hdr_reservoir);
+        need.saux = P * 2 * sizeof(float4);
+    }
+    if (deferred) {
+        need.rgb2 = P * 2 * sizeof(float4);
+        need.flags = P * sizeof(uint32_t);
+        need.queue = P * sizeof(ShadowQuery);
+    }
+    if (wave) {
+        need.wstate = P * kWaveFields * sizeof(float4);
+        // traced frames: hit records of odd and even vertices, escape masks and the escape queue
+        need.whit = (waveTraced ? 2 : 1) * P * sizeof(GRec);
+        if (waveTraced) {
+            plan.escapes = traceGenerated ? P * size_t(nCand - nNee) : 0;
+            need.wesc = P * sizeof(unsigned int);
+            need.wesq = std::max<size_t>(1, plan.escapes) * 2 * sizeof(float4);
+        }
+        need.wray = P * 2 * sizeof(float4);
+        need.queue = P * sizeof(ShadowQuery);
+        need.wcq = P * sizeof(float4);
+        need.walive = 2 * P * sizeof(unsigned int) + 8 * sizeof(unsigned int);  // two parities + counts
+        if (spatial) {  // per pixel: emission + reused direct, visible / occluded; visibility + slot
+            need.rgb2 = P * 2 * sizeof(float4);
+            need.flags = P * sizeof(uint32_t);
+        }
+    }
+    // candidate weights in LDS: the NEE candidates on the deferred path (its BSDF candidates carry
+    // no weight), every candidate on the general path
+    plan.lds = size_t((deferred || (wave && !waveTraced)) ? nNee : (ris ? nCand : 1)) * kLanes * sizeof(double);
+    plan.deferred = deferred;
+    plan.wave = wave;
+    plan.wave_lean0 = wave && bound0 < kMtN - 1;
+    plan.wave_traced = waveTraced;
+    plan.depth = pipe->max_depth;
+    plan.lean_pt = !deferred && !ris && !scene->any_translucent && bound < kMtN - 1;
+    plan.spatial = spatial;
+    plan.spatial_nd = spatial ? spatial_domains(pipe->spatial_count) : 0;
+    plan.tiles = unsigned(((W + 7) / 8) * ((H + 7) / 8));
+    return HDR_OK;
+}
+
+// Validates a frame, makes sure the view's buffers for it exist and fills its FrameArgs.
+// shared_queues: a batch substitutes its own wavefront queues, so the view's are not allocated.
+static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
+                                const hdr_pipeline_desc *pipe, bool debug, FrameArgs &a, FramePlan &plan,
+                                bool shared_queues = false) {
+    if (!view || !scene || !frame || !pipe) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    if (frame->struct_size != sizeof(hdr_frame_desc)) return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_frame_desc size");
+    if (view->ctx->cancelled) return fail(HDR_ERR_CANCELLED, "render cancelled");
     const int W = view->W, H = view->H;
     const size_t P = size_t(W) * H;
+    FrameNeeds need;
+    if (hdr_status st = plan_pipeline(scene, W, H, pipe, debug, plan, need)) return st;
+    HIP_TRY(hipSetDevice(view->ctx->device));
 
     std::memset(&a, 0, sizeof a);
     a.s = scene->ds;
@@ -694,8 +807,6 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
     a.enable_ss = pipe->enable_subsurface != 0;
     a.ibl_bg = pipe->render_ibl_background != 0;
     a.roulette = pipe->roulette_aggressiveness;
-    // technique split (ris_direct_light_integrator.cpp:93-100); the environment is always one of the
-    // lights, so the light sampler covers it and no separate sky technique exists.
     const int M = pipe->candidate_count;
     a.n_nee = std::max(1, M / 2);
     a.n_sky = 0;
@@ -711,30 +822,20 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
     a.accum = static_cast<float4 *>(view->accum.p);
     a.color = static_cast<float4 *>(view->color.p);
 
-    // upper bound of RNG draws per pixel; past 311 the engine needs its spill slots
-    int bound = 5, bound0 = 5;  // bound0: through the primary vertex only
-    {
-        const int sb = scene->max_sample_draws_per_vertex;
-        int perVertex = 1 + sb + 1;  // opacity, bounce sample, roulette
-        if (a.direct == HDR_DIRECT_MIS) perVertex += 3;
-        else perVertex += a.n_nee * 3 + std::max(0, a.n_bsdf - 1) * sb + a.n_cand + 1;
-        bound += pipe->max_depth * perVertex;
-        bound0 += perVertex;
-    }
     // HDRGPU_TEST_WITHHOLD_SPILL=1: a test hook that withholds the spill slots a frame needs, to check
     // that the kernels report the overrun (their draws past the slot return 0; no memory is touched)
     const char *withholdEnv = std::getenv("HDRGPU_TEST_WITHHOLD_SPILL");
     const bool withholdSpill = withholdEnv && withholdEnv[0] == '1';
-    if ((scene->any_translucent || bound >= kMtN) && !withholdSpill) {
-        if (hdr_status s = ensure(view->spill, P * 2 * kMtN * sizeof(uint64_t))) return s;
+    if (need.spill && !withholdSpill) {
+        if (hdr_status s = ensure(view->spill, need.spill)) return s;
         a.spill = static_cast<uint64_t *>(view->spill.p);
     }
-    if (a.firefly) {
-        if (hdr_status s = ensure(view->frame, P * sizeof(float4))) return s;
+    if (need.frame) {
+        if (hdr_status s = ensure(view->frame, need.frame)) return s;
         a.frame_rgb = static_cast<float4 *>(view->frame.p);
     }
-    if (debug) {
-        if (hdr_status s = ensure(view->dbg, P * sizeof(hdr_pixel_debug))) return s;
+    if (need.dbg) {
+        if (hdr_status s = ensure(view->dbg, need.dbg)) return s;
         a.dbg = static_cast<hdr_pixel_debug *>(view->dbg.p);
     }
     // AOVs are written every frame once allocated (first read allocates them)
@@ -744,71 +845,43 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
         a.aov_normal = static_cast<float *>(view->normal.p);
     }
     a.gbuf = static_cast<GRec *>(view->gbuf.p) + size_t(view->gpar & 1) * P;  // this frame's half
-    // The deferred-shadow path applies when the winner's shadow ray is the only ray after the
-    // G-buffer: RIS at depth 1, skipVisibility, no translucent material (no pass-through rays),
-    // nothing BSDF-sampled can reach an emitter (no emissive material, mesh light or environment),
-    // and no pixel draws past RNG output 310 (the lean stream of that path has no spill slot).
-    const bool deferred = a.direct == HDR_DIRECT_RIS && a.max_depth == 1 && a.skip_vis && !scene->any_translucent &&
-                          !a.trace_generated && !a.trace_last_connection && bound < kMtN - 1;
-    // The wavefront path: the same conditions at any depth and RNG range (frames the deferred path
-    // does not take).
-    // With an environment or emissive materials, BSDF-sampled candidates can see an emitter: the
-    // wavefront then emits each vertex's bounce ray and generated candidates' escape rays before
-    // its RIS (traced frames; at most 32 candidates, one bit each in the escape mask).
-    const bool needTraced = a.trace_generated || a.trace_last_connection;
-    const bool wave = !deferred && a.direct == HDR_DIRECT_RIS && a.skip_vis && !scene->any_translucent &&
-                      !scene->ds.has_mesh_lights && !(needTraced && a.n_cand > 32);
-    const bool waveTraced = wave && needTraced;
-    // Spatial reuse (an extension) runs on the deferred and wavefront paths with temporal
-    // reservoirs.
-    const bool spatial = pipe->spatial_count > 0;
-    if (spatial && !((deferred || wave) && a.use_res))
-        return fail(HDR_ERR_UNSUPPORTED,
-                    "spatial reuse needs ReSTIR (use_reservoir) with skip_visibility, opaque materials and no mesh "
-                    "light");
-    if (spatial) {
-        if (hdr_status st = ensure(view->res_t, P * sizeof(hdr_reservoir))) return st;
-        if (hdr_status st = ensure(view->saux, P * 2 * sizeof(float4))) return st;
+    if (plan.spatial) {
+        if (hdr_status st = ensure(view->res_t, need.res_t)) return st;
+        if (hdr_status st = ensure(view->saux, need.saux)) return st;
         a.res_t = static_cast<hdr_reservoir *>(view->res_t.p);
         a.spatial_aux = static_cast<float4 *>(view->saux.p);
         a.spatial_count = pipe->spatial_count;
         a.spatial_radius = int(pipe->spatial_radius);
     }
-    // candidate weights in LDS: the NEE candidates on the deferred path (its BSDF candidates carry
-    // no weight), every candidate on the general path
-    const size_t lds = size_t((deferred || (wave && !waveTraced)) ? a.n_nee
-                                                                  : (a.direct == HDR_DIRECT_RIS ? a.n_cand : 1)) *
-                       kLanes * sizeof(double);
-    if (deferred) {
-        if (hdr_status st = ensure(view->rgb2, P * 2 * sizeof(float4))) return st;
-        if (hdr_status st = ensure(view->flags, P * sizeof(uint32_t))) return st;
-        if (hdr_status st = ensure(view->queue, P * sizeof(ShadowQuery))) return st;
+    if (need.rgb2) {
+        if (hdr_status st = ensure(view->rgb2, need.rgb2)) return st;
+        if (hdr_status st = ensure(view->flags, need.flags)) return st;
         a.rgb2 = static_cast<float4 *>(view->rgb2.p);
         a.flags = static_cast<uint32_t *>(view->flags.p);
+    }
+    if (plan.deferred) {
+        if (hdr_status st = ensure(view->queue, need.queue)) return st;
         a.queue = static_cast<ShadowQuery *>(view->queue.p);
         a.qcount = static_cast<unsigned int *>(view->qcount.p);
     }
-    if (wave) {
-        if (hdr_status st = ensure(view->wstate, P * kWaveFields * sizeof(float4))) return st;
-        // traced frames: hit records of odd and even vertices, escape masks and the escape queue
-        if (hdr_status st = ensure(view->whit, (waveTraced ? 2 : 1) * P * sizeof(GRec))) return st;
-        if (waveTraced) {
-            plan.escapes = a.trace_generated ? P * size_t(a.n_cand - a.n_nee) : 0;
-            if (hdr_status st = ensure(view->wesc, P * sizeof(unsigned int))) return st;
+    if (plan.wave) {
+        if (hdr_status st = ensure(view->wstate, need.wstate)) return st;
+        if (hdr_status st = ensure(view->whit, need.whit)) return st;
+        if (plan.wave_traced) {
+            if (hdr_status st = ensure(view->wesc, need.wesc)) return st;
             a.whit2 = static_cast<GRec *>(view->whit.p) + P;
             a.wesc = static_cast<unsigned int *>(view->wesc.p);
         }
         a.wstate = static_cast<float4 *>(view->wstate.p);
         a.whit = static_cast<GRec *>(view->whit.p);
         if (!shared_queues) {  // the view's own queues
-            if (hdr_status st = ensure(view->wray, P * 2 * sizeof(float4))) return st;
-            if (hdr_status st = ensure(view->queue, P * sizeof(ShadowQuery))) return st;
-            if (hdr_status st = ensure(view->wcq, P * sizeof(float4))) return st;
+            if (hdr_status st = ensure(view->wray, need.wray)) return st;
+            if (hdr_status st = ensure(view->queue, need.queue)) return st;
+            if (hdr_status st = ensure(view->wcq, need.wcq)) return st;
             if (hdr_status st = ensure(view->walive, 2 * P * sizeof(unsigned int))) return st;  // two parities
             if (hdr_status st = ensure(view->wcnt, 8 * sizeof(unsigned int))) return st;
-            if (waveTraced) {
-                if (hdr_status st = ensure(view->wesq, std::max<size_t>(1, plan.escapes) * 2 * sizeof(float4)))
-                    return st;
+            if (plan.wave_traced) {
+                if (hdr_status st = ensure(view->wesq, need.wesq)) return st;
                 a.wesq = static_cast<float4 *>(view->wesq.p);
             }
             a.wray = static_cast<float4 *>(view->wray.p);
@@ -818,24 +891,20 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
             a.walive2 = a.walive + P;
             a.wcnt = static_cast<unsigned int *>(view->wcnt.p);
         }
-        if (spatial) {  // per pixel: emission + reused direct, visible / occluded; visibility + slot
-            if (hdr_status st = ensure(view->rgb2, P * 2 * sizeof(float4))) return st;
-            if (hdr_status st = ensure(view->flags, P * sizeof(uint32_t))) return st;
-            a.rgb2 = static_cast<float4 *>(view->rgb2.p);
-            a.flags = static_cast<uint32_t *>(view->flags.p);
-        }
     }
     a.view = 0;
-    plan.deferred = deferred;
-    plan.wave = wave;
-    plan.wave_lean0 = wave && bound0 < kMtN - 1;
-    plan.wave_traced = waveTraced;
-    plan.depth = pipe->max_depth;
-    plan.lean_pt = !deferred && a.direct == HDR_DIRECT_MIS && !scene->any_translucent && bound < kMtN - 1;
-    plan.spatial = spatial;
-    plan.spatial_nd = spatial ? spatial_domains(pipe->spatial_count) : 0;
-    plan.lds = lds;
-    plan.tiles = unsigned(((W + 7) / 8) * ((H + 7) / 8));
+    return HDR_OK;
+}
+
+hdr_status hdr_view_plan_bytes(const hdr_scene *scene, int32_t width, int32_t height, const hdr_pipeline_desc *pipe,
+                               uint64_t *bytes) {
+    if (!bytes) return fail(HDR_ERR_INVALID_ARGUMENT, "bytes is null");
+    if (width <= 0 || height <= 0 || int64_t(width) * height >= (int64_t(1) << 24))
+        return fail(HDR_ERR_INVALID_ARGUMENT, "invalid view size (at most 2^24 - 1 pixels)");
+    FramePlan plan;
+    FrameNeeds need;
+    if (hdr_status st = plan_pipeline(scene, width, height, pipe, false, plan, need)) return st;
+    *bytes = uint64_t(view_base_bytes(size_t(width) * size_t(height)) + need.sum());
     return HDR_OK;
 }
 
@@ -1234,9 +1303,11 @@ struct hdr_batch {
         PinnedBuf<char> host;
         hipEvent_t done = nullptr;
         bool pending = false;
+        uint64_t step = 0;  // number of the step the slot last carried (1, 2, ...)
     };
     Slot slots[kSlots];
     int next = 0;
+    uint64_t issued = 0;  // steps queued so far
     LaunchShape shape;    // queue-driven launches (hdr_batch_set_launch_config / hdr_batch_autotune)
     bool graphs = true;   // steps replay captured graphs
     struct Graph {
@@ -1287,6 +1358,22 @@ hdr_status hdr_batch_synchronize(hdr_batch *b) {
     HIP_TRY(hipSetDevice(b->ctx->device));
     HIP_TRY(hipStreamSynchronize(b->gstream));
     HIP_TRY(hipStreamSynchronize(b->stream));
+    return HDR_OK;
+}
+
+hdr_status hdr_batch_poll(hdr_batch *b, uint64_t *issued, uint64_t *completed) {
+    if (!b || !issued || !completed) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    uint64_t done = b->issued;  // steps run in order on the batch's stream
+    for (const hdr_batch::Slot &sl : b->slots) {
+        if (!sl.pending) continue;
+        const hipError_t e = hipEventQuery(sl.done);
+        if (e == hipSuccess) continue;
+        if (e != hipErrorNotReady) HIP_TRY(e);
+        done = std::min(done, sl.step - 1);
+    }
+    *issued = b->issued;
+    *completed = done;
     return HDR_OK;
 }
 
@@ -1579,6 +1666,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
     HIP_TRY(hipEventRecord(b->end, b->stream));
     HIP_TRY(hipEventRecord(sl.done, b->stream));
     sl.pending = true;
+    sl.step = ++b->issued;
     b->timed = true;
     b->next = (b->next + 1) % hdr_batch::kSlots;
     // later work of each view (reads, single frames, resets) follows this step

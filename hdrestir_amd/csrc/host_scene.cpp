@@ -193,6 +193,14 @@ std::string build_host_scene(const hdr_scene_desc &d, HostScene &out) {
     if ((d.num_meshes && !d.meshes) || (d.num_materials && !d.materials) || (d.num_instances && !d.instances) ||
         (d.num_lights && !d.lights))
         return "null array with a non-zero count in hdr_scene_desc";
+    for (int i = 0; i < d.num_meshes; ++i)
+        if (d.meshes[i].struct_size != sizeof(hdr_mesh_desc)) return "hdr_mesh_desc.struct_size mismatch";
+    for (int i = 0; i < d.num_materials; ++i)
+        if (d.materials[i].struct_size != sizeof(hdr_material_desc)) return "hdr_material_desc.struct_size mismatch";
+    for (int i = 0; i < d.num_instances; ++i)
+        if (d.instances[i].struct_size != sizeof(hdr_instance_desc)) return "hdr_instance_desc.struct_size mismatch";
+    for (int i = 0; i < d.num_lights; ++i)
+        if (d.lights[i].struct_size != sizeof(hdr_light_desc)) return "hdr_light_desc.struct_size mismatch";
     if (d.environment >= d.num_lights || (d.environment >= 0 && d.lights[d.environment].type != HDR_LIGHT_DOME))
         return "environment must index a dome light";
 

@@ -118,6 +118,9 @@ def lib():
             "hdr_server_step": [P, C.POINTER(C.c_int32)],
             "hdr_server_peek_result": [P, C.POINTER(abi.RequestResult), C.POINTER(C.c_int32)],
             "hdr_server_take_result": [P, C.c_uint64, P],
+            "hdr_server_wait_result": [P, I, C.POINTER(abi.RequestResult), C.POINTER(C.c_int32)],
+            "hdr_view_plan_bytes": [P, I, I, C.POINTER(abi.PipelineDesc), C.POINTER(C.c_uint64)],
+            "hdr_batch_poll": [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
             "hdr_server_get_stats": [P, C.POINTER(abi.ServerStats)],
             "hdr_scene_intersect": [P, P, P, I, P, P, P],
             "hdr_rng_raw": [P, P, I, I, P],
@@ -206,6 +209,13 @@ class GpuScene:
         t = np.zeros(n, np.float32)
         check(lib().hdr_scene_intersect(self.h, _p(origins), _p(dirs), n, _p(inst), _p(tri), _p(t)))
         return inst, tri, t
+
+    def plan_bytes(self, width, height, pipe):
+        """Device bytes a view of width x height holds to render frames of `pipe` on this scene
+        (hdr_view_plan_bytes; validates the pipeline as a render would)."""
+        b = C.c_uint64()
+        check(lib().hdr_view_plan_bytes(self.h, int(width), int(height), C.byref(pipe), C.byref(b)))
+        return int(b.value)
 
     def close(self):
         if self.h:
@@ -348,13 +358,14 @@ class Server:
     """Render-request server (hdr_server_*): continuous batching (batched=True) or the plain
     per-request path, with admission by device-memory footprint."""
 
-    def __init__(self, ctx, batched=True, max_active=64, hbm_budget_bytes=64 << 30):
+    def __init__(self, ctx, batched=True, max_active=64, hbm_budget_bytes=64 << 30, own_thread=False):
         self.ctx = ctx
         d = abi.ServerDesc()
         d.struct_size = C.sizeof(abi.ServerDesc)
         d.max_active = max_active
         d.hbm_budget_bytes = hbm_budget_bytes
         d.batched = int(batched)
+        d.own_thread = int(own_thread)
         self.h = C.c_void_p()
         check(lib().hdr_server_create(ctx.h, C.byref(d), C.byref(self.h)))
 
@@ -374,21 +385,26 @@ class Server:
         check(lib().hdr_server_step(self.h, C.byref(n)))
         return int(n.value)
 
-    def results(self, images=False):
-        """Takes every finished request: list of (result dict, image or None)."""
+    def results(self, images=False, wait_ms=None):
+        """Takes every finished request: list of (result dict, image or None). wait_ms: with the
+        server's own thread, wait up to that long for the first result."""
         out = []
         while True:
             res = abi.RequestResult()
             avail = C.c_int32()
-            check(lib().hdr_server_peek_result(self.h, C.byref(res), C.byref(avail)))
+            if wait_ms is not None and not out:
+                check(lib().hdr_server_wait_result(self.h, int(wait_ms), C.byref(res), C.byref(avail)))
+            else:
+                check(lib().hdr_server_peek_result(self.h, C.byref(res), C.byref(avail)))
             if not avail.value:
                 return out
             img = None
-            if images:
+            if images and res.status == abi.HDR_OK:
                 img = np.zeros((res.height, res.width, 4), np.float32)
-            check(lib().hdr_server_take_result(self.h, C.c_uint64(res.id), _p(img) if images else None))
+            check(lib().hdr_server_take_result(self.h, C.c_uint64(res.id), _p(img) if img is not None else None))
             out.append(({"id": int(res.id), "width": res.width, "height": res.height, "spp": res.spp,
-                         "queued_ms": res.queued_ms, "latency_ms": res.latency_ms}, img))
+                         "status": int(res.status), "queued_ms": res.queued_ms, "latency_ms": res.latency_ms},
+                        img))
 
     def stats(self):
         st = abi.ServerStats()

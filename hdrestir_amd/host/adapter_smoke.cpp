@@ -31,6 +31,10 @@ struct BoxScene {
             const int t[6] = {f[0], f[1], f[2], f[0], f[2], f[3]};
             indices.insert(indices.end(), t, t + 6);
         }
+        mesh.struct_size = sizeof mesh;
+        mat.struct_size = sizeof mat;
+        inst.struct_size = sizeof inst;
+        light.struct_size = sizeof light;
         mesh.points = points.data();
         mesh.num_points = 8;
         mesh.indices = indices.data();

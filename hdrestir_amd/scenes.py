@@ -301,6 +301,7 @@ class Scene:
         md = (abi.MeshDesc * max(1, len(self.meshes)))()
         for k, m in enumerate(self.meshes):
             d = md[k]
+            d.struct_size = C.sizeof(abi.MeshDesc)
             d.points, d.num_points = fptr(m["points"])
             keep.append(m["indices"])
             d.indices, d.num_triangles = m["indices"].ctypes.data_as(abi.IP), m["indices"].shape[0]
@@ -309,7 +310,8 @@ class Scene:
             d.colors, d.num_colors = fptr(m["colors"])
         mat = (abi.MaterialDesc * max(1, len(self.materials)))()
         for k, m in enumerate(self.materials):
-            for name, _ in abi.MaterialDesc._fields_:
+            mat[k].struct_size = C.sizeof(abi.MaterialDesc)
+            for name, _ in abi.MaterialDesc._fields_[1:]:
                 val = m[name]
                 if isinstance(val, (tuple, list, np.ndarray)):
                     arr = getattr(mat[k], name)
@@ -319,12 +321,14 @@ class Scene:
                     setattr(mat[k], name, val)
         inst = (abi.InstanceDesc * max(1, len(self.instances)))()
         for k, m in enumerate(self.instances):
+            inst[k].struct_size = C.sizeof(abi.InstanceDesc)
             for j in range(16):
                 inst[k].transform[j] = float(m["transform"].flat[j])
             inst[k].mesh, inst[k].material, inst[k].light = m["mesh"], m["material"], m["light"]
         li = (abi.LightDesc * max(1, len(self.lights)))()
         for k, m in enumerate(self.lights):
             d = li[k]
+            d.struct_size = C.sizeof(abi.LightDesc)
             d.type = m["type"]
             for j in range(3):
                 d.color[j] = float(m["color"][j])

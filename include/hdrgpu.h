@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define HDRGPU_ABI_VERSION 1u
+#define HDRGPU_ABI_VERSION 2u
 
 typedef enum hdr_status {
     HDR_OK = 0,
@@ -56,6 +56,7 @@ typedef enum hdr_status {
  * (NULL / 0) and are either per-vertex (count == num_points) or face-varying
  * (count == 3 * num_triangles), as in BVH::Build. */
 typedef struct hdr_mesh_desc {
+    uint32_t       struct_size; /* sizeof(hdr_mesh_desc) */
     const float   *points;     /* xyz, num_points */
     int32_t        num_points;
     const int32_t *indices;    /* 3 per triangle */
@@ -71,6 +72,7 @@ typedef struct hdr_mesh_desc {
 /* UsdPreviewSurface parameters (reference: PreviewSurfaceParams,
  * source/materials/preview_surface_params.h:12-52), textures excluded. */
 typedef struct hdr_material_desc {
+    uint32_t struct_size;  /* sizeof(hdr_material_desc) */
     float diffuse_color[3];
     float metallic;
     float roughness;
@@ -98,6 +100,7 @@ typedef struct hdr_material_desc {
 /* One mesh instance (reference: Scene::MeshInstance, source/scene/scene.h:112-122).
  * transform is a GfMatrix4f in row-vector convention (p' = p * M), row-major. */
 typedef struct hdr_instance_desc {
+    uint32_t struct_size;  /* sizeof(hdr_instance_desc) */
     float   transform[16];
     int32_t mesh;       /* index into hdr_scene_desc::meshes */
     int32_t material;   /* index into materials, or -1 for DefaultMaterial */
@@ -117,6 +120,7 @@ typedef enum hdr_light_type {
  * row-vector convention, row-major. For HDR_LIGHT_DOME, dome_pixels holds
  * dome_width*dome_height linear RGB float texels, row 0 first. */
 typedef struct hdr_light_desc {
+    uint32_t     struct_size;  /* sizeof(hdr_light_desc) */
     int32_t      type;
     float        color[3];
     float        intensity;
@@ -275,6 +279,15 @@ hdr_status hdr_view_destroy(hdr_view *view);
 hdr_status hdr_view_reset(hdr_view *view);
 hdr_status hdr_view_device_bytes(const hdr_view *view, uint64_t *out);
 
+/* Device bytes a view of width x height holds to render frames of `pipeline` on `scene`: the buffers
+ * hdr_view_create allocates plus those hdr_render_frame allocates for this pipeline (RNG spill slots,
+ * the deferred or wavefront path's buffers, spatial reuse's, the firefly filter's frame). Validates
+ * the pipeline on the scene as hdr_render_frame does (HDR_ERR_INVALID_ARGUMENT, HDR_ERR_UNSUPPORTED)
+ * and makes no device call. No counterpart in the reference (its buffers live in host memory); used
+ * for admission by memory footprint. */
+hdr_status hdr_view_plan_bytes(const hdr_scene *scene, int32_t width, int32_t height,
+                               const hdr_pipeline_desc *pipeline, uint64_t *bytes);
+
 /* One frame of RaycastPass -> {RIS,}PathTracePass -> AccumulationPass on the device.
  * Asynchronous with respect to the host unless an output below is requested.
  * color_out (optional, host, width*height*4 floats): accumulated "Color" (RGBA32F).
@@ -333,6 +346,9 @@ hdr_status hdr_render_batch(hdr_batch *batch, int32_t n, hdr_view *const *views,
 hdr_status hdr_batch_synchronize(hdr_batch *batch);
 /* Device time (ms) of the last step, from its first launch to its last. */
 hdr_status hdr_batch_last_timing(hdr_batch *batch, float *step_ms);
+/* Steps queued so far (numbered 1, 2, ...) and how many of them have finished on the device,
+ * without waiting. */
+hdr_status hdr_batch_poll(hdr_batch *batch, uint64_t *issued, uint64_t *completed);
 
 /* ---- launch shape and autotuning ----
  * No counterpart in the reference (its passes run on CPU threads). The queue-driven launches --
@@ -372,21 +388,28 @@ hdr_status hdr_batch_autotune(hdr_batch *batch, int32_t n, hdr_view *const *view
  * each admitted request its own view (reservoirs, accumulation) on the device, admits queued
  * requests first-in first-out while their views fit hbm_budget_bytes and fewer than max_active are
  * running, and renders either
- *   batched = 1: continuous batching -- every hdr_server_step renders one frame of every running
- *                request in one merged launch set (hdr_render_batch); requests join when memory and
- *                a slot free up and leave when their last frame is done; or
+ *   batched = 1: continuous batching -- every step renders one frame of every running request in
+ *                one merged launch set (hdr_render_batch); requests join when memory and a slot free
+ *                up and leave when their last frame is done; steps run ahead on the device and a
+ *                request is retired when the step that carries its last frame has finished (the
+ *                host waits only when nothing is left to queue); or
  *   batched = 0: the plain per-request path -- one request at a time, all its frames through
  *                hdr_render_frame.
+ * Admission takes queued requests in arrival order; one whose view does not fit the free memory is
+ * passed over by later ones that fit, at most 8 times, after which admission waits for it.
  * A request renders frames frame.frame_index .. + spp - 1 of its own view, so its image is the
  * same alone and in any mix. Finished views return to a pool (reused for the same size, freed
- * oldest-first when memory is needed). Used from one host thread. */
+ * oldest-first when memory is needed). The caller steps the server (hdr_server_step), or with
+ * own_thread the server runs on a host thread of its own and the caller only submits and collects
+ * (hdr_server_wait_result). The calls are serialised by the server's lock. */
 typedef struct hdr_server hdr_server;
 typedef struct hdr_server_desc {
     uint32_t struct_size;
     int32_t  max_active;        /* requests on the device at once, 1..255 (1 when batched = 0) */
     uint64_t hbm_budget_bytes;  /* device memory the server's views may hold */
     int32_t  batched;
-    int32_t  _reserved;
+    int32_t  own_thread;        /* 1: the server admits and renders on a host thread of its own (the caller
+                                   submits and collects; hdr_server_step is then HDR_ERR_UNSUPPORTED) */
 } hdr_server_desc;
 typedef struct hdr_request_desc {
     uint32_t struct_size;
@@ -398,7 +421,8 @@ typedef struct hdr_request_desc {
 } hdr_request_desc;
 typedef struct hdr_request_result {
     uint64_t id;
-    int32_t  width, height, spp, _reserved;
+    int32_t  width, height, spp;
+    int32_t  status;            /* hdr_status of the request: HDR_OK, or the error its frames failed with */
     double   queued_ms;         /* submission -> admission */
     double   latency_ms;        /* submission -> last frame finished */
 } hdr_request_result;
@@ -409,13 +433,20 @@ typedef struct hdr_server_stats {
 } hdr_server_stats;
 hdr_status hdr_server_create(hdr_context *ctx, const hdr_server_desc *desc, hdr_server **out);
 hdr_status hdr_server_destroy(hdr_server *server);
-/* Queues a request; HDR_ERR_OUT_OF_MEMORY when its view alone exceeds the budget. */
+/* Validates the request's pipeline on its scene (hdr_view_plan_bytes: HDR_ERR_INVALID_ARGUMENT,
+ * HDR_ERR_UNSUPPORTED, as hdr_render_frame would) and queues it; HDR_ERR_OUT_OF_MEMORY when its view
+ * alone exceeds the budget. */
 hdr_status hdr_server_submit(hdr_server *server, const hdr_request_desc *req, uint64_t *id);
-/* Admits what fits, renders (one merged frame for every running request, or one whole request),
- * waits for the device and retires finished requests (*n_finished of them). */
+/* Admits what fits, renders (one merged frame for every running request, or one whole request) and
+ * retires the requests the device has finished (*n_finished of them). A request whose frames fail
+ * finishes with that status in its result; the others go on. */
 hdr_status hdr_server_step(hdr_server *server, int32_t *n_finished);
 /* The oldest finished request not yet taken (*available = 0 if none). */
 hdr_status hdr_server_peek_result(hdr_server *server, hdr_request_result *out, int32_t *available);
+/* As hdr_server_peek_result, waiting up to timeout_ms for a result when the server runs its own
+ * thread; returns the worker's error if it stopped on one. */
+hdr_status hdr_server_wait_result(hdr_server *server, int32_t timeout_ms, hdr_request_result *out,
+                                  int32_t *available);
 /* Copies a finished request's image (RGBA32F, width*height*4; rgba may be null) and releases it. */
 hdr_status hdr_server_take_result(hdr_server *server, uint64_t id, float *rgba);
 hdr_status hdr_server_get_stats(hdr_server *server, hdr_server_stats *out);

@@ -83,3 +83,84 @@ def test_server_admission_respects_memory_budget(gpu_ctx):
     assert waited  # requests had to wait for memory, and all of them finished
     srv.close()
     gbox.close()
+
+
+def test_server_rejects_an_unsupported_request_and_serves_the_rest(gpu_ctx):
+    """A request whose pipeline its scene cannot run (spatial reuse on a scene with translucent
+    materials and a mesh light) is refused at submission, as hdr_render_frame would refuse it; the
+    valid request next to it is served as usual."""
+    from hdrestir_amd import gpu
+    box = scenes.box_spheres(64, 64, frames=1)
+    feat = scenes.features(32, 32, frames=1)
+    gbox, gfeat = gpu.GpuScene(gpu_ctx, box), gpu.GpuScene(gpu_ctx, feat)
+    srv = gpu.Server(gpu_ctx, batched=True, max_active=4)
+    with pytest.raises(gpu.HdrError, match="UNSUPPORTED"):
+        srv.submit(gfeat, 32, 32, 2, _frame(feat, 32, 32, 0), RESTIR)
+    rid = srv.submit(gbox, 48, 32, 3, _frame(box, 48, 32, 0), RESTIR)
+    got = []
+    for _ in range(50):
+        srv.step()
+        got += srv.results(images=True)
+        if got:
+            break
+    assert len(got) == 1 and got[0][0]["id"] == rid and got[0][0]["status"] == 0
+    ref = _alone(gpu, gpu_ctx, gbox, box, 48, 32, 3, RESTIR)
+    assert np.array_equal(got[0][1], ref)
+    srv.close()
+    gbox.close()
+    gfeat.close()
+
+
+def test_plan_bytes_bound_the_views_device_memory(gpu_ctx):
+    """hdr_view_plan_bytes (what admission charges) covers what a view holds after rendering frames
+    of the pipeline, within the argument ring and status word it does not report, on an opaque scene
+    and on a translucent one (RNG spill slots: 2 x 2.5 KB per pixel)."""
+    from hdrestir_amd import gpu
+    cases = [(scenes.box_spheres(48, 40, frames=1), (PT, RESTIR, abi.pipeline("ris", candidate_count=32, max_depth=6,
+                                                                             skip_visibility=True))),
+             (scenes.features(48, 40, frames=1), (PT, abi.pipeline("ris", candidate_count=8, max_depth=5)))]
+    plans = {}
+    for sc, pipes in cases:
+        gs = gpu.GpuScene(gpu_ctx, sc)
+        for k, pipe in enumerate(pipes):
+            plan = gs.plan_bytes(48, 40, pipe)
+            r = gpu.Renderer(gpu_ctx, gs, 48, 40)
+            for f in range(2):
+                r.render(_frame(sc, 48, 40, f), pipe, read_color=False)
+            r.synchronize()
+            held = r.device_bytes()
+            print(sc.name, k, "plan", plan, "held", held)
+            assert held <= plan <= held + (64 << 10), (plan, held)
+            plans[(sc.name, k)] = plan
+            r.close()
+        gs.close()
+    assert plans[("features", 0)] > plans[("box_spheres", 0)] + 48 * 40 * 4096  # spill slots counted
+
+
+def test_server_on_its_own_thread(gpu_ctx):
+    """With own_thread the server admits and renders on its own host thread: the caller only
+    submits and waits for results, and every image equals the request rendered alone."""
+    from hdrestir_amd import gpu
+    box = scenes.box_spheres(64, 64, frames=1)
+    feat = scenes.features(64, 64, frames=1, dome=False, emissive=False, opaque=True)
+    gbox, gfeat = gpu.GpuScene(gpu_ctx, box), gpu.GpuScene(gpu_ctx, feat)
+    gsc = {id(box): gbox, id(feat): gfeat}
+    srv = gpu.Server(gpu_ctx, batched=True, max_active=5, own_thread=True)
+    with pytest.raises(gpu.HdrError, match="UNSUPPORTED"):
+        srv.step()
+    reqs = _requests(box, feat)
+    ids = {srv.submit(gsc[id(sc)], w, h, spp, _frame(sc, w, h, 0), pipe): k
+           for k, (sc, w, h, spp, pipe) in enumerate(reqs)}
+    got = {}
+    for _ in range(400):
+        for res, img in srv.results(images=True, wait_ms=50):
+            got[ids[res["id"]]] = img
+        if len(got) == len(reqs):
+            break
+    assert len(got) == len(reqs)
+    for k, (sc, w, h, spp, pipe) in enumerate(reqs):
+        ref = _alone(gpu, gpu_ctx, gsc[id(sc)], sc, w, h, spp, pipe)
+        assert np.array_equal(got[k], ref), k
+    srv.close()
+    gbox.close()
+    gfeat.close()

@@ -54,6 +54,7 @@ def _p(a):
 def _light_desc(type_, color=(1.0, 1.0, 1.0), intensity=1.0, exposure=0.0, transform=None, width=1.0, height=1.0,
                 cone_angle=180.0, cone_softness=0.0, dome=None):
     d = abi.LightDesc()
+    d.struct_size = C.sizeof(abi.LightDesc)
     d.type = type_
     for i in range(3):
         d.color[i] = color[i]
