@@ -192,6 +192,7 @@ struct hdr_context {
 
 struct hdr_scene {
     hdr_context *ctx = nullptr;
+    uint64_t id = 0;  // unique per upload (G-buffer reuse keys on it)
     DevBuf mem;
     DeviceScene ds{};
     HostScene hs;
@@ -318,7 +319,21 @@ struct hdr_view {
     hipStream_t gstream = nullptr;
     hipEvent_t shade_done[2] = {nullptr, nullptr}, gbuf_done = nullptr, serial = nullptr;
     bool shade_recorded[2] = {false, false};
-    int gpar = 0;                    // G-buffer half of the next frame
+    // G-buffer reuse: the pixel-centre rays (RaycastPass) depend only on the scene, the camera and
+    // the view size, so while they stay the same (and the camera has no depth of field, whose lens
+    // draws change every frame) the G-buffer of an earlier frame is this frame's; gcur is the half
+    // holding the latest G-buffer and gkey what it was computed for.
+    struct GbufKey {
+        uint64_t scene_id;
+        double view[16], proj[16];
+        int32_t has_cam;
+        hdr_camera_params cam;
+        int32_t aovs;
+    };
+    GbufKey gkey;
+    bool gvalid = false;
+    bool gnext_valid = false;        // the G-buffer being written is reusable (no depth of field)
+    int gcur = 1;                    // the first frame writes half 0
     bool single_since_batch = true;  // a single frame used the G-buffer after the last batch step
     // Per-kernel timing: one slot of events per frame in a ring, read back lazily (when the slot is
     // reused or the sums are requested), so timing never stalls the streams.
@@ -407,6 +422,8 @@ hdr_status hdr_scene_upload(hdr_context *ctx, const hdr_scene_desc *desc, hdr_sc
     std::unique_ptr<hdr_scene> sc(new (std::nothrow) hdr_scene());
     if (!sc) return fail(HDR_ERR_OUT_OF_MEMORY, "host allocation failed");
     sc->ctx = ctx;
+    static std::atomic<uint64_t> nextSceneId{1};
+    sc->id = nextSceneId.fetch_add(1);
     const std::string err = build_host_scene(*desc, sc->hs);
     if (!err.empty()) return fail(HDR_ERR_INVALID_ARGUMENT, err);
     HostScene &h = sc->hs;
@@ -641,6 +658,8 @@ struct FramePlan {
     int depth = 0;          // path vertices (max_depth)
     size_t lds = 0;         // dynamic LDS of the shading / integration kernel
     unsigned tiles = 0;     // 8x8 tiles of the view
+    bool gbuf_reuse = false;  // the view's latest G-buffer is this frame's (no G-buffer launch)
+    int gbuf_half = 0;        // the G-buffer half this frame reads (and, without reuse, writes)
 };
 
 // Device bytes of the buffers a frame's plan needs beyond those every view holds (0: not needed).
@@ -844,7 +863,25 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
         a.aov_albedo = static_cast<float *>(view->albedo.p);
         a.aov_normal = static_cast<float *>(view->normal.p);
     }
-    a.gbuf = static_cast<GRec *>(view->gbuf.p) + size_t(view->gpar & 1) * P;  // this frame's half
+    {  // this frame's G-buffer: the latest one if it was computed for the same rays, else a new one
+        hdr_view::GbufKey key;
+        std::memset(&key, 0, sizeof key);
+        key.scene_id = scene->id;
+        std::memcpy(key.view, frame->view, sizeof key.view);
+        std::memcpy(key.proj, frame->proj, sizeof key.proj);
+        key.has_cam = frame->has_camera_params != 0;
+        if (key.has_cam) key.cam = frame->camera;
+        key.aovs = view->depth.p != nullptr;
+        const bool dof = key.has_cam && frame->camera.enable_dof;
+        plan.gbuf_reuse = view->gvalid && !dof && !view->counting && std::memcmp(&key, &view->gkey, sizeof key) == 0;
+        plan.gbuf_half = plan.gbuf_reuse ? view->gcur : (view->gcur ^ 1);
+        if (!plan.gbuf_reuse) {  // recorded here, committed once the frame is queued
+            view->gkey = key;
+            view->gvalid = false;
+            view->gnext_valid = !dof;
+        }
+    }
+    a.gbuf = static_cast<GRec *>(view->gbuf.p) + size_t(plan.gbuf_half) * P;
     if (plan.spatial) {
         if (hdr_status st = ensure(view->res_t, need.res_t)) return st;
         if (hdr_status st = ensure(view->saux, need.saux)) return st;
@@ -917,7 +954,7 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     const size_t P = size_t(W) * H;
     const bool deferred = plan.deferred, spatial = plan.spatial;
     const size_t lds = plan.lds;
-    const int gp = view->gpar & 1;  // this frame's G-buffer half (prepare_frame chose it)
+    const int gp = plan.gbuf_half;  // this frame's G-buffer half (prepare_frame chose it)
     const dim3 grid(plan.tiles);  // one block per 8x8 tile
     if (view->counting) {
         HIP_TRY(hipMemsetAsync(view->counters.p, 0, view->counters.n, view->stream));
@@ -931,22 +968,25 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     const FrameArgs *da = static_cast<const FrameArgs *>(view->dargs.p) + view->ring_next;
     const int *dt = static_cast<const int *>(view->dtile.p);
     // G-buffer on the second stream: after the previous frame's shading has read the G-buffer, or,
-    // when counting, after everything queued so far (counters are per frame).
+    // when counting, after everything queued so far (counters are per frame). A reused G-buffer
+    // needs no launch (only the argument copy runs there).
     if (view->counting) {
         HIP_TRY(hipEventRecord(view->serial, view->stream));
         HIP_TRY(hipStreamWaitEvent(view->gstream, view->serial, 0));
-    } else if (view->shade_recorded[gp]) {
+    } else if (!plan.gbuf_reuse && view->shade_recorded[gp]) {
         HIP_TRY(hipStreamWaitEvent(view->gstream, view->shade_done[gp], 0));
     }
     HIP_TRY(hipMemcpyAsync(const_cast<FrameArgs *>(da), &view->hargs.p[view->ring_next], sizeof(FrameArgs),
                            hipMemcpyHostToDevice, view->gstream));
-    HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_GBUFFER], view->gstream));
-    if (view->counting) launch_gbuffer<true>(view->shape, plan.tiles, view->gstream, da, dt, 1);
-    else launch_gbuffer<false>(view->shape, plan.tiles, view->gstream, da, dt, 1);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_GBUFFER], view->gstream));
+    if (!plan.gbuf_reuse) {
+        HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_GBUFFER], view->gstream));
+        if (view->counting) launch_gbuffer<true>(view->shape, plan.tiles, view->gstream, da, dt, 1);
+        else launch_gbuffer<false>(view->shape, plan.tiles, view->gstream, da, dt, 1);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_GBUFFER], view->gstream));
+        ts.ran |= 1u << HDR_STAGE_GBUFFER;
+    }
     HIP_TRY(hipEventRecord(view->gbuf_done, view->gstream));
-    ts.ran |= 1u << HDR_STAGE_GBUFFER;
     HIP_TRY(hipStreamWaitEvent(view->stream, view->gbuf_done, 0));
     HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_INTEGRATE], view->stream));
     if (deferred) {
@@ -1098,7 +1138,10 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     view->ring_last = view->ring_next;
     view->ring_next = (view->ring_next + 1) % hdr_view::kRing;
     view->frames++;
-    view->gpar ^= 1;
+    if (!plan.gbuf_reuse) {
+        view->gcur = gp;
+        view->gvalid = view->gnext_valid;
+    }
     view->single_since_batch = true;
     if (color_out) {
         HIP_TRY(hipMemcpyAsync(color_out, view->color.p, P * sizeof(float4), hipMemcpyDeviceToHost, view->stream));
@@ -1130,9 +1173,10 @@ hdr_status hdr_view_last_timing(hdr_view *view, float *gbuffer_ms, float *integr
         hdr_view::StageSlot &s = view->ring[view->ring_last];
         HIP_TRY(hipEventSynchronize(s.fin));
         const int last = (s.ran & (1u << HDR_STAGE_ACCUMULATE)) ? HDR_STAGE_ACCUMULATE : HDR_STAGE_INTEGRATE;
-        HIP_TRY(hipEventElapsedTime(&g, s.beg[HDR_STAGE_GBUFFER], s.end[HDR_STAGE_GBUFFER]));
+        const bool gb = (s.ran & (1u << HDR_STAGE_GBUFFER)) != 0;  // not when the G-buffer was reused
+        if (gb) HIP_TRY(hipEventElapsedTime(&g, s.beg[HDR_STAGE_GBUFFER], s.end[HDR_STAGE_GBUFFER]));
         HIP_TRY(hipEventElapsedTime(&i, s.beg[HDR_STAGE_INTEGRATE], s.end[last]));
-        HIP_TRY(hipEventElapsedTime(&f, s.beg[HDR_STAGE_GBUFFER], s.fin));
+        HIP_TRY(hipEventElapsedTime(&f, s.beg[gb ? HDR_STAGE_GBUFFER : HDR_STAGE_INTEGRATE], s.fin));
     }
     if (gbuffer_ms) *gbuffer_ms = g;
     if (integrate_ms) *integrate_ms = i;
@@ -1177,6 +1221,7 @@ hdr_status hdr_view_read_aovs(hdr_view *view, float *depth, float *albedo, float
     const size_t P = size_t(view->W) * view->H;
     if (!view->depth.p) {
         // first request: allocate; AOVs are produced from the next frame on
+        view->gvalid = false;  // the AOVs are written by the G-buffer kernel from the next frame on
         if (hdr_status s = ensure(view->depth, P * sizeof(float))) return s;
         if (hdr_status s = ensure(view->albedo, P * 3 * sizeof(float))) return s;
         if (hdr_status s = ensure(view->normal, P * 3 * sizeof(float))) return s;
@@ -1499,12 +1544,14 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
             args[i].queue = static_cast<ShadowQuery *>(b->queue.p);
         }
     }
-    // launch groups: every view (G-buffer, shadow), deferred views (shading, accumulation), their
-    // spatial-reuse subset, and megakernel views; each gets its own argument array and tile table
-    enum { kAll, kDef, kSpa, kMega, kLean, kWave, kWaveT, kGroups };
+    // launch groups: every view (shadow), the views that need a new G-buffer, deferred views
+    // (shading, accumulation), their spatial-reuse subset, and megakernel views; each gets its own
+    // argument array and tile table
+    enum { kAll, kGb, kDef, kSpa, kMega, kLean, kWave, kWaveT, kGroups };
     std::vector<int> members[kGroups];
     for (int i = 0; i < n; ++i) {
         members[kAll].push_back(i);
+        if (!plans[i].gbuf_reuse) members[kGb].push_back(i);
         if (plans[i].deferred) members[kDef].push_back(i);
         if (plans[i].deferred && plans[i].spatial) members[kSpa].push_back(i);
         if (!plans[i].deferred && !plans[i].lean_pt && !plans[i].wave) members[kMega].push_back(i);
@@ -1574,7 +1621,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
     auto enqueue_g = [&]() -> hipError_t {
         const hipError_t e = hipMemcpyAsync(sl.dev.p, sl.host.p, off, hipMemcpyHostToDevice, b->gstream);
         if (e != hipSuccess) return e;
-        launch_gbuffer<false>(ls, tiles[kAll], b->gstream, A(kAll), T(kAll), N(kAll));
+        if (N(kGb) > 0) launch_gbuffer<false>(ls, tiles[kGb], b->gstream, A(kGb), T(kGb), N(kGb));
         return hipGetLastError();
     };
     auto enqueue = [&]() -> hipError_t {
@@ -1674,7 +1721,10 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         HIP_TRY(hipStreamWaitEvent(views[i]->stream, sl.done, 0));
         HIP_TRY(hipStreamWaitEvent(views[i]->gstream, sl.done, 0));
         views[i]->frames++;
-        views[i]->gpar ^= 1;
+        if (!plans[i].gbuf_reuse) {
+            views[i]->gcur = plans[i].gbuf_half;
+            views[i]->gvalid = views[i]->gnext_valid;
+        }
         views[i]->single_since_batch = false;
     }
     return HDR_OK;

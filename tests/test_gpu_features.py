@@ -167,6 +167,54 @@ def test_overlapped_frames_match_serial_frames(gpu_ctx, oracle_mod):
     gs.close()
 
 
+@pytest.mark.parametrize("dof", [False, True])
+def test_gbuffer_reuse_keeps_results(gpu_ctx, oracle_mod, dof):
+    """While the scene, camera and view size stay the same and the camera has no depth of field, a
+    frame reuses the G-buffer of the previous one (its pixel-centre rays are the same): one G-buffer
+    launch for the static frames, a new one after a camera move and on every frame with depth of
+    field; images equal the oracle's frame by frame, and a batch of views reuses likewise."""
+    from hdrestir_amd import gpu
+    sc = scenes.features(48, 40, frames=6, dof=dof, dome=False, emissive=False, opaque=True)
+    pipe = DEFERRED_PIPES["restir16_d1_skipvis"]
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    o = oracle_mod.Oracle(sc, threads=8)
+    moved = [scenes.frame_with_camera(sc, f) for f in range(6)]
+    for f in range(3, 6):  # a camera move at frame 3
+        for i in range(16):
+            moved[f].view[i] = moved[f].view[i] + (0.25 if i == 12 else 0.0)
+    r.stage_times()
+    for f in range(6):
+        if f == 3:  # the renderer clears its history on a camera move
+            r.reset()
+            o.reset()
+        img = r.render(moved[f], pipe, debug=True).copy()
+        ref = o.render(moved[f], pipe).copy()
+        parity.check(f"gbuffer reuse dof={dof} frame {f}", _match(r, o), r.debug["sample_rgb"], o.debug["sample_rgb"],
+                     img, ref)
+        assert np.array_equal(r.debug["prim_instance"], o.debug["prim_instance"])
+    launches = r.stage_times()["gbuffer"][1]
+    assert launches == (6 if dof else 2), launches
+    r.close()
+    # batches: two views, one static and one moving every step
+    va, vb = gpu.Renderer(gpu_ctx, gs, 48, 40), gpu.Renderer(gpu_ctx, gs, 48, 40)
+    ra, rb = gpu.Renderer(gpu_ctx, gs, 48, 40), gpu.Renderer(gpu_ctx, gs, 48, 40)
+    batch = gpu.Batch(gpu_ctx)
+    for f in range(4):
+        fa = scenes.frame_with_camera(sc, f)
+        fb = scenes.frame_with_camera(sc, f)
+        fb.view[12] += 0.1 * f
+        batch.render([va, vb], [fa, fb], [pipe, pipe])
+        ra.render(fa, pipe, read_color=False)
+        rb.render(fb, pipe, read_color=False)
+    batch.synchronize()
+    assert np.array_equal(va.read_color(), ra.read_color()) and np.array_equal(vb.read_color(), rb.read_color())
+    batch.close()
+    for v in (va, vb, ra, rb):
+        v.close()
+    gs.close()
+
+
 SPATIAL_PIPES = {
     "restir16_spatial3_r30": abi.pipeline("ris", candidate_count=16, use_reservoir=True, skip_visibility=True,
                                           max_depth=1, spatial_count=3, spatial_radius=30.0),
