@@ -171,20 +171,13 @@ def config0_measurement(gpu, abi, ctx, cpu=True, cpu_frames=5):
     return out
 
 
-def look_at_origin_from_sphere(rng, radius):
-    from hdrestir_amd import scenes
-    v = rng.normal(size=3)
-    v[1] = abs(v[1]) + 0.3  # above the ground
-    v = v / np.linalg.norm(v) * radius
-    return scenes.look_at_view(tuple(v), (0.0, 0.0, 0.0))
-
-
 def serving_requests(rng, n, many_view, box_view, only_restir_1080p=False, small_only=False):
     """The serving stream's request mix: resolution uniform over {256^2, 512^2, 1280x720,
     1920x1080}, spp = round(2^U[0,6]), PathTracer (MIS, depth 4) or ReSTIR (M = 32, 1 temporal +
     3 spatial) with equal probability, scene uniform over the resident ones: the box (config 0),
     the many-lights scene seen from its camera (config 1) and from a random point of a 40 m
     sphere (config 3's cameras)."""
+    from hdrestir_amd import scenes
     sizes = [(256, 256), (512, 512)] if small_only else [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
     reqs = []
     for _ in range(n):
@@ -200,7 +193,7 @@ def serving_requests(rng, n, many_view, box_view, only_restir_1080p=False, small
         elif scene == 1:
             view = many_view
         else:
-            view = look_at_origin_from_sphere(rng, 40.0)
+            view = scenes.look_at_origin_from_sphere(rng, 40.0)
         reqs.append(dict(w=w, h=h, spp=spp, integ=integ, scene=scene, view=view))
     return reqs
 
@@ -379,7 +372,7 @@ def multiview_measurement(gpu, abi, ctx, gs, steps, seed=3):
     views, cams = [], []
     for _ in range(16):
         w, h = int(rng.integers(320, 1921)), int(rng.integers(240, 1081))
-        cams.append((look_at_origin_from_sphere(rng, 40.0), scenes.perspective(50.0, w / h, 0.05, 1000.0)))
+        cams.append((scenes.look_at_origin_from_sphere(rng, 40.0), scenes.perspective(50.0, w / h, 0.05, 1000.0)))
         views.append(gpu.Renderer(ctx, gs, w, h))
     pixels = sum(v.W * v.H for v in views)
 

@@ -220,6 +220,15 @@ def look_at_view(eye, target, up=(0.0, 1.0, 0.0)):
     return v
 
 
+def look_at_origin_from_sphere(rng, radius):
+    """A camera at a random point of a sphere of `radius` around the origin (above the ground plane),
+    looking at the origin: the multi-view workload's cameras."""
+    v = rng.normal(size=3)
+    v[1] = abs(v[1]) + 0.3  # above the ground
+    v = v / np.linalg.norm(v) * radius
+    return look_at_view(tuple(v), (0.0, 0.0, 0.0))
+
+
 def perspective(fovy_deg, aspect, near=0.1, far=1000.0):
     """GfFrustum-style perspective projection, row-vector convention (clip = p * P)."""
     ys = 1.0 / math.tan(math.radians(fovy_deg) / 2)

@@ -263,3 +263,76 @@ def test_closest_hit_ties_keep_the_references_winner(gpu_ctx, oracle_mod):
     assert np.array_equal(gt[hit], ot[hit]), f"triangle mismatch on {(gt[hit] != ot[hit]).sum()} rays"
     assert np.array_equal(gtt[hit], ott[hit])
     gs.close()
+
+
+def _crop_sel(sc, x0, y0, x1, y1):
+    sel = np.zeros((sc.height, sc.width), bool)
+    sel[y0:y1, x0:x1] = True
+    return sel.reshape(-1)
+
+
+@pytest.mark.parametrize("camera", ["static", "dolly"])
+def test_render_parity_many_lights_spatial_crop(gpu_ctx, oracle_mod, camera):
+    """The benchmarked 1080p many-lights pipeline -- ReSTIR M=32 with one shadow ray, the reference's
+    temporal reuse and 3 spatial neighbours within 30 px -- on the static camera (frames 0, 1) and on
+    the linear dolly (each frame from a cleared view at frame index 0): GPU full frame, oracle on a
+    rectangle 60 px wider on every side. A pixel's result depends on neighbours up to 30 px away per
+    frame of history (frame f: 30 (f + 1) px), so frame f is compared on the rectangle shrunk by that
+    much (the oracle's neighbours outside its rectangle are not rendered)."""
+    gpu = _gpu()
+    sc = scenes.many_lights(frames=4, dolly=(camera == "dolly"))
+    pipe = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=1,
+                        spatial_count=3, spatial_radius=30.0)
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    o = oracle_mod.Oracle(sc, threads=8)
+    x0, y0, x1, y1 = 900 - 60, 500 - 60, 996 + 60, 564 + 60
+    for f in range(2):
+        if camera == "dolly":  # a camera move clears the view and restarts the frame count
+            r.reset()
+            o.reset()
+            fr = sc.frame(0, camera_index=f)
+        else:
+            fr = sc.frame(f)
+        r.render(fr, pipe, debug=True)
+        o.render(fr, pipe, rect=(x0, y0, x1, y1))
+        m = 30 * (f + 1) if camera == "static" else 30
+        sel = _crop_sel(sc, x0 + m, y0 + m, x1 - m, y1 - m)
+        assert np.array_equal(r.debug["prim_instance"][sel], o.debug["prim_instance"][sel])
+        parity.check(f"many-lights spatial {camera} frame {f}", _stream_match(r, o)[sel], r.debug["sample_rgb"][sel],
+                     o.debug["sample_rgb"][sel])
+    if camera == "static":
+        m = 60
+        sel = _crop_sel(sc, x0 + m, y0 + m, x1 - m, y1 - m)
+        gr, orr = r.reservoirs()[sel], o.reservoirs()[sel]
+        assert np.array_equal(gr["flags_light"], orr["flags_light"]) and np.array_equal(gr["count"], orr["count"])
+    r.close()
+    gs.close()
+
+
+def test_render_parity_large_scene_spatial_crop(gpu_ctx, oracle_mod):
+    """The large-scene workload as benchmarked: ReSTIR M=32 at every vertex of depth-8 paths with the
+    reference's temporal reuse and 5 spatial neighbours within 30 px (traced candidates, the
+    environment's CDFs built on the device), frames 0 and 1: GPU full frame, oracle on a rectangle
+    60 px wider on every side, frame f compared on the rectangle shrunk by 30 (f + 1) px."""
+    gpu = _gpu()
+    sc = scenes.large_scene(frames=2)
+    pipe = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=8,
+                        spatial_count=5, spatial_radius=30.0)
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    o = oracle_mod.Oracle(sc, threads=8)
+    x0, y0, x1, y1 = 1584 - 60, 288 - 60, 1632 + 60, 320 + 60
+    for f in range(2):
+        r.render(sc.frame(f), pipe, debug=True)
+        o.render(sc.frame(f), pipe, rect=(x0, y0, x1, y1))
+        m = 30 * (f + 1)
+        sel = _crop_sel(sc, x0 + m, y0 + m, x1 - m, y1 - m)
+        print("large spatial crop", f, "draws max", o.debug["draws"][sel].max(), "past 624",
+              int((o.debug["draws"][sel] > 624).sum()))
+        assert np.array_equal(r.debug["prim_instance"][sel], o.debug["prim_instance"][sel])
+        # the dome's nearest-texel lookups (see test_render_parity_large_scene_crop): a few outliers
+        parity.check(f"large spatial crop frame {f}", _stream_match(r, o)[sel], r.debug["sample_rgb"][sel],
+                     o.debug["sample_rgb"][sel], outlier_pixels=4)
+    r.close()
+    gs.close()

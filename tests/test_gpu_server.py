@@ -164,3 +164,48 @@ def test_server_on_its_own_thread(gpu_ctx):
     srv.close()
     gbox.close()
     gfeat.close()
+
+
+def test_config2_served_mix_over_three_resident_scenes(gpu_ctx):
+    """The serving workload's mix on its three resident scenes -- the box (256^2 .. 1280x720), the
+    many-lights scene from its camera and from a point of a 40 m sphere -- with 1080p ReSTIR
+    (M=32, 3 spatial neighbours within 30 px) and 1080p PathTracer (depth 4) requests among smaller
+    ones, served with continuous batching: every image equals the request rendered alone."""
+    from hdrestir_amd import gpu
+    box = scenes.box_spheres(256, 256, frames=1)
+    many = scenes.many_lights(frames=1, dolly=False)
+    gbox, gmany = gpu.GpuScene(gpu_ctx, box), gpu.GpuScene(gpu_ctx, many)
+    rng = np.random.default_rng(5)
+    sphere_view = scenes.look_at_origin_from_sphere(rng, 40.0)
+    restir = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=1,
+                          spatial_count=3, spatial_radius=30.0)
+    pt = abi.pipeline("mis", max_depth=4)
+    box_view, many_view = box.cameras[0][0], many.cameras[0][0]
+    # (scene handle, view matrix, width, height, spp, pipeline)
+    reqs = [(gmany, many_view, 1920, 1080, 3, restir), (gmany, sphere_view, 1920, 1080, 2, pt),
+            (gbox, box_view, 256, 256, 5, pt), (gbox, box_view, 512, 512, 4, restir),
+            (gmany, many_view, 1280, 720, 2, pt), (gmany, sphere_view, 512, 512, 3, restir),
+            (gbox, box_view, 1280, 720, 1, restir), (gmany, many_view, 256, 256, 6, restir)]
+
+    def frame(view, w, h, f):
+        return abi.frame(view, scenes.perspective(50.0, w / h, 0.05, 1000.0), f)
+    srv = gpu.Server(gpu_ctx, batched=True, max_active=64, hbm_budget_bytes=64 << 30)
+    ids = {srv.submit(g, w, h, spp, frame(v, w, h, 0), p): k for k, (g, v, w, h, spp, p) in enumerate(reqs)}
+    got = {}
+    for _ in range(200):
+        srv.step()
+        for res, img in srv.results(images=True):
+            assert res["status"] == 0
+            got[ids[res["id"]]] = img
+        if len(got) == len(reqs):
+            break
+    assert len(got) == len(reqs)
+    for k, (g, v, w, h, spp, p) in enumerate(reqs):
+        r = gpu.Renderer(gpu_ctx, g, w, h)
+        for f in range(spp):
+            r.render(frame(v, w, h, f), p, read_color=False)
+        assert np.array_equal(got[k], r.read_color()), k
+        r.close()
+    srv.close()
+    gbox.close()
+    gmany.close()
