@@ -192,10 +192,14 @@ struct hdr_context {
 
 struct hdr_scene {
     hdr_context *ctx = nullptr;
-    uint64_t id = 0;  // unique per upload (G-buffer reuse keys on it)
-    DevBuf mem;
+    uint64_t id = 0;  // unique per upload and per update (G-buffer reuse keys on it)
+    DevBuf nodes;     // BLAS nodes of every mesh, then room for tlas_cap TLAS node records
+    DevBuf geo;       // triangles, then their shading attributes
+    DevBuf state;     // TLAS slot table, instances, materials, lights, dome records
+    std::vector<std::unique_ptr<DevBuf>> dome_mem;  // per dome: texels, row CDF, column CDFs
+    size_t tlas_cap = 0;
     DeviceScene ds{};
-    HostScene hs;
+    HostScene hs;     // the host build (triangles and attributes dropped after the upload)
     uint64_t bytes = 0;
     int max_sample_draws_per_vertex = 3;  // SampleBounce draws: 3, +3 per coat / sheen layer present
     bool any_translucent = false;         // some material has opacity < 0.999 (unbounded draws)
@@ -417,104 +421,203 @@ hdr_status hdr_context_set_cancel(hdr_context *ctx, int32_t cancelled) {
     return HDR_OK;
 }
 
-hdr_status hdr_scene_upload(hdr_context *ctx, const hdr_scene_desc *desc, hdr_scene **out) {
-    if (!ctx || !desc || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
-    std::unique_ptr<hdr_scene> sc(new (std::nothrow) hdr_scene());
-    if (!sc) return fail(HDR_ERR_OUT_OF_MEMORY, "host allocation failed");
-    sc->ctx = ctx;
-    static std::atomic<uint64_t> nextSceneId{1};
-    sc->id = nextSceneId.fetch_add(1);
-    const std::string err = build_host_scene(*desc, sc->hs);
-    if (!err.empty()) return fail(HDR_ERR_INVALID_ARGUMENT, err);
-    HostScene &h = sc->hs;
-    for (int i = 0; i < desc->num_materials; ++i) {
-        if (desc->materials[i].coat > 0.0f) sc->max_sample_draws_per_vertex += 3;
-        if (desc->materials[i].sheen > 0.0f) sc->max_sample_draws_per_vertex += 3;
-        if (desc->materials[i].opacity < 0.999f) sc->any_translucent = true;
+namespace {
+
+void swap_buf(DevBuf &a, DevBuf &b) {
+    std::swap(a.p, b.p);
+    std::swap(a.n, b.n);
+}
+
+// Per-material draw bound and translucency of a scene (plan_pipeline reads them per frame).
+void material_traits(const hdr_scene_desc &d, hdr_scene *sc) {
+    int draws = 3;
+    bool translucent = false;
+    for (int i = 0; i < d.num_materials; ++i) {
+        if (d.materials[i].coat > 0.0f) draws += 3;
+        if (d.materials[i].sheen > 0.0f) draws += 3;
+        if (d.materials[i].opacity < 0.999f) translucent = true;
     }
-    sc->max_sample_draws_per_vertex = std::min(sc->max_sample_draws_per_vertex, 9);
-    // one allocation, 256-byte aligned sub-ranges
+    sc->max_sample_draws_per_vertex = std::min(draws, 9);
+    sc->any_translucent = translucent;
+}
+
+// Places the host build's TLAS, instances, materials and lights in device memory: the TLAS nodes
+// after the BLAS nodes (the node allocation grows when they do not fit its room), everything
+// else in the state allocation (reused when large enough). Dome k keeps the texels and CDFs it
+// has on the device when keep[k] (its record then comes from kept_rec[k]); the others are
+// uploaded and their CDFs (DomeLight::Prepare) built on the device. The device is idle.
+hdr_status place_state(hdr_scene *sc, const std::vector<char> &keep, const std::vector<GDome> &kept_rec,
+                       std::vector<std::unique_ptr<DevBuf>> &old_dome_mem) {
+    HostScene &h = sc->hs;
+    const size_t tlasNodes = h.nodes.size() - size_t(h.blas_nodes);
+    if (tlasNodes > sc->tlas_cap || !sc->nodes.p) {
+        // TLAS of n instances: at most 4n - 1 records plus the odd-base pad (children are
+        // allocated for leaves too), so this room holds every rebuild with as many instances
+        const size_t cap = std::max(tlasNodes, size_t(4) * h.instances.size() + 2);
+        DevBuf nn;
+        HIP_TRY(nn.alloc((size_t(h.blas_nodes) + cap) * sizeof(GNode)));
+        if (sc->nodes.p && h.blas_nodes > 0)
+            HIP_TRY(hipMemcpy(nn.p, sc->nodes.p, size_t(h.blas_nodes) * sizeof(GNode), hipMemcpyDeviceToDevice));
+        else if (h.blas_nodes > 0)
+            HIP_TRY(hipMemcpy(nn.p, h.nodes.data(), size_t(h.blas_nodes) * sizeof(GNode), hipMemcpyHostToDevice));
+        swap_buf(sc->nodes, nn);
+        sc->tlas_cap = cap;
+    }
+    if (tlasNodes)
+        HIP_TRY(hipMemcpy(static_cast<GNode *>(sc->nodes.p) + h.blas_nodes, h.nodes.data() + h.blas_nodes,
+                          tlasNodes * sizeof(GNode), hipMemcpyHostToDevice));
+
     size_t off = 0;
     auto place = [&](size_t bytes) {
         const size_t o = off;
         off += (bytes + 255) & ~size_t(255);
         return o;
     };
-    const size_t oNodes = place(h.nodes.size() * sizeof(GNode));
-    const size_t oTris = place(h.tris.size() * sizeof(GTri));
-    const size_t oAttrs = place(h.attrs.size() * sizeof(GTriAttr));
     const size_t oTidx = place(h.tlas_idx.size() * sizeof(int));
     const size_t oInst = place(h.instances.size() * sizeof(GInstance));
     const size_t oMat = place(h.materials.size() * sizeof(GMaterial));
     const size_t oLights = place(h.lights.size() * sizeof(GLight));
     const size_t oDomes = place(h.domes.size() * sizeof(GDome));
-    std::vector<size_t> oDomePix(h.domes.size()), oDomeRow(h.domes.size()), oDomeCol(h.domes.size());
-    for (size_t k = 0; k < h.domes.size(); ++k) {
-        oDomePix[k] = place(h.domes[k].pixels.size() * sizeof(float));
-        oDomeRow[k] = place(size_t(h.domes[k].h + 1) * sizeof(float));
-        oDomeCol[k] = place(size_t(h.domes[k].h) * size_t(h.domes[k].w + 1) * sizeof(float));
+    if (off > sc->state.n || !sc->state.p) {
+        DevBuf ns;
+        HIP_TRY(ns.alloc(off));
+        swap_buf(sc->state, ns);
     }
-    HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(sc->mem.alloc(off));
-    char *base = static_cast<char *>(sc->mem.p);
+    char *base = static_cast<char *>(sc->state.p);
     auto up = [&](size_t o, const void *src, size_t bytes) -> hipError_t {
         return bytes ? hipMemcpy(base + o, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
     };
-    HIP_TRY(up(oNodes, h.nodes.data(), h.nodes.size() * sizeof(GNode)));
-    HIP_TRY(up(oTris, h.tris.data(), h.tris.size() * sizeof(GTri)));
-    HIP_TRY(up(oAttrs, h.attrs.data(), h.attrs.size() * sizeof(GTriAttr)));
     HIP_TRY(up(oTidx, h.tlas_idx.data(), h.tlas_idx.size() * sizeof(int)));
     HIP_TRY(up(oInst, h.instances.data(), h.instances.size() * sizeof(GInstance)));
     HIP_TRY(up(oMat, h.materials.data(), h.materials.size() * sizeof(GMaterial)));
     HIP_TRY(up(oLights, h.lights.data(), h.lights.size() * sizeof(GLight)));
+
+    std::vector<std::unique_ptr<DevBuf>> domeMem(h.domes.size());
     std::vector<GDome> gd(h.domes.size());
+    std::vector<const float *> sinT(h.domes.size(), nullptr);  // per-row weights of new domes
+    GDome *recs = reinterpret_cast<GDome *>(base + oDomes);
     for (size_t k = 0; k < h.domes.size(); ++k) {
-        HIP_TRY(up(oDomePix[k], h.domes[k].pixels.data(), h.domes[k].pixels.size() * sizeof(float)));
-        gd[k].pixels = reinterpret_cast<const float *>(base + oDomePix[k]);
-        gd[k].row_cdf = reinterpret_cast<const float *>(base + oDomeRow[k]);
-        gd[k].col_cdf = reinterpret_cast<const float *>(base + oDomeCol[k]);
-        gd[k].w = h.domes[k].w;
-        gd[k].h = h.domes[k].h;
+        const HostScene::Dome &dm = h.domes[k];
+        if (k < keep.size() && keep[k]) {
+            domeMem[k] = std::move(old_dome_mem[k]);
+            gd[k] = kept_rec[k];
+            continue;
+        }
+        size_t doff = 0;
+        auto dplace = [&](size_t bytes) {
+            const size_t o = doff;
+            doff += (bytes + 255) & ~size_t(255);
+            return o;
+        };
+        const size_t oPix = dplace(dm.pixels.size() * sizeof(float));
+        const size_t oRow = dplace(size_t(dm.h + 1) * sizeof(float));
+        const size_t oCol = dplace(size_t(dm.h) * size_t(dm.w + 1) * sizeof(float));
+        const size_t oSin = dplace(size_t(dm.h) * sizeof(float));
+        domeMem[k].reset(new DevBuf());
+        HIP_TRY(domeMem[k]->alloc(doff));
+        char *db = static_cast<char *>(domeMem[k]->p);
+        HIP_TRY(hipMemcpy(db + oPix, dm.pixels.data(), dm.pixels.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(db + oSin, dm.sin_theta.data(), size_t(dm.h) * sizeof(float), hipMemcpyHostToDevice));
+        sinT[k] = reinterpret_cast<const float *>(db + oSin);
+        gd[k].pixels = reinterpret_cast<const float *>(db + oPix);
+        gd[k].row_cdf = reinterpret_cast<const float *>(db + oRow);
+        gd[k].col_cdf = reinterpret_cast<const float *>(db + oCol);
+        gd[k].w = dm.w;
+        gd[k].h = dm.h;
         gd[k].total_lum = 0.0f;  // set by dome_cdf_total_kernel
         gd[k].pdf_scale = 0.0f;
     }
-    HIP_TRY(up(oDomes, gd.data(), gd.size() * sizeof(GDome)));
+    if (!gd.empty()) HIP_TRY(up(oDomes, gd.data(), gd.size() * sizeof(GDome)));
     // DomeLight::Prepare on the device: column CDFs one row per thread, then the row CDF
     for (size_t k = 0; k < h.domes.size(); ++k) {
+        if (k < keep.size() && keep[k]) continue;
         const HostScene::Dome &dm = h.domes[k];
-        DevBuf sinT;
-        HIP_TRY(sinT.alloc(size_t(dm.h) * sizeof(float)));
-        HIP_TRY(hipMemcpy(sinT.p, dm.sin_theta.data(), size_t(dm.h) * sizeof(float), hipMemcpyHostToDevice));
-        float *row = reinterpret_cast<float *>(base + oDomeRow[k]);
-        float *col = reinterpret_cast<float *>(base + oDomeCol[k]);
-        hipLaunchKernelGGL(dome_cdf_rows_kernel, dim3((dm.h + 63) / 64), dim3(64), 0, 0,
-                           reinterpret_cast<const float *>(base + oDomePix[k]), static_cast<const float *>(sinT.p),
+        float *row = const_cast<float *>(gd[k].row_cdf);
+        float *col = const_cast<float *>(gd[k].col_cdf);
+        hipLaunchKernelGGL(dome_cdf_rows_kernel, dim3((dm.h + 63) / 64), dim3(64), 0, 0, gd[k].pixels, sinT[k],
                            dm.w, dm.h, row, col);
-        hipLaunchKernelGGL(dome_cdf_total_kernel, dim3(1), dim3(1), 0, 0, row, dm.w, dm.h,
-                           reinterpret_cast<GDome *>(base + oDomes) + k);
+        hipLaunchKernelGGL(dome_cdf_total_kernel, dim3(1), dim3(1), 0, 0, row, dm.w, dm.h, recs + k);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipDeviceSynchronize());
     }
+    HIP_TRY(hipDeviceSynchronize());
+    old_dome_mem.clear();
+    sc->dome_mem = std::move(domeMem);
+
     DeviceScene &ds = sc->ds;
-    ds.nodes = reinterpret_cast<const GNode *>(base + oNodes);
-    ds.tris = reinterpret_cast<const GTri *>(base + oTris);
-    ds.attrs = reinterpret_cast<const GTriAttr *>(base + oAttrs);
+    ds.nodes = static_cast<const GNode *>(sc->nodes.p);
     ds.tlas_idx = reinterpret_cast<const int *>(base + oTidx);
     ds.instances = reinterpret_cast<const GInstance *>(base + oInst);
     ds.materials = reinterpret_cast<const GMaterial *>(base + oMat);
     ds.lights = reinterpret_cast<const GLight *>(base + oLights);
-    ds.domes = reinterpret_cast<const GDome *>(base + oDomes);
+    ds.domes = recs;
     ds.tlas_root = h.tlas_root;
     ds.num_instances = int(h.instances.size());
-    ds.num_materials = desc->num_materials;
+    ds.num_materials = h.num_materials;
     ds.num_lights = int(h.lights.size());
     ds.environment = h.environment;
     ds.env_dome = h.env_dome;
     ds.has_mesh_lights = h.has_mesh_lights;
     ds.has_emission = h.has_emission;
-    sc->bytes = off;
+    uint64_t bytes = sc->nodes.n + sc->geo.n + sc->state.n;
+    for (const auto &m : sc->dome_mem) bytes += m->n;
+    sc->bytes = bytes;
+    static std::atomic<uint64_t> nextSceneId{1};
+    sc->id = nextSceneId.fetch_add(1);
+    return HDR_OK;
+}
+
+}  // namespace
+
+hdr_status hdr_scene_upload(hdr_context *ctx, const hdr_scene_desc *desc, hdr_scene **out) {
+    if (!ctx || !desc || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    std::unique_ptr<hdr_scene> sc(new (std::nothrow) hdr_scene());
+    if (!sc) return fail(HDR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    sc->ctx = ctx;
+    const std::string err = build_host_scene(*desc, sc->hs);
+    if (!err.empty()) return fail(HDR_ERR_INVALID_ARGUMENT, err);
+    material_traits(*desc, sc.get());
+    HostScene &h = sc->hs;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t triBytes = (h.tris.size() * sizeof(GTri) + 255) & ~size_t(255);
+    HIP_TRY(sc->geo.alloc(triBytes + h.attrs.size() * sizeof(GTriAttr)));
+    char *g = static_cast<char *>(sc->geo.p);
+    if (!h.tris.empty()) {
+        HIP_TRY(hipMemcpy(g, h.tris.data(), h.tris.size() * sizeof(GTri), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(g + triBytes, h.attrs.data(), h.attrs.size() * sizeof(GTriAttr), hipMemcpyHostToDevice));
+    }
+    sc->ds.tris = reinterpret_cast<const GTri *>(g);
+    sc->ds.attrs = reinterpret_cast<const GTriAttr *>(g + triBytes);
+    std::vector<std::unique_ptr<DevBuf>> none;
+    const hdr_status st = place_state(sc.get(), {}, {}, none);
+    if (st != HDR_OK) return st;
+    std::vector<GTri>().swap(h.tris);
+    std::vector<GTriAttr>().swap(h.attrs);
     *out = sc.release();
     return HDR_OK;
+}
+
+hdr_status hdr_scene_update(hdr_scene *scene, const hdr_scene_desc *desc) {
+    if (!scene || !desc) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipSetDevice(scene->ctx->device));
+    // frames in flight read the state this call rewrites
+    HIP_TRY(hipDeviceSynchronize());
+    std::vector<HostScene::Dome> oldDomes;
+    oldDomes.swap(scene->hs.domes);
+    const std::string err = rebuild_host_state(*desc, scene->hs);
+    if (!err.empty()) {
+        scene->hs.domes.swap(oldDomes);
+        return fail(HDR_ERR_INVALID_ARGUMENT, err);
+    }
+    // a dome whose texels are unchanged keeps its device texels, CDFs and record
+    const std::vector<HostScene::Dome> &nd = scene->hs.domes;
+    std::vector<char> keep(nd.size(), 0);
+    std::vector<GDome> keptRec(nd.size());
+    for (size_t k = 0; k < nd.size() && k < oldDomes.size(); ++k) {
+        keep[k] = nd[k].w == oldDomes[k].w && nd[k].h == oldDomes[k].h && nd[k].pixels == oldDomes[k].pixels;
+        if (keep[k]) HIP_TRY(hipMemcpy(&keptRec[k], scene->ds.domes + k, sizeof(GDome), hipMemcpyDeviceToHost));
+    }
+    material_traits(*desc, scene);
+    return place_state(scene, keep, keptRec, scene->dome_mem);
 }
 
 hdr_status hdr_scene_destroy(hdr_scene *scene) {

@@ -186,15 +186,15 @@ bool finite_all(const float *p, size_t n) {
 
 }  // namespace
 
-std::string build_host_scene(const hdr_scene_desc &d, HostScene &out) {
+namespace {
+
+// Counts, arrays and record sizes of everything but the meshes.
+std::string check_state_arrays(const hdr_scene_desc &d) {
     if (d.struct_size != sizeof(hdr_scene_desc)) return "hdr_scene_desc.struct_size mismatch";
     if (d.num_meshes < 0 || d.num_materials < 0 || d.num_instances < 0 || d.num_lights < 0)
         return "negative element count in hdr_scene_desc";
-    if ((d.num_meshes && !d.meshes) || (d.num_materials && !d.materials) || (d.num_instances && !d.instances) ||
-        (d.num_lights && !d.lights))
+    if ((d.num_materials && !d.materials) || (d.num_instances && !d.instances) || (d.num_lights && !d.lights))
         return "null array with a non-zero count in hdr_scene_desc";
-    for (int i = 0; i < d.num_meshes; ++i)
-        if (d.meshes[i].struct_size != sizeof(hdr_mesh_desc)) return "hdr_mesh_desc.struct_size mismatch";
     for (int i = 0; i < d.num_materials; ++i)
         if (d.materials[i].struct_size != sizeof(hdr_material_desc)) return "hdr_material_desc.struct_size mismatch";
     for (int i = 0; i < d.num_instances; ++i)
@@ -203,10 +203,35 @@ std::string build_host_scene(const hdr_scene_desc &d, HostScene &out) {
         if (d.lights[i].struct_size != sizeof(hdr_light_desc)) return "hdr_light_desc.struct_size mismatch";
     if (d.environment >= d.num_lights || (d.environment >= 0 && d.lights[d.environment].type != HDR_LIGHT_DOME))
         return "environment must index a dome light";
+    return std::string();
+}
+
+// Materials, instances, TLAS and lights of a scene, built against its meshes' BLAS.
+struct StateBuild {
+    std::vector<GNode> tlas;  // TLAS nodes with global indices (placed from tlas_base on)
+    int tlas_base = 0;
+    HostScene s;              // every field but the geometry
+};
+std::string build_state(const hdr_scene_desc &d, const HostScene &geo, StateBuild &sb);
+// Replaces the state of `out` with the built one; the BLAS nodes stay in front of the TLAS.
+void commit_state(StateBuild &sb, HostScene &out);
+
+}  // namespace
+
+std::string build_host_scene(const hdr_scene_desc &d, HostScene &out) {
+    if (d.struct_size != sizeof(hdr_scene_desc)) return "hdr_scene_desc.struct_size mismatch";
+    if (d.num_meshes > 0 && !d.meshes) return "null array with a non-zero count in hdr_scene_desc";
+    for (int i = 0; i < d.num_meshes; ++i)
+        if (d.meshes[i].struct_size != sizeof(hdr_mesh_desc)) return "hdr_mesh_desc.struct_size mismatch";
+    {
+        const std::string e = check_state_arrays(d);
+        if (!e.empty()) return e;
+    }
 
     // ---- BLAS per mesh (BVH::Build, bvh.cpp:66-182)
-    std::vector<int> meshRoot(d.num_meshes, -1);
-    std::vector<Box> meshRange(d.num_meshes);
+    std::vector<int> &meshRoot = out.mesh_root;
+    meshRoot.assign(size_t(d.num_meshes), -1);
+    out.mesh_range.assign(size_t(d.num_meshes) * 6, 0.0f);
     for (int mi = 0; mi < d.num_meshes; ++mi) {
         const hdr_mesh_desc &m = d.meshes[mi];
         if (m.num_points < 0 || m.num_triangles < 0) return "negative mesh counts";
@@ -258,7 +283,8 @@ std::string build_host_scene(const hdr_scene_desc &d, HostScene &out) {
                 const int id = m.indices[3 * i + k];
                 if (id >= 0 && id < m.num_points) range.extend(P(id));
             }
-        meshRange[mi] = range;
+        const float rv[6] = {range.mn.x, range.mn.y, range.mn.z, range.mx.x, range.mx.y, range.mx.z};
+        std::memcpy(&out.mesh_range[size_t(mi) * 6], rv, sizeof rv);
         if (bb.tris.empty()) continue;
         bb.nodes.reserve(size_t(nIdx) * 2);
         bb.nodes.push_back(GNode{});
@@ -290,6 +316,35 @@ std::string build_host_scene(const hdr_scene_desc &d, HostScene &out) {
         meshRoot[mi] = nodeBase;
     }
 
+    out.num_meshes = d.num_meshes;
+    out.blas_nodes = int(out.nodes.size());
+    if (out.max_blas_depth > kStackDepth - 2) return "BVH deeper than the traversal stack";
+    if (out.tris.size() >= (size_t(1) << 28))
+        return "more than 2^28 triangles or instances (leaf reference range of the traversal)";
+    StateBuild sb;
+    const std::string e = build_state(d, out, sb);
+    if (!e.empty()) return e;
+    commit_state(sb, out);
+    return std::string();
+}
+
+std::string rebuild_host_state(const hdr_scene_desc &d, HostScene &out) {
+    {
+        const std::string e = check_state_arrays(d);
+        if (!e.empty()) return e;
+    }
+    if (d.num_meshes != out.num_meshes) return "an update must keep the scene's mesh count";
+    StateBuild sb;
+    const std::string e = build_state(d, out, sb);
+    if (!e.empty()) return e;
+    commit_state(sb, out);
+    return std::string();
+}
+
+namespace {
+
+std::string build_state(const hdr_scene_desc &d, const HostScene &geo, StateBuild &sb) {
+    HostScene &out = sb.s;
     // ---- materials (+ default material record at index num_materials)
     for (int i = 0; i < d.num_materials; ++i) {
         const hdr_material_desc &s = d.materials[i];
@@ -360,14 +415,17 @@ std::string build_host_scene(const hdr_scene_desc &d, HostScene &out) {
         invert4(a, o);
         for (int r = 0; r < 4; ++r)
             for (int c = 0; c < 4; ++c) g.inv[r * 4 + c] = float(o[r][c]);
-        g.root = meshRoot[s.mesh];
+        g.root = geo.mesh_root[size_t(s.mesh)];
         g.has_params = s.material >= 0 ? 1 : 0;
         g.material = s.material >= 0 ? s.material : d.num_materials;
         g.light = s.light;
         if (s.light >= 0) out.has_mesh_lights = 1;
         out.instances.push_back(g);
         // _TransformBounds, scene.cpp:100-116
-        const Box &r = meshRange[s.mesh];
+        const float *rv = &geo.mesh_range[size_t(s.mesh) * 6];
+        Box r;
+        r.mn = F3{rv[0], rv[1], rv[2]};
+        r.mx = F3{rv[3], rv[4], rv[5]};
         Box b;
         if (r.empty()) {
             b = r;
@@ -427,18 +485,18 @@ std::string build_host_scene(const hdr_scene_desc &d, HostScene &out) {
         };
         Rec::go(tl, idx, instBounds, instCentroid, 0, 0, d.num_instances, 1, tdepth);
         out.max_tlas_depth = tdepth;
-        pad_to_odd(out.nodes);
-        const int base = int(out.nodes.size());
+        // odd base, as pad_to_odd places every tree
+        const int base = geo.blas_nodes % 2 == 0 ? geo.blas_nodes + 1 : geo.blas_nodes;
         for (GNode n : tl) {
             if (n.left >= 0 && n.count == 0) n.left += base;
-            out.nodes.push_back(n);
+            sb.tlas.push_back(n);
         }
+        sb.tlas_base = base;
         out.tlas_idx = idx;
         out.tlas_root = base;
     }
-    if (out.max_blas_depth > kStackDepth - 2 || out.max_tlas_depth > kStackDepth - 2)
-        return "BVH deeper than the traversal stack";
-    if (out.tris.size() >= (size_t(1) << 28) || out.tlas_idx.size() >= (size_t(1) << 28))
+    if (out.max_tlas_depth > kStackDepth - 2) return "BVH deeper than the traversal stack";
+    if (out.tlas_idx.size() >= (size_t(1) << 28))
         return "more than 2^28 triangles or instances (leaf reference range of the traversal)";
 
     // ---- lights (per-sample constants of SampleLight)
@@ -491,7 +549,30 @@ std::string build_host_scene(const hdr_scene_desc &d, HostScene &out) {
     }
     out.environment = d.environment;
     out.env_dome = d.environment >= 0 ? out.lights[d.environment].dome : -1;
+    out.num_materials = d.num_materials;
     return std::string();
 }
 
+void commit_state(StateBuild &sb, HostScene &out) {
+    out.nodes.resize(size_t(out.blas_nodes));
+    if (!sb.tlas.empty()) {
+        pad_to_odd(out.nodes);
+        out.nodes.insert(out.nodes.end(), sb.tlas.begin(), sb.tlas.end());
+    }
+    HostScene &s = sb.s;
+    out.tlas_idx = std::move(s.tlas_idx);
+    out.instances = std::move(s.instances);
+    out.materials = std::move(s.materials);
+    out.lights = std::move(s.lights);
+    out.domes = std::move(s.domes);
+    out.tlas_root = s.tlas_root;
+    out.environment = s.environment;
+    out.env_dome = s.env_dome;
+    out.has_mesh_lights = s.has_mesh_lights;
+    out.has_emission = s.has_emission;
+    out.max_tlas_depth = s.max_tlas_depth;
+    out.num_materials = s.num_materials;
+}
+
+}  // namespace
 }  // namespace hdr

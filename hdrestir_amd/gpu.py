@@ -89,6 +89,7 @@ def lib():
             "hdr_context_set_cancel": [P, I],
             "hdr_scene_upload": [P, C.POINTER(abi.SceneDesc), C.POINTER(P)],
             "hdr_scene_destroy": [P],
+            "hdr_scene_update": [P, C.POINTER(abi.SceneDesc)],
             "hdr_scene_device_bytes": [P, C.POINTER(C.c_uint64)],
             "hdr_scene_read_dome": [P, I, P, P, C.POINTER(C.c_float)],
             "hdr_view_create": [P, I, I, C.POINTER(P)],
@@ -191,6 +192,12 @@ class GpuScene:
         v = C.c_uint64()
         check(lib().hdr_scene_device_bytes(self.h, C.byref(v)))
         return int(v.value)
+
+    def update(self, scene):
+        """hdr_scene_update: the instances, materials and lights of `scene` (the uploaded scene,
+        edited, with the same meshes) replace the resident ones; the BLAS stay."""
+        check(lib().hdr_scene_update(self.h, C.byref(scene.desc)))
+        self.scene = scene
 
     def dome_cdf(self, light, w, h):
         """(row_cdf, col_cdf (h, w+1), total luminance) of dome light `light`, as built on the device."""

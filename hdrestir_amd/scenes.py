@@ -288,6 +288,12 @@ class Scene:
                                 dome=None if dome_pixels is None else np.ascontiguousarray(dome_pixels, np.float32)))
         return len(self.lights) - 1
 
+    def changed(self):
+        """Drops the cached descriptor after the lists were edited (instances, materials, lights);
+        the next `.desc` describes the edited scene. Arrays of earlier descriptors stay alive."""
+        if hasattr(self, "_desc"):
+            del self._desc
+
     # -- stats
     def triangle_counts(self):
         unique = sum(len(m["indices"]) for m in self.meshes)

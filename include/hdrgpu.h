@@ -263,6 +263,21 @@ hdr_status hdr_context_set_cancel(hdr_context *ctx, int32_t cancelled);
  * uploaded texels. The scene stays resident until hdr_scene_destroy. */
 hdr_status hdr_scene_upload(hdr_context *ctx, const hdr_scene_desc *desc, hdr_scene **out);
 hdr_status hdr_scene_destroy(hdr_scene *scene);
+/* Scene change between frames (reference: the Hydra prim syncs (HdRestirMesh / HdRestirInstancer
+ * transforms, HdRestirMaterial, HdRestirLight in source/hydra/) followed by
+ * Scene::BuildRenderState, source/scene/scene.cpp:118-271, which rebuilds the instance list, the
+ * TLAS and the lights every frame and keeps each mesh's BVH). The instances (transforms, bound
+ * materials and lights, their number), materials and lights of `desc` replace the scene's; the
+ * meshes and their BLAS stay as uploaded: desc->num_meshes must equal the uploaded count and
+ * desc->meshes is not read. The TLAS is rebuilt, not refitted, so its tree -- and the choice among
+ * equal-distance hits -- is the reference's for the new transforms. A dome light whose texels are
+ * unchanged keeps its device CDFs; others are rebuilt. The call waits for the device's queued work
+ * first; no frame of the scene may be issued from another thread during it. The scene gets a new
+ * identity (a view's G-buffer of the old scene is not reused). As in the reference, whose render
+ * thread restarts on a scene change, accumulating views are reset by the caller (hdr_view_reset,
+ * frame index 0). HDR_ERR_INVALID_ARGUMENT leaves the scene unchanged; after HDR_ERR_DEVICE or
+ * HDR_ERR_OUT_OF_MEMORY the scene can only be destroyed. */
+hdr_status hdr_scene_update(hdr_scene *scene, const hdr_scene_desc *desc);
 /* device bytes held by the scene */
 hdr_status hdr_scene_device_bytes(const hdr_scene *scene, uint64_t *out);
 /* Reads back the dome CDFs of light `light` as built on the device (DomeLight::Prepare,
