@@ -11,6 +11,8 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "DEVICE",
 
 HDR_LIGHT_DISTANT, HDR_LIGHT_DOME, HDR_LIGHT_RECT, HDR_LIGHT_POINT = 0, 1, 2, 3
 HDR_DIRECT_MIS, HDR_DIRECT_RIS = 0, 1
+HDR_AOV_COLOR, HDR_AOV_DEPTH, HDR_AOV_ALBEDO, HDR_AOV_NORMAL = 0, 1, 2, 3
+HDR_FORMAT_FLOAT32, HDR_FORMAT_FLOAT16, HDR_FORMAT_UNORM8, HDR_FORMAT_SNORM8, HDR_FORMAT_INT32 = 0, 1, 2, 3, 4
 
 HDR_RES_HAS_SAMPLE = 0x01
 HDR_RES_HAS_CANDIDATE = 0x02

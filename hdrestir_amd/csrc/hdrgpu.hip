@@ -1,6 +1,7 @@
 // hdrgpu.hip -- host side and C-ABI of libhdrgpu.so (declared in include/hdrgpu.h), plus the small
 // kernels of scene upload, queue clears, the firefly filter and the RNG parity entry point. The
 // rendering kernels live in k_*.hip (declared in kernels.h).
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -110,6 +111,44 @@ __global__ void firefly_accumulate_kernel(const float4 *frame, float4 *accum, fl
     acc.w += c.w;
     accum[i] = acc;
     color[i] = make_float4(acc.x * invN, acc.y * invN, acc.z * invN, acc.w * invN);
+}
+
+// One render-buffer write of a resolved output: UpscalePass's nearest texel (upscale_pass.cpp:17-31,
+// source x = min(sw - 1, x * sw / dw) in integer arithmetic, likewise y) of an sc-component source,
+// stored as `comps` components of `fmt` the way HdRestirRenderBuffer::_WriteOutput converts
+// (hd_restir_render_buffer.cpp:46-74): missing components 0, half rounded to nearest even, UNorm8
+// clamp(v * 255, 0, 255) and SNorm8 clamp(v * 127, -128, 127) truncated, Int32 truncated (NaN: 0).
+__global__ void resolve_kernel(const float *__restrict__ src, int sc, int sw, int sh, int dw, int dh, int fmt,
+                               int comps, unsigned char *__restrict__ dst) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= dw || y >= dh) return;
+    const int sx = min(sw - 1, int((long long)x * sw / dw));
+    const int sy = min(sh - 1, int((long long)y * sh / dh));
+    const float *v = src + (size_t(sy) * sw + sx) * sc;
+    const size_t o = size_t(y) * dw + x;
+    for (int c = 0; c < comps; ++c) {
+        const float f = c < sc ? v[c] : 0.0f;
+        switch (fmt) {
+        case HDR_FORMAT_FLOAT32: reinterpret_cast<float *>(dst)[o * comps + c] = f; break;
+        case HDR_FORMAT_FLOAT16:
+            reinterpret_cast<unsigned short *>(dst)[o * comps + c] = __half_as_ushort(__float2half_rn(f));
+            break;
+        case HDR_FORMAT_UNORM8: {
+            float t = f * 255.0f;
+            t = t < 0.0f ? 0.0f : (255.0f < t ? 255.0f : t);
+            dst[o * comps + c] = t == t ? (unsigned char)(unsigned int)t : 0;
+            break;
+        }
+        case HDR_FORMAT_SNORM8: {
+            float t = f * 127.0f;
+            t = t < -128.0f ? -128.0f : (127.0f < t ? 127.0f : t);
+            reinterpret_cast<signed char *>(dst)[o * comps + c] = t == t ? (signed char)(int)t : 0;
+            break;
+        }
+        default: reinterpret_cast<int *>(dst)[o * comps + c] = f == f ? int(f) : 0; break;
+        }
+    }
 }
 
 __global__ void rng_kernel(const uint32_t *seeds, int n, int nd, uint64_t *raw, float *flt, uint64_t *spill) {
@@ -316,6 +355,8 @@ struct hdr_view {
     LaunchShape shape;           // queue-driven launches (hdr_view_set_launch_config)
     int counting = 0;
     int frames = 0;
+    int32_t samples = 0;  // accumulated samples: frame_index + 1 of the last frame issued, 0 after a reset
+    DevBuf resolved;      // staging of hdr_view_resolve
     // Frames overlap: the G-buffer is double-buffered by frame parity, and the G-buffer kernel of
     // frame N+1 runs on `gstream` as soon as frame N-1's shading has read that buffer (event
     // `shade_done[parity]`), beside frame N's shading, shadow rays and accumulation on `stream`;
@@ -719,6 +760,7 @@ hdr_status hdr_view_reset(hdr_view *view) {
     HIP_TRY(hipMemsetAsync(view->accum.p, 0, view->accum.n, view->stream));
     HIP_TRY(hipMemsetAsync(view->status.p, 0, view->status.n, view->stream));
     view->frames = 0;
+    view->samples = 0;
     return HDR_OK;
 }
 
@@ -1241,6 +1283,7 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     view->ring_last = view->ring_next;
     view->ring_next = (view->ring_next + 1) % hdr_view::kRing;
     view->frames++;
+    view->samples = frame->frame_index + 1;
     if (!plan.gbuf_reuse) {
         view->gcur = gp;
         view->gvalid = view->gnext_valid;
@@ -1318,26 +1361,103 @@ hdr_status hdr_view_read_reservoirs(hdr_view *view, hdr_reservoir *out) {
     return view_status(view);
 }
 
+// First request of the AOVs: allocate them (zeroed); the G-buffer kernel writes them from the next
+// frame on.
+static hdr_status enable_aovs(hdr_view *view) {
+    if (view->depth.p) return HDR_OK;
+    const size_t P = size_t(view->W) * view->H;
+    view->gvalid = false;  // the next frame runs the G-buffer kernel, which writes the AOVs
+    if (hdr_status s = ensure(view->depth, P * sizeof(float))) return s;
+    if (hdr_status s = ensure(view->albedo, P * 3 * sizeof(float))) return s;
+    if (hdr_status s = ensure(view->normal, P * 3 * sizeof(float))) return s;
+    HIP_TRY(hipMemsetAsync(view->depth.p, 0, view->depth.n, view->stream));
+    HIP_TRY(hipMemsetAsync(view->albedo.p, 0, view->albedo.n, view->stream));
+    HIP_TRY(hipMemsetAsync(view->normal.p, 0, view->normal.n, view->stream));
+    return HDR_OK;
+}
+
 hdr_status hdr_view_read_aovs(hdr_view *view, float *depth, float *albedo, float *normal) {
     if (!view) return fail(HDR_ERR_INVALID_ARGUMENT, "view is null");
     HIP_TRY(hipSetDevice(view->ctx->device));
     const size_t P = size_t(view->W) * view->H;
-    if (!view->depth.p) {
-        // first request: allocate; AOVs are produced from the next frame on
-        view->gvalid = false;  // the AOVs are written by the G-buffer kernel from the next frame on
-        if (hdr_status s = ensure(view->depth, P * sizeof(float))) return s;
-        if (hdr_status s = ensure(view->albedo, P * 3 * sizeof(float))) return s;
-        if (hdr_status s = ensure(view->normal, P * 3 * sizeof(float))) return s;
-        HIP_TRY(hipMemsetAsync(view->depth.p, 0, view->depth.n, view->stream));
-        HIP_TRY(hipMemsetAsync(view->albedo.p, 0, view->albedo.n, view->stream));
-        HIP_TRY(hipMemsetAsync(view->normal.p, 0, view->normal.n, view->stream));
-    }
+    if (hdr_status s = enable_aovs(view)) return s;
     if (depth) HIP_TRY(hipMemcpyAsync(depth, view->depth.p, P * sizeof(float), hipMemcpyDeviceToHost, view->stream));
     if (albedo)
         HIP_TRY(hipMemcpyAsync(albedo, view->albedo.p, P * 3 * sizeof(float), hipMemcpyDeviceToHost, view->stream));
     if (normal)
         HIP_TRY(hipMemcpyAsync(normal, view->normal.p, P * 3 * sizeof(float), hipMemcpyDeviceToHost, view->stream));
     HIP_TRY(hipStreamSynchronize(view->stream));
+    return HDR_OK;
+}
+
+hdr_status hdr_rendered_size(int32_t window_width, int32_t window_height, int32_t level, int32_t *width,
+                             int32_t *height) {
+    if (!width || !height) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    if (window_width <= 0 || window_height <= 0) return fail(HDR_ERR_INVALID_ARGUMENT, "window size must be positive");
+    const int div = 1 << std::clamp(level, 0, 4);
+    *width = std::max(1, (window_width + div - 1) / div);
+    *height = std::max(1, (window_height + div - 1) / div);
+    return HDR_OK;
+}
+
+hdr_status hdr_view_sample_count(const hdr_view *view, int32_t *samples) {
+    if (!view || !samples) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    *samples = view->samples;
+    return HDR_OK;
+}
+
+hdr_status hdr_view_resolve(hdr_view *view, int32_t aov, int32_t window_width, int32_t window_height,
+                            int32_t format, int32_t components, void *dst) {
+    if (!view || !dst) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    if (aov < HDR_AOV_COLOR || aov > HDR_AOV_NORMAL) return fail(HDR_ERR_INVALID_ARGUMENT, "unknown aov");
+    if (format < HDR_FORMAT_FLOAT32 || format > HDR_FORMAT_INT32) return fail(HDR_ERR_INVALID_ARGUMENT, "unknown format");
+    if (components < 1 || components > 4) return fail(HDR_ERR_INVALID_ARGUMENT, "components must be 1 .. 4");
+    if (window_width <= 0 || window_height <= 0 || int64_t(window_width) * window_height >= (int64_t(1) << 28))
+        return fail(HDR_ERR_INVALID_ARGUMENT, "window size out of range");
+    HIP_TRY(hipSetDevice(view->ctx->device));
+    if (aov != HDR_AOV_COLOR)
+        if (hdr_status s = enable_aovs(view)) return s;
+    const float *src = nullptr;
+    int sc = 0;
+    switch (aov) {
+    case HDR_AOV_COLOR: src = static_cast<const float *>(view->color.p), sc = 4; break;
+    case HDR_AOV_DEPTH: src = static_cast<const float *>(view->depth.p), sc = 1; break;
+    case HDR_AOV_ALBEDO: src = static_cast<const float *>(view->albedo.p), sc = 3; break;
+    default: src = static_cast<const float *>(view->normal.p), sc = 3; break;
+    }
+    const size_t esize = (format == HDR_FORMAT_FLOAT32 || format == HDR_FORMAT_INT32) ? 4
+                         : format == HDR_FORMAT_FLOAT16                               ? 2
+                                                                                      : 1;
+    const size_t bytes = size_t(window_width) * window_height * size_t(components) * esize;
+    if (hdr_status s = ensure(view->resolved, bytes)) return s;
+    const dim3 b(16, 16), g((window_width + 15) / 16, (window_height + 15) / 16);
+    hipLaunchKernelGGL(resolve_kernel, g, b, 0, view->stream, src, sc, view->W, view->H, window_width, window_height,
+                       format, components, static_cast<unsigned char *>(view->resolved.p));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(dst, view->resolved.p, bytes, hipMemcpyDeviceToHost, view->stream));
+    HIP_TRY(hipStreamSynchronize(view->stream));
+    return view_status(view);
+}
+
+hdr_status hdr_render_progressive(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
+                                  const hdr_pipeline_desc *pipe, int32_t target_samples, int32_t max_frames,
+                                  int32_t *converged) {
+    if (!view || !scene || !frame || !pipe || !converged) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    if (max_frames < 0) return fail(HDR_ERR_INVALID_ARGUMENT, "max_frames must be >= 0");
+    if (frame->struct_size != sizeof(hdr_frame_desc)) return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_frame_desc.struct_size mismatch");
+    hdr_frame_desc f = *frame;
+    for (int k = 0; k < max_frames && view->samples < target_samples; ++k) {
+        if (view->ctx->cancelled) {
+            *converged = 0;
+            return fail(HDR_ERR_CANCELLED, "cancelled");
+        }
+        f.frame_index = view->samples;
+        if (hdr_status s = hdr_render_frame(view, scene, &f, pipe, nullptr, nullptr)) {
+            *converged = view->samples >= target_samples ? 1 : 0;
+            return s;
+        }
+    }
+    *converged = view->samples >= target_samples ? 1 : 0;
     return HDR_OK;
 }
 
@@ -1824,6 +1944,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         HIP_TRY(hipStreamWaitEvent(views[i]->stream, sl.done, 0));
         HIP_TRY(hipStreamWaitEvent(views[i]->gstream, sl.done, 0));
         views[i]->frames++;
+        views[i]->samples = frames[i].frame_index + 1;
         if (!plans[i].gbuf_reuse) {
             views[i]->gcur = plans[i].gbuf_half;
             views[i]->gvalid = views[i]->gnext_valid;

@@ -101,6 +101,11 @@ def lib():
             "hdr_view_read_reservoirs": [P, P],
             "hdr_view_read_aovs": [P, P, P, P],
             "hdr_view_color_device_ptr": [P, C.POINTER(P)],
+            "hdr_rendered_size": [I, I, I, C.POINTER(I), C.POINTER(I)],
+            "hdr_view_sample_count": [P, C.POINTER(I)],
+            "hdr_view_resolve": [P, I, I, I, I, I, P],
+            "hdr_render_progressive": [P, P, C.POINTER(abi.FrameDesc), C.POINTER(abi.PipelineDesc), I, I,
+                                       C.POINTER(I)],
             "hdr_view_synchronize": [P],
             "hdr_view_last_timing": [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)],
             "hdr_view_read_stage_times": [P, P, P],
@@ -178,6 +183,13 @@ class Context:
         out = np.zeros((len(seeds), n_draws), np.float32)
         check(lib().hdr_rng_float(self.h, _p(seeds), len(seeds), n_draws, _p(out)))
         return out
+
+
+def rendered_size(window_width, window_height, level):
+    """hdr_rendered_size: the (width, height) a window renders at for a resolution level."""
+    w, h = C.c_int32(), C.c_int32()
+    check(lib().hdr_rendered_size(window_width, window_height, level, C.byref(w), C.byref(h)))
+    return int(w.value), int(h.value)
 
 
 class GpuScene:
@@ -289,6 +301,30 @@ class Renderer:
         n = np.zeros((self.H * self.W, 3), np.float32)
         check(lib().hdr_view_read_aovs(self.h, _p(d), _p(a), _p(n)))
         return d, a, n
+
+    _FORMAT_DTYPES = {abi.HDR_FORMAT_FLOAT32: np.float32, abi.HDR_FORMAT_FLOAT16: np.float16,
+                      abi.HDR_FORMAT_UNORM8: np.uint8, abi.HDR_FORMAT_SNORM8: np.int8,
+                      abi.HDR_FORMAT_INT32: np.int32}
+
+    def resolve(self, aov, window_width, window_height, fmt=abi.HDR_FORMAT_FLOAT32, components=4):
+        """hdr_view_resolve: output `aov` upscaled to the window and converted to `components` of
+        `fmt`; an array (window_height, window_width, components)."""
+        out = np.zeros((window_height, window_width, components), self._FORMAT_DTYPES.get(fmt, np.float32))
+        check(lib().hdr_view_resolve(self.h, aov, window_width, window_height, fmt, components, _p(out)))
+        return out
+
+    def sample_count(self):
+        v = C.c_int32()
+        check(lib().hdr_view_sample_count(self.h, C.byref(v)))
+        return int(v.value)
+
+    def render_progressive(self, frame, pipe, target_samples, max_frames):
+        """hdr_render_progressive: renders `frame` with frame_index = the sample count until the view
+        holds target_samples samples (at most max_frames frames); returns whether it converged."""
+        conv = C.c_int32()
+        check(lib().hdr_render_progressive(self.h, self.gscene.h, C.byref(frame), C.byref(pipe), target_samples,
+                                           max_frames, C.byref(conv)))
+        return bool(conv.value)
 
     def set_counting(self, on):
         check(lib().hdr_view_set_counting(self.h, int(on)))

@@ -318,6 +318,49 @@ hdr_status hdr_view_read_reservoirs(hdr_view *view, hdr_reservoir *out);
 hdr_status hdr_view_read_aovs(hdr_view *view, float *depth, float *albedo, float *normal);
 /* Device pointer of the accumulated colour (RGBA32F) for zero-copy consumers. */
 hdr_status hdr_view_color_device_ptr(hdr_view *view, void **out);
+
+/* ---- resolve, upscale and the progressive render loop ---- */
+
+/* Outputs of a view (reference: the AOV output specs Color / Depth / Albedo / Normal,
+ * source/passes/output_names.h) and the render-buffer component formats the reference writes
+ * (HdFormatFloat32, Float16, UNorm8, SNorm8, Int32; hd_restir_render_buffer.cpp:46-74). */
+enum { HDR_AOV_COLOR = 0, HDR_AOV_DEPTH = 1, HDR_AOV_ALBEDO = 2, HDR_AOV_NORMAL = 3 };
+enum { HDR_FORMAT_FLOAT32 = 0, HDR_FORMAT_FLOAT16 = 1, HDR_FORMAT_UNORM8 = 2, HDR_FORMAT_SNORM8 = 3,
+       HDR_FORMAT_INT32 = 4 };
+
+/* The size a window renders at for a resolution level (reference: CameraFrame::RenderedWidth /
+ * RenderedHeight, source/renderContext/camera_frame.h:34-41,72-76): ceil(window / 2^level) with
+ * the level clamped to 0 .. 4, at least 1. Create the view at this size; hdr_view_resolve upscales
+ * to the window. Host-only. */
+hdr_status hdr_rendered_size(int32_t window_width, int32_t window_height, int32_t level, int32_t *width,
+                             int32_t *height);
+
+/* Renderer::ResolveTargets -> HdRestirRenderBuffer::CopyFromFrameBuffer (renderer.cpp:536-563,
+ * hd_restir_render_buffer.cpp:113-145) with UpscalePass (upscale_pass.cpp:17-94) on the device:
+ * output `aov` of the view, nearest-texel resampled from the view's size to window_width x
+ * window_height (equal sizes copy), converted to `components` (1-4) components of `format` per pixel
+ * as _WriteOutput does (missing components 0; Float16 rounded to nearest even; UNorm8 / SNorm8
+ * clamp(v * 255 | 127) truncated; Int32 truncated; NaN converts to 0 in the integer formats), and
+ * copied to dst (host, window_width * window_height * components elements, rows packed). Waits for
+ * the view's frames. Depth / albedo / normal are enabled by their first request, as with
+ * hdr_view_read_aovs, and hold zeros until the next frame. */
+hdr_status hdr_view_resolve(hdr_view *view, int32_t aov, int32_t window_width, int32_t window_height,
+                            int32_t format, int32_t components, void *dst);
+
+/* Accumulated samples of the view: frame_index + 1 of the last frame issued to it, 0 after
+ * hdr_view_reset (reference: Camera::FrameCount, renderer.cpp:401-406,477). */
+hdr_status hdr_view_sample_count(const hdr_view *view, int32_t *samples);
+
+/* The render thread's loop (reference: _RenderCallback, hd_restir_render_delegate.cpp:45-54, with
+ * Renderer::IsConverged, renderer.cpp:565-569, and RendererPipelineState::IsConverged,
+ * renderer_pipeline_state.cpp:184-190): while the view holds fewer than target_samples samples and
+ * the context is not cancelled, renders `frame` with frame_index = the view's sample count, at most
+ * max_frames frames per call (so a caller can resolve between calls, as Hydra resolves its render
+ * buffers while the thread runs). *converged = sample count >= target_samples. HDR_ERR_CANCELLED
+ * when the context's cancel flag stops it. */
+hdr_status hdr_render_progressive(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
+                                  const hdr_pipeline_desc *pipeline, int32_t target_samples, int32_t max_frames,
+                                  int32_t *converged);
 /* Blocks until all work queued on the view's stream has finished. */
 hdr_status hdr_view_synchronize(hdr_view *view);
 /* Device time (ms) of the last hdr_render_frame's kernels: G-buffer (RaycastPass), integration,

@@ -87,3 +87,18 @@ def test_cpp_adapter_reports_missing_device():
     r = subprocess.run([gpu.ADAPTER_SMOKE, "--no-device"], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "no device" in r.stdout
+
+
+def test_rendered_size_is_ceil_of_window_over_two_to_the_level():
+    """hdr_rendered_size (host-only): CameraFrame::RenderedWidth/Height, ceil(window / 2^level)
+    with the level clamped to 0 .. 4 and at least 1 pixel; non-positive windows are refused."""
+    from hdrestir_amd import gpu
+    if not os.path.exists(gpu.LIB_PATH):
+        pytest.skip("libhdrgpu.so not built")
+    for w, h in ((1920, 1080), (67, 45), (1, 1), (3, 1000)):
+        for level in range(-2, 8):
+            div = 2 ** min(max(level, 0), 4)
+            assert gpu.rendered_size(w, h, level) == (max(1, -(-w // div)), max(1, -(-h // div))), (w, h, level)
+    assert gpu.rendered_size(1920, 1080, 2) == (480, 270) and gpu.rendered_size(1921, 1081, 1) == (961, 541)
+    with pytest.raises(gpu.HdrError, match="INVALID_ARGUMENT"):
+        gpu.rendered_size(0, 10, 1)
