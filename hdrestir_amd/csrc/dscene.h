@@ -228,9 +228,13 @@ constexpr int kStackDepth = 64;
 // pops on the traversal's critical path read LDS.
 template <bool kTa>
 struct PrivStack {
-    int ref[kStackDepth];
-    float ta[kTa ? kStackDepth : 1];  // any-hit queries keep no entry distances
+    // The entries live in arrays of the caller (private memory); the cursor is kept apart from them
+    // so it stays in a register (a struct holding both would keep it in scratch beside the
+    // dynamically indexed array).
+    int *ref;
+    float *ta;  // entry distances (kTa); any-hit queries keep none
     int sp;
+    __device__ __forceinline__ PrivStack(int *r, float *t) : ref(r), ta(t), sp(0) {}
     __device__ __forceinline__ void reset() { sp = 0; }
     __device__ __forceinline__ bool empty() const { return sp == 0; }
     __device__ __forceinline__ void push(int r, float t) {
@@ -250,12 +254,14 @@ struct RingStack {
     static_assert((kRing & (kRing - 1)) == 0, "ring size must be a power of two");
     int *lr;    // this lane's LDS column of references (slot i at lr[i * 64])
     float *lt;  // and of entry distances (kTa)
-    int ref[kStackDepth];
-    float ta[kStackDepth];
+    int *ref;   // older entries: arrays of the caller in private memory (kept apart from the cursors)
+    float *ta;
     int sp, valid;  // entries on the stack; how many of the top ones are in the ring
-    __device__ __forceinline__ void bind(int *r, float *t) {
+    __device__ __forceinline__ void bind(int *r, float *t, int *pr, float *pt) {
         lr = r;
         lt = t;
+        ref = pr;
+        ta = pt;
         sp = valid = 0;
     }
     __device__ __forceinline__ void reset() { sp = valid = 0; }
@@ -696,7 +702,9 @@ __device__ __forceinline__ bool trace_closest_wave(const DeviceScene &s, bool ac
 
 template <bool kCount>
 __device__ __forceinline__ bool trace_closest(const DeviceScene &s, V3 o, V3 d, RawHit &hit, TraceStats &st) {
-    PrivStack<true> t, b;
+    int tr[kStackDepth], br[kStackDepth];
+    float tt[kStackDepth], bt[kStackDepth];
+    PrivStack<true> t(tr, tt), b(br, bt);
     return trace_closest_s<kCount>(s, o, d, hit, st, t, b);
 }
 
@@ -798,7 +806,8 @@ __device__ __forceinline__ bool trace_any_s(const DeviceScene &s, V3 o, V3 d, fl
 }
 template <bool kCount>
 __device__ __forceinline__ bool trace_any(const DeviceScene &s, V3 o, V3 d, float tmax, TraceStats &st) {
-    PrivStack<false> t, b;
+    int tr[kStackDepth], br[kStackDepth];
+    PrivStack<false> t(tr, nullptr), b(br, nullptr);
     return trace_any_s<kCount>(s, o, d, tmax, st, t, b);
 }
 

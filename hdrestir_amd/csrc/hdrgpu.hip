@@ -63,8 +63,12 @@ __global__ void dome_cdf_total_kernel(float *row, int w, int h, GDome *rec) {
     rec->pdf_scale = total * (float(M_PI) / float(h)) * (2.0f * float(M_PI) / float(w));
 }
 
-// Clears a counter (a kernel rather than a memset, so the step can sit in a captured graph).
-__global__ void zero_u32_kernel(unsigned int *p) { *p = 0u; }
+// Clears the shadow-query count and the persistent shadow launch's claim counter (a kernel rather
+// than a memset, so the step can sit in a captured graph).
+__global__ void zero_u32x2_kernel(unsigned int *p) {
+    p[0] = 0u;
+    p[1] = 0u;
+}
 
 // ---------------------------------------------------------------------------- wavefront path
 // RIS / ReSTIR at every vertex of paths of any depth (wavefront.h). Per frame: the G-buffer, then
@@ -227,6 +231,7 @@ struct PinnedBuf {
 struct hdr_context {
     int device = 0;
     int cancelled = 0;
+    int cus = 0;  // compute units of the device (persistent launches fill them)
 };
 
 struct hdr_scene {
@@ -315,10 +320,21 @@ static void launch_shadow_l(const LaunchShape &ls, int blocks, hipStream_t st, c
     default: hipLaunchKernelGGL((shadow_kernel<C, 1, L>), dim3(blocks), dim3(kLanes), 0, st, a); break;
     }
 }
+// Shadow rays of a launch's queue (at most `cap` queries). With private-memory stacks the persistent
+// kernel runs (4 waves per SIMD at most 128 VGPRs, lanes refilled in groups of 16): one wave per SIMD
+// slot of the device, or queue_waves of the launch shape. one_scene: every view of the launch
+// renders args[0]'s scene.
 template <bool C>
-static void launch_shadow(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a) {
-    if (ls.lds_stack) launch_shadow_l<C, true>(ls, blocks, st, a);
-    else launch_shadow_l<C, false>(ls, blocks, st, a);
+static void launch_shadow(const LaunchShape &ls, int cus, size_t cap, hipStream_t st, const FrameArgs *a,
+                          bool one_scene) {
+    if (!ls.lds_stack) {
+        const size_t slots = ls.queue_waves > 0 ? size_t(ls.queue_waves) : size_t(cus) * 4 * 4;
+        const dim3 g(unsigned(std::min(slots, std::max<size_t>(1, (cap + kLanes - 1) / kLanes))));
+        if (one_scene) hipLaunchKernelGGL((shadow_persistent_kernel<C, 4, 16, true>), g, dim3(kLanes), 0, st, a);
+        else hipLaunchKernelGGL((shadow_persistent_kernel<C, 4, 16, false>), g, dim3(kLanes), 0, st, a);
+        return;
+    }
+    launch_shadow_l<C, true>(ls, queue_blocks(ls, cap), st, a);
 }
 template <bool C>
 static void launch_gbuffer(const LaunchShape &ls, unsigned tiles, hipStream_t st, const FrameArgs *a, const int *t,
@@ -447,6 +463,8 @@ hdr_status hdr_context_create(int32_t device, hdr_context **out) {
     auto *c = new (std::nothrow) hdr_context();
     if (!c) return fail(HDR_ERR_OUT_OF_MEMORY, "host allocation failed");
     c->device = device;
+    if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || c->cus <= 0)
+        c->cus = 256;
     *out = c;
     return HDR_OK;
 }
@@ -705,7 +723,7 @@ hdr_status hdr_view_create(hdr_context *ctx, int32_t width, int32_t height, hdr_
     HIP_TRY(v->counters.alloc(sizeof(hdr_frame_counters)));
     HIP_TRY(v->status.alloc(sizeof(unsigned int)));
     HIP_TRY(v->gbuf.alloc(2 * P * sizeof(GRec)));  // two halves, by frame parity
-    HIP_TRY(v->qcount.alloc(sizeof(unsigned int)));
+    HIP_TRY(v->qcount.alloc(2 * sizeof(unsigned int)));  // query count, persistent claim counter
     HIP_TRY(v->dargs.alloc(hdr_view::kRing * sizeof(FrameArgs)));
     HIP_TRY(v->dtile.alloc(2 * sizeof(int)));
     HIP_TRY(v->hargs.alloc(hdr_view::kRing));
@@ -1135,7 +1153,7 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     HIP_TRY(hipStreamWaitEvent(view->stream, view->gbuf_done, 0));
     HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_INTEGRATE], view->stream));
     if (deferred) {
-        HIP_TRY(hipMemsetAsync(a.qcount, 0, sizeof(unsigned int), view->stream));
+        HIP_TRY(hipMemsetAsync(a.qcount, 0, 2 * sizeof(unsigned int), view->stream));
 #define HDR_SHADE(C, D) \
     hipLaunchKernelGGL((shade_direct_kernel<C, D>), grid, dim3(kLanes), lds, view->stream, da, dt, 1)
         if (view->counting) {
@@ -1166,9 +1184,8 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         HIP_TRY(hipEventRecord(view->shade_done[gp], view->stream));
         view->shade_recorded[gp] = true;
         HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_SHADOW], view->stream));
-        const int sblocks = queue_blocks(view->shape, P);
-        if (view->counting) launch_shadow<true>(view->shape, sblocks, view->stream, da);
-        else launch_shadow<false>(view->shape, sblocks, view->stream, da);
+        if (view->counting) launch_shadow<true>(view->shape, view->ctx->cus, P, view->stream, da, true);
+        else launch_shadow<false>(view->shape, view->ctx->cus, P, view->stream, da, true);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_SHADOW], view->stream));
         HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_ACCUMULATE], view->stream));
@@ -1597,7 +1614,7 @@ hdr_status hdr_batch_create(hdr_context *ctx, hdr_batch **out) {
     HIP_TRY(hipEventCreate(&b->beg));
     HIP_TRY(hipEventCreate(&b->end));
     for (auto &sl : b->slots) HIP_TRY(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
-    HIP_TRY(b->qcount.alloc(sizeof(unsigned int)));
+    HIP_TRY(b->qcount.alloc(2 * sizeof(unsigned int)));  // query count, persistent claim counter
     HIP_TRY(b->wcnt.alloc(8 * sizeof(unsigned int)));
     *out = b.release();
     return HDR_OK;
@@ -1725,7 +1742,9 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
     int waveDepth[2] = {0, 0};
     bool waveLean0[2] = {true, true};  // every view's primary vertex stays below RNG output 311
     bool waveSpatial[2] = {false, false};  // some view of the group reuses spatially
+    bool oneScene = true;                  // every view renders the first view's scene
     for (int i = 0; i < n; ++i) {
+        oneScene = oneScene && scenes[i] == scenes[0];
         if (hdr_status st =
                 prepare_frame(views[i], scenes[i], &frames[i], &pipes[i], false, args[i], plans[i], true))
             return st;
@@ -1848,7 +1867,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         return hipGetLastError();
     };
     auto enqueue = [&]() -> hipError_t {
-        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, b->stream, static_cast<unsigned int *>(b->qcount.p));
+        hipLaunchKernelGGL(zero_u32x2_kernel, dim3(1), dim3(1), 0, b->stream, static_cast<unsigned int *>(b->qcount.p));
         if (N(kDef) > 0)
             hipLaunchKernelGGL((shade_direct_kernel<false, false>), dim3(tiles[kDef]), dim3(kLanes), lds[kDef],
                                b->stream, A(kDef), T(kDef), N(kDef));
@@ -1879,7 +1898,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
                                A(wg), T(wg), N(wg));
         }
         if (N(kDef) > 0) {
-            launch_shadow<false>(ls, queue_blocks(ls, queueCap), b->stream, A(kAll));
+            launch_shadow<false>(ls, b->ctx->cus, queueCap, b->stream, A(kAll), oneScene);
             hipLaunchKernelGGL((accumulate_deferred_kernel<false>), dim3(tiles[kDef]), dim3(kLanes), 0, b->stream,
                                A(kDef), T(kDef), N(kDef));
         }
@@ -1900,7 +1919,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
                                      uint64_t(waveDepth[1]), uint64_t(waveLean0[0]), uint64_t(waveLean0[1]),
                                      uint64_t(waveSpatial[0]), uint64_t(waveSpatial[1]),
                                      uint64_t(ls.queue_waves), uint64_t(ls.trace_occ), uint64_t(ls.shade_occ),
-                                     uint64_t(ls.lds_stack)};
+                                     uint64_t(ls.lds_stack), uint64_t(oneScene)};
         for (int g = 0; g < kGroups; ++g) {
             key.push_back(uint64_t(tiles[g]));
             key.push_back(uint64_t(lds[g]));

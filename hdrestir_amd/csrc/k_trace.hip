@@ -63,6 +63,172 @@ shadow_kernel(const FrameArgs *__restrict__ args) {
     }
 }
 
+// The same shadow rays with persistent lanes. With one ray per lane, a wave runs until its longest
+// traversal ends, and the counters show about a sixth of the lanes active per instruction. Here a
+// lane whose ray is decided takes the next unclaimed query (claimed in groups through the counter
+// after the query count, qcount[1], zero at launch), so lanes stay busy until the queue is drained.
+// Lanes refill, between leaves, when at least kRefill of them are idle, or all are. kOneScene: every view of the
+// launch renders args[0]'s scene, whose tables are then wave-uniform. The traversal of each ray is
+// trace_any_s's, step for step, so the verdicts are the same.
+template <bool kCount, int kOcc, int kRefill, bool kOneScene>
+__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
+shadow_persistent_kernel(const FrameArgs *__restrict__ args) {
+    const int lane = threadIdx.x;
+    const FrameArgs &a0 = args[0];
+    const unsigned int n = a0.qcount[0];
+    unsigned int *cursor = a0.qcount + 1;
+    TraceStats st = {};
+    int tref[kStackDepth], bref[kStackDepth];
+    int tsp = 0, bsp = 0;
+    RayBox R;
+    V3 o = v3s(0.0f), d = v3s(0.0f), oo = o, od = d;
+    float tmax = 0.0f;
+    int cur = kNoNode, lnext = 0, lend = 0;
+    bool blas = false, world = true;
+    unsigned int view = 0;  // the lane's view (args index) and pixel
+    int pixel = 0;
+    bool live = false, drained = false;  // drained: every query is claimed (wave-uniform)
+    for (;;) {
+        if (!drained) {
+            const unsigned long long idle = __ballot(!live);
+            const int nidle = __popcll(idle);
+            if (nidle >= kRefill || nidle == kLanes) {
+                const int leader = __ffsll((unsigned long long)idle) - 1;
+                unsigned int base = 0;
+                if (lane == leader) base = atomicAdd(cursor, unsigned(nidle));
+                base = __shfl(base, leader);
+                if (base + unsigned(nidle) >= n) drained = true;
+                if (!live) {
+                    const unsigned int i = base + unsigned(__popcll(idle & ((1ULL << lane) - 1ULL)));
+                    if (i < n) {
+                        const ShadowQuery q = a0.queue[i];
+                        const uint32_t code = __float_as_uint(q.d.w);
+                        view = code >> 24;
+                        pixel = int(code & 0xffffffu);
+                        const DeviceScene &s = kOneScene ? a0.s : args[view].s;
+                        o = v3(q.o.x, q.o.y, q.o.z);
+                        d = v3(q.d.x, q.d.y, q.d.z);
+                        tmax = q.o.w;
+                        if (s.tlas_root >= 0) {  // the root test of trace_any_s
+                            R = make_raybox<true>(o, d);
+                            const GNode nr = load_node(s.nodes, s.tlas_root);
+                            if (kCount) ++st.nodes;
+                            float ta;
+                            if (box_hit<true>(R, nr, ta) && !(ta > tmax)) {
+                                cur = node_ref(nr);
+                                tsp = bsp = 0;
+                                blas = false;
+                                world = true;
+                                lnext = lend = 0;
+                                oo = o;
+                                od = d;
+                                live = true;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        if (__ballot(live) == 0ULL) {
+            if (drained) break;
+            continue;
+        }
+        if (!live) continue;
+        const DeviceScene &s = kOneScene ? a0.s : args[view].s;
+        // trace_any_s's iterations, grouped as descent to a leaf (the inner-node loop), the leaf, then
+        // the pop -- one group per pass, so the wave reaches the refill between leaves
+        while (cur >= 0) {
+            const GNode l = load_node(s.nodes, cur), r = load_node(s.nodes, cur + 1);
+            if (kCount) st.nodes += 2;
+            float tl, tr;
+            const bool hl = box_hit<true>(R, l, tl) && !(tl > tmax);
+            const bool hr = box_hit<true>(R, r, tr) && !(tr > tmax);
+            if (!(hl || hr)) {
+                cur = kNoNode;
+                break;
+            }
+            const bool leftFirst = hl && (!hr || !(tr < tl));
+            const int first = leftFirst ? node_ref(l) : node_ref(r);
+            if (hl && hr) {
+                const int second = leftFirst ? node_ref(r) : node_ref(l);
+                if (blas) bref[bsp++] = second;
+                else tref[tsp++] = second;
+            }
+            cur = first;
+        }
+        if (cur != kNoNode) {
+            const int code = -cur - 1;
+            const int start = code >> 3, count = code & 7;
+            bool hit = false;
+            if (blas) {
+                for (int k = 0; k < count; ++k) {
+                    const GTri tri = load_tri(s.tris, start + k);
+                    if (kCount) ++st.tris;
+                    float tt, uu, vv;
+                    if (tri_hit(tri, oo, od, tt, uu, vv) && tt < tmax) {
+                        hit = true;
+                        break;
+                    }
+                }
+            } else {
+                lnext = start;
+                lend = start + count;
+            }
+            if (hit) {
+                const FrameArgs &a = args[view];
+                a.flags[pixel] = 3u;
+                hdr_reservoir &rv = a.res[pixel];
+                rv.throughput[0] = rv.throughput[1] = rv.throughput[2] = rv.throughput[3] = 0.0f;
+                live = false;
+                continue;
+            }
+        }
+        cur = kNoNode;
+        if (blas) {
+            if (bsp > 0) {
+                cur = bref[--bsp];
+                continue;
+            }
+            blas = false;
+        }
+        while (lnext < lend) {
+            float minv[16];
+            const int root = load_inst_inv(s.instances, gload_i(s.tlas_idx + lnext++), minv);
+            if (kCount) ++st.instances;
+            if (root < 0) continue;
+            oo = xform_point(minv, o);
+            od = xform_dir(minv, d);
+            R = make_raybox<false>(oo, od);
+            world = false;
+            bsp = 0;
+            const GNode b = load_node(s.nodes, root);
+            if (kCount) ++st.nodes;
+            float ta;
+            if (box_hit<true>(R, b, ta) && !(ta > tmax)) {
+                cur = node_ref(b);
+                blas = true;
+                break;
+            }
+        }
+        if (blas) continue;
+        if (!world) {
+            R = make_raybox<true>(o, d);
+            world = true;
+        }
+        if (tsp == 0) {
+            live = false;  // no occluder
+            continue;
+        }
+        cur = tref[--tsp];
+    }
+    if (kCount) {
+        unsigned long long *c = a0.counters;
+        wave_add(c + 5, st.nodes);
+        wave_add(c + 6, st.tris);
+        wave_add(c + 7, st.instances);
+    }
+}
+
 // Shadow ray i of the queue (vertex `vtx`): an unoccluded winner's contribution joins the path
 // total (a spatial winner's colour is already staged); an occluded primary winner keeps zero
 // throughput in its reservoir (ris_direct_light_integrator.cpp:163-175).
@@ -215,21 +381,17 @@ template __global__ void gbuffer_kernel<true, false>(const FrameArgs *__restrict
 template __global__ void gbuffer_kernel<false, true>(const FrameArgs *__restrict__, const int *__restrict__, int);
 template __global__ void gbuffer_kernel<false, false>(const FrameArgs *__restrict__, const int *__restrict__, int);
 template __global__ void shadow_kernel<true, 1, true>(const FrameArgs *__restrict__);
-template __global__ void shadow_kernel<true, 1, false>(const FrameArgs *__restrict__);
 template __global__ void shadow_kernel<true, 4, true>(const FrameArgs *__restrict__);
-template __global__ void shadow_kernel<true, 4, false>(const FrameArgs *__restrict__);
 template __global__ void shadow_kernel<true, 6, true>(const FrameArgs *__restrict__);
-template __global__ void shadow_kernel<true, 6, false>(const FrameArgs *__restrict__);
 template __global__ void shadow_kernel<true, 8, true>(const FrameArgs *__restrict__);
-template __global__ void shadow_kernel<true, 8, false>(const FrameArgs *__restrict__);
 template __global__ void shadow_kernel<false, 1, true>(const FrameArgs *__restrict__);
-template __global__ void shadow_kernel<false, 1, false>(const FrameArgs *__restrict__);
 template __global__ void shadow_kernel<false, 4, true>(const FrameArgs *__restrict__);
-template __global__ void shadow_kernel<false, 4, false>(const FrameArgs *__restrict__);
 template __global__ void shadow_kernel<false, 6, true>(const FrameArgs *__restrict__);
-template __global__ void shadow_kernel<false, 6, false>(const FrameArgs *__restrict__);
 template __global__ void shadow_kernel<false, 8, true>(const FrameArgs *__restrict__);
-template __global__ void shadow_kernel<false, 8, false>(const FrameArgs *__restrict__);
+template __global__ void shadow_persistent_kernel<true, 4, 16, true>(const FrameArgs *__restrict__);
+template __global__ void shadow_persistent_kernel<true, 4, 16, false>(const FrameArgs *__restrict__);
+template __global__ void shadow_persistent_kernel<false, 4, 16, true>(const FrameArgs *__restrict__);
+template __global__ void shadow_persistent_kernel<false, 4, 16, false>(const FrameArgs *__restrict__);
 template __global__ void wave_trace_kernel<true, 1, true>(const FrameArgs *__restrict__, int);
 template __global__ void wave_trace_kernel<true, 1, false>(const FrameArgs *__restrict__, int);
 template __global__ void wave_trace_kernel<true, 4, true>(const FrameArgs *__restrict__, int);

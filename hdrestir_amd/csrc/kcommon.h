@@ -34,10 +34,12 @@ template <bool kCount, bool kLds>
 __device__ __forceinline__ bool closest_hit(const DeviceScene &s, V3 o, V3 d, RawHit &rh, TraceStats &ts, int *tr,
                                             float *tt, int *br, float *bt) {
     if (kLds) {
+        int ptr[kStackDepth], pbr[kStackDepth];
+        float ptt[kStackDepth], pbt[kStackDepth];
         RingStack<kRingT, true> t;
         RingStack<kRingB, true> b;
-        t.bind(tr, tt);
-        b.bind(br, bt);
+        t.bind(tr, tt, ptr, ptt);
+        b.bind(br, bt, pbr, pbt);
         return trace_closest_s<kCount>(s, o, d, rh, ts, t, b);
     }
     return trace_closest<kCount>(s, o, d, rh, ts);
@@ -46,10 +48,11 @@ template <bool kCount, bool kLds>
 __device__ __forceinline__ bool any_hit(const DeviceScene &s, V3 o, V3 d, float tmax, TraceStats &ts, int *tr,
                                         int *br) {
     if (kLds) {
+        int ptr[kStackDepth], pbr[kStackDepth];
         RingStack<kRingT, false> t;
         RingStack<kRingB, false> b;
-        t.bind(tr, nullptr);
-        b.bind(br, nullptr);
+        t.bind(tr, nullptr, ptr, nullptr);
+        b.bind(br, nullptr, pbr, nullptr);
         return trace_any_s<kCount>(s, o, d, tmax, ts, t, b);
     }
     return trace_any<kCount>(s, o, d, tmax, ts);

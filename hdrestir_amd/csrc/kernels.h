@@ -14,6 +14,8 @@ template <bool kCount, bool kLds>
 __global__ void gbuffer_kernel(const FrameArgs *__restrict__ args, const int *__restrict__ tile_base, int n_views);
 template <bool kCount, int kOcc, bool kLds>
 __global__ void shadow_kernel(const FrameArgs *__restrict__ args);
+template <bool kCount, int kOcc, int kRefill, bool kOneScene>
+__global__ void shadow_persistent_kernel(const FrameArgs *__restrict__ args);
 template <bool kCount, int kOcc, bool kLds>
 __global__ void wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce);
 template <bool kCount, int kOcc, bool kLds>

@@ -419,14 +419,17 @@ hdr_status hdr_batch_poll(hdr_batch *batch, uint64_t *issued, uint64_t *complete
 typedef struct hdr_launch_config {
     uint32_t struct_size;       /* sizeof(hdr_launch_config) */
     int32_t  queue_waves;       /* waves per queue-driven launch: 0 = one per 64 queue slots, at most
-                                   32768; else 64 .. 65536 */
+                                   32768 (the persistent shadow launch: 4 per SIMD of the device);
+                                   else 64 .. 65536 */
     int32_t  trace_occupancy;   /* traversal kernels: 4, 6, 8 = at least that many waves per SIMD, 1 = the
                                    compiler's register budget; 0 = 8 */
     int32_t  shade_occupancy;   /* wavefront shading kernels: 3 = at least 3 waves per SIMD, 1 = the
                                    compiler's register budget; 0 = 3 */
     int32_t  use_graphs;        /* batches: 1 = replay captured graphs, 0 = launch every kernel */
     int32_t  lds_stack;         /* traversal kernels (G-buffer, shadow, wavefront traversal): 1 = the top
-                                   4 TLAS / 8 BLAS stack entries of each lane in LDS, 0 = private memory */
+                                   4 TLAS / 8 BLAS stack entries of each lane in LDS, 0 = private memory
+                                   (shadow rays then run on persistent lanes that take the next query as
+                                   their ray is decided) */
     int32_t  _reserved[2];
 } hdr_launch_config;
 hdr_status hdr_view_set_launch_config(hdr_view *view, const hdr_launch_config *cfg);
