@@ -80,7 +80,7 @@ __global__ void zero_u32x2_kernel(unsigned int *p) {
 
 // Clears the wavefront queue counts (a kernel, so a step can sit in a captured graph).
 __global__ void wave_zero_counts_kernel(unsigned int *c) {
-    if (threadIdx.x < 8) c[threadIdx.x] = 0u;
+    if (threadIdx.x < kWaveCounts) c[threadIdx.x] = 0u;
 }
 
 // Out-of-place 3x3 firefly clamp (the reference's GPU accumulation shader semantics,
@@ -275,10 +275,13 @@ static void launch_wave_trace_l(const LaunchShape &ls, int blocks, hipStream_t s
     }
 #undef HDR_WT
 }
-// Traversal launch b; traced frames: wave_trace_traced_kernel (bounce, escape and shadow rays).
+// Traversal launch b: the bounce rays (traced frames: wave_trace_traced_kernel), then the launch's
+// any-hit rays -- escape rays on traced frames, shadow rays -- on persistent lanes
+// (wave_anyhit_kernel, 4 waves per SIMD of the device). one_scene: every view of the launch renders
+// a[0]'s scene.
 template <bool C>
 static void launch_wave_trace(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a, int b,
-                              bool traced) {
+                              bool traced, int cus, bool one_scene) {
     if (traced) {
         if (ls.lds_stack) launch_wave_trace_l<C, true, true>(ls, blocks, st, a, b);
         else launch_wave_trace_l<C, false, true>(ls, blocks, st, a, b);
@@ -286,6 +289,16 @@ static void launch_wave_trace(const LaunchShape &ls, int blocks, hipStream_t st,
         if (ls.lds_stack) launch_wave_trace_l<C, true, false>(ls, blocks, st, a, b);
         else launch_wave_trace_l<C, false, false>(ls, blocks, st, a, b);
     }
+    const dim3 g(unsigned(ls.queue_waves > 0 ? ls.queue_waves : cus * 4 * 4)), l(kLanes);
+#define HDR_AH(T, O) hipLaunchKernelGGL((wave_anyhit_kernel<C, T, O>), g, l, 0, st, a, b)
+    if (traced) {
+        if (one_scene) HDR_AH(true, true);
+        else HDR_AH(true, false);
+    } else {
+        if (one_scene) HDR_AH(false, true);
+        else HDR_AH(false, false);
+    }
+#undef HDR_AH
 }
 // Shading launch b >= 1; traced frames: wave_shade_traced_kernel.
 template <bool C, bool D>
@@ -925,7 +938,7 @@ static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_
         need.wray = P * 2 * sizeof(float4);
         need.queue = P * sizeof(ShadowQuery);
         need.wcq = P * sizeof(float4);
-        need.walive = 2 * P * sizeof(unsigned int) + 8 * sizeof(unsigned int);  // two parities + counts
+        need.walive = 2 * P * sizeof(unsigned int) + kWaveCounts * sizeof(unsigned int);  // two parities + counts
         if (spatial) {  // per pixel: emission + reused direct, visible / occluded; visibility + slot
             need.rgb2 = P * 2 * sizeof(float4);
             need.flags = P * sizeof(uint32_t);
@@ -1079,7 +1092,7 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
             if (hdr_status st = ensure(view->queue, need.queue)) return st;
             if (hdr_status st = ensure(view->wcq, need.wcq)) return st;
             if (hdr_status st = ensure(view->walive, 2 * P * sizeof(unsigned int))) return st;  // two parities
-            if (hdr_status st = ensure(view->wcnt, 8 * sizeof(unsigned int))) return st;
+            if (hdr_status st = ensure(view->wcnt, kWaveCounts * sizeof(unsigned int))) return st;
             if (plan.wave_traced) {
                 if (hdr_status st = ensure(view->wesq, need.wesq)) return st;
                 a.wesq = static_cast<float4 *>(view->wesq.p);
@@ -1241,8 +1254,11 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
                 HIP_TRY(hipEventRecord(view->shade_done[gp], view->stream));
                 view->shade_recorded[gp] = true;
             }
-            if (view->counting) launch_wave_trace<true>(view->shape, qblocks, view->stream, da, b, plan.wave_traced);
-            else launch_wave_trace<false>(view->shape, qblocks, view->stream, da, b, plan.wave_traced);
+            if (view->counting)
+                launch_wave_trace<true>(view->shape, qblocks, view->stream, da, b, plan.wave_traced, view->ctx->cus, true);
+            else
+                launch_wave_trace<false>(view->shape, qblocks, view->stream, da, b, plan.wave_traced, view->ctx->cus,
+                                         true);
             HIP_TRY(hipGetLastError());
         }
 #undef HDR_W0
@@ -1615,7 +1631,7 @@ hdr_status hdr_batch_create(hdr_context *ctx, hdr_batch **out) {
     HIP_TRY(hipEventCreate(&b->end));
     for (auto &sl : b->slots) HIP_TRY(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
     HIP_TRY(b->qcount.alloc(2 * sizeof(unsigned int)));  // query count, persistent claim counter
-    HIP_TRY(b->wcnt.alloc(8 * sizeof(unsigned int)));
+    HIP_TRY(b->wcnt.alloc(kWaveCounts * sizeof(unsigned int)));
     *out = b.release();
     return HDR_OK;
 }
@@ -1892,7 +1908,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
                 if (v > 0) launch_wave_shade<false, false>(ls, qblocks, lds[wg], b->stream, A(wg), v, t == 1);
                 if (waveSpatial[t] && v == t)
                     launch_wave_spatial<false>(spatialNd[wg], dim3(tiles[wg]), b->stream, A(wg), T(wg), N(wg));
-                launch_wave_trace<false>(ls, qblocks, b->stream, A(wg), v, t == 1);
+                launch_wave_trace<false>(ls, qblocks, b->stream, A(wg), v, t == 1, b->ctx->cus, oneScene);
             }
             hipLaunchKernelGGL((wave_resolve_kernel<false, false>), dim3(tiles[wg]), dim3(kLanes), 0, b->stream,
                                A(wg), T(wg), N(wg));

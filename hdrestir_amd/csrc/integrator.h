@@ -66,7 +66,8 @@ struct FrameArgs {
     unsigned int *wesc;                // traced frames: per pixel, bit k = candidate k's ray escaped
     unsigned int *walive;              // codes of the live paths of even vertices (bounce ray hit)
     unsigned int *walive2;             // and of odd vertices
-    unsigned int *wcnt;                // queue counts by vertex parity: rays[2], shadows[2], alive[2], escapes[2]
+    unsigned int *wcnt;                // queue counts by vertex parity: rays[2], shadows[2], alive[2], escapes[2],
+                                       // then the any-hit claim counters: escapes[2], shadows[2] (kWaveCounts)
 };
 
 struct PixelStats {
@@ -225,6 +226,7 @@ __device__ __forceinline__ void flag_rng_overrun(const FrameArgs &a, bool overru
     if (overrun && a.status) atomicOr(a.status, kStatusRngOverrun);
 }
 
+constexpr int kWaveCounts = 12;  // words of FrameArgs::wcnt
 constexpr int kLanes = 64;  // threads per block; candidate weights live at wlds[k * kLanes + lane]
 
 // Evaluation of a light sample at a vertex (RisDirectLightIntegrator::_evaluateLightSample,

@@ -63,21 +63,20 @@ shadow_kernel(const FrameArgs *__restrict__ args) {
     }
 }
 
-// The same shadow rays with persistent lanes. With one ray per lane, a wave runs until its longest
-// traversal ends, and the counters show about a sixth of the lanes active per instruction. Here a
-// lane whose ray is decided takes the next unclaimed query (claimed in groups through the counter
-// after the query count, qcount[1], zero at launch), so lanes stay busy until the queue is drained.
-// Lanes refill, between leaves, when at least kRefill of them are idle, or all are. kOneScene: every view of the
-// launch renders args[0]'s scene, whose tables are then wave-uniform. The traversal of each ray is
-// trace_any_s's, step for step, so the verdicts are the same.
-template <bool kCount, int kOcc, int kRefill, bool kOneScene>
-__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
-shadow_persistent_kernel(const FrameArgs *__restrict__ args) {
+// Any-hit rays of a queue on persistent lanes. With one ray per lane, a wave runs until its longest
+// traversal ends (the counters show about a sixth of the lanes active per instruction on shadow
+// rays). Here a lane whose ray is decided takes the next unclaimed ray of the queue -- rays are
+// claimed in groups through *cursor, zero at launch -- so lanes stay busy until the queue is
+// drained. Lanes refill between leaves once at least kRefill of them are idle, or all are.
+// query(i, view, o, d, tmax) decodes ray i and the view (args index) it belongs to; done(i, view,
+// hit) records its verdict. kOneScene: every view of the launch renders args[0]'s scene, whose tables
+// are then wave-uniform. Each ray's traversal is trace_any_s's, step for step, so the verdicts are
+// the same.
+template <bool kCount, int kRefill, bool kOneScene, class Query, class Done>
+__device__ __forceinline__ void anyhit_queue(const FrameArgs *__restrict__ args, unsigned int n,
+                                             unsigned int *cursor, TraceStats &st, Query query, Done done) {
     const int lane = threadIdx.x;
     const FrameArgs &a0 = args[0];
-    const unsigned int n = a0.qcount[0];
-    unsigned int *cursor = a0.qcount + 1;
-    TraceStats st = {};
     int tref[kStackDepth], bref[kStackDepth];
     int tsp = 0, bsp = 0;
     RayBox R;
@@ -85,9 +84,8 @@ shadow_persistent_kernel(const FrameArgs *__restrict__ args) {
     float tmax = 0.0f;
     int cur = kNoNode, lnext = 0, lend = 0;
     bool blas = false, world = true;
-    unsigned int view = 0;  // the lane's view (args index) and pixel
-    int pixel = 0;
-    bool live = false, drained = false;  // drained: every query is claimed (wave-uniform)
+    unsigned int ray = 0, view = 0;  // the lane's ray (queue index) and view
+    bool live = false, drained = false;  // drained: every ray is claimed (wave-uniform)
     for (;;) {
         if (!drained) {
             const unsigned long long idle = __ballot(!live);
@@ -101,14 +99,10 @@ shadow_persistent_kernel(const FrameArgs *__restrict__ args) {
                 if (!live) {
                     const unsigned int i = base + unsigned(__popcll(idle & ((1ULL << lane) - 1ULL)));
                     if (i < n) {
-                        const ShadowQuery q = a0.queue[i];
-                        const uint32_t code = __float_as_uint(q.d.w);
-                        view = code >> 24;
-                        pixel = int(code & 0xffffffu);
+                        ray = i;
+                        query(i, view, o, d, tmax);
                         const DeviceScene &s = kOneScene ? a0.s : args[view].s;
-                        o = v3(q.o.x, q.o.y, q.o.z);
-                        d = v3(q.d.x, q.d.y, q.d.z);
-                        tmax = q.o.w;
+                        bool miss = true;
                         if (s.tlas_root >= 0) {  // the root test of trace_any_s
                             R = make_raybox<true>(o, d);
                             const GNode nr = load_node(s.nodes, s.tlas_root);
@@ -123,8 +117,10 @@ shadow_persistent_kernel(const FrameArgs *__restrict__ args) {
                                 oo = o;
                                 od = d;
                                 live = true;
+                                miss = false;
                             }
                         }
+                        if (miss) done(i, view, false);
                     }
                 }
             }
@@ -175,10 +171,7 @@ shadow_persistent_kernel(const FrameArgs *__restrict__ args) {
                 lend = start + count;
             }
             if (hit) {
-                const FrameArgs &a = args[view];
-                a.flags[pixel] = 3u;
-                hdr_reservoir &rv = a.res[pixel];
-                rv.throughput[0] = rv.throughput[1] = rv.throughput[2] = rv.throughput[3] = 0.0f;
+                done(ray, view, true);
                 live = false;
                 continue;
             }
@@ -216,11 +209,39 @@ shadow_persistent_kernel(const FrameArgs *__restrict__ args) {
             world = true;
         }
         if (tsp == 0) {
-            live = false;  // no occluder
+            done(ray, view, false);  // no occluder
+            live = false;
             continue;
         }
         cur = tref[--tsp];
     }
+}
+
+// The deferred path's shadow rays on persistent lanes (anyhit_queue over the launch's query queue;
+// qcount[1] is its claim counter): an occluded winner keeps zero throughput in its reservoir (the
+// visibility fix-up of ris_direct_light_integrator.cpp:163-175).
+template <bool kCount, int kOcc, int kRefill, bool kOneScene>
+__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
+shadow_persistent_kernel(const FrameArgs *__restrict__ args) {
+    const FrameArgs &a0 = args[0];
+    TraceStats st = {};
+    anyhit_queue<kCount, kRefill, kOneScene>(
+        args, a0.qcount[0], a0.qcount + 1, st,
+        [&](unsigned int i, unsigned int &view, V3 &o, V3 &d, float &tmax) {
+            const ShadowQuery q = a0.queue[i];
+            view = __float_as_uint(q.d.w) >> 24;
+            o = v3(q.o.x, q.o.y, q.o.z);
+            d = v3(q.d.x, q.d.y, q.d.z);
+            tmax = q.o.w;
+        },
+        [&](unsigned int i, unsigned int view, bool hit) {
+            if (!hit) return;
+            const FrameArgs &a = args[view];
+            const int pixel = int(__float_as_uint(a0.queue[i].d.w) & 0xffffffu);
+            a.flags[pixel] = 3u;
+            hdr_reservoir &rv = a.res[pixel];
+            rv.throughput[0] = rv.throughput[1] = rv.throughput[2] = rv.throughput[3] = 0.0f;
+        });
     if (kCount) {
         unsigned long long *c = a0.counters;
         wave_add(c + 5, st.nodes);
@@ -229,17 +250,15 @@ shadow_persistent_kernel(const FrameArgs *__restrict__ args) {
     }
 }
 
-// Shadow ray i of the queue (vertex `vtx`): an unoccluded winner's contribution joins the path
-// total (a spatial winner's colour is already staged); an occluded primary winner keeps zero
-// throughput in its reservoir (ris_direct_light_integrator.cpp:163-175).
-template <bool kCount, bool kLds>
-__device__ __forceinline__ void wave_shadow(const FrameArgs *__restrict__ args, unsigned int i, int vtx,
-                                            TraceStats &ts, int *ring_tr, int *ring_br) {
-    const ShadowQuery q = args[0].queue[i];
-    const uint32_t code = __float_as_uint(q.d.w);
+// Verdict of shadow ray i of the wavefront queue (vertex `vtx`): an unoccluded winner's contribution
+// joins the path total (a spatial winner's colour is already staged); an occluded primary winner
+// keeps zero throughput in its reservoir (ris_direct_light_integrator.cpp:163-175).
+__device__ __forceinline__ void wave_shadow_done(const FrameArgs *__restrict__ args, unsigned int i, int vtx,
+                                                 bool occluded) {
+    const uint32_t code = __float_as_uint(args[0].queue[i].d.w);
     const FrameArgs &a = args[code >> 24];
     const int pixel = int(code & 0xffffffu);
-    if (!any_hit<kCount, kLds>(a.s, v3(q.o.x, q.o.y, q.o.z), v3(q.d.x, q.d.y, q.d.z), q.o.w, ts, ring_tr, ring_br)) {
+    if (!occluded) {
         if (!(vtx == 0 && a.spatial_count > 0)) {
             float4 *t = a.wstate + 4 * wave_pixels(a) + pixel;
             const Spec tot = spec_of(*t) + spec_of(args[0].wcq[i]);
@@ -252,10 +271,8 @@ __device__ __forceinline__ void wave_shadow(const FrameArgs *__restrict__ args, 
     }
 }
 
-// The bounce rays (closest hit: the next vertex, appended to the live list when there is one) and
-// the shadow rays (any hit: an unoccluded winner's contribution joins the path total; an occluded
-// primary winner keeps zero throughput in its reservoir, ris_direct_light_integrator.cpp:163-175)
-// of vertex `bounce`.
+// The bounce rays of vertex `bounce` (closest hit: the next vertex, appended to the live list when
+// there is one); its shadow rays follow in wave_anyhit_kernel.
 template <bool kCount, int kOcc = 1, bool kLds = false>
 __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
 wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce) {
@@ -263,7 +280,7 @@ wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce) {
     const int lane = threadIdx.x;
     const int par = bounce & 1;
     unsigned int *cnt = args[0].wcnt;
-    const unsigned int nr = cnt[par], ns = cnt[2 + par];
+    const unsigned int nr = cnt[par];
     if (blockIdx.x == 0 && lane == 0) {
         cnt[par ^ 1] = 0u;        // rays of the next vertex
         cnt[2 + (par ^ 1)] = 0u;  // shadow queries of the next vertex
@@ -289,8 +306,6 @@ wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce) {
         const unsigned int ai = wave_append(cnt + 4 + (par ^ 1), hit, lane);
         if (hit) alive_list(args[0], bounce + 1)[ai] = code;
     }
-    for (unsigned int i = blockIdx.x * kLanes + unsigned(lane); i < ns; i += gridDim.x * kLanes)
-        wave_shadow<kCount, kLds>(args, i, bounce, ts, ring_tr + lane, ring_br + lane);
     if (kCount) {
         unsigned long long *c = args[0].counters;
         wave_add(c + 1, nb);
@@ -301,8 +316,8 @@ wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce) {
 }
 
 // Traced frames: traversal launch v (0 .. depth) after shading launch v -- vertex v's bounce rays
-// (closest hit: vertex v+1's hit record, or a miss record) and escape rays (any hit up to 1e30:
-// the candidate's bit of the pixel's escape mask when none), and vertex v-1's shadow rays.
+// (closest hit: vertex v+1's hit record, or a miss record); vertex v's escape rays and vertex v-1's
+// shadow rays follow in wave_anyhit_kernel.
 template <bool kCount, int kOcc = 1, bool kLds = false>
 __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
 wave_trace_traced_kernel(const FrameArgs *__restrict__ args, int v) {
@@ -310,7 +325,7 @@ wave_trace_traced_kernel(const FrameArgs *__restrict__ args, int v) {
     const int lane = threadIdx.x;
     const int par = v & 1;
     unsigned int *cnt = args[0].wcnt;
-    const unsigned int nr = cnt[par], ne = cnt[6 + par], ns = v > 0 ? cnt[2 + (par ^ 1)] : 0u;
+    const unsigned int nr = cnt[par];
     if (blockIdx.x == 0 && lane == 0) {  // the counts the next shading launch appends to
         cnt[par ^ 1] = 0u;        // bounce rays of vertex v+1
         cnt[6 + (par ^ 1)] = 0u;  // escape rays of vertex v+1
@@ -334,23 +349,64 @@ wave_trace_traced_kernel(const FrameArgs *__restrict__ args, int v) {
             store_grec_miss(dst);
         ++nb;
     }
-    for (unsigned int i = blockIdx.x * kLanes + unsigned(lane); i < ne; i += gridDim.x * kLanes) {
-        const float4 o4 = args[0].wesq[2 * i], d4 = args[0].wesq[2 * i + 1];
-        const uint32_t code = __float_as_uint(o4.w);
-        const FrameArgs &a = args[code >> 24];
-        const int pixel = int(code & 0xffffffu);
-        if (!any_hit<kCount, kLds>(a.s, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), 1e30f, ts, ring_tr + lane,
-                                   ring_br + lane))
-            atomicOr(a.wesc + pixel, 1u << __float_as_int(d4.w));
-    }
-    for (unsigned int i = blockIdx.x * kLanes + unsigned(lane); i < ns; i += gridDim.x * kLanes)
-        wave_shadow<kCount, kLds>(args, i, v - 1, ts, ring_tr + lane, ring_br + lane);
     if (kCount) {
         unsigned long long *c = args[0].counters;
         wave_add(c + 1, nb);
         wave_add(c + 5, ts.nodes);
         wave_add(c + 6, ts.tris);
         wave_add(c + 7, ts.instances);
+    }
+}
+
+// The any-hit rays of wavefront traversal launch v (bounce parameter v), on persistent lanes: on
+// traced frames vertex v's escape rays (any hit up to 1e30: the candidate's bit of the pixel's escape
+// mask when none) and vertex v-1's shadow rays, otherwise vertex v's shadow rays. The queues' claim
+// counters are wcnt[8 + parity] (escape rays) and wcnt[10 + parity] (shadow rays); this launch
+// clears the other parity's for the next one (the frame's first pair is cleared with the counts).
+template <bool kCount, bool kTraced, bool kOneScene>
+__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(4)))
+wave_anyhit_kernel(const FrameArgs *__restrict__ args, int v) {
+    const FrameArgs &a0 = args[0];
+    const int par = v & 1;
+    unsigned int *cnt = a0.wcnt;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cnt[8 + (par ^ 1)] = 0u;
+        cnt[10 + (par ^ 1)] = 0u;
+    }
+    TraceStats st = {};
+    if (kTraced) {
+        anyhit_queue<kCount, 16, kOneScene>(
+            args, cnt[6 + par], cnt + 8 + par, st,
+            [&](unsigned int i, unsigned int &view, V3 &o, V3 &d, float &tmax) {
+                const float4 o4 = a0.wesq[2 * i], d4 = a0.wesq[2 * i + 1];
+                view = __float_as_uint(o4.w) >> 24;
+                o = v3(o4.x, o4.y, o4.z);
+                d = v3(d4.x, d4.y, d4.z);
+                tmax = 1e30f;
+            },
+            [&](unsigned int i, unsigned int view, bool hit) {
+                if (hit) return;
+                const uint32_t code = __float_as_uint(a0.wesq[2 * i].w);
+                atomicOr(args[view].wesc + int(code & 0xffffffu), 1u << __float_as_int(a0.wesq[2 * i + 1].w));
+            });
+    }
+    const int vtx = kTraced ? v - 1 : v;
+    const unsigned int ns = kTraced ? (v > 0 ? cnt[2 + (par ^ 1)] : 0u) : cnt[2 + par];
+    anyhit_queue<kCount, 16, kOneScene>(
+        args, ns, cnt + 10 + par, st,
+        [&](unsigned int i, unsigned int &view, V3 &o, V3 &d, float &tmax) {
+            const ShadowQuery q = a0.queue[i];
+            view = __float_as_uint(q.d.w) >> 24;
+            o = v3(q.o.x, q.o.y, q.o.z);
+            d = v3(q.d.x, q.d.y, q.d.z);
+            tmax = q.o.w;
+        },
+        [&](unsigned int i, unsigned int, bool hit) { wave_shadow_done(args, i, vtx, hit); });
+    if (kCount) {
+        unsigned long long *c = a0.counters;
+        wave_add(c + 5, st.nodes);
+        wave_add(c + 6, st.tris);
+        wave_add(c + 7, st.instances);
     }
 }
 
@@ -392,6 +448,14 @@ template __global__ void shadow_persistent_kernel<true, 4, 16, true>(const Frame
 template __global__ void shadow_persistent_kernel<true, 4, 16, false>(const FrameArgs *__restrict__);
 template __global__ void shadow_persistent_kernel<false, 4, 16, true>(const FrameArgs *__restrict__);
 template __global__ void shadow_persistent_kernel<false, 4, 16, false>(const FrameArgs *__restrict__);
+template __global__ void wave_anyhit_kernel<true, true, true>(const FrameArgs *__restrict__, int);
+template __global__ void wave_anyhit_kernel<true, true, false>(const FrameArgs *__restrict__, int);
+template __global__ void wave_anyhit_kernel<true, false, true>(const FrameArgs *__restrict__, int);
+template __global__ void wave_anyhit_kernel<true, false, false>(const FrameArgs *__restrict__, int);
+template __global__ void wave_anyhit_kernel<false, true, true>(const FrameArgs *__restrict__, int);
+template __global__ void wave_anyhit_kernel<false, true, false>(const FrameArgs *__restrict__, int);
+template __global__ void wave_anyhit_kernel<false, false, true>(const FrameArgs *__restrict__, int);
+template __global__ void wave_anyhit_kernel<false, false, false>(const FrameArgs *__restrict__, int);
 template __global__ void wave_trace_kernel<true, 1, true>(const FrameArgs *__restrict__, int);
 template __global__ void wave_trace_kernel<true, 1, false>(const FrameArgs *__restrict__, int);
 template __global__ void wave_trace_kernel<true, 4, true>(const FrameArgs *__restrict__, int);

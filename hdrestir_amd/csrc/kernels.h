@@ -20,6 +20,8 @@ template <bool kCount, int kOcc, bool kLds>
 __global__ void wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce);
 template <bool kCount, int kOcc, bool kLds>
 __global__ void wave_trace_traced_kernel(const FrameArgs *__restrict__ args, int v);
+template <bool kCount, bool kTraced, bool kOneScene>
+__global__ void wave_anyhit_kernel(const FrameArgs *__restrict__ args, int v);
 __global__ void intersect_kernel(DeviceScene s, const float *o, const float *d, int n, int *inst, int *tri, float *t);
 
 // k_mega.hip
