@@ -275,10 +275,10 @@ static void launch_wave_trace_l(const LaunchShape &ls, int blocks, hipStream_t s
     }
 #undef HDR_WT
 }
-// Traversal launch b: the bounce rays (traced frames: wave_trace_traced_kernel), then the launch's
-// any-hit rays -- escape rays on traced frames, shadow rays -- on persistent lanes
-// (wave_anyhit_kernel, 4 waves per SIMD of the device). one_scene: every view of the launch renders
-// a[0]'s scene.
+// Traversal launch b: bounce and shadow rays one per lane (wave_trace_kernel); on traced frames the
+// bounce rays (wave_trace_traced_kernel), then the escape and shadow rays on persistent lanes
+// (wave_anyhit_kernel, 4 waves per SIMD of the device, or queue_waves). one_scene: every view of
+// the launch renders a[0]'s scene.
 template <bool C>
 static void launch_wave_trace(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a, int b,
                               bool traced, int cus, bool one_scene) {
@@ -289,16 +289,11 @@ static void launch_wave_trace(const LaunchShape &ls, int blocks, hipStream_t st,
         if (ls.lds_stack) launch_wave_trace_l<C, true, false>(ls, blocks, st, a, b);
         else launch_wave_trace_l<C, false, false>(ls, blocks, st, a, b);
     }
-    const dim3 g(unsigned(ls.queue_waves > 0 ? ls.queue_waves : cus * 4 * 4)), l(kLanes);
-#define HDR_AH(T, O) hipLaunchKernelGGL((wave_anyhit_kernel<C, T, O>), g, l, 0, st, a, b)
     if (traced) {
-        if (one_scene) HDR_AH(true, true);
-        else HDR_AH(true, false);
-    } else {
-        if (one_scene) HDR_AH(false, true);
-        else HDR_AH(false, false);
+        const dim3 g(unsigned(ls.queue_waves > 0 ? ls.queue_waves : cus * 4 * 4)), l(kLanes);
+        if (one_scene) hipLaunchKernelGGL((wave_anyhit_kernel<C, true, true>), g, l, 0, st, a, b);
+        else hipLaunchKernelGGL((wave_anyhit_kernel<C, true, false>), g, l, 0, st, a, b);
     }
-#undef HDR_AH
 }
 // Shading launch b >= 1; traced frames: wave_shade_traced_kernel.
 template <bool C, bool D>

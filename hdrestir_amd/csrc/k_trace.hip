@@ -271,8 +271,8 @@ __device__ __forceinline__ void wave_shadow_done(const FrameArgs *__restrict__ a
     }
 }
 
-// The bounce rays of vertex `bounce` (closest hit: the next vertex, appended to the live list when
-// there is one); its shadow rays follow in wave_anyhit_kernel.
+// The bounce rays (closest hit: the next vertex, appended to the live list when there is one) and
+// the shadow rays (any hit, one per lane: wave_shadow_done) of vertex `bounce`.
 template <bool kCount, int kOcc = 1, bool kLds = false>
 __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
 wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce) {
@@ -280,7 +280,7 @@ wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce) {
     const int lane = threadIdx.x;
     const int par = bounce & 1;
     unsigned int *cnt = args[0].wcnt;
-    const unsigned int nr = cnt[par];
+    const unsigned int nr = cnt[par], ns = cnt[2 + par];
     if (blockIdx.x == 0 && lane == 0) {
         cnt[par ^ 1] = 0u;        // rays of the next vertex
         cnt[2 + (par ^ 1)] = 0u;  // shadow queries of the next vertex
@@ -305,6 +305,13 @@ wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce) {
         }
         const unsigned int ai = wave_append(cnt + 4 + (par ^ 1), hit, lane);
         if (hit) alive_list(args[0], bounce + 1)[ai] = code;
+    }
+    for (unsigned int i = blockIdx.x * kLanes + unsigned(lane); i < ns; i += gridDim.x * kLanes) {
+        const ShadowQuery q = args[0].queue[i];
+        const FrameArgs &a = args[__float_as_uint(q.d.w) >> 24];
+        wave_shadow_done(args, i, bounce,
+                         any_hit<kCount, kLds>(a.s, v3(q.o.x, q.o.y, q.o.z), v3(q.d.x, q.d.y, q.d.z), q.o.w, ts,
+                                               ring_tr + lane, ring_br + lane));
     }
     if (kCount) {
         unsigned long long *c = args[0].counters;
@@ -358,11 +365,12 @@ wave_trace_traced_kernel(const FrameArgs *__restrict__ args, int v) {
     }
 }
 
-// The any-hit rays of wavefront traversal launch v (bounce parameter v), on persistent lanes: on
-// traced frames vertex v's escape rays (any hit up to 1e30: the candidate's bit of the pixel's escape
-// mask when none) and vertex v-1's shadow rays, otherwise vertex v's shadow rays. The queues' claim
-// counters are wcnt[8 + parity] (escape rays) and wcnt[10 + parity] (shadow rays); this launch
-// clears the other parity's for the next one (the frame's first pair is cleared with the counts).
+// The any-hit rays of traced wavefront traversal launch v, on persistent lanes: vertex v's escape
+// rays (any hit up to 1e30: the candidate's bit of the pixel's escape mask when none) and vertex
+// v-1's shadow rays. The queues' claim counters are wcnt[8 + parity] (escape rays) and
+// wcnt[10 + parity] (shadow rays); this launch clears the other parity's for the next one (the
+// frame's first pair is cleared with the counts). (Untraced frames trace their shadow rays in
+// wave_trace_kernel, one per lane: persistent lanes measured slower on those shorter queues.)
 template <bool kCount, bool kTraced, bool kOneScene>
 __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(4)))
 wave_anyhit_kernel(const FrameArgs *__restrict__ args, int v) {
@@ -450,12 +458,8 @@ template __global__ void shadow_persistent_kernel<false, 4, 16, true>(const Fram
 template __global__ void shadow_persistent_kernel<false, 4, 16, false>(const FrameArgs *__restrict__);
 template __global__ void wave_anyhit_kernel<true, true, true>(const FrameArgs *__restrict__, int);
 template __global__ void wave_anyhit_kernel<true, true, false>(const FrameArgs *__restrict__, int);
-template __global__ void wave_anyhit_kernel<true, false, true>(const FrameArgs *__restrict__, int);
-template __global__ void wave_anyhit_kernel<true, false, false>(const FrameArgs *__restrict__, int);
 template __global__ void wave_anyhit_kernel<false, true, true>(const FrameArgs *__restrict__, int);
 template __global__ void wave_anyhit_kernel<false, true, false>(const FrameArgs *__restrict__, int);
-template __global__ void wave_anyhit_kernel<false, false, true>(const FrameArgs *__restrict__, int);
-template __global__ void wave_anyhit_kernel<false, false, false>(const FrameArgs *__restrict__, int);
 template __global__ void wave_trace_kernel<true, 1, true>(const FrameArgs *__restrict__, int);
 template __global__ void wave_trace_kernel<true, 1, false>(const FrameArgs *__restrict__, int);
 template __global__ void wave_trace_kernel<true, 4, true>(const FrameArgs *__restrict__, int);
