@@ -42,7 +42,7 @@ RGB2_BYTES = 32    # deferred path: visible + occluded colour written by shading
 FLAG_BYTES = 4     # deferred path: per pixel visibility flag
 QUERY_BYTES = 32   # deferred path: one shadow query (origin, tmax, direction, pixel)
 AUX_BYTES = 32     # spatial reuse: per pixel {ray direction, wavelength u}, {throughput}
-KERNEL_NAMES = {"gbuffer": "gbuffer_kernel", "integrate": "shade_direct_kernel", "shadow": "shadow_kernel",
+KERNEL_NAMES = {"gbuffer": "gbuffer_kernel", "integrate": "shade_direct_kernel", "shadow": "shadow_persistent_kernel",
                 "accumulate": "accumulate_deferred_kernel", "firefly": "firefly_accumulate_kernel",
                 "spatial": "spatial_kernel"}
 
