@@ -190,6 +190,12 @@ class RequestResult(C.Structure):
                 ("status", C.c_int32), ("queued_ms", C.c_double), ("latency_ms", C.c_double)]
 
 
+class VarianceStats(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("sample_count", C.c_int32), ("valid", C.c_int32), ("_pad", C.c_int32),
+                ("mean_estimator_variance", C.c_double), ("max_estimator_variance", C.c_double),
+                ("mean_luminance", C.c_double)]
+
+
 class ServerStats(C.Structure):
     _fields_ = [("queued", C.c_int32), ("active", C.c_int32), ("finished", C.c_int32),
                 ("pooled_views", C.c_int32), ("hbm_in_use", C.c_uint64), ("steps", C.c_uint64)]

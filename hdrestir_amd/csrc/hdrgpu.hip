@@ -85,8 +85,8 @@ __global__ void wave_zero_counts_kernel(unsigned int *c) {
 
 // Out-of-place 3x3 firefly clamp (the reference's GPU accumulation shader semantics,
 // accumulation.slang:55-81; threshold 4 * max neighbour luminance + 0.5) + accumulation.
-__global__ void firefly_accumulate_kernel(const float4 *frame, float4 *accum, float4 *color, int W, int H,
-                                          float invN) {
+__global__ void firefly_accumulate_kernel(const float4 *frame, float4 *accum, float4 *color, double *lum_sum,
+                                          double *lum_sq, int W, int H, float invN) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= W || y >= H) return;
@@ -108,13 +108,53 @@ __global__ void firefly_accumulate_kernel(const float4 *frame, float4 *accum, fl
         c.y *= s;
         c.z *= s;
     }
-    float4 acc = accum[i];
-    acc.x += c.x;
-    acc.y += c.y;
-    acc.z += c.z;
-    acc.w += c.w;
-    accum[i] = acc;
-    color[i] = make_float4(acc.x * invN, acc.y * invN, acc.z * invN, acc.w * invN);
+    accumulate_sample(accum, color, lum_sum, lum_sq, i, c, invN);
+}
+
+// The Variance metrics of AccumulationPass (accumulation_pass.cpp:108-136), first pass: per block
+// of 256 pixels, the sum and maximum of the per-pixel estimator variance and the sum of the
+// per-pixel mean luminance (n = accumulated samples >= 2), reduced in a fixed order.
+__global__ void variance_partial_kernel(const double *lum_sum, const double *lum_sq, int P, int n, double *partial) {
+    __shared__ double ssum[256], smax[256], slum[256];
+    const int t = threadIdx.x, i = blockIdx.x * 256 + t;
+    double ev = 0.0, mean = 0.0;
+    if (i < P) {
+        const double nn = double(n);
+        mean = lum_sum[i] / nn;
+        const double centered = lum_sq[i] - nn * mean * mean;
+        const double sampleVariance = fmax(0.0, centered / (nn - 1.0));
+        ev = sampleVariance / nn;
+    }
+    ssum[t] = ev;
+    smax[t] = ev;
+    slum[t] = mean;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) {
+            ssum[t] += ssum[t + h];
+            smax[t] = fmax(smax[t], smax[t + h]);
+            slum[t] += slum[t + h];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        partial[3 * blockIdx.x] = ssum[0];
+        partial[3 * blockIdx.x + 1] = smax[0];
+        partial[3 * blockIdx.x + 2] = slum[0];
+    }
+}
+// Second pass: the blocks' partials in order; out = {mean, max estimator variance, mean luminance}.
+__global__ void variance_final_kernel(const double *partial, int blocks, int P, double *out) {
+    if (threadIdx.x != 0) return;
+    double sum = 0.0, mx = 0.0, lum = 0.0;
+    for (int b = 0; b < blocks; ++b) {
+        sum += partial[3 * b];
+        mx = fmax(mx, partial[3 * b + 1]);
+        lum += partial[3 * b + 2];
+    }
+    out[0] = sum / double(P);
+    out[1] = mx;
+    out[2] = lum / double(P);
 }
 
 // One render-buffer write of a resolved output: UpscalePass's nearest texel (upscale_pass.cpp:17-31,
@@ -203,12 +243,18 @@ hdr_status fail(hdr_status s, const std::string &msg) {
 struct DevBuf {
     void *p = nullptr;
     size_t n = 0;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
     hipError_t alloc(size_t bytes) {
         n = bytes;
         return hipMalloc(&p, bytes ? bytes : 1);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
     }
 };
 
@@ -381,6 +427,7 @@ struct hdr_view {
     int frames = 0;
     int32_t samples = 0;  // accumulated samples: frame_index + 1 of the last frame issued, 0 after a reset
     DevBuf resolved;      // staging of hdr_view_resolve
+    DevBuf lum_sum, lum_sq, variance;  // Variance metrics (hdr_view_set_metrics): per-pixel sums, reduction
     // Frames overlap: the G-buffer is double-buffered by frame parity, and the G-buffer kernel of
     // frame N+1 runs on `gstream` as soon as frame N-1's shading has read that buffer (event
     // `shade_done[parity]`), beside frame N's shading, shadow rays and accumulation on `stream`;
@@ -785,6 +832,10 @@ hdr_status hdr_view_reset(hdr_view *view) {
     HIP_TRY(hipMemsetAsync(view->res.p, 0, view->res.n, view->stream));
     HIP_TRY(hipMemsetAsync(view->accum.p, 0, view->accum.n, view->stream));
     HIP_TRY(hipMemsetAsync(view->status.p, 0, view->status.n, view->stream));
+    if (view->lum_sum.p) {
+        HIP_TRY(hipMemsetAsync(view->lum_sum.p, 0, view->lum_sum.n, view->stream));
+        HIP_TRY(hipMemsetAsync(view->lum_sq.p, 0, view->lum_sq.n, view->stream));
+    }
     view->frames = 0;
     view->samples = 0;
     return HDR_OK;
@@ -792,7 +843,7 @@ hdr_status hdr_view_reset(hdr_view *view) {
 
 hdr_status hdr_view_device_bytes(const hdr_view *v, uint64_t *out) {
     if (!v || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
-    *out = v->res.n + v->accum.n + v->color.n + v->frame.n + v->dbg.n + v->depth.n + v->albedo.n + v->normal.n +
+    *out = v->lum_sum.n + v->lum_sq.n + v->variance.n + v->resolved.n + v->res.n + v->accum.n + v->color.n + v->frame.n + v->dbg.n + v->depth.n + v->albedo.n + v->normal.n +
            v->counters.n + v->spill.n + v->gbuf.n + v->rgb2.n + v->flags.n + v->queue.n + v->qcount.n + v->res_t.n +
            v->saux.n + v->wstate.n + v->whit.n + v->wray.n + v->wcq.n + v->walive.n + v->wcnt.n + v->wesc.n +
            v->wesq.n;
@@ -1011,6 +1062,8 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
     a.status = static_cast<unsigned int *>(view->status.p);
     a.accum = static_cast<float4 *>(view->accum.p);
     a.color = static_cast<float4 *>(view->color.p);
+    a.lum_sum = static_cast<double *>(view->lum_sum.p);  // null unless the metrics are on
+    a.lum_sq = static_cast<double *>(view->lum_sq.p);
 
     // HDRGPU_TEST_WITHHOLD_SPILL=1: a test hook that withholds the spill slots a frame needs, to check
     // that the kernels report the overrun (their draws past the slot return 0; no memory is touched)
@@ -1300,7 +1353,8 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     if (a.firefly) {
         HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_FIREFLY], view->stream));
         const dim3 b2(16, 16), g2((W + 15) / 16, (H + 15) / 16);
-        hipLaunchKernelGGL(firefly_accumulate_kernel, g2, b2, 0, view->stream, a.frame_rgb, a.accum, a.color, W, H,
+        hipLaunchKernelGGL(firefly_accumulate_kernel, g2, b2, 0, view->stream, a.frame_rgb, a.accum, a.color,
+                           a.lum_sum, a.lum_sq, W, H,
                            a.inv_n);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_FIREFLY], view->stream));
@@ -1487,6 +1541,60 @@ hdr_status hdr_render_progressive(hdr_view *view, const hdr_scene *scene, const 
     }
     *converged = view->samples >= target_samples ? 1 : 0;
     return HDR_OK;
+}
+
+hdr_status hdr_view_set_metrics(hdr_view *view, int32_t enable) {
+    if (!view) return fail(HDR_ERR_INVALID_ARGUMENT, "view is null");
+    HIP_TRY(hipSetDevice(view->ctx->device));
+    HIP_TRY(hipStreamSynchronize(view->stream));
+    HIP_TRY(hipStreamSynchronize(view->gstream));
+    if (!enable) {
+        view->lum_sum.release();
+        view->lum_sq.release();
+        view->variance.release();
+        return HDR_OK;
+    }
+    if (view->lum_sum.p) return HDR_OK;
+    const size_t P = size_t(view->W) * view->H;
+    if (hdr_status s = ensure(view->lum_sum, P * sizeof(double))) return s;
+    if (hdr_status s = ensure(view->lum_sq, P * sizeof(double))) return s;
+    HIP_TRY(hipMemsetAsync(view->lum_sum.p, 0, view->lum_sum.n, view->stream));
+    HIP_TRY(hipMemsetAsync(view->lum_sq.p, 0, view->lum_sq.n, view->stream));
+    return HDR_OK;
+}
+
+hdr_status hdr_view_read_variance(hdr_view *view, hdr_variance_stats *out, double *lum_sum, double *lum_sum_sq) {
+    if (!view || !out) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    if (out->struct_size != sizeof(hdr_variance_stats))
+        return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_variance_stats.struct_size mismatch");
+    if (!view->lum_sum.p) return fail(HDR_ERR_INVALID_ARGUMENT, "the view's metrics are off (hdr_view_set_metrics)");
+    HIP_TRY(hipSetDevice(view->ctx->device));
+    const int P = view->W * view->H;
+    out->sample_count = view->samples;
+    out->valid = view->samples >= 2 ? 1 : 0;
+    out->mean_estimator_variance = out->max_estimator_variance = out->mean_luminance = 0.0;
+    if (out->valid) {
+        const int blocks = (P + 255) / 256;
+        if (hdr_status s = ensure(view->variance, (size_t(blocks) * 3 + 3) * sizeof(double))) return s;
+        double *part = static_cast<double *>(view->variance.p);
+        hipLaunchKernelGGL(variance_partial_kernel, dim3(blocks), dim3(256), 0, view->stream,
+                           static_cast<const double *>(view->lum_sum.p), static_cast<const double *>(view->lum_sq.p),
+                           P, view->samples, part + 3);
+        hipLaunchKernelGGL(variance_final_kernel, dim3(1), dim3(64), 0, view->stream, part + 3, blocks, P, part);
+        HIP_TRY(hipGetLastError());
+        double r[3];
+        HIP_TRY(hipMemcpyAsync(r, part, sizeof r, hipMemcpyDeviceToHost, view->stream));
+        HIP_TRY(hipStreamSynchronize(view->stream));
+        out->mean_estimator_variance = r[0];
+        out->max_estimator_variance = r[1];
+        out->mean_luminance = r[2];
+    }
+    if (lum_sum)
+        HIP_TRY(hipMemcpyAsync(lum_sum, view->lum_sum.p, view->lum_sum.n, hipMemcpyDeviceToHost, view->stream));
+    if (lum_sum_sq)
+        HIP_TRY(hipMemcpyAsync(lum_sum_sq, view->lum_sq.p, view->lum_sq.n, hipMemcpyDeviceToHost, view->stream));
+    HIP_TRY(hipStreamSynchronize(view->stream));
+    return view_status(view);
 }
 
 hdr_status hdr_view_color_device_ptr(hdr_view *view, void **out) {

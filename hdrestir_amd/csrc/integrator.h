@@ -43,6 +43,7 @@ struct FrameArgs {
     float4 *accum;
     float4 *frame_rgb;                 // this frame's colour (firefly path / debugging)
     float4 *color;                     // accumulated colour
+    double *lum_sum, *lum_sq;          // Variance metrics: per-pixel luminance sums (null when off)
     hdr_pixel_debug *dbg;
     float *aov_depth, *aov_albedo, *aov_normal;
     uint64_t *spill;                   // 2 x 312 words per pixel, or null
@@ -221,6 +222,26 @@ struct PixelCtx {
 // stream was set up for (the lean stream past output 310, a stream without a spill slot past
 // 311 or 623); such draws return 0 instead of the engine's value, and the host reports the view as
 // failed (HDR_ERR_CAPACITY) at its next synchronisation.
+// AccumulationPass (accumulation_pass.cpp:64-106) for one pixel's frame sample c (after the firefly
+// clamp): accum += c, colour = accum / (frame_index + 1); with the Variance metrics on
+// (METRICS_ENABLED, :95-98) the sample's luminance -- in float, as GetLuminance -- and its square
+// are added to the pixel's sums in double.
+__device__ __forceinline__ void accumulate_sample(float4 *accum, float4 *color, double *lum_sum, double *lum_sq,
+                                                  int pixel, float4 c, float inv_n) {
+    if (lum_sum) {
+        const double l = double(0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z);
+        lum_sum[pixel] += l;
+        lum_sq[pixel] += l * l;
+    }
+    float4 acc = accum[pixel];
+    acc.x += c.x;
+    acc.y += c.y;
+    acc.z += c.z;
+    acc.w += c.w;
+    accum[pixel] = acc;
+    color[pixel] = make_float4(acc.x * inv_n, acc.y * inv_n, acc.z * inv_n, acc.w * inv_n);
+}
+
 constexpr unsigned int kStatusRngOverrun = 1u;
 __device__ __forceinline__ void flag_rng_overrun(const FrameArgs &a, bool overrun) {
     if (overrun && a.status) atomicOr(a.status, kStatusRngOverrun);

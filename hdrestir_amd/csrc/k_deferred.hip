@@ -173,13 +173,7 @@ __global__ void __launch_bounds__(kLanes) accumulate_deferred_kernel(const Frame
         a.frame_rgb[i] = c4;
         return;
     }
-    float4 acc = a.accum[i];
-    acc.x += c4.x;
-    acc.y += c4.y;
-    acc.z += c4.z;
-    acc.w += c4.w;
-    a.accum[i] = acc;
-    a.color[i] = make_float4(acc.x * a.inv_n, acc.y * a.inv_n, acc.z * a.inv_n, acc.w * a.inv_n);
+    accumulate_sample(a.accum, a.color, a.lum_sum, a.lum_sq, i, c4, a.inv_n);
 }
 
 // Spatial reuse of the primary vertex (deferred.h: spatial_reuse_pixel), after wave_shade0 has
@@ -251,13 +245,7 @@ __global__ void __launch_bounds__(kLanes) wave_resolve_kernel(const FrameArgs *_
         if (a.firefly) {
             a.frame_rgb[pixel] = c4;
         } else {
-            float4 acc = a.accum[pixel];
-            acc.x += c4.x;
-            acc.y += c4.y;
-            acc.z += c4.z;
-            acc.w += c4.w;
-            a.accum[pixel] = acc;
-            a.color[pixel] = make_float4(acc.x * a.inv_n, acc.y * a.inv_n, acc.z * a.inv_n, acc.w * a.inv_n);
+            accumulate_sample(a.accum, a.color, a.lum_sum, a.lum_sq, pixel, c4, a.inv_n);
         }
         draws = __float_as_uint(r2.z);
         flag_rng_overrun(a, !wave_slot(a, pixel) && draws > 624u);  // outputs from 624 on need the slot

@@ -173,13 +173,7 @@ __device__ __forceinline__ void pt_lean_pixel(const FrameArgs &a, int x, int y, 
     if (a.firefly) {
         a.frame_rgb[pixel] = c4;
     } else {  // AccumulationPass (accumulation_pass.cpp:98-106)
-        float4 acc = a.accum[pixel];
-        acc.x += c4.x;
-        acc.y += c4.y;
-        acc.z += c4.z;
-        acc.w += c4.w;
-        a.accum[pixel] = acc;
-        a.color[pixel] = make_float4(acc.x * a.inv_n, acc.y * a.inv_n, acc.z * a.inv_n, acc.w * a.inv_n);
+        accumulate_sample(a.accum, a.color, a.lum_sum, a.lum_sq, pixel, c4, a.inv_n);
     }
     if (kDebug) {
         hdr_pixel_debug d;

@@ -102,6 +102,8 @@ def lib():
             "hdr_view_read_aovs": [P, P, P, P],
             "hdr_view_color_device_ptr": [P, C.POINTER(P)],
             "hdr_rendered_size": [I, I, I, C.POINTER(I), C.POINTER(I)],
+            "hdr_view_set_metrics": [P, I],
+            "hdr_view_read_variance": [P, C.POINTER(abi.VarianceStats), P, P],
             "hdr_view_sample_count": [P, C.POINTER(I)],
             "hdr_view_resolve": [P, I, I, I, I, I, P],
             "hdr_render_progressive": [P, P, C.POINTER(abi.FrameDesc), C.POINTER(abi.PipelineDesc), I, I,
@@ -312,6 +314,21 @@ class Renderer:
         out = np.zeros((window_height, window_width, components), self._FORMAT_DTYPES.get(fmt, np.float32))
         check(lib().hdr_view_resolve(self.h, aov, window_width, window_height, fmt, components, _p(out)))
         return out
+
+    def set_metrics(self, on):
+        """hdr_view_set_metrics: per-pixel luminance sums of the accumulated samples (Variance)."""
+        check(lib().hdr_view_set_metrics(self.h, int(on)))
+
+    def variance(self):
+        """hdr_view_read_variance: ({sample_count, valid, mean_estimator_variance,
+        max_estimator_variance, mean_luminance}, per-pixel luminance sums, per-pixel sums of squares)."""
+        st = abi.VarianceStats()
+        st.struct_size = C.sizeof(abi.VarianceStats)
+        ls = np.zeros(self.H * self.W, np.float64)
+        lq = np.zeros(self.H * self.W, np.float64)
+        check(lib().hdr_view_read_variance(self.h, C.byref(st), _p(ls), _p(lq)))
+        stats = {k: getattr(st, k) for k, _ in abi.VarianceStats._fields_ if k not in ("struct_size", "_pad")}
+        return stats, ls, lq
 
     def sample_count(self):
         v = C.c_int32()

@@ -319,6 +319,27 @@ hdr_status hdr_view_read_aovs(hdr_view *view, float *depth, float *albedo, float
 /* Device pointer of the accumulated colour (RGBA32F) for zero-copy consumers. */
 hdr_status hdr_view_color_device_ptr(hdr_view *view, void **out);
 
+/* AccumulationPass's Variance metrics (reference: accumulation_pass.cpp:84-137, compiled there with
+ * METRICS_ENABLED). With the metrics on, every accumulated frame also adds its sample's luminance
+ * (0.2126 R + 0.7152 G + 0.0722 B in float, after the firefly clamp) and its square to per-pixel
+ * sums in double (16 B per pixel; cleared by hdr_view_reset). Turning them on clears the sums;
+ * turning them off frees them. */
+hdr_status hdr_view_set_metrics(hdr_view *view, int32_t enable);
+typedef struct hdr_variance_stats {
+    uint32_t struct_size;            /* sizeof(hdr_variance_stats) */
+    int32_t  sample_count;           /* frame_index + 1 of the view's last frame */
+    int32_t  valid;                  /* 0 below 2 samples (the reference computes nothing there) */
+    int32_t  _pad;
+    double   mean_estimator_variance; /* mean over pixels of max(0, (S2 - n mean^2) / (n - 1)) / n */
+    double   max_estimator_variance;
+    double   mean_luminance;          /* mean over pixels of S / n */
+} hdr_variance_stats;
+/* The statistics the reference logs and stores as its Variance output (VarianceStats {mean, max}),
+ * over the whole view, reduced on the device in a fixed order; lum_sum / lum_sum_sq (optional, host,
+ * width*height doubles) receive the per-pixel sums. HDR_ERR_INVALID_ARGUMENT when the metrics are
+ * off. Waits for the view's frames. */
+hdr_status hdr_view_read_variance(hdr_view *view, hdr_variance_stats *out, double *lum_sum, double *lum_sum_sq);
+
 /* ---- resolve, upscale and the progressive render loop ---- */
 
 /* Outputs of a view (reference: the AOV output specs Color / Depth / Albedo / Normal,

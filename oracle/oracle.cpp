@@ -2288,6 +2288,8 @@ struct oracle_ctx {
     std::vector<Reservoir> resT;            // spatial reuse: post-temporal reservoirs of this frame
     std::vector<SpatialAux> aux;
     std::vector<float> accum;               // __accum_color (RGBA)
+    std::vector<double> lumSum, lumSumSq;   // __accum_lum_sum / __accum_lum_sum_sq (METRICS_ENABLED)
+    int sampleCount = 0;                    // frameIndex + 1 of the last frame
     bool accumValid = false;
     std::vector<std::optional<HitRecord>> gbuf;
     std::vector<float> depth, albedo, normal;
@@ -2308,6 +2310,8 @@ oracle_ctx *oracle_create(const hdr_scene_desc *scene, int32_t width, int32_t he
     c->resT.assign(n, Reservoir{});
     c->aux.assign(n, SpatialAux{});
     c->accum.assign(n * 4, 0.0f);
+    c->lumSum.assign(n, 0.0);
+    c->lumSumSq.assign(n, 0.0);
     c->gbuf.assign(n, std::nullopt);
     c->depth.assign(n, 1.0f);
     c->albedo.assign(n * 3, 0.0f);
@@ -2320,6 +2324,9 @@ void oracle_destroy(oracle_ctx *c) { delete c; }
 void oracle_reset(oracle_ctx *c) {
     std::fill(c->reservoirs.begin(), c->reservoirs.end(), Reservoir{});
     std::fill(c->accum.begin(), c->accum.end(), 0.0f);
+    std::fill(c->lumSum.begin(), c->lumSum.end(), 0.0);
+    std::fill(c->lumSumSq.begin(), c->lumSumSq.end(), 0.0);
+    c->sampleCount = 0;
 }
 
 void oracle_set_firefly_in_place(oracle_ctx *c, int32_t v) { c->fireflyInPlace = v != 0; }
@@ -2612,13 +2619,50 @@ double oracle_render_frame(oracle_ctx *c, const hdr_frame_desc *frame, const hdr
     for (int y = y0; y < y1; ++y)
         for (int x = x0; x < x1; ++x) {
             const size_t i = size_t(y) * W + x;
+            // the Variance metrics (METRICS_ENABLED, accumulation_pass.cpp:95-98): the luminance of the
+            // (clamped) frame sample, accumulated in double
+            const double luminance = double(lum(&fb[4 * i]));
+            c->lumSum[i] += luminance;
+            c->lumSumSq[i] += luminance * luminance;
             for (int k = 0; k < 4; ++k) {
                 c->accum[4 * i + k] += fb[4 * i + k];
                 if (color_out) color_out[4 * i + k] = c->accum[4 * i + k] * invN;
             }
         }
+    c->sampleCount = frameIndex + 1;
     c->last = cnt;
     return secs;
+}
+
+// AccumulationPass's Variance metrics (accumulation_pass.cpp:108-136) over the pixel rectangle
+// [x0,x1) x [y0,y1): out = {mean estimator variance, max estimator variance, mean luminance}; returns
+// 0 (nothing computed) below 2 samples, as the reference logs nothing there. lum_sum / lum_sum_sq
+// (optional, width*height doubles) receive the per-pixel sums.
+int32_t oracle_read_variance(oracle_ctx *c, int32_t x0, int32_t y0, int32_t x1, int32_t y1, double *out,
+                             double *lum_sum, double *lum_sum_sq) {
+    if (lum_sum) std::copy(c->lumSum.begin(), c->lumSum.end(), lum_sum);
+    if (lum_sum_sq) std::copy(c->lumSumSq.begin(), c->lumSumSq.end(), lum_sum_sq);
+    const int sampleCount = c->sampleCount;
+    if (sampleCount < 2) return 0;
+    double meanEstimatorVariance = 0.0, maxEstimatorVariance = 0.0, meanLuminance = 0.0;
+    size_t count = 0;
+    for (int y = y0; y < y1; ++y)
+        for (int x = x0; x < x1; ++x) {
+            const size_t i = size_t(y) * c->W + x;
+            const double n = double(sampleCount);
+            const double mean = c->lumSum[i] / n;
+            const double centeredSumSquares = c->lumSumSq[i] - n * mean * mean;
+            const double sampleVariance = std::max(0.0, centeredSumSquares / (n - 1.0));
+            const double estimatorVariance = sampleVariance / n;
+            meanEstimatorVariance += estimatorVariance;
+            maxEstimatorVariance = std::max(maxEstimatorVariance, estimatorVariance);
+            meanLuminance += mean;
+            ++count;
+        }
+    out[0] = meanEstimatorVariance / double(count);
+    out[1] = maxEstimatorVariance;
+    out[2] = meanLuminance / double(count);
+    return 1;
 }
 
 void oracle_read_reservoirs(oracle_ctx *c, hdr_reservoir *out) {

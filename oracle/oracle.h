@@ -42,6 +42,11 @@ double oracle_render_frame(oracle_ctx *ctx, const hdr_frame_desc *frame, const h
                            hdr_pixel_debug *debug_out);
 
 void oracle_read_reservoirs(oracle_ctx *ctx, hdr_reservoir *out);
+/* AccumulationPass's Variance metrics over a pixel rectangle: out[3] = mean and max estimator
+ * variance, mean luminance; 0 below 2 samples. lum_sum / lum_sum_sq (optional) get the per-pixel
+ * luminance sums of the samples and of their squares. */
+int32_t oracle_read_variance(oracle_ctx *ctx, int32_t x0, int32_t y0, int32_t x1, int32_t y1, double *out,
+                             double *lum_sum, double *lum_sum_sq);
 void oracle_read_aovs(oracle_ctx *ctx, float *depth, float *albedo, float *normal);
 void oracle_read_counters(oracle_ctx *ctx, hdr_frame_counters *out);
 

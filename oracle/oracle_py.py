@@ -36,6 +36,8 @@ def lib():
         L.oracle_render_frame.argtypes = [P, C.POINTER(abi.FrameDesc), C.POINTER(abi.PipelineDesc), C.c_int32,
                                           C.c_int32, C.c_int32, C.c_int32, P, P]
         L.oracle_read_reservoirs.argtypes = [P, P]
+        L.oracle_read_variance.restype = C.c_int32
+        L.oracle_read_variance.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P, P]
         L.oracle_read_aovs.argtypes = [P, P, P, P]
         L.oracle_read_counters.argtypes = [P, C.POINTER(abi.FrameCounters)]
         L.oracle_intersect.argtypes = [P, P, P, C.c_int32, P, P, P]
@@ -89,6 +91,18 @@ class Oracle:
         out = np.zeros(self.H * self.W, abi.reservoir_dtype())
         lib().oracle_read_reservoirs(self.h, _p(out))
         return out
+
+    def variance(self, rect=None):
+        """AccumulationPass's Variance metrics over `rect` (default: the whole image): (valid,
+        {mean_estimator_variance, max_estimator_variance, mean_luminance}, per-pixel luminance sums,
+        per-pixel sums of squares)."""
+        x0, y0, x1, y1 = rect if rect else (0, 0, self.W, self.H)
+        out = np.zeros(3, np.float64)
+        ls = np.zeros(self.H * self.W, np.float64)
+        lq = np.zeros(self.H * self.W, np.float64)
+        ok = lib().oracle_read_variance(self.h, x0, y0, x1, y1, _p(out), _p(ls), _p(lq))
+        stats = dict(mean_estimator_variance=out[0], max_estimator_variance=out[1], mean_luminance=out[2])
+        return bool(ok), stats, ls, lq
 
     def aovs(self):
         d = np.zeros(self.H * self.W, np.float32)

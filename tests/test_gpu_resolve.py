@@ -126,3 +126,51 @@ def test_progressive_loop_converges_and_cancels(gpu_ctx):
     r.close()
     ref.close()
     gs.close()
+
+
+@pytest.mark.parametrize("case", ["pt", "pt_firefly", "restir_deferred", "wavefront_d3"])
+def test_variance_metrics_match_the_oracle(gpu_ctx, oracle_mod, case):
+    """AccumulationPass's Variance metrics on every accumulation path (the megakernel's or lean path
+    tracer's, the firefly kernel's, the deferred path's and the wavefront's accumulation): per-pixel
+    luminance sums within the colour bar (1e-3 of the oracle's value, floor 1e-3) on every pixel,
+    the statistics within 1e-3; no statistics after one frame; a reset clears the sums; metrics off
+    refuses the read."""
+    from hdrestir_amd import gpu
+    import parity
+    W, H = 48, 32
+    sc = scenes.box_spheres(W, H, frames=6)
+    pipe = {"pt": abi.pipeline("mis", max_depth=3),
+            "pt_firefly": abi.pipeline("mis", max_depth=3, firefly=True),
+            "restir_deferred": abi.pipeline("ris", candidate_count=16, use_reservoir=True, skip_visibility=True,
+                                            max_depth=1, spatial_count=3, spatial_radius=8.0),
+            "wavefront_d3": abi.pipeline("ris", candidate_count=16, use_reservoir=True, skip_visibility=True,
+                                         max_depth=3)}[case]
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, W, H)
+    r.set_metrics(True)
+    o = oracle_mod.Oracle(sc, threads=8)
+    o.set_firefly_in_place(False)
+    for f in range(5):
+        r.render(sc.frame(f), pipe, read_color=False)
+        o.render(sc.frame(f), pipe)
+        if f == 0:
+            assert not r.variance()[0]["valid"] and not o.variance()[0]
+    st, ls, lq = r.variance()
+    ok, ost, ols, olq = o.variance()
+    assert ok and st["valid"] and st["sample_count"] == 5
+    rel = np.abs(ls - ols) / np.maximum(np.abs(ols), parity.REL_FLOOR)
+    print(case, "lum sum max rel", rel.max(), st, ost)
+    assert rel.max() < parity.MAX_REL
+    relq = np.abs(lq - olq) / np.maximum(np.abs(olq), parity.REL_FLOOR)
+    assert relq.max() < 2 * parity.MAX_REL
+    for k in ("mean_estimator_variance", "max_estimator_variance", "mean_luminance"):
+        assert abs(st[k] - ost[k]) <= 1e-3 * abs(ost[k]) + 1e-9, (k, st[k], ost[k])
+    assert st["mean_estimator_variance"] > 0.0
+    r.reset()
+    st0, ls0, _ = r.variance()
+    assert st0["sample_count"] == 0 and not st0["valid"] and not ls0.any()
+    r.set_metrics(False)
+    with pytest.raises(gpu.HdrError, match="INVALID_ARGUMENT"):
+        r.variance()
+    r.close()
+    gs.close()

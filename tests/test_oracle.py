@@ -445,3 +445,40 @@ def test_spatial_reuse_rejects_unsupported_pipelines(oracle_mod):
     pipe = abi.pipeline("ris", candidate_count=8, use_reservoir=False, max_depth=1, spatial_count=3)
     with pytest.raises(ValueError):
         o.render(sc.frame(0), pipe)
+
+
+def test_variance_metrics_from_the_frame_samples(oracle_mod):
+    """AccumulationPass's Variance metrics (accumulation_pass.cpp:84-137): the per-pixel sums are the
+    frames' sample luminances (GetLuminance in float, 0.2126 R + 0.7152 G + 0.0722 B left to right)
+    accumulated in double, checked bit-for-bit against the per-frame samples; the statistics equal a
+    two-pass computation of the same quantities (numpy's unbiased variance of each pixel's samples,
+    divided by n: the one-pass formula of the reference must agree to rounding). One frame gives no
+    statistics."""
+    sc = scenes.box_spheres(16, 12, frames=8)
+    o = oracle_mod.Oracle(sc, threads=4)
+    pipe = abi.pipeline("mis", max_depth=2)
+    lums = []
+    for f in range(6):
+        o.render(sc.frame(f), pipe)
+        rgb = o.debug["sample_rgb"].astype(np.float32)
+        l = (np.float32(0.2126) * rgb[:, 0] + np.float32(0.7152) * rgb[:, 1]) + np.float32(0.0722) * rgb[:, 2]
+        lums.append(l.astype(np.float64))
+        if f == 0:
+            assert not o.variance()[0]
+    ok, st, ls, lq = o.variance()
+    lums = np.array(lums)
+    assert ok
+    s = np.zeros(lums.shape[1])
+    q = np.zeros(lums.shape[1])
+    for l in lums:  # frame order, as the pass accumulates
+        s += l
+        q += l * l
+    assert np.array_equal(ls, s) and np.array_equal(lq, q)
+    n = lums.shape[0]
+    ev = lums.var(axis=0, ddof=1) / n
+    assert (lums.std(axis=0) > 0).mean() > 0.5  # most pixels vary between frames
+    assert np.isclose(st["mean_estimator_variance"], ev.mean(), rtol=1e-9, atol=1e-15)
+    assert np.isclose(st["max_estimator_variance"], ev.max(), rtol=1e-9, atol=1e-15)
+    assert np.isclose(st["mean_luminance"], lums.mean(), rtol=1e-12)
+    o.reset()
+    assert not o.variance()[0]
