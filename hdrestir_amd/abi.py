@@ -155,6 +155,20 @@ def reservoir_dtype():
 
 
 
+class PostDesc(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("chromaticity_blur", C.c_int32), ("oidn", C.c_int32),
+                ("lens_flare", C.c_int32), ("chromatic_aberration", C.c_float)]
+
+
+def post_desc(chromaticity_blur=False, lens_flare=False, chromatic_aberration=0.0, oidn=False):
+    """Denoiser::Config (its chromaticity blur; OIDN is not part of the library) and PostProcess::Config."""
+    d = PostDesc()
+    d.struct_size = C.sizeof(PostDesc)
+    d.chromaticity_blur, d.oidn, d.lens_flare = int(bool(chromaticity_blur)), int(bool(oidn)), int(bool(lens_flare))
+    d.chromatic_aberration = float(chromatic_aberration)
+    return d
+
+
 class ServerDesc(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("max_active", C.c_int32), ("hbm_budget_bytes", C.c_uint64),
                 ("batched", C.c_int32), ("own_thread", C.c_int32)]

@@ -304,6 +304,13 @@ struct LaunchShape {
     bool lds_stack = false;  // traversal stacks with LDS rings (RingStack) instead of private memory
 };
 constexpr int kDefaultQueueWaves = 32768;
+// Entries of a lane's any-hit stack in LDS (8 KB per wave; 4 waves per SIMD hold 128 of a CU's 160 KB).
+constexpr int kLdsStack = 32;
+// The persistent any-hit lanes of a scene fit their stack in kLdsStack LDS entries: a tree whose
+// deepest leaf is at depth D (root at 1) stacks at most D - 1 entries, the TLAS's below the BLAS's.
+static bool lds_stack_fits(const hdr_scene *s) {
+    return s->hs.max_tlas_depth + s->hs.max_blas_depth - 2 <= kLdsStack;
+}
 static int queue_blocks(const LaunchShape &ls, size_t cap) {
     const size_t need = std::max<size_t>(1, (cap + kLanes - 1) / kLanes);
     return int(std::min<size_t>(need, size_t(ls.queue_waves > 0 ? ls.queue_waves : kDefaultQueueWaves)));
@@ -327,7 +334,7 @@ static void launch_wave_trace_l(const LaunchShape &ls, int blocks, hipStream_t s
 // the launch renders a[0]'s scene.
 template <bool C>
 static void launch_wave_trace(const LaunchShape &ls, int blocks, hipStream_t st, const FrameArgs *a, int b,
-                              bool traced, int cus, bool one_scene) {
+                              bool traced, int cus, bool one_scene, bool lds_fit) {
     if (traced) {
         if (ls.lds_stack) launch_wave_trace_l<C, true, true>(ls, blocks, st, a, b);
         else launch_wave_trace_l<C, false, true>(ls, blocks, st, a, b);
@@ -337,8 +344,13 @@ static void launch_wave_trace(const LaunchShape &ls, int blocks, hipStream_t st,
     }
     if (traced) {
         const dim3 g(unsigned(ls.queue_waves > 0 ? ls.queue_waves : cus * 4 * 4)), l(kLanes);
-        if (one_scene) hipLaunchKernelGGL((wave_anyhit_kernel<C, true, true>), g, l, 0, st, a, b);
-        else hipLaunchKernelGGL((wave_anyhit_kernel<C, true, false>), g, l, 0, st, a, b);
+        if (lds_fit) {
+            if (one_scene) hipLaunchKernelGGL((wave_anyhit_kernel<C, true, true, kLdsStack>), g, l, 0, st, a, b);
+            else hipLaunchKernelGGL((wave_anyhit_kernel<C, true, false, kLdsStack>), g, l, 0, st, a, b);
+        } else {
+            if (one_scene) hipLaunchKernelGGL((wave_anyhit_kernel<C, true, true, 0>), g, l, 0, st, a, b);
+            else hipLaunchKernelGGL((wave_anyhit_kernel<C, true, false, 0>), g, l, 0, st, a, b);
+        }
     }
 }
 // Shading launch b >= 1; traced frames: wave_shade_traced_kernel.
@@ -374,18 +386,24 @@ static void launch_shadow_l(const LaunchShape &ls, int blocks, hipStream_t st, c
     default: hipLaunchKernelGGL((shadow_kernel<C, 1, L>), dim3(blocks), dim3(kLanes), 0, st, a); break;
     }
 }
-// Shadow rays of a launch's queue (at most `cap` queries). With private-memory stacks the persistent
-// kernel runs (4 waves per SIMD at most 128 VGPRs, lanes refilled in groups of 16): one wave per SIMD
-// slot of the device, or queue_waves of the launch shape. one_scene: every view of the launch
-// renders args[0]'s scene.
+// Shadow rays of a launch's queue (at most `cap` queries). Without the launch shape's LDS rings the
+// persistent kernel runs (4 waves per SIMD at most 128 VGPRs, lanes refilled in groups of 16): one
+// wave per SIMD slot of the device, or queue_waves of the launch shape. one_scene: every view of the
+// launch renders args[0]'s scene. lds_fit: every scene's stacks fit kLdsStack LDS entries per lane
+// (lds_stack_fits), so the persistent kernel keeps them in LDS instead of private memory.
 template <bool C>
 static void launch_shadow(const LaunchShape &ls, int cus, size_t cap, hipStream_t st, const FrameArgs *a,
-                          bool one_scene) {
+                          bool one_scene, bool lds_fit) {
     if (!ls.lds_stack) {
         const size_t slots = ls.queue_waves > 0 ? size_t(ls.queue_waves) : size_t(cus) * 4 * 4;
-        const dim3 g(unsigned(std::min(slots, std::max<size_t>(1, (cap + kLanes - 1) / kLanes))));
-        if (one_scene) hipLaunchKernelGGL((shadow_persistent_kernel<C, 4, 16, true>), g, dim3(kLanes), 0, st, a);
-        else hipLaunchKernelGGL((shadow_persistent_kernel<C, 4, 16, false>), g, dim3(kLanes), 0, st, a);
+        const dim3 g(unsigned(std::min(slots, std::max<size_t>(1, (cap + kLanes - 1) / kLanes)))), l(kLanes);
+        if (lds_fit) {
+            if (one_scene) hipLaunchKernelGGL((shadow_persistent_kernel<C, 4, 16, true, kLdsStack>), g, l, 0, st, a);
+            else hipLaunchKernelGGL((shadow_persistent_kernel<C, 4, 16, false, kLdsStack>), g, l, 0, st, a);
+        } else {
+            if (one_scene) hipLaunchKernelGGL((shadow_persistent_kernel<C, 4, 16, true, 0>), g, l, 0, st, a);
+            else hipLaunchKernelGGL((shadow_persistent_kernel<C, 4, 16, false, 0>), g, l, 0, st, a);
+        }
         return;
     }
     launch_shadow_l<C, true>(ls, queue_blocks(ls, cap), st, a);
@@ -427,6 +445,7 @@ struct hdr_view {
     int frames = 0;
     int32_t samples = 0;  // accumulated samples: frame_index + 1 of the last frame issued, 0 after a reset
     DevBuf resolved;      // staging of hdr_view_resolve
+    DevBuf post;          // hdr_view_post_process: two RGBA planes, the bright pass, per-row flags
     DevBuf lum_sum, lum_sq, variance;  // Variance metrics (hdr_view_set_metrics): per-pixel sums, reduction
     // Frames overlap: the G-buffer is double-buffered by frame parity, and the G-buffer kernel of
     // frame N+1 runs on `gstream` as soon as frame N-1's shading has read that buffer (event
@@ -1179,6 +1198,9 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     const bool deferred = plan.deferred, spatial = plan.spatial;
     const size_t lds = plan.lds;
     const int gp = plan.gbuf_half;  // this frame's G-buffer half (prepare_frame chose it)
+    // LDS stacks for the any-hit lanes only when no G-buffer launch runs beside them on the second
+    // stream: that launch's waves need the LDS too (dolly headline frame 3.04 -> 3.12 ms with them)
+    const bool ldsFit = lds_stack_fits(scene) && plan.gbuf_reuse;
     const dim3 grid(plan.tiles);  // one block per 8x8 tile
     if (view->counting) {
         HIP_TRY(hipMemsetAsync(view->counters.p, 0, view->counters.n, view->stream));
@@ -1245,8 +1267,8 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         HIP_TRY(hipEventRecord(view->shade_done[gp], view->stream));
         view->shade_recorded[gp] = true;
         HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_SHADOW], view->stream));
-        if (view->counting) launch_shadow<true>(view->shape, view->ctx->cus, P, view->stream, da, true);
-        else launch_shadow<false>(view->shape, view->ctx->cus, P, view->stream, da, true);
+        if (view->counting) launch_shadow<true>(view->shape, view->ctx->cus, P, view->stream, da, true, ldsFit);
+        else launch_shadow<false>(view->shape, view->ctx->cus, P, view->stream, da, true, ldsFit);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_SHADOW], view->stream));
         HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_ACCUMULATE], view->stream));
@@ -1303,10 +1325,11 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
                 view->shade_recorded[gp] = true;
             }
             if (view->counting)
-                launch_wave_trace<true>(view->shape, qblocks, view->stream, da, b, plan.wave_traced, view->ctx->cus, true);
+                launch_wave_trace<true>(view->shape, qblocks, view->stream, da, b, plan.wave_traced, view->ctx->cus,
+                                        true, ldsFit);
             else
                 launch_wave_trace<false>(view->shape, qblocks, view->stream, da, b, plan.wave_traced, view->ctx->cus,
-                                         true);
+                                         true, ldsFit);
             HIP_TRY(hipGetLastError());
         }
 #undef HDR_W0
@@ -1519,6 +1542,47 @@ hdr_status hdr_view_resolve(hdr_view *view, int32_t aov, int32_t window_width, i
     HIP_TRY(hipMemcpyAsync(dst, view->resolved.p, bytes, hipMemcpyDeviceToHost, view->stream));
     HIP_TRY(hipStreamSynchronize(view->stream));
     return view_status(view);
+}
+
+hdr_status hdr_view_post_process(hdr_view *view, const hdr_post_desc *desc) {
+    if (!view || !desc) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    if (desc->struct_size != sizeof(hdr_post_desc)) return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_post_desc size");
+    if (desc->oidn) return fail(HDR_ERR_UNSUPPORTED, "the OIDN denoiser is not part of this library");
+    const float ca = desc->chromatic_aberration;
+    // shifts beyond the int range have no defined conversion in the reference (post_process.cpp:122-125)
+    if (!(ca <= 1.0e6f)) return fail(HDR_ERR_INVALID_ARGUMENT, "chromatic_aberration must be a number <= 1e6");
+    if (!desc->chromaticity_blur && !desc->lens_flare && !(ca > 0.0f)) return HDR_OK;  // both passes return early
+    HIP_TRY(hipSetDevice(view->ctx->device));
+    const int W = view->W, H = view->H;
+    const size_t P = size_t(W) * H;
+    if (hdr_status s = ensure(view->post, 3 * P * sizeof(float4) + size_t(H) * sizeof(int))) return s;
+    float4 *color = static_cast<float4 *>(view->color.p);
+    float4 *plane[2] = {static_cast<float4 *>(view->post.p), static_cast<float4 *>(view->post.p) + P};
+    float4 *bloom = plane[1] + P;
+    int *rowAny = reinterpret_cast<int *>(bloom + P);
+    const dim3 b(16, 16), g((W + 15) / 16, (H + 15) / 16);
+    // each pass reads `cur` and writes the other plane; the last result is copied back into Color
+    const float4 *cur = color;
+    int next = 0;
+    if (desc->chromaticity_blur) {
+        hipLaunchKernelGGL(chroma_blur_kernel, g, b, 0, view->stream, cur, plane[next], W, H);
+        cur = plane[next];
+        next ^= 1;
+    }
+    if (desc->lens_flare) {
+        const int blur = std::max(1, std::min(W, H) / 20);  // post_process.cpp:44
+        hipLaunchKernelGGL(flare_bright_kernel, dim3(H), dim3(256), 0, view->stream, cur, bloom, rowAny, W);
+        hipLaunchKernelGGL(flare_gather_kernel, g, b, 0, view->stream, cur, bloom, rowAny, plane[next], W, H, blur);
+        cur = plane[next];
+        next ^= 1;
+    }
+    if (ca > 0.0f) {
+        hipLaunchKernelGGL(chromatic_kernel, g, b, 0, view->stream, cur, plane[next], W, H, ca);
+        cur = plane[next];
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(color, cur, P * sizeof(float4), hipMemcpyDeviceToDevice, view->stream));
+    return HDR_OK;
 }
 
 hdr_status hdr_render_progressive(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
@@ -1862,11 +1926,13 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
     bool waveLean0[2] = {true, true};  // every view's primary vertex stays below RNG output 311
     bool waveSpatial[2] = {false, false};  // some view of the group reuses spatially
     bool oneScene = true;                  // every view renders the first view's scene
+    bool ldsFit = true;  // every scene's any-hit stacks fit in LDS and no view launches a G-buffer
     for (int i = 0; i < n; ++i) {
         oneScene = oneScene && scenes[i] == scenes[0];
         if (hdr_status st =
                 prepare_frame(views[i], scenes[i], &frames[i], &pipes[i], false, args[i], plans[i], true))
             return st;
+        ldsFit = ldsFit && lds_stack_fits(scenes[i]) && plans[i].gbuf_reuse;  // as in hdr_render_frame
         if (plans[i].deferred) queueCap += size_t(views[i]->W) * views[i]->H;
         if (plans[i].wave) {
             const int t = plans[i].wave_traced ? 1 : 0;
@@ -2011,13 +2077,13 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
                 if (v > 0) launch_wave_shade<false, false>(ls, qblocks, lds[wg], b->stream, A(wg), v, t == 1);
                 if (waveSpatial[t] && v == t)
                     launch_wave_spatial<false>(spatialNd[wg], dim3(tiles[wg]), b->stream, A(wg), T(wg), N(wg));
-                launch_wave_trace<false>(ls, qblocks, b->stream, A(wg), v, t == 1, b->ctx->cus, oneScene);
+                launch_wave_trace<false>(ls, qblocks, b->stream, A(wg), v, t == 1, b->ctx->cus, oneScene, ldsFit);
             }
             hipLaunchKernelGGL((wave_resolve_kernel<false, false>), dim3(tiles[wg]), dim3(kLanes), 0, b->stream,
                                A(wg), T(wg), N(wg));
         }
         if (N(kDef) > 0) {
-            launch_shadow<false>(ls, b->ctx->cus, queueCap, b->stream, A(kAll), oneScene);
+            launch_shadow<false>(ls, b->ctx->cus, queueCap, b->stream, A(kAll), oneScene, ldsFit);
             hipLaunchKernelGGL((accumulate_deferred_kernel<false>), dim3(tiles[kDef]), dim3(kLanes), 0, b->stream,
                                A(kDef), T(kDef), N(kDef));
         }
@@ -2038,7 +2104,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
                                      uint64_t(waveDepth[1]), uint64_t(waveLean0[0]), uint64_t(waveLean0[1]),
                                      uint64_t(waveSpatial[0]), uint64_t(waveSpatial[1]),
                                      uint64_t(ls.queue_waves), uint64_t(ls.trace_occ), uint64_t(ls.shade_occ),
-                                     uint64_t(ls.lds_stack), uint64_t(oneScene)};
+                                     uint64_t(ls.lds_stack), uint64_t(oneScene), uint64_t(ldsFit)};
         for (int g = 0; g < kGroups; ++g) {
             key.push_back(uint64_t(tiles[g]));
             key.push_back(uint64_t(lds[g]));

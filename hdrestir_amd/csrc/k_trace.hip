@@ -72,13 +72,20 @@ shadow_kernel(const FrameArgs *__restrict__ args) {
 // hit) records its verdict. kOneScene: every view of the launch renders args[0]'s scene, whose tables
 // are then wave-uniform. Each ray's traversal is trace_any_s's, step for step, so the verdicts are
 // the same.
-template <bool kCount, int kRefill, bool kOneScene, class Query, class Done>
+// kStk = 0: the TLAS and BLAS stacks are private arrays of kStackDepth entries (scratch memory).
+// kStk > 0: one stack of kStk entries per lane in LDS -- the lane's column of the wave's array `lds`
+// (kStk * kLanes ints), entry k at lds[k * kLanes + lane] -- holding the TLAS entries with the current
+// BLAS's above them (bsp is then the index where the BLAS's entries start). A tree whose deepest leaf
+// is at depth D (root at 1) puts at most D - 1 entries on the stack, one per level below its root, so
+// the host takes this variant only for scenes whose TLAS and BLAS depths add up to at most kStk + 2.
+template <bool kCount, int kRefill, bool kOneScene, int kStk, class Query, class Done>
 __device__ __forceinline__ void anyhit_queue(const FrameArgs *__restrict__ args, unsigned int n,
-                                             unsigned int *cursor, TraceStats &st, Query query, Done done) {
+                                             unsigned int *cursor, TraceStats &st, int *lds, Query query, Done done) {
     const int lane = threadIdx.x;
     const FrameArgs &a0 = args[0];
-    int tref[kStackDepth], bref[kStackDepth];
-    int tsp = 0, bsp = 0;
+    int tref[kStk > 0 ? 1 : kStackDepth], bref[kStk > 0 ? 1 : kStackDepth];
+    int *const col = lds + lane;
+    int tsp = 0, bsp = 0;  // kStk > 0: tsp counts every entry of the lane's stack
     RayBox R;
     V3 o = v3s(0.0f), d = v3s(0.0f), oo = o, od = d;
     float tmax = 0.0f;
@@ -147,7 +154,8 @@ __device__ __forceinline__ void anyhit_queue(const FrameArgs *__restrict__ args,
             const int first = leftFirst ? node_ref(l) : node_ref(r);
             if (hl && hr) {
                 const int second = leftFirst ? node_ref(r) : node_ref(l);
-                if (blas) bref[bsp++] = second;
+                if constexpr (kStk > 0) col[kLanes * tsp++] = second;
+                else if (blas) bref[bsp++] = second;
                 else tref[tsp++] = second;
             }
             cur = first;
@@ -178,7 +186,12 @@ __device__ __forceinline__ void anyhit_queue(const FrameArgs *__restrict__ args,
         }
         cur = kNoNode;
         if (blas) {
-            if (bsp > 0) {
+            if constexpr (kStk > 0) {
+                if (tsp > bsp) {
+                    cur = col[kLanes * --tsp];
+                    continue;
+                }
+            } else if (bsp > 0) {
                 cur = bref[--bsp];
                 continue;
             }
@@ -193,7 +206,7 @@ __device__ __forceinline__ void anyhit_queue(const FrameArgs *__restrict__ args,
             od = xform_dir(minv, d);
             R = make_raybox<false>(oo, od);
             world = false;
-            bsp = 0;
+            bsp = kStk > 0 ? tsp : 0;
             const GNode b = load_node(s.nodes, root);
             if (kCount) ++st.nodes;
             float ta;
@@ -213,20 +226,22 @@ __device__ __forceinline__ void anyhit_queue(const FrameArgs *__restrict__ args,
             live = false;
             continue;
         }
-        cur = tref[--tsp];
+        if constexpr (kStk > 0) cur = col[kLanes * --tsp];
+        else cur = tref[--tsp];
     }
 }
 
 // The deferred path's shadow rays on persistent lanes (anyhit_queue over the launch's query queue;
 // qcount[1] is its claim counter): an occluded winner keeps zero throughput in its reservoir (the
 // visibility fix-up of ris_direct_light_integrator.cpp:163-175).
-template <bool kCount, int kOcc, int kRefill, bool kOneScene>
+template <bool kCount, int kOcc, int kRefill, bool kOneScene, int kStk>
 __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
 shadow_persistent_kernel(const FrameArgs *__restrict__ args) {
     const FrameArgs &a0 = args[0];
+    __shared__ int stk[kStk > 0 ? kStk * kLanes : 1];
     TraceStats st = {};
-    anyhit_queue<kCount, kRefill, kOneScene>(
-        args, a0.qcount[0], a0.qcount + 1, st,
+    anyhit_queue<kCount, kRefill, kOneScene, kStk>(
+        args, a0.qcount[0], a0.qcount + 1, st, stk,
         [&](unsigned int i, unsigned int &view, V3 &o, V3 &d, float &tmax) {
             const ShadowQuery q = a0.queue[i];
             view = __float_as_uint(q.d.w) >> 24;
@@ -371,10 +386,11 @@ wave_trace_traced_kernel(const FrameArgs *__restrict__ args, int v) {
 // wcnt[10 + parity] (shadow rays); this launch clears the other parity's for the next one (the
 // frame's first pair is cleared with the counts). (Untraced frames trace their shadow rays in
 // wave_trace_kernel, one per lane: persistent lanes measured slower on those shorter queues.)
-template <bool kCount, bool kTraced, bool kOneScene>
+template <bool kCount, bool kTraced, bool kOneScene, int kStk>
 __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(4)))
 wave_anyhit_kernel(const FrameArgs *__restrict__ args, int v) {
     const FrameArgs &a0 = args[0];
+    __shared__ int stk[kStk > 0 ? kStk * kLanes : 1];
     const int par = v & 1;
     unsigned int *cnt = a0.wcnt;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -383,8 +399,8 @@ wave_anyhit_kernel(const FrameArgs *__restrict__ args, int v) {
     }
     TraceStats st = {};
     if (kTraced) {
-        anyhit_queue<kCount, 16, kOneScene>(
-            args, cnt[6 + par], cnt + 8 + par, st,
+        anyhit_queue<kCount, 16, kOneScene, kStk>(
+            args, cnt[6 + par], cnt + 8 + par, st, stk,
             [&](unsigned int i, unsigned int &view, V3 &o, V3 &d, float &tmax) {
                 const float4 o4 = a0.wesq[2 * i], d4 = a0.wesq[2 * i + 1];
                 view = __float_as_uint(o4.w) >> 24;
@@ -400,8 +416,8 @@ wave_anyhit_kernel(const FrameArgs *__restrict__ args, int v) {
     }
     const int vtx = kTraced ? v - 1 : v;
     const unsigned int ns = kTraced ? (v > 0 ? cnt[2 + (par ^ 1)] : 0u) : cnt[2 + par];
-    anyhit_queue<kCount, 16, kOneScene>(
-        args, ns, cnt + 10 + par, st,
+    anyhit_queue<kCount, 16, kOneScene, kStk>(
+        args, ns, cnt + 10 + par, st, stk,
         [&](unsigned int i, unsigned int &view, V3 &o, V3 &d, float &tmax) {
             const ShadowQuery q = a0.queue[i];
             view = __float_as_uint(q.d.w) >> 24;
@@ -452,14 +468,22 @@ template __global__ void shadow_kernel<false, 1, true>(const FrameArgs *__restri
 template __global__ void shadow_kernel<false, 4, true>(const FrameArgs *__restrict__);
 template __global__ void shadow_kernel<false, 6, true>(const FrameArgs *__restrict__);
 template __global__ void shadow_kernel<false, 8, true>(const FrameArgs *__restrict__);
-template __global__ void shadow_persistent_kernel<true, 4, 16, true>(const FrameArgs *__restrict__);
-template __global__ void shadow_persistent_kernel<true, 4, 16, false>(const FrameArgs *__restrict__);
-template __global__ void shadow_persistent_kernel<false, 4, 16, true>(const FrameArgs *__restrict__);
-template __global__ void shadow_persistent_kernel<false, 4, 16, false>(const FrameArgs *__restrict__);
-template __global__ void wave_anyhit_kernel<true, true, true>(const FrameArgs *__restrict__, int);
-template __global__ void wave_anyhit_kernel<true, true, false>(const FrameArgs *__restrict__, int);
-template __global__ void wave_anyhit_kernel<false, true, true>(const FrameArgs *__restrict__, int);
-template __global__ void wave_anyhit_kernel<false, true, false>(const FrameArgs *__restrict__, int);
+template __global__ void shadow_persistent_kernel<true, 4, 16, true, 0>(const FrameArgs *__restrict__);
+template __global__ void shadow_persistent_kernel<true, 4, 16, true, 32>(const FrameArgs *__restrict__);
+template __global__ void shadow_persistent_kernel<true, 4, 16, false, 0>(const FrameArgs *__restrict__);
+template __global__ void shadow_persistent_kernel<true, 4, 16, false, 32>(const FrameArgs *__restrict__);
+template __global__ void shadow_persistent_kernel<false, 4, 16, true, 0>(const FrameArgs *__restrict__);
+template __global__ void shadow_persistent_kernel<false, 4, 16, true, 32>(const FrameArgs *__restrict__);
+template __global__ void shadow_persistent_kernel<false, 4, 16, false, 0>(const FrameArgs *__restrict__);
+template __global__ void shadow_persistent_kernel<false, 4, 16, false, 32>(const FrameArgs *__restrict__);
+template __global__ void wave_anyhit_kernel<true, true, true, 0>(const FrameArgs *__restrict__, int);
+template __global__ void wave_anyhit_kernel<true, true, true, 32>(const FrameArgs *__restrict__, int);
+template __global__ void wave_anyhit_kernel<true, true, false, 0>(const FrameArgs *__restrict__, int);
+template __global__ void wave_anyhit_kernel<true, true, false, 32>(const FrameArgs *__restrict__, int);
+template __global__ void wave_anyhit_kernel<false, true, true, 0>(const FrameArgs *__restrict__, int);
+template __global__ void wave_anyhit_kernel<false, true, true, 32>(const FrameArgs *__restrict__, int);
+template __global__ void wave_anyhit_kernel<false, true, false, 0>(const FrameArgs *__restrict__, int);
+template __global__ void wave_anyhit_kernel<false, true, false, 32>(const FrameArgs *__restrict__, int);
 template __global__ void wave_trace_kernel<true, 1, true>(const FrameArgs *__restrict__, int);
 template __global__ void wave_trace_kernel<true, 1, false>(const FrameArgs *__restrict__, int);
 template __global__ void wave_trace_kernel<true, 4, true>(const FrameArgs *__restrict__, int);

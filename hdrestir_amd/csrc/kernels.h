@@ -14,13 +14,13 @@ template <bool kCount, bool kLds>
 __global__ void gbuffer_kernel(const FrameArgs *__restrict__ args, const int *__restrict__ tile_base, int n_views);
 template <bool kCount, int kOcc, bool kLds>
 __global__ void shadow_kernel(const FrameArgs *__restrict__ args);
-template <bool kCount, int kOcc, int kRefill, bool kOneScene>
+template <bool kCount, int kOcc, int kRefill, bool kOneScene, int kStk>
 __global__ void shadow_persistent_kernel(const FrameArgs *__restrict__ args);
 template <bool kCount, int kOcc, bool kLds>
 __global__ void wave_trace_kernel(const FrameArgs *__restrict__ args, int bounce);
 template <bool kCount, int kOcc, bool kLds>
 __global__ void wave_trace_traced_kernel(const FrameArgs *__restrict__ args, int v);
-template <bool kCount, bool kTraced, bool kOneScene>
+template <bool kCount, bool kTraced, bool kOneScene, int kStk>
 __global__ void wave_anyhit_kernel(const FrameArgs *__restrict__ args, int v);
 __global__ void intersect_kernel(DeviceScene s, const float *o, const float *d, int n, int *inst, int *tri, float *t);
 
@@ -56,5 +56,13 @@ template <bool kCount, bool kDebug, int kOcc>
 __global__ void wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce);
 template <bool kCount, bool kDebug, int kOcc>
 __global__ void wave_shade_traced_kernel(const FrameArgs *__restrict__ args, int v);
+
+// k_post.hip
+__global__ void chroma_blur_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst, int W, int H);
+__global__ void flare_bright_kernel(const float4 *__restrict__ src, float4 *__restrict__ bloom,
+                                    int *__restrict__ row_any, int W);
+__global__ void flare_gather_kernel(const float4 *__restrict__ src, const float4 *__restrict__ bloom,
+                                    const int *__restrict__ row_any, float4 *__restrict__ dst, int W, int H, int B);
+__global__ void chromatic_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst, int W, int H, float ca);
 
 }  // namespace hdr

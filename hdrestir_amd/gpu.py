@@ -22,7 +22,7 @@ HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-cont
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 # the rendering kernels are split over several translation units so they compile in parallel
 SOURCES = ["k_wshade.hip", "k_wshade_traced.hip", "k_mega.hip", "k_ptlean.hip", "k_wave0.hip", "k_trace.hip",
-           "k_deferred.hip", "hdrgpu.hip",
+           "k_deferred.hip", "k_post.hip", "hdrgpu.hip",
            "host_scene.cpp", "server.cpp"]
 OBJ_DIR = os.path.join(_HERE, "_obj")
 ADAPTER_SMOKE = os.path.join(_HERE, "host", "build", "adapter_smoke")
@@ -106,6 +106,7 @@ def lib():
             "hdr_view_read_variance": [P, C.POINTER(abi.VarianceStats), P, P],
             "hdr_view_sample_count": [P, C.POINTER(I)],
             "hdr_view_resolve": [P, I, I, I, I, I, P],
+            "hdr_view_post_process": [P, C.POINTER(abi.PostDesc)],
             "hdr_render_progressive": [P, P, C.POINTER(abi.FrameDesc), C.POINTER(abi.PipelineDesc), I, I,
                                        C.POINTER(I)],
             "hdr_view_synchronize": [P],
@@ -314,6 +315,12 @@ class Renderer:
         out = np.zeros((window_height, window_width, components), self._FORMAT_DTYPES.get(fmt, np.float32))
         check(lib().hdr_view_resolve(self.h, aov, window_width, window_height, fmt, components, _p(out)))
         return out
+
+    def post_process(self, chromaticity_blur=False, lens_flare=False, chromatic_aberration=0.0, oidn=False):
+        """hdr_view_post_process: DenoiserPass's chromaticity blur, then PostProcessPass's lens flare
+        and chromatic aberration, on the view's Color in place (read it with read_color)."""
+        d = abi.post_desc(chromaticity_blur, lens_flare, chromatic_aberration, oidn)
+        check(lib().hdr_view_post_process(self.h, C.byref(d)))
 
     def set_metrics(self, on):
         """hdr_view_set_metrics: per-pixel luminance sums of the accumulated samples (Variance)."""

@@ -131,6 +131,36 @@ int main(int argc, char **argv) {
     }
     std::printf("mean RIS %.5f PathTracer %.5f\n", means[0], means[1]);
     if (!(means[0] > 0.0 && means[1] > 0.0) || std::fabs(means[0] - means[1]) > 0.1 * means[1]) return 1;
+    // the PathTracerPost passes: the chromaticity blur keeps the mean luma of a frame up to the
+    // clamp at 0 and the edge pixels; the OIDN switch is refused
+    {
+        hdrgpu::PipelineSettings s;
+        s.kind = hdrgpu::PipelineSettings::Kind::PathTracer;
+        s.MaxDepth = 3;
+        hdrgpu::GpuPathTracePass plain(ctx, scene, W, H, s);
+        s.Denoiser.EnableChromaticityBlur = true;
+        s.PostProcess.ChromaticAberration = 2.0f;
+        hdrgpu::GpuPathTracePass post(ctx, scene, W, H, s);
+        std::vector<float> a(color.size()), b(color.size());
+        plain.Execute(Camera(0), a);
+        post.Execute(Camera(0), b);
+        double la = 0, lb = 0;
+        for (size_t i = 0; i < a.size(); i += 4) {
+            la += 0.25 * a[i] + 0.5 * a[i + 1] + 0.25 * a[i + 2];
+            lb += 0.25 * b[i] + 0.5 * b[i + 1] + 0.25 * b[i + 2];
+        }
+        std::printf("post: mean luma %.5f -> %.5f\n", la / (W * H), lb / (W * H));
+        if (a == b || std::fabs(la - lb) > 0.05 * la) return 1;
+        s.Denoiser.EnableDenoiser = true;
+        hdrgpu::GpuPathTracePass oidn(ctx, scene, W, H, s);
+        try {
+            oidn.Execute(Camera(0), b);
+            std::printf("OIDN request accepted\n");
+            return 1;
+        } catch (const hdrgpu::Error &e) {
+            if (e.status != HDR_ERR_UNSUPPORTED) return 1;
+        }
+    }
     // cancellation: a cancelled context refuses the next frame with HDR_ERR_CANCELLED
     hdrgpu::GpuPathTracePass pass(ctx, scene, W, H, hdrgpu::PipelineSettings{});
     ctx.SetCancelled(true);

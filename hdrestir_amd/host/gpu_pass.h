@@ -48,7 +48,21 @@ struct PathTracePassSettings {
     float RouletteAggressiveness{1.0f};
 };
 
+// Denoiser::Config (source/passes/denoiser.h:15-20; its firefly filter is AccumulationPass's, in
+// PipelineSettings) and PostProcess::Config (source/passes/post_process.h:13-17). EnableDenoiser asks
+// for the OIDN network, which the library does not carry (Execute then throws HDR_ERR_UNSUPPORTED).
+struct DenoiserConfig {
+    bool EnableDenoiser{false};
+    bool EnableChromaticityBlur{false};
+};
+struct PostProcessConfig {
+    bool EnableLensFlare{false};
+    float ChromaticAberration{0.0f};
+};
+
 // Pipeline selection: PathTracer (MIS direct light) or RIS (source/restir_pipeline/ris_pipeline.h:15-21).
+// With Denoiser / PostProcess set, the passes after accumulation run as in the reference's
+// PathTracerPost pipeline (source/pipeline/path_tracer_pipeline.h:224-229).
 struct PipelineSettings {
     enum class Kind { PathTracer, RIS } kind{Kind::RIS};
     PathTracePassSettings PathTrace{};
@@ -57,6 +71,24 @@ struct PipelineSettings {
     bool UseReservoir{true};
     bool SkipVisibility{false};
     bool EnableFireflyFilter{false};
+    DenoiserConfig Denoiser{};
+    PostProcessConfig PostProcess{};
+
+    // DenoiserPass / PostProcessPass run when one of their switches is on (denoiser_pass.cpp:17-20,
+    // post_process_pass.cpp:14-17).
+    [[nodiscard]] bool HasPostPasses() const {
+        return Denoiser.EnableDenoiser || Denoiser.EnableChromaticityBlur || PostProcess.EnableLensFlare ||
+               PostProcess.ChromaticAberration > 0.0f;
+    }
+    [[nodiscard]] hdr_post_desc PostDesc() const {
+        hdr_post_desc d{};
+        d.struct_size = sizeof(d);
+        d.chromaticity_blur = Denoiser.EnableChromaticityBlur;
+        d.oidn = Denoiser.EnableDenoiser;
+        d.lens_flare = PostProcess.EnableLensFlare;
+        d.chromatic_aberration = PostProcess.ChromaticAberration;
+        return d;
+    }
 
     [[nodiscard]] hdr_pipeline_desc Desc() const {
         hdr_pipeline_desc d{};
@@ -141,7 +173,14 @@ class GpuPathTracePass {
         f.has_camera_params = ctx.cameraParams.has_value();
         if (ctx.cameraParams) f.camera = *ctx.cameraParams;
         const hdr_pipeline_desc p = _settings.Desc();
-        Check(hdr_render_frame(_view, _scene.Get(), &f, &p, color.data(), nullptr), "hdr_render_frame");
+        if (!_settings.HasPostPasses()) {
+            Check(hdr_render_frame(_view, _scene.Get(), &f, &p, color.data(), nullptr), "hdr_render_frame");
+            return;
+        }
+        Check(hdr_render_frame(_view, _scene.Get(), &f, &p, nullptr, nullptr), "hdr_render_frame");
+        const hdr_post_desc post = _settings.PostDesc();
+        Check(hdr_view_post_process(_view, &post), "hdr_view_post_process");
+        Check(hdr_view_read_color(_view, color.data()), "hdr_view_read_color");
     }
 
     // RenderPipeline::ClearPersistentBuffers (camera / settings / AOV change).

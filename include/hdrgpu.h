@@ -368,6 +368,29 @@ hdr_status hdr_rendered_size(int32_t window_width, int32_t window_height, int32_
 hdr_status hdr_view_resolve(hdr_view *view, int32_t aov, int32_t window_width, int32_t window_height,
                             int32_t format, int32_t components, void *dst);
 
+/* ---- after accumulation: the denoiser's prefilter and the post-process ---- */
+
+/* Denoiser::Config (denoiser.h:15-20) and PostProcess::Config (post_process.h:13-17). */
+typedef struct hdr_post_desc {
+    uint32_t struct_size;       /* sizeof(hdr_post_desc) */
+    int32_t chromaticity_blur;  /* EnableChromaticityBlur: the denoiser's 3x3 chroma average */
+    int32_t oidn;               /* EnableDenoiser's OIDN network: not part of this library, must be 0 */
+    int32_t lens_flare;         /* EnableLensFlare */
+    float chromatic_aberration; /* ChromaticAberration: channel shift in pixels at the image corners (> 0: on,
+                                   at most 1e6) */
+} hdr_post_desc;
+
+/* DenoiserPass then PostProcessPass on the view's Color output, in place and in the order of the
+ * reference's PathTracerPost pipeline (path_tracer_pipeline.h:224-229): the chromaticity blur
+ * (denoiser_pass.cpp:13-38, denoiser.cpp:55-97,163-201 -- the reference denoiser's whole output when
+ * it is built without OIDN), then the lens flare and the chromatic aberration
+ * (post_process_pass.cpp:10-21, post_process.cpp:23-154). Colour channels change, alpha is kept.
+ * Color holds the mean of the view's accumulated frames; the accumulation is not touched, so the
+ * next frame writes the plain mean again (AccumulationPass rewrites Color from its persistent sum,
+ * accumulation_pass.cpp:102-103). Queued on the view's stream; hdr_view_read_color /
+ * hdr_view_resolve read the result. HDR_ERR_UNSUPPORTED when oidn is set. */
+hdr_status hdr_view_post_process(hdr_view *view, const hdr_post_desc *desc);
+
 /* Accumulated samples of the view: frame_index + 1 of the last frame issued to it, 0 after
  * hdr_view_reset (reference: Camera::FrameCount, renderer.cpp:401-406,477). */
 hdr_status hdr_view_sample_count(const hdr_view *view, int32_t *samples);

@@ -69,6 +69,13 @@ void oracle_rng_float(const uint32_t *seeds, int32_t n, int32_t n_draws, float *
 void oracle_camera_rays(const hdr_frame_desc *frame, int32_t width, int32_t height, int32_t centre,
                         float *origins, float *dirs);
 
+/* The passes after accumulation (post_process.cpp): DenoiserPass's prefilter (the chromaticity
+ * blur; the reference's denoiser output without OIDN) when chroma_blur, then PostProcessPass's lens
+ * flare and chromatic aberration (ca > 0), on width x height RGBA floats in place (alpha kept).
+ * Returns -1 for an empty image. */
+int32_t oracle_post_process(float *rgba, int32_t width, int32_t height, int32_t chroma_blur, int32_t lens_flare,
+                            float ca);
+
 #ifdef __cplusplus
 }
 #endif

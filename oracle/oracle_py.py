@@ -46,6 +46,8 @@ def lib():
         L.oracle_rng_raw.argtypes = [P, C.c_int32, C.c_int32, P]
         L.oracle_rng_float.argtypes = [P, C.c_int32, C.c_int32, P]
         L.oracle_camera_rays.argtypes = [C.POINTER(abi.FrameDesc), C.c_int32, C.c_int32, C.c_int32, P, P]
+        L.oracle_post_process.restype = C.c_int32
+        L.oracle_post_process.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float]
         _lib = L
     return _lib
 
@@ -155,3 +157,13 @@ def camera_rays(frame, width, height, centre):
     d = np.zeros((height * width, 3), np.float32)
     lib().oracle_camera_rays(C.byref(frame), width, height, int(centre), _p(o), _p(d))
     return o, d
+
+
+def post_process(rgba, chroma_blur=False, lens_flare=False, chromatic_aberration=0.0):
+    """DenoiserPass's prefilter then PostProcessPass on an H x W x 4 float32 image; returns a new image."""
+    out = np.ascontiguousarray(rgba, dtype=np.float32).copy()
+    h, w = out.shape[:2]
+    if lib().oracle_post_process(_p(out), w, h, int(bool(chroma_blur)), int(bool(lens_flare)),
+                                 float(chromatic_aberration)) != 0:
+        raise ValueError("empty image")
+    return out

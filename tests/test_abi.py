@@ -42,7 +42,7 @@ STRUCTS = {
     "hdr_pixel_debug": abi.PixelDebug, "hdr_frame_counters": abi.FrameCounters,
     "hdr_server_desc": abi.ServerDesc, "hdr_request_desc": abi.RequestDesc, "hdr_request_result": abi.RequestResult,
     "hdr_server_stats": abi.ServerStats, "hdr_launch_config": abi.LaunchConfig,
-    "hdr_variance_stats": abi.VarianceStats,
+    "hdr_variance_stats": abi.VarianceStats, "hdr_post_desc": abi.PostDesc,
 }
 
 
