@@ -79,11 +79,13 @@ __device__ __forceinline__ int pick_light_lean(const FrameArgs &a, R &rng, Trace
     return int(idx);
 }
 
-// One light-sampler NEE candidate evaluated under skipVisibility (_generateNEECandidates :407-443
-// with _evaluateLightSample :482-527): returns the RIS weight, fills `out` when given.
+// One light-sampler NEE candidate (_generateNEECandidates :407-443 with _evaluateLightSample
+// :482-527): returns the RIS weight, fills `out` when given. vis < 0: skipVisibility (no shadow ray,
+// VisibilityTested false); 0 / 1: the candidate's shadow ray, traced beforehand, was occluded /
+// unoccluded (wave_emit_visibility) -- an occluded candidate keeps zero contribution.
 template <class R>
 __device__ __forceinline__ double nee_candidate_lean(const FrameArgs &a, const SpecTab &tab, const Vertex &v,
-                                                     R &rng, int li, Cand *out) {
+                                                     R &rng, int li, Cand *out, int vis = -1) {
     const DeviceScene &s = a.s;
     const GLight &L = s.lights[li];
     LSample ls;
@@ -102,8 +104,8 @@ __device__ __forceinline__ double nee_candidate_lean(const FrameArgs &a, const S
         pBsdf = ggx_pdf(v.c, v.sn, wo, ls.dir);
         const V3 bsdf = ggx_eval(v.c, v.sn, wo, ls.dir);
         const Spec tmul = rgb2spec(tab, bsdf * nDotL);
-        contribution = rgb2spec(tab, ls.color) * tmul;
-        visTested = false;
+        if (vis != 0) contribution = rgb2spec(tab, ls.color) * tmul;
+        visTested = vis >= 0;
     }
     const float target = spec_lum(tab, contribution);
     const float mis = !delta ? power3(pNee, a.n_nee, 0.0f, a.n_sky, pBsdf, a.n_bsdf) : 1.0f;
@@ -125,10 +127,10 @@ __device__ __forceinline__ double nee_candidate_lean(const FrameArgs &a, const S
 }
 
 // Replays the candidate stream from `rng` (a copy taken before candidate 0) and rebuilds NEE
-// candidate `kk`; earlier candidates only consume their draws.
+// candidate `kk` (vis: as nee_candidate_lean); earlier candidates only consume their draws.
 template <bool kDebug, class R>
 __device__ __forceinline__ void rebuild_nee_lean(const FrameArgs &a, const SpecTab &tab, const Vertex &v,
-                                                 R rng, int kk, Trace &dbg, Cand &out) {
+                                                 R rng, int kk, Trace &dbg, Cand &out, int vis = -1) {
     const DeviceScene &s = a.s;
     for (int k = 0; k < kk; ++k) {
         const int li = pick_light_lean<kDebug>(a, rng, dbg, false);
@@ -136,7 +138,7 @@ __device__ __forceinline__ void rebuild_nee_lean(const FrameArgs &a, const SpecT
         for (int q = 0; q < nd; ++q) rng.discard();
     }
     const int li = pick_light_lean<kDebug>(a, rng, dbg, false);
-    (void)nee_candidate_lean(a, tab, v, rng, li, &out);
+    (void)nee_candidate_lean(a, tab, v, rng, li, &out, vis);
 }
 
 // The winner's deferred shadow ray (the visibility fix-up of :159-175 through _evaluateLightSample
@@ -400,7 +402,8 @@ __device__ __forceinline__ double gen_env_weight(const FrameArgs &a, const SpecT
 
 // Rebuilds candidate kk by replaying the stream from `rng` (taken before candidate 0): NEE
 // candidates as rebuild_nee_lean, the connection from its hit, a generated candidate by replaying
-// the draws before it and sampling it again (esc: which generated candidates escaped).
+// the draws before it and sampling it again (esc: which generated candidates escaped and, without
+// skipVisibility, which NEE candidates' shadow rays were unoccluded).
 template <bool kDebug, class R>
 __device__ __forceinline__ void rebuild_traced(const FrameArgs &a, const SpecTab &tab, const Vertex &v,
                                                const Bounce *conn, bool connHit, const HitRec &connRec,
@@ -408,7 +411,7 @@ __device__ __forceinline__ void rebuild_traced(const FrameArgs &a, const SpecTab
     const DeviceScene &s = a.s;
     const int nNee = a.n_nee;
     if (kk < nNee) {
-        rebuild_nee_lean<kDebug>(a, tab, v, rng, kk, dbg, out);
+        rebuild_nee_lean<kDebug>(a, tab, v, rng, kk, dbg, out, a.skip_vis ? -1 : int((esc >> kk) & 1u));
         return;
     }
     for (int k = 0; k < nNee; ++k) {
@@ -434,7 +437,8 @@ __device__ __forceinline__ void rebuild_traced(const FrameArgs &a, const SpecTab
 // RisDirectLightIntegrator::Li (:84-215) with traced BSDF candidates; weights of every candidate in
 // LDS (wl[k * kLanes], k < n_cand). Shadow query and return value as ris_direct_lean. The
 // generated candidates' escape tests (any hit up to 1e30, from the sampled ray) were answered
-// beforehand: bit k of `escm` is set when candidate k's ray escapes.
+// beforehand: bit k of `escm` is set when candidate k's ray escapes. Without skipVisibility the NEE
+// candidates' shadow rays were answered too: bit k < n_nee is set when candidate k's is unoccluded.
 template <bool kCount, bool kDebug, class R>
 __device__ __forceinline__ Spec ris_direct_traced(const FrameArgs &a, const SpecTab &tab, const Vertex &v,
                                                   const Bounce *conn, bool connHit, const HitRec &connRec, R &rng,
@@ -448,7 +452,7 @@ __device__ __forceinline__ Spec ris_direct_traced(const FrameArgs &a, const Spec
         double w = 0.0;
         if (s.num_lights > 0) {
             const int li = pick_light_lean<kDebug>(a, rng, dbg, true);
-            w = nee_candidate_lean(a, tab, v, rng, li, nullptr);
+            w = nee_candidate_lean(a, tab, v, rng, li, nullptr, a.skip_vis ? -1 : int((escm >> k) & 1u));
         }
         wl[k * kLanes] = w;
     }
@@ -458,7 +462,8 @@ __device__ __forceinline__ Spec ris_direct_traced(const FrameArgs &a, const Spec
         wl[k * kLanes] = conn_weight_lean(a, tab, *conn, connHit, connRec, nullptr);
         ++k;
     }
-    uint32_t esc = 0;
+    // the NEE candidates' visibility travels with the escape bits to the winner's rebuild
+    uint32_t esc = a.skip_vis ? 0u : (escm & ((nNee < 32 ? (1u << nNee) : 0u) - 1u));
     for (; k < nCand; ++k) {
         double w = 0.0;
         if (!a.trace_generated) {  // nothing a generated candidate can reach: only its draws

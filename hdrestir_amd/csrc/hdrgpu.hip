@@ -865,7 +865,7 @@ hdr_status hdr_view_device_bytes(const hdr_view *v, uint64_t *out) {
     *out = v->lum_sum.n + v->lum_sq.n + v->variance.n + v->resolved.n + v->res.n + v->accum.n + v->color.n + v->frame.n + v->dbg.n + v->depth.n + v->albedo.n + v->normal.n +
            v->counters.n + v->spill.n + v->gbuf.n + v->rgb2.n + v->flags.n + v->queue.n + v->qcount.n + v->res_t.n +
            v->saux.n + v->wstate.n + v->whit.n + v->wray.n + v->wcq.n + v->walive.n + v->wcnt.n + v->wesc.n +
-           v->wesq.n;
+           v->wesq.n + v->post.n;
     return HDR_OK;
 }
 
@@ -913,6 +913,15 @@ struct FrameNeeds {
                walive + wesq;
     }
 };
+
+// Smallest view that renders RIS without skipVisibility on the wavefront (plan_pipeline).
+constexpr size_t kVisWavefrontPixels = size_t(1) << 17;
+
+// The escape queue of `cap` entries: two float4 each, then a float (tmax) each.
+static size_t escape_queue_bytes(size_t cap) {
+    cap = std::max<size_t>(1, cap);
+    return cap * (2 * sizeof(float4) + sizeof(float));
+}
 
 // Device bytes every view holds from hdr_view_create on: reservoirs, accumulation, colour, the two
 // G-buffer halves, counters, status word, argument ring and tile table.
@@ -972,13 +981,19 @@ static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_
     // does not take). With an environment or emissive materials, BSDF-sampled candidates can see an
     // emitter: the wavefront then emits each vertex's bounce ray and generated candidates' escape
     // rays before its RIS (traced frames; at most 32 candidates, one bit each in the escape mask).
-    const bool needTraced = traceGenerated || traceLastConnection;
-    const bool wave = !deferred && ris && skipVis && !scene->any_translucent && !scene->ds.has_mesh_lights &&
-                      !(needTraced && nCand > 32);
+    // Without skipVisibility the NEE candidates' shadow rays are emitted the same way (their bits
+    // are the NEE candidates' indices, below the generated candidates').
+    // Views under kVisWavefrontPixels keep visibility frames on the megakernel: a 256 x 256 frame of
+    // RIS M = 16 at depth 4 takes 2.8 ms there and 3.4 ms as a wavefront (its ~1000 waves leave the
+    // GPU's SIMDs underfilled in every one of the wavefront's launches), while 384 x 384 takes 4.7 ms
+    // as a wavefront and 1080p 20 ms (187 ms on the megakernel). Both give the same results.
+    const bool needTraced = traceGenerated || traceLastConnection || !skipVis;
+    const bool wave = !deferred && ris && !scene->any_translucent && !scene->ds.has_mesh_lights &&
+                      !(needTraced && nCand > 32) && (skipVis || P >= kVisWavefrontPixels);
     const bool waveTraced = wave && needTraced;
     // Spatial reuse (an extension) runs on the deferred and wavefront paths with temporal reservoirs.
     const bool spatial = pipe->spatial_count > 0;
-    if (spatial && !((deferred || wave) && pipe->use_reservoir))
+    if (spatial && !((deferred || wave) && pipe->use_reservoir && skipVis))
         return fail(HDR_ERR_UNSUPPORTED,
                     "spatial reuse needs ReSTIR (use_reservoir) with skip_visibility, opaque materials and no mesh "
                     "light");
@@ -995,10 +1010,10 @@ static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_
         need.wstate = P * kWaveFields * sizeof(float4);
         // traced frames: hit records of odd and even vertices, escape masks and the escape queue
         need.whit = (waveTraced ? 2 : 1) * P * sizeof(GRec);
-        if (waveTraced) {
-            plan.escapes = traceGenerated ? P * size_t(nCand - nNee) : 0;
+        if (waveTraced) {  // per vertex: the generated candidates' escape rays, the NEE candidates' shadow rays
+            plan.escapes = (traceGenerated ? P * size_t(nCand - nNee) : 0) + (skipVis ? 0 : P * size_t(nNee));
             need.wesc = P * sizeof(unsigned int);
-            need.wesq = std::max<size_t>(1, plan.escapes) * 2 * sizeof(float4);
+            need.wesq = escape_queue_bytes(plan.escapes);
         }
         need.wray = P * 2 * sizeof(float4);
         need.queue = P * sizeof(ShadowQuery);
@@ -1163,6 +1178,7 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
             if (plan.wave_traced) {
                 if (hdr_status st = ensure(view->wesq, need.wesq)) return st;
                 a.wesq = static_cast<float4 *>(view->wesq.p);
+                a.wesq_t = reinterpret_cast<float *>(a.wesq + 2 * std::max<size_t>(1, plan.escapes));
             }
             a.wray = static_cast<float4 *>(view->wray.p);
             a.queue = static_cast<ShadowQuery *>(view->queue.p);
@@ -1944,7 +1960,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         }
     }
     if (queueCap * sizeof(ShadowQuery) > b->queue.n || waveCap * sizeof(ShadowQuery) > b->wqueue.n ||
-        escCap * 2 * sizeof(float4) > b->wesq.n) {
+        (escCap > 0 && escape_queue_bytes(escCap) > b->wesq.n)) {
         HIP_TRY(hipStreamSynchronize(b->stream));  // earlier steps may still read the old queues
         if (hdr_status st = ensure(b->queue, queueCap * sizeof(ShadowQuery))) return st;
         if (waveCap > 0) {
@@ -1954,7 +1970,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
             if (hdr_status st = ensure(b->walive, 2 * waveCap * sizeof(unsigned int))) return st;
         }
         if (escCap > 0)
-            if (hdr_status st = ensure(b->wesq, escCap * 2 * sizeof(float4))) return st;
+            if (hdr_status st = ensure(b->wesq, escape_queue_bytes(escCap))) return st;
     }
     for (int i = 0; i < n; ++i) {
         args[i].view = i;
@@ -1967,6 +1983,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
             args[i].walive2 = args[i].walive + waveCap;
             args[i].wcnt = static_cast<unsigned int *>(b->wcnt.p);
             args[i].wesq = static_cast<float4 *>(b->wesq.p);
+            args[i].wesq_t = reinterpret_cast<float *>(args[i].wesq + 2 * std::max<size_t>(1, escCap));
         } else {
             args[i].queue = static_cast<ShadowQuery *>(b->queue.p);
         }

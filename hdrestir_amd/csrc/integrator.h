@@ -64,6 +64,7 @@ struct FrameArgs {
     float4 *wray;                      // bounce-ray queue: {origin, code}, {direction, -}
     float4 *wcq;                       // contribution of each shadow query (index-aligned with queue)
     float4 *wesq;                      // traced frames: escape-ray queue {origin, code}, {direction, k}
+    float *wesq_t;                     // and the tmax of its entries whose k carries kEscTmax
     unsigned int *wesc;                // traced frames: per pixel, bit k = candidate k's ray escaped
     unsigned int *walive;              // codes of the live paths of even vertices (bounce ray hit)
     unsigned int *walive2;             // and of odd vertices
@@ -248,6 +249,9 @@ __device__ __forceinline__ void flag_rng_overrun(const FrameArgs &a, bool overru
 }
 
 constexpr int kWaveCounts = 12;  // words of FrameArgs::wcnt
+// Flag in an escape-queue entry's k: the ray ends at wesq_t[entry] (an NEE candidate's shadow ray)
+// instead of 1e30 (a generated candidate's escape ray).
+constexpr int kEscTmax = 0x100;
 constexpr int kLanes = 64;  // threads per block; candidate weights live at wlds[k * kLanes + lane]
 
 // Evaluation of a light sample at a vertex (RisDirectLightIntegrator::_evaluateLightSample,

@@ -406,12 +406,12 @@ wave_anyhit_kernel(const FrameArgs *__restrict__ args, int v) {
                 view = __float_as_uint(o4.w) >> 24;
                 o = v3(o4.x, o4.y, o4.z);
                 d = v3(d4.x, d4.y, d4.z);
-                tmax = 1e30f;
+                tmax = (__float_as_int(d4.w) & kEscTmax) ? a0.wesq_t[i] : 1e30f;
             },
-            [&](unsigned int i, unsigned int view, bool hit) {
+            [&](unsigned int i, unsigned int view, bool hit) {  // escaped / unoccluded: bit k
                 if (hit) return;
                 const uint32_t code = __float_as_uint(a0.wesq[2 * i].w);
-                atomicOr(args[view].wesc + int(code & 0xffffffu), 1u << __float_as_int(a0.wesq[2 * i + 1].w));
+                atomicOr(args[view].wesc + int(code & 0xffffffu), 1u << (__float_as_int(a0.wesq[2 * i + 1].w) & 31));
             });
     }
     const int vtx = kTraced ? v - 1 : v;

@@ -22,7 +22,7 @@ __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__
     R gr;           // traced: the stream copy wave_gen leaves at the first generated candidate
     Vertex gv;      // and the vertex
     SpecTab tab;
-    int k0 = 0, nc = 0;
+    int k0 = 0, nc = 0, nv = 0;  // nv: NEE candidates whose shadow rays wave_emit_visibility emits
     if (valid) {
         const int pixel = y * a.W + x;
         const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
@@ -48,6 +48,7 @@ __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__
                 gr = rng_fork(p.rng, wave_slot(a, pixel) ? wave_slot(a, pixel) + kMtN : nullptr);
                 wave_gen(a, tab, 0, g, p, pixel, gr, gv, bray, ro, rd, k0);
                 nc = a.n_cand;
+                nv = a.skip_vis == 0 && a.s.num_lights > 0 ? a.n_nee : 0;
             }
         } else if (p.primary_hit) {
             wave_vertex<kCount, kDebug, kTraced>(a, tab, 0, g, p, wl + lane, pixel, st, query, q, cq, ray, ro, rd,
@@ -69,6 +70,7 @@ __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__
             a.dbg[pixel].prim_t = p.primary_hit ? g.depth : 0.0f;
         }
     }
+    unsigned int nvis = 0;  // traced frames without skipVisibility: the NEE candidates' shadow rays
     const unsigned int qi = wave_append(cnt + 2, query, lane);
     if (query) {
         args[0].queue[qi] = q;
@@ -77,6 +79,7 @@ __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__
     if (kTraced) {  // vertex 0's live list, bounce rays and escape rays
         const unsigned int ai = wave_append(cnt + 4, ray, lane);
         if (ray) alive_list(args[0], 0)[ai] = code;
+        nvis = wave_emit_visibility(cnt + 6, args[0].wesq, args[0].wesq_t, &a, gv, gr, nv, code, lane);
         wave_emit_escapes(cnt + 6, args[0].wesq, tab, gv, gr, k0, nc, code, lane);
         if (kLean && valid && ray) flag_rng_overrun(a, gr.k > 311u);  // the generation's lean stream too
         ray = bray;
@@ -89,7 +92,7 @@ __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__
     if (kCount) {
         unsigned long long *c = a.counters;
         wave_add(c + 1, st.bounce);
-        wave_add(c + 2, query ? 1ULL : 0ULL);
+        wave_add(c + 2, (query ? 1ULL : 0ULL) + nvis);
         wave_add(c + 3, st.cand);
         wave_add(c + 4, st.cand_skipped);
         wave_add(c + 5, st.ts.nodes);

@@ -5,8 +5,8 @@ namespace hdr {
 
 // Traced frames (wavefront.h: wave_gen): launch v (1 .. depth) shades vertex u = v - 1 of every
 // path in vertex u's live list -- its ray queries were answered by the traversal launch before --
-// and, for the paths that go on, emits vertex v's ray queries (bounce ray, escape rays) and
-// appends the path to vertex v's live list. Vertex u's shadow query joins the queue of parity u.
+// and, for the paths that go on, emits vertex v's ray queries (bounce ray, the NEE candidates'
+// shadow rays without skipVisibility, escape rays) and appends the path to vertex v's live list. Vertex u's shadow query joins the queue of parity u.
 template <bool kCount, bool kDebug, int kOcc = 1>
 __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
 wave_shade_traced_kernel(const FrameArgs *__restrict__ args, int v) {
@@ -26,7 +26,8 @@ wave_shade_traced_kernel(const FrameArgs *__restrict__ args, int v) {
         Mt64Deep gr;
         Vertex gv;
         SpecTab tab;
-        int k0 = 0, nc = 0;
+        int k0 = 0, nc = 0, nv = 0;  // nv: NEE candidates whose shadow rays wave_emit_visibility emits
+        const FrameArgs *ga = nullptr;
         if (i < n) {
             code = alive_list(args[0], u)[i];
             const FrameArgs &a = args[code >> 24];
@@ -44,6 +45,8 @@ wave_shade_traced_kernel(const FrameArgs *__restrict__ args, int v) {
                 gr = rng_fork(p.rng, slot ? slot + kMtN : nullptr);
                 wave_gen(a, tab, v, next, p, pixel, gr, gv, bray, bo, bd, k0);
                 nc = a.n_cand;
+                nv = a.skip_vis == 0 && a.s.num_lights > 0 ? a.n_nee : 0;
+                ga = &a;
             }
             store_path(a, pixel, p, u == 0);
         }
@@ -52,6 +55,7 @@ wave_shade_traced_kernel(const FrameArgs *__restrict__ args, int v) {
             args[0].queue[qi] = q;
             args[0].wcq[qi] = cq;
         }
+        nq += wave_emit_visibility(cnt + 6 + (v & 1), args[0].wesq, args[0].wesq_t, ga, gv, gr, nv, code, lane);
         wave_emit_escapes(cnt + 6 + (v & 1), args[0].wesq, tab, gv, gr, k0, nc, code, lane);
         const unsigned int ri = wave_append(cnt + (v & 1), bray, lane);
         if (bray) {

@@ -191,7 +191,8 @@ __device__ __forceinline__ void wave_vertex(const FrameArgs &a, const SpecTab &t
     if (kTraced)
         d = ris_direct_traced<kCount, kDebug>(a, tab, v, ok ? &b : nullptr, cHit, next, p.rng, wl, pixel, st, p.dbg,
                                               first && a.use_res, slot ? slot + kMtN : nullptr,
-                                              a.trace_generated ? a.wesc[pixel] : 0u, qq, qo, qd, qtmax);
+                                              (a.trace_generated || a.skip_vis == 0) ? a.wesc[pixel] : 0u, qq,
+                                              qo, qd, qtmax);
     else
         d = ris_direct_lean<kCount, kDebug>(a, tab, v, ok && b.pdf > 0.0f, p.rng, wl, pixel, st, p.dbg,
                                             first && a.use_res, slot ? slot + kMtN : nullptr, qq, qo, qd, qtmax);
@@ -228,9 +229,11 @@ __device__ __forceinline__ void wave_vertex(const FrameArgs &a, const SpecTab &t
 // its generated candidates -- the bounce sample and the NEE candidates' draws -- on a copy of the
 // path's stream (the record keeps the position before the vertex) and emits the rays whose
 // answers the vertex needs: the bounce ray (closest hit; not at the last vertex unless the last
-// connection is traced) and, through wave_emit_escapes, each sampled generated candidate's escape
-// ray (any hit up to 1e30). r and vx are left at the first generated candidate k0 (k0 = n_cand when
-// nothing generated can see an emitter). Clears the pixel's escape mask.
+// connection is traced), without skipVisibility each NEE candidate's shadow ray (through
+// wave_emit_visibility: wave_gen then leaves r before the NEE candidates) and, through
+// wave_emit_escapes, each sampled generated candidate's escape ray (any hit up to 1e30). r and vx
+// are left at the first generated candidate k0 (k0 = n_cand when nothing generated can see an
+// emitter). Clears the pixel's escape mask.
 template <class R>
 __device__ __forceinline__ void wave_gen(const FrameArgs &a, const SpecTab &tab, int bounce, const HitRec &h,
                                          const WavePath<R> &p, int pixel, R &r, Vertex &vx, bool &bray, float4 &bo,
@@ -251,9 +254,10 @@ __device__ __forceinline__ void wave_gen(const FrameArgs &a, const SpecTab &tab,
         bd = make_float4(b.d.x, b.d.y, b.d.z, 0.0f);
     }
     k0 = a.n_cand;
-    if (!a.trace_generated) return;
+    const bool vis = a.skip_vis == 0;  // the NEE candidates' shadow rays follow (wave_emit_visibility)
+    if (!a.trace_generated && !vis) return;
     a.wesc[pixel] = 0u;
-    if (s.num_lights > 0) {  // the NEE candidates' draws (ris_direct_traced's order)
+    if (!vis && s.num_lights > 0) {  // the NEE candidates' draws (ris_direct_traced's order)
         Trace none;
         for (int k = 0; k < a.n_nee; ++k) {
             const int li = pick_light_lean<false>(a, r, none, false);
@@ -261,6 +265,7 @@ __device__ __forceinline__ void wave_gen(const FrameArgs &a, const SpecTab &tab,
             for (int q = 0; q < nd; ++q) r.discard();
         }
     }
+    if (!a.trace_generated) return;
     k0 = a.n_nee + a.n_sky + ((ok && b.pdf > 0.0f) ? 1 : 0);  // after the connection candidate
 }
 
@@ -275,6 +280,56 @@ __device__ __forceinline__ unsigned int wave_append(unsigned int *count, bool wa
         base = __shfl(base, 0);
     }
     return base + unsigned(__popcll(mask & ((1ULL << lane) - 1ULL)));
+}
+
+// Without skipVisibility: the shadow rays of this lane's n NEE candidates (wave_gen's state, r
+// before candidate 0; n = 0 for a lane with none), appended to the escape queue {origin, code},
+// {direction, k | kEscTmax} with their tmax in wesq_t. Each candidate makes the draws and takes the
+// decisions of nee_candidate_lean -- light pick, light sample, then the ray only when the candidate
+// can contribute (n . l > 0, a positive pdf, the light's front face) -- and its ray is
+// _evaluateLightSample's: from pos + n_s 1e-4 toward the sample, ending 1e-3 short of it
+// (ris_direct_light_integrator.cpp:482-527). Leaves r after the NEE candidates. All 64 lanes of the
+// wave call it together; it returns the lane's ray count.
+template <class R>
+__device__ __forceinline__ unsigned int wave_emit_visibility(unsigned int *count, float4 *queue, float *tq,
+                                                             const FrameArgs *a, const Vertex &vx, R &r, int n,
+                                                             uint32_t code, int lane) {
+    unsigned int emitted = 0;
+    for (int k = 0;; ++k) {
+        const bool live = k < n;
+        if (__ballot(live) == 0ULL) break;
+        bool want = false;
+        V3 so = v3s(0.0f), dir = v3s(0.0f);
+        float tmax = 0.0f;
+        if (live) {
+            const DeviceScene &s = a->s;
+            Trace none;
+            const int li = pick_light_lean<false>(*a, r, none, false);
+            const GLight &L = s.lights[li];
+            LSample ls;
+            if (sample_light_inl(s, L, vx.h.pos, r, ls)) {
+                ls.pdf = ls.pdf * a->light_select_pdf;
+                const float nDotL = dot(vx.sn, ls.dir);
+                const float dist2 = ls.dist * ls.dist;
+                const float cosY = smax(0.0f, dot(-ls.dir, ls.lnormal));
+                const float pNee = to_solid_angle(ls.pdf, ls.solid_angle, dist2, smax(cosY, 1e-6f));
+                if (!(nDotL <= 0.0f || pNee <= 0.0f || (cosY <= 0.0f && L.is_delta == 0))) {
+                    want = true;
+                    so = vx.h.pos + vx.sn * 1e-4f;
+                    dir = ls.dir;
+                    tmax = ls.dist - 1e-3f;
+                }
+            }
+        }
+        const unsigned int qi = wave_append(count, want, lane);
+        if (want) {
+            queue[2 * qi] = make_float4(so.x, so.y, so.z, __uint_as_float(code));
+            queue[2 * qi + 1] = make_float4(dir.x, dir.y, dir.z, __int_as_float(k | kEscTmax));
+            tq[qi] = tmax;
+            ++emitted;
+        }
+    }
+    return emitted;
 }
 
 // Escape rays of this lane's generated candidates k0 .. n_cand-1 (wave_gen's state), appended to

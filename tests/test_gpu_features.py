@@ -463,3 +463,98 @@ def test_ray_counters_match_the_oracle(gpu_ctx, oracle_mod, case):
             assert g["bounce_rays"] == c["bounce_rays"] and g["candidate_rays"] == c["candidate_rays"]
     r.close()
     gs.close()
+
+
+
+# views of 384 x 352 = 135168 pixels: at least 2^17, so RIS without skipVisibility takes the
+# wavefront (smaller views keep the megakernel); the oracle renders a 48 x 40 crop of each
+VW, VH = 384, 352
+VIS_CASES = {
+    # config 0's pipeline: RIS M = 16 (8 + 8) with visibility at depth 4, reservoir off and on
+    "box_ris16_d4": (lambda: scenes.box_spheres(VW, VH, frames=3),
+                     abi.pipeline("ris", candidate_count=16, use_reservoir=False, max_depth=4)),
+    "box_restir16_d4": (lambda: scenes.box_spheres(VW, VH, frames=3),
+                        abi.pipeline("ris", candidate_count=16, use_reservoir=True, max_depth=4)),
+    # an environment and emissive materials: the NEE candidates' shadow rays share the escape mask
+    # with the generated candidates' escape rays
+    "features_restir8_d4": (lambda: scenes.features(VW, VH, dome=True, emissive=True, mesh_light=False, opaque=True),
+                            abi.pipeline("ris", candidate_count=8, use_reservoir=True, max_depth=4)),
+    "features_ris32_d2_dof": (lambda: scenes.features(VW, VH, dome=True, emissive=False, mesh_light=False,
+                                                      opaque=True, dof=True),
+                              abi.pipeline("ris", candidate_count=32, use_reservoir=False, max_depth=2)),
+}
+
+
+def _vis_crop():
+    x0, y0 = VW // 2 - 24, VH // 2 - 20
+    sel = np.zeros((VH, VW), bool)
+    sel[y0:y0 + 40, x0:x0 + 48] = True
+    return (x0, y0, x0 + 48, y0 + 40), sel.reshape(-1)
+
+
+@pytest.mark.parametrize("case", list(VIS_CASES))
+def test_ris_with_visibility_on_the_wavefront(gpu_ctx, oracle_mod, case):
+    """RIS without skipVisibility on opaque scenes without mesh lights runs on the wavefront: each
+    NEE candidate's shadow ray is traced by the traversal launch before its vertex is shaded (the
+    megakernel traces it inline). On the oracle's crop, streams equal on every pixel, colours within
+    the shared bar, reservoirs equal; the stage record shows the wavefront (accumulation launches,
+    no deferred shadow launch)."""
+    from hdrestir_amd import gpu
+    make, pipe = VIS_CASES[case]
+    sc = make()
+    rect, sel = _vis_crop()
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    o = oracle_mod.Oracle(sc, threads=8)
+    r.stage_times()
+    for f in range(3):
+        fr = scenes.frame_with_camera(sc, f)
+        r.render(fr, pipe, debug=True)
+        o.render(fr, pipe, rect=rect)
+        parity.check(f"{case} frame {f}", _match(r, o)[sel], r.debug["sample_rgb"][sel], o.debug["sample_rgb"][sel])
+        if pipe.use_reservoir:
+            gr, orr = r.reservoirs()[sel], o.reservoirs()[sel]
+            assert np.array_equal(gr["flags_light"], orr["flags_light"])
+            assert np.array_equal(gr["count"], orr["count"])
+    st = r.stage_times()
+    assert st["accumulate"][1] == 3 and st["shadow"][1] == 0, st  # the wavefront ran
+    r.close()
+    gs.close()
+
+
+def test_small_views_keep_visibility_frames_on_the_megakernel(gpu_ctx):
+    """Under 2^17 pixels a visibility frame runs on the megakernel (no accumulation launch)."""
+    from hdrestir_amd import gpu
+    sc = scenes.box_spheres(256, 256, frames=1)
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    r.stage_times()
+    r.render(sc.frame(0), abi.pipeline("ris", candidate_count=16, use_reservoir=True, max_depth=4), read_color=False)
+    st = r.stage_times()
+    assert st["integrate"][1] == 1 and st["accumulate"][1] == 0, st
+    r.close()
+    gs.close()
+
+
+def test_ris_with_visibility_counts_its_shadow_rays(gpu_ctx, oracle_mod):
+    """The NEE candidates' shadow rays are the frame's shadow rays: the wavefront counts as many as
+    the oracle on a view it renders whole (the counters sum over every pixel)."""
+    from hdrestir_amd import gpu
+    sc = scenes.box_spheres(VW, VH, frames=2)
+    pipe = abi.pipeline("ris", candidate_count=16, use_reservoir=True, max_depth=4)
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    r.set_counting(True)
+    o = oracle_mod.Oracle(sc, threads=16)
+    fr = scenes.frame_with_camera(sc, 0)
+    r.render(fr, pipe, read_color=False)
+    r.synchronize()
+    g = r.counters()
+    o.render(fr, pipe)
+    c = o.counters()
+    assert g["shadow_rays"] == c["shadow_rays"] > sc.width * sc.height, (g, c)
+    assert g["primary_rays"] == c["primary_rays"]
+    assert (g["bounce_rays"] + g["candidate_rays"] + g["candidate_rays_skipped"] ==
+            c["bounce_rays"] + c["candidate_rays"])
+    r.close()
+    gs.close()
