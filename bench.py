@@ -292,10 +292,13 @@ def serving_measurement(gpu, abi, ctx, sc_many, g_many, n_requests, seed=1):
     return out
 
 
-def deep_path_measurement(gpu, abi, ctx, sc, gs, steps, depth=6):
-    """The many-lights frame at path depth `depth` with ReSTIR at every vertex (M = 32, one shadow
-    ray per vertex): the wavefront path. Frames/s and rays/s of one 1080p view, static camera."""
-    pipe = abi.pipeline("ris", candidate_count=32, use_reservoir=True, skip_visibility=True, max_depth=depth)
+def deep_path_measurement(gpu, abi, ctx, sc, gs, steps, depth=6, m=32, skip_visibility=True):
+    """The many-lights frame at path depth `depth` with ReSTIR at every vertex: M = 32 with one shadow
+    ray per vertex (skipVisibility), or M candidates whose NEE half each traces its shadow ray
+    (without skipVisibility). The wavefront path. Frames/s and rays/s of one 1080p view, static
+    camera."""
+    pipe = abi.pipeline("ris", candidate_count=m, use_reservoir=True, skip_visibility=skip_visibility,
+                        max_depth=depth)
     r = gpu.Renderer(ctx, gs, sc.width, sc.height)
     r.set_counting(True)
     r.render(sc.frame(0, camera_index=0), pipe, read_color=False)
@@ -314,7 +317,9 @@ def deep_path_measurement(gpu, abi, ctx, sc, gs, steps, depth=6):
     ms = (time.perf_counter() - t0) * 1000.0 / steps
     rays = c["primary_rays"] + c["bounce_rays"] + c["shadow_rays"]
     r.close()
-    return {"workload": f"many_lights_1080p_restir_depth{depth}", "max_depth": depth, "candidate_count": 32,
+    name = f"many_lights_1080p_restir_depth{depth}" if skip_visibility else \
+        f"many_lights_1080p_restir{m}_depth{depth}_visibility"
+    return {"workload": name, "max_depth": depth, "candidate_count": m, "skip_visibility": skip_visibility,
             "frames": steps, "ms_per_frame": round(ms, 3), "frames_per_s": round(1000.0 / ms, 3),
             "rays_per_frame": {"primary": c["primary_rays"], "bounce": c["bounce_rays"], "shadow": c["shadow_rays"]},
             "mrays_per_s": round(rays / ms / 1e3, 2)}
@@ -415,6 +420,29 @@ def multiview_measurement(gpu, abi, ctx, gs, steps, seed=3):
             "autotune_step_ms": round(tuned_ms, 3), "batched_ms_per_step": round(ms_batched, 3),
             "batched_mpixels_per_s": round(pixels / ms_batched / 1e3, 2),
             "plain_ms_per_step": round(ms_plain, 3), "speedup": round(ms_plain / ms_batched, 3)}
+
+
+def post_process_measurement(r, frame, pipe, reps=8):
+    """hdr_view_post_process (DenoiserPass's chromaticity blur, PostProcessPass's lens flare and
+    chromatic aberration) on the view's 1080p colour: each pass alone and all three, wall time per
+    call from launch to the end of its kernels (a fresh frame before each call, outside the timing)."""
+    out = {}
+    cases = (("chromaticity_blur", dict(chromaticity_blur=True)), ("lens_flare", dict(lens_flare=True)),
+             ("chromatic_aberration", dict(chromatic_aberration=4.0)),
+             ("all", dict(chromaticity_blur=True, lens_flare=True, chromatic_aberration=4.0)))
+    for name, kw in cases:
+        ts = []
+        for k in range(reps + 1):
+            r.reset()
+            r.render(frame, pipe, read_color=False)
+            r.synchronize()
+            t0 = time.perf_counter()
+            r.post_process(**kw)
+            r.synchronize()
+            if k:
+                ts.append((time.perf_counter() - t0) * 1e3)
+        out[name + "_ms"] = round(float(np.median(ts)), 4)
+    return out
 
 
 def run_frames(r, desc, pipe, warmup, steps, reset_each):
@@ -644,11 +672,17 @@ def main():
         "static_camera": static_line,
         "image_mean_rgb": [round(float(v), 3) for v in img[..., :3].reshape(-1, 3).mean(0)],
     }
+    if not args.headline_only:
+        line["post_process"] = post_process_measurement(r, dolly(0), pipe)
+        _log(f"post process {line['post_process']}")
     r.close()
     if not args.headline_only and not args.no_deep:
         _log("deep paths")
         line["deep_paths"] = deep_path_measurement(gpu, abi, ctx, sc, gs, max(4, args.steps // 4))
         _log(f"deep paths {line['deep_paths']}")
+        line["visibility_paths"] = deep_path_measurement(gpu, abi, ctx, sc, gs, max(4, args.steps // 8), depth=4,
+                                                         m=16, skip_visibility=False)
+        _log(f"visibility paths {line['visibility_paths']}")
         _log("multi-view batch")
         line["multiview"] = multiview_measurement(gpu, abi, ctx, gs, 4)
         _log(f"multi-view {line['multiview']}")
