@@ -1565,7 +1565,7 @@ hdr_status hdr_view_post_process(hdr_view *view, const hdr_post_desc *desc) {
     if (desc->struct_size != sizeof(hdr_post_desc)) return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_post_desc size");
     if (desc->oidn) return fail(HDR_ERR_UNSUPPORTED, "the OIDN denoiser is not part of this library");
     const float ca = desc->chromatic_aberration;
-    // shifts beyond the int range have no defined conversion in the reference (post_process.cpp:122-125)
+    // shifts beyond the int range have no defined conversion in the reference (post_process.cpp:128-131)
     if (!(ca <= 1.0e6f)) return fail(HDR_ERR_INVALID_ARGUMENT, "chromatic_aberration must be a number <= 1e6");
     if (!desc->chromaticity_blur && !desc->lens_flare && !(ca > 0.0f)) return HDR_OK;  // both passes return early
     HIP_TRY(hipSetDevice(view->ctx->device));
@@ -1586,7 +1586,7 @@ hdr_status hdr_view_post_process(hdr_view *view, const hdr_post_desc *desc) {
         next ^= 1;
     }
     if (desc->lens_flare) {
-        const int blur = std::max(1, std::min(W, H) / 20);  // post_process.cpp:44
+        const int blur = std::max(1, std::min(W, H) / 20);  // post_process.cpp:43
         hipLaunchKernelGGL(flare_bright_kernel, dim3(H), dim3(256), 0, view->stream, cur, bloom, rowAny, W);
         hipLaunchKernelGGL(flare_gather_kernel, g, b, 0, view->stream, cur, bloom, rowAny, plane[next], W, H, blur);
         cur = plane[next];

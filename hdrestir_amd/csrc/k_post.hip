@@ -14,7 +14,7 @@ namespace {
 __device__ __forceinline__ float pos(float v) { return 0.0f < v ? v : 0.0f; }
 }  // namespace
 
-// Denoiser::_applyChromaticityBlur (denoiser.cpp:55-97): luma Y of the pixel, chroma (Co, Cg) averaged
+// Denoiser::_applyChromaticityBlur (denoiser.cpp:54-94): luma Y of the pixel, chroma (Co, Cg) averaged
 // over its 3x3 neighbourhood with clamped edges, back to RGB clamped at 0. One pixel per lane of a
 // 16x16 block; the nine neighbours are L1 / L2 hits of the block's own rows.
 __global__ void __launch_bounds__(256) chroma_blur_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst,
@@ -76,7 +76,7 @@ __device__ __forceinline__ void flare_add(float3 &acc, const float4 &b, float w,
 }
 }  // namespace
 
-// The lens flare's blur and composite (post_process.cpp:43-105) as a gather. The CPU loop scatters:
+// The lens flare's blur and composite (post_process.cpp:43-108) as a gather. The CPU loop scatters:
 // each bright source (x, y) adds bloom * 1/(|d|+1) to (x+d, y) and (x, y+d), and half of that to
 // (x+d, y+d) and (x+d, y-d), for d = -B, -B+2, ..., B (d != 0, B = max(1, min(W, H) / 20)). A source
 // reaches a target along at most one of the four directions (they differ in dy = 0, dx = 0, dx = dy,
@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(256) flare_gather_kernel(const float4 *__restr
     dst[i] = make_float4(c.x + acc.x * 0.1f, c.y + acc.y * 0.1f, c.z + acc.z * 0.1f, c.w);
 }
 
-// PostProcess::_applyChromaticAberration (post_process.cpp:108-137): red read ca * r / r_max pixels
+// PostProcess::_applyChromaticAberration (post_process.cpp:111-142): red read ca * r / r_max pixels
 // further in along the pixel's radius from the image centre, blue as far out (truncated, clamped to
 // the image), green and alpha kept.
 __global__ void __launch_bounds__(256) chromatic_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst,

@@ -48,7 +48,7 @@ struct PathTracePassSettings {
     float RouletteAggressiveness{1.0f};
 };
 
-// Denoiser::Config (source/passes/denoiser.h:15-20; its firefly filter is AccumulationPass's, in
+// Denoiser::Config (source/passes/denoiser.h:14-19; its firefly filter is AccumulationPass's, in
 // PipelineSettings) and PostProcess::Config (source/passes/post_process.h:13-17). EnableDenoiser asks
 // for the OIDN network, which the library does not carry (Execute then throws HDR_ERR_UNSUPPORTED).
 struct DenoiserConfig {

@@ -370,7 +370,7 @@ hdr_status hdr_view_resolve(hdr_view *view, int32_t aov, int32_t window_width, i
 
 /* ---- after accumulation: the denoiser's prefilter and the post-process ---- */
 
-/* Denoiser::Config (denoiser.h:15-20) and PostProcess::Config (post_process.h:13-17). */
+/* Denoiser::Config (denoiser.h:14-19) and PostProcess::Config (post_process.h:13-17). */
 typedef struct hdr_post_desc {
     uint32_t struct_size;       /* sizeof(hdr_post_desc) */
     int32_t chromaticity_blur;  /* EnableChromaticityBlur: the denoiser's 3x3 chroma average */
@@ -382,9 +382,9 @@ typedef struct hdr_post_desc {
 
 /* DenoiserPass then PostProcessPass on the view's Color output, in place and in the order of the
  * reference's PathTracerPost pipeline (path_tracer_pipeline.h:224-229): the chromaticity blur
- * (denoiser_pass.cpp:13-38, denoiser.cpp:55-97,163-201 -- the reference denoiser's whole output when
+ * (denoiser_pass.cpp:13-38, denoiser.cpp:54-94,154-201 -- the reference denoiser's whole output when
  * it is built without OIDN), then the lens flare and the chromatic aberration
- * (post_process_pass.cpp:10-21, post_process.cpp:23-154). Colour channels change, alpha is kept.
+ * (post_process_pass.cpp:10-21, post_process.cpp:23-160). Colour channels change, alpha is kept.
  * Color holds the mean of the view's accumulated frames; the accumulation is not touched, so the
  * next frame writes the plain mean again (AccumulationPass rewrites Color from its persistent sum,
  * accumulation_pass.cpp:102-103). Queued on the view's stream; hdr_view_read_color /

@@ -4,14 +4,14 @@
 // TEST INFRASTRUCTURE ONLY (see oracle.h). A line-by-line restatement of the reference's CPU loops,
 // in their order of operations (compiled with -ffp-contract=off, so every product and sum rounds as
 // written):
-//   ChromaticityBlur ....... Denoiser::_applyChromaticityBlur (source/passes/denoiser.cpp:55-97)
-//   LensFlare .............. PostProcess::_applyLensFlare (source/passes/post_process.cpp:23-106),
+//   ChromaticityBlur ....... Denoiser::_applyChromaticityBlur (source/passes/denoiser.cpp:54-94)
+//   LensFlare .............. PostProcess::_applyLensFlare (source/passes/post_process.cpp:23-109),
 //                            its scatter loop over source pixels in raster order
-//   ChromaticAberration .... PostProcess::_applyChromaticAberration (post_process.cpp:108-137)
+//   ChromaticAberration .... PostProcess::_applyChromaticAberration (post_process.cpp:111-142)
 //   oracle_post_process .... DenoiserPass::_execute -> Denoiser::Run without OIDN
-//                            (denoiser_pass.cpp:13-38, denoiser.cpp:163-201), then
+//                            (denoiser_pass.cpp:13-38, denoiser.cpp:154-201), then
 //                            PostProcessPass::_execute -> PostProcess::Run (post_process_pass.cpp:10-21,
-//                            post_process.cpp:139-154, 186-201): the order of the reference's
+//                            post_process.cpp:144-160, 197-210): the order of the reference's
 //                            PathTracerPost pipeline (path_tracer_pipeline.h:224-229)
 // What pins each part (tests/test_oracle_pins.py; a planted mistake fails its pin): the blur of a
 // constant image returns the image (the YCoCg round trip) and of a grey image its luminance; a lone
@@ -37,7 +37,7 @@ std::vector<float> PackRgb(const float *rgba, std::size_t n) {
     return c;
 }
 
-// denoiser.cpp:55-97
+// denoiser.cpp:54-94
 void ChromaticityBlur(std::vector<float> &out, const std::vector<float> &color, unsigned width, unsigned height) {
     for (int y = 0; y < int(height); ++y) {
         for (int x = 0; x < int(width); ++x) {
@@ -65,7 +65,7 @@ void ChromaticityBlur(std::vector<float> &out, const std::vector<float> &color, 
     }
 }
 
-// post_process.cpp:23-106: threshold 2 on Rec.709 luminance, then each bright pixel scatters its
+// post_process.cpp:23-109: threshold 2 on Rec.709 luminance, then each bright pixel scatters its
 // excess to every other offset d in [-blurSize, blurSize] along the row, the column and both
 // diagonals; 0.1 of the scattered sum is added to the colour.
 void LensFlare(std::vector<float> &finalColor, const std::vector<float> &color, unsigned width, unsigned height) {
@@ -109,7 +109,7 @@ void LensFlare(std::vector<float> &finalColor, const std::vector<float> &color, 
         for (int k = 0; k < 3; ++k) finalColor[i * 3 + k] += blurred[i * 3 + k] * 0.1f;
 }
 
-// post_process.cpp:108-137: red from ca * r / r_max pixels further in along the radius, blue from as
+// post_process.cpp:111-142: red from ca * r / r_max pixels further in along the radius, blue from as
 // far out, green kept.
 void ChromaticAberration(std::vector<float> &finalColor, unsigned width, unsigned height, float ca) {
     std::vector<float> caColor = finalColor;
@@ -148,14 +148,14 @@ int32_t oracle_post_process(float *rgba, int32_t width, int32_t height, int32_t 
     if (width <= 0 || height <= 0) return -1;
     const unsigned W = unsigned(width), H = unsigned(height);
     const std::size_t n = std::size_t(W) * H;
-    if (chroma_blur) {  // denoiser.cpp:163-181 (no OIDN: the output is the prefiltered colour)
+    if (chroma_blur) {  // denoiser.cpp:154-176 (no OIDN: the output is the prefiltered colour)
         const std::vector<float> color = PackRgb(rgba, n);
         std::vector<float> prefiltered(n * 3);
         ChromaticityBlur(prefiltered, color, W, H);
         for (std::size_t i = 0; i < n; ++i)
             for (int k = 0; k < 3; ++k) rgba[i * 4 + k] = prefiltered[i * 3 + k];
     }
-    if (lens_flare || ca > 0.0f) {  // post_process.cpp:139-154
+    if (lens_flare || ca > 0.0f) {  // post_process.cpp:144-160
         const std::vector<float> color = PackRgb(rgba, n);
         std::vector<float> finalColor = color;
         if (lens_flare) LensFlare(finalColor, color, W, H);

@@ -1,18 +1,18 @@
 """Pins the oracle's passes after accumulation (oracle/post_process.cpp) against what the
 mathematics fixes. CPU only:
 
-  * chromaticity blur (denoiser.cpp:55-97): in the lifting YCoCg basis Y = (r + 2g + b) / 4,
+  * chromaticity blur (denoiser.cpp:54-94): in the lifting YCoCg basis Y = (r + 2g + b) / 4,
     Co = (r - b) / 2, Cg = (-r + 2g - b) / 4 with inverse r = Y + Co - Cg, g = Y + Cg, b = Y - Co - Cg,
     the output keeps each pixel's own luma and takes the mean chroma of its 3x3 neighbourhood with
     edge pixels repeated (float64 brute force over the neighbourhood), clamped at 0; a constant image
     is returned unchanged;
-  * lens flare (post_process.cpp:23-106): a lone pixel of colour (c, c, c), c > 2, adds
+  * lens flare (post_process.cpp:23-109): a lone pixel of colour (c, c, c), c > 2, adds
     0.1 (c - 2) / (|d| + 1) at offsets d = -B, -B + 2, ..., B (d != 0, B = max(1, min(W, H) / 20))
     along its row and column and half of that along both diagonals, nothing elsewhere; the flare is
     additive over separated sources; an image whose Rec.709 luminance stays at or under 2 is
     unchanged, and a pure green pixel scatters (every channel's excess over 2) just above 2 / 0.7152
     and not just below;
-  * chromatic aberration (post_process.cpp:108-137): red comes from a pixel closer to the image
+  * chromatic aberration (post_process.cpp:111-142): red comes from a pixel closer to the image
     centre along the pixel's radius, blue from one further out, both ca r / r_max away (up to the
     truncation to whole pixels); green, alpha and the centre pixel are kept, and ca = 0 is the
     identity.
