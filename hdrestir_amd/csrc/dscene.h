@@ -10,12 +10,15 @@
 //                         36 bytes the Moller-Trumbore test reads, one 16-B aligned record.
 //   attrs      GTriAttr[] 72 B: shading normals and vertex colours, read once per ray for the
 //                         winning triangle only.
-//   tlas_idx   int[ ]         TLAS leaf slots -> instance index (TlasInstanceIndices).
+//   tlas_idx   int[ ]         TLAS leaf slots -> instance index.
+//   tlas_rank  int[ ]         TLAS leaf slots -> the instance's slot in the reference's TLAS.
 //   instances  GInstance[] 144 B: inverse / forward GfMatrix4f, BLAS root, material, light.
-// The trees are the reference's midpoint-split BVH2 (leaf <= 4 triangles) and TLAS (leaf <= 2
-// instances), built on the host in the reference's order (source/renderContext/bvh.cpp:184-241,
-// source/scene/scene.cpp:352-407), so traversal order -- and with it the winner among equal-t
-// hits -- is the reference's.
+// The BLAS are the reference's midpoint-split BVH2 (leaf <= 4 triangles), built on the host in the
+// reference's order (source/renderContext/bvh.cpp:184-241), so a triangle's record index ranks it
+// as the reference's traversal would. The TLAS is a binned-SAH tree (one instance per leaf); the
+// reference's midpoint TLAS (source/scene/scene.cpp:352-407) is built too, only to give each
+// instance its rank, the slot it has there. Closest-hit ties are broken on (rank, record), which is
+// the reference's winner whatever tree is traversed (trace_closest_s).
 #pragma once
 
 #include <climits>
@@ -102,6 +105,7 @@ struct DeviceScene {
     const GTri *tris;
     const GTriAttr *attrs;
     const int *tlas_idx;
+    const int *tlas_rank;  // slot -> the instance's slot in the reference's TLAS (closest-hit tie key)
     const GInstance *instances;
     const GMaterial *materials;
     const GLight *lights;
@@ -370,12 +374,15 @@ constexpr int kNoNode = INT_MIN;
 // Scene::Intersect (scene.cpp:409-513) + BVH::_IntersectNode (bvh.cpp:251-305) return the closest
 // hit; among hits at the same t the reference keeps the one it meets first in its left-first
 // depth-first order (strict t < best at every level, instances in TLAS-leaf order). That order is
-// the order of the arrays: both builds partition a node's range into [left child, right child),
-// so TLAS slots and each BLAS's triangle records are laid out in left-first DFS order. This
-// traversal visits the nearer child first (a closer hit is found sooner and prunes more) and
-// keeps the reference's winner by breaking t ties on (TLAS slot, triangle record): the earliest
-// in the reference's order wins, whatever order the candidates are met in. A box is pruned when
-// its entry distance exceeds the best t (ties are still visited), at push and again at pop.
+// the order of the arrays: both of its builds partition a node's range into [left child, right
+// child), so its TLAS slots and each BLAS's triangle records are laid out in left-first DFS
+// order. This traversal visits the nearer child first (a closer hit is found sooner and prunes
+// more) and keeps the reference's winner by breaking t ties on (the instance's slot in the
+// reference's TLAS -- tlas_rank --, triangle record): the earliest in the reference's order wins,
+// whatever order the candidates are met in and whatever tree holds the instances. A box is pruned
+// when its entry distance exceeds the best t (ties are still visited), at push and again at pop;
+// a parent's box contains its children's and the slab test is monotone in the box, so every
+// triangle at or below the best t is tested.
 //
 // One loop serves both levels: a lane is either in the TLAS (world ray) or in an instance's BLAS
 // (object ray), and an inner node of either level runs the same code, so the lanes of a wave do not
@@ -398,8 +405,8 @@ __device__ __forceinline__ bool trace_closest_s(const DeviceScene &s, V3 o, V3 d
     }
     bool was = false, blas = false, world = true;
     int lnext = 0, lend = 0;  // instance slots of the current TLAS leaf still to traverse
-    int ii = -1, slot = -1;   // current instance and its TLAS slot
-    int bslot = 0x7fffffff;   // TLAS slot of the best hit (tie-break key with hit.tri)
+    int ii = -1, slot = -1;   // current instance and its rank (tlas_rank of its TLAS slot)
+    int bslot = 0x7fffffff;   // rank of the best hit's instance (tie-break key with hit.tri)
     V3 oo = o, od = d;
     for (;;) {
         if (cur >= 0) {  // inner node of either level
@@ -457,8 +464,8 @@ __device__ __forceinline__ bool trace_closest_s(const DeviceScene &s, V3 o, V3 d
             blas = false;  // the instance is done
         }
         while (lnext < lend) {  // the leaf's next instance
-            slot = lnext++;
-            ii = gload_i(s.tlas_idx + slot);
+            ii = gload_i(s.tlas_idx + lnext);
+            slot = gload_i(s.tlas_rank + lnext++);
             float minv[16];
             const int root = load_inst_inv(s.instances, ii, minv);
             if (kCount) ++st.instances;
@@ -669,8 +676,8 @@ __device__ __forceinline__ bool trace_closest_wave(const DeviceScene &s, bool ac
             blas = false;  // the instance is done
         }
         while (lnext < lend) {  // the leaf's next instance
-            slot = lnext++;
-            ii = *(const __attribute__((address_space(4))) int *)(s.tlas_idx + slot);
+            ii = *(const __attribute__((address_space(4))) int *)(s.tlas_idx + lnext);
+            slot = *(const __attribute__((address_space(4))) int *)(s.tlas_rank + lnext++);
             float minv[16];
             const int root = load_inst_inv_u(s.instances, ii, minv);
             if (root < 0) continue;

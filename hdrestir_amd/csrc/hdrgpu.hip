@@ -607,6 +607,7 @@ hdr_status place_state(hdr_scene *sc, const std::vector<char> &keep, const std::
         return o;
     };
     const size_t oTidx = place(h.tlas_idx.size() * sizeof(int));
+    const size_t oTrank = place(h.tlas_rank.size() * sizeof(int));
     const size_t oInst = place(h.instances.size() * sizeof(GInstance));
     const size_t oMat = place(h.materials.size() * sizeof(GMaterial));
     const size_t oLights = place(h.lights.size() * sizeof(GLight));
@@ -621,6 +622,7 @@ hdr_status place_state(hdr_scene *sc, const std::vector<char> &keep, const std::
         return bytes ? hipMemcpy(base + o, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
     };
     HIP_TRY(up(oTidx, h.tlas_idx.data(), h.tlas_idx.size() * sizeof(int)));
+    HIP_TRY(up(oTrank, h.tlas_rank.data(), h.tlas_rank.size() * sizeof(int)));
     HIP_TRY(up(oInst, h.instances.data(), h.instances.size() * sizeof(GInstance)));
     HIP_TRY(up(oMat, h.materials.data(), h.materials.size() * sizeof(GMaterial)));
     HIP_TRY(up(oLights, h.lights.data(), h.lights.size() * sizeof(GLight)));
@@ -679,6 +681,7 @@ hdr_status place_state(hdr_scene *sc, const std::vector<char> &keep, const std::
     DeviceScene &ds = sc->ds;
     ds.nodes = static_cast<const GNode *>(sc->nodes.p);
     ds.tlas_idx = reinterpret_cast<const int *>(base + oTidx);
+    ds.tlas_rank = reinterpret_cast<const int *>(base + oTrank);
     ds.instances = reinterpret_cast<const GInstance *>(base + oInst);
     ds.materials = reinterpret_cast<const GMaterial *>(base + oMat);
     ds.lights = reinterpret_cast<const GLight *>(base + oLights);

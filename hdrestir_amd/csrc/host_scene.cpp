@@ -343,6 +343,98 @@ std::string rebuild_host_state(const hdr_scene_desc &d, HostScene &out) {
 
 namespace {
 
+float half_area(const Box &b) {
+    if (b.empty()) return 0.0f;
+    const F3 e = b.size();
+    return e.x * e.y + e.y * e.z + e.z * e.x;
+}
+
+// Binned SAH over the instances idx[start, end) (16 bins per axis over the centroid bounds; cost of
+// a split = half-area(left) * n_left + half-area(right) * n_right), down to one instance per leaf;
+// where every centroid coincides, the range is halved. Children are allocated as a pair (right =
+// left + 1) before the recursion, as the midpoint builds do, so the node layout is the same.
+void sah_tlas(std::vector<GNode> &tl, std::vector<int> &idx, const std::vector<Box> &bounds,
+              const std::vector<F3> &cen, int nodeIdx, int start, int end, int dpt, int &maxd) {
+    maxd = std::max(maxd, dpt);
+    Box b, cb;
+    for (int i = start; i < end; ++i) {
+        b.extend(bounds[size_t(idx[size_t(i)])].mn);
+        b.extend(bounds[size_t(idx[size_t(i)])].mx);
+        cb.extend(cen[size_t(idx[size_t(i)])]);
+    }
+    const int count = end - start;
+    if (count <= 1) {
+        tl[size_t(nodeIdx)] = make_node(b, -start - 1, count);
+        return;
+    }
+    constexpr int kBins = 16;
+    int bestAxis = -1, bestBin = 0;
+    float bestCost = FLT_MAX;
+    const F3 ce = cb.size();
+    for (int axis = 0; axis < 3; ++axis) {
+        const float lo = comp(cb.mn, axis), ext = comp(ce, axis);
+        if (!(ext > 0.0f)) continue;
+        Box bb[kBins];
+        int bn[kBins] = {};
+        for (int i = start; i < end; ++i) {
+            const int k = idx[size_t(i)];
+            int bin = int(float(kBins) * ((comp(cen[size_t(k)], axis) - lo) / ext));
+            bin = std::min(kBins - 1, std::max(0, bin));
+            ++bn[bin];
+            bb[bin].extend(bounds[size_t(k)].mn);
+            bb[bin].extend(bounds[size_t(k)].mx);
+        }
+        float rightArea[kBins];
+        int rightN[kBins];
+        Box acc;
+        int n = 0;
+        for (int bin = kBins - 1; bin > 0; --bin) {
+            if (bn[bin]) {  // an empty bin's box is the empty sentinel: no corners to add
+                acc.extend(bb[bin].mn);
+                acc.extend(bb[bin].mx);
+            }
+            n += bn[bin];
+            rightArea[bin] = half_area(acc);
+            rightN[bin] = n;
+        }
+        acc = Box();
+        n = 0;
+        for (int bin = 1; bin < kBins; ++bin) {  // split between bin - 1 and bin
+            if (bn[bin - 1]) {
+                acc.extend(bb[bin - 1].mn);
+                acc.extend(bb[bin - 1].mx);
+            }
+            n += bn[bin - 1];
+            if (n == 0 || rightN[bin] == 0) continue;
+            const float cost = half_area(acc) * float(n) + rightArea[bin] * float(rightN[bin]);
+            if (cost < bestCost) {
+                bestCost = cost;
+                bestAxis = axis;
+                bestBin = bin;
+            }
+        }
+    }
+    int mid;
+    if (bestAxis < 0) {
+        mid = start + count / 2;
+    } else {
+        const float lo = comp(cb.mn, bestAxis), ext = comp(ce, bestAxis);
+        auto left = [&](int k) {
+            int bin = int(float(kBins) * ((comp(cen[size_t(k)], bestAxis) - lo) / ext));
+            bin = std::min(kBins - 1, std::max(0, bin));
+            return bin < bestBin;
+        };
+        mid = int(std::partition(idx.begin() + start, idx.begin() + end, left) - idx.begin());
+        if (mid == start || mid == end) mid = start + count / 2;
+    }
+    const int leftIdx = int(tl.size());
+    tl.push_back(GNode{});
+    tl.push_back(GNode{});
+    tl[size_t(nodeIdx)] = make_node(b, leftIdx, 0);
+    sah_tlas(tl, idx, bounds, cen, leftIdx, start, mid, dpt + 1, maxd);
+    sah_tlas(tl, idx, bounds, cen, leftIdx + 1, mid, end, dpt + 1, maxd);
+}
+
 std::string build_state(const hdr_scene_desc &d, const HostScene &geo, StateBuild &sb) {
     HostScene &out = sb.s;
     // ---- materials (+ default material record at index num_materials)
@@ -445,6 +537,9 @@ std::string build_state(const hdr_scene_desc &d, const HostScene &geo, StateBuil
         tl.reserve(size_t(d.num_instances) * 2);
         tl.push_back(GNode{});
         int tdepth = 0;
+        // The reference's TLAS (below) fixes which of two equally distant hits on different
+        // instances wins: the one in the earlier leaf slot. Its slots become the instances' tie
+        // ranks; the tree the device traverses is a binned-SAH one (sah_tlas).
         // Scene::_SubdivideTLAS, scene.cpp:352-407 (children allocated before the leaf test)
         struct Rec {
             static void go(std::vector<GNode> &tl, std::vector<int> &idx, const std::vector<Box> &bounds,
@@ -484,6 +579,13 @@ std::string build_state(const hdr_scene_desc &d, const HostScene &geo, StateBuil
             }
         };
         Rec::go(tl, idx, instBounds, instCentroid, 0, 0, d.num_instances, 1, tdepth);
+        std::vector<int> rank(size_t(d.num_instances));
+        for (int slot = 0; slot < d.num_instances; ++slot) rank[size_t(idx[size_t(slot)])] = slot;
+        for (int i = 0; i < d.num_instances; ++i) idx[size_t(i)] = i;
+        tl.clear();
+        tl.push_back(GNode{});
+        tdepth = 0;
+        sah_tlas(tl, idx, instBounds, instCentroid, 0, 0, d.num_instances, 1, tdepth);
         out.max_tlas_depth = tdepth;
         // odd base, as pad_to_odd places every tree
         const int base = geo.blas_nodes % 2 == 0 ? geo.blas_nodes + 1 : geo.blas_nodes;
@@ -493,6 +595,8 @@ std::string build_state(const hdr_scene_desc &d, const HostScene &geo, StateBuil
         }
         sb.tlas_base = base;
         out.tlas_idx = idx;
+        out.tlas_rank.resize(idx.size());
+        for (size_t slot = 0; slot < idx.size(); ++slot) out.tlas_rank[slot] = rank[size_t(idx[slot])];
         out.tlas_root = base;
     }
     if (out.max_tlas_depth > kStackDepth - 2) return "BVH deeper than the traversal stack";
@@ -561,6 +665,7 @@ void commit_state(StateBuild &sb, HostScene &out) {
     }
     HostScene &s = sb.s;
     out.tlas_idx = std::move(s.tlas_idx);
+    out.tlas_rank = std::move(s.tlas_rank);
     out.instances = std::move(s.instances);
     out.materials = std::move(s.materials);
     out.lights = std::move(s.lights);

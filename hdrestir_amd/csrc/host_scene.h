@@ -13,7 +13,8 @@ struct HostScene {
     std::vector<GNode> nodes;        // BLAS nodes of all meshes, then TLAS nodes (from blas_nodes on)
     std::vector<GTri> tris;
     std::vector<GTriAttr> attrs;
-    std::vector<int> tlas_idx;
+    std::vector<int> tlas_idx;       // TLAS leaf slot -> instance (in the traversed tree's leaf order)
+    std::vector<int> tlas_rank;      // TLAS leaf slot -> the instance's slot in the reference's tree
     std::vector<GInstance> instances;
     std::vector<GMaterial> materials; // num_materials records + the default material
     std::vector<GLight> lights;
