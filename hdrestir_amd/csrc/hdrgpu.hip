@@ -902,6 +902,9 @@ struct FramePlan {
     int depth = 0;          // path vertices (max_depth)
     size_t lds = 0;         // dynamic LDS of the shading / integration kernel
     unsigned tiles = 0;     // 8x8 tiles of the view
+    int prep_window = 0;      // wavefront: draws a shading launch can make on a path's stream (0: no
+                              // stream reaches output 624, no slot preparation)
+    int prep_vertex = 0;      // the first shaded vertex whose launch can reach output 624
     bool gbuf_reuse = false;  // the view's latest G-buffer is this frame's (no G-buffer launch)
     int gbuf_half = 0;        // the G-buffer half this frame reads (and, without reuse, writes)
 };
@@ -961,9 +964,10 @@ static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_
     const bool traceLastConnection = scene->ds.has_emission || scene->ds.environment >= 0;
     // upper bound of RNG draws per pixel; past 311 the engine needs its spill slots
     int bound = 5, bound0 = 5;  // bound0: through the primary vertex only
+    int perVertex = 0;
     {
         const int sb = scene->max_sample_draws_per_vertex;
-        int perVertex = 1 + sb + 1;  // opacity, bounce sample, roulette
+        perVertex = 1 + sb + 1;  // opacity, bounce sample, roulette
         if (!ris) perVertex += 3;
         else perVertex += nNee * 3 + std::max(0, nBsdf - 1) * sb + nCand + 1;
         bound += pipe->max_depth * perVertex;
@@ -1033,6 +1037,12 @@ static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_
     plan.deferred = deferred;
     plan.wave = wave;
     plan.wave_lean0 = wave && bound0 < kMtN - 1;
+    if (wave && bound >= 2 * kMtN) {  // streams may reach output 624: slots prepared ahead of it
+        // a shading launch draws a vertex on the stream and, on traced frames, the next vertex's
+        // generation on its copy; vertex u's stream starts at most at 5 + u * perVertex
+        plan.prep_window = (waveTraced ? 2 : 1) * perVertex;
+        plan.prep_vertex = std::max(0, (2 * kMtN - 5 - plan.prep_window + perVertex - 1) / perVertex);
+    }
     plan.wave_traced = waveTraced;
     plan.depth = pipe->max_depth;
     plan.lean_pt = !deferred && !ris && !scene->any_translucent && bound < kMtN - 1;
@@ -1328,6 +1338,10 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         const int launches = plan.wave_traced ? plan.depth + 1 : plan.depth;
         for (int b = 0; b < launches; ++b) {
             if (b > 0) {
+                const int u = plan.wave_traced ? b - 1 : b;  // the vertex the launch shades
+                if (plan.prep_window > 0 && u >= plan.prep_vertex)
+                    hipLaunchKernelGGL(wave_prepare_slots_kernel, dim3(qblocks), dim3(kLanes), 0, view->stream, da, u,
+                                       plan.prep_window);
                 if (view->counting) {
                     if (debug_out) HDR_WS(true, true);
                     else HDR_WS(true, false);
@@ -1944,6 +1958,8 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
     int waveDepth[2] = {0, 0};
     bool waveLean0[2] = {true, true};  // every view's primary vertex stays below RNG output 311
     bool waveSpatial[2] = {false, false};  // some view of the group reuses spatially
+    int wavePrepWindow[2] = {0, 0};        // slot preparation (FramePlan::prep_window): the largest window
+    int wavePrepVertex[2] = {INT_MAX, INT_MAX};  // and the first vertex that needs it, over the group
     bool oneScene = true;                  // every view renders the first view's scene
     bool ldsFit = true;  // every scene's any-hit stacks fit in LDS and no view launches a G-buffer
     for (int i = 0; i < n; ++i) {
@@ -1959,6 +1975,10 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
             escCap += plans[i].escapes;
             waveDepth[t] = std::max(waveDepth[t], plans[i].depth);
             waveLean0[t] = waveLean0[t] && plans[i].wave_lean0;
+            if (plans[i].prep_window > 0) {
+                wavePrepWindow[t] = std::max(wavePrepWindow[t], plans[i].prep_window);
+                wavePrepVertex[t] = std::min(wavePrepVertex[t], plans[i].prep_vertex);
+            }
             waveSpatial[t] = waveSpatial[t] || plans[i].spatial;
         }
     }
@@ -2094,7 +2114,13 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
             // post-temporal reservoirs are stored by launch 1
             const int launches = t == 1 ? waveDepth[t] + 1 : waveDepth[t];
             for (int v = 0; v < launches; ++v) {
-                if (v > 0) launch_wave_shade<false, false>(ls, qblocks, lds[wg], b->stream, A(wg), v, t == 1);
+                if (v > 0) {
+                    const int u = t == 1 ? v - 1 : v;  // the vertex the launch shades
+                    if (wavePrepWindow[t] > 0 && u >= wavePrepVertex[t])
+                        hipLaunchKernelGGL(wave_prepare_slots_kernel, dim3(qblocks), dim3(kLanes), 0, b->stream, A(wg),
+                                           u, wavePrepWindow[t]);
+                    launch_wave_shade<false, false>(ls, qblocks, lds[wg], b->stream, A(wg), v, t == 1);
+                }
                 if (waveSpatial[t] && v == t)
                     launch_wave_spatial<false>(spatialNd[wg], dim3(tiles[wg]), b->stream, A(wg), T(wg), N(wg));
                 launch_wave_trace<false>(ls, qblocks, b->stream, A(wg), v, t == 1, b->ctx->cus, oneScene, ldsFit);
@@ -2123,6 +2149,8 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
                                      uint64_t(queueCap), uint64_t(waveCap), uint64_t(waveDepth[0]),
                                      uint64_t(waveDepth[1]), uint64_t(waveLean0[0]), uint64_t(waveLean0[1]),
                                      uint64_t(waveSpatial[0]), uint64_t(waveSpatial[1]),
+                                     uint64_t(wavePrepWindow[0]), uint64_t(wavePrepWindow[1]),
+                                     uint64_t(uint32_t(wavePrepVertex[0])), uint64_t(uint32_t(wavePrepVertex[1])),
                                      uint64_t(ls.queue_waves), uint64_t(ls.trace_occ), uint64_t(ls.shade_occ),
                                      uint64_t(ls.lds_stack), uint64_t(oneScene), uint64_t(ldsFit)};
         for (int g = 0; g < kGroups; ++g) {

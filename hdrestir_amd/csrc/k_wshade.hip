@@ -57,6 +57,34 @@ wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce) {
     }
 }
 
+// Before a shading launch: each live path of vertex u whose stream can reach output 624 within
+// `window` draws, and whose spill slot is not prepared yet, gets the slot's state at output 624 (the
+// third twist, mt_prepare_third) now. Built lazily at output 624, that state took one pass of ~1250
+// dependent global-memory steps per lane group reaching it, and the lanes of a wave reach it at
+// different draws, so a few waves ran for milliseconds alone at the end of deep frames. Here every
+// lane of a wave builds its slot in the same pass, with out-of-place twists whose chunks of loads
+// are in flight together; the record's ready bit tells the stream. Paths whose streams never come
+// close are only read (two fields of the record).
+__global__ void __launch_bounds__(kLanes) wave_prepare_slots_kernel(const FrameArgs *__restrict__ args, int u,
+                                                                    int window) {
+    const unsigned int n = args[0].wcnt[4 + (u & 1)];
+    for (unsigned int i = blockIdx.x * kLanes + threadIdx.x; i < n; i += gridDim.x * kLanes) {
+        const uint32_t code = alive_list(args[0], u)[i];
+        const FrameArgs &a = args[code >> 24];
+        const int pixel = int(code & 0xffffffu);
+        uint64_t *slot = wave_slot(a, pixel);
+        if (!slot) continue;
+        const size_t P = wave_pixels(a);
+        const uint32_t k = __float_as_uint(a.wstate[2 * P + pixel].z);
+        float *bitsp = &a.wstate[7 * P + pixel].x;
+        const uint32_t bits = __float_as_uint(*bitsp);
+        if ((bits & kWaveSlotReady) || k >= 624u || k + uint32_t(window) < 624u) continue;
+        const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
+        mt_prepare_third(slot, slot + kMtN, seed);
+        *bitsp = __uint_as_float(bits | kWaveSlotReady);
+    }
+}
+
 // ---------------------------------------------------------------------------- instantiations
 // (launched from hdrgpu.hip, which sees the declarations in kernels.h)
 template __global__ void wave_shade_kernel<true, true, 1>(const FrameArgs *__restrict__, int);

@@ -56,6 +56,7 @@ template <bool kCount, bool kDebug, int kOcc>
 __global__ void wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce);
 template <bool kCount, bool kDebug, int kOcc>
 __global__ void wave_shade_traced_kernel(const FrameArgs *__restrict__ args, int v);
+__global__ void wave_prepare_slots_kernel(const FrameArgs *__restrict__ args, int u, int window);
 
 // k_post.hip
 __global__ void chroma_blur_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst, int W, int H);

@@ -57,12 +57,14 @@ struct Mt64 {
     uint32_t seed;
     uint32_t k;                 // outputs produced so far
     uint32_t j;                 // cursor position
+    bool ready;                 // Mt64Deep: the slot already holds the third twist (mt_prepare_third)
 
     HD void init(uint32_t sd, uint64_t *spill) {
         seed = sd;
         slot = spill;
         k = 0;
         j = 0;
+        ready = false;
         lo = uint64_t(sd);
         lo1 = mt_step(lo, 1);
         uint64_t x = lo1;
@@ -222,13 +224,17 @@ struct Mt64Deep : Mt64 {
         }
     }
     // Output 624 and beyond: the spill slot holds the state after the third twist, twisted again
-    // every 312 outputs.
+    // every 312 outputs. A slot prepared ahead of output 624 (ready) is used as it is.
     HD uint64_t third() {
         if (!slot) return 0;  // the host allocates slots whenever a pixel can draw this far
         const uint32_t idx = (k - 624u) % uint32_t(kMtN);
         if (idx == 0u) {
-            if (k == 624u) materialise_second();
-            twist_in_place(slot);
+            if (k != 624u) {
+                twist_in_place(slot);
+            } else if (!ready) {
+                materialise_second();
+                twist_in_place(slot);
+            }
         }
         return slot[idx];
     }
@@ -250,8 +256,59 @@ static __host__ __device__ __noinline__ void mt_deep_discard(Mt64Deep &r) {
     else (void)r.third();
     ++r.k;
 }
-static __host__ __device__ __noinline__ void mt_copy_slot(uint64_t *dst, const uint64_t *src) {
+static __host__ __device__ __noinline__ void mt_copy_slot(uint64_t *__restrict__ dst,
+                                                          const uint64_t *__restrict__ src) {
+#pragma unroll 8
     for (int q = 0; q < kMtN; ++q) dst[q] = src[q];
+}
+
+// One twist from `in` to `out` (two distinct slots): the in-place twist's recurrence with every
+// old word read from `in`, so the loads of a chunk do not wait on the chunk's stores. Chunks of 8
+// words: loads first, then the words.
+HD void mt_twist_into(uint64_t *__restrict__ out, const uint64_t *__restrict__ in) {
+    constexpr int kC = 8;
+    for (int q0 = 0; q0 < kMtN - kMtM; q0 += kC) {  // 156 = 19 chunks + 4
+        uint64_t a[kC + 1], b[kC];
+        const int n = (kMtN - kMtM - q0) < kC ? (kMtN - kMtM - q0) : kC;
+#pragma unroll
+        for (int c = 0; c <= kC; ++c)
+            if (c <= n) a[c] = in[q0 + c];
+#pragma unroll
+        for (int c = 0; c < kC; ++c)
+            if (c < n) b[c] = in[q0 + c + kMtM];
+#pragma unroll
+        for (int c = 0; c < kC; ++c)
+            if (c < n) out[q0 + c] = b[c] ^ mt_twist(a[c], a[c + 1]);
+    }
+    for (int q0 = kMtN - kMtM; q0 < kMtN - 1; q0 += kC) {  // 155 = 19 chunks + 3
+        uint64_t a[kC + 1], b[kC];
+        const int n = (kMtN - 1 - q0) < kC ? (kMtN - 1 - q0) : kC;
+#pragma unroll
+        for (int c = 0; c <= kC; ++c)
+            if (c <= n) a[c] = in[q0 + c];
+#pragma unroll
+        for (int c = 0; c < kC; ++c)
+            if (c < n) b[c] = out[q0 + c + kMtM - kMtN];  // new words, written by the first half
+#pragma unroll
+        for (int c = 0; c < kC; ++c)
+            if (c < n) out[q0 + c] = b[c] ^ mt_twist(a[c], a[c + 1]);
+    }
+    out[kMtN - 1] = out[kMtM - 1] ^ mt_twist(in[kMtN - 1], out[0]);
+}
+
+// The spill slot's state at output 624 (the third twist) computed ahead of time: the seeding words
+// into `scratch`, then three twists ping-ponging scratch -> slot -> scratch -> slot. Equal to what
+// Mt64Deep::third computes lazily at output 624 (materialise_second, then twist_in_place).
+HD void mt_prepare_third(uint64_t *__restrict__ slot, uint64_t *__restrict__ scratch, uint32_t seed) {
+    uint64_t x = uint64_t(seed);
+    scratch[0] = x;
+    for (int i = 1; i < kMtN; ++i) {
+        x = mt_step(x, uint32_t(i));
+        scratch[i] = x;
+    }
+    mt_twist_into(slot, scratch);
+    mt_twist_into(scratch, slot);
+    mt_twist_into(slot, scratch);
 }
 
 HD uint64_t Mt64Deep::next_raw() {

@@ -37,7 +37,7 @@ namespace hdr {
 // copy works on the spare slot.
 __device__ __forceinline__ Mt64Deep rng_fork(const Mt64Deep &r, uint64_t *slot2) {
     Mt64Deep c = r;
-    if (r.k > 624u && slot2 && r.slot) mt_copy_slot(slot2, r.slot);
+    if ((r.k > 624u || r.ready) && slot2 && r.slot) mt_copy_slot(slot2, r.slot);  // a prepared slot goes along
     c.slot = slot2;
     return c;
 }
@@ -47,6 +47,7 @@ __device__ __forceinline__ Mt64Deep rng_fork(const Mt64Deep &r, uint64_t *slot2)
 // of a wave's paths is one coalesced 1 KB access.
 constexpr int kWaveFields = 9;
 constexpr uint32_t kWavePrimaryHit = 1u << 31;  // in field 7.x: the G-buffer ray hit
+constexpr uint32_t kWaveSlotReady = 1u << 30;   // in field 7.x: the spill slot holds the third twist
 
 // R: Mt64Deep, or Mt64Lean on the primary vertex when the host has checked that no pixel can draw
 // past output 310 there (the two share Mt64's cursor state, so the record is the same).
@@ -93,7 +94,7 @@ __device__ __forceinline__ void store_path(const FrameArgs &a, int pixel, const 
     w[5 * P + pixel] = make_float4(p.dir.x, p.dir.y, p.dir.z, p.utab);
     if (outer) w[6 * P + pixel] = spec4(p.outer);
     const uint32_t bits = (uint32_t(p.refl) & 0x7fffu) | ((uint32_t(p.refr) & 0x7fffu) << 15) |
-                          (p.primary_hit ? kWavePrimaryHit : 0u);
+                          (p.primary_hit ? kWavePrimaryHit : 0u) | (p.rng.ready ? kWaveSlotReady : 0u);
     w[7 * P + pixel] = make_float4(__uint_as_float(bits), __uint_as_float(p.dbg.lhash), __uint_as_float(p.dbg.shash),
                                    __int_as_float(p.dbg.picks));
     if constexpr (std::is_same<R, Mt64Deep>::value)
@@ -125,6 +126,7 @@ __device__ __forceinline__ void load_path(const FrameArgs &a, int pixel, WavePat
     p.refl = int(bits & 0x7fffu);
     p.refr = int((bits >> 15) & 0x7fffu);
     p.primary_hit = (bits & kWavePrimaryHit) != 0;
+    p.rng.ready = (bits & kWaveSlotReady) != 0;
     p.dbg.lhash = __float_as_uint(r7.y);
     p.dbg.shash = __float_as_uint(r7.z);
     p.dbg.picks = __float_as_int(r7.w);

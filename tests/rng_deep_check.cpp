@@ -1,6 +1,8 @@
 // Host check of the device RNG engines against libstdc++ std::mt19937_64: every output, random
 // patterns of draws and discards, and replay copies (forks) taken at random points, for streams
-// long enough to cross the first, second and third twists. Prints "ok" or the first mismatch.
+// long enough to cross the first, second and third twists; and streams whose spill slot is
+// prepared ahead of output 624 (mt_prepare_third, at a random point, forks carrying it along).
+// Prints "ok" or the first mismatch.
 #include <cstdio>
 #include <random>
 #include <vector>
@@ -9,9 +11,10 @@
 
 using namespace hdr;
 
+// wavefront.h: rng_fork
 static Mt64Deep fork_of(const Mt64Deep &r, uint64_t *slot2) {
     Mt64Deep c = r;
-    if (r.k > 624u) mt_copy_slot(slot2, r.slot);
+    if (r.k > 624u || r.ready) mt_copy_slot(slot2, r.slot);
     c.slot = slot2;
     return c;
 }
@@ -37,12 +40,32 @@ int main() {
                 return 1;
             }
         }
-        // draws and discards in a pseudo-random pattern, with forks replayed against the reference
+        // the prepared slot equals the lazily built one at output 624
+        {
+            std::vector<uint64_t> lazy(2 * kMtN), prep(2 * kMtN);
+            Mt64Deep a;
+            a.init(seed, lazy.data());
+            for (int i = 0; i <= 624; ++i) (void)a.next_raw();
+            mt_prepare_third(prep.data(), prep.data() + kMtN, seed);
+            for (int q = 0; q < kMtN; ++q)
+                if (prep[size_t(q)] != lazy[size_t(q)]) {
+                    std::printf("seed %u prepared slot word %d differs\n", seed, q);
+                    return 1;
+                }
+        }
+        // draws and discards in a pseudo-random pattern, with forks replayed against the reference;
+        // odd repetitions prepare the slot at a random point before output 624
         uint64_t lcg = seed * 2862933555777941757ULL + 3037000493ULL;
         for (int rep = 0; rep < 20; ++rep) {
             Mt64Deep r;
             r.init(seed, slots.data());
+            lcg = lcg * 6364136223846793005ULL + 1442695040888963407ULL;
+            const int prepAt = (rep & 1) ? 300 + int((lcg >> 33) % 324u) : -1;
             for (int i = 0; i < kDraws; ++i) {
+                if (i == prepAt) {
+                    mt_prepare_third(r.slot, slots.data() + kMtN, seed);
+                    r.ready = true;
+                }
                 lcg = lcg * 6364136223846793005ULL + 1442695040888963407ULL;
                 const int op = int((lcg >> 33) % 7u);
                 if (op < 3) {
