@@ -130,7 +130,8 @@ def config0_measurement(gpu, abi, ctx, cpu=True, cpu_frames=5):
     """Config 0: the box of 12 triangles with two icospheres (4840 triangles) and one rect light at
     256x256, 16 spp (frames 0 .. 15), depth 4: PathTracer (MIS) and RIS with M = 16 (8 light + 8
     BSDF candidates, visibility tested, without and with the reservoir). GPU: frames/s (= spp/s) over
-    the 16 frames after an untimed pass; CPU: the oracle on whole frames, median of `cpu_frames`
+    the 16 frames after an untimed pass, frame by frame, and the 16-spp image through
+    hdr_render_frames; CPU: the oracle on whole frames, median of `cpu_frames`
     frames after one warm-up."""
     from hdrestir_amd import scenes
     sc = scenes.box_spheres(256, 256, frames=16)
@@ -151,6 +152,16 @@ def config0_measurement(gpu, abi, ctx, cpu=True, cpu_frames=5):
             ms = (time.perf_counter() - t0) * 1000.0 / 16
         out[name] = {"gpu_ms_per_frame": round(ms, 4), "gpu_frames_per_s": round(1000.0 / ms, 2),
                      "gpu_image_ms_16spp": round(16 * ms, 3)}
+        # the same 16 frames of the static view through hdr_render_frames: frames that keep no state
+        # from frame to frame share launches (ReSTIR's reservoirs do, so it renders frame by frame)
+        for _ in range(2):
+            r.reset()
+            r.synchronize()
+            t0 = time.perf_counter()
+            r.render_frames(sc.frame(0), pipe, 16)
+            r.synchronize()
+            ims = (time.perf_counter() - t0) * 1000.0
+        out[name]["gpu_image_ms_16spp_render_frames"] = round(ims, 3)
     r.close()
     gs.close()
     if cpu:

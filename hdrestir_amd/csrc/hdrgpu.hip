@@ -85,6 +85,18 @@ __global__ void wave_zero_counts_kernel(unsigned int *c) {
 
 // Out-of-place 3x3 firefly clamp (the reference's GPU accumulation shader semantics,
 // accumulation.slang:55-81; threshold 4 * max neighbour luminance + 0.5) + accumulation.
+// AccumulationPass (accumulation_pass.cpp:98-106) once per frame over `frames` staged samples of
+// consecutive frames, in frame order: the sums, the metrics and the final colour of that many
+// frame-by-frame accumulations.
+__global__ void accumulate_frames_kernel(const float4 *stage, int frames, size_t P, int first_frame, float4 *accum,
+                                         float4 *color, double *lum_sum, double *lum_sq) {
+    const size_t pixel = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (pixel >= P) return;
+    for (int f = 0; f < frames; ++f)
+        accumulate_sample(accum, color, lum_sum, lum_sq, int(pixel), stage[size_t(f) * P + pixel],
+                          1.0f / float(first_frame + f + 1));
+}
+
 __global__ void firefly_accumulate_kernel(const float4 *frame, float4 *accum, float4 *color, double *lum_sum,
                                           double *lum_sq, int W, int H, float invN) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1105,6 +1117,7 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
     a.light_select_pdf = scene->ds.num_lights > 0 ? 1.0f / float(scene->ds.num_lights) : 0.0f;
     a.firefly = pipe->enable_firefly_filter != 0;
     a.inv_n = 1.0f / float(frame->frame_index + 1);
+    a.frames = 1;
     a.res = static_cast<hdr_reservoir *>(view->res.p);
     a.status = static_cast<unsigned int *>(view->status.p);
     a.accum = static_cast<float4 *>(view->accum.p);
@@ -1217,13 +1230,21 @@ hdr_status hdr_view_plan_bytes(const hdr_scene *scene, int32_t width, int32_t he
     return HDR_OK;
 }
 
-hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
-                            const hdr_pipeline_desc *pipe, float *color_out, hdr_pixel_debug *debug_out) {
+// One frame, or `frames` consecutive frames of a static view in one launch (multi_frame_ok:
+// megakernel or lean path tracer frames that keep no state from frame to frame).
+static hdr_status render_frames_impl(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
+                                     const hdr_pipeline_desc *pipe, float *color_out, hdr_pixel_debug *debug_out,
+                                     int frames) {
     FrameArgs a;
     FramePlan plan;
     if (hdr_status st = prepare_frame(view, scene, frame, pipe, debug_out != nullptr, a, plan)) return st;
     const int W = view->W, H = view->H;
     const size_t P = size_t(W) * H;
+    if (frames > 1) {  // the frames' samples are staged, then accumulated in frame order
+        if (hdr_status st = ensure(view->frame, size_t(frames) * P * sizeof(float4))) return st;
+        a.frame_rgb = static_cast<float4 *>(view->frame.p);
+        a.frames = frames;
+    }
     const bool deferred = plan.deferred, spatial = plan.spatial;
     const size_t lds = plan.lds;
     const int gp = plan.gbuf_half;  // this frame's G-buffer half (prepare_frame chose it)
@@ -1383,7 +1404,7 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_ACCUMULATE], view->stream));
         ts.ran |= (1u << HDR_STAGE_INTEGRATE) | (1u << HDR_STAGE_ACCUMULATE);
     } else if (plan.lean_pt) {
-#define HDR_PT(C, D) hipLaunchKernelGGL((pt_lean_kernel<C, D>), grid, dim3(kLanes), 0, view->stream, da, dt, 1)
+#define HDR_PT(C, D) hipLaunchKernelGGL((pt_lean_kernel<C, D>), dim3(plan.tiles, frames), dim3(kLanes), 0, view->stream, da, dt, 1)
         if (view->counting) {
             if (debug_out) HDR_PT(true, true);
             else HDR_PT(true, false);
@@ -1397,9 +1418,15 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
         else hipLaunchKernelGGL((integrate_kernel<true, false>), grid, dim3(kLanes), lds, view->stream, da, dt, 1);
     } else {
         if (debug_out) hipLaunchKernelGGL((integrate_kernel<false, true>), grid, dim3(kLanes), lds, view->stream, da, dt, 1);
-        else hipLaunchKernelGGL((integrate_kernel<false, false>), grid, dim3(kLanes), lds, view->stream, da, dt, 1);
+        else hipLaunchKernelGGL((integrate_kernel<false, false>), dim3(plan.tiles, frames), dim3(kLanes), lds, view->stream,
+                                da, dt, 1);
     }
     HIP_TRY(hipGetLastError());
+    if (frames > 1) {
+        hipLaunchKernelGGL(accumulate_frames_kernel, dim3(unsigned((P + 255) / 256)), dim3(256), 0, view->stream,
+                           a.frame_rgb, frames, P, frame->frame_index, a.accum, a.color, a.lum_sum, a.lum_sq);
+        HIP_TRY(hipGetLastError());
+    }
     if (!deferred && !plan.wave) {
         ts.ran |= 1u << HDR_STAGE_INTEGRATE;
         HIP_TRY(hipEventRecord(ts.end[HDR_STAGE_INTEGRATE], view->stream));
@@ -1421,7 +1448,7 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     view->ring_last = view->ring_next;
     view->ring_next = (view->ring_next + 1) % hdr_view::kRing;
     view->frames++;
-    view->samples = frame->frame_index + 1;
+    view->samples = frame->frame_index + frames;
     if (!plan.gbuf_reuse) {
         view->gcur = gp;
         view->gvalid = view->gnext_valid;
@@ -1437,6 +1464,48 @@ hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_fr
     if (color_out || debug_out) {
         HIP_TRY(hipStreamSynchronize(view->stream));
         return view_status(view);
+    }
+    return HDR_OK;
+}
+
+hdr_status hdr_render_frame(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
+                            const hdr_pipeline_desc *pipe, float *color_out, hdr_pixel_debug *debug_out) {
+    return render_frames_impl(view, scene, frame, pipe, color_out, debug_out, 1);
+}
+
+// Frames that can share a launch: the megakernel or the lean path tracer without spill slots (each
+// frame's pixels then use nothing per pixel but their own sample), no state carried from frame to
+// frame (MIS, or RIS without the reservoir; no spatial reuse), no firefly filter (its clamp reads
+// each frame's neighbourhood), no depth of field (each frame's lens draws make its own G-buffer),
+// and no counting.
+static bool multi_frame_ok(const hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
+                           const hdr_pipeline_desc *pipe) {
+    FramePlan plan;
+    FrameNeeds need;
+    if (plan_pipeline(scene, view->W, view->H, pipe, false, plan, need) != HDR_OK) return false;
+    if (plan.deferred || plan.wave || plan.spill) return false;
+    if (pipe->enable_firefly_filter || pipe->spatial_count > 0) return false;
+    if (pipe->direct_light == HDR_DIRECT_RIS && pipe->use_reservoir) return false;
+    if (frame->has_camera_params && frame->camera.enable_dof) return false;
+    return !view->counting;
+}
+
+hdr_status hdr_render_frames(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
+                             const hdr_pipeline_desc *pipe, int32_t count) {
+    if (!view || !scene || !frame || !pipe) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
+    if (count < 0) return fail(HDR_ERR_INVALID_ARGUMENT, "count must be >= 0");
+    if (frame->struct_size != sizeof(hdr_frame_desc)) return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_frame_desc.struct_size mismatch");
+    if (pipe->struct_size != sizeof(hdr_pipeline_desc)) return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_pipeline_desc size");
+    const size_t P = size_t(view->W) * view->H;
+    // frames per launch: the staged samples within 256 MiB, at most 64
+    const int maxF = int(std::max<size_t>(1, std::min<size_t>(64, (size_t(256) << 20) / (P * sizeof(float4)))));
+    const bool multi = count > 1 && maxF > 1 && multi_frame_ok(view, scene, frame, pipe);
+    hdr_frame_desc f = *frame;
+    for (int done = 0; done < count;) {
+        const int n = multi ? std::min(count - done, maxF) : 1;
+        f.frame_index = frame->frame_index + done;
+        if (hdr_status st = render_frames_impl(view, scene, &f, pipe, nullptr, nullptr, n)) return st;
+        done += n;
     }
     return HDR_OK;
 }
@@ -1624,20 +1693,18 @@ hdr_status hdr_render_progressive(hdr_view *view, const hdr_scene *scene, const 
     if (!view || !scene || !frame || !pipe || !converged) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
     if (max_frames < 0) return fail(HDR_ERR_INVALID_ARGUMENT, "max_frames must be >= 0");
     if (frame->struct_size != sizeof(hdr_frame_desc)) return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_frame_desc.struct_size mismatch");
-    hdr_frame_desc f = *frame;
-    for (int k = 0; k < max_frames && view->samples < target_samples; ++k) {
-        if (view->ctx->cancelled) {
-            *converged = 0;
-            return fail(HDR_ERR_CANCELLED, "cancelled");
-        }
-        f.frame_index = view->samples;
-        if (hdr_status s = hdr_render_frame(view, scene, &f, pipe, nullptr, nullptr)) {
-            *converged = view->samples >= target_samples ? 1 : 0;
-            return s;
-        }
+    if (view->ctx->cancelled && max_frames > 0 && view->samples < target_samples) {
+        *converged = 0;
+        return fail(HDR_ERR_CANCELLED, "cancelled");
     }
+    // the call's frames go through hdr_render_frames, which shares launches among frames that keep
+    // no state from one to the next (the cancel flag is read before each launch)
+    hdr_frame_desc f = *frame;
+    f.frame_index = view->samples;
+    const int n = int(std::min<int64_t>(max_frames, std::max<int64_t>(0, int64_t(target_samples) - view->samples)));
+    const hdr_status s = hdr_render_frames(view, scene, &f, pipe, n);
     *converged = view->samples >= target_samples ? 1 : 0;
-    return HDR_OK;
+    return s;
 }
 
 hdr_status hdr_view_set_metrics(hdr_view *view, int32_t enable) {

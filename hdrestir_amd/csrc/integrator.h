@@ -35,6 +35,7 @@ struct FrameArgs {
     float light_select_pdf;            // 1 / number of lights
     int firefly;
     float inv_n;                       // 1 / (frame_index + 1)
+    int frames;                        // frames of this launch (megakernel / lean path tracer; else 1)
     hdr_reservoir *res;                // persistent reservoirs (read by the temporal merge, written last)
     hdr_reservoir *res_t;              // spatial reuse: this frame's post-temporal reservoirs
     float4 *spatial_aux;               // spatial reuse: per pixel {ray dir, wavelength u}, {throughput}

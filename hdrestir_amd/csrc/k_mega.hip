@@ -3,12 +3,14 @@
 
 namespace hdr {
 
-// IntegrationPass (integration_pass.cpp:208-243) + fused accumulation for one pixel.
+// IntegrationPass (integration_pass.cpp:208-243) + fused accumulation for one pixel. fidx: the
+// frame index (seed); stage: the frame's sample goes there instead of into the accumulation (a
+// launch of several frames, accumulated in frame order afterwards), or null.
 template <bool kCount, bool kDebug>
 __device__ __forceinline__ void integrate_pixel(const FrameArgs &a, const GRec *gbuf, int x, int y, double *wlane,
-                                                PixelStats &st, uint32_t &draws) {
+                                                PixelStats &st, uint32_t &draws, int fidx, float4 *stage) {
     const int pixel = y * a.W + x;
-    const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
+    const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)fidx * 12345ULL);
     Tracer<kCount, false> tracer{a.s, st};
     HitRec g;
     const bool gHit = load_grec(gbuf + pixel, g);
@@ -28,7 +30,9 @@ __device__ __forceinline__ void integrate_pixel(const FrameArgs &a, const GRec *
     V3 rgb = spec2rgb(tab, L);
     if (a.has_cam && a.cam.enable_exposure) rgb = rgb * a.exposure_mul;
     const float4 c4 = make_float4(rgb.x, rgb.y, rgb.z, 1.0f);
-    if (a.firefly) {
+    if (stage) {
+        stage[pixel] = c4;
+    } else if (a.firefly) {
         a.frame_rgb[pixel] = c4;
     } else {
         // AccumulationPass (accumulation_pass.cpp:98-106): accum += c; out = accum * invN
@@ -52,7 +56,8 @@ __device__ __forceinline__ void integrate_pixel(const FrameArgs &a, const GRec *
     flag_rng_overrun(a, !slot && rng.k > 312u);  // outputs from 312 on need the spill slot
 }
 
-// IntegrationPass + (fused) AccumulationPass for one 8x8 pixel tile per wave.
+// IntegrationPass + (fused) AccumulationPass for one 8x8 pixel tile per wave. Block row blockIdx.y
+// renders frame frame_index + blockIdx.y of a launch of a.frames frames.
 template <bool kCount, bool kDebug>
 __global__ void __launch_bounds__(kLanes) integrate_kernel(const FrameArgs *__restrict__ args,
                                                            const int *__restrict__ tile_base, int n_views) {
@@ -60,7 +65,10 @@ __global__ void __launch_bounds__(kLanes) integrate_kernel(const FrameArgs *__re
     HDR_TILE(args, tile_base, n_views)
     PixelStats st = {};
     uint32_t draws = 0;
-    if (valid) integrate_pixel<kCount, kDebug>(a, a.gbuf, x, y, wl + lane, st, draws);
+    const int fo = int(blockIdx.y);
+    if (valid)
+        integrate_pixel<kCount, kDebug>(a, a.gbuf, x, y, wl + lane, st, draws, a.frame_index + fo,
+                                        a.frames > 1 ? a.frame_rgb + size_t(fo) * size_t(a.W) * size_t(a.H) : nullptr);
     if (kCount) {
         unsigned long long *c = a.counters;
         wave_add(c + 1, st.bounce);

@@ -3,14 +3,18 @@
 
 namespace hdr {
 
-// The path tracer on opaque frames whose pixels stay below RNG output 311 (pathlean.h).
+// The path tracer on opaque frames whose pixels stay below RNG output 311 (pathlean.h). Block row
+// blockIdx.y renders frame frame_index + blockIdx.y of a launch of a.frames frames.
 template <bool kCount, bool kDebug>
 __global__ void __launch_bounds__(kLanes) pt_lean_kernel(const FrameArgs *__restrict__ args,
                                                          const int *__restrict__ tile_base, int n_views) {
     HDR_TILE(args, tile_base, n_views)
     PixelStats st = {};
     uint32_t draws = 0;
-    if (valid) pt_lean_pixel<kCount, kDebug>(a, x, y, st, draws);
+    const int fo = int(blockIdx.y);
+    if (valid)
+        pt_lean_pixel<kCount, kDebug>(a, x, y, st, draws, a.frame_index + fo,
+                                      a.frames > 1 ? a.frame_rgb + size_t(fo) * size_t(a.W) * size_t(a.H) : nullptr);
     if (kCount) {
         unsigned long long *c = a.counters;
         wave_add(c + 1, st.bounce);

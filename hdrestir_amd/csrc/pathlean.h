@@ -106,11 +106,14 @@ __device__ __forceinline__ Spec mis_direct_lean(const FrameArgs &a, const SpecTa
 
 // PathIntegrator::Li for one pixel, as path_li (integrator.h) with MIS direct lighting; the first
 // vertex is iteration 0 of the bounce loop (its roulette test can never fire, and 1 * x = x).
+// fidx: the frame index (seed); stage: the frame's sample goes there instead of into the
+// accumulation (a launch of several frames, accumulated in frame order afterwards), or null.
 template <bool kCount, bool kDebug>
-__device__ __forceinline__ void pt_lean_pixel(const FrameArgs &a, int x, int y, PixelStats &st, uint32_t &draws) {
+__device__ __forceinline__ void pt_lean_pixel(const FrameArgs &a, int x, int y, PixelStats &st, uint32_t &draws,
+                                              int fidx, float4 *stage) {
     const DeviceScene &s = a.s;
     const int pixel = y * a.W + x;
-    const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)a.frame_index * 12345ULL);
+    const uint32_t seed = uint32_t(pixel) ^ uint32_t((unsigned long long)(long long)fidx * 12345ULL);
     HitRec g;
     const bool gHit = load_grec(a.gbuf + pixel, g);
     Mt64Lean rng;
@@ -170,7 +173,9 @@ __device__ __forceinline__ void pt_lean_pixel(const FrameArgs &a, int x, int y, 
     V3 rgb = spec2rgb(tab, L);
     if (a.has_cam && a.cam.enable_exposure) rgb = rgb * a.exposure_mul;
     const float4 c4 = make_float4(rgb.x, rgb.y, rgb.z, 1.0f);
-    if (a.firefly) {
+    if (stage) {
+        stage[pixel] = c4;
+    } else if (a.firefly) {
         a.frame_rgb[pixel] = c4;
     } else {  // AccumulationPass (accumulation_pass.cpp:98-106)
         accumulate_sample(a.accum, a.color, a.lum_sum, a.lum_sq, pixel, c4, a.inv_n);

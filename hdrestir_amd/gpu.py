@@ -107,6 +107,7 @@ def lib():
             "hdr_view_sample_count": [P, C.POINTER(I)],
             "hdr_view_resolve": [P, I, I, I, I, I, P],
             "hdr_view_post_process": [P, C.POINTER(abi.PostDesc)],
+            "hdr_render_frames": [P, P, C.POINTER(abi.FrameDesc), C.POINTER(abi.PipelineDesc), I],
             "hdr_render_progressive": [P, P, C.POINTER(abi.FrameDesc), C.POINTER(abi.PipelineDesc), I, I,
                                        C.POINTER(I)],
             "hdr_view_synchronize": [P],
@@ -269,6 +270,10 @@ class Renderer:
                                      _p(self.color) if read_color else None,
                                      _p(self.debug) if debug else None))
         return self.color
+
+    def render_frames(self, frame, pipe, count):
+        """hdr_render_frames: frames frame.frame_index .. + count - 1, as many render() calls would."""
+        check(lib().hdr_render_frames(self.h, self.gscene.h, C.byref(frame), C.byref(pipe), int(count)))
 
     def synchronize(self):
         check(lib().hdr_view_synchronize(self.h))

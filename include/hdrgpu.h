@@ -405,6 +405,16 @@ hdr_status hdr_view_sample_count(const hdr_view *view, int32_t *samples);
 hdr_status hdr_render_progressive(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
                                   const hdr_pipeline_desc *pipeline, int32_t target_samples, int32_t max_frames,
                                   int32_t *converged);
+/* Frames frame->frame_index .. frame->frame_index + count - 1 of `frame`'s camera, with the same
+ * results as `count` hdr_render_frame calls (accumulation, Variance sums and sample count; no
+ * colour copy). An addition with no counterpart in the reference, which renders one frame per
+ * Renderer::Render (renderer.cpp:390-478): frames that keep no state from one to the next -- MIS,
+ * or RIS without the reservoir; no spatial reuse, no firefly filter, no depth of field -- and run
+ * on the integration megakernel or the lean path tracer without RNG spill slots share launches
+ * (up to 64 frames, with their samples staged in at most 256 MiB and accumulated in frame order);
+ * other frames are rendered one by one. hdr_render_progressive renders its frames this way. */
+hdr_status hdr_render_frames(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
+                             const hdr_pipeline_desc *pipeline, int32_t count);
 /* Blocks until all work queued on the view's stream has finished. */
 hdr_status hdr_view_synchronize(hdr_view *view);
 /* Device time (ms) of the last hdr_render_frame's kernels: G-buffer (RaycastPass), integration,

@@ -1,0 +1,100 @@
+"""hdr_render_frames: several frames of a static view in shared launches, against the same frames
+rendered one by one (hdr_render_frame). Bar: bit-identical accumulation, colour and Variance sums
+(the staged samples are accumulated in frame order, so every sum is the frame-by-frame one); the
+stage record shows which frames shared launches, and frames that cannot (a reservoir, depth of
+field, RNG spill slots, the deferred path) go one by one with the same result."""
+import numpy as np
+import pytest
+
+from hdrestir_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # (scene, pipeline, shares launches)
+    "box_pt_d4": (lambda: scenes.box_spheres(64, 48, frames=1), abi.pipeline("mis", max_depth=4), True),
+    "box_ris16_d4": (lambda: scenes.box_spheres(64, 48, frames=1),
+                     abi.pipeline("ris", candidate_count=16, use_reservoir=False, max_depth=4), True),
+    "features_opaque_pt_d6": (lambda: scenes.features(48, 40, opaque=True), abi.pipeline("mis", max_depth=6), True),
+    "features_ris8_d3": (lambda: scenes.features(48, 40, opaque=True, dome=True, emissive=True, mesh_light=True),
+                         abi.pipeline("ris", candidate_count=8, use_reservoir=False, max_depth=3), True),
+    "box_restir16_d4": (lambda: scenes.box_spheres(64, 48, frames=1),
+                        abi.pipeline("ris", candidate_count=16, use_reservoir=True, max_depth=4), False),
+    "features_translucent_pt_d5": (lambda: scenes.features(48, 40), abi.pipeline("mis", max_depth=5), False),
+    "features_pt_dof": (lambda: scenes.features(48, 40, opaque=True, dof=True), abi.pipeline("mis", max_depth=3),
+                        False),
+    "box_wave_ris16_skipvis": (lambda: scenes.box_spheres(64, 48, frames=1),
+                               abi.pipeline("ris", candidate_count=16, use_reservoir=False, skip_visibility=True,
+                                            max_depth=3), False),
+}
+
+
+def _frame(sc, index):
+    """The scene's first camera (with its lens parameters) at frame_index `index`."""
+    f = scenes.frame_with_camera(sc, 0)
+    f.frame_index = index
+    return f
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_render_frames_equals_frames_one_by_one(gpu_ctx, case):
+    from hdrestir_amd import gpu
+    make, pipe, shared = CASES[case]
+    sc = make()
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    a = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    b = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    a.set_metrics(True)
+    b.set_metrics(True)
+    n = 20
+    a.stage_times()
+    a.render_frames(_frame(sc, 3), pipe, n)
+    st = a.stage_times()
+    for f in range(n):
+        b.render(_frame(sc, 3 + f), pipe, read_color=False)
+    ca, cb = a.read_color().copy(), b.read_color().copy()
+    assert np.array_equal(ca.view(np.uint32), cb.view(np.uint32)), np.abs(ca - cb).max()
+    assert a.sample_count() == b.sample_count() == 3 + n
+    va, sa, qa = a.variance()
+    vb, sb, qb = b.variance()
+    assert np.array_equal(sa, sb) and np.array_equal(qa, qb)
+    assert va == vb
+    launches = st["integrate"][1]
+    assert launches == (1 if shared else n), st
+    a.close()
+    b.close()
+    gs.close()
+
+
+def test_render_frames_in_several_launches(gpu_ctx):
+    """A 1024 x 1024 view stages at most 16 frames (256 MiB): 20 frames take two launches and equal
+    the frames one by one."""
+    from hdrestir_amd import gpu
+    sc = scenes.box_spheres(1024, 1024, frames=1)
+    pipe = abi.pipeline("mis", max_depth=3)
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    a = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    b = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    a.stage_times()
+    a.render_frames(sc.frame(0), pipe, 20)
+    assert a.stage_times()["integrate"][1] == 2
+    for f in range(20):
+        b.render(sc.frame(f, camera_index=0), pipe, read_color=False)
+    assert a.sample_count() == b.sample_count() == 20
+    assert np.array_equal(a.read_color(), b.read_color())
+    a.close()
+    b.close()
+    gs.close()
+
+
+def test_render_frames_arguments(gpu_ctx):
+    from hdrestir_amd import gpu
+    sc = scenes.box_spheres(16, 16, frames=1)
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, 16, 16)
+    r.render_frames(sc.frame(0), abi.pipeline("mis", max_depth=2), 0)  # nothing to render
+    assert r.sample_count() == 0
+    with pytest.raises(gpu.HdrError, match="INVALID"):
+        r.render_frames(sc.frame(0), abi.pipeline("mis", max_depth=2), -1)
+    r.close()
+    gs.close()
