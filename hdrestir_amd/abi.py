@@ -81,7 +81,8 @@ class PipelineDesc(C.Structure):
                 ("max_reflection_bounces", C.c_int32), ("max_refraction_bounces", C.c_int32),
                 ("enable_subsurface", C.c_int32), ("render_ibl_background", C.c_int32),
                 ("roulette_aggressiveness", C.c_float), ("enable_firefly_filter", C.c_int32),
-                ("spatial_count", C.c_int32), ("spatial_radius", C.c_float)]
+                ("spatial_count", C.c_int32), ("spatial_radius", C.c_float),
+                ("temporal_reprojection", C.c_int32)]
 
 
 class Reservoir(C.Structure):
@@ -109,7 +110,7 @@ class FrameCounters(C.Structure):
 
 def pipeline(direct="ris", candidate_count=16, use_reservoir=True, skip_visibility=False, max_depth=4,
              max_refl=8, max_refr=8, enable_subsurface=True, render_ibl_background=True,
-             roulette=1.0, firefly=False, spatial_count=0, spatial_radius=30.0):
+             roulette=1.0, firefly=False, spatial_count=0, spatial_radius=30.0, reprojection=False):
     """PathTracePassSettings + RISPipelineSettings as an hdr_pipeline_desc."""
     p = PipelineDesc()
     p.struct_size = C.sizeof(PipelineDesc)
@@ -126,6 +127,7 @@ def pipeline(direct="ris", candidate_count=16, use_reservoir=True, skip_visibili
     p.enable_firefly_filter = int(firefly)
     p.spatial_count = spatial_count
     p.spatial_radius = spatial_radius
+    p.temporal_reprojection = int(reprojection)
     return p
 
 

@@ -208,7 +208,7 @@ __device__ __forceinline__ Spec ris_direct_lean(const FrameArgs &a, const SpecTa
         }
         count += zeroCands;
         // temporal merge with the stored reservoir of this pixel (:127-155)
-        const hdr_reservoir prev = a.res[pixel];
+        const hdr_reservoir prev = temporal_prev(a, pixel);
         const uint32_t pflags = prev.flags_light >> 24;
         Cand prevC;
         if ((pflags & HDR_RES_HAS_SAMPLE) && (pflags & HDR_RES_HAS_CANDIDATE)) {
@@ -496,7 +496,7 @@ __device__ __forceinline__ Spec ris_direct_traced(const FrameArgs &a, const Spec
             wsum += w;
             if (rng.next_float() <= w / wsum) sel = q;
         }
-        const hdr_reservoir prev = a.res[pixel];  // temporal merge (:127-155)
+        const hdr_reservoir prev = temporal_prev(a, pixel);  // temporal merge (:127-155)
         const uint32_t pflags = prev.flags_light >> 24;
         Cand prevC;
         if ((pflags & HDR_RES_HAS_SAMPLE) && (pflags & HDR_RES_HAS_CANDIDATE)) {
@@ -606,6 +606,9 @@ __device__ __forceinline__ Spec ris_direct_traced(const FrameArgs &a, const Spec
 // frame in the reservoir form of reservoir.h (WSum = W * frames * p_q). Neighbour picks and
 // selections draw from a splitmix64 stream seeded by (pixel, frame): integer-only, so results are
 // identical on every target, and the reference's per-pixel mt19937_64 stream is untouched.
+// With reprojected temporal reuse on a frame whose camera moved, domain 1 (before the neighbours)
+// is the previous frame's final reservoir at the pixel that saw q's hit (reproject_pixel), with
+// that pixel's vertex of the previous frame; its slot in the debug record is kMaxSpatial.
 
 constexpr int kMaxSpatial = 8;
 // Domains of a pixel's spatial reuse (itself + up to spatial_count neighbours), rounded up to the
@@ -717,19 +720,64 @@ __device__ __forceinline__ Spec reeval_sample(const SpecTab &tab, const Vertex &
     return rgb2spec(tab, y.color) * rgb2spec(tab, bsdf * nDotL);
 }
 
-// The shading vertex of a pixel for the spatial pass, from its G-buffer record and the temporal
+// Domain d of a pixel's spatial reuse: pixel d of this frame (d < P), or for d = P + q the previous
+// frame's records that reproject_kernel gathered for pixel q (reprojected temporal reuse).
+__device__ __forceinline__ const GRec *domain_grec(const FrameArgs &a, const GRec *gbuf, int d, int P) {
+    return d < P ? gbuf + d : a.rp_g + (d - P);
+}
+__device__ __forceinline__ const hdr_reservoir &domain_res(const FrameArgs &a, int d, int P) {
+    return d < P ? a.res_t[d] : a.rp_res[d - P];
+}
+__device__ __forceinline__ float4 domain_aux(const FrameArgs &a, int d, int P) {
+    return d < P ? a.spatial_aux[2 * d] : a.rp_aux[d - P];
+}
+
+// The shading vertex of a domain for the spatial pass, from its G-buffer record and the temporal
 // pass's record {ray direction, wavelength u} (u < 0: the pixel was not shaded).
-__device__ __forceinline__ bool load_spatial_vertex(const FrameArgs &a, const GRec *gbuf, int pixel, Vertex &v,
-                                                    SpecTab &tab) {
-    const float4 x0 = a.spatial_aux[2 * pixel];
+__device__ __forceinline__ bool load_domain_vertex(const FrameArgs &a, const GRec *gbuf, int d, int P, Vertex &v,
+                                                   SpecTab &tab) {
+    const float4 x0 = domain_aux(a, d, P);
     if (x0.w < 0.0f) return false;
     HitRec g;
-    if (!load_grec(gbuf + pixel, g)) return false;
+    if (!load_grec(domain_grec(a, gbuf, d, P), g)) return false;
     tab = make_tab(x0.w);
     const Clo c = get_closure(a.s, g, a.enable_ss != 0);
     Spec unused = spec(1.0f);
     v = make_vertex(a, tab, g, c, v3(x0.x, x0.y, x0.z), unused);
     return true;
+}
+__device__ __forceinline__ bool load_spatial_vertex(const FrameArgs &a, const GRec *gbuf, int pixel, Vertex &v,
+                                                    SpecTab &tab) {
+    return load_domain_vertex(a, gbuf, pixel, a.W * a.H, v, tab);
+}
+
+// Reprojected temporal reuse (an extension): the pixel of the previous frame that saw hit g, or -1.
+// The hit point goes through the previous frame's view and projection matrices (row vectors, in
+// double; clip w > 0), its NDC [-1, 1] maps to [0, W) x [0, H) as camera_ray's inverse, and the
+// pixel containing it qualifies if it was shaded then (its {ray, u} record has u >= 0) and its
+// G-buffer record is similar: depth within 10 % of the hit's distance from the previous camera,
+// geometric normals within 25 degrees (spatially_similar's thresholds).
+__device__ __forceinline__ int reproject_pixel(const FrameArgs &a, const HitRec &g) {
+    const double X = double(g.pos.x), Y = double(g.pos.y), Z = double(g.pos.z);
+    const double *V = a.prev_view, *Q = a.prev_proj;
+    const double cx = X * V[0] + Y * V[4] + Z * V[8] + V[12];
+    const double cy = X * V[1] + Y * V[5] + Z * V[9] + V[13];
+    const double cz = X * V[2] + Y * V[6] + Z * V[10] + V[14];
+    const double cw = X * V[3] + Y * V[7] + Z * V[11] + V[15];
+    const double hx = cx * Q[0] + cy * Q[4] + cz * Q[8] + cw * Q[12];
+    const double hy = cx * Q[1] + cy * Q[5] + cz * Q[9] + cw * Q[13];
+    const double hw = cx * Q[3] + cy * Q[7] + cz * Q[11] + cw * Q[15];
+    if (!(hw > 0.0)) return -1;
+    const double px = (hx / hw + 1.0) * (0.5 * double(a.W));
+    const double py = (hy / hw + 1.0) * (0.5 * double(a.H));
+    if (!(px >= 0.0 && py >= 0.0 && px < double(a.W) && py < double(a.H))) return -1;
+    const int j = int(py) * a.W + int(px);
+    if (a.spatial_aux[2 * j].w < 0.0f) return -1;
+    HitRec gj;
+    if (!load_grec(a.prev_gbuf + j, gj)) return -1;
+    const float dist = length(g.pos - v3(a.prev_origin[0], a.prev_origin[1], a.prev_origin[2]));
+    if (fabsf(gj.depth - dist) > 0.1f * dist) return -1;
+    return dot(g.ng, gj.ng) >= 0.90630779f ? j : -1;  // cos 25 degrees
 }
 
 // Spatial reuse at pixel (x, y). dl / cl: this lane's domain pixels and confidences in LDS
@@ -746,7 +794,9 @@ __device__ __forceinline__ Spec spatial_reuse_pixel(const FrameArgs &a, const GR
                                                     const Vertex &vq, const SpecTab &tabq, int *dl, float *cl,
                                                     bool &query, V3 &qo, V3 &qd, float &qtmax, int &ssel) {
     const int pixel = y * a.W + x;
-    const int K = a.spatial_count < kND - 1 ? a.spatial_count : kND - 1;
+    const int P = a.W * a.H;
+    const int maxK = kND - 1 - (a.reproject ? 1 : 0);
+    const int K = a.spatial_count < maxK ? a.spatial_count : maxK;
     const int R = a.spatial_radius;
     const int M = a.n_cand > 1 ? a.n_cand : 1;
     const int cap = 20 * M;
@@ -754,6 +804,11 @@ __device__ __forceinline__ Spec spatial_reuse_pixel(const FrameArgs &a, const GR
     dl[0] = pixel;
     ++nd;
     int slotBits = 0;  // slot of domain d (>= 1) in bits 4(d-1) .. 4(d-1)+3
+    if (a.reproject && a.rp_aux[pixel].w >= 0.0f) {  // the previous frame's reservoir where q was seen
+        dl[nd * kLanes] = P + pixel;
+        slotBits |= kMaxSpatial << (4 * (nd - 1));
+        ++nd;
+    }
     SpatialRng srng(uint32_t(pixel), a.frame_index);
 #pragma unroll 1
     for (int k = 0; k < K; ++k) {
@@ -770,7 +825,7 @@ __device__ __forceinline__ Spec spatial_reuse_pixel(const FrameArgs &a, const GR
     int countSum = 0;
 #pragma unroll 1
     for (int d = 0; d < nd; ++d) {
-        const int c = a.res_t[dl[d * kLanes]].count;
+        const int c = domain_res(a, dl[d * kLanes], P).count;
         const int cc = c < cap ? c : cap;
         cl[d * kLanes] = float(cc);
         countSum += cc;
@@ -787,14 +842,15 @@ __device__ __forceinline__ Spec spatial_reuse_pixel(const FrameArgs &a, const GR
         phJ[j] = jac[j] = phQ[j] = wJ[j] = den[j] = 0.0f;
         if (j >= nd) continue;
         const int dj = dl[j * kLanes];
-        const hdr_reservoir rj = a.res_t[dj];
+        const hdr_reservoir rj = domain_res(a, dj, P);
         const uint32_t fj = rj.flags_light >> 24;
         if (j == 0) vis0 = (fj & HDR_RES_VIS_TESTED) != 0;
         if (!(fj & HDR_RES_HAS_SAMPLE) || !(fj & HDR_RES_HAS_CANDIDATE) || rj.w <= 0.0f) continue;
         const float phatJ = rj.target;
         if (phatJ <= 0.0f) continue;
         const RSample atJ = rsample_of(rj);
-        const V3 posJ = v3(gbuf[dj].a.x, gbuf[dj].a.y, gbuf[dj].a.z);
+        const float4 pj = domain_grec(a, gbuf, dj, P)->a;
+        const V3 posJ = v3(pj.x, pj.y, pj.z);
         if (!retarget(a, atJ, posJ, vq.h.pos, yq[j])) continue;
         const float phatQ = spec_lum(tabq, reeval_sample(tabq, vq, yq[j]));
         if (phatQ <= 0.0f) continue;
@@ -816,7 +872,7 @@ __device__ __forceinline__ Spec spatial_reuse_pixel(const FrameArgs &a, const GR
         for (int j = 0; j < kND; ++j) any = any || (ok[j] && j != k);
         Vertex vk;
         SpecTab tabk;
-        const bool loaded = any && load_spatial_vertex(a, gbuf, dl[k * kLanes], vk, tabk);
+        const bool loaded = any && load_domain_vertex(a, gbuf, dl[k * kLanes], P, vk, tabk);
         const float ck = cl[k * kLanes];
 #pragma unroll
         for (int j = 0; j < kND; ++j) {

@@ -459,6 +459,16 @@ struct hdr_view {
     DevBuf resolved;      // staging of hdr_view_resolve
     DevBuf post;          // hdr_view_post_process: two RGBA planes, the bright pass, per-row flags
     DevBuf lum_sum, lum_sq, variance;  // Variance metrics (hdr_view_set_metrics): per-pixel sums, reduction
+    // Reprojected temporal reuse: the previous frame's camera (hist_cam) and whether that frame left
+    // what reproject_kernel reads (hist_records: a single-view spatial frame with
+    // temporal_reprojection, whose final reservoirs, {ray, u} records and G-buffer half gcur stay
+    // until the next frame's gather); rp holds the gathered records (GRec, float4, hdr_reservoir
+    // planes of P entries).
+    bool hist_cam = false, hist_records = false;
+    double hist_view[16] = {}, hist_proj[16] = {};
+    float hist_origin[3] = {};
+    uint64_t hist_scene = 0;
+    DevBuf rp;
     // Frames overlap: the G-buffer is double-buffered by frame parity, and the G-buffer kernel of
     // frame N+1 runs on `gstream` as soon as frame N-1's shading has read that buffer (event
     // `shade_done[parity]`), beside frame N's shading, shadow rays and accumulation on `stream`;
@@ -872,6 +882,20 @@ hdr_status hdr_view_reset(hdr_view *view) {
     }
     view->frames = 0;
     view->samples = 0;
+    view->hist_cam = false;
+    view->hist_records = false;
+    return HDR_OK;
+}
+
+hdr_status hdr_view_reset_accumulation(hdr_view *view) {
+    if (!view) return fail(HDR_ERR_INVALID_ARGUMENT, "view is null");
+    HIP_TRY(hipSetDevice(view->ctx->device));
+    HIP_TRY(hipMemsetAsync(view->accum.p, 0, view->accum.n, view->stream));
+    if (view->lum_sum.p) {
+        HIP_TRY(hipMemsetAsync(view->lum_sum.p, 0, view->lum_sum.n, view->stream));
+        HIP_TRY(hipMemsetAsync(view->lum_sq.p, 0, view->lum_sq.n, view->stream));
+    }
+    view->samples = 0;
     return HDR_OK;
 }
 
@@ -880,7 +904,7 @@ hdr_status hdr_view_device_bytes(const hdr_view *v, uint64_t *out) {
     *out = v->lum_sum.n + v->lum_sq.n + v->variance.n + v->resolved.n + v->res.n + v->accum.n + v->color.n + v->frame.n + v->dbg.n + v->depth.n + v->albedo.n + v->normal.n +
            v->counters.n + v->spill.n + v->gbuf.n + v->rgb2.n + v->flags.n + v->queue.n + v->qcount.n + v->res_t.n +
            v->saux.n + v->wstate.n + v->whit.n + v->wray.n + v->wcq.n + v->walive.n + v->wcnt.n + v->wesc.n +
-           v->wesq.n + v->post.n;
+           v->wesq.n + v->post.n + v->rp.n;
     return HDR_OK;
 }
 
@@ -926,9 +950,10 @@ struct FrameNeeds {
     size_t spill = 0, frame = 0, dbg = 0, res_t = 0, saux = 0, rgb2 = 0, flags = 0, queue = 0;
     size_t wstate = 0, whit = 0, wesc = 0;            // the wavefront's per-view state
     size_t wray = 0, wcq = 0, walive = 0, wesq = 0;   // and its queues (a batch's are shared)
+    size_t rp = 0;                                     // reprojected temporal reuse: gathered records
     size_t sum() const {
         return spill + frame + dbg + res_t + saux + rgb2 + flags + queue + wstate + whit + wesc + wray + wcq +
-               walive + wesq;
+               walive + wesq + rp;
     }
 };
 
@@ -1016,6 +1041,13 @@ static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_
         return fail(HDR_ERR_UNSUPPORTED,
                     "spatial reuse needs ReSTIR (use_reservoir) with skip_visibility, opaque materials and no mesh "
                     "light");
+    if (pipe->temporal_reprojection) {
+        if (!spatial)
+            return fail(HDR_ERR_UNSUPPORTED, "temporal_reprojection runs in the spatial pass (spatial_count >= 1)");
+        if (pipe->spatial_count > kMaxSpatial - 1)
+            return fail(HDR_ERR_INVALID_ARGUMENT, "spatial_count must be in [1, 7] with temporal_reprojection");
+        need.rp = P * (sizeof(GRec) + sizeof(float4) + sizeof(hdr_reservoir));
+    }
     if (spatial) {
         need.res_t = P * sizeof(hdr_reservoir);
         need.saux = P * 2 * sizeof(float4);
@@ -1059,7 +1091,7 @@ static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_
     plan.depth = pipe->max_depth;
     plan.lean_pt = !deferred && !ris && !scene->any_translucent && bound < kMtN - 1;
     plan.spatial = spatial;
-    plan.spatial_nd = spatial ? spatial_domains(pipe->spatial_count) : 0;
+    plan.spatial_nd = spatial ? spatial_domains(pipe->spatial_count + (pipe->temporal_reprojection ? 1 : 0)) : 0;
     plan.tiles = unsigned(((W + 7) / 8) * ((H + 7) / 8));
     return HDR_OK;
 }
@@ -1174,6 +1206,25 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
         a.spatial_count = pipe->spatial_count;
         a.spatial_radius = int(pipe->spatial_radius);
     }
+    if (plan.spatial && pipe->temporal_reprojection) {  // reprojected temporal reuse on a camera move
+        const bool moved = view->hist_cam && (std::memcmp(frame->view, view->hist_view, sizeof view->hist_view) != 0 ||
+                                              std::memcmp(frame->proj, view->hist_proj, sizeof view->hist_proj) != 0);
+        const bool distortion = a.has_cam && a.cam.lens_distortion != 0.0f;
+        a.temporal_skip = moved ? 1 : 0;
+        a.reproject = (moved && view->hist_records && view->hist_scene == scene->id && !distortion) ? 1 : 0;
+        if (a.reproject) {
+            if (hdr_status st = ensure(view->rp, need.rp)) return st;
+            a.rp_g = static_cast<GRec *>(view->rp.p);
+            a.rp_aux = reinterpret_cast<float4 *>(a.rp_g + P);
+            a.rp_res = reinterpret_cast<hdr_reservoir *>(a.rp_aux + P);
+            std::memcpy(a.prev_view, view->hist_view, sizeof a.prev_view);
+            std::memcpy(a.prev_proj, view->hist_proj, sizeof a.prev_proj);
+            for (int k = 0; k < 3; ++k) a.prev_origin[k] = view->hist_origin[k];
+            // the previous frame's G-buffer: the half it used (a moved camera never reuses one)
+            a.prev_gbuf = static_cast<const GRec *>(view->gbuf.p) + size_t(view->gcur) * P;
+            if (plan.gbuf_half == view->gcur) a.reproject = 0;
+        }
+    }
     if (need.rgb2) {
         if (hdr_status st = ensure(view->rgb2, need.rgb2)) return st;
         if (hdr_status st = ensure(view->flags, need.flags)) return st;
@@ -1228,6 +1279,21 @@ hdr_status hdr_view_plan_bytes(const hdr_scene *scene, int32_t width, int32_t he
     if (hdr_status st = plan_pipeline(scene, width, height, pipe, false, plan, need)) return st;
     *bytes = uint64_t(view_base_bytes(size_t(width) * size_t(height)) + need.sum());
     return HDR_OK;
+}
+
+// The frame's camera, for the next frame's move test and reprojection; `records`: the frame left
+// reservoirs, {ray, u} records and a G-buffer for reprojected temporal reuse.
+static void remember_camera(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame, const FrameArgs &a,
+                            bool records) {
+    view->hist_cam = true;
+    view->hist_records = records;
+    view->hist_scene = scene->id;
+    std::memcpy(view->hist_view, frame->view, sizeof view->hist_view);
+    std::memcpy(view->hist_proj, frame->proj, sizeof view->hist_proj);
+    const V3D o = xform_point_d(a.inv_view, V3D{0.0, 0.0, 0.0});  // camera_ray's origin
+    view->hist_origin[0] = float(o.x);
+    view->hist_origin[1] = float(o.y);
+    view->hist_origin[2] = float(o.z);
 }
 
 // One frame, or `frames` consecutive frames of a static view in one launch (multi_frame_ok:
@@ -1285,6 +1351,13 @@ static hdr_status render_frames_impl(hdr_view *view, const hdr_scene *scene, con
     HIP_TRY(hipEventRecord(view->gbuf_done, view->gstream));
     HIP_TRY(hipStreamWaitEvent(view->stream, view->gbuf_done, 0));
     HIP_TRY(hipEventRecord(ts.beg[HDR_STAGE_INTEGRATE], view->stream));
+    if (a.reproject) {  // the previous frame's records, gathered before this frame overwrites them
+        hipLaunchKernelGGL(reproject_kernel, grid, dim3(kLanes), 0, view->stream, da, dt, 1);
+        HIP_TRY(hipGetLastError());
+        // the previous G-buffer half has been read: the next frame's G-buffer may overwrite it
+        HIP_TRY(hipEventRecord(view->shade_done[gp ^ 1], view->stream));
+        view->shade_recorded[gp ^ 1] = true;
+    }
     if (deferred) {
         HIP_TRY(hipMemsetAsync(a.qcount, 0, 2 * sizeof(unsigned int), view->stream));
 #define HDR_SHADE(C, D) \
@@ -1454,6 +1527,7 @@ static hdr_status render_frames_impl(hdr_view *view, const hdr_scene *scene, con
         view->gvalid = view->gnext_valid;
     }
     view->single_since_batch = true;
+    remember_camera(view, scene, frame, a, spatial && pipe->temporal_reprojection);
     if (color_out) {
         HIP_TRY(hipMemcpyAsync(color_out, view->color.p, P * sizeof(float4), hipMemcpyDeviceToHost, view->stream));
     }
@@ -2013,6 +2087,8 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
         if (views[i]->counting) return fail(HDR_ERR_UNSUPPORTED, "counting views render one at a time");
         if (pipes[i].struct_size == sizeof(hdr_pipeline_desc) && pipes[i].enable_firefly_filter)
             return fail(HDR_ERR_UNSUPPORTED, "the firefly filter renders one view at a time");
+        if (pipes[i].struct_size == sizeof(hdr_pipeline_desc) && pipes[i].temporal_reprojection)
+            return fail(HDR_ERR_UNSUPPORTED, "temporal_reprojection renders one view at a time");
         for (int j = 0; j < i; ++j)
             if (views[j] == views[i]) return fail(HDR_ERR_INVALID_ARGUMENT, "a view appears twice in a batch");
     }
@@ -2269,6 +2345,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
             views[i]->gvalid = views[i]->gnext_valid;
         }
         views[i]->single_since_batch = false;
+        remember_camera(views[i], scenes[i], &frames[i], args[i], false);
     }
     return HDR_OK;
 }

@@ -41,6 +41,18 @@ struct FrameArgs {
     float4 *spatial_aux;               // spatial reuse: per pixel {ray dir, wavelength u}, {throughput}
     int spatial_count;                 // spatial reuse: neighbours per pixel (0 = off)
     int spatial_radius;                // spatial reuse: pick radius in pixels
+    // reprojected temporal reuse (an extension): temporal_skip = the camera moved, so the
+    // same-pixel temporal merge is skipped; reproject = the previous frame's records are usable,
+    // so reproject_kernel gathers them and the spatial pass adds them as a domain
+    int temporal_skip;
+    int reproject;
+    double prev_view[16];              // the previous frame's view and projection matrices
+    double prev_proj[16];
+    float prev_origin[3];              // the previous camera's position (camera_ray's origin)
+    const GRec *prev_gbuf;             // the previous frame's G-buffer
+    GRec *rp_g;                        // per pixel: the previous G-buffer record where its hit was seen,
+    float4 *rp_aux;                    //   that pixel's {ray dir, wavelength u} (u < 0: none),
+    hdr_reservoir *rp_res;             //   and its final reservoir of the previous frame
     float4 *accum;
     float4 *frame_rgb;                 // this frame's colour (firefly path / debugging)
     float4 *color;                     // accumulated colour
@@ -72,6 +84,21 @@ struct FrameArgs {
     unsigned int *wcnt;                // queue counts by vertex parity: rays[2], shadows[2], alive[2], escapes[2],
                                        // then the any-hit claim counters: escapes[2], shadows[2] (kWaveCounts)
 };
+
+// The stored reservoir the temporal merge reads (ris_direct_light_integrator.cpp:127-155): none on
+// a frame whose camera moved under reprojected temporal reuse, where the pixel's stored sample
+// belongs to another shading point (the spatial pass reuses the reprojected one instead).
+__device__ __forceinline__ hdr_reservoir temporal_prev(const FrameArgs &a, int pixel) {
+    if (a.temporal_skip) {
+        hdr_reservoir none;
+        none.flags_light = 0u;
+        none.count = 0;
+        none.wsum = 0.0f;
+        none.target = 0.0f;
+        return none;
+    }
+    return a.res[pixel];
+}
 
 struct PixelStats {
     uint32_t primary, bounce, shadow, cand, cand_skipped;
@@ -668,7 +695,7 @@ __device__ Spec ris_direct(Ctx &p, const Vertex &v, const Conn *conn, bool prima
             if (p.rng.next_float() <= w / wsum) sel = q;
         }
         // temporal merge with the stored reservoir of this pixel (:127-155)
-        hdr_reservoir prev = a.res[pixel];
+        hdr_reservoir prev = temporal_prev(a, pixel);
         const uint32_t pflags = prev.flags_light >> 24;
         Cand prevC;
         if ((pflags & HDR_RES_HAS_SAMPLE) && (pflags & HDR_RES_HAS_CANDIDATE)) {

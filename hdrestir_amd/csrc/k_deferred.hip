@@ -215,6 +215,27 @@ __global__ void __launch_bounds__(kLanes) wave_spatial_kernel(const FrameArgs *_
     if (kCount) wave_add(a.counters + 2, query ? 1ULL : 0ULL);
 }
 
+// Reprojected temporal reuse (an extension): before this frame's shading overwrites them, copies
+// the previous frame's records at the pixel that saw each pixel's primary hit (reproject_pixel) --
+// its G-buffer record, its {ray dir, wavelength u} and its final reservoir -- into the pixel's
+// rp_* slots, which the spatial pass reads as one more domain; rp_aux.w = -1 when there is none.
+// Per pixel 64 + 16 + 80 B read and written, coalesced on the write side.
+__global__ void __launch_bounds__(kLanes) reproject_kernel(const FrameArgs *__restrict__ args,
+                                                           const int *__restrict__ tile_base, int n_views) {
+    HDR_TILE(args, tile_base, n_views)
+    if (!valid) return;
+    const int pixel = y * a.W + x;
+    HitRec g;
+    const int j = load_grec(a.gbuf + pixel, g) ? reproject_pixel(a, g) : -1;
+    if (j < 0) {
+        a.rp_aux[pixel] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        return;
+    }
+    a.rp_g[pixel] = a.prev_gbuf[j];
+    a.rp_aux[pixel] = a.spatial_aux[2 * j];
+    a.rp_res[pixel] = a.res[j];
+}
+
 // The path's radiance (outer throughput x total) to RGB, exposure, accumulation
 // (integration_pass.cpp:236-243, accumulation_pass.cpp:98-106).
 template <bool kCount, bool kDebug>

@@ -45,6 +45,7 @@ __global__ void wave_spatial_kernel(const FrameArgs *__restrict__ args, const in
 template <bool kCount, bool kDebug>
 __global__ void wave_resolve_kernel(const FrameArgs *__restrict__ args, const int *__restrict__ tile_base,
                                     int n_views);
+__global__ void reproject_kernel(const FrameArgs *__restrict__ args, const int *__restrict__ tile_base, int n_views);
 
 // k_wave0.hip
 template <bool kCount, bool kDebug, bool kLean, bool kTraced>

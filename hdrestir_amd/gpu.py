@@ -95,6 +95,7 @@ def lib():
             "hdr_view_create": [P, I, I, C.POINTER(P)],
             "hdr_view_destroy": [P],
             "hdr_view_reset": [P],
+            "hdr_view_reset_accumulation": [P],
             "hdr_view_device_bytes": [P, C.POINTER(C.c_uint64)],
             "hdr_render_frame": [P, P, C.POINTER(abi.FrameDesc), C.POINTER(abi.PipelineDesc), P, P],
             "hdr_view_read_color": [P, P],
@@ -264,6 +265,10 @@ class Renderer:
 
     def reset(self):
         check(lib().hdr_view_reset(self.h))
+
+    def reset_accumulation(self):
+        """hdr_view_reset_accumulation: a camera move under reprojected temporal reuse."""
+        check(lib().hdr_view_reset_accumulation(self.h))
 
     def render(self, frame, pipe, read_color=True, debug=False):
         check(lib().hdr_render_frame(self.h, self.gscene.h, C.byref(frame), C.byref(pipe),

@@ -73,6 +73,11 @@ struct PipelineSettings {
     bool EnableFireflyFilter{false};
     DenoiserConfig Denoiser{};
     PostProcessConfig PostProcess{};
+    // Extensions beyond the reference (hdr_pipeline_desc): spatial reuse among SpatialCount
+    // neighbours within SpatialRadius pixels, and reprojected temporal reuse under camera motion.
+    int SpatialCount{0};
+    float SpatialRadius{30.0f};
+    bool TemporalReprojection{false};
 
     // DenoiserPass / PostProcessPass run when one of their switches is on (denoiser_pass.cpp:17-20,
     // post_process_pass.cpp:14-17).
@@ -104,6 +109,9 @@ struct PipelineSettings {
         d.render_ibl_background = PathTrace.RenderIblBackground;
         d.roulette_aggressiveness = PathTrace.RouletteAggressiveness;
         d.enable_firefly_filter = EnableFireflyFilter;
+        d.spatial_count = SpatialCount;
+        d.spatial_radius = SpatialRadius;
+        d.temporal_reprojection = TemporalReprojection;
         return d;
     }
 };
@@ -185,6 +193,16 @@ class GpuPathTracePass {
 
     // RenderPipeline::ClearPersistentBuffers (camera / settings / AOV change).
     void ClearPersistentBuffers() { Check(hdr_view_reset(_view), "hdr_view_reset"); }
+
+    // The camera moved (renderer.cpp:401-406 clears the persistent buffers here). With
+    // TemporalReprojection the reservoirs stay for the next frame's reprojected reuse and only the
+    // accumulation restarts.
+    void CameraMoved() {
+        if (_settings.TemporalReprojection)
+            Check(hdr_view_reset_accumulation(_view), "hdr_view_reset_accumulation");
+        else
+            ClearPersistentBuffers();
+    }
 
     void ReadAovs(std::span<float> depth, std::span<float> albedo, std::span<float> normal) {
         Check(hdr_view_read_aovs(_view, depth.data(), albedo.data(), normal.data()), "hdr_view_read_aovs");

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define HDRGPU_ABI_VERSION 2u
+#define HDRGPU_ABI_VERSION 3u
 
 typedef enum hdr_status {
     HDR_OK = 0,
@@ -205,6 +205,23 @@ typedef struct hdr_pipeline_desc {
      * hdr_render_frame returns HDR_ERR_UNSUPPORTED. */
     int32_t  spatial_count;
     float    spatial_radius;
+    /* Extension beyond the reference (0 = reference behaviour), with spatial reuse (spatial_count
+     * 1 .. 7): reprojected temporal reuse under camera motion. The reference clears the reservoirs
+     * on every camera move (renderer.cpp:401-406) and merges the same pixel's reservoir otherwise
+     * (ris_direct_light_integrator.cpp:127-155). With this set, a frame whose view or projection
+     * matrix differs from the view's previous frame skips that same-pixel merge (its sample belongs
+     * to another shading point) and instead offers the previous frame's final reservoir at the
+     * pixel where this pixel's primary hit was seen -- the hit projected with the previous camera,
+     * depth within 10 % of the previous G-buffer's, geometric normals within 25 degrees -- as one
+     * more domain of the spatial resampling, shifted to this pixel's vertex and weighted with the
+     * generalized balance heuristic like the spatial neighbours. Frames with an unchanged camera
+     * merge as the reference does, so their results are the same as without the flag. A caller
+     * keeps the reservoirs on a camera move by clearing the accumulation with
+     * hdr_view_reset_accumulation instead of hdr_view_reset. The history is the previous frame of
+     * the view rendered with hdr_render_frame with this flag set; lens distortion turns the
+     * reprojection off (the same-pixel merge is still skipped on a move). Not supported by
+     * hdr_render_batch (HDR_ERR_UNSUPPORTED). */
+    int32_t  temporal_reprojection;
 } hdr_pipeline_desc;
 
 /* Per-pixel persistent reservoir (reference: WeightedReservoir<RISLightCandidate>,
@@ -292,6 +309,11 @@ hdr_status hdr_view_create(hdr_context *ctx, int32_t width, int32_t height, hdr_
 hdr_status hdr_view_destroy(hdr_view *view);
 /* == RenderPipeline::ClearPersistentBuffers: zero reservoirs and accumulation */
 hdr_status hdr_view_reset(hdr_view *view);
+/* Zero the accumulation, the Variance sums and the sample count, and keep the reservoirs and the
+ * previous frame's records: what a caller does on a camera move with temporal_reprojection set
+ * (hdr_pipeline_desc). An addition with no counterpart in the reference, whose camera moves clear
+ * both (RenderPipeline::ClearPersistentBuffers via renderer.cpp:401-406). */
+hdr_status hdr_view_reset_accumulation(hdr_view *view);
 hdr_status hdr_view_device_bytes(const hdr_view *view, uint64_t *out);
 
 /* Device bytes a view of width x height holds to render frames of `pipeline` on `scene`: the buffers

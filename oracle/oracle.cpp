@@ -2294,7 +2294,44 @@ struct oracle_ctx {
     std::vector<std::optional<HitRecord>> gbuf;
     std::vector<float> depth, albedo, normal;
     Counters last;
+    // Reprojected temporal reuse (an extension, hdr_pipeline_desc::temporal_reprojection): the
+    // previous frame's camera and whether it left its records (a spatial frame with the flag);
+    // prevAux / prevRes are those records, copied at the start of a frame that reprojects, and
+    // reproj[i] the previous frame's pixel that saw pixel i's hit (-1: none).
+    bool histCam = false, histRecords = false;
+    double histView[16] = {}, histProj[16] = {};
+    V3 histOrigin;
+    std::vector<SpatialAux> prevAux;
+    std::vector<Reservoir> prevRes;
+    std::vector<int> reproj;
 };
+
+// The pixel of the previous frame that saw hit h, or -1 (the reprojection of the spatial pass's
+// extra domain): h's position through the previous view and projection matrices (row vectors, in
+// double; clip w > 0), NDC [-1, 1] to [0, W) x [0, H) as CameraRay's inverse, and the pixel
+// containing it if it was shaded then and its hit is similar -- depth within 10 % of h's distance
+// from the previous camera, geometric normals within 25 degrees (SpatiallySimilar's thresholds).
+int ReprojectPixel(const oracle_ctx &c, const HitRecord &h) {
+    const double X = double(h.Position[0]), Y = double(h.Position[1]), Z = double(h.Position[2]);
+    const double *V = c.histView, *Q = c.histProj;
+    const double cx = X * V[0] + Y * V[4] + Z * V[8] + V[12];
+    const double cy = X * V[1] + Y * V[5] + Z * V[9] + V[13];
+    const double cz = X * V[2] + Y * V[6] + Z * V[10] + V[14];
+    const double cw = X * V[3] + Y * V[7] + Z * V[11] + V[15];
+    const double hx = cx * Q[0] + cy * Q[4] + cz * Q[8] + cw * Q[12];
+    const double hy = cx * Q[1] + cy * Q[5] + cz * Q[9] + cw * Q[13];
+    const double hw = cx * Q[3] + cy * Q[7] + cz * Q[11] + cw * Q[15];
+    if (!(hw > 0.0)) return -1;
+    const double px = (hx / hw + 1.0) * (0.5 * double(c.W));
+    const double py = (hy / hw + 1.0) * (0.5 * double(c.H));
+    if (!(px >= 0.0 && py >= 0.0 && px < double(c.W) && py < double(c.H))) return -1;
+    const int j = int(py) * c.W + int(px);
+    const SpatialAux &pj = c.prevAux[size_t(j)];
+    if (!pj.valid) return -1;
+    const float dist = Length(h.Position - c.histOrigin);
+    if (std::fabs(pj.hit.Depth - dist) > 0.1f * dist) return -1;
+    return Dot(h.Normal, pj.hit.Normal) >= 0.90630779f ? j : -1;
+}
 
 extern "C" {
 
@@ -2323,6 +2360,23 @@ void oracle_destroy(oracle_ctx *c) { delete c; }
 
 void oracle_reset(oracle_ctx *c) {
     std::fill(c->reservoirs.begin(), c->reservoirs.end(), Reservoir{});
+    std::fill(c->accum.begin(), c->accum.end(), 0.0f);
+    std::fill(c->lumSum.begin(), c->lumSum.end(), 0.0);
+    std::fill(c->lumSumSq.begin(), c->lumSumSq.end(), 0.0);
+    c->sampleCount = 0;
+    c->histCam = c->histRecords = false;
+}
+
+// Per pixel, the previous frame's pixel whose reservoir the last frame reused as its reprojected
+// domain (-1: none, or the last frame did not reproject).
+void oracle_read_reprojection(oracle_ctx *c, int32_t *out) {
+    const size_t n = size_t(c->W) * c->H;
+    for (size_t i = 0; i < n; ++i) out[i] = i < c->reproj.size() ? c->reproj[i] : -1;
+}
+
+// hdr_view_reset_accumulation: the accumulation restarts, the reservoirs and the previous frame's
+// records stay (a camera move under reprojected temporal reuse).
+void oracle_reset_accumulation(oracle_ctx *c) {
     std::fill(c->accum.begin(), c->accum.end(), 0.0f);
     std::fill(c->lumSum.begin(), c->lumSum.end(), 0.0);
     std::fill(c->lumSumSq.begin(), c->lumSumSq.end(), 0.0);
@@ -2370,6 +2424,22 @@ double oracle_render_frame(oracle_ctx *c, const hdr_frame_desc *frame, const hdr
     const bool spatial = pipe->spatial_count > 0;
     if (spatial && (pipe->direct_light != HDR_DIRECT_RIS || !pipe->use_reservoir))
         return -1.0;  // spatial reuse is defined for ReSTIR direct lighting (at the primary vertex)
+    if (pipe->temporal_reprojection && (!spatial || pipe->spatial_count > kMaxSpatial - 1))
+        return -1.0;  // reprojected temporal reuse is a domain of the spatial pass (at most 7 neighbours)
+    // reprojected temporal reuse: on a camera move the same-pixel merge is skipped and, with the
+    // previous frame's records, the spatial pass adds the reprojected domain
+    bool temporalSkip = false, reproject = false;
+    if (pipe->temporal_reprojection) {
+        const bool moved = c->histCam && (std::memcmp(frame->view, c->histView, sizeof c->histView) != 0 ||
+                                          std::memcmp(frame->proj, c->histProj, sizeof c->histProj) != 0);
+        temporalSkip = moved;
+        reproject = moved && c->histRecords && !(hasCam && frame->camera.lens_distortion != 0.0f);
+    }
+    c->reproj.assign(reproject ? size_t(W) * H : 0, -1);
+    if (reproject) {
+        c->prevAux = c->aux;
+        c->prevRes = c->reservoirs;
+    }
     if (spatial)
         for (SpatialAux &a : c->aux) a.valid = false;
     constexpr int kTile = 16;
@@ -2410,6 +2480,13 @@ double oracle_render_frame(oracle_ctx *c, const hdr_frame_desc *frame, const hdr
                   },
                   cnt);
 
+    if (reproject)
+        for (int y = y0; y < y1; ++y)
+            for (int x = x0; x < x1; ++x) {
+                const size_t i = size_t(y) * W + x;
+                if (c->gbuf[i]) c->reproj[i] = ReprojectPixel(*c, *c->gbuf[i]);
+            }
+
     // IntegrationPass, integration_pass.cpp:157-252
     std::vector<float> fb(size_t(W) * H * 4, 0.0f);
     const auto t0 = std::chrono::steady_clock::now();
@@ -2427,6 +2504,7 @@ double oracle_render_frame(oracle_ctx *c, const hdr_frame_desc *frame, const hdr
                                                   : CameraRay(invView, invProj, px, py, W, H);
                               const Lambda lam = Lambda::SampleUniform(rng.NextFloat());
                               PixelTrace tr;
+                              Reservoir none;  // a camera move under reprojection: no same-pixel merge
                               const PathCtx pc{c->scene,
                                                *pipe,
                                                rng,
@@ -2434,7 +2512,7 @@ double oracle_render_frame(oracle_ctx *c, const hdr_frame_desc *frame, const hdr
                                                tr,
                                                lc,
                                                c->hoist ? &hoistedSet : nullptr,
-                                               &c->reservoirs[i],
+                                               temporalSkip ? &none : &c->reservoirs[i],
                                                spatial ? &c->resT[i] : nullptr,
                                                spatial ? &c->aux[i] : nullptr};
                               const Spec s = PathLi(pc, ray, c->gbuf[i]);
@@ -2475,26 +2553,36 @@ double oracle_render_frame(oracle_ctx *c, const hdr_frame_desc *frame, const hdr
                     for (int x = std::max(sx, x0); x < std::min(sx + kTile, x1); ++x) {
                         const size_t i = size_t(y) * W + x;
                         if (!c->aux[i].valid) continue;
-                        // the domains: q itself, then the picked neighbours that pass the similarity test
-                        size_t dom[kMaxSpatial + 1];
+                        // the domains: q itself, the reprojected previous frame's pixel (on a camera
+                        // move under reprojected temporal reuse), then the picked neighbours that
+                        // pass the similarity test; each with its vertex and reservoir
+                        const SpatialAux *dAux[kMaxSpatial + 1];
+                        const Reservoir *dRes[kMaxSpatial + 1];
                         int slot[kMaxSpatial + 1];
                         int nd = 0;
-                        dom[nd] = i;
+                        dAux[nd] = &c->aux[i];
+                        dRes[nd] = &c->resT[i];
                         slot[nd++] = -1;
+                        if (reproject && c->reproj[i] >= 0) {
+                            dAux[nd] = &c->prevAux[size_t(c->reproj[i])];
+                            dRes[nd] = &c->prevRes[size_t(c->reproj[i])];
+                            slot[nd++] = kMaxSpatial;
+                        }
                         SpatialRng srng(uint32_t(i), frameIndex);
                         for (int k = 0; k < K; ++k) {
                             int nx, ny;
                             if (!PickNeighbour(srng, x, y, W, H, R, nx, ny)) continue;
                             const size_t n = size_t(ny) * W + nx;
                             if (!c->aux[n].valid || !SpatiallySimilar(c->aux[i].hit, c->aux[n].hit)) continue;
-                            dom[nd] = n;
+                            dAux[nd] = &c->aux[n];
+                            dRes[nd] = &c->resT[n];
                             slot[nd++] = k;
                         }
                         float conf[kMaxSpatial + 1];
                         int countSum = 0;
                         for (int d = 0; d < nd; ++d) {
-                            conf[d] = float(std::min(c->resT[dom[d]].CandidateCount, cap));
-                            countSum += std::min(c->resT[dom[d]].CandidateCount, cap);
+                            conf[d] = float(std::min(dRes[d]->CandidateCount, cap));
+                            countSum += std::min(dRes[d]->CandidateCount, cap);
                         }
                         const SpatialAux &q = c->aux[i];
                         const ShadingPoint sp{&q.cl, q.sn, &q.lam, q.inside};
@@ -2502,13 +2590,13 @@ double oracle_render_frame(oracle_ctx *c, const hdr_frame_desc *frame, const hdr
                         float wsum = 0.0f;
                         int ssel = -1;
                         for (int j = 0; j < nd; ++j) {
-                            const Reservoir &rj = c->resT[dom[j]];
+                            const Reservoir &rj = *dRes[j];
                             if (!rj.ChosenSample || !rj.ChosenSample->Candidate || rj.W <= 0.0f) continue;
                             const float phatJ = float(rj.ChosenSample->targetFunction);
                             if (phatJ <= 0.0f) continue;
                             const Light &light = *rj.ChosenSample->Candidate->light;
                             const Light::Sample &atJ = rj.ChosenSample->Candidate->ls;
-                            const auto atQ = RetargetSample(light, atJ, c->aux[dom[j]].hit.Position, q.hit.Position,
+                            const auto atQ = RetargetSample(light, atJ, dAux[j]->hit.Position, q.hit.Position,
                                                             selPdf);
                             if (!atQ) continue;
                             const Spec reevalQ = RisReEvaluate(sp, q.rayDir, LightSampler::Candidate{&light, *atQ});
@@ -2525,7 +2613,7 @@ double oracle_render_frame(oracle_ctx *c, const hdr_frame_desc *frame, const hdr
                                 } else if (k == 0) {
                                     pk = phatQ;
                                 } else {
-                                    const SpatialAux &ak = c->aux[dom[k]];
+                                    const SpatialAux &ak = *dAux[k];
                                     const auto atK = RetargetSample(light, *atQ, q.hit.Position, ak.hit.Position, selPdf);
                                     pk = 0.0f;
                                     if (atK) {
@@ -2631,6 +2719,11 @@ double oracle_render_frame(oracle_ctx *c, const hdr_frame_desc *frame, const hdr
         }
     c->sampleCount = frameIndex + 1;
     c->last = cnt;
+    c->histCam = true;
+    c->histRecords = spatial && pipe->temporal_reprojection;
+    std::memcpy(c->histView, frame->view, sizeof c->histView);
+    std::memcpy(c->histProj, frame->proj, sizeof c->histProj);
+    c->histOrigin = ToF(invView.Transform({{0.0, 0.0, 0.0}}));  // CameraRay's origin
     return secs;
 }
 

@@ -29,6 +29,11 @@ oracle_ctx *oracle_create(const hdr_scene_desc *scene, int32_t width, int32_t he
 void        oracle_destroy(oracle_ctx *ctx);
 /* RenderPipeline::ClearPersistentBuffers: drop reservoirs and accumulation */
 void        oracle_reset(oracle_ctx *ctx);
+/* hdr_view_reset_accumulation: drop the accumulation, keep reservoirs and the previous frame's
+ * records (reprojected temporal reuse, an extension) */
+void        oracle_reset_accumulation(oracle_ctx *ctx);
+/* Per pixel, the previous frame's pixel the last frame reprojected it to (-1: none). */
+void        oracle_read_reprojection(oracle_ctx *ctx, int32_t *out);
 /* firefly_in_place = 1: the CPU AccumulationPass (raster-sequential, in place);
  * 0: out-of-place 3x3 clamp on the unclamped frame (the reference's GPU shader). */
 void        oracle_set_firefly_in_place(oracle_ctx *ctx, int32_t in_place);

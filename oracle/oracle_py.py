@@ -31,6 +31,8 @@ def lib():
         L.oracle_create.argtypes = [C.POINTER(abi.SceneDesc), C.c_int32, C.c_int32, C.c_int32, C.c_int32]
         L.oracle_destroy.argtypes = [P]
         L.oracle_reset.argtypes = [P]
+        L.oracle_reset_accumulation.argtypes = [P]
+        L.oracle_read_reprojection.argtypes = [P, P]
         L.oracle_set_firefly_in_place.argtypes = [P, C.c_int32]
         L.oracle_render_frame.restype = C.c_double
         L.oracle_render_frame.argtypes = [P, C.POINTER(abi.FrameDesc), C.POINTER(abi.PipelineDesc), C.c_int32,
@@ -78,6 +80,15 @@ class Oracle:
     def reset(self):
         lib().oracle_reset(self.h)
 
+    def reset_accumulation(self):
+        lib().oracle_reset_accumulation(self.h)
+
+    def reprojection(self):
+        """Per pixel (H x W), the previous frame's pixel index the last frame reprojected it to (-1: none)."""
+        out = np.empty(self.H * self.W, np.int32)
+        lib().oracle_read_reprojection(self.h, _p(out))
+        return out.reshape(self.H, self.W)
+
     def set_firefly_in_place(self, v):
         lib().oracle_set_firefly_in_place(self.h, int(v))
 
@@ -86,7 +97,8 @@ class Oracle:
         self.last_seconds = lib().oracle_render_frame(self.h, C.byref(frame), C.byref(pipe), x0, y0, x1, y1,
                                                       _p(self.color), _p(self.debug) if debug else None)
         if self.last_seconds < 0:
-            raise ValueError("the oracle does not define this pipeline (spatial reuse needs ReSTIR: RIS with the reservoir)")
+            raise ValueError("the oracle does not define this pipeline (spatial reuse needs ReSTIR: RIS with the "
+                             "reservoir; temporal reprojection needs spatial reuse with at most 7 neighbours)")
         return self.color
 
     def reservoirs(self):
