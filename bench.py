@@ -456,6 +456,68 @@ def post_process_measurement(r, frame, pipe, reps=8):
     return out
 
 
+def dolly_reprojection_measurement(gpu, abi, ctx, sc, gs, r, pipe, warmup, steps, ref_frames=256):
+    """Config 1's dolly with reprojected temporal reuse (an extension: hdr_pipeline_desc::
+    temporal_reprojection). Each move restarts the accumulation and the frame index (as on the
+    headline) but keeps the reservoirs, and the previous frame's reservoir where each pixel's hit was
+    seen joins the spatial pass. Frames/s over the headline's frames, and the mean squared error of
+    the 1-spp frames at dolly frames 16, 32, 48 and 63 (rendered in order from frame 0) against
+    converged images of those cameras -- RIS M = 32 without a reservoir (unbiased), `ref_frames`
+    frames -- next to the headline's frames, whose moves clear the reservoirs."""
+    import ctypes as C
+    rp = abi.PipelineDesc()
+    C.memmove(C.byref(rp), C.byref(pipe), C.sizeof(rp))
+    rp.temporal_reprojection = 1
+    n = len(sc.cameras)
+
+    def dolly(i):
+        return sc.frame(0, camera_index=i % n)
+
+    r.reset()
+    for i in range(warmup):
+        r.reset_accumulation()
+        r.render(dolly(i), rp, read_color=False)
+    r.synchronize()
+    r.stage_times()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        r.reset_accumulation()
+        r.render(dolly(warmup + i), rp, read_color=False)
+    r.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    stages = {k: round(v[0] / v[1], 4) for k, v in r.stage_times().items() if v[1] > 0}
+
+    checks = (16, 32, 48, n - 1)
+    images = {"reset_each_move": {}, "reprojection": {}}
+    for name, p in (("reset_each_move", pipe), ("reprojection", rp)):
+        r.reset()
+        for i in range(n):
+            if name == "reset_each_move":
+                r.reset()
+            else:
+                r.reset_accumulation()
+            r.render(dolly(i), p, read_color=False)
+            if i in checks:
+                images[name][i] = r.read_color()[..., :3].astype(np.float64).copy()
+    ref_pipe = abi.pipeline("ris", candidate_count=32, use_reservoir=False, skip_visibility=True, max_depth=1)
+    v = gpu.Renderer(ctx, gs, sc.width, sc.height)
+    mse = {"reset_each_move": [], "reprojection": []}
+    for i in checks:
+        v.reset()
+        for f in range(ref_frames):
+            v.render(sc.frame(f, camera_index=i), ref_pipe, read_color=False)
+        ref = v.read_color()[..., :3].astype(np.float64)
+        for name in mse:
+            mse[name].append(float(np.mean((images[name][i] - ref) ** 2)))
+    v.close()
+    ratio = [b / a for a, b in zip(mse["reset_each_move"], mse["reprojection"])]
+    return {"frames_per_s": round(1000.0 / ms, 3), "ms_per_step": round(ms, 4), "stage_ms": stages,
+            "mse_frames": list(checks), "mse_reset_each_move": [round(x, 6) for x in mse["reset_each_move"]],
+            "mse_reprojection": [round(x, 6) for x in mse["reprojection"]],
+            "mse_ratio_reprojection_to_reset": [round(x, 3) for x in ratio],
+            "reference": f"RIS M=32 without a reservoir, {ref_frames} frames per camera"}
+
+
 def run_frames(r, desc, pipe, warmup, steps, reset_each):
     """`warmup` untimed frames desc(0 ..), then `steps` timed frames; with reset_each every frame starts
     from a cleared view (a camera move). Returns (ms per frame by the host clock around the timed
@@ -686,6 +748,9 @@ def main():
     if not args.headline_only:
         line["post_process"] = post_process_measurement(r, dolly(0), pipe)
         _log(f"post process {line['post_process']}")
+        line["dolly_reprojection"] = dolly_reprojection_measurement(gpu, abi, ctx, sc, gs, r, pipe, args.warmup,
+                                                                    args.steps)
+        _log(f"dolly with reprojected temporal reuse {line['dolly_reprojection']}")
     r.close()
     if not args.headline_only and not args.no_deep:
         _log("deep paths")

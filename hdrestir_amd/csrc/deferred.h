@@ -755,7 +755,7 @@ __device__ __forceinline__ bool load_spatial_vertex(const FrameArgs &a, const GR
 // The hit point goes through the previous frame's view and projection matrices (row vectors, in
 // double; clip w > 0), its NDC [-1, 1] maps to [0, W) x [0, H) as camera_ray's inverse, and the
 // pixel containing it qualifies if it was shaded then (its {ray, u} record has u >= 0) and its
-// G-buffer record is similar: depth within 10 % of the hit's distance from the previous camera,
+// G-buffer record (the copy the previous spatial pass kept) is similar: depth within 10 % of the hit's distance from the previous camera,
 // geometric normals within 25 degrees (spatially_similar's thresholds).
 __device__ __forceinline__ int reproject_pixel(const FrameArgs &a, const HitRec &g) {
     const double X = double(g.pos.x), Y = double(g.pos.y), Z = double(g.pos.z);
@@ -774,7 +774,7 @@ __device__ __forceinline__ int reproject_pixel(const FrameArgs &a, const HitRec 
     const int j = int(py) * a.W + int(px);
     if (a.spatial_aux[2 * j].w < 0.0f) return -1;
     HitRec gj;
-    if (!load_grec(a.prev_gbuf + j, gj)) return -1;
+    if (!load_grec(a.hist_g + j, gj)) return -1;
     const float dist = length(g.pos - v3(a.prev_origin[0], a.prev_origin[1], a.prev_origin[2]));
     if (fabsf(gj.depth - dist) > 0.1f * dist) return -1;
     return dot(g.ng, gj.ng) >= 0.90630779f ? j : -1;  // cos 25 degrees

@@ -461,9 +461,9 @@ struct hdr_view {
     DevBuf lum_sum, lum_sq, variance;  // Variance metrics (hdr_view_set_metrics): per-pixel sums, reduction
     // Reprojected temporal reuse: the previous frame's camera (hist_cam) and whether that frame left
     // what reproject_kernel reads (hist_records: a single-view spatial frame with
-    // temporal_reprojection, whose final reservoirs, {ray, u} records and G-buffer half gcur stay
-    // until the next frame's gather); rp holds the gathered records (GRec, float4, hdr_reservoir
-    // planes of P entries).
+    // temporal_reprojection, whose final reservoirs, {ray, u} records and kept G-buffer records stay
+    // until the next frame's gather); rp holds the kept G-buffer records and the gathered records
+    // (GRec, GRec, float4, hdr_reservoir planes of P entries).
     bool hist_cam = false, hist_records = false;
     double hist_view[16] = {}, hist_proj[16] = {};
     float hist_origin[3] = {};
@@ -1046,7 +1046,7 @@ static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_
             return fail(HDR_ERR_UNSUPPORTED, "temporal_reprojection runs in the spatial pass (spatial_count >= 1)");
         if (pipe->spatial_count > kMaxSpatial - 1)
             return fail(HDR_ERR_INVALID_ARGUMENT, "spatial_count must be in [1, 7] with temporal_reprojection");
-        need.rp = P * (sizeof(GRec) + sizeof(float4) + sizeof(hdr_reservoir));
+        need.rp = P * (2 * sizeof(GRec) + sizeof(float4) + sizeof(hdr_reservoir));
     }
     if (spatial) {
         need.res_t = P * sizeof(hdr_reservoir);
@@ -1212,17 +1212,15 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
         const bool distortion = a.has_cam && a.cam.lens_distortion != 0.0f;
         a.temporal_skip = moved ? 1 : 0;
         a.reproject = (moved && view->hist_records && view->hist_scene == scene->id && !distortion) ? 1 : 0;
+        if (hdr_status st = ensure(view->rp, need.rp)) return st;
+        a.hist_g = static_cast<GRec *>(view->rp.p);  // kept by every spatial pass of the pipeline
+        a.rp_g = a.hist_g + P;
+        a.rp_aux = reinterpret_cast<float4 *>(a.rp_g + P);
+        a.rp_res = reinterpret_cast<hdr_reservoir *>(a.rp_aux + P);
         if (a.reproject) {
-            if (hdr_status st = ensure(view->rp, need.rp)) return st;
-            a.rp_g = static_cast<GRec *>(view->rp.p);
-            a.rp_aux = reinterpret_cast<float4 *>(a.rp_g + P);
-            a.rp_res = reinterpret_cast<hdr_reservoir *>(a.rp_aux + P);
             std::memcpy(a.prev_view, view->hist_view, sizeof a.prev_view);
             std::memcpy(a.prev_proj, view->hist_proj, sizeof a.prev_proj);
             for (int k = 0; k < 3; ++k) a.prev_origin[k] = view->hist_origin[k];
-            // the previous frame's G-buffer: the half it used (a moved camera never reuses one)
-            a.prev_gbuf = static_cast<const GRec *>(view->gbuf.p) + size_t(view->gcur) * P;
-            if (plan.gbuf_half == view->gcur) a.reproject = 0;
         }
     }
     if (need.rgb2) {
@@ -1354,9 +1352,6 @@ static hdr_status render_frames_impl(hdr_view *view, const hdr_scene *scene, con
     if (a.reproject) {  // the previous frame's records, gathered before this frame overwrites them
         hipLaunchKernelGGL(reproject_kernel, grid, dim3(kLanes), 0, view->stream, da, dt, 1);
         HIP_TRY(hipGetLastError());
-        // the previous G-buffer half has been read: the next frame's G-buffer may overwrite it
-        HIP_TRY(hipEventRecord(view->shade_done[gp ^ 1], view->stream));
-        view->shade_recorded[gp ^ 1] = true;
     }
     if (deferred) {
         HIP_TRY(hipMemsetAsync(a.qcount, 0, 2 * sizeof(unsigned int), view->stream));

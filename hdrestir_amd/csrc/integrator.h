@@ -49,7 +49,8 @@ struct FrameArgs {
     double prev_view[16];              // the previous frame's view and projection matrices
     double prev_proj[16];
     float prev_origin[3];              // the previous camera's position (camera_ray's origin)
-    const GRec *prev_gbuf;             // the previous frame's G-buffer
+    GRec *hist_g;                      // per pixel, the G-buffer record of the last spatial pass (kept by
+                                       //   it for the next frame's gather; null: not kept)
     GRec *rp_g;                        // per pixel: the previous G-buffer record where its hit was seen,
     float4 *rp_aux;                    //   that pixel's {ray dir, wavelength u} (u < 0: none),
     hdr_reservoir *rp_res;             //   and its final reservoir of the previous frame

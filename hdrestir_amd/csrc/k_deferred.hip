@@ -123,6 +123,7 @@ __global__ void __launch_bounds__(kLanes) spatial_kernel(const FrameArgs *__rest
     ShadowQuery q;
     if (valid) {
         const int pixel = y * a.W + x;
+        if (a.hist_g) a.hist_g[pixel] = gbuf[pixel];  // for the next frame's reprojection
         Vertex vq;
         SpecTab tab;
         if (load_spatial_vertex(a, gbuf, pixel, vq, tab)) {
@@ -192,6 +193,7 @@ __global__ void __launch_bounds__(kLanes) wave_spatial_kernel(const FrameArgs *_
     ShadowQuery q;
     if (valid && a.spatial_count > 0) {
         const int pixel = y * a.W + x;
+        if (a.hist_g) a.hist_g[pixel] = a.gbuf[pixel];  // for the next frame's reprojection
         Vertex vq;
         SpecTab tab;
         if (load_spatial_vertex(a, a.gbuf, pixel, vq, tab)) {
@@ -215,11 +217,13 @@ __global__ void __launch_bounds__(kLanes) wave_spatial_kernel(const FrameArgs *_
     if (kCount) wave_add(a.counters + 2, query ? 1ULL : 0ULL);
 }
 
-// Reprojected temporal reuse (an extension): before this frame's shading overwrites them, copies
-// the previous frame's records at the pixel that saw each pixel's primary hit (reproject_pixel) --
-// its G-buffer record, its {ray dir, wavelength u} and its final reservoir -- into the pixel's
-// rp_* slots, which the spatial pass reads as one more domain; rp_aux.w = -1 when there is none.
-// Per pixel 64 + 16 + 80 B read and written, coalesced on the write side.
+// Reprojected temporal reuse (an extension): before this frame's shading and spatial passes
+// overwrite them, copies the previous frame's records at the pixel that saw each pixel's primary
+// hit (reproject_pixel) -- its G-buffer record (as the previous spatial pass kept it), its {ray dir,
+// wavelength u} and its final reservoir -- into the pixel's rp_* slots, which the spatial pass reads
+// as one more domain; rp_aux.w = -1 when there is none. Per pixel 64 + 16 + 80 B read and written,
+// coalesced on the write side. Everything it reads is written on the view's stream, so the next
+// frame's G-buffer launch on the second stream does not wait for it.
 __global__ void __launch_bounds__(kLanes) reproject_kernel(const FrameArgs *__restrict__ args,
                                                            const int *__restrict__ tile_base, int n_views) {
     HDR_TILE(args, tile_base, n_views)
@@ -231,7 +235,7 @@ __global__ void __launch_bounds__(kLanes) reproject_kernel(const FrameArgs *__re
         a.rp_aux[pixel] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
         return;
     }
-    a.rp_g[pixel] = a.prev_gbuf[j];
+    a.rp_g[pixel] = a.hist_g[j];
     a.rp_aux[pixel] = a.spatial_aux[2 * j];
     a.rp_res[pixel] = a.res[j];
 }

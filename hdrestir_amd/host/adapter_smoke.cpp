@@ -79,11 +79,11 @@ struct BoxScene {
     }
 };
 
-hdrgpu::RenderContext Camera(int frame) {
+hdrgpu::RenderContext Camera(int frame, double z = 0.9) {
     hdrgpu::RenderContext c;
-    // camera at z = 0.9 looking down -Z: view = translate(0, 0, -0.9)
+    // camera at (0, 0, z) looking down -Z: view = translate(0, 0, -z)
     for (int i = 0; i < 16; ++i) c.viewMatrix[i] = (i % 5 == 0) ? 1.0 : 0.0;
-    c.viewMatrix[14] = -0.9;
+    c.viewMatrix[14] = -z;
     const double f = 1.0 / std::tan(30.0 * 3.14159265358979 / 180.0), n = 0.01, fa = 100.0;
     c.projMatrix = {f, 0, 0, 0, 0, f, 0, 0, 0, 0, -(fa + n) / (fa - n), -1, 0, 0, -2 * fa * n / (fa - n), 0};
     c.frameIndex = frame;
@@ -131,6 +131,33 @@ int main(int argc, char **argv) {
     }
     std::printf("mean RIS %.5f PathTracer %.5f\n", means[0], means[1]);
     if (!(means[0] > 0.0 && means[1] > 0.0) || std::fabs(means[0] - means[1]) > 0.1 * means[1]) return 1;
+    // reprojected temporal reuse (an extension): a dolly of four frames where each move keeps the
+    // reservoirs (CameraMoved clears only the accumulation); every frame stays finite and lit
+    {
+        hdrgpu::PipelineSettings s;
+        s.kind = hdrgpu::PipelineSettings::Kind::RIS;
+        s.MaxDepth = 1;
+        s.SkipVisibility = true;
+        s.SpatialCount = 2;
+        s.SpatialRadius = 8.0f;
+        s.TemporalReprojection = true;
+        hdrgpu::GpuPathTracePass dolly(ctx, scene, W, H, s);
+        for (int f = 0; f < 4; ++f) {
+            if (f > 0) dolly.CameraMoved();
+            dolly.Execute(Camera(0, 0.9 - 0.02 * f), color);
+            double sum = 0.0;
+            for (size_t i = 0; i < color.size(); i += 4) {
+                if (!std::isfinite(color[i]) || !std::isfinite(color[i + 1]) || !std::isfinite(color[i + 2])) {
+                    std::printf("non-finite pixel with reprojection\n");
+                    return 1;
+                }
+                sum += color[i] + color[i + 1] + color[i + 2];
+            }
+            const double mean = sum / (3.0 * W * H);
+            std::printf("reprojected dolly frame %d mean %.5f\n", f, mean);
+            if (!(mean > 0.5 * means[1] && mean < 2.0 * means[1])) return 1;
+        }
+    }
     // the PathTracerPost passes: the chromaticity blur keeps the mean luma of a frame up to the
     // clamp at 0 and the edge pixels; the OIDN switch is refused
     {
