@@ -4,10 +4,12 @@
 namespace hdr {
 
 // The path tracer on opaque frames whose pixels stay below RNG output 311 (pathlean.h). Block row
-// blockIdx.y renders frame frame_index + blockIdx.y of a launch of a.frames frames.
-template <bool kCount, bool kDebug>
-__global__ void __launch_bounds__(kLanes) pt_lean_kernel(const FrameArgs *__restrict__ args,
-                                                         const int *__restrict__ tile_base, int n_views) {
+// blockIdx.y renders frame frame_index + blockIdx.y of a launch of a.frames frames. kOcc: waves per
+// SIMD the registers are allocated for (2: 256 VGPRs, no spills beyond the stacks; 3: 168 VGPRs,
+// spilling, faster once a launch holds several waves per SIMD).
+template <bool kCount, bool kDebug, int kOcc>
+__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
+pt_lean_kernel(const FrameArgs *__restrict__ args, const int *__restrict__ tile_base, int n_views) {
     HDR_TILE(args, tile_base, n_views)
     PixelStats st = {};
     uint32_t draws = 0;
@@ -30,9 +32,13 @@ __global__ void __launch_bounds__(kLanes) pt_lean_kernel(const FrameArgs *__rest
 
 // ---------------------------------------------------------------------------- instantiations
 // (launched from hdrgpu.hip, which sees the declarations in kernels.h)
-template __global__ void pt_lean_kernel<true, true>(const FrameArgs *__restrict__, const int *__restrict__, int);
-template __global__ void pt_lean_kernel<true, false>(const FrameArgs *__restrict__, const int *__restrict__, int);
-template __global__ void pt_lean_kernel<false, true>(const FrameArgs *__restrict__, const int *__restrict__, int);
-template __global__ void pt_lean_kernel<false, false>(const FrameArgs *__restrict__, const int *__restrict__, int);
+#define HDR_PT_LEAN(C, D)                                                                                        \
+    template __global__ void pt_lean_kernel<C, D, 2>(const FrameArgs *__restrict__, const int *__restrict__, int); \
+    template __global__ void pt_lean_kernel<C, D, 3>(const FrameArgs *__restrict__, const int *__restrict__, int);
+HDR_PT_LEAN(true, true)
+HDR_PT_LEAN(true, false)
+HDR_PT_LEAN(false, true)
+HDR_PT_LEAN(false, false)
+#undef HDR_PT_LEAN
 
 }  // namespace hdr

@@ -27,7 +27,7 @@ __global__ void intersect_kernel(DeviceScene s, const float *o, const float *d, 
 // k_mega.hip
 template <bool kCount, bool kDebug>
 __global__ void integrate_kernel(const FrameArgs *__restrict__ args, const int *__restrict__ tile_base, int n_views);
-template <bool kCount, bool kDebug>
+template <bool kCount, bool kDebug, int kOcc>
 __global__ void pt_lean_kernel(const FrameArgs *__restrict__ args, const int *__restrict__ tile_base, int n_views);
 
 // k_deferred.hip
