@@ -315,10 +315,11 @@ __device__ __forceinline__ Spec ris_direct_lean(const FrameArgs &a, const SpecTa
 // ---------------------------------------------------------------- traced candidates
 // RisDirectLightIntegrator::Li on a vertex whose BSDF-sampled candidates can see an emitter: the
 // environment (escaped generated candidates) or an emissive material (the path's own connection).
-// No mesh light is present (the host checks), so a generated candidate that hits geometry has zero
-// weight (:341-368) and only needs to know whether its ray escapes -- an any-hit query with no bound,
-// which accepts exactly the hits the reference's closest-hit query would find. The connection
-// candidate uses the bounce ray's closest hit, traced by the caller before this call.
+// Without mesh lights a generated candidate that hits geometry has zero weight (:341-368) and only
+// needs to know whether its ray escapes -- an any-hit query with no bound, which accepts exactly the
+// hits the reference's closest-hit query would find. With mesh lights (gen_inline) its closest hit
+// decides it, traced in the shading launch (gen_traced_weight). The connection candidate uses the
+// bounce ray's closest hit, traced by the caller before this call.
 
 // _evaluateBSDFConnection (:547-634) via the connection's hit, with the candidate's RIS weight
 // (:263-294).
@@ -400,11 +401,63 @@ __device__ __forceinline__ double gen_env_weight(const FrameArgs &a, const SpecT
     return mis * double(target) / (double(pBsdf) + 1e-9);
 }
 
+// A generated BSDF candidate in a scene with mesh lights (_generateBSDFCandidates :296-402, as the
+// integration megakernel's gen_bsdf_candidate): its ray's closest hit, traced here out of line,
+// decides it. An escape sees the environment (gen_env_weight); a hit on an instance registered as a
+// light sees that light's emission (_evaluateBSDFConnection :547-634, weighted against the light's
+// pdf); any other hit is no candidate. `escaped` reports the escape.
+template <bool kCount>
+__device__ __forceinline__ double gen_traced_weight(const FrameArgs &a, const SpecTab &tab, const Bounce &b,
+                                                    TraceStats &ts, bool &escaped, Cand *out) {
+    const DeviceScene &s = a.s;
+    RawHit rh;
+    escaped = !trace_closest_call<kCount>(s, b.o, b.d, rh, ts);
+    if (escaped) return gen_env_weight(a, tab, b, out);
+    const HitRec hit = make_hit_record_call(s, b.o, b.d, rh);
+    const int hl = s.instances[hit.inst].light;
+    if (hl < 0) return 0.0;
+    const V3 emissionRGB = hit_emission(s, hit);
+    const Spec emission = rgb2spec(tab, emissionRGB);
+    const float sel = a.light_select_pdf;
+    float pNee = 0.0f;
+    {
+        const V3 off = hit.pos - b.o;
+        const float dist2 = dot(off, off);
+        const float cosY = smax(0.0f, dot(-b.d, hit.ng));
+        if (cosY > 0.0f && sel > 0.0f) {
+            bool solid;
+            const float ap = light_eval_pdf(s, s.lights[hl], b.d, solid);
+            pNee = sel * to_solid_angle(ap, solid, dist2, cosY);
+        }
+    }
+    const Spec contribution = emission * b.imul;
+    const float target = spec_lum(tab, contribution);
+    const double mis = double(power3(b.pdf, a.n_bsdf, pNee, a.n_nee, 0.0f, a.n_sky));
+    if (out) {
+        bool solid;
+        float lp = light_eval_pdf(s, s.lights[hl], b.d, solid);
+        lp *= sel;
+        out->has_cand = true;
+        out->delta = s.lights[hl].is_delta != 0;
+        out->solid = solid;
+        out->vis_tested = true;
+        out->light = hl;
+        out->dir = b.d;
+        out->color = emissionRGB;
+        out->lnormal = hit.ng;
+        out->dist = hit.depth;
+        out->pdf = lp;
+        out->tp = contribution;
+        out->target = target;
+    }
+    return mis * double(target) / (double(b.pdf) + 1e-9);
+}
+
 // Rebuilds candidate kk by replaying the stream from `rng` (taken before candidate 0): NEE
 // candidates as rebuild_nee_lean, the connection from its hit, a generated candidate by replaying
 // the draws before it and sampling it again (esc: which generated candidates escaped and, without
 // skipVisibility, which NEE candidates' shadow rays were unoccluded).
-template <bool kDebug, class R>
+template <bool kDebug, bool kGen, class R>
 __device__ __forceinline__ void rebuild_traced(const FrameArgs &a, const SpecTab &tab, const Vertex &v,
                                                const Bounce *conn, bool connHit, const HitRec &connRec,
                                                bool connCand, R rng, uint32_t esc, int kk, Trace &dbg, Cand &out) {
@@ -430,8 +483,14 @@ __device__ __forceinline__ void rebuild_traced(const FrameArgs &a, const SpecTab
     for (; k < kk; ++k) skip_bounce_draws_inl(tab, v.c, v.sn, v.inside, v.rayDir, rng);
     Bounce b;
     int refl = 0, refr = 0;
-    (void)sample_bounce_inl(tab, v.c, v.sn, v.inside, v.h.pos, v.rayDir, 1, 1, refl, refr, rng, b);
-    if ((esc >> kk) & 1u) (void)gen_env_weight(a, tab, b, &out);
+    const bool sampled = sample_bounce_inl(tab, v.c, v.sn, v.inside, v.h.pos, v.rayDir, 1, 1, refl, refr, rng, b);
+    if (kGen && a.gen_inline) {  // the winner's ray again (not counted: the weight pass counted it)
+        TraceStats unused = {};
+        bool escaped;
+        if (sampled) (void)gen_traced_weight<false>(a, tab, b, unused, escaped, &out);
+    } else if ((esc >> kk) & 1u) {
+        (void)gen_env_weight(a, tab, b, &out);
+    }
 }
 
 // RisDirectLightIntegrator::Li (:84-215) with traced BSDF candidates; weights of every candidate in
@@ -439,7 +498,7 @@ __device__ __forceinline__ void rebuild_traced(const FrameArgs &a, const SpecTab
 // generated candidates' escape tests (any hit up to 1e30, from the sampled ray) were answered
 // beforehand: bit k of `escm` is set when candidate k's ray escapes. Without skipVisibility the NEE
 // candidates' shadow rays were answered too: bit k < n_nee is set when candidate k's is unoccluded.
-template <bool kCount, bool kDebug, class R>
+template <bool kCount, bool kDebug, bool kGen, class R>
 __device__ __forceinline__ Spec ris_direct_traced(const FrameArgs &a, const SpecTab &tab, const Vertex &v,
                                                   const Bounce *conn, bool connHit, const HitRec &connRec, R &rng,
                                                   double *wl, int pixel, PixelStats &st, Trace &dbg, bool reservoir,
@@ -474,7 +533,11 @@ __device__ __forceinline__ Spec ris_direct_traced(const FrameArgs &a, const Spec
             int refl = 0, refr = 0;
             if (sample_bounce_inl(tab, v.c, v.sn, v.inside, v.h.pos, v.rayDir, 1, 1, refl, refr, rng, b)) {
                 if (kCount) ++st.cand;
-                if ((escm >> k) & 1u) {
+                if (kGen && a.gen_inline) {
+                    bool escaped;
+                    w = gen_traced_weight<kCount>(a, tab, b, st.ts, escaped, nullptr);
+                    if (escaped) esc |= 1u << k;
+                } else if ((escm >> k) & 1u) {
                     esc |= 1u << k;
                     w = gen_env_weight(a, tab, b, nullptr);
                 }
@@ -537,7 +600,7 @@ __device__ __forceinline__ Spec ris_direct_traced(const FrameArgs &a, const Spec
         }
         if (kDebug) dbg.shash = fnv(dbg.shash, uint32_t(sel));
         if (sel >= 0) {
-            rebuild_traced<kDebug>(a, tab, v, conn, connHit, connRec, connCand, start, esc, sel, dbg, ch);
+            rebuild_traced<kDebug, kGen>(a, tab, v, conn, connHit, connRec, connCand, start, esc, sel, dbg, ch);
             has = true;
         } else if (sel == -2) {
             ch = prevC;
@@ -586,7 +649,7 @@ __device__ __forceinline__ Spec ris_direct_traced(const FrameArgs &a, const Spec
     }
     if (kDebug) dbg.shash = fnv(dbg.shash, uint32_t(chosen));
     if (chosen < 0) return spec(0.0f);
-    rebuild_traced<kDebug>(a, tab, v, conn, connHit, connRec, connCand, start, esc, chosen, dbg, ch);
+    rebuild_traced<kDebug, kGen>(a, tab, v, conn, connHit, connRec, connCand, start, esc, chosen, dbg, ch);
     const float W = float(sum / double(ch.target));
     Spec tp = ch.tp;
     if (!ch.vis_tested && ch.has_cand) tp = winner_contribution_lean(tab, v, ch, query, qo, qd, qtmax);

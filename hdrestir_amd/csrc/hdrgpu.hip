@@ -368,11 +368,14 @@ static void launch_wave_trace(const LaunchShape &ls, int blocks, hipStream_t st,
 // Shading launch b >= 1; traced frames: wave_shade_traced_kernel.
 template <bool C, bool D>
 static void launch_wave_shade(const LaunchShape &ls, int blocks, size_t lds, hipStream_t st, const FrameArgs *a, int b,
-                              bool traced) {
+                              bool traced, bool gen = false) {
     const dim3 g(blocks), l(kLanes);
-    if (traced) {
-        if (ls.shade_occ == 3) hipLaunchKernelGGL((wave_shade_traced_kernel<C, D, 3>), g, l, lds, st, a, b);
-        else hipLaunchKernelGGL((wave_shade_traced_kernel<C, D, 1>), g, l, lds, st, a, b);
+    if (traced && gen) {  // a view with mesh lights: the build that traces generated candidates
+        if (ls.shade_occ == 3) hipLaunchKernelGGL((wave_shade_traced_kernel<C, D, 3, true>), g, l, lds, st, a, b);
+        else hipLaunchKernelGGL((wave_shade_traced_kernel<C, D, 1, true>), g, l, lds, st, a, b);
+    } else if (traced) {
+        if (ls.shade_occ == 3) hipLaunchKernelGGL((wave_shade_traced_kernel<C, D, 3, false>), g, l, lds, st, a, b);
+        else hipLaunchKernelGGL((wave_shade_traced_kernel<C, D, 1, false>), g, l, lds, st, a, b);
     } else {
         if (ls.shade_occ == 3) hipLaunchKernelGGL((wave_shade_kernel<C, D, 3>), g, l, lds, st, a, b);
         else hipLaunchKernelGGL((wave_shade_kernel<C, D, 1>), g, l, lds, st, a, b);
@@ -1045,12 +1048,16 @@ static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_
     // GPU's SIMDs underfilled in every one of the wavefront's launches), while 384 x 384 takes 4.7 ms
     // as a wavefront and 1080p 20 ms (187 ms on the megakernel). Both give the same results.
     const bool needTraced = traceGenerated || traceLastConnection || !skipVis;
-    const bool wave = !deferred && ris && !scene->any_translucent && !scene->ds.has_mesh_lights &&
-                      !(needTraced && nCand > 32) && (skipVis || P >= kVisWavefrontPixels);
+    // With mesh lights a generated candidate's closest hit decides it: the traced shading launches
+    // trace those rays themselves (gen_traced_weight) instead of emitting escape rays.
+    // Visibility frames with mesh lights stay on the megakernel, which measured faster there (1080p
+    // ReSTIR M = 8 at depth 2: 20.3 ms against 21.5 ms as a wavefront).
+    const bool wave = !deferred && ris && !scene->any_translucent && !(needTraced && nCand > 32) &&
+                      (skipVis || (P >= kVisWavefrontPixels && !scene->ds.has_mesh_lights));
     const bool waveTraced = wave && needTraced;
     // Spatial reuse (an extension) runs on the deferred and wavefront paths with temporal reservoirs.
     const bool spatial = pipe->spatial_count > 0;
-    if (spatial && !((deferred || wave) && pipe->use_reservoir && skipVis))
+    if (spatial && !((deferred || wave) && pipe->use_reservoir && skipVis && !scene->ds.has_mesh_lights))
         return fail(HDR_ERR_UNSUPPORTED,
                     "spatial reuse needs ReSTIR (use_reservoir) with skip_visibility, opaque materials and no mesh "
                     "light");
@@ -1075,7 +1082,8 @@ static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_
         // traced frames: hit records of odd and even vertices, escape masks and the escape queue
         need.whit = (waveTraced ? 2 : 1) * P * sizeof(GRec);
         if (waveTraced) {  // per vertex: the generated candidates' escape rays, the NEE candidates' shadow rays
-            plan.escapes = (traceGenerated ? P * size_t(nCand - nNee) : 0) + (skipVis ? 0 : P * size_t(nNee));
+            const bool genEscapes = traceGenerated && !scene->ds.has_mesh_lights;  // else traced inline
+            plan.escapes = (genEscapes ? P * size_t(nCand - nNee) : 0) + (skipVis ? 0 : P * size_t(nNee));
             need.wesc = P * sizeof(unsigned int);
             need.wesq = escape_queue_bytes(plan.escapes);
         }
@@ -1159,6 +1167,7 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
     a.n_cand = a.n_nee + a.n_sky + a.n_bsdf;
     a.trace_generated = (scene->ds.has_mesh_lights || scene->ds.environment >= 0) ? 1 : 0;
     a.trace_last_connection = (scene->ds.has_emission || scene->ds.environment >= 0) ? 1 : 0;
+    a.gen_inline = (plan.wave && scene->ds.has_mesh_lights) ? 1 : 0;
     a.light_select_pdf = scene->ds.num_lights > 0 ? 1.0f / float(scene->ds.num_lights) : 0.0f;
     a.firefly = pipe->enable_firefly_filter != 0;
     a.inv_n = 1.0f / float(frame->frame_index + 1);
@@ -1415,7 +1424,8 @@ static hdr_status render_frames_impl(hdr_view *view, const hdr_scene *scene, con
         hipLaunchKernelGGL(wave_zero_counts_kernel, dim3(1), dim3(kLanes), 0, view->stream, a.wcnt);
 #define HDR_W0(C, D) \
     launch_wave_shade0<C, D>(plan.wave_lean0, plan.wave_traced, plan.tiles, lds, view->stream, da, dt, 1)
-#define HDR_WS(C, D) launch_wave_shade<C, D>(view->shape, qblocks, lds, view->stream, da, b, plan.wave_traced)
+#define HDR_WS(C, D) \
+    launch_wave_shade<C, D>(view->shape, qblocks, lds, view->stream, da, b, plan.wave_traced, a.gen_inline != 0)
         if (view->counting) {
             if (debug_out) HDR_W0(true, true);
             else HDR_W0(true, false);
@@ -2109,6 +2119,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
     int waveDepth[2] = {0, 0};
     bool waveLean0[2] = {true, true};  // every view's primary vertex stays below RNG output 311
     bool waveSpatial[2] = {false, false};  // some view of the group reuses spatially
+    bool waveGen[2] = {false, false};      // some view of the group traces generated candidates (mesh lights)
     int wavePrepWindow[2] = {0, 0};        // slot preparation (FramePlan::prep_window): the largest window
     int wavePrepVertex[2] = {INT_MAX, INT_MAX};  // and the first vertex that needs it, over the group
     bool oneScene = true;                  // every view renders the first view's scene
@@ -2131,6 +2142,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
                 wavePrepVertex[t] = std::min(wavePrepVertex[t], plans[i].prep_vertex);
             }
             waveSpatial[t] = waveSpatial[t] || plans[i].spatial;
+            waveGen[t] = waveGen[t] || args[i].gen_inline != 0;
         }
     }
     if (queueCap * sizeof(ShadowQuery) > b->queue.n || waveCap * sizeof(ShadowQuery) > b->wqueue.n ||
@@ -2269,7 +2281,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
                     if (wavePrepWindow[t] > 0 && u >= wavePrepVertex[t])
                         hipLaunchKernelGGL(wave_prepare_slots_kernel, dim3(qblocks), dim3(kLanes), 0, b->stream, A(wg),
                                            u, wavePrepWindow[t]);
-                    launch_wave_shade<false, false>(ls, qblocks, lds[wg], b->stream, A(wg), v, t == 1);
+                    launch_wave_shade<false, false>(ls, qblocks, lds[wg], b->stream, A(wg), v, t == 1, waveGen[t]);
                 }
                 if (waveSpatial[t] && v == t)
                     launch_wave_spatial<false>(spatialNd[wg], dim3(tiles[wg]), b->stream, A(wg), T(wg), N(wg));
@@ -2298,7 +2310,7 @@ hdr_status hdr_render_batch(hdr_batch *b, int32_t n, hdr_view *const *views, con
                                      uint64_t(uintptr_t(b->wcq.p)), uint64_t(uintptr_t(b->walive.p)),
                                      uint64_t(queueCap), uint64_t(waveCap), uint64_t(waveDepth[0]),
                                      uint64_t(waveDepth[1]), uint64_t(waveLean0[0]), uint64_t(waveLean0[1]),
-                                     uint64_t(waveSpatial[0]), uint64_t(waveSpatial[1]),
+                                     uint64_t(waveSpatial[0]), uint64_t(waveSpatial[1]), uint64_t(waveGen[1]),
                                      uint64_t(wavePrepWindow[0]), uint64_t(wavePrepWindow[1]),
                                      uint64_t(uint32_t(wavePrepVertex[0])), uint64_t(uint32_t(wavePrepVertex[1])),
                                      uint64_t(ls.queue_waves), uint64_t(ls.trace_occ), uint64_t(ls.shade_occ),

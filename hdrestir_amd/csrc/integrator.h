@@ -32,6 +32,8 @@ struct FrameArgs {
     int n_nee, n_sky, n_bsdf, n_cand;  // RIS technique split (ris_direct_light_integrator.cpp:93-100)
     int trace_generated;               // 0: generated BSDF candidates provably cannot contribute
     int trace_last_connection;         // 0: the last vertex's BSDF connection ray provably cannot contribute
+    int gen_inline;                    // wavefront with mesh lights: a generated candidate's closest hit is
+                                       //   traced in the shading launch (an escape ray cannot decide it)
     float light_select_pdf;            // 1 / number of lights
     int firefly;
     float inv_n;                       // 1 / (frame_index + 1)

@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(kLanes) wave_shade0_kernel(const FrameArgs *__
                 nv = a.skip_vis == 0 && a.s.num_lights > 0 ? a.n_nee : 0;
             }
         } else if (p.primary_hit) {
-            wave_vertex<kCount, kDebug, kTraced>(a, tab, 0, g, p, wl + lane, pixel, st, query, q, cq, ray, ro, rd,
+            wave_vertex<kCount, kDebug, kTraced, false>(a, tab, 0, g, p, wl + lane, pixel, st, query, q, cq, ray, ro, rd,
                                                  next);
         }
         if (!p.primary_hit && a.s.environment >= 0 && a.ibl_bg)  // a miss sees the environment (PathIntegrator::Li, path_integrator.cpp:48-209)

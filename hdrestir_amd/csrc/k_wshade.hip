@@ -29,7 +29,7 @@ wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce) {
             const SpecTab tab = make_tab(p.utab);
             HitRec h, next;
             (void)load_grec(a.whit + pixel, h);
-            wave_vertex<kCount, kDebug, false>(a, tab, bounce, h, p, wl + lane, pixel, st, query, q, cq, ray, ro, rd,
+            wave_vertex<kCount, kDebug, false, false>(a, tab, bounce, h, p, wl + lane, pixel, st, query, q, cq, ray, ro, rd,
                                                next);
             store_path(a, pixel, p, false);
         }

@@ -7,7 +7,9 @@ namespace hdr {
 // path in vertex u's live list -- its ray queries were answered by the traversal launch before --
 // and, for the paths that go on, emits vertex v's ray queries (bounce ray, the NEE candidates'
 // shadow rays without skipVisibility, escape rays) and appends the path to vertex v's live list. Vertex u's shadow query joins the queue of parity u.
-template <bool kCount, bool kDebug, int kOcc = 1>
+// kGen: some view of the launch has mesh lights, so its generated candidates are traced here
+// (gen_traced_weight); the other builds hold no traversal code.
+template <bool kCount, bool kDebug, int kOcc = 1, bool kGen = false>
 __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(kOcc)))
 wave_shade_traced_kernel(const FrameArgs *__restrict__ args, int v) {
     extern __shared__ double wl[];
@@ -37,7 +39,7 @@ wave_shade_traced_kernel(const FrameArgs *__restrict__ args, int v) {
             tab = make_tab(p.utab);
             HitRec h, next;
             (void)load_grec((u == 0 ? a.gbuf : traced_hits(a, u)) + pixel, h);
-            wave_vertex<kCount, kDebug, true>(a, tab, u, h, p, wl + lane, pixel, st, query, q, cq, cont, ro, rd,
+            wave_vertex<kCount, kDebug, true, kGen>(a, tab, u, h, p, wl + lane, pixel, st, query, q, cq, cont, ro, rd,
                                               next);
             if (u == 0 && a.spatial_count > 0) a.spatial_aux[2 * pixel + 1] = spec4(p.outer);
             if (cont) {
@@ -71,18 +73,25 @@ wave_shade_traced_kernel(const FrameArgs *__restrict__ args, int v) {
         wave_add(c + 2, nq);
         wave_add(c + 3, st.cand);
         wave_add(c + 4, st.cand_skipped);
+        if (kGen) {  // the generated candidates' traversals
+            wave_add(c + 5, st.ts.nodes);
+            wave_add(c + 6, st.ts.tris);
+            wave_add(c + 7, st.ts.instances);
+        }
     }
 }
 
 // ---------------------------------------------------------------------------- instantiations
 // (launched from hdrgpu.hip, which sees the declarations in kernels.h)
-template __global__ void wave_shade_traced_kernel<true, true, 1>(const FrameArgs *__restrict__, int);
-template __global__ void wave_shade_traced_kernel<true, true, 3>(const FrameArgs *__restrict__, int);
-template __global__ void wave_shade_traced_kernel<true, false, 1>(const FrameArgs *__restrict__, int);
-template __global__ void wave_shade_traced_kernel<true, false, 3>(const FrameArgs *__restrict__, int);
-template __global__ void wave_shade_traced_kernel<false, true, 1>(const FrameArgs *__restrict__, int);
-template __global__ void wave_shade_traced_kernel<false, true, 3>(const FrameArgs *__restrict__, int);
-template __global__ void wave_shade_traced_kernel<false, false, 1>(const FrameArgs *__restrict__, int);
-template __global__ void wave_shade_traced_kernel<false, false, 3>(const FrameArgs *__restrict__, int);
+#define HDR_WST(C, D)                                                                              \
+    template __global__ void wave_shade_traced_kernel<C, D, 1, false>(const FrameArgs *__restrict__, int); \
+    template __global__ void wave_shade_traced_kernel<C, D, 3, false>(const FrameArgs *__restrict__, int); \
+    template __global__ void wave_shade_traced_kernel<C, D, 1, true>(const FrameArgs *__restrict__, int);  \
+    template __global__ void wave_shade_traced_kernel<C, D, 3, true>(const FrameArgs *__restrict__, int);
+HDR_WST(true, true)
+HDR_WST(true, false)
+HDR_WST(false, true)
+HDR_WST(false, false)
+#undef HDR_WST
 
 }  // namespace hdr

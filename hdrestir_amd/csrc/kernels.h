@@ -55,7 +55,7 @@ __global__ void wave_shade0_kernel(const FrameArgs *__restrict__ args, const int
 // k_wshade.hip
 template <bool kCount, bool kDebug, int kOcc>
 __global__ void wave_shade_kernel(const FrameArgs *__restrict__ args, int bounce);
-template <bool kCount, bool kDebug, int kOcc>
+template <bool kCount, bool kDebug, int kOcc, bool kGen>
 __global__ void wave_shade_traced_kernel(const FrameArgs *__restrict__ args, int v);
 __global__ void wave_prepare_slots_kernel(const FrameArgs *__restrict__ args, int u, int window);
 

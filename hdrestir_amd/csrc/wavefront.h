@@ -155,7 +155,7 @@ __device__ __forceinline__ GRec *traced_hits(const FrameArgs &a, int v) { return
 // the hit buffer of vertex bounce + 1, the generated candidates' escape tests from the pixel's
 // escape mask -- and the next vertex's hit is returned in `next` (ray = a next vertex exists)
 // instead of a bounce ray.
-template <bool kCount, bool kDebug, bool kTraced, class R>
+template <bool kCount, bool kDebug, bool kTraced, bool kGen, class R>
 __device__ __forceinline__ void wave_vertex(const FrameArgs &a, const SpecTab &tab, int bounce, const HitRec &h,
                                             WavePath<R> &p, double *wl, int pixel, PixelStats &st, bool &query,
                                             ShadowQuery &q, float4 &cq, bool &ray, float4 &ro, float4 &rd,
@@ -191,7 +191,7 @@ __device__ __forceinline__ void wave_vertex(const FrameArgs &a, const SpecTab &t
     uint64_t *slot = wave_slot(a, pixel);
     Spec d;
     if (kTraced)
-        d = ris_direct_traced<kCount, kDebug>(a, tab, v, ok ? &b : nullptr, cHit, next, p.rng, wl, pixel, st, p.dbg,
+        d = ris_direct_traced<kCount, kDebug, kGen>(a, tab, v, ok ? &b : nullptr, cHit, next, p.rng, wl, pixel, st, p.dbg,
                                               first && a.use_res, slot ? slot + kMtN : nullptr,
                                               (a.trace_generated || a.skip_vis == 0) ? a.wesc[pixel] : 0u, qq,
                                               qo, qd, qtmax);
@@ -267,7 +267,8 @@ __device__ __forceinline__ void wave_gen(const FrameArgs &a, const SpecTab &tab,
             for (int q = 0; q < nd; ++q) r.discard();
         }
     }
-    if (!a.trace_generated) return;
+    // with mesh lights the shading launch traces the generated candidates itself (gen_traced_weight)
+    if (!a.trace_generated || a.gen_inline) return;
     k0 = a.n_nee + a.n_sky + ((ok && b.pdf > 0.0f) ? 1 : 0);  // after the connection candidate
 }
 

@@ -429,6 +429,12 @@ COUNTER_CASES = {
                                                         opaque=True),
                                 abi.pipeline("ris", candidate_count=8, use_reservoir=True, skip_visibility=True,
                                              max_depth=4), True),
+    # wavefront with a mesh light: the generated candidates' closest hits traced in the shading
+    # launches count once each (the winner's second trace is not counted)
+    "mesh_restir8_skipvis_d3": (lambda: scenes.features(48, 40, dome=True, emissive=True, mesh_light=True,
+                                                        opaque=True),
+                                abi.pipeline("ris", candidate_count=8, use_reservoir=True, skip_visibility=True,
+                                             max_depth=3), True),
 }
 
 
@@ -556,5 +562,77 @@ def test_ris_with_visibility_counts_its_shadow_rays(gpu_ctx, oracle_mod):
     assert g["primary_rays"] == c["primary_rays"]
     assert (g["bounce_rays"] + g["candidate_rays"] + g["candidate_rays_skipped"] ==
             c["bounce_rays"] + c["candidate_rays"])
+    r.close()
+    gs.close()
+
+
+MESH_CASES = {
+    # ReSTIR at every vertex, emissive sphere registered as a mesh light, no environment
+    "mesh_restir8_skipvis_d3": (lambda: scenes.features(48, 40, dome=False, emissive=True, mesh_light=True,
+                                                        opaque=True),
+                                abi.pipeline("ris", candidate_count=8, use_reservoir=True, skip_visibility=True,
+                                             max_depth=3)),
+    # with the dome as well: generated candidates that escape see the environment
+    "mesh_dome_ris16_skipvis_d2": (lambda: scenes.features(48, 40, dome=True, emissive=True, mesh_light=True,
+                                                           opaque=True),
+                                   abi.pipeline("ris", candidate_count=16, use_reservoir=False, skip_visibility=True,
+                                                max_depth=2)),
+}
+
+
+@pytest.mark.parametrize("case", list(MESH_CASES))
+def test_mesh_lights_on_the_wavefront(gpu_ctx, oracle_mod, case):
+    """Opaque frames with a mesh light run on the wavefront: a generated candidate's closest hit
+    decides it (a hit on the mesh light's instance sees its emission, weighted against the light's
+    pdf), traced in the shading launch; bounce and shadow rays stay on the traversal launches.
+    Streams equal the oracle's on every pixel, colours within the shared bar, reservoirs equal; the
+    stage record shows the wavefront."""
+    from hdrestir_amd import gpu
+    make, pipe = MESH_CASES[case]
+    sc = make()
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    o = oracle_mod.Oracle(sc, threads=8)
+    r.stage_times()
+    for f in range(3):
+        fr = scenes.frame_with_camera(sc, f)
+        img = r.render(fr, pipe, debug=True).copy()
+        ref = o.render(fr, pipe).copy()
+        assert np.array_equal(r.debug["prim_instance"], o.debug["prim_instance"])
+        parity.check(f"{case} frame {f}", _match(r, o), r.debug["sample_rgb"], o.debug["sample_rgb"], img, ref)
+        if pipe.use_reservoir:
+            gr, orr = r.reservoirs(), o.reservoirs()
+            assert np.array_equal(gr["flags_light"], orr["flags_light"])
+            assert np.array_equal(gr["count"], orr["count"])
+    st = r.stage_times()
+    assert st["accumulate"][1] == 3 and st["shadow"][1] == 0, st  # the wavefront ran
+    r.close()
+    gs.close()
+
+
+def test_mesh_light_visibility_frames_stay_on_the_megakernel(gpu_ctx):
+    """With a mesh light, RIS without skipVisibility keeps the megakernel at any view size (it
+    measured faster there than the wavefront with generated candidates traced in the shading
+    launches): no accumulation launch."""
+    from hdrestir_amd import gpu
+    sc = scenes.features(VW, VH, dome=False, emissive=True, mesh_light=True, opaque=True)
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    r.stage_times()
+    r.render(sc.frame(0), abi.pipeline("ris", candidate_count=8, use_reservoir=True, max_depth=2), read_color=False)
+    st = r.stage_times()
+    assert st["integrate"][1] == 1 and st["accumulate"][1] == 0, st
+    r.close()
+    gs.close()
+
+
+def test_spatial_reuse_with_a_mesh_light_is_refused(gpu_ctx):
+    from hdrestir_amd import gpu
+    sc = scenes.features(16, 16, dome=False, emissive=True, mesh_light=True, opaque=True)
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    with pytest.raises(gpu.HdrError, match="UNSUPPORTED"):
+        r.render(sc.frame(0), abi.pipeline("ris", candidate_count=8, use_reservoir=True, skip_visibility=True,
+                                           max_depth=2, spatial_count=2, spatial_radius=4.0))
     r.close()
     gs.close()
