@@ -975,6 +975,11 @@ struct FrameNeeds {
 
 // Smallest view that renders RIS without skipVisibility on the wavefront (plan_pipeline).
 constexpr size_t kVisWavefrontPixels = size_t(1) << 17;
+// Smallest view that renders the path tracer on the wavefront (plan_pipeline). Measured on the
+// many-lights scene, lean path tracer against wavefront, ms per frame at depth 4 / depth 8:
+// 256^2 1.72 / 2.45 against 2.26 / 4.54, 512^2 2.04 / 3.06 against 2.63 / 5.10, 1280x720
+// 3.93 / 5.49 against 3.58 / 6.20, 1920x1080 6.84 / 9.28 against 5.47 / 8.36 (identical images).
+constexpr size_t kPtWavefrontPixels = size_t(1) << 20;
 
 // The escape queue of `cap` entries: two float4 each, then a float (tmax) each.
 static size_t escape_queue_bytes(size_t cap) {
@@ -991,8 +996,10 @@ static size_t view_base_bytes(size_t P) {
 
 // Validates a pipeline on a scene and decides the frame's kernel path and the buffers it needs.
 // Host-only: no device call; hdr_view_plan_bytes and the server's admission use it too.
+// multi: the frame shares a launch with others (hdr_render_frames), so a path-traced frame keeps
+// the lean path tracer, which takes several frames per launch, rather than the wavefront.
 static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_pipeline_desc *pipe, bool debug,
-                                FramePlan &plan, FrameNeeds &need) {
+                                FramePlan &plan, FrameNeeds &need, bool multi = false) {
     if (!scene || !pipe) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
     if (pipe->struct_size != sizeof(hdr_pipeline_desc))
         return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_pipeline_desc size");
@@ -1052,12 +1059,17 @@ static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_
     // trace those rays themselves (gen_traced_weight) instead of emitting escape rays.
     // Visibility frames with mesh lights stay on the megakernel, which measured faster there (1080p
     // ReSTIR M = 8 at depth 2: 20.3 ms against 21.5 ms as a wavefront).
-    const bool wave = !deferred && ris && !scene->any_translucent && !(needTraced && nCand > 32) &&
-                      (skipVis || (P >= kVisWavefrontPixels && !scene->ds.has_mesh_lights));
-    const bool waveTraced = wave && needTraced;
+    // The path tracer (MIS) takes the untraced wavefront on views from kPtWavefrontPixels when its
+    // BSDF rays can reach no emitter (mis_nee_wave: the connection then adds nothing).
+    const bool ptWave = !ris && !scene->any_translucent && !traceLastConnection && !scene->ds.has_mesh_lights &&
+                        P >= kPtWavefrontPixels && !multi;
+    const bool wave = (!deferred && ris && !scene->any_translucent && !(needTraced && nCand > 32) &&
+                       (skipVis || (P >= kVisWavefrontPixels && !scene->ds.has_mesh_lights))) ||
+                      ptWave;
+    const bool waveTraced = wave && ris && needTraced;
     // Spatial reuse (an extension) runs on the deferred and wavefront paths with temporal reservoirs.
     const bool spatial = pipe->spatial_count > 0;
-    if (spatial && !((deferred || wave) && pipe->use_reservoir && skipVis && !scene->ds.has_mesh_lights))
+    if (spatial && !(ris && (deferred || wave) && pipe->use_reservoir && skipVis && !scene->ds.has_mesh_lights))
         return fail(HDR_ERR_UNSUPPORTED,
                     "spatial reuse needs ReSTIR (use_reservoir) with skip_visibility, opaque materials and no mesh "
                     "light");
@@ -1098,7 +1110,7 @@ static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_
     }
     // candidate weights in LDS: the NEE candidates on the deferred path (its BSDF candidates carry
     // no weight), every candidate on the general path
-    plan.lds = size_t((deferred || (wave && !waveTraced)) ? nNee : (ris ? nCand : 1)) * kLanes * sizeof(double);
+    plan.lds = size_t(!ris ? 1 : ((deferred || (wave && !waveTraced)) ? nNee : nCand)) * kLanes * sizeof(double);
     plan.deferred = deferred;
     plan.wave = wave;
     plan.wave_lean0 = wave && bound0 < kMtN - 1;
@@ -1110,7 +1122,7 @@ static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_
     }
     plan.wave_traced = waveTraced;
     plan.depth = pipe->max_depth;
-    plan.lean_pt = !deferred && !ris && !scene->any_translucent && bound < kMtN - 1;
+    plan.lean_pt = !deferred && !wave && !ris && !scene->any_translucent && bound < kMtN - 1;
     plan.spatial = spatial;
     plan.spatial_nd = spatial ? spatial_domains(pipe->spatial_count + (pipe->temporal_reprojection ? 1 : 0)) : 0;
     plan.tiles = unsigned(((W + 7) / 8) * ((H + 7) / 8));
@@ -1121,14 +1133,14 @@ static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_
 // shared_queues: a batch substitutes its own wavefront queues, so the view's are not allocated.
 static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hdr_frame_desc *frame,
                                 const hdr_pipeline_desc *pipe, bool debug, FrameArgs &a, FramePlan &plan,
-                                bool shared_queues = false) {
+                                bool shared_queues = false, bool multi = false) {
     if (!view || !scene || !frame || !pipe) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
     if (frame->struct_size != sizeof(hdr_frame_desc)) return fail(HDR_ERR_INVALID_ARGUMENT, "hdr_frame_desc size");
     if (view->ctx->cancelled) return fail(HDR_ERR_CANCELLED, "render cancelled");
     const int W = view->W, H = view->H;
     const size_t P = size_t(W) * H;
     FrameNeeds need;
-    if (hdr_status st = plan_pipeline(scene, W, H, pipe, debug, plan, need)) return st;
+    if (hdr_status st = plan_pipeline(scene, W, H, pipe, debug, plan, need, multi)) return st;
     HIP_TRY(hipSetDevice(view->ctx->device));
 
     std::memset(&a, 0, sizeof a);
@@ -1152,7 +1164,7 @@ static hdr_status prepare_frame(hdr_view *view, const hdr_scene *scene, const hd
         a.exposure_mul = 1.0f;
     }
     a.direct = pipe->direct_light;
-    a.use_res = pipe->use_reservoir != 0;
+    a.use_res = (pipe->direct_light == HDR_DIRECT_RIS && pipe->use_reservoir != 0) ? 1 : 0;  // reservoirs are RIS state
     a.skip_vis = pipe->skip_visibility != 0;
     a.max_depth = pipe->max_depth;
     a.max_refl = pipe->max_reflection_bounces;
@@ -1323,7 +1335,8 @@ static hdr_status render_frames_impl(hdr_view *view, const hdr_scene *scene, con
                                      int frames) {
     FrameArgs a;
     FramePlan plan;
-    if (hdr_status st = prepare_frame(view, scene, frame, pipe, debug_out != nullptr, a, plan)) return st;
+    if (hdr_status st = prepare_frame(view, scene, frame, pipe, debug_out != nullptr, a, plan, false, frames > 1))
+        return st;
     const int W = view->W, H = view->H;
     const size_t P = size_t(W) * H;
     if (frames > 1) {  // the frames' samples are staged, then accumulated in frame order
@@ -1574,7 +1587,7 @@ static bool multi_frame_ok(const hdr_view *view, const hdr_scene *scene, const h
                            const hdr_pipeline_desc *pipe) {
     FramePlan plan;
     FrameNeeds need;
-    if (plan_pipeline(scene, view->W, view->H, pipe, false, plan, need) != HDR_OK) return false;
+    if (plan_pipeline(scene, view->W, view->H, pipe, false, plan, need, true) != HDR_OK) return false;
     if (plan.deferred || plan.wave || plan.spill) return false;
     if (pipe->enable_firefly_filter || pipe->spatial_count > 0) return false;
     if (pipe->direct_light == HDR_DIRECT_RIS && pipe->use_reservoir) return false;

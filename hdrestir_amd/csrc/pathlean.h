@@ -104,6 +104,47 @@ __device__ __forceinline__ Spec mis_direct_lean(const FrameArgs &a, const SpecTa
     return total;
 }
 
+// MisDirectLightIntegrator::Li at one wavefront vertex of a frame whose BSDF rays can reach no
+// emitter (no emissive material, environment or mesh light): the light-sampler NEE sample with the
+// draws of mis_direct_lean, its shadow ray recorded as a query for the traversal launch instead of
+// traced; the returned contribution counts when that ray is unoccluded. The path's own BSDF
+// connection (_integrateBSDFConnection) can only reach an emitter-free surface or nothing, so it
+// adds nothing and only its existence (hasConn) enters the NEE sample's MIS weight.
+template <class R>
+__device__ __forceinline__ Spec mis_nee_wave(const FrameArgs &a, const SpecTab &tab, const Vertex &v, R &rng,
+                                             bool hasConn, Trace &dbg, bool &query, V3 &qo, V3 &qd, float &qtmax) {
+    const DeviceScene &s = a.s;
+    query = false;
+    if (s.num_lights <= 0) return spec(0.0f);
+    const uint32_t n = uint32_t(s.num_lights);
+    const float u = rng.next_float();
+    uint32_t idx = uint32_t(u * float(n));
+    if (idx > n - 1u) idx = n - 1u;
+    dbg.lhash = fnv(dbg.lhash, idx);
+    ++dbg.picks;
+    const GLight &L = s.lights[idx];
+    LSample ls;
+    if (!sample_light_inl(s, L, v.h.pos, rng, ls)) return spec(0.0f);
+    ls.pdf = ls.pdf * a.light_select_pdf;
+    const bool delta = L.is_delta != 0;
+    const float nDotL = dot(v.sn, ls.dir);
+    const float dist2 = ls.dist * ls.dist;
+    const float cosY = smax(0.0f, dot(-ls.dir, ls.lnormal));
+    if (!(nDotL > 0.0f && !(cosY <= 0.0f && !delta))) return spec(0.0f);
+    const float pNee = to_solid_angle(ls.pdf, ls.solid_angle, dist2, smax(cosY, 1e-6f));
+    if (!(pNee > 0.0f)) return spec(0.0f);
+    query = true;
+    qo = v.h.pos + v.sn * 1e-4f;
+    qd = ls.dir;
+    qtmax = ls.dist - 1e-3f;
+    const V3 wo = -v.rayDir;
+    const float bsdfPdf = ggx_pdf(v.c, v.sn, wo, ls.dir);
+    const V3 bsdf = ggx_eval(v.c, v.sn, wo, ls.dir);
+    const Spec radiance = rgb2spec(tab, bsdf) * rgb2spec(tab, ls.color) * (nDotL / pNee);
+    const float w = (delta || !hasConn) ? 1.0f : power2(pNee, bsdfPdf);
+    return spec(0.0f) + radiance * w;
+}
+
 // PathIntegrator::Li for one pixel, as path_li (integrator.h) with MIS direct lighting; the first
 // vertex is iteration 0 of the bounce loop (its roulette test can never fire, and 1 * x = x).
 // fidx: the frame index (seed); stage: the frame's sample goes there instead of into the

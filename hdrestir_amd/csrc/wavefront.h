@@ -27,7 +27,7 @@
 
 #include <type_traits>
 
-#include "deferred.h"
+#include "pathlean.h"
 
 namespace hdr {
 
@@ -195,6 +195,8 @@ __device__ __forceinline__ void wave_vertex(const FrameArgs &a, const SpecTab &t
                                               first && a.use_res, slot ? slot + kMtN : nullptr,
                                               (a.trace_generated || a.skip_vis == 0) ? a.wesc[pixel] : 0u, qq,
                                               qo, qd, qtmax);
+    else if (a.direct == HDR_DIRECT_MIS)  // the path tracer (frames whose BSDF rays reach no emitter)
+        d = mis_nee_wave(a, tab, v, p.rng, ok, p.dbg, qq, qo, qd, qtmax);
     else
         d = ris_direct_lean<kCount, kDebug>(a, tab, v, ok && b.pdf > 0.0f, p.rng, wl, pixel, st, p.dbg,
                                             first && a.use_res, slot ? slot + kMtN : nullptr, qq, qo, qd, qtmax);

@@ -636,3 +636,42 @@ def test_spatial_reuse_with_a_mesh_light_is_refused(gpu_ctx):
                                            max_depth=2, spatial_count=2, spatial_radius=4.0))
     r.close()
     gs.close()
+
+
+PT_WAVE_CASES = {
+    # 1024 x 1024: from 2^20 pixels the path tracer runs on the wavefront when its BSDF rays reach no
+    # emitter; depth 6 includes Russian roulette (vertices past the fourth)
+    "box_pt_d4": (lambda: scenes.box_spheres(1024, 1024, frames=3), abi.pipeline("mis", max_depth=4)),
+    "box_pt_d6": (lambda: scenes.box_spheres(1024, 1024, frames=3), abi.pipeline("mis", max_depth=6)),
+    "features_pt_d5": (lambda: scenes.features(1024, 1024, dome=False, emissive=False, opaque=True),
+                       abi.pipeline("mis", max_depth=5)),
+}
+
+
+@pytest.mark.parametrize("case", list(PT_WAVE_CASES))
+def test_path_tracer_on_the_wavefront(gpu_ctx, oracle_mod, case):
+    """The path tracer (MIS) on the untraced wavefront: each vertex's light sample records its shadow
+    ray for the traversal launch (mis_nee_wave) in the lean path tracer's draw order. On a 48 x 40
+    crop of the oracle, streams equal on every pixel and colours within the shared bar; the stage
+    record shows the wavefront."""
+    from hdrestir_amd import gpu
+    make, pipe = PT_WAVE_CASES[case]
+    sc = make()
+    x0, y0 = sc.width // 2 - 24, sc.height // 2 - 20
+    rect = (x0, y0, x0 + 48, y0 + 40)
+    sel = np.zeros((sc.height, sc.width), bool)
+    sel[y0:y0 + 40, x0:x0 + 48] = True
+    sel = sel.reshape(-1)
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    o = oracle_mod.Oracle(sc, threads=8)
+    r.stage_times()
+    for f in range(2):
+        fr = scenes.frame_with_camera(sc, f)
+        r.render(fr, pipe, debug=True)
+        o.render(fr, pipe, rect=rect)
+        parity.check(f"{case} frame {f}", _match(r, o)[sel], r.debug["sample_rgb"][sel], o.debug["sample_rgb"][sel])
+    st = r.stage_times()
+    assert st["accumulate"][1] == 2 and st["shadow"][1] == 0, st  # the wavefront ran
+    r.close()
+    gs.close()
