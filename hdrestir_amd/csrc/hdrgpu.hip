@@ -946,7 +946,7 @@ struct FramePlan {
     bool lean_pt = false;   // the inlined path tracer (pathlean.h) instead of the megakernel
     bool spatial = false;   // spatial reuse after the temporal pass (deferred and wavefront paths)
     int spatial_nd = 0;     // its domains per pixel, rounded up to a compiled size (spatial_domains)
-    bool wave = false;      // the wavefront path (RIS at every vertex of deep paths, wavefront.h)
+    bool wave = false;      // the wavefront path (RIS at every vertex of deep paths, or the path tracer; wavefront.h)
     bool wave_lean0 = false;  // its primary vertex cannot draw past RNG output 310 (lean stream there)
     bool wave_traced = false; // BSDF-sampled candidates can see an emitter: bounce and escape rays per vertex
     bool spill = false;       // RNG spill slots (a pixel can draw past output 311)

@@ -1,10 +1,12 @@
 // wavefront.h -- PathIntegrator::Li (path_integrator.cpp:48-209) with RIS / ReSTIR direct lighting
-// (ris_direct_light_integrator.cpp:84-215) at every vertex, for paths of any depth, run as a
+// (ris_direct_light_integrator.cpp:84-215), or MIS direct lighting (the path tracer,
+// mis_direct_light_integrator.cpp:70-271), at every vertex, for paths of any depth, run as a
 // wavefront: one shading launch per path vertex and one traversal launch between them, with the
-// path's state in HBM. A shading launch samples the vertex's bounce, ranks the RIS candidates and
-// records two ray queries: the bounce ray (closest hit, the next vertex) and the RIS winner's
-// shadow ray (any hit). The traversal launch answers both queues and compacts the paths that go on
-// into the next vertex's work list, so later (sparser) vertices launch only over live paths.
+// path's state in HBM. A shading launch samples the vertex's bounce, evaluates its direct light and
+// records two ray queries: the bounce ray (closest hit, the next vertex) and the direct light's
+// shadow ray (any hit: the RIS winner's or the MIS light sample's). The traversal launch answers
+// both queues and compacts the paths that go on into the next vertex's work list, so later
+// (sparser) vertices launch only over live paths.
 //
 // Spatial reuse (an extension, deferred.h: spatial_reuse_pixel): the primary vertex stores its
 // post-temporal reservoir and the spatial launch after wave_shade0 resamples it with the
@@ -12,17 +14,17 @@
 // occlusion flag. The path total then holds the later vertices from zero, and the pixel's
 // radiance is outer * ((emission + reused direct) + total).
 //
-// The host admits a frame here under the same conditions as the deferred path except depth and
-// RNG range: skipVisibility (candidates are ranked without shadow rays), no translucent material,
-// and nothing BSDF-sampled can reach an emitter (no emissive material, mesh light or environment),
-// so the BSDF candidates reduce to their RNG draws and the winner's shadow ray is the only query
-// besides the bounce ray. Streams may run past RNG output 310: the second-twist machinery (spill
-// slots) stays out of line.
+// Frames the host admits here have opaque materials (no pass-through rays). Untraced frames: RIS
+// with skipVisibility, or the path tracer, where nothing BSDF-sampled can reach an emitter, so the
+// BSDF candidates reduce to their RNG draws. Traced frames (wave_gen below): an environment,
+// emissive materials or mesh lights, or RIS without skipVisibility; the vertex's ray queries are
+// emitted and answered before it is shaded. Streams may run past RNG output 310: the second-twist
+// machinery (spill slots) stays out of line.
 //
 // Results are identical to the megakernel's: every vertex consumes the same draws in the same order,
 // and the radiance sum is formed in the same order -- vertex b's term `tp * d` is added to the path
 // total by the vertex's shading launch, or by the traversal launch right after it when d waits for
-// the winner's shadow ray; the next vertex adds its term only after that launch.
+// its shadow ray; the next vertex adds its term only after that launch.
 #pragma once
 
 #include <type_traits>
