@@ -535,6 +535,9 @@ static hdr_status view_status(hdr_view *v) {
     if (word & kStatusRngOverrun)
         return fail(HDR_ERR_CAPACITY, "a pixel drew past the random numbers its stream was set up for (RNG spill "
                                       "slot missing); the view's frames since its last reset are wrong");
+    if (word & kStatusStackOverflow)
+        return fail(HDR_ERR_CAPACITY, "a shadow ray's traversal stack in LDS overflowed; the view's frames since its "
+                                      "last reset are wrong");
     return HDR_OK;
 }
 

@@ -275,6 +275,11 @@ __device__ __forceinline__ void accumulate_sample(float4 *accum, float4 *color, 
 }
 
 constexpr unsigned int kStatusRngOverrun = 1u;
+// kStatusStackOverflow: an any-hit traversal with its stack in LDS would have pushed past the
+// stack's kStk entries. The host admits that variant only for scenes whose tree depths fit it, so
+// this does not happen; if it did, the entry is dropped (no write outside the stack) and the view
+// reports HDR_ERR_CAPACITY at its next synchronisation.
+constexpr unsigned int kStatusStackOverflow = 2u;
 __device__ __forceinline__ void flag_rng_overrun(const FrameArgs &a, bool overrun) {
     if (overrun && a.status) atomicOr(a.status, kStatusRngOverrun);
 }

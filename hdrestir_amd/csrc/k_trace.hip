@@ -154,8 +154,14 @@ __device__ __forceinline__ void anyhit_queue(const FrameArgs *__restrict__ args,
             const int first = leftFirst ? node_ref(l) : node_ref(r);
             if (hl && hr) {
                 const int second = leftFirst ? node_ref(r) : node_ref(l);
-                if constexpr (kStk > 0) col[kLanes * tsp++] = second;
-                else if (blas) bref[bsp++] = second;
+                if constexpr (kStk > 0) {
+                    if (tsp < kStk) {
+                        col[kLanes * tsp++] = second;
+                    } else {  // beyond the depth bound the host checked: dropped, and reported
+                        const FrameArgs &av = args[view];
+                        if (av.status) atomicOr(av.status, kStatusStackOverflow);
+                    }
+                } else if (blas) bref[bsp++] = second;
                 else tref[tsp++] = second;
             }
             cur = first;
