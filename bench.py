@@ -51,6 +51,19 @@ def _log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def _section(name, fn):
+    """Runs one measurement after the headline; a failure is reported in the line as {"error": ...}
+    (and logged) rather than losing the headline's numbers."""
+    _log(name)
+    try:
+        out = fn()
+    except Exception as e:  # noqa: BLE001 -- any failure of an optional measurement is reported
+        _log(f"{name} failed: {type(e).__name__}: {e}")
+        return {"error": f"{type(e).__name__}: {e}"}
+    _log(f"{name} {out}")
+    return out
+
+
 def read_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary, or None."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_many_lights_pmc.txt")
@@ -746,34 +759,30 @@ def main():
         "image_mean_rgb": [round(float(v), 3) for v in img[..., :3].reshape(-1, 3).mean(0)],
     }
     if not args.headline_only:
-        line["post_process"] = post_process_measurement(r, dolly(0), pipe)
-        _log(f"post process {line['post_process']}")
-        line["dolly_reprojection"] = dolly_reprojection_measurement(gpu, abi, ctx, sc, gs, r, pipe, args.warmup,
-                                                                    args.steps)
-        _log(f"dolly with reprojected temporal reuse {line['dolly_reprojection']}")
+        line["post_process"] = _section("post process", lambda: post_process_measurement(r, dolly(0), pipe))
+        line["dolly_reprojection"] = _section(
+            "dolly with reprojected temporal reuse",
+            lambda: dolly_reprojection_measurement(gpu, abi, ctx, sc, gs, r, pipe, args.warmup, args.steps))
     r.close()
     if not args.headline_only and not args.no_deep:
-        _log("deep paths")
-        line["deep_paths"] = deep_path_measurement(gpu, abi, ctx, sc, gs, max(4, args.steps // 4))
-        _log(f"deep paths {line['deep_paths']}")
-        line["visibility_paths"] = deep_path_measurement(gpu, abi, ctx, sc, gs, max(4, args.steps // 8), depth=4,
-                                                         m=16, skip_visibility=False)
-        _log(f"visibility paths {line['visibility_paths']}")
-        _log("multi-view batch")
-        line["multiview"] = multiview_measurement(gpu, abi, ctx, gs, 4)
-        _log(f"multi-view {line['multiview']}")
-        _log("large scene")
-        line["large_scene"] = large_scene_measurement(gpu, abi, ctx)
-        _log(f"large scene {line['large_scene']}")
+        line["deep_paths"] = _section(
+            "deep paths", lambda: deep_path_measurement(gpu, abi, ctx, sc, gs, max(4, args.steps // 4)))
+        line["visibility_paths"] = _section(
+            "visibility paths", lambda: deep_path_measurement(gpu, abi, ctx, sc, gs, max(4, args.steps // 8), depth=4,
+                                                              m=16, skip_visibility=False))
+        line["multiview"] = _section("multi-view batch", lambda: multiview_measurement(gpu, abi, ctx, gs, 4))
+        line["large_scene"] = _section("large scene", lambda: large_scene_measurement(gpu, abi, ctx))
     if not args.headline_only:
-        _log("config 0")
-        line["config0"] = config0_measurement(gpu, abi, ctx, cpu=not args.no_cpu_baseline)
-        _log(f"config 0 {line['config0']}")
+        line["config0"] = _section("config 0",
+                                   lambda: config0_measurement(gpu, abi, ctx, cpu=not args.no_cpu_baseline))
     if not args.no_serving and not args.headline_only:
-        _log("serving")
-        line["serving"] = serving_measurement(gpu, abi, ctx, sc, gs, args.serving_requests)
-    gs.close()
-    ctx.close()
+        line["serving"] = _section("serving",
+                                   lambda: serving_measurement(gpu, abi, ctx, sc, gs, args.serving_requests))
+    try:
+        gs.close()
+        ctx.close()
+    except Exception as e:  # noqa: BLE001 -- a device error after the measurements
+        _log(f"closing the GPU context failed: {type(e).__name__}: {e}")
     if not args.no_cpu_baseline:
         _log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(sc, pipe, rows=args.cpu_rows)
