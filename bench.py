@@ -16,9 +16,9 @@ Workload (one step = one frame of the per-pixel hot path over the whole image):
 
 Prints one JSON line. `value` = ReSTIR frames/s at 1080p with scene, reservoirs and accumulation
 resident in HBM. The CPU baseline is the oracle (a faithful C++ restatement of the reference's CPU
-path) timed on a bounded sample of the same frames on this host. Further entries: config 0 (the
-box at 256x256, 16 spp, PathTracer and RIS), the depth-6 frame, the 16-view batch, the large scene
-and the serving stream.
+path) timed on a bounded sample of the same frames on this host. Further entries: the box at
+256x256 (16 spp, PathTracer and RIS), the depth-6 frame, the 16-view batch, the large scene and the
+serving stream.
 """
 import argparse
 import json
@@ -139,7 +139,7 @@ def cpu_baseline(scene, pipe, rows=540, timed=2):
             "host": host_cpu(), "compiler": "g++ -O2 -std=c++20 -ffp-contract=off (oracle/Makefile)"}
 
 
-def config0_measurement(gpu, abi, ctx, cpu=True, cpu_frames=5):
+def box_measurement(gpu, abi, ctx, cpu=True, cpu_frames=5):
     """Config 0: the box of 12 triangles with two icospheres (4840 triangles) and one rect light at
     256x256, 16 spp (frames 0 .. 15), depth 4: PathTracer (MIS) and RIS with M = 16 (8 light + 8
     BSDF candidates, visibility tested, without and with the reservoir). GPU: frames/s (= spp/s) over
@@ -198,9 +198,9 @@ def config0_measurement(gpu, abi, ctx, cpu=True, cpu_frames=5):
 def serving_requests(rng, n, many_view, box_view, only_restir_1080p=False, small_only=False):
     """The serving stream's request mix: resolution uniform over {256^2, 512^2, 1280x720,
     1920x1080}, spp = round(2^U[0,6]), PathTracer (MIS, depth 4) or ReSTIR (M = 32, 1 temporal +
-    3 spatial) with equal probability, scene uniform over the resident ones: the box (config 0),
-    the many-lights scene seen from its camera (config 1) and from a random point of a 40 m
-    sphere (config 3's cameras)."""
+    3 spatial) with equal probability, scene uniform over the resident ones: the 256² box,
+    the many-lights scene seen from its camera and from a random point of a 40 m sphere (the
+    16-view workload's cameras)."""
     from hdrestir_amd import scenes
     sizes = [(256, 256), (512, 512)] if small_only else [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
     reqs = []
@@ -269,8 +269,8 @@ def serving_measurement(gpu, abi, ctx, sc_many, g_many, n_requests, seed=1):
     g_box = gpu.GpuScene(ctx, box)
     gscenes = {0: g_box, 1: g_many, 2: g_many}
     many_view, box_view = sc_many.cameras[0][0], box.cameras[0][0]
-    out = {"requests": n_requests, "scenes": "box (config 0), many-lights (config 1), many-lights from a "
-           "40 m sphere (config 3 cameras); the last two share one resident upload",
+    out = {"requests": n_requests, "scenes": "256x256 box, 1080p many-lights from its camera, many-lights from a "
+           "40 m sphere (the 16-view workload's cameras); the last two share one resident upload",
            "mix": "resolution U{256^2, 512^2, 1280x720, 1920x1080}, spp round(2^U[0,6]), PathTracer depth 4 or "
                   "ReSTIR M=32 1 temporal + 3 spatial (50/50)"}
     # same images alone and in the mix: small requests, images compared exactly
@@ -673,7 +673,7 @@ def main():
         ctx.close()
         return
     _log("building scene")
-    sc = scenes.many_lights(frames=64, dolly=True)  # config 1's 64-frame linear dolly
+    sc = scenes.many_lights(frames=64, dolly=True)  # the 1080p many-lights scene's 64-frame linear dolly
     if args.deep_only:
         ctx = gpu.Context(0)
         gs = gpu.GpuScene(ctx, sc)
@@ -690,7 +690,7 @@ def main():
     r = gpu.Renderer(ctx, gs, sc.width, sc.height)
     P = sc.width * sc.height
 
-    # config 1 as written: a 64-frame linear dolly at 1 spp per frame. Each camera move clears the
+    # the headline workload: a 64-frame linear dolly at 1 spp per frame. Each camera move clears the
     # reservoirs and the accumulation and restarts the frame count (renderer.cpp:401-406), so every
     # frame is rendered from a cleared view at frame index 0.
     def dolly(i):
@@ -773,8 +773,8 @@ def main():
         line["multiview"] = _section("multi-view batch", lambda: multiview_measurement(gpu, abi, ctx, gs, 4))
         line["large_scene"] = _section("large scene", lambda: large_scene_measurement(gpu, abi, ctx))
     if not args.headline_only:
-        line["config0"] = _section("config 0",
-                                   lambda: config0_measurement(gpu, abi, ctx, cpu=not args.no_cpu_baseline))
+        line["box_256"] = _section("256x256 box, 16 spp",
+                                   lambda: box_measurement(gpu, abi, ctx, cpu=not args.no_cpu_baseline))
     if not args.no_serving and not args.headline_only:
         line["serving"] = _section("serving",
                                    lambda: serving_measurement(gpu, abi, ctx, sc, gs, args.serving_requests))

@@ -147,7 +147,7 @@ def test_graphs_launch_shapes_and_autotune_keep_results(gpu_ctx):
         g.close()
 
 
-def test_config3_sixteen_views_batched_with_graphs_and_autotune(gpu_ctx, oracle_mod):
+def test_sixteen_views_of_the_many_lights_scene_batched_with_graphs_and_autotune(gpu_ctx, oracle_mod):
     """The multi-view workload: 16 viewports of the many-lights scene (sizes uniform in
     [320, 1920] x [240, 1080], cameras on a 40 m sphere), ReSTIR M=32 at every vertex of depth-6
     paths, steps replayed from captured graphs with the autotuned launch shape: every view's image and

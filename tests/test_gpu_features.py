@@ -476,7 +476,7 @@ def test_ray_counters_match_the_oracle(gpu_ctx, oracle_mod, case):
 # wavefront (smaller views keep the megakernel); the oracle renders a 48 x 40 crop of each
 VW, VH = 384, 352
 VIS_CASES = {
-    # config 0's pipeline: RIS M = 16 (8 + 8) with visibility at depth 4, reservoir off and on
+    # the 256x256 box workload's pipeline: RIS M = 16 (8 + 8) with visibility at depth 4, reservoir off and on
     "box_ris16_d4": (lambda: scenes.box_spheres(VW, VH, frames=3),
                      abi.pipeline("ris", candidate_count=16, use_reservoir=False, max_depth=4)),
     "box_restir16_d4": (lambda: scenes.box_spheres(VW, VH, frames=3),

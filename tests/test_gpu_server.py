@@ -166,7 +166,7 @@ def test_server_on_its_own_thread(gpu_ctx):
     gfeat.close()
 
 
-def test_config2_served_mix_over_three_resident_scenes(gpu_ctx):
+def test_served_mix_over_three_resident_scenes(gpu_ctx):
     """The serving workload's mix on its three resident scenes -- the box (256^2 .. 1280x720), the
     many-lights scene from its camera and from a point of a 40 m sphere -- with 1080p ReSTIR
     (M=32, 3 spatial neighbours within 30 px) and 1080p PathTracer (depth 4) requests among smaller
