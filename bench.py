@@ -758,12 +758,6 @@ def main():
         "static_camera": static_line,
         "image_mean_rgb": [round(float(v), 3) for v in img[..., :3].reshape(-1, 3).mean(0)],
     }
-    if not args.headline_only:
-        line["post_process"] = _section("post process", lambda: post_process_measurement(r, dolly(0), pipe))
-        line["dolly_reprojection"] = _section(
-            "dolly with reprojected temporal reuse",
-            lambda: dolly_reprojection_measurement(gpu, abi, ctx, sc, gs, r, pipe, args.warmup, args.steps))
-    r.close()
     if not args.headline_only and not args.no_deep:
         line["deep_paths"] = _section(
             "deep paths", lambda: deep_path_measurement(gpu, abi, ctx, sc, gs, max(4, args.steps // 4)))
@@ -778,7 +772,14 @@ def main():
     if not args.no_serving and not args.headline_only:
         line["serving"] = _section("serving",
                                    lambda: serving_measurement(gpu, abi, ctx, sc, gs, args.serving_requests))
+    # the headline view's extensions last (the view stays open until here)
+    if not args.headline_only:
+        line["post_process"] = _section("post process", lambda: post_process_measurement(r, dolly(0), pipe))
+        line["dolly_reprojection"] = _section(
+            "dolly with reprojected temporal reuse",
+            lambda: dolly_reprojection_measurement(gpu, abi, ctx, sc, gs, r, pipe, args.warmup, args.steps))
     try:
+        r.close()
         gs.close()
         ctx.close()
     except Exception as e:  # noqa: BLE001 -- a device error after the measurements
