@@ -127,3 +127,31 @@ def test_reprojection_refusals(gpu_ctx):
     batch.close()
     r.close()
     gs.close()
+
+
+def test_reset_accumulation_keeps_the_reservoirs(gpu_ctx):
+    """hdr_view_reset_accumulation clears the accumulation and the sample count and keeps the
+    reservoirs (the next frame at frame index 0 shows its own sample alone); hdr_view_reset clears
+    both."""
+    from hdrestir_amd import gpu
+    sc = scenes.box_spheres(64, 48, frames=1)
+    pipe = abi.pipeline("ris", candidate_count=8, use_reservoir=True, skip_visibility=True, max_depth=1,
+                        spatial_count=2, spatial_radius=6.0)
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    r = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    for f in range(3):
+        r.render(sc.frame(f, camera_index=0), pipe, read_color=False)
+    before = r.reservoirs().copy()
+    assert r.sample_count() == 3
+    r.reset_accumulation()
+    assert r.sample_count() == 0
+    assert np.array_equal(r.reservoirs(), before)  # the reservoirs stay
+    img = r.render(sc.frame(0, camera_index=0), pipe, debug=True).copy()
+    assert r.sample_count() == 1
+    # the accumulation restarted: at frame index 0 the colour is this frame's sample alone
+    np.testing.assert_array_equal(img.reshape(-1, 4)[:, :3], r.debug["sample_rgb"])
+    r.reset()
+    assert r.sample_count() == 0
+    assert (r.reservoirs()["flags_light"] == 0).all()  # hdr_view_reset clears them
+    r.close()
+    gs.close()
