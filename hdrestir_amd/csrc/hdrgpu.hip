@@ -983,6 +983,10 @@ constexpr size_t kVisWavefrontPixels = size_t(1) << 17;
 // 256^2 1.72 / 2.45 against 2.26 / 4.54, 512^2 2.04 / 3.06 against 2.63 / 5.10, 1280x720
 // 3.93 / 5.49 against 3.58 / 6.20, 1920x1080 6.84 / 9.28 against 5.47 / 8.36 (identical images).
 constexpr size_t kPtWavefrontPixels = size_t(1) << 20;
+// and from which it also beats the lean path tracer's multi-frame launches (hdr_render_frames):
+// 1280x720 3.11 ms per frame in 16-frame lean launches against 3.58 on the wavefront, 1920x1080
+// 6.01 against 5.47 (depth 4).
+constexpr size_t kPtWavefrontMultiPixels = size_t(3) << 19;
 
 // The escape queue of `cap` entries: two float4 each, then a float (tmax) each.
 static size_t escape_queue_bytes(size_t cap) {
@@ -999,8 +1003,9 @@ static size_t view_base_bytes(size_t P) {
 
 // Validates a pipeline on a scene and decides the frame's kernel path and the buffers it needs.
 // Host-only: no device call; hdr_view_plan_bytes and the server's admission use it too.
-// multi: the frame shares a launch with others (hdr_render_frames), so a path-traced frame keeps
-// the lean path tracer, which takes several frames per launch, rather than the wavefront.
+// multi: the frame could share a launch with others (hdr_render_frames): a path-traced frame then
+// keeps the lean path tracer, which takes several frames per launch, below
+// kPtWavefrontMultiPixels, and takes the wavefront frame by frame from there.
 static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_pipeline_desc *pipe, bool debug,
                                 FramePlan &plan, FrameNeeds &need, bool multi = false) {
     if (!scene || !pipe) return fail(HDR_ERR_INVALID_ARGUMENT, "null argument");
@@ -1065,7 +1070,7 @@ static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_
     // The path tracer (MIS) takes the untraced wavefront on views from kPtWavefrontPixels when its
     // BSDF rays can reach no emitter (mis_nee_wave: the connection then adds nothing).
     const bool ptWave = !ris && !scene->any_translucent && !traceLastConnection && !scene->ds.has_mesh_lights &&
-                        P >= kPtWavefrontPixels && !multi;
+                        P >= (multi ? kPtWavefrontMultiPixels : kPtWavefrontPixels);
     const bool wave = (!deferred && ris && !scene->any_translucent && !(needTraced && nCand > 32) &&
                        (skipVis || (P >= kVisWavefrontPixels && !scene->ds.has_mesh_lights))) ||
                       ptWave;

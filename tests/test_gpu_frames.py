@@ -98,3 +98,26 @@ def test_render_frames_arguments(gpu_ctx):
         r.render_frames(sc.frame(0), abi.pipeline("mis", max_depth=2), -1)
     r.close()
     gs.close()
+
+
+def test_render_frames_of_a_1080p_path_tracer_go_to_the_wavefront(gpu_ctx):
+    """From 1.5 Mpixels a path-traced view renders frame by frame on the wavefront (faster there than
+    the lean path tracer's multi-frame launches): 4 frames take 4 launches with their accumulation
+    launches, and equal the frames one by one."""
+    from hdrestir_amd import gpu
+    sc = scenes.box_spheres(1920, 1080, frames=1)
+    pipe = abi.pipeline("mis", max_depth=3)
+    gs = gpu.GpuScene(gpu_ctx, sc)
+    a = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    b = gpu.Renderer(gpu_ctx, gs, sc.width, sc.height)
+    a.stage_times()
+    a.render_frames(sc.frame(0), pipe, 4)
+    st = a.stage_times()
+    assert st["integrate"][1] == 4 and st["accumulate"][1] == 4, st
+    for f in range(4):
+        b.render(sc.frame(f, camera_index=0), pipe, read_color=False)
+    assert np.array_equal(a.read_color(), b.read_color())
+    assert a.sample_count() == b.sample_count() == 4
+    a.close()
+    b.close()
+    gs.close()
