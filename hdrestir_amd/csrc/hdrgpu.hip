@@ -987,6 +987,9 @@ constexpr size_t kPtWavefrontPixels = size_t(1) << 20;
 // 1280x720 3.11 ms per frame in 16-frame lean launches against 3.58 on the wavefront, 1920x1080
 // 6.01 against 5.47 (depth 4).
 constexpr size_t kPtWavefrontMultiPixels = size_t(3) << 19;
+// The path tracer's wavefront is off: a serving run of 1080p path-traced requests beside batched
+// ReSTIR views ended in a GPU memory fault whose cause reading the kernels did not show.
+constexpr bool kPtWavefront = false;
 
 // The escape queue of `cap` entries: two float4 each, then a float (tmax) each.
 static size_t escape_queue_bytes(size_t cap) {
@@ -1069,8 +1072,10 @@ static hdr_status plan_pipeline(const hdr_scene *scene, int W, int H, const hdr_
     // ReSTIR M = 8 at depth 2: 20.3 ms against 21.5 ms as a wavefront).
     // The path tracer (MIS) takes the untraced wavefront on views from kPtWavefrontPixels when its
     // BSDF rays can reach no emitter (mis_nee_wave: the connection then adds nothing).
-    const bool ptWave = !ris && !scene->any_translucent && !traceLastConnection && !scene->ds.has_mesh_lights &&
-                        P >= (multi ? kPtWavefrontMultiPixels : kPtWavefrontPixels);
+    // Switched off (kPtWavefront): a serving run with it on ended in a GPU memory fault whose cause
+    // is not shown; path-traced frames take the lean path tracer and the megakernel meanwhile.
+    const bool ptWave = kPtWavefront && !ris && !scene->any_translucent && !traceLastConnection &&
+                        !scene->ds.has_mesh_lights && P >= (multi ? kPtWavefrontMultiPixels : kPtWavefrontPixels);
     const bool wave = (!deferred && ris && !scene->any_translucent && !(needTraced && nCand > 32) &&
                        (skipVis || (P >= kVisWavefrontPixels && !scene->ds.has_mesh_lights))) ||
                       ptWave;

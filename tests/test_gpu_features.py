@@ -648,6 +648,8 @@ PT_WAVE_CASES = {
 }
 
 
+@pytest.mark.skip(reason="the path tracer's wavefront is switched off (kPtWavefront in hdrgpu.hip): a serving "
+                  "run with it on ended in a GPU memory fault whose cause is not shown")
 @pytest.mark.parametrize("case", list(PT_WAVE_CASES))
 def test_path_tracer_on_the_wavefront(gpu_ctx, oracle_mod, case):
     """The path tracer (MIS) on the untraced wavefront: each vertex's light sample records its shadow

@@ -100,6 +100,8 @@ def test_render_frames_arguments(gpu_ctx):
     gs.close()
 
 
+@pytest.mark.skip(reason="the path tracer's wavefront is switched off (kPtWavefront in hdrgpu.hip): a serving "
+                  "run with it on ended in a GPU memory fault whose cause is not shown")
 def test_render_frames_of_a_1080p_path_tracer_go_to_the_wavefront(gpu_ctx):
     """From 1.5 Mpixels a path-traced view renders frame by frame on the wavefront (faster there than
     the lean path tracer's multi-frame launches): 4 frames take 4 launches with their accumulation
